@@ -1,0 +1,130 @@
+/* dwf_hip.h -- C ABI of libdwfhip.so, the MI355X (gfx950) fermion hot path for 2D U(1)
+ * Wilson and Shamir domain-wall HMC and measurements.
+ *
+ * The reference (wurdahl/Three_Part_DWF_Study) has no FFI: its hot path is a set of
+ * header-only C++ functions over Eigen types.  Each entry point below replaces one of
+ * them (cited file:line); include/dwf_hip.hpp wraps these entry points back into the
+ * reference's own function names, result structs and exception messages.
+ *
+ * Conventions
+ *   - Host buffers use the reference layout, interleaved complex double:
+ *       DWF spinor     psi[s][t][x][spin]    ((s*Nt+t)*Nx+x)*2+spin  (domain_wall/include/indexing.hpp:5-8)
+ *       Wilson spinor  psi[t][x][spin]       (t*Nx+x)*2+spin         (wilson/include/indexing.hpp:5-8)
+ *       gauge angles   theta[mu][t][x]       (mu*Nt+t)*Nx+x, mu=0 is t (indexing.hpp:10-13)
+ *     A batch of nrhs vectors is nrhs such vectors back to back.
+ *   - *_device entry points take device pointers in the same layout (FP64: double,
+ *     FP32: float) and run asynchronously on the context's stream.
+ *   - The caller owns every buffer; the library never keeps a host pointer.  The
+ *     context owns its device memory.
+ *   - Every function returns DWF_OK (0) or an error code; no C++ exception crosses the
+ *     ABI.  dwf_last_error() gives the message of the calling thread's last failure.
+ *   - A context is used by one host thread at a time.  Contexts are independent, so
+ *     each OpenMP thread of an analyzer holds its own (each has its own HIP stream).
+ */
+#ifndef DWF_HIP_H
+#define DWF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum dwf_status {
+    DWF_OK = 0,
+    DWF_ERR_ARG = 1,             /* bad argument (sizes, op, null pointer)            */
+    DWF_ERR_HIP = 2,             /* HIP runtime error or no GPU                       */
+    DWF_ERR_NOT_CONVERGED = 3,   /* CG reached maxiter                                */
+    DWF_ERR_ZERO_DENOM = 4,      /* CG encountered a zero denominator                 */
+    DWF_ERR_CERTIFY = 5          /* FP64 true residual above tolerance after fallback */
+};
+
+enum dwf_op {
+    DWF_OP_D = 0,      /* D          dwf_operator.hpp:15-92 / wilson_operator.hpp:10-62 */
+    DWF_OP_DDAG = 1,   /* D^dagger   dwf_operator.hpp:94-171                             */
+    DWF_OP_DDDAG = 2,  /* D D^dagger dwf_operator.hpp:173-176                            */
+    DWF_OP_DDAGD = 3   /* D^dagger D dwf_operator.hpp:178-181                            */
+};
+
+enum dwf_precision {
+    DWF_PREC_FP64 = 0,
+    DWF_PREC_FP32 = 1
+};
+
+enum dwf_method {
+    /* FP64 CG, the reference's algorithm (domain_wall/include/cg_solver.hpp:17-64):
+     * stops on the recursive residual, as the reference does. */
+    DWF_SOLVE_CG64 = 0,
+    /* FP32 CG with FP64 reliable updates; stops on the FP64 true residual. */
+    DWF_SOLVE_MIXED = 1
+};
+
+typedef struct dwf_ctx dwf_ctx;
+
+/* Per right-hand side solve statistics. */
+typedef struct dwf_solve_stats {
+    int iterations;        /* CG iterations (FP32 ones for DWF_SOLVE_MIXED)            */
+    int converged;         /* 1 if the stopping rule was met                           */
+    int reliable_updates;  /* FP64 true-residual recomputations (mixed only)           */
+    int fp64_iterations;   /* FP64 iterations (CG64, or the mixed solver's fallback)   */
+    double relres;         /* the solver's own ||r||/||b|| at exit (recursive for CG64) */
+    double true_relres;    /* FP64 ||b - A x|| / ||b|| recomputed at exit              */
+} dwf_solve_stats;
+
+/* ---- context --------------------------------------------------------------------- */
+
+/* Shamir domain-wall operator on an Nt x Nx x Ls lattice: diagonal 3-M5, wall coupling
+ * +mf, antiperiodic in t and periodic in x (dwf_operator.hpp:15-92).  Replaces the
+ * process-global sizes of domain_wall/include/parameters.hpp:7-12 with arguments. */
+int dwf_ctx_create_dwf(dwf_ctx** out, int Nt, int Nx, int Ls, double M5, double mf);
+
+/* Wilson operator: diagonal 2+m0, boundary signs bc_t at t=Nt-1 and bc_x at x=Nx-1
+ * (wilson/include/wilson_operator.hpp:10-62, wilson/include/parameters.hpp:6-38). */
+int dwf_ctx_create_wilson(dwf_ctx** out, int Nt, int Nx, double m0, double bc_t, double bc_x);
+
+int dwf_ctx_destroy(dwf_ctx* ctx);
+
+/* The context's HIP stream (hipStream_t), for callers that order their own work
+ * against the *_device entry points. */
+void* dwf_ctx_stream(dwf_ctx* ctx);
+
+/* Upload gauge angles theta[2][Nt][Nx] (host).  The library keeps them on the device in
+ * FP64 and as cos/sin link arrays in FP64 and FP32; U_mu(x) = exp(i theta_mu(x))
+ * (dwf_operator.hpp:10-13). */
+int dwf_set_gauge(dwf_ctx* ctx, const double* theta);
+int dwf_set_gauge_device(dwf_ctx* ctx, const double* theta_device);
+
+/* ---- operators --------------------------------------------------------------------- */
+
+/* out = op(in) for nrhs vectors in FP64 (host buffers).  Replaces apply_D,
+ * apply_D_dagger, apply_DD_dagger, apply_D_dagger_D (dwf_operator.hpp:15-181) and the
+ * products with build_wilson_matrix (wilson_operator.hpp:10-62). */
+int dwf_apply(dwf_ctx* ctx, int op, const double* in, double* out, int nrhs);
+
+/* Same on device buffers in the given precision (in != out). */
+int dwf_apply_device(dwf_ctx* ctx, int op, int precision, const void* in, void* out, int nrhs);
+
+/* ---- solvers ----------------------------------------------------------------------- */
+
+/* Solve A x = b for nrhs right-hand sides with A = D D^dagger (DWF_OP_DDDAG) or
+ * D^dagger D (DWF_OP_DDAGD).  x0 (nullable) is the initial guess for every right-hand
+ * side's own slot.  stats (nullable) receives nrhs records.  Replaces
+ * conjugate_gradient (domain_wall/include/cg_solver.hpp:17-64); the Wilson solves use
+ * the same entry point (wilson/include/cg_solver.hpp:19-78).  Returns
+ * DWF_ERR_NOT_CONVERGED if any right-hand side missed the tolerance (x still holds
+ * the last iterate) and DWF_ERR_ZERO_DENOM as cg_solver.hpp:45-46. */
+int dwf_solve(dwf_ctx* ctx, int normal_op, int method, const double* b, const double* x0,
+              double rtol, int maxiter, int nrhs, double* x, dwf_solve_stats* stats);
+
+/* Device-buffer form (FP64 b, x0 nullable, x).  Synchronises the context stream before
+ * returning because convergence is decided on the host. */
+int dwf_solve_device(dwf_ctx* ctx, int normal_op, int method, const double* b_device,
+                     const double* x0_device, double rtol, int maxiter, int nrhs,
+                     double* x_device, dwf_solve_stats* stats);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* dwf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DWF_HIP_H */

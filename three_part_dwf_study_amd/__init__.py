@@ -1,0 +1,24 @@
+"""MI355X (gfx950) fermion hot path for 2D U(1) Wilson and Shamir domain-wall HMC.
+
+The product is ``libdwfhip.so`` (HIP kernels + C ABI, declared in ``include/dwf_hip.h``)
+with C++ drop-in wrappers in ``include/dwf_hip.hpp``.  This package holds its sources
+(``csrc/``) and a thin ctypes mirror of the C ABI (:mod:`.capi`) used by the tests, the
+smoke check and ``bench.py``.  There is no CPU fallback: every call goes to the GPU and
+fails loudly (``DwfError``) when the library or the device is missing.
+"""
+
+from .capi import (  # noqa: F401
+    DWF_OP_D,
+    DWF_OP_DDAG,
+    DWF_OP_DDDAG,
+    DWF_OP_DDAGD,
+    DWF_PREC_FP32,
+    DWF_PREC_FP64,
+    DWF_SOLVE_CG64,
+    DWF_SOLVE_MIXED,
+    DwfContext,
+    DwfError,
+    build_library,
+    library_path,
+    load_library,
+)

@@ -1,0 +1,244 @@
+"""ctypes mirror of the C ABI in include/dwf_hip.h.
+
+Host arrays follow the reference layouts (interleaved complex128):
+DWF spinors ``[nrhs][Ls][Nt][Nx][2]``, Wilson spinors ``[nrhs][Nt][Nx][2]`` and gauge
+angles ``[2][Nt][Nx]`` (domain_wall/include/indexing.hpp:5-13,
+wilson/include/indexing.hpp:5-12).  Device buffers are passed as raw pointers
+(``int``), e.g. ``tensor.data_ptr()`` of a torch tensor on ``cuda:0``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_NAME = "libdwfhip.so"
+
+DWF_OK = 0
+DWF_ERR_ARG = 1
+DWF_ERR_HIP = 2
+DWF_ERR_NOT_CONVERGED = 3
+DWF_ERR_ZERO_DENOM = 4
+DWF_ERR_CERTIFY = 5
+
+DWF_OP_D = 0
+DWF_OP_DDAG = 1
+DWF_OP_DDDAG = 2
+DWF_OP_DDAGD = 3
+
+DWF_PREC_FP64 = 0
+DWF_PREC_FP32 = 1
+
+DWF_SOLVE_CG64 = 0
+DWF_SOLVE_MIXED = 1
+
+
+class DwfError(RuntimeError):
+    """A non-zero status from libdwfhip; ``code`` is the C ABI status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("reliable_updates", ctypes.c_int),
+        ("fp64_iterations", ctypes.c_int),
+        ("relres", ctypes.c_double),
+        ("true_relres", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, _LIB_NAME)
+
+
+def build_library(jobs: int = 8) -> str:
+    """Compile libdwfhip.so for gfx950 in-tree (hipcc via csrc/Makefile)."""
+    subprocess.run(["make", "-C", os.path.join(_HERE, "csrc"), f"-j{jobs}"], check=True)
+    return library_path()
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> (restype, argtypes); kept in the order of include/dwf_hip.h
+PROTOTYPES = {
+    "dwf_ctx_create_dwf": (_I, [_PP, _I, _I, _I, _D, _D]),
+    "dwf_ctx_create_wilson": (_I, [_PP, _I, _I, _D, _D, _D]),
+    "dwf_ctx_destroy": (_I, [_P]),
+    "dwf_ctx_stream": (_P, [_P]),
+    "dwf_set_gauge": (_I, [_P, _P]),
+    "dwf_set_gauge_device": (_I, [_P, _P]),
+    "dwf_apply": (_I, [_P, _I, _P, _P, _I]),
+    "dwf_apply_device": (_I, [_P, _I, _I, _P, _P, _I]),
+    "dwf_solve": (_I, [_P, _I, _I, _P, _P, _D, _I, _I, _P, _P]),
+    "dwf_solve_device": (_I, [_P, _I, _I, _P, _P, _D, _I, _I, _P, _P]),
+    "dwf_last_error": (ctypes.c_char_p, []),
+}
+
+
+def load_library(path: str | None = None):
+    """Load libdwfhip.so.  Raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or library_path()
+    if not os.path.exists(p):
+        raise DwfError(DWF_ERR_HIP, f"{p} is missing: build it first (__graft_entry__.build())")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(lib, rc: int, what: str):
+    if rc != DWF_OK:
+        msg = lib.dwf_last_error().decode()
+        raise DwfError(rc, f"{what}: {msg} (status {rc})")
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+@dataclass
+class Geometry:
+    Nt: int
+    Nx: int
+    Ls: int  # 1 for Wilson
+
+    @property
+    def V4(self) -> int:
+        return self.Nt * self.Nx
+
+    @property
+    def ndof(self) -> int:
+        """Complex components of one spinor."""
+        return self.Ls * self.Nt * self.Nx * 2
+
+
+class DwfContext:
+    """One device context (HIP stream, gauge field, workspace) of libdwfhip."""
+
+    def __init__(self, handle, geom: Geometry, kind: str, lib):
+        self._h = handle
+        self.geom = geom
+        self.kind = kind
+        self._lib = lib
+
+    # construction -----------------------------------------------------------------
+    @classmethod
+    def dwf(cls, Nt: int, Nx: int, Ls: int, M5: float, mf: float) -> "DwfContext":
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib, lib.dwf_ctx_create_dwf(ctypes.byref(h), Nt, Nx, Ls, M5, mf), "dwf_ctx_create_dwf")
+        return cls(h, Geometry(Nt, Nx, Ls), "dwf", lib)
+
+    @classmethod
+    def wilson(cls, Nt: int, Nx: int, m0: float, bc_t: float, bc_x: float) -> "DwfContext":
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib, lib.dwf_ctx_create_wilson(ctypes.byref(h), Nt, Nx, m0, bc_t, bc_x),
+               "dwf_ctx_create_wilson")
+        return cls(h, Geometry(Nt, Nx, 1), "wilson", lib)
+
+    def close(self):
+        if self._h:
+            self._lib.dwf_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return self._lib.dwf_ctx_stream(self._h) or 0
+
+    # gauge --------------------------------------------------------------------------
+    def set_gauge(self, theta: np.ndarray):
+        th = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+        if th.size != 2 * self.geom.V4:
+            raise ValueError("theta must hold 2*Nt*Nx angles")
+        _check(self._lib, self._lib.dwf_set_gauge(self._h, _ptr(th)), "dwf_set_gauge")
+
+    def set_gauge_device(self, theta_ptr: int):
+        _check(self._lib, self._lib.dwf_set_gauge_device(self._h, theta_ptr), "dwf_set_gauge_device")
+
+    # operators ----------------------------------------------------------------------
+    def _as_batch(self, v: np.ndarray) -> tuple[np.ndarray, int]:
+        a = np.ascontiguousarray(v, dtype=np.complex128)
+        n = self.geom.ndof
+        if a.size % n:
+            raise ValueError(f"vector size {a.size} is not a multiple of {n}")
+        return a.reshape(-1), a.size // n
+
+    def apply(self, op: int, psi: np.ndarray) -> np.ndarray:
+        a, nrhs = self._as_batch(psi)
+        out = np.empty_like(a)
+        _check(self._lib, self._lib.dwf_apply(self._h, op, _ptr(a), _ptr(out), nrhs), "dwf_apply")
+        return out.reshape(np.shape(psi))
+
+    def apply_device(self, op: int, precision: int, in_ptr: int, out_ptr: int, nrhs: int):
+        _check(self._lib, self._lib.dwf_apply_device(self._h, op, precision, in_ptr, out_ptr, nrhs),
+               "dwf_apply_device")
+
+    # solvers ------------------------------------------------------------------------
+    def solve(self, normal: int, b: np.ndarray, x0: np.ndarray | None = None, rtol: float = 1e-10,
+              maxiter: int = 10000, method: int = DWF_SOLVE_CG64, raise_on_fail: bool = True):
+        """Solve A x = b, A = D D^dag or D^dag D; returns (x, [stats dict per RHS])."""
+        bb, nrhs = self._as_batch(b)
+        x = np.empty_like(bb)
+        x0p = None
+        if x0 is not None:
+            x0a, n0 = self._as_batch(x0)
+            if n0 != nrhs:
+                raise ValueError("x0 and b differ in batch size")
+            x0p = _ptr(x0a)
+        stats = (SolveStats * nrhs)()
+        rc = self._lib.dwf_solve(self._h, normal, method, _ptr(bb), x0p, rtol, maxiter, nrhs,
+                                 _ptr(x), ctypes.cast(stats, ctypes.c_void_p))
+        if raise_on_fail:
+            _check(self._lib, rc, "dwf_solve")
+        return x.reshape(np.shape(b)), [s.as_dict() for s in stats], rc
+
+    def solve_device(self, normal: int, method: int, b_ptr: int, x0_ptr: int | None, rtol: float,
+                     maxiter: int, nrhs: int, x_ptr: int, raise_on_fail: bool = True):
+        stats = (SolveStats * nrhs)()
+        rc = self._lib.dwf_solve_device(self._h, normal, method, b_ptr, x0_ptr, rtol, maxiter, nrhs,
+                                        x_ptr, ctypes.cast(stats, ctypes.c_void_p))
+        if raise_on_fail:
+            _check(self._lib, rc, "dwf_solve_device")
+        return [s.as_dict() for s in stats], rc

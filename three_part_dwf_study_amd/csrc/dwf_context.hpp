@@ -1,0 +1,71 @@
+// dwf_context.hpp -- the opaque dwf_ctx of the C ABI (host side).
+#pragma once
+
+#include "dwf_internal.hpp"
+
+#include <functional>
+#include <stdexcept>
+#include <string>
+
+// Error carrying a C ABI status code; caught at the ABI boundary.
+struct DwfError : std::runtime_error {
+    int code;
+    DwfError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void dwf_set_error(const std::string& msg);
+
+// Workspace slots (grow-only device buffers).
+enum BufId : int {
+    B_HOST_IN = 0,
+    B_HOST_OUT,
+    B_APPLY_TMP,
+    B_R64,
+    B_T64,
+    B_P64,
+    B_R32,
+    B_T32,
+    B_P32,
+    B_X32,
+    B_PA,
+    B_PB,
+    B_PR,
+    B_STATE,
+    B_AUX0,
+    B_AUX1,
+    B_AUX2,
+    B_AUX3,
+    B_COUNT
+};
+
+struct dwf_ctx {
+    dwfk::Lat L{};
+    int kind = 0;  // 0 DWF, 1 Wilson
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double* theta = nullptr;  // [2][V4] FP64 angles on the device
+    double2* l64 = nullptr;   // [2][V4] (cos, sin) FP64
+    float2* l32 = nullptr;    // [2][V4] (cos, sin) FP32
+    bool have_gauge = false;
+    int chunks_override = 0;      // s-chunks per 4D site for the stencils (0: heuristic)
+    double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
+
+    struct Buf {
+        void* p = nullptr;
+        size_t n = 0;
+    } bufs[B_COUNT];
+
+    dwfk::CgState* st_host = nullptr;  // pinned mirror of the per-RHS solver states
+    int st_host_n = 0;
+    int last_slot = 0;
+
+    ~dwf_ctx();
+    void* buf(int id, size_t bytes);
+    dwfk::CgState* host_states(int nrhs);
+    static void check(hipError_t e, const char* what);
+    static void check_launch(const char* what);
+};
+
+void dwf_apply_device_impl(dwf_ctx* c, int op, int prec, const void* in, void* out, int nrhs);
+int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0,
+                          double rtol, int maxiter, int nrhs, double* x);

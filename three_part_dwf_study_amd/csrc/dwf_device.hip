@@ -1,0 +1,558 @@
+// dwf_device.hip -- gfx950 kernels of the Wilson / Shamir domain-wall hot path.
+//
+// Operator (reference domain_wall/include/dwf_operator.hpp:15-171 and
+// wilson/include/wilson_operator.hpp:10-62):
+//   (D psi)(s,x) = diag psi - 1/2 sum_mu [ bc U_mu(x) (1 - g.gamma_mu) psi(x+mu)
+//                                        + bc U_mu(x-mu)^* (1 + g.gamma_mu) psi(x-mu) ] + D5
+// with gamma_t = sigma1, gamma_x = sigma2, g = +1 for D and -1 for D^dagger, and the
+// Shamir fifth-dimension hops D5 (dwf_operator.hpp:71-86, 150-165).
+//
+// Layout in HBM: spinors keep the reference order [rhs][s][t][x][spin] with the two
+// complex spin components of a site packed in one float4 / double4, so a wavefront's
+// 64 consecutive 4D sites load 1 KiB (FP32) contiguously.  Links are [mu][t][x] complex
+// (cos theta, sin theta), evaluated once per gauge update, never per slice.
+//
+// One thread owns one 4D site and walks `spt` consecutive s-slices: the four links are
+// loaded once and reused for every slice, and the fifth-dimension neighbours come from
+// a register window (the previous and next centre spinors), so each slice costs the
+// centre + four 4D neighbours (the neighbours hit L1/L2; the algorithmic traffic is one
+// read of the input, one write of the output and one read of the links).
+//
+// Reductions are deterministic: fixed wavefront shuffle trees, one partial per block,
+// partials summed in a fixed order by whichever kernel consumes them.
+
+#include "dwf_internal.hpp"
+
+#include <cmath>
+
+namespace dwfk {
+namespace {
+
+template <typename R> struct Vec4;
+template <> struct Vec4<float> { using T = float4; };
+template <> struct Vec4<double> { using T = double4; };
+template <typename R> struct Cx;
+template <> struct Cx<float> { using T = float2; };
+template <> struct Cx<double> { using T = double2; };
+
+// Two complex spin components of one site: (a_re, a_im) spin 0, (b_re, b_im) spin 1.
+template <typename R> struct Sp {
+    R ar, ai, br, bi;
+};
+
+template <typename R> __device__ __forceinline__ Sp<R> ldsp(const R* base, long long i)
+{
+    const typename Vec4<R>::T v = reinterpret_cast<const typename Vec4<R>::T*>(base)[i];
+    return Sp<R>{v.x, v.y, v.z, v.w};
+}
+
+template <typename R> __device__ __forceinline__ void stsp(R* base, long long i, const Sp<R>& s)
+{
+    typename Vec4<R>::T v;
+    v.x = s.ar;
+    v.y = s.ai;
+    v.z = s.br;
+    v.w = s.bi;
+    reinterpret_cast<typename Vec4<R>::T*>(base)[i] = v;
+}
+
+template <typename R> __device__ __forceinline__ double norm2(const Sp<R>& s)
+{
+    const double a = s.ar, b = s.ai, c = s.br, d = s.bi;
+    return a * a + b * b + c * c + d * d;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum of two accumulators; the result is valid in thread 0.
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* sh)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[w] = a;
+        sh[4 + w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        b = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+    }
+}
+
+// Sum of element `which` of n (value, value) partial pairs, same result in every block
+// (fixed order).  Every thread of the block must call it.
+__device__ __forceinline__ double sum_partials(const double* p, int n, int which, double* sh)
+{
+    if (threadIdx.x < 64) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < n; i += 64) v += p[2 * i + which];
+        v = wave_sum(v);
+        if (threadIdx.x == 0) sh[0] = v;
+    }
+    __syncthreads();
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ------------------------------------------------------------------ stencil kernel
+template <typename R, bool DAG, bool D5, int EPI>
+__global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
+{
+    const int rhs = blockIdx.z;
+    const Lat& L = a.L;
+    if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
+
+    __shared__ double sh[8];
+    const int nblk = a.cfg.nblk();
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+
+    R alpha = 0;
+    if constexpr (EPI == EPI_CGUPD) {
+        const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
+        if (pAp == 0.0) return;  // block-uniform; the p-update kernel records the failure
+        alpha = (R)(a.st[rhs].rr[a.slot] / pAp);
+    }
+
+    const int V4 = L.V4;
+    const long long V5 = L.V5();
+    const long long rb = (long long)rhs * V5 * 4;  // offset in reals
+    const int site = blockIdx.x * kBlock + threadIdx.x;
+    double acc0 = 0.0, acc1 = 0.0;
+
+    const int s0 = blockIdx.y * a.cfg.spt;
+    const int s1 = min(L.Ls, s0 + a.cfg.spt);
+    if (site < V4 && s0 < s1) {
+        const int Nx = L.Nx, Nt = L.Nt;
+        const int t = site / Nx;
+        const int x = site - t * Nx;
+        const int tp = (t + 1 == Nt) ? 0 : t + 1;
+        const int tm = (t == 0) ? Nt - 1 : t - 1;
+        const int xp = (x + 1 == Nx) ? 0 : x + 1;
+        const int xm = (x == 0) ? Nx - 1 : x - 1;
+        const R stp = (t == Nt - 1) ? (R)L.bct : (R)1;
+        const R stm = (t == 0) ? (R)L.bct : (R)1;
+        const R sxp = (x == Nx - 1) ? (R)L.bcx : (R)1;
+        const R sxm = (x == 0) ? (R)L.bcx : (R)1;
+        using C = typename Cx<R>::T;
+        const C* lk = static_cast<const C*>(a.links);
+        const C ut = lk[site];
+        const C ux = lk[V4 + site];
+        const C utm = lk[tm * Nx + x];
+        const C uxm = lk[V4 + t * Nx + xm];
+        const int ntp = tp * Nx + x, ntm = tm * Nx + x, nxp = t * Nx + xp, nxm = t * Nx + xm;
+        const R diag = (R)L.diag, mf = (R)L.mf, h = (R)-0.5;
+        constexpr R sg = DAG ? (R)-1 : (R)1;
+
+        const R* in = a.in + rb;
+        Sp<R> cur = ldsp(in, (long long)s0 * V4 + site);
+        Sp<R> dn{0, 0, 0, 0};
+        if constexpr (D5) dn = ldsp(in, (long long)((s0 == 0) ? L.Ls - 1 : s0 - 1) * V4 + site);
+
+        for (int s = s0; s < s1; ++s) {
+            const long long sl = (long long)s * V4;
+            Sp<R> up{0, 0, 0, 0};
+            if constexpr (D5) up = ldsp(in, (long long)((s + 1 == L.Ls) ? 0 : s + 1) * V4 + site);
+            const Sp<R> vtp = ldsp(in, sl + ntp);
+            const Sp<R> vtm = ldsp(in, sl + ntm);
+            const Sp<R> vxp = ldsp(in, sl + nxp);
+            const Sp<R> vxm = ldsp(in, sl + nxm);
+
+            Sp<R> o{diag * cur.ar, diag * cur.ai, diag * cur.br, diag * cur.bi};
+            {   // forward t: U_t(x) (1 - g sigma1) psi(x+t): upper a = v0 - g v1, lower -g a
+                const R ar = stp * (vtp.ar - sg * vtp.br), ai = stp * (vtp.ai - sg * vtp.bi);
+                const R hr = ut.x * ar - ut.y * ai, hi = ut.x * ai + ut.y * ar;
+                o.ar += h * hr;
+                o.ai += h * hi;
+                o.br -= h * sg * hr;
+                o.bi -= h * sg * hi;
+            }
+            {   // backward t: U_t(x-t)^* (1 + g sigma1) psi(x-t): upper b = v0 + g v1, lower g b
+                const R br = stm * (vtm.ar + sg * vtm.br), bi = stm * (vtm.ai + sg * vtm.bi);
+                const R hr = utm.x * br + utm.y * bi, hi = utm.x * bi - utm.y * br;
+                o.ar += h * hr;
+                o.ai += h * hi;
+                o.br += h * sg * hr;
+                o.bi += h * sg * hi;
+            }
+            {   // forward x: U_x(x) (1 - g sigma2) psi(x+x): upper c = v0 + g i v1, lower -g i c
+                const R cr = sxp * (vxp.ar - sg * vxp.bi), ci = sxp * (vxp.ai + sg * vxp.br);
+                const R hr = ux.x * cr - ux.y * ci, hi = ux.x * ci + ux.y * cr;
+                o.ar += h * hr;
+                o.ai += h * hi;
+                o.br += h * sg * hi;   // -g i (hr + i hi) = g hi - i g hr
+                o.bi -= h * sg * hr;
+            }
+            {   // backward x: U_x(x-x)^* (1 + g sigma2) psi(x-x): upper d = v0 - g i v1, lower g i d
+                const R dr = sxm * (vxm.ar + sg * vxm.bi), di = sxm * (vxm.ai - sg * vxm.br);
+                const R hr = uxm.x * dr + uxm.y * di, hi = uxm.x * di - uxm.y * dr;
+                o.ar += h * hr;
+                o.ai += h * hi;
+                o.br -= h * sg * hi;   // g i (hr + i hi) = -g hi + i g hr
+                o.bi += h * sg * hr;
+            }
+            if constexpr (D5) {
+                // D:   spin1 <- s+1 (wall: +mf psi(0)),    spin0 <- s-1 (wall: +mf psi(Ls-1))
+                // D^+: spin1 <- s-1 (wall: +mf psi(Ls-1)), spin0 <- s+1 (wall: +mf psi(0))
+                const R cup = (s + 1 == L.Ls) ? mf : (R)-1;
+                const R cdn = (s == 0) ? mf : (R)-1;
+                if constexpr (!DAG) {
+                    o.br += cup * up.br;
+                    o.bi += cup * up.bi;
+                    o.ar += cdn * dn.ar;
+                    o.ai += cdn * dn.ai;
+                } else {
+                    o.br += cdn * dn.br;
+                    o.bi += cdn * dn.bi;
+                    o.ar += cup * up.ar;
+                    o.ai += cup * up.ai;
+                }
+            }
+
+            const long long idx = sl + site;
+            if constexpr (EPI == EPI_STORE) {
+                stsp(a.out + rb, idx, o);
+            } else if constexpr (EPI == EPI_NORM) {
+                stsp(a.out + rb, idx, o);
+                acc0 += norm2(o);
+            } else if constexpr (EPI == EPI_CGUPD) {
+                const Sp<R> pv = ldsp(a.p + rb, idx);
+                Sp<R> xv = ldsp(a.x + rb, idx);
+                Sp<R> rv = ldsp(a.r + rb, idx);
+                xv.ar += alpha * pv.ar;
+                xv.ai += alpha * pv.ai;
+                xv.br += alpha * pv.br;
+                xv.bi += alpha * pv.bi;
+                rv.ar -= alpha * o.ar;
+                rv.ai -= alpha * o.ai;
+                rv.br -= alpha * o.br;
+                rv.bi -= alpha * o.bi;
+                stsp(a.x + rb, idx, xv);
+                stsp(a.r + rb, idx, rv);
+                acc0 += norm2(rv);
+            } else {  // EPI_RESID (FP64 only)
+                if constexpr (sizeof(R) == 8) {
+                    const Sp<double> bv = ldsp(a.b + rb, idx);
+                    const Sp<double> rv{bv.ar - o.ar, bv.ai - o.ai, bv.br - o.br, bv.bi - o.bi};
+                    stsp(a.r64 + rb, idx, rv);
+                    if (a.p64) stsp(a.p64 + rb, idx, rv);
+                    if (a.r32 || a.p32) {
+                        const Sp<float> fv{(float)rv.ar, (float)rv.ai, (float)rv.br, (float)rv.bi};
+                        if (a.r32) stsp(a.r32 + rb, idx, fv);
+                        if (a.p32) stsp(a.p32 + rb, idx, fv);
+                    }
+                    acc0 += norm2(rv);
+                    acc1 += norm2(bv);
+                }
+            }
+
+            if constexpr (D5) {
+                dn = cur;
+                cur = up;
+            } else if (s + 1 < s1) {
+                cur = ldsp(in, sl + V4 + site);
+            }
+        }
+    }
+
+    if constexpr (EPI != EPI_STORE) {
+        block_sum2(acc0, acc1, sh);
+        if (threadIdx.x == 0) {
+            double* pp = a.part + ((size_t)rhs * nblk + bid) * 2;
+            pp[0] = acc0;
+            pp[1] = acc1;
+        }
+    }
+}
+
+// ------------------------------------------------------------- CG direction update
+// Reduces r.r and p^dag A p from the stencil partials, decides convergence, and sets
+// p = r + beta p.  Block 0 writes the next slot's scalars; the other blocks only read
+// the current slot, so nothing in one launch races.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cg_p(const long long V5, const int nblk, R* r, R* p,
+                                                 const double* partA, const double* partB,
+                                                 CgState* st, const int slot, const int mode)
+{
+    const int rhs = blockIdx.z;
+    CgState& S = st[rhs];
+    const int f = S.flags[slot];
+    const int ns = slot ^ 1;
+    const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+    if (f & (F_DONE | F_FROZEN)) {
+        if (lead) {
+            S.flags[ns] = f;
+            S.rr[ns] = S.rr[slot];
+        }
+        return;
+    }
+    __shared__ double sh[2];
+    const double pAp = sum_partials(partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
+    const bool zero = (pAp == 0.0);
+    const double rr_new = zero ? 0.0 : sum_partials(partB + (size_t)rhs * nblk * 2, nblk, 0, sh);
+    const double rr_old = S.rr[slot];
+    const double relres = std::sqrt(rr_new) / std::sqrt(S.bnorm2);
+    const bool conv = relres <= S.tol;
+    const bool freeze = (mode == MODE_MIXED) && (conv || rr_new <= S.delta * S.delta * S.rr_ref);
+    if (lead) {
+        int nf = f;
+        if (zero) {
+            S.status = 4;
+            nf |= F_DONE;
+            S.rr[ns] = rr_old;
+        } else {
+            S.rr[ns] = rr_new;
+            S.relres = relres;
+            S.iters += 1;
+            if (mode == MODE_CG64 && conv) nf |= F_DONE;
+            if (freeze) nf |= F_FROZEN;
+        }
+        S.flags[ns] = nf;
+    }
+    if (zero || (mode == MODE_CG64 && conv)) return;
+    const R beta = (R)(rr_new / rr_old);
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i < V5) {
+        const long long rb = (long long)rhs * V5 * 4;
+        const Sp<R> rv = ldsp(r + rb, i);
+        Sp<R> pv = ldsp(p + rb, i);
+        pv.ar = rv.ar + beta * pv.ar;
+        pv.ai = rv.ai + beta * pv.ai;
+        pv.br = rv.br + beta * pv.br;
+        pv.bi = rv.bi + beta * pv.bi;
+        stsp(p + rb, i, pv);
+    }
+}
+
+// ------------------------------------------------------------------ scalar kernels
+// One 64-thread block per right-hand side.
+__global__ void k_cg_start(const int nblk, const double* partR, CgState* st, const double tol,
+                           const double delta)
+{
+    const int rhs = blockIdx.x;
+    const double* p = partR + (size_t)rhs * nblk * 2;
+    double rr = 0.0, bb = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 64) {
+        rr += p[2 * i];
+        bb += p[2 * i + 1];
+    }
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    if (threadIdx.x == 0) {
+        CgState S{};
+        S.bnorm2 = bb;
+        S.tol = tol;
+        S.delta = delta;
+        S.rr[0] = S.rr[1] = rr;
+        S.rr_ref = rr;
+        int flags = 0;
+        const double bnorm = std::sqrt(bb);
+        if (bnorm == 0.0) {
+            flags = F_DONE;
+            S.relres = 0.0;
+            S.true_relres = 0.0;
+        } else {
+            S.relres = std::sqrt(rr) / bnorm;
+            S.true_relres = S.relres;
+            if (S.relres <= tol) flags = F_DONE;
+        }
+        S.flags[0] = S.flags[1] = flags;
+        st[rhs] = S;
+    }
+}
+
+// After a reliable update: FP64 true residual -> certify, or restart the FP32 cycle.
+__global__ void k_ru_commit(const int nblk, const double* partR, CgState* st, const int slot)
+{
+    const int rhs = blockIdx.x;
+    CgState& S = st[rhs];
+    if (S.flags[slot] & F_DONE) return;
+    const double* p = partR + (size_t)rhs * nblk * 2;
+    double rr = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 64) rr += p[2 * i];
+    rr = wave_sum(rr);
+    if (threadIdx.x == 0) {
+        const double tr = std::sqrt(rr) / std::sqrt(S.bnorm2);
+        S.true_relres = tr;
+        S.reliable += 1;
+        if (tr <= S.tol) {
+            S.flags[slot] |= F_DONE;
+        } else {
+            S.rr[slot] = rr;
+            S.rr_ref = rr;
+            S.flags[slot] &= ~F_FROZEN;
+        }
+    }
+}
+
+__global__ void k_true_resid(const int nblk, const double* partR, CgState* st)
+{
+    const int rhs = blockIdx.x;
+    const double* p = partR + (size_t)rhs * nblk * 2;
+    double rr = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 64) rr += p[2 * i];
+    rr = wave_sum(rr);
+    if (threadIdx.x == 0) {
+        const double bn = std::sqrt(st[rhs].bnorm2);
+        st[rhs].true_relres = bn == 0.0 ? 0.0 : std::sqrt(rr) / bn;
+    }
+}
+
+__global__ void k_set_frozen_all(CgState* st, const int slot, const int nrhs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrhs && !(st[i].flags[slot] & F_DONE)) st[i].flags[slot] |= F_FROZEN;
+}
+
+// x64 += x32; x32 = 0 for the right-hand sides still running.
+__global__ __launch_bounds__(kBlock) void k_accum_x(const long long n4, double* x64, float* x32,
+                                                    const CgState* st, const int slot)
+{
+    const int rhs = blockIdx.z;
+    if (st[rhs].flags[slot] & F_DONE) return;
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;  // site
+    if (i * 4 >= n4) return;
+    const long long rb = (long long)rhs * n4;
+    const Sp<float> d = ldsp(x32 + rb, i);
+    Sp<double> v = ldsp(x64 + rb, i);
+    v.ar += d.ar;
+    v.ai += d.ai;
+    v.br += d.br;
+    v.bi += d.bi;
+    stsp(x64 + rb, i, v);
+    stsp(x32 + rb, i, Sp<float>{0.f, 0.f, 0.f, 0.f});
+}
+
+template <typename R> __global__ void k_zero(R* p, const long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (R)0;
+}
+
+__global__ void k_pack_links(const int n, const double* theta, double2* l64, float2* l32)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s, c;
+    sincos(theta[i], &s, &c);
+    l64[i] = make_double2(c, s);
+    l32[i] = make_float2((float)c, (float)s);
+}
+
+template <typename R, bool DAG, bool D5>
+void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
+{
+    switch (epi) {
+        case EPI_STORE: k_hop<R, DAG, D5, EPI_STORE><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_NORM: k_hop<R, DAG, D5, EPI_NORM><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_CGUPD: k_hop<R, DAG, D5, EPI_CGUPD><<<grid, kBlock, 0, s>>>(a); break;
+        default:
+            if constexpr (sizeof(R) == 8) k_hop<R, DAG, D5, EPI_RESID><<<grid, kBlock, 0, s>>>(a);
+            break;
+    }
+}
+
+}  // namespace
+
+HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override)
+{
+    HopCfg c{};
+    c.bx = (L.V4 + kBlock - 1) / kBlock;
+    int chunks = 1;
+    if (chunks_override > 0) {
+        chunks = chunks_override;
+    } else {
+        // Enough blocks to cover 256 CUs several times over; fewer chunks keep more
+        // slices per thread (more link reuse) once the batch alone fills the chip.
+        while (chunks < L.Ls && (long long)c.bx * chunks * nrhs < 2048) chunks *= 2;
+    }
+    if (chunks > L.Ls) chunks = L.Ls;
+    if (chunks < 1) chunks = 1;
+    c.spt = (L.Ls + chunks - 1) / chunks;
+    c.chunks = (L.Ls + c.spt - 1) / c.spt;
+    return c;
+}
+
+template <typename R> void launch_hop(bool dagger, int epi, const HopArgs<R>& a, int nrhs, hipStream_t s)
+{
+    const dim3 grid(a.cfg.bx, a.cfg.chunks, nrhs);
+    if (a.L.d5) {
+        if (dagger)
+            dispatch_epi<R, true, true>(epi, a, grid, s);
+        else
+            dispatch_epi<R, false, true>(epi, a, grid, s);
+    } else {
+        if (dagger)
+            dispatch_epi<R, true, false>(epi, a, grid, s);
+        else
+            dispatch_epi<R, false, false>(epi, a, grid, s);
+    }
+}
+template void launch_hop<float>(bool, int, const HopArgs<float>&, int, hipStream_t);
+template void launch_hop<double>(bool, int, const HopArgs<double>&, int, hipStream_t);
+
+template <typename R>
+void launch_cg_p(const Lat& L, const HopCfg& cfg, R* r, R* p, const double* partA, const double* partB,
+                 CgState* st, int slot, int mode, int nrhs, hipStream_t s)
+{
+    const long long V5 = L.V5();
+    const dim3 grid((unsigned)((V5 + kBlock - 1) / kBlock), 1, nrhs);
+    k_cg_p<R><<<grid, kBlock, 0, s>>>(V5, cfg.nblk(), r, p, partA, partB, st, slot, mode);
+}
+template void launch_cg_p<float>(const Lat&, const HopCfg&, float*, float*, const double*, const double*,
+                                 CgState*, int, int, int, hipStream_t);
+template void launch_cg_p<double>(const Lat&, const HopCfg&, double*, double*, const double*,
+                                  const double*, CgState*, int, int, int, hipStream_t);
+
+void launch_cg_start(const HopCfg& cfg, const double* partR, CgState* st, double tol, double delta,
+                     int nrhs, hipStream_t s)
+{
+    k_cg_start<<<nrhs, 64, 0, s>>>(cfg.nblk(), partR, st, tol, delta);
+}
+
+void launch_ru_commit(const HopCfg& cfg, const double* partR, CgState* st, int slot, int nrhs,
+                      hipStream_t s)
+{
+    k_ru_commit<<<nrhs, 64, 0, s>>>(cfg.nblk(), partR, st, slot);
+}
+
+void launch_true_resid(const HopCfg& cfg, const double* partR, CgState* st, int nrhs, hipStream_t s)
+{
+    k_true_resid<<<nrhs, 64, 0, s>>>(cfg.nblk(), partR, st);
+}
+
+void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s)
+{
+    k_set_frozen_all<<<(nrhs + 63) / 64, 64, 0, s>>>(st, slot, nrhs);
+}
+
+void launch_accum_x(long long n4, double* x64, float* x32, const CgState* st, int slot, int nrhs,
+                    hipStream_t s)
+{
+    const long long sites = n4 / 4;
+    const dim3 grid((unsigned)((sites + kBlock - 1) / kBlock), 1, nrhs);
+    k_accum_x<<<grid, kBlock, 0, s>>>(n4, x64, x32, st, slot);
+}
+
+template <typename R> void launch_zero(R* p, long long n, hipStream_t s)
+{
+    if (n <= 0) return;
+    k_zero<R><<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(p, n);
+}
+template void launch_zero<float>(float*, long long, hipStream_t);
+template void launch_zero<double>(double*, long long, hipStream_t);
+
+void launch_pack_links(const Lat& L, const double* theta, double2* l64, float2* l32, hipStream_t s)
+{
+    const int n = 2 * L.V4;
+    k_pack_links<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(n, theta, l64, l32);
+}
+
+}  // namespace dwfk

@@ -1,0 +1,101 @@
+// dwf_internal.hpp -- types shared by the gfx950 kernels and the host driver of
+// libdwfhip.so.  Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace dwfk {
+
+constexpr int kBlock = 256;  // threads per block for every stencil/BLAS kernel
+
+// Per right-hand-side CG state.  Fields a kernel reads are never written by the same
+// launch: scalars the next iteration needs live in slot (it+1)&1 while iteration `it`
+// reads slot it&1, so blocks of one launch never race.
+enum : int { F_DONE = 1, F_FROZEN = 2 };
+enum : int { MODE_CG64 = 0, MODE_MIXED = 1 };
+
+struct CgState {
+    double rr[2];       // r.r of the current residual, by slot
+    double bnorm2;      // b.b
+    double rr_ref;      // mixed: r.r at the last reliable update
+    double relres;      // last recursive ||r||/||b||
+    double true_relres; // last FP64 ||b-Ax||/||b||
+    double tol;         // relative tolerance
+    double delta;       // mixed: reliable update when ||r|| < delta ||r_ref||
+    int flags[2];       // F_DONE / F_FROZEN, by slot
+    int iters;          // CG iterations performed in this precision
+    int status;         // 0, or 4 for a zero denominator
+    int reliable;       // reliable updates performed
+    int fp64_iters;     // FP64 iterations (fallback)
+};
+
+// Lattice and operator constants.  Ls = 1 and d5 = 0 give the Wilson operator.
+struct Lat {
+    int Nt, Nx, Ls, V4;
+    int d5;             // 1: Shamir fifth-dimension hops with wall coupling mf
+    double diag;        // 3 - M5 (DWF) or 2 + m0 (Wilson)
+    double mf;
+    double bct, bcx;    // boundary signs across t = Nt-1 -> 0 and x = Nx-1 -> 0
+    __host__ __device__ long long V5() const { return (long long)V4 * Ls; }
+};
+
+// Launch shape of the stencil kernels: one thread per 4D site per s-chunk; each thread
+// walks spt consecutive slices reusing its four links.
+struct HopCfg {
+    int bx;      // blocks along the 4D sites
+    int chunks;  // s-chunks (grid.y)
+    int spt;     // slices per thread
+    __host__ __device__ int nblk() const { return bx * chunks; }
+};
+
+enum Epi : int {
+    EPI_STORE = 0,  // out = Op in
+    EPI_NORM = 1,   // out = Op in, partial |out|^2
+    EPI_CGUPD = 2,  // alpha = rr/sum(partA); x += alpha p; r -= alpha (Op in); partial |r|^2
+    EPI_RESID = 3   // r64 = b - Op in (+ copies to p64/r32/p32); partials |r|^2, |b|^2
+};
+
+template <typename R> struct HopArgs {
+    Lat L;
+    HopCfg cfg;
+    const void* links;   // [2][V4] complex R: (cos theta, sin theta)
+    const R* in;         // [nrhs][V5] spinors of 4 R
+    R* out;              // EPI_STORE / EPI_NORM
+    R* x;                // EPI_CGUPD
+    R* r;                // EPI_CGUPD
+    const R* p;          // EPI_CGUPD
+    const double* partA; // EPI_CGUPD: partials of p^dag A p from the first stencil
+    const double* b;     // EPI_RESID
+    double* r64;         // EPI_RESID
+    double* p64;         // EPI_RESID (nullable)
+    float* r32;          // EPI_RESID (nullable)
+    float* p32;          // EPI_RESID (nullable)
+    double* part;        // partial sums, 2 per block, [nrhs][nblk][2]
+    const CgState* st;   // nullable
+    int slot;
+    int skip;            // skip a right-hand side whose flags[slot] & skip
+};
+
+// Host launchers (dwf_device.hip).
+void launch_pack_links(const Lat& L, const double* theta, double2* l64, float2* l32, hipStream_t s);
+template <typename R>
+void launch_hop(bool dagger, int epi, const HopArgs<R>& a, int nrhs, hipStream_t s);
+template <typename R>
+void launch_cg_p(const Lat& L, const HopCfg& cfg, R* r, R* p, const double* partA,
+                 const double* partB, CgState* st, int slot, int mode, int nrhs, hipStream_t s);
+void launch_cg_start(const HopCfg& cfg, const double* partR, CgState* st, double tol, double delta,
+                     int nrhs, hipStream_t s);
+void launch_ru_commit(const HopCfg& cfg, const double* partR, CgState* st, int slot, int nrhs,
+                      hipStream_t s);
+void launch_true_resid(const HopCfg& cfg, const double* partR, CgState* st, int nrhs, hipStream_t s);
+void launch_accum_x(long long n4, double* x64, float* x32, const CgState* st, int slot, int nrhs,
+                    hipStream_t s);
+template <typename R> void launch_zero(R* p, long long n, hipStream_t s);
+void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s);
+
+HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override = 0);
+
+}  // namespace dwfk
