@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark of the DWF fermion hot path on one MI355X.
+
+Workload (one "step"): solve D^dag D x = D^dag b for a batch of boundary sources on a
+96x96 lattice with Ls=16, M5=1.0, m_f=0.05 and links theta ~ N(0, 1/beta), beta=4, to an
+FP64 true residual ||D^dag b - D^dag D x|| / ||D^dag b|| <= 1e-10.  Sources are complex
+Gaussian eta(t,x,spin) placed as P_L eta at s=0 and P_R eta at s=Ls-1, as
+domain_wall/include/correlator.hpp:60-75 places its noise.
+
+``value`` is the solves/s of the fast solver (FP32 CG with FP64 reliable updates,
+certified by the FP64 true residual); the plain FP64 CG (the reference's algorithm) is
+timed on the same batch beside it.  The roofline object covers the dominant kernel, the
+FP32 Dirac stencil, measured with HIP events on the library's stream over a batch of
+128 right-hand sides (604 MB, larger than the 256 MB MALL).  The CPU baseline is the
+oracle's FP64 CG (the reference's algorithm and loop order) on one host core.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DWF solves/s 96²,Ls=16 @1e-10: learned+FP32 vs plain FP64 CG; Dslash HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+NT = NX = 96
+LS = 16
+M5, MF, BETA = 1.0, 0.05, 4.0
+TOL = 1e-10
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def boundary_sources(nrhs, seed):
+    rng = np.random.default_rng(seed)
+    eta = rng.normal(size=(nrhs, NT, NX, 2)) + 1j * rng.normal(size=(nrhs, NT, NX, 2))
+    src = np.zeros((nrhs, LS, NT, NX, 2), np.complex128)
+    src[:, 0, :, :, 1] = eta[..., 1]        # P_L = diag(0, 1) at s = 0
+    src[:, LS - 1, :, :, 0] = eta[..., 0]   # P_R = diag(1, 0) at s = Ls-1
+    return src
+
+
+def dslash_bytes(prec_bytes, nrhs):
+    """Algorithmic bytes of one D apply: read psi, write out, read cos/sin links."""
+    V4 = NT * NX
+    Bv = V4 * LS * 4 * prec_bytes
+    BL = V4 * 4 * prec_bytes
+    return nrhs * 2 * Bv + BL
+
+
+def time_dslash(torch, ctx, dwf, prec, nrhs, reps):
+    dt = torch.float32 if prec == dwf.DWF_PREC_FP32 else torch.float64
+    n = nrhs * LS * NT * NX * 4
+    g = torch.Generator(device="cuda")
+    g.manual_seed(77)
+    src = torch.randn(n, dtype=dt, device="cuda", generator=g)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    for _ in range(3):
+        ctx.apply_device(dwf.DWF_OP_D, prec, src.data_ptr(), dst.data_ptr(), nrhs)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        ctx.apply_device(dwf.DWF_OP_D, prec, src.data_ptr(), dst.data_ptr(), nrhs)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = dslash_bytes(4 if prec == dwf.DWF_PREC_FP32 else 8, nrhs)
+    del src, dst
+    torch.cuda.empty_cache()
+    return ms, nbytes / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(theta, b1, judge_iters, sample_iters):
+    import oracle
+
+    t0 = time.perf_counter()
+    _, it, _, _, _ = oracle.dwf_cg(3, theta, b1, NT, NX, LS, M5, MF, TOL, sample_iters)
+    dt = time.perf_counter() - t0
+    per_iter = dt / max(1, it)
+    return {
+        "value": 1.0 / (per_iter * judge_iters),
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle FP64 CG (reference algorithm, 1 core) on one 96x96x16 source: "
+                   f"{it} iterations timed ({dt:.1f} s, {per_iter * 1e3:.1f} ms/iteration incl. the "
+                   f"initial residual), scaled by the {judge_iters:.0f} iterations the FP64 judge needs"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nrhs", type=int, default=16, help="right-hand sides per step")
+    ap.add_argument("--dslash-rhs", type=int, default=128)
+    ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (independent "
+              "configurations shard without communication); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import three_part_dwf_study_amd as dwf
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU (libdwfhip has no CPU path)")
+
+    rng = np.random.default_rng(1234)
+    theta = rng.normal(0.0, 1.0 / np.sqrt(BETA), size=2 * NT * NX)
+    nrhs = args.nrhs
+    src = boundary_sources(nrhs, 918273)
+    ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
+    ctx.set_gauge(theta)
+
+    n = nrhs * LS * NT * NX * 4
+    d_src = torch.from_numpy(src.reshape(-1).view(np.float64).copy()).cuda()
+    d_b = torch.empty(n, dtype=torch.float64, device="cuda")
+    d_x = torch.empty_like(d_b)
+    torch.cuda.synchronize()
+    ctx.apply_device(dwf.DWF_OP_DDAG, dwf.DWF_PREC_FP64, d_src.data_ptr(), d_b.data_ptr(), nrhs)
+    torch.cuda.synchronize()
+
+    def solve(method):
+        st, rc = ctx.solve_device(dwf.DWF_OP_DDAGD, method, d_b.data_ptr(), None, TOL, 20000, nrhs,
+                                  d_x.data_ptr())
+        return st
+
+    results = {}
+    for name, method, steps in (("mixed", dwf.DWF_SOLVE_MIXED, args.steps),
+                                ("fp64", dwf.DWF_SOLVE_CG64, max(1, args.steps))):
+        for _ in range(args.warmup):
+            solve(method)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st = solve(method)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        tr = max(s["true_relres"] for s in st)
+        if tr > TOL or not all(s["converged"] for s in st):
+            raise SystemExit(f"{name} solve failed certification: max true residual {tr:.3e}")
+        results[name] = dict(seconds=dt, steps=steps, solves_per_s=steps * nrhs / dt,
+                             ms_per_step=dt / steps * 1e3,
+                             iterations=float(np.mean([s["iterations"] for s in st])),
+                             reliable_updates=float(np.mean([s["reliable_updates"] for s in st])),
+                             fp64_iterations=float(np.mean([s["fp64_iterations"] for s in st])),
+                             max_true_relres=tr)
+        log(f"{name}: {results[name]}")
+
+    ms32, gbs32 = time_dslash(torch, ctx, dwf, dwf.DWF_PREC_FP32, args.dslash_rhs, 20)
+    ms64, gbs64 = time_dslash(torch, ctx, dwf, dwf.DWF_PREC_FP64, args.dslash_rhs // 2, 20)
+    ms1, gbs1 = time_dslash(torch, ctx, dwf, dwf.DWF_PREC_FP32, 1, 200)
+    log(f"dslash fp32 x{args.dslash_rhs}: {ms32:.4f} ms {gbs32:.0f} GB/s; fp64 x{args.dslash_rhs // 2}: "
+        f"{ms64:.4f} ms {gbs64:.0f} GB/s; fp32 x1 (cache-resident): {ms1 * 1e3:.1f} us {gbs1:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        b1 = d_b[: LS * NT * NX * 4].cpu().numpy().view(np.complex128)
+        cpu = cpu_baseline(theta, b1, results["fp64"]["iterations"], args.cpu_iters)
+        log(f"cpu baseline: {cpu}")
+
+    mixed, judge = results["mixed"], results["fp64"]
+    line = {
+        "metric": METRIC,
+        "value": round(mixed["solves_per_s"], 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": mixed["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "DWF CGNR solves, 96x96 Ls=16 M5=1 mf=0.05, theta~N(0,1/4) seed 1234, "
+                        "Gaussian boundary sources, FP64 true residual <= 1e-10",
+            "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs, "solver": "FP32 CG + FP64 reliable updates",
+        },
+        "fp64_cg_solves_per_s": round(judge["solves_per_s"], 3),
+        "speedup_vs_fp64_cg": round(mixed["solves_per_s"] / judge["solves_per_s"], 3),
+        "iterations": {"mixed_fp32": mixed["iterations"], "mixed_reliable_updates": mixed["reliable_updates"],
+                       "fp64_cg": judge["iterations"]},
+        "max_true_relres": {"mixed": mixed["max_true_relres"], "fp64_cg": judge["max_true_relres"]},
+        "dslash_gbs": {"fp32_batch%d" % args.dslash_rhs: round(gbs32, 1),
+                       "fp64_batch%d" % (args.dslash_rhs // 2): round(gbs64, 1),
+                       "fp32_single_rhs_cache_resident": round(gbs1, 1)},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(gbs32, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(gbs32 / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k_hop<float,D,DWF,EPI_STORE>",
+            "avg_launch_us": round(ms32 * 1e3, 2),
+            "bytes_per_launch": dslash_bytes(4, args.dslash_rhs),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

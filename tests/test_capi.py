@@ -1,0 +1,62 @@
+"""Host-side checks of the C ABI that need no GPU: the library loads, exports every
+symbol include/*.h declares, and fails loudly (no CPU fallback) without a device."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+import three_part_dwf_study_amd as dwf
+from three_part_dwf_study_amd import capi
+
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if f.endswith(".h")]
+
+
+def declared_symbols():
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(dwf_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    for must in ("dwf_ctx_create_dwf", "dwf_apply", "dwf_solve", "dwf_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = dwf.load_library()
+    out = subprocess.run(["nm", "-D", "--defined-only", dwf.library_path()], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r"\s[TW]\s+(\w+)$", out, flags=re.M))
+    for name in declared_symbols():
+        assert name in exported, name
+        assert getattr(lib, name) is not None
+        assert name in capi.PROTOTYPES, f"{name} has no ctypes prototype"
+
+
+def test_no_device_fails_loudly():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(dwf.DwfError) as e:
+        dwf.DwfContext.dwf(4, 4, 4, 1.0, 0.05)
+    assert e.value.code == capi.DWF_ERR_HIP
+    assert "no HIP device" in str(e.value)
+
+
+def test_bad_arguments_rejected():
+    lib = dwf.load_library()
+    h = ctypes.c_void_p()
+    assert lib.dwf_ctx_create_dwf(ctypes.byref(h), 0, 4, 4, 1.0, 0.05) == capi.DWF_ERR_ARG
+    assert b"positive" in lib.dwf_last_error()
+    assert lib.dwf_apply(None, 0, None, None, 1) == capi.DWF_ERR_ARG
+    assert lib.dwf_solve(None, 3, 0, None, None, 1e-8, 10, 1, None, None) == capi.DWF_ERR_ARG

@@ -1,0 +1,83 @@
+"""GPU parity of the Dirac operators (through the C ABI) against the oracle.
+
+Bar: FP64 <= 1e-12 relative (the north-star Dirac-apply tolerance), FP32 <= 2e-6
+relative (single-precision rounding of O(10) terms per component)."""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import print_case
+from dense_ref import rand_spinor, rand_theta
+
+import three_part_dwf_study_amd as dwf
+
+pytestmark = pytest.mark.gpu
+
+M5, MF = 1.0, 0.05
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("Nt,Nx,Ls", [(16, 16, 8), (6, 10, 4), (4, 4, 1), (96, 96, 16)])
+def test_dwf_ops_fp64(Nt, Nx, Ls):
+    print_case(test="dwf_ops_fp64", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(Nt * 1000 + Nx * 10 + Ls)
+    th = rand_theta(rng, Nt, Nx, 0.5)
+    psi = rand_spinor(rng, Ls * Nt * Nx * 2)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        for op in (0, 1, 2, 3):
+            got = ctx.apply(op, psi)
+            want = oracle.dwf_apply(op, th, psi, Nt, Nx, Ls, M5, MF)
+            assert rel(got, want) <= 1e-12, (op, rel(got, want))
+
+
+def test_dwf_batch_matches_single():
+    Nt, Nx, Ls, n = 8, 12, 6, 3
+    print_case(test="dwf_batch", Nt=Nt, Nx=Nx, Ls=Ls, nrhs=n)
+    rng = np.random.default_rng(7)
+    th = rand_theta(rng, Nt, Nx)
+    psis = rand_spinor(rng, n * Ls * Nt * Nx * 2).reshape(n, -1)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        batch = ctx.apply(0, psis)
+        for i in range(n):
+            assert rel(batch[i], oracle.dwf_apply(0, th, psis[i], Nt, Nx, Ls, M5, MF)) <= 1e-12
+
+
+@pytest.mark.parametrize("Nt,Nx,bct,bcx", [(16, 16, -1.0, 1.0), (5, 7, 1.0, -1.0), (48, 128, -1.0, 1.0)])
+def test_wilson_ops_fp64(Nt, Nx, bct, bcx):
+    print_case(test="wilson_ops_fp64", Nt=Nt, Nx=Nx)
+    rng = np.random.default_rng(11)
+    th = rng.uniform(-np.pi, np.pi, size=2 * Nt * Nx)
+    psi = rand_spinor(rng, Nt * Nx * 2)
+    m0 = 0.15
+    with dwf.DwfContext.wilson(Nt, Nx, m0, bct, bcx) as ctx:
+        ctx.set_gauge(th)
+        for op in (0, 1):
+            got = ctx.apply(op, psi)
+            want = oracle.wilson_apply(op, th, psi, Nt, Nx, m0, bct, bcx)
+            assert rel(got, want) <= 1e-12
+
+
+@pytest.mark.parametrize("Nt,Nx,Ls", [(16, 16, 8), (96, 96, 16)])
+def test_dwf_ops_fp32_device(Nt, Nx, Ls):
+    torch = pytest.importorskip("torch")
+    print_case(test="dwf_ops_fp32", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(3)
+    th = rand_theta(rng, Nt, Nx)
+    psi = rand_spinor(rng, Ls * Nt * Nx * 2)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        for prec, dt, tol in ((dwf.DWF_PREC_FP32, torch.float32, 2e-6), (dwf.DWF_PREC_FP64, torch.float64, 1e-12)):
+            src = torch.from_numpy(psi.view(np.float64).astype(np.float64)).to(dt).cuda()
+            dst = torch.empty_like(src)
+            for op in (0, 1, 3):
+                ctx.apply_device(op, prec, src.data_ptr(), dst.data_ptr(), 1)
+                torch.cuda.synchronize()
+                got = dst.double().cpu().numpy().view(np.complex128)
+                want = oracle.dwf_apply(op, th, psi, Nt, Nx, Ls, M5, MF)
+                assert rel(got, want) <= tol, (prec, op, rel(got, want))
