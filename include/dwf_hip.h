@@ -120,6 +120,97 @@ int dwf_solve_device(dwf_ctx* ctx, int normal_op, int method, const double* b_de
                      const double* x0_device, double rtol, int maxiter, int nrhs,
                      double* x_device, dwf_solve_stats* stats);
 
+/* ---- pseudofermion force, gauge action, HMC ---------------------------------------- */
+
+/* eta = (D D^dagger)^-1 phi by CG (method, initial guess eta_guess nullable),
+ * action = Re phi^dagger eta, force[2][Nt][Nx] = dS/dtheta with xi = D^dagger eta.
+ * Replaces pseudofermion_action_force (domain_wall/include/pseudofermion.hpp:22-166)
+ * and, on a Wilson context, pseudofermion_action_force + solve_DDdag
+ * (wilson/include/gauge.hpp:68-122, wilson/include/cg_solver.hpp:53-78).  eta and stats
+ * may be NULL.  A DWF solve that misses rtol returns DWF_ERR_NOT_CONVERGED with the
+ * message "Pseudofermion CG failed to converge"; a Wilson one fails, as the reference
+ * does, only when its true residual exceeds 20 x rtol. */
+int dwf_pf_action_force(dwf_ctx* ctx, int method, const double* phi, const double* eta_guess,
+                        double rtol, int maxiter, double* action, double* force, double* eta,
+                        dwf_solve_stats* stats);
+
+/* The force pass alone for a given eta (pseudofermion.hpp:109-157). */
+int dwf_force_from_eta(dwf_ctx* ctx, const double* eta, double* force);
+
+/* Wilson plaquette gauge action beta sum(1 - cos P) and its force dS/dtheta
+ * (domain_wall/include/gauge.hpp:26-62, wilson/include/gauge.hpp:38-62).  force may be
+ * NULL. */
+int dwf_gauge_action_force(dwf_ctx* ctx, double beta, double* action, double* force);
+
+/* average_plaquette (domain_wall/include/gauge.hpp:64-73). */
+int dwf_average_plaquette(dwf_ctx* ctx, double* plaquette);
+
+/* Download the context's current gauge angles theta[2][Nt][Nx]. */
+int dwf_get_gauge(dwf_ctx* ctx, double* theta);
+
+enum dwf_integrator {
+    /* domain_wall/generate_configs.cpp:52-169: no angle wrap, chronological eta guess */
+    DWF_HMC_DOMAIN_WALL = 0,
+    /* wilson/include/gauge.hpp:132-180: wrap_angle after each drift, zero initial guess */
+    DWF_HMC_WILSON = 1
+};
+
+typedef struct dwf_hmc_params {
+    double beta;      /* gauge coupling                                    */
+    double epsilon;   /* leapfrog step                                     */
+    int steps;        /* leapfrog steps                                    */
+    int integrator;   /* enum dwf_integrator                               */
+    int method;       /* force solver: DWF_SOLVE_CG64 or DWF_SOLVE_MIXED   */
+    int maxiter;      /* force CG iteration limit                          */
+    double rtol;      /* force CG tolerance                                */
+} dwf_hmc_params;
+
+typedef struct dwf_hmc_result {
+    int accepted;
+    int solves;              /* pseudofermion solves in the trajectory            */
+    int cg_iterations;       /* their CG iterations (FP32 ones for the mixed solver) */
+    int pad;
+    double acceptance_probability;
+    double delta_h;
+    double h_initial;
+    double h_final;
+    double plaquette;        /* average plaquette of the configuration after the accept step */
+} dwf_hmc_result;
+
+/* One HMC trajectory from the context's current gauge field, entirely on the device in
+ * FP64 (angles, momenta, forces, actions, Delta H).  The caller draws the random
+ * numbers on the host in the reference's order and passes them in: chi (the
+ * pseudofermion heat-bath vector, phi = D chi), the momenta, and u, the uniform draw
+ * of the accept step (accepted = u < min(1, exp(-dH))).  On acceptance the context's
+ * gauge field becomes the new configuration, otherwise it is left unchanged.
+ * Replaces the trajectory body of domain_wall/generate_configs.cpp:52-169 and
+ * hmc_step (wilson/include/gauge.hpp:132-180). */
+int dwf_hmc_trajectory(dwf_ctx* ctx, const dwf_hmc_params* params, const double* chi,
+                       const double* momentum, double u, dwf_hmc_result* result);
+
+/* ---- host random numbers of the callers (reference draw order) --------------------- */
+
+/* A std::mt19937_64 engine with the reference's normal(0,1) and uniform[0,1)
+ * distributions (libstdc++), for the HMC heat-bath and accept draws. */
+typedef struct dwf_rng dwf_rng;
+dwf_rng* dwf_rng_create(unsigned long long seed);
+void dwf_rng_destroy(dwf_rng* rng);
+
+/* chi[ndof] complex = (N + iN)/sqrt(2), then momentum[ngauge] = N
+ * (domain_wall/generate_configs.cpp:56-69). */
+void dwf_rng_dwf_heatbath(dwf_rng* rng, int ndof, int ngauge, double* chi, double* momentum);
+
+/* momentum[ngauge] = N first, then chi[ntot] complex (wilson/include/gauge.hpp:134-141). */
+void dwf_rng_wilson_heatbath(dwf_rng* rng, int ngauge, int ntot, double* momentum, double* chi);
+
+/* The accept draw uniform(rng) (generate_configs.cpp:162-163, gauge.hpp:178). */
+double dwf_rng_uniform(dwf_rng* rng);
+
+/* Z4 noise {1, i, -1, -i} of estimate_pseudoscalar_loop_density
+ * (domain_wall/include/correlator.hpp:35-58): nnoise vectors of sites_spin = Nt*Nx*2
+ * complex entries from one mt19937_64(seed). */
+void dwf_z4_noise(unsigned long long seed, int nnoise, int sites_spin, double* eta);
+
 /* Message of the calling thread's last failure ("" if none). */
 const char* dwf_last_error(void);
 

@@ -60,3 +60,19 @@ def test_bad_arguments_rejected():
     assert b"positive" in lib.dwf_last_error()
     assert lib.dwf_apply(None, 0, None, None, 1) == capi.DWF_ERR_ARG
     assert lib.dwf_solve(None, 3, 0, None, None, 1e-8, 10, 1, None, None) == capi.DWF_ERR_ARG
+
+
+def test_host_rng_matches_reference_order():
+    """The product's host draws equal the oracle's (same libstdc++ engine and order)."""
+    import numpy as np
+
+    import oracle
+
+    ndof, ng = 2 * 4 * 6 * 4, 2 * 4 * 6
+    chi_o, mom_o, u_o = oracle.dwf_heatbath_draws(ndof, ng, 1234)
+    r = capi.HostRng(1234)
+    chi, mom = r.dwf_heatbath(ndof, ng)
+    u = r.uniform()
+    assert np.array_equal(chi, chi_o) and np.array_equal(mom, mom_o) and u == u_o
+    z = capi.z4_noise(918273, 3, 4, 6)
+    assert np.array_equal(z, oracle.z4_noise(4, 6, 3, 918273))

@@ -1,0 +1,129 @@
+"""GPU parity of the pseudofermion force, gauge action/force, plaquette and full HMC
+trajectories (through the C ABI) against the oracle.  Bars (north star): force <= 1e-10
+relative for a given eta, Delta H <= 1e-8 absolute on same-seed trajectories."""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import print_case
+from dense_ref import rand_spinor, rand_theta
+
+import three_part_dwf_study_amd as dwf
+from three_part_dwf_study_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+M5, MF = 1.0, 0.05
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("Nt,Nx,Ls", [(16, 16, 8), (6, 10, 4), (96, 96, 16)])
+def test_dwf_force_from_eta(Nt, Nx, Ls):
+    print_case(test="dwf_force_from_eta", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(Nt + Ls)
+    th = rand_theta(rng, Nt, Nx, 0.7)
+    eta = rand_spinor(rng, Ls * Nt * Nx * 2)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        got = ctx.force_from_eta(eta)
+    want = oracle.dwf_force_from_eta(th, eta, Nt, Nx, Ls, M5, MF)
+    assert rel(got, want) <= 1e-10, rel(got, want)
+
+
+@pytest.mark.parametrize("bct,bcx", [(-1.0, 1.0), (1.0, -1.0)])
+def test_wilson_force_from_eta(bct, bcx):
+    Nt, Nx, m0 = 8, 12, 0.15
+    print_case(test="wilson_force_from_eta", bct=bct, bcx=bcx)
+    rng = np.random.default_rng(3)
+    th = rng.uniform(-np.pi, np.pi, 2 * Nt * Nx)
+    eta = rand_spinor(rng, Nt * Nx * 2)
+    with dwf.DwfContext.wilson(Nt, Nx, m0, bct, bcx) as ctx:
+        ctx.set_gauge(th)
+        got = ctx.force_from_eta(eta)
+    want = oracle.wilson_force_from_eta(th, eta, Nt, Nx, m0, bct, bcx)
+    assert rel(got, want) <= 1e-10
+
+
+def test_pf_action_force_and_gauge():
+    Nt, Nx, Ls, beta = 16, 12, 8, 3.0
+    print_case(test="pf_action_force")
+    rng = np.random.default_rng(21)
+    th = rand_theta(rng, Nt, Nx)
+    phi = rand_spinor(rng, Ls * Nt * Nx * 2)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        S, F, eta, st, rc = ctx.pf_action_force(phi, 1e-12, 5000)
+        So, Fo, etao, *_ = oracle.dwf_pf_action_force(th, phi, Nt, Nx, Ls, M5, MF, 1e-12, 5000)
+        assert abs(S - So) <= 1e-10 * abs(So)
+        assert rel(F, Fo) <= 1e-9
+        Sm, Fm, *_ = ctx.pf_action_force(phi, 1e-12, 5000, method=dwf.DWF_SOLVE_MIXED)
+        assert abs(Sm - So) <= 1e-10 * abs(So) and rel(Fm, Fo) <= 1e-9
+        Sg, Fg = ctx.gauge_action_force(beta)
+        Sgo, Fgo = oracle.gauge_action_force(th, Nt, Nx, beta)
+        assert abs(Sg - Sgo) <= 1e-12 * abs(Sgo) and rel(Fg, Fgo) <= 1e-12
+        assert abs(ctx.average_plaquette() - oracle.average_plaquette(th, Nt, Nx)) < 1e-14
+        # non-convergence surfaces the reference's message
+        _, _, _, st, rc = ctx.pf_action_force(phi, 1e-14, 2, raise_on_fail=False)
+        assert rc == capi.DWF_ERR_NOT_CONVERGED
+        assert b"Pseudofermion CG failed to converge" in ctx._lib.dwf_last_error()
+
+
+def _gpu_dwf_hmc(Nt, Nx, Ls, beta, eps, steps, ntraj, seed, rtol, method=dwf.DWF_SOLVE_CG64):
+    r = capi.HostRng(seed)
+    out = {"delta_h": [], "plaquette": [], "accepted": []}
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(np.ones(2 * Nt * Nx))  # cold start theta = 1 (generate_configs.cpp:27-28)
+        for _ in range(ntraj):
+            chi, mom = r.dwf_heatbath(Ls * Nt * Nx * 2, 2 * Nt * Nx)
+            res = ctx.hmc_trajectory(chi, mom, r.uniform(), beta, eps, steps, rtol, 10000, method=method)
+            for k in out:
+                out[k].append(res[k])
+        out["theta"] = ctx.get_gauge()
+    return {k: np.array(v) for k, v in out.items()}
+
+
+@pytest.mark.parametrize("Nt,Nx,Ls", [(8, 8, 4), (16, 12, 8)])
+def test_dwf_hmc_same_seed_history(Nt, Nx, Ls):
+    beta, eps, steps, ntraj, seed, rtol = 3.0, 0.05, 8, 4, 1234, 1e-12
+    print_case(test="dwf_hmc_history", Nt=Nt, Nx=Nx, Ls=Ls)
+    g = _gpu_dwf_hmc(Nt, Nx, Ls, beta, eps, steps, ntraj, seed, rtol)
+    o = oracle.dwf_hmc(np.ones(2 * Nt * Nx), Nt, Nx, Ls, M5, MF, beta, eps, steps, ntraj, seed, rtol, 10000)
+    assert np.max(np.abs(g["delta_h"] - o["delta_h"])) <= 1e-8
+    assert np.array_equal(g["accepted"], o["accepted"])
+    assert np.max(np.abs(g["plaquette"] - o["plaquette"])) <= 1e-10
+    assert np.max(np.abs(g["theta"] - o["theta"])) <= 1e-9
+
+
+def test_dwf_hmc_mixed_solver_matches_fp64():
+    Nt, Nx, Ls = 16, 12, 8
+    print_case(test="dwf_hmc_mixed")
+    a = _gpu_dwf_hmc(Nt, Nx, Ls, 3.0, 0.05, 8, 3, 99, 1e-11)
+    b = _gpu_dwf_hmc(Nt, Nx, Ls, 3.0, 0.05, 8, 3, 99, 1e-11, method=dwf.DWF_SOLVE_MIXED)
+    assert np.max(np.abs(a["delta_h"] - b["delta_h"])) <= 1e-8
+    assert np.array_equal(a["accepted"], b["accepted"])
+
+
+def test_wilson_hmc_same_seed_history():
+    Nt, Nx, m0, beta, tau, steps, ntraj, seed, tol = 16, 16, 0.1, 2.0, 1.0, 10, 5, 1234, 1e-12
+    print_case(test="wilson_hmc_history")
+    o = oracle.wilson_hmc(np.zeros(2 * Nt * Nx), Nt, Nx, m0, -1.0, 1.0, beta, tau, steps, ntraj, seed, tol, 5000)
+    r = capi.HostRng(seed)
+    dh, acc, plaq = [], [], []
+    with dwf.DwfContext.wilson(Nt, Nx, m0, -1.0, 1.0) as ctx:
+        ctx.set_gauge(np.zeros(2 * Nt * Nx))
+        for _ in range(ntraj):
+            mom, chi = r.wilson_heatbath(2 * Nt * Nx, Nt * Nx * 2)
+            res = ctx.hmc_trajectory(chi, mom, r.uniform(), beta, tau / steps, steps, tol, 5000,
+                                     integrator=capi.DWF_HMC_WILSON)
+            dh.append(res["delta_h"])
+            acc.append(res["accepted"])
+            plaq.append(res["plaquette"])
+        th = ctx.get_gauge()
+    assert np.max(np.abs(np.array(dh) - o["delta_h"])) <= 1e-8
+    assert np.array_equal(np.array(acc), o["accepted"])
+    assert np.max(np.abs(np.array(plaq) - o["plaquette"])) <= 1e-10
+    assert np.max(np.abs(th - o["theta"])) <= 1e-9

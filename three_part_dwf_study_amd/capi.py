@@ -60,6 +60,39 @@ class SolveStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+DWF_HMC_DOMAIN_WALL = 0
+DWF_HMC_WILSON = 1
+
+
+class HmcParams(ctypes.Structure):
+    _fields_ = [
+        ("beta", ctypes.c_double),
+        ("epsilon", ctypes.c_double),
+        ("steps", ctypes.c_int),
+        ("integrator", ctypes.c_int),
+        ("method", ctypes.c_int),
+        ("maxiter", ctypes.c_int),
+        ("rtol", ctypes.c_double),
+    ]
+
+
+class HmcResult(ctypes.Structure):
+    _fields_ = [
+        ("accepted", ctypes.c_int),
+        ("solves", ctypes.c_int),
+        ("cg_iterations", ctypes.c_int),
+        ("pad", ctypes.c_int),
+        ("acceptance_probability", ctypes.c_double),
+        ("delta_h", ctypes.c_double),
+        ("h_initial", ctypes.c_double),
+        ("h_final", ctypes.c_double),
+        ("plaquette", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 def library_path() -> str:
     return os.path.join(_HERE, _LIB_NAME)
 
@@ -89,6 +122,18 @@ PROTOTYPES = {
     "dwf_apply_device": (_I, [_P, _I, _I, _P, _P, _I]),
     "dwf_solve": (_I, [_P, _I, _I, _P, _P, _D, _I, _I, _P, _P]),
     "dwf_solve_device": (_I, [_P, _I, _I, _P, _P, _D, _I, _I, _P, _P]),
+    "dwf_pf_action_force": (_I, [_P, _I, _P, _P, _D, _I, _P, _P, _P, _P]),
+    "dwf_force_from_eta": (_I, [_P, _P, _P]),
+    "dwf_gauge_action_force": (_I, [_P, _D, _P, _P]),
+    "dwf_average_plaquette": (_I, [_P, _P]),
+    "dwf_get_gauge": (_I, [_P, _P]),
+    "dwf_hmc_trajectory": (_I, [_P, _P, _P, _P, _D, _P]),
+    "dwf_rng_create": (_P, [ctypes.c_ulonglong]),
+    "dwf_rng_destroy": (None, [_P]),
+    "dwf_rng_dwf_heatbath": (None, [_P, _I, _I, _P, _P]),
+    "dwf_rng_wilson_heatbath": (None, [_P, _I, _I, _P, _P]),
+    "dwf_rng_uniform": (_D, [_P]),
+    "dwf_z4_noise": (None, [ctypes.c_ulonglong, _I, _I, _P]),
     "dwf_last_error": (ctypes.c_char_p, []),
 }
 
@@ -242,3 +287,94 @@ class DwfContext:
         if raise_on_fail:
             _check(self._lib, rc, "dwf_solve_device")
         return [s.as_dict() for s in stats], rc
+
+    # force, gauge, HMC -----------------------------------------------------------------
+    def pf_action_force(self, phi: np.ndarray, rtol: float, maxiter: int, eta_guess=None,
+                        method: int = DWF_SOLVE_CG64, raise_on_fail: bool = True):
+        """(action, force[2*V4], eta, stats, rc) as pseudofermion_action_force."""
+        ph, n = self._as_batch(phi)
+        if n != 1:
+            raise ValueError("one pseudofermion field")
+        g = None
+        if eta_guess is not None:
+            g, _ = self._as_batch(eta_guess)
+        act = ctypes.c_double()
+        force = np.zeros(2 * self.geom.V4)
+        eta = np.empty_like(ph)
+        st = SolveStats()
+        rc = self._lib.dwf_pf_action_force(self._h, method, _ptr(ph), None if g is None else _ptr(g), rtol,
+                                           maxiter, ctypes.byref(act), _ptr(force), _ptr(eta), ctypes.byref(st))
+        if raise_on_fail:
+            _check(self._lib, rc, "dwf_pf_action_force")
+        return act.value, force, eta, st.as_dict(), rc
+
+    def force_from_eta(self, eta: np.ndarray) -> np.ndarray:
+        e, _ = self._as_batch(eta)
+        force = np.zeros(2 * self.geom.V4)
+        _check(self._lib, self._lib.dwf_force_from_eta(self._h, _ptr(e), _ptr(force)), "dwf_force_from_eta")
+        return force
+
+    def gauge_action_force(self, beta: float):
+        act = ctypes.c_double()
+        force = np.zeros(2 * self.geom.V4)
+        _check(self._lib, self._lib.dwf_gauge_action_force(self._h, beta, ctypes.byref(act), _ptr(force)),
+               "dwf_gauge_action_force")
+        return act.value, force
+
+    def average_plaquette(self) -> float:
+        v = ctypes.c_double()
+        _check(self._lib, self._lib.dwf_average_plaquette(self._h, ctypes.byref(v)), "dwf_average_plaquette")
+        return v.value
+
+    def get_gauge(self) -> np.ndarray:
+        th = np.zeros(2 * self.geom.V4)
+        _check(self._lib, self._lib.dwf_get_gauge(self._h, _ptr(th)), "dwf_get_gauge")
+        return th
+
+    def hmc_trajectory(self, chi: np.ndarray, momentum: np.ndarray, u: float, beta: float, epsilon: float,
+                       steps: int, rtol: float, maxiter: int, integrator: int = DWF_HMC_DOMAIN_WALL,
+                       method: int = DWF_SOLVE_CG64) -> dict:
+        c, _ = self._as_batch(chi)
+        p = np.ascontiguousarray(momentum, dtype=np.float64)
+        P = HmcParams(beta, epsilon, steps, integrator, method, maxiter, rtol)
+        R = HmcResult()
+        _check(self._lib, self._lib.dwf_hmc_trajectory(self._h, ctypes.byref(P), _ptr(c), _ptr(p), u,
+                                                       ctypes.byref(R)), "dwf_hmc_trajectory")
+        return R.as_dict()
+
+
+class HostRng:
+    """The callers' host random stream (mt19937_64 + libstdc++ distributions)."""
+
+    def __init__(self, seed: int):
+        self._lib = load_library()
+        self._h = self._lib.dwf_rng_create(seed)
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.dwf_rng_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def dwf_heatbath(self, ndof: int, ngauge: int):
+        chi = np.zeros(ndof, np.complex128)
+        mom = np.zeros(ngauge)
+        self._lib.dwf_rng_dwf_heatbath(self._h, ndof, ngauge, _ptr(chi), _ptr(mom))
+        return chi, mom
+
+    def wilson_heatbath(self, ngauge: int, ntot: int):
+        chi = np.zeros(ntot, np.complex128)
+        mom = np.zeros(ngauge)
+        self._lib.dwf_rng_wilson_heatbath(self._h, ngauge, ntot, _ptr(mom), _ptr(chi))
+        return mom, chi
+
+    def uniform(self) -> float:
+        return self._lib.dwf_rng_uniform(self._h)
+
+
+def z4_noise(seed: int, nnoise: int, Nt: int, Nx: int) -> np.ndarray:
+    eta = np.zeros(nnoise * Nt * Nx * 2, np.complex128)
+    load_library().dwf_z4_noise(seed, nnoise, Nt * Nx * 2, _ptr(eta))
+    return eta.reshape(nnoise, Nt, Nx, 2)

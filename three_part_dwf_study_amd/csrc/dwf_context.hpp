@@ -35,6 +35,15 @@ enum BufId : int {
     B_AUX1,
     B_AUX2,
     B_AUX3,
+    B_PHI,     // HMC: pseudofermion field
+    B_ETA,     // HMC: (D D^dag)^-1 phi (also the chronological guess)
+    B_XI,      // HMC: D^dag eta
+    B_MOM,     // HMC: momenta
+    B_FG,      // HMC: gauge force
+    B_FPF,     // HMC: pseudofermion force
+    B_THETA0,  // HMC: configuration at the start of the trajectory
+    B_SCAL,    // FP64 scalar results
+    B_RPART,   // reduction partials
     B_COUNT
 };
 
@@ -67,5 +76,8 @@ struct dwf_ctx {
 };
 
 void dwf_apply_device_impl(dwf_ctx* c, int op, int prec, const void* in, void* out, int nrhs);
+// Scalar slots in the B_SCAL buffer.
+enum ScalarSlot : int { S_GAUGE = 0, S_PLAQ, S_PF, S_KIN, S_COUNT = 8 };
+
 int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0,
                           double rtol, int maxiter, int nrhs, double* x);

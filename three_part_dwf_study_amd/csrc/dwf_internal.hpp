@@ -98,4 +98,18 @@ void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s);
 
 HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override = 0);
 
+// HMC kernels (dwf_hmc.hip).  `scalars` is a small device array of FP64 results; each
+// reduction folds its block partials in index order into scalars[slot].
+int reduce_blocks(long long n);
+void launch_gauge(const Lat& L, const double* th, double beta, double* force, double* part_s,
+                  double* part_c, double* scalars, int slot_s, int slot_c, hipStream_t s);
+void launch_pf_force(const Lat& L, const double2* l64, const double* eta, const double* xi, double* force,
+                     hipStream_t s);
+void launch_redot(const double* a, const double* b, long long ncomplex, double* part, double* scalars, int slot,
+                  hipStream_t s);
+void launch_mom(double* p, const double* fa, const double* fb, int n, double c, double* part_k, double* scalars,
+                int slot, hipStream_t s);
+void launch_theta(double* th, const double* p, int n, double eps, int wrap, double2* l64, float2* l32,
+                  hipStream_t s);
+
 }  // namespace dwfk
