@@ -188,6 +188,39 @@ typedef struct dwf_hmc_result {
 int dwf_hmc_trajectory(dwf_ctx* ctx, const dwf_hmc_params* params, const double* chi,
                        const double* momentum, double u, dwf_hmc_result* result);
 
+/* ---- analyzer measurements ---------------------------------------------------------- */
+
+/* Which of the DWF analyzer's per-configuration measurements to run.  All requested
+ * sources go through one batched CG on D^dagger D (rhs D^dagger src). */
+typedef struct dwf_measure_spec {
+    int method;                    /* DWF_SOLVE_CG64 (reference) or DWF_SOLVE_MIXED       */
+    int maxiter;                   /* dwf.propagator_maxiter                             */
+    double rtol;                   /* dwf.propagator_rtol                                */
+    int wall;                      /* 1: compute_CPP_CJ5, 4 wall-source solves          */
+    int point;                     /* 1: compute_momentum_CPP, 4 point-source solves     */
+    int max_momentum;              /* analysis.max_momentum (0..Nx/2)                    */
+    int noise_vectors;             /* eta.noise_vectors (0 skips the loop density)       */
+    unsigned long long noise_seed; /* eta.random_seed + 104729 * config                  */
+} dwf_measure_spec;
+
+typedef struct dwf_measure_out {
+    double* CPP;             /* [Nt]  C_PP(dt)   (correlator.hpp:243-310), if wall       */
+    double* CJ5;             /* [Nt]  C_J5(dt)                                          */
+    double* momentum_CPP;    /* [max_momentum+1][Nt] (correlator.hpp:111-181), if point   */
+    double* loop;            /* [Nt][Nx] complex L(t,x) (correlator.hpp:27-109), if noise */
+    int* iterations;         /* nullable: [4 wall][4 point][noise] CG iterations           */
+    double* dirac_residual;  /* nullable: [4] ||D x - src|| / ||src|| of the wall solves  */
+    int wall_nonconverged;   /* wall solves that missed rtol ("CGNR info = nonzero")      */
+    int pad;
+} dwf_measure_out;
+
+/* One configuration's measurements (the loop body of domain_wall/analyze_configs.cpp:57-75:
+ * compute_CPP_CJ5, estimate_pseudoscalar_loop_density, compute_momentum_CPP).  Errors
+ * carry the reference's messages: "Momentum propagator CGNR failed to converge",
+ * "Eta loop CGNR failed to converge", "analysis.max_momentum is outside 0..Nx/2".  Wall
+ * solves that miss rtol are counted, not fatal, as in the reference. */
+int dwf_measure(dwf_ctx* ctx, const dwf_measure_spec* spec, dwf_measure_out* out);
+
 /* ---- host random numbers of the callers (reference draw order) --------------------- */
 
 /* A std::mt19937_64 engine with the reference's normal(0,1) and uniform[0,1)

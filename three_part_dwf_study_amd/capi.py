@@ -93,6 +93,32 @@ class HmcResult(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
+class MeasureSpec(ctypes.Structure):
+    _fields_ = [
+        ("method", ctypes.c_int),
+        ("maxiter", ctypes.c_int),
+        ("rtol", ctypes.c_double),
+        ("wall", ctypes.c_int),
+        ("point", ctypes.c_int),
+        ("max_momentum", ctypes.c_int),
+        ("noise_vectors", ctypes.c_int),
+        ("noise_seed", ctypes.c_ulonglong),
+    ]
+
+
+class MeasureOut(ctypes.Structure):
+    _fields_ = [
+        ("CPP", ctypes.c_void_p),
+        ("CJ5", ctypes.c_void_p),
+        ("momentum_CPP", ctypes.c_void_p),
+        ("loop", ctypes.c_void_p),
+        ("iterations", ctypes.c_void_p),
+        ("dirac_residual", ctypes.c_void_p),
+        ("wall_nonconverged", ctypes.c_int),
+        ("pad", ctypes.c_int),
+    ]
+
+
 def library_path() -> str:
     return os.path.join(_HERE, _LIB_NAME)
 
@@ -128,6 +154,7 @@ PROTOTYPES = {
     "dwf_average_plaquette": (_I, [_P, _P]),
     "dwf_get_gauge": (_I, [_P, _P]),
     "dwf_hmc_trajectory": (_I, [_P, _P, _P, _P, _D, _P]),
+    "dwf_measure": (_I, [_P, _P, _P]),
     "dwf_rng_create": (_P, [ctypes.c_ulonglong]),
     "dwf_rng_destroy": (None, [_P]),
     "dwf_rng_dwf_heatbath": (None, [_P, _I, _I, _P, _P]),
@@ -341,6 +368,22 @@ class DwfContext:
         _check(self._lib, self._lib.dwf_hmc_trajectory(self._h, ctypes.byref(P), _ptr(c), _ptr(p), u,
                                                        ctypes.byref(R)), "dwf_hmc_trajectory")
         return R.as_dict()
+
+    # analyzer ----------------------------------------------------------------------------
+    def measure(self, rtol: float, maxiter: int, wall: bool = True, point: bool = True, max_momentum: int = 0,
+                noise_vectors: int = 0, noise_seed: int = 0, method: int = DWF_SOLVE_CG64) -> dict:
+        Nt, Nx = self.geom.Nt, self.geom.Nx
+        nrhs = (4 if wall else 0) + (4 if point else 0) + noise_vectors
+        cpp, cj5 = np.zeros(Nt), np.zeros(Nt)
+        mom = np.zeros((max_momentum + 1) * Nt)
+        loop = np.zeros(Nt * Nx, np.complex128)
+        iters = np.zeros(max(nrhs, 1), np.int32)
+        dres = np.zeros(4)
+        spec = MeasureSpec(method, maxiter, rtol, int(wall), int(point), max_momentum, noise_vectors, noise_seed)
+        out = MeasureOut(_ptr(cpp), _ptr(cj5), _ptr(mom), _ptr(loop), _ptr(iters), _ptr(dres), 0, 0)
+        _check(self._lib, self._lib.dwf_measure(self._h, ctypes.byref(spec), ctypes.byref(out)), "dwf_measure")
+        return dict(CPP=cpp, CJ5=cj5, momentum_CPP=mom.reshape(max_momentum + 1, Nt), loop=loop.reshape(Nt, Nx),
+                    iterations=iters[:nrhs], dirac_residual=dres, wall_nonconverged=out.wall_nonconverged)
 
 
 class HostRng:

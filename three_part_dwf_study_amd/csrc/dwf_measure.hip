@@ -1,0 +1,299 @@
+// dwf_measure.hip -- the DWF analyzer's propagator solves and contractions
+// (domain_wall/include/correlator.hpp:27-310) on the device.
+//
+// One configuration's sources -- 4 boundary wall sources (compute_CPP_CJ5), 4 boundary
+// point sources (compute_momentum_CPP) and N Z4 noise vectors
+// (estimate_pseudoscalar_loop_density) -- go through ONE batched multi-RHS CG on
+// D^dag D with right-hand sides D^dag src.  The contractions run in FP64 on the device;
+// only Nt-length correlators and the V4 loop density leave it.
+//
+// Quark field of a 5D solution X: q(x) = P_L X(s=0, x) + P_R X(s=Ls-1, x), i.e.
+// q0 = X(Ls-1, x).spin0 and q1 = X(0, x).spin1 (P_R = diag(1,0), P_L = diag(0,1)).
+
+#include "../../include/dwf_hip.h"
+#include "dwf_context.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace dwfk;
+
+namespace {
+
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over the block of v (result in every thread).  Every thread must call it.
+__device__ __forceinline__ double bsum(double v, double* sh)
+{
+    v = wsum(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+    __syncthreads();
+    return r;
+}
+
+// Sources.  kind 0: wall at t_src (all x), 1: point at (t_src, x=0); the source spin ss
+// fills P_L e_ss at s=0 and P_R e_ss at s=Ls-1 (correlator.hpp:131-138, 199-211).
+// The 4 wall / 4 point right-hand sides are ordered (t_src=0, ss=0), (0, 1),
+// (Nt/2, 0), (Nt/2, 1).  Noise: src = P_L eta at s=0, + P_R eta at s=Ls-1 (:60-75).
+__global__ void k_sources(const int Nt, const int Nx, const int Ls, const int nwall, const int npoint,
+                          const int nnoise, const double* noise, double* src)
+{
+    const int V4 = Nt * Nx;
+    const long long V5 = (long long)V4 * Ls;
+    const int rhs = blockIdx.y;
+    double4* out = reinterpret_cast<double4*>(src) + rhs * V5;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V4; i += gridDim.x * blockDim.x) {
+        const int t = i / Nx, x = i - t * Nx;
+        double4 lo = make_double4(0, 0, 0, 0), hi = make_double4(0, 0, 0, 0);  // s = 0, s = Ls-1
+        if (rhs < nwall + npoint) {
+            const int k = rhs < nwall ? rhs : rhs - nwall;
+            const int t_src = (k >> 1) ? Nt / 2 : 0;
+            const int ss = k & 1;
+            const bool on = (t == t_src) && (rhs < nwall || x == 0);
+            if (on) {
+                if (ss == 1) lo.z = 1.0;  // P_L e_1 = (0, 1) at s = 0
+                else hi.x = 1.0;          // P_R e_0 = (1, 0) at s = Ls-1
+            }
+        } else {
+            const int n = rhs - nwall - npoint;
+            const double* e = noise + ((long long)n * V4 + i) * 4;  // eta(t,x,spin) complex
+            lo.z = e[2];
+            lo.w = e[3];
+            hi.x = e[0];
+            hi.y = e[1];
+        }
+        if (Ls == 1) {
+            out[i] = make_double4(hi.x, hi.y, lo.z, lo.w);
+        } else {
+            out[i] = lo;
+            out[(long long)(Ls - 1) * V4 + i] = hi;
+        }
+        for (int s = 1; s < Ls - 1; ++s) out[(long long)s * V4 + i] = make_double4(0, 0, 0, 0);
+    }
+}
+
+// C_PP / C_J5 of the wall solutions (correlator.hpp:258-300): block per (t, source time),
+// outputs per source time k: pp[k][t], j5[k][t] summed over x and source spin.
+__global__ void k_contract_wall(const int Nt, const int Nx, const int Ls, const double* X, double* pp,
+                                double* j5)
+{
+    __shared__ double sh[8];
+    const int t = blockIdx.x, k = blockIdx.y;
+    const int V4 = Nt * Nx;
+    const long long V5 = (long long)V4 * Ls;
+    const double4* P = reinterpret_cast<const double4*>(X);
+    double a = 0.0, b = 0.0;
+    for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+        const int i = t * Nx + x;
+        for (int ss = 0; ss < 2; ++ss) {
+            const double4* Y = P + (2 * k + ss) * V5;
+            const double4 l = Y[i], r = Y[(long long)(Ls - 1) * V4 + i], m = Y[(long long)(Ls / 2) * V4 + i];
+            const double q0r = r.x, q0i = r.y, q1r = l.z, q1i = l.w;
+            a += q0r * q0r + q0i * q0i + q1r * q1r + q1i * q1i;
+            b += m.x * q0r + m.y * q0i + m.z * q1r + m.w * q1i;  // Re(mid^* q), mid unprojected
+        }
+    }
+    a = bsum(a, sh);
+    b = bsum(b, sh);
+    if (threadIdx.x == 0) {
+        pp[k * Nt + t] = a;
+        j5[k * Nt + t] = b;
+    }
+}
+
+// Momentum-projected point correlator (correlator.hpp:149-174): block per (t, source
+// time); out[k][n][t] = sum_x cos(2 pi n x / Nx) sum_ss |q|^2.
+__global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const int nmom, const double* X,
+                                 double* out)
+{
+    __shared__ double sh[8];
+    const int t = blockIdx.x, k = blockIdx.y;
+    const int V4 = Nt * Nx;
+    const long long V5 = (long long)V4 * Ls;
+    const double4* P = reinterpret_cast<const double4*>(X);
+    const double pi = 3.141592653589793238462643383279502884;
+    for (int n = 0; n < nmom; ++n) {
+        const double p = 2.0 * pi * n / Nx;
+        double v = 0.0;
+        for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+            const int i = t * Nx + x;
+            double local = 0.0;
+            for (int ss = 0; ss < 2; ++ss) {
+                const double4* Y = P + (2 * k + ss) * V5;
+                const double4 l = Y[i], r = Y[(long long)(Ls - 1) * V4 + i];
+                local += r.x * r.x + r.y * r.y + l.z * l.z + l.w * l.w;
+            }
+            v += cos(p * x) * local;
+        }
+        v = bsum(v, sh);
+        if (threadIdx.x == 0) out[((long long)k * nmom + n) * Nt + t] = v;
+    }
+}
+
+// Loop density (correlator.hpp:83-107): L(x) = (1/N) sum_noise eta(x)^dag sigma3 q(x).
+__global__ void k_contract_loop(const int Nt, const int Nx, const int Ls, const int nnoise, const double* noise,
+                                const double* X, double* loop)
+{
+    const int V4 = Nt * Nx;
+    const long long V5 = (long long)V4 * Ls;
+    const double4* P = reinterpret_cast<const double4*>(X);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V4; i += gridDim.x * blockDim.x) {
+        double lr = 0.0, li = 0.0;
+        for (int n = 0; n < nnoise; ++n) {
+            const double* e = noise + ((long long)n * V4 + i) * 4;
+            const double4* Y = P + n * V5;
+            const double4 l = Y[i], r = Y[(long long)(Ls - 1) * V4 + i];
+            // conj(e0) q0 - conj(e1) q1
+            lr += e[0] * r.x + e[1] * r.y - (e[2] * l.z + e[3] * l.w);
+            li += e[0] * r.y - e[1] * r.x - (e[2] * l.w - e[3] * l.z);
+        }
+        loop[2 * i] = lr / nnoise;
+        loop[2 * i + 1] = li / nnoise;
+    }
+}
+
+// ||v_k||^2 per right-hand side (block per rhs, fixed order).
+__global__ void k_norms(const double* v, const long long n_per, double* out)
+{
+    __shared__ double sh[8];
+    const double* a = v + blockIdx.x * n_per;
+    double s = 0.0;
+    for (long long i = threadIdx.x; i < n_per; i += blockDim.x) s += a[i] * a[i];
+    s = bsum(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// d = a - b elementwise
+__global__ void k_sub(const double* a, const double* b, double* d, const long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        d[i] = a[i] - b[i];
+}
+
+int guard(const std::function<int()>& f)
+{
+    try {
+        return f();
+    } catch (const DwfError& e) {
+        dwf_set_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        dwf_set_error(e.what());
+        return DWF_ERR_ARG;
+    }
+}
+
+}  // namespace
+
+extern "C" int dwf_measure(dwf_ctx* c, const dwf_measure_spec* spec, dwf_measure_out* out)
+{
+    return guard([&]() -> int {
+        if (!c || !spec || !out) throw DwfError(DWF_ERR_ARG, "dwf_measure: null argument");
+        if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_measure: no gauge field set");
+        if (c->kind != 0) throw DwfError(DWF_ERR_ARG, "dwf_measure: needs a domain-wall context");
+        const Lat& L = c->L;
+        const int Nt = L.Nt, Nx = L.Nx, Ls = L.Ls, V4 = L.V4;
+        const int nwall = spec->wall ? 4 : 0;
+        const int npoint = spec->point ? 4 : 0;
+        const int nnoise = spec->noise_vectors;
+        if (nnoise < 0) throw DwfError(DWF_ERR_ARG, "eta.noise_vectors must be positive");
+        if (spec->point && (spec->max_momentum < 0 || spec->max_momentum > Nx / 2))
+            throw DwfError(DWF_ERR_ARG, "analysis.max_momentum is outside 0..Nx/2");
+        if ((spec->wall && (!out->CPP || !out->CJ5)) || (spec->point && !out->momentum_CPP) ||
+            (nnoise > 0 && !out->loop) || spec->maxiter < 0 ||
+            (spec->method != DWF_SOLVE_CG64 && spec->method != DWF_SOLVE_MIXED))
+            throw DwfError(DWF_ERR_ARG, "dwf_measure: bad argument");
+        const int nrhs = nwall + npoint + nnoise;
+        out->wall_nonconverged = 0;
+        if (nrhs == 0) return DWF_OK;
+        const long long V5 = L.V5();
+        const size_t nv = (size_t)V5 * 4;  // doubles per spinor
+        hipStream_t s = c->stream;
+
+        // Z4 noise from the host stream of correlator.hpp:35-58 (seed per configuration)
+        double* dnoise = nullptr;
+        if (nnoise > 0) {
+            std::vector<double> h((size_t)nnoise * V4 * 4);
+            dwf_z4_noise(spec->noise_seed, nnoise, V4 * 2, h.data());
+            dnoise = (double*)c->buf(B_AUX3, h.size() * 8);
+            c->check(hipMemcpyAsync(dnoise, h.data(), h.size() * 8, hipMemcpyHostToDevice, s), "noise upload");
+            c->check(hipStreamSynchronize(s), "noise upload");
+        }
+        double* src = (double*)c->buf(B_AUX0, nv * nrhs * 8);
+        double* rhs = (double*)c->buf(B_AUX1, nv * nrhs * 8);
+        double* X = (double*)c->buf(B_AUX2, nv * nrhs * 8);
+        k_sources<<<dim3((V4 + 255) / 256, nrhs), 256, 0, s>>>(Nt, Nx, Ls, nwall, npoint, nnoise, dnoise, src);
+        c->check_launch("sources");
+        dwf_apply_device_impl(c, DWF_OP_DDAG, DWF_PREC_FP64, src, rhs, nrhs);  // rhs = D^dag src
+        dwf_solve_device_impl(c, DWF_OP_DDAGD, spec->method, rhs, nullptr, spec->rtol, spec->maxiter, nrhs, X);
+        std::vector<CgState> st(c->st_host, c->st_host + nrhs);
+        for (int k = 0; k < nrhs; ++k) {
+            if (st[k].status == 4) throw DwfError(DWF_ERR_ZERO_DENOM, "CG encountered a zero denominator");
+            if (out->iterations) out->iterations[k] = st[k].iters;
+        }
+        auto converged = [&](int k) { return (st[k].flags[c->last_slot] & F_DONE) != 0; };
+        for (int k = nwall; k < nwall + npoint; ++k)
+            if (!converged(k)) throw DwfError(DWF_ERR_NOT_CONVERGED, "Momentum propagator CGNR failed to converge");
+        for (int k = nwall + npoint; k < nrhs; ++k)
+            if (!converged(k)) throw DwfError(DWF_ERR_NOT_CONVERGED, "Eta loop CGNR failed to converge");
+
+        double* res = (double*)c->buf(B_SCAL, 64 * 8);
+        if (nwall) {
+            for (int k = 0; k < nwall; ++k)
+                if (!converged(k)) out->wall_nonconverged += 1;  // "CGNR info = nonzero" (:222-223)
+            // Dirac residual ||D x - src|| / ||src|| of the wall solves (:225-227)
+            double* dx = (double*)c->buf(B_APPLY_TMP, nv * nwall * 8);
+            double* diff = (double*)c->buf(B_T64, nv * nwall * 8);
+            dwf_apply_device_impl(c, DWF_OP_D, DWF_PREC_FP64, X, dx, nwall);
+            k_sub<<<1024, 256, 0, s>>>(dx, src, diff, (long long)nv * nwall);
+            k_norms<<<nwall, 256, 0, s>>>(diff, (long long)nv, res);
+            k_norms<<<nwall, 256, 0, s>>>(src, (long long)nv, res + 8);
+            double* pp = (double*)c->buf(B_PA, 4 * Nt * 8);
+            double* j5 = (double*)c->buf(B_PB, 4 * Nt * 8);
+            k_contract_wall<<<dim3(Nt, 2), 256, 0, s>>>(Nt, Nx, Ls, X, pp, j5);
+            c->check_launch("wall contraction");
+            std::vector<double> hpp(2 * Nt), hj5(2 * Nt), hr(16);
+            c->check(hipMemcpyAsync(hpp.data(), pp, 2 * Nt * 8, hipMemcpyDeviceToHost, s), "readback");
+            c->check(hipMemcpyAsync(hj5.data(), j5, 2 * Nt * 8, hipMemcpyDeviceToHost, s), "readback");
+            c->check(hipMemcpyAsync(hr.data(), res, 16 * 8, hipMemcpyDeviceToHost, s), "readback");
+            c->check(hipStreamSynchronize(s), "readback");
+            for (int dt = 0; dt < Nt; ++dt) {
+                out->CPP[dt] = (hpp[dt] + hpp[Nt + (dt + Nt / 2) % Nt]) / 2.0;
+                out->CJ5[dt] = (hj5[dt] + hj5[Nt + (dt + Nt / 2) % Nt]) / 2.0;
+            }
+            if (out->dirac_residual)
+                for (int k = 0; k < nwall; ++k) out->dirac_residual[k] = std::sqrt(hr[k]) / std::sqrt(hr[8 + k]);
+        }
+        if (npoint) {
+            const int nmom = spec->max_momentum + 1;
+            double* pm = (double*)c->buf(B_PR, 2 * nmom * Nt * 8);
+            k_contract_point<<<dim3(Nt, 2), 256, 0, s>>>(Nt, Nx, Ls, nmom, X + nv * nwall, pm);
+            c->check_launch("point contraction");
+            std::vector<double> h(2 * nmom * Nt);
+            c->check(hipMemcpyAsync(h.data(), pm, h.size() * 8, hipMemcpyDeviceToHost, s), "readback");
+            c->check(hipStreamSynchronize(s), "readback");
+            for (int n = 0; n < nmom; ++n)
+                for (int dt = 0; dt < Nt; ++dt)
+                    out->momentum_CPP[n * Nt + dt] =
+                        (h[n * Nt + dt] + h[(nmom + n) * Nt + (dt + Nt / 2) % Nt]) / 2.0;
+        }
+        if (nnoise) {
+            double* lp = (double*)c->buf(B_PR, (size_t)V4 * 2 * 8 + 2 * 64 * Nt * 8);
+            k_contract_loop<<<(V4 + 255) / 256, 256, 0, s>>>(Nt, Nx, Ls, nnoise, dnoise, X + nv * (nwall + npoint), lp);
+            c->check_launch("loop contraction");
+            c->check(hipMemcpyAsync(out->loop, lp, (size_t)V4 * 2 * 8, hipMemcpyDeviceToHost, s), "readback");
+            c->check(hipStreamSynchronize(s), "readback");
+        }
+        return DWF_OK;
+    });
+}
