@@ -57,28 +57,35 @@ def dslash_bytes(prec_bytes, nrhs):
     return nrhs * 2 * Bv + BL
 
 
-def time_dslash(torch, ctx, dwf, prec, nrhs, reps):
-    dt = torch.float32 if prec == dwf.DWF_PREC_FP32 else torch.float64
-    n = nrhs * LS * NT * NX * 4
-    g = torch.Generator(device="cuda")
-    g.manual_seed(77)
-    src = torch.randn(n, dtype=dt, device="cuda", generator=g)
-    dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    stream = torch.cuda.ExternalStream(ctx.stream)
+def time_kernel(ctx, capi, launch, reps):
+    """Average duration (ms) of `launch` over `reps` back-to-back launches, from HIP
+    events recorded on the library's stream around the timed region."""
     for _ in range(3):
-        ctx.apply_device(dwf.DWF_OP_D, prec, src.data_ptr(), dst.data_ptr(), nrhs)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
+        launch()
+    t = capi.Timer()
+    t.start(ctx)
     for _ in range(reps):
-        ctx.apply_device(dwf.DWF_OP_D, prec, src.data_ptr(), dst.data_ptr(), nrhs)
-    e1.record(stream)
-    e1.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+        launch()
+    t.stop(ctx)
+    return t.elapsed_ms() / reps
+
+
+def time_dslash(ctx, dwf, capi, prec, nrhs, reps):
+    n = nrhs * LS * NT * NX * 4
+    dt = np.float32 if prec == dwf.DWF_PREC_FP32 else np.float64
+    host = np.random.default_rng(77).normal(size=n).astype(dt)
+    src = capi.DeviceArray.from_host(host)
+    dst = capi.DeviceArray(host.nbytes)
+    ms = time_kernel(ctx, capi, lambda: ctx.apply_device(dwf.DWF_OP_D, prec, src.ptr, dst.ptr, nrhs), reps)
     nbytes = dslash_bytes(4 if prec == dwf.DWF_PREC_FP32 else 8, nrhs)
-    del src, dst
-    torch.cuda.empty_cache()
     return ms, nbytes / (ms * 1e-3) / 1e9
+
+
+def time_copy(ctx, capi, nbytes, reps):
+    src = capi.DeviceArray(nbytes)
+    dst = capi.DeviceArray(nbytes)
+    ms = time_kernel(ctx, capi, lambda: ctx.copy_device(dst.ptr, src.ptr, nbytes), reps)
+    return ms, 2 * nbytes / (ms * 1e-3) / 1e9
 
 
 def cpu_baseline(theta, b1, judge_iters, sample_iters):
@@ -114,12 +121,8 @@ def main():
               "configurations shard without communication); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     import three_part_dwf_study_amd as dwf
-
-    if not torch.cuda.is_available():
-        raise SystemExit("bench.py needs a GPU (libdwfhip has no CPU path)")
+    from three_part_dwf_study_amd import capi
 
     rng = np.random.default_rng(1234)
     theta = rng.normal(0.0, 1.0 / np.sqrt(BETA), size=2 * NT * NX)
@@ -129,16 +132,14 @@ def main():
     ctx.set_gauge(theta)
 
     n = nrhs * LS * NT * NX * 4
-    d_src = torch.from_numpy(src.reshape(-1).view(np.float64).copy()).cuda()
-    d_b = torch.empty(n, dtype=torch.float64, device="cuda")
-    d_x = torch.empty_like(d_b)
-    torch.cuda.synchronize()
-    ctx.apply_device(dwf.DWF_OP_DDAG, dwf.DWF_PREC_FP64, d_src.data_ptr(), d_b.data_ptr(), nrhs)
-    torch.cuda.synchronize()
+    d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
+    d_b = capi.DeviceArray(n * 8)
+    d_x = capi.DeviceArray(n * 8)
+    ctx.apply_device(dwf.DWF_OP_DDAG, dwf.DWF_PREC_FP64, d_src.ptr, d_b.ptr, nrhs)
+    ctx.synchronize()
 
     def solve(method):
-        st, rc = ctx.solve_device(dwf.DWF_OP_DDAGD, method, d_b.data_ptr(), None, TOL, 20000, nrhs,
-                                  d_x.data_ptr())
+        st, rc = ctx.solve_device(dwf.DWF_OP_DDAGD, method, d_b.ptr, None, TOL, 20000, nrhs, d_x.ptr)
         return st
 
     results = {}
@@ -146,11 +147,11 @@ def main():
                                 ("fp64", dwf.DWF_SOLVE_CG64, max(1, args.steps))):
         for _ in range(args.warmup):
             solve(method)
-        torch.cuda.synchronize()
+        ctx.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
             st = solve(method)
-        torch.cuda.synchronize()
+        ctx.synchronize()
         dt = time.perf_counter() - t0
         tr = max(s["true_relres"] for s in st)
         if tr > TOL or not all(s["converged"] for s in st):
@@ -163,15 +164,17 @@ def main():
                              max_true_relres=tr)
         log(f"{name}: {results[name]}")
 
-    ms32, gbs32 = time_dslash(torch, ctx, dwf, dwf.DWF_PREC_FP32, args.dslash_rhs, 20)
-    ms64, gbs64 = time_dslash(torch, ctx, dwf, dwf.DWF_PREC_FP64, args.dslash_rhs // 2, 20)
-    ms1, gbs1 = time_dslash(torch, ctx, dwf, dwf.DWF_PREC_FP32, 1, 200)
+    ms32, gbs32 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, args.dslash_rhs, 20)
+    ms64, gbs64 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP64, args.dslash_rhs // 2, 20)
+    ms1, gbs1 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, 1, 200)
+    msc, gbsc = time_copy(ctx, capi, dslash_bytes(4, args.dslash_rhs) // 2 // 16 * 16, 20)
+    log(f"copy kernel ({dslash_bytes(4, args.dslash_rhs) // 2 >> 20} MiB each way): {msc:.4f} ms {gbsc:.0f} GB/s")
     log(f"dslash fp32 x{args.dslash_rhs}: {ms32:.4f} ms {gbs32:.0f} GB/s; fp64 x{args.dslash_rhs // 2}: "
         f"{ms64:.4f} ms {gbs64:.0f} GB/s; fp32 x1 (cache-resident): {ms1 * 1e3:.1f} us {gbs1:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
-        b1 = d_b[: LS * NT * NX * 4].cpu().numpy().view(np.complex128)
+        b1 = d_b.download(np.float64, LS * NT * NX * 4).view(np.complex128)
         cpu = cpu_baseline(theta, b1, results["fp64"]["iterations"], args.cpu_iters)
         log(f"cpu baseline: {cpu}")
 
@@ -201,7 +204,8 @@ def main():
         "max_true_relres": {"mixed": mixed["max_true_relres"], "fp64_cg": judge["max_true_relres"]},
         "dslash_gbs": {"fp32_batch%d" % args.dslash_rhs: round(gbs32, 1),
                        "fp64_batch%d" % (args.dslash_rhs // 2): round(gbs64, 1),
-                       "fp32_single_rhs_cache_resident": round(gbs1, 1)},
+                       "fp32_single_rhs_cache_resident": round(gbs1, 1),
+                       "copy_kernel_achievable": round(gbsc, 1)},
         "roofline": {
             "bound": "hbm",
             "achieved": round(gbs32, 1),

@@ -86,6 +86,14 @@ int dwf_ctx_destroy(dwf_ctx* ctx);
  * against the *_device entry points. */
 void* dwf_ctx_stream(dwf_ctx* ctx);
 
+/* Launch-shape and solver knobs (no counterpart in the reference; set by the
+ * autotuner or by hand).  DWF_TUNE_SCHUNKS: s-chunks per 4D site of the stencils
+ * (0 = heuristic, 1..Ls); DWF_TUNE_XCD_REMAP: 0/1 XCD-aware block order;
+ * DWF_TUNE_RELIABLE_DELTA: residual drop (0 < delta < 1) that triggers a reliable update
+ * of the mixed solver. */
+enum dwf_tune_param { DWF_TUNE_SCHUNKS = 0, DWF_TUNE_XCD_REMAP = 1, DWF_TUNE_RELIABLE_DELTA = 2 };
+int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
+
 /* Upload gauge angles theta[2][Nt][Nx] (host).  The library keeps them on the device in
  * FP64 and as cos/sin link arrays in FP64 and FP32; U_mu(x) = exp(i theta_mu(x))
  * (dwf_operator.hpp:10-13). */
@@ -243,6 +251,28 @@ double dwf_rng_uniform(dwf_rng* rng);
  * (domain_wall/include/correlator.hpp:35-58): nnoise vectors of sites_spin = Nt*Nx*2
  * complex entries from one mt19937_64(seed). */
 void dwf_z4_noise(unsigned long long seed, int nnoise, int sites_spin, double* eta);
+
+/* ---- device memory and timing helpers -----------------------------------------------
+ * No counterpart in the reference (it has no device).  Plumbing for callers that keep
+ * their fields in device memory and time the *_device entry points, so they need no
+ * second GPU runtime in their process. */
+void* dwf_device_alloc(unsigned long long bytes);              /* NULL on failure */
+int dwf_device_free(void* ptr);
+int dwf_memcpy_htod(void* dst_device, const void* src_host, unsigned long long bytes);
+int dwf_memcpy_dtoh(void* dst_host, const void* src_device, unsigned long long bytes);
+int dwf_ctx_synchronize(dwf_ctx* ctx);
+
+/* HIP events recorded on a context's stream. */
+typedef struct dwf_timer dwf_timer;
+dwf_timer* dwf_timer_create(void);
+void dwf_timer_destroy(dwf_timer* timer);
+int dwf_timer_start(dwf_timer* timer, dwf_ctx* ctx);
+int dwf_timer_stop(dwf_timer* timer, dwf_ctx* ctx);
+double dwf_timer_elapsed_ms(dwf_timer* timer);  /* waits for the stop event; < 0 on error */
+
+/* dst = src (bytes a multiple of 16) with a plain vector copy kernel on the context's
+ * stream: the achievable-bandwidth reference for the stencils' roofline. */
+int dwf_copy_device(dwf_ctx* ctx, void* dst, const void* src, unsigned long long bytes);
 
 /* Message of the calling thread's last failure ("" if none). */
 const char* dwf_last_error(void);

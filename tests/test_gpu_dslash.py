@@ -11,6 +11,7 @@ from conftest import print_case
 from dense_ref import rand_spinor, rand_theta
 
 import three_part_dwf_study_amd as dwf
+from three_part_dwf_study_amd import capi
 
 pytestmark = pytest.mark.gpu
 
@@ -65,19 +66,34 @@ def test_wilson_ops_fp64(Nt, Nx, bct, bcx):
 
 @pytest.mark.parametrize("Nt,Nx,Ls", [(16, 16, 8), (96, 96, 16)])
 def test_dwf_ops_fp32_device(Nt, Nx, Ls):
-    torch = pytest.importorskip("torch")
     print_case(test="dwf_ops_fp32", Nt=Nt, Nx=Nx, Ls=Ls)
     rng = np.random.default_rng(3)
     th = rand_theta(rng, Nt, Nx)
     psi = rand_spinor(rng, Ls * Nt * Nx * 2)
     with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
         ctx.set_gauge(th)
-        for prec, dt, tol in ((dwf.DWF_PREC_FP32, torch.float32, 2e-6), (dwf.DWF_PREC_FP64, torch.float64, 1e-12)):
-            src = torch.from_numpy(psi.view(np.float64).astype(np.float64)).to(dt).cuda()
-            dst = torch.empty_like(src)
+        for prec, dt, tol in ((dwf.DWF_PREC_FP32, np.float32, 2e-6), (dwf.DWF_PREC_FP64, np.float64, 1e-12)):
+            host = psi.view(np.float64).astype(dt)
+            src = capi.DeviceArray.from_host(host)
+            dst = capi.DeviceArray(host.nbytes)
             for op in (0, 1, 3):
-                ctx.apply_device(op, prec, src.data_ptr(), dst.data_ptr(), 1)
-                torch.cuda.synchronize()
-                got = dst.double().cpu().numpy().view(np.complex128)
+                ctx.apply_device(op, prec, src.ptr, dst.ptr, 1)
+                ctx.synchronize()
+                got = dst.download(dt).astype(np.float64).view(np.complex128)
                 want = oracle.dwf_apply(op, th, psi, Nt, Nx, Ls, M5, MF)
                 assert rel(got, want) <= tol, (prec, op, rel(got, want))
+
+
+def test_copy_kernel_and_timer():
+    print_case(test="copy_kernel")
+    n = 1 << 20
+    a = np.arange(n, dtype=np.float32)
+    with dwf.DwfContext.dwf(4, 4, 4, M5, MF) as ctx:
+        src = capi.DeviceArray.from_host(a)
+        dst = capi.DeviceArray(a.nbytes)
+        t = capi.Timer()
+        t.start(ctx)
+        ctx.copy_device(dst.ptr, src.ptr, a.nbytes)
+        t.stop(ctx)
+        assert t.elapsed_ms() >= 0.0
+        assert np.array_equal(dst.download(np.float32), a)

@@ -37,6 +37,10 @@ DWF_PREC_FP32 = 1
 DWF_SOLVE_CG64 = 0
 DWF_SOLVE_MIXED = 1
 
+DWF_TUNE_SCHUNKS = 0
+DWF_TUNE_XCD_REMAP = 1
+DWF_TUNE_RELIABLE_DELTA = 2
+
 
 class DwfError(RuntimeError):
     """A non-zero status from libdwfhip; ``code`` is the C ABI status."""
@@ -142,6 +146,7 @@ PROTOTYPES = {
     "dwf_ctx_create_wilson": (_I, [_PP, _I, _I, _D, _D, _D]),
     "dwf_ctx_destroy": (_I, [_P]),
     "dwf_ctx_stream": (_P, [_P]),
+    "dwf_ctx_tune": (_I, [_P, _I, _D]),
     "dwf_set_gauge": (_I, [_P, _P]),
     "dwf_set_gauge_device": (_I, [_P, _P]),
     "dwf_apply": (_I, [_P, _I, _P, _P, _I]),
@@ -155,6 +160,17 @@ PROTOTYPES = {
     "dwf_get_gauge": (_I, [_P, _P]),
     "dwf_hmc_trajectory": (_I, [_P, _P, _P, _P, _D, _P]),
     "dwf_measure": (_I, [_P, _P, _P]),
+    "dwf_device_alloc": (_P, [ctypes.c_ulonglong]),
+    "dwf_device_free": (_I, [_P]),
+    "dwf_memcpy_htod": (_I, [_P, _P, ctypes.c_ulonglong]),
+    "dwf_memcpy_dtoh": (_I, [_P, _P, ctypes.c_ulonglong]),
+    "dwf_ctx_synchronize": (_I, [_P]),
+    "dwf_timer_create": (_P, []),
+    "dwf_timer_destroy": (None, [_P]),
+    "dwf_timer_start": (_I, [_P, _P]),
+    "dwf_timer_stop": (_I, [_P, _P]),
+    "dwf_timer_elapsed_ms": (_D, [_P]),
+    "dwf_copy_device": (_I, [_P, _P, _P, ctypes.c_ulonglong]),
     "dwf_rng_create": (_P, [ctypes.c_ulonglong]),
     "dwf_rng_destroy": (None, [_P]),
     "dwf_rng_dwf_heatbath": (None, [_P, _I, _I, _P, _P]),
@@ -258,6 +274,15 @@ class DwfContext:
     @property
     def stream(self) -> int:
         return self._lib.dwf_ctx_stream(self._h) or 0
+
+    def tune(self, param: int, value: float):
+        _check(self._lib, self._lib.dwf_ctx_tune(self._h, param, value), "dwf_ctx_tune")
+
+    def synchronize(self):
+        _check(self._lib, self._lib.dwf_ctx_synchronize(self._h), "dwf_ctx_synchronize")
+
+    def copy_device(self, dst_ptr: int, src_ptr: int, nbytes: int):
+        _check(self._lib, self._lib.dwf_copy_device(self._h, dst_ptr, src_ptr, nbytes), "dwf_copy_device")
 
     # gauge --------------------------------------------------------------------------
     def set_gauge(self, theta: np.ndarray):
@@ -421,3 +446,71 @@ def z4_noise(seed: int, nnoise: int, Nt: int, Nx: int) -> np.ndarray:
     eta = np.zeros(nnoise * Nt * Nx * 2, np.complex128)
     load_library().dwf_z4_noise(seed, nnoise, Nt * Nx * 2, _ptr(eta))
     return eta.reshape(nnoise, Nt, Nx, 2)
+
+
+class DeviceArray:
+    """A device buffer owned through libdwfhip (no other GPU runtime needed)."""
+
+    def __init__(self, nbytes: int):
+        self._lib = load_library()
+        self.nbytes = int(nbytes)
+        self.ptr = self._lib.dwf_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise DwfError(DWF_ERR_HIP, self._lib.dwf_last_error().decode())
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("host array larger than the device buffer")
+        _check(self._lib, self._lib.dwf_memcpy_htod(self.ptr, _ptr(a), a.nbytes), "dwf_memcpy_htod")
+
+    def download(self, dtype, count: int | None = None) -> np.ndarray:
+        itemsize = np.dtype(dtype).itemsize
+        n = self.nbytes // itemsize if count is None else count
+        out = np.empty(n, dtype)
+        _check(self._lib, self._lib.dwf_memcpy_dtoh(_ptr(out), self.ptr, out.nbytes), "dwf_memcpy_dtoh")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self._lib.dwf_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Timer:
+    """HIP events on a context's stream."""
+
+    def __init__(self):
+        self._lib = load_library()
+        self._h = self._lib.dwf_timer_create()
+
+    def start(self, ctx: DwfContext):
+        _check(self._lib, self._lib.dwf_timer_start(self._h, ctx.handle), "dwf_timer_start")
+
+    def stop(self, ctx: DwfContext):
+        _check(self._lib, self._lib.dwf_timer_stop(self._h, ctx.handle), "dwf_timer_stop")
+
+    def elapsed_ms(self) -> float:
+        v = self._lib.dwf_timer_elapsed_ms(self._h)
+        if v < 0:
+            raise DwfError(DWF_ERR_HIP, "dwf_timer_elapsed_ms failed")
+        return v
+
+    def __del__(self):
+        try:
+            self._lib.dwf_timer_destroy(self._h)
+        except Exception:
+            pass

@@ -57,6 +57,7 @@ struct dwf_ctx {
     float2* l32 = nullptr;    // [2][V4] (cos, sin) FP32
     bool have_gauge = false;
     int chunks_override = 0;      // s-chunks per 4D site for the stencils (0: heuristic)
+    int xcd_remap = 1;            // XCD-aware block order for the stencils
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
 
     struct Buf {

@@ -106,13 +106,18 @@ __device__ __forceinline__ double sum_partials(const double* p, int n, int which
 template <typename R, bool DAG, bool D5, int EPI>
 __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
 {
-    const int rhs = blockIdx.z;
+    // 1D grid over (tile, s-chunk, rhs), optionally in XCD-aware order
+    const int nblk = a.cfg.nblk();
+    const int total = gridDim.x;
+    const int lb = a.cfg.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
+    const int rhs = lb / nblk;
+    const int bid = lb - rhs * nblk;            // block within this right-hand side
+    const int chunk = bid / a.cfg.bx;
+    const int tile = bid - chunk * a.cfg.bx;
     const Lat& L = a.L;
     if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
 
     __shared__ double sh[8];
-    const int nblk = a.cfg.nblk();
-    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
 
     R alpha = 0;
     if constexpr (EPI == EPI_CGUPD) {
@@ -124,10 +129,10 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
     const int V4 = L.V4;
     const long long V5 = L.V5();
     const long long rb = (long long)rhs * V5 * 4;  // offset in reals
-    const int site = blockIdx.x * kBlock + threadIdx.x;
+    const int site = tile * kBlock + threadIdx.x;
     double acc0 = 0.0, acc1 = 0.0;
 
-    const int s0 = blockIdx.y * a.cfg.spt;
+    const int s0 = chunk * a.cfg.spt;
     const int s1 = min(L.Ls, s0 + a.cfg.spt);
     if (site < V4 && s0 < s1) {
         const int Nx = L.Nx, Nt = L.Nt;
@@ -318,9 +323,8 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const long long V5, const int n
     }
     if (zero || (mode == MODE_CG64 && conv)) return;
     const R beta = (R)(rr_new / rr_old);
-    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (i < V5) {
-        const long long rb = (long long)rhs * V5 * 4;
+    const long long rb = (long long)rhs * V5 * 4;
+    for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < V5; i += (long long)gridDim.x * kBlock) {
         const Sp<R> rv = ldsp(r + rb, i);
         Sp<R> pv = ldsp(p + rb, i);
         pv.ar = rv.ar + beta * pv.ar;
@@ -461,9 +465,10 @@ void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 
 }  // namespace
 
-HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override)
+HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap)
 {
     HopCfg c{};
+    c.remap = remap;
     c.bx = (L.V4 + kBlock - 1) / kBlock;
     int chunks = 1;
     if (chunks_override > 0) {
@@ -482,7 +487,7 @@ HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override)
 
 template <typename R> void launch_hop(bool dagger, int epi, const HopArgs<R>& a, int nrhs, hipStream_t s)
 {
-    const dim3 grid(a.cfg.bx, a.cfg.chunks, nrhs);
+    const dim3 grid((unsigned)(a.cfg.nblk() * nrhs));
     if (a.L.d5) {
         if (dagger)
             dispatch_epi<R, true, true>(epi, a, grid, s);
@@ -503,7 +508,8 @@ void launch_cg_p(const Lat& L, const HopCfg& cfg, R* r, R* p, const double* part
                  CgState* st, int slot, int mode, int nrhs, hipStream_t s)
 {
     const long long V5 = L.V5();
-    const dim3 grid((unsigned)((V5 + kBlock - 1) / kBlock), 1, nrhs);
+    const long long per = (V5 + 4LL * kBlock - 1) / (4LL * kBlock);  // 4 sites per thread
+    const dim3 grid((unsigned)per, 1, nrhs);
     k_cg_p<R><<<grid, kBlock, 0, s>>>(V5, cfg.nblk(), r, p, partA, partB, st, slot, mode);
 }
 template void launch_cg_p<float>(const Lat&, const HopCfg&, float*, float*, const double*, const double*,

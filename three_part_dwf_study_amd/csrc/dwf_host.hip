@@ -96,7 +96,7 @@ template <typename R> HopArgs<R> hop_base(const dwf_ctx* c, const HopCfg& cfg)
 // out = op(in) on device buffers; tmp is needed for the normal operators.
 template <typename R> void apply_op(dwf_ctx* c, int op, const R* in, R* out, R* tmp, int nrhs)
 {
-    const HopCfg cfg = hop_cfg(c->L, nrhs, c->chunks_override);
+    const HopCfg cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap);
     HopArgs<R> a = hop_base<R>(c, cfg);
     switch (op) {
         case DWF_OP_D:
@@ -220,7 +220,7 @@ int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, c
     const size_t nv = (size_t)n4 * nrhs;
     const bool first_dag = (normal == DWF_OP_DDDAG);
     const bool mixed = (method == DWF_SOLVE_MIXED);
-    const HopCfg cfg = hop_cfg(L, nrhs, c->chunks_override);
+    const HopCfg cfg = hop_cfg(L, nrhs, c->chunks_override, c->xcd_remap);
     const size_t npart = (size_t)nrhs * cfg.nblk() * 2;
 
     SolveBufs B{};
@@ -408,6 +408,28 @@ int dwf_ctx_destroy(dwf_ctx* ctx)
 }
 
 void* dwf_ctx_stream(dwf_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
+{
+    if (!ctx) return DWF_ERR_ARG;
+    switch (param) {
+        case DWF_TUNE_SCHUNKS:
+            if (value < 0 || value > ctx->L.Ls) break;
+            ctx->chunks_override = (int)value;
+            return DWF_OK;
+        case DWF_TUNE_XCD_REMAP:
+            ctx->xcd_remap = value != 0.0 ? 1 : 0;
+            return DWF_OK;
+        case DWF_TUNE_RELIABLE_DELTA:
+            if (!(value > 0.0 && value < 1.0)) break;
+            ctx->reliable_delta = value;
+            return DWF_OK;
+        default:
+            break;
+    }
+    dwf_set_error("dwf_ctx_tune: bad parameter or value");
+    return DWF_ERR_ARG;
+}
 
 int dwf_set_gauge_device(dwf_ctx* c, const double* theta_device)
 {

@@ -46,10 +46,24 @@ struct Lat {
 // walks spt consecutive slices reusing its four links.
 struct HopCfg {
     int bx;      // blocks along the 4D sites
-    int chunks;  // s-chunks (grid.y)
+    int chunks;  // s-chunks
     int spt;     // slices per thread
+    int remap;   // 1: XCD-aware block order (contiguous tiles per XCD)
     __host__ __device__ int nblk() const { return bx * chunks; }
 };
+
+// Logical block id for a 1D grid of `total` blocks.  Workgroups are dealt round-robin
+// to the 8 XCDs, each with its own L2; this bijection hands every XCD a contiguous run
+// of logical blocks, so neighbouring tiles (which share halo rows) share one L2.
+__device__ __forceinline__ int xcd_logical_block(int hw, int total)
+{
+    constexpr int kXcd = 8;
+    const int xcd = hw % kXcd, local = hw / kXcd;
+    const int per = (total + kXcd - 1) / kXcd;
+    int tall = total % kXcd;
+    if (tall == 0) tall = kXcd;
+    return xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+}
 
 enum Epi : int {
     EPI_STORE = 0,  // out = Op in
@@ -96,7 +110,7 @@ void launch_accum_x(long long n4, double* x64, float* x32, const CgState* st, in
 template <typename R> void launch_zero(R* p, long long n, hipStream_t s);
 void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s);
 
-HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override = 0);
+HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override = 0, int remap = 1);
 
 // HMC kernels (dwf_hmc.hip).  `scalars` is a small device array of FP64 results; each
 // reduction folds its block partials in index order into scalars[slot].
