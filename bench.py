@@ -133,18 +133,16 @@ def main():
 
     n = nrhs * LS * NT * NX * 4
     d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
-    d_b = capi.DeviceArray(n * 8)
     d_x = capi.DeviceArray(n * 8)
-    ctx.apply_device(dwf.DWF_OP_DDAG, dwf.DWF_PREC_FP64, d_src.ptr, d_b.ptr, nrhs)
-    ctx.synchronize()
 
     def solve(method):
-        st, rc = ctx.solve_device(dwf.DWF_OP_DDAGD, method, d_b.ptr, None, TOL, 20000, nrhs, d_x.ptr)
+        st, rc = ctx.solve_dirac_device(method, d_src.ptr, None, TOL, 20000, nrhs, d_x.ptr)
         return st
 
     results = {}
-    for name, method, steps in (("mixed", dwf.DWF_SOLVE_MIXED, args.steps),
-                                ("fp64", dwf.DWF_SOLVE_CG64, max(1, args.steps))):
+    for name, method in (("eo_mixed", dwf.DWF_SOLVE_EO_MIXED), ("mixed", dwf.DWF_SOLVE_MIXED),
+                         ("fp64_cg", dwf.DWF_SOLVE_CG64)):
+        steps = args.steps
         for _ in range(args.warmup):
             solve(method)
         ctx.synchronize()
@@ -160,7 +158,7 @@ def main():
                              ms_per_step=dt / steps * 1e3,
                              iterations=float(np.mean([s["iterations"] for s in st])),
                              reliable_updates=float(np.mean([s["reliable_updates"] for s in st])),
-                             fp64_iterations=float(np.mean([s["fp64_iterations"] for s in st])),
+                             passes=float(np.mean([s["passes"] for s in st])),
                              max_true_relres=tr)
         log(f"{name}: {results[name]}")
 
@@ -174,19 +172,21 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        b1 = d_b.download(np.float64, LS * NT * NX * 4).view(np.complex128)
-        cpu = cpu_baseline(theta, b1, results["fp64"]["iterations"], args.cpu_iters)
+        import oracle
+
+        b1 = oracle.dwf_apply(1, theta, src[0].reshape(-1), NT, NX, LS, M5, MF)
+        cpu = cpu_baseline(theta, b1, results["fp64_cg"]["iterations"], args.cpu_iters)
         log(f"cpu baseline: {cpu}")
 
-    mixed, judge = results["mixed"], results["fp64"]
+    fast, judge = results["eo_mixed"], results["fp64_cg"]
     line = {
         "metric": METRIC,
-        "value": round(mixed["solves_per_s"], 3),
+        "value": round(fast["solves_per_s"], 3),
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": mixed["steps"],
+        "steps": fast["steps"],
         "warmup": args.warmup,
-        "ms_per_step": round(mixed["ms_per_step"], 3),
+        "ms_per_step": round(fast["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -195,13 +195,15 @@ def main():
         "config": {
             "workload": "DWF CGNR solves, 96x96 Ls=16 M5=1 mf=0.05, theta~N(0,1/4) seed 1234, "
                         "Gaussian boundary sources, FP64 true residual <= 1e-10",
-            "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs, "solver": "FP32 CG + FP64 reliable updates",
+            "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs,
+            "solver": "even-odd Schur complement, FP32 CGNR + FP64 reliable updates + FP64 defect correction",
         },
         "fp64_cg_solves_per_s": round(judge["solves_per_s"], 3),
-        "speedup_vs_fp64_cg": round(mixed["solves_per_s"] / judge["solves_per_s"], 3),
-        "iterations": {"mixed_fp32": mixed["iterations"], "mixed_reliable_updates": mixed["reliable_updates"],
-                       "fp64_cg": judge["iterations"]},
-        "max_true_relres": {"mixed": mixed["max_true_relres"], "fp64_cg": judge["max_true_relres"]},
+        "mixed_solves_per_s": round(results["mixed"]["solves_per_s"], 3),
+        "speedup_vs_fp64_cg": round(fast["solves_per_s"] / judge["solves_per_s"], 3),
+        "iterations": {k: v["iterations"] for k, v in results.items()},
+        "reliable_updates": {k: v["reliable_updates"] for k, v in results.items()},
+        "max_true_relres": {k: v["max_true_relres"] for k, v in results.items()},
         "dslash_gbs": {"fp32_batch%d" % args.dslash_rhs: round(gbs32, 1),
                        "fp64_batch%d" % (args.dslash_rhs // 2): round(gbs64, 1),
                        "fp32_single_rhs_cache_resident": round(gbs1, 1),

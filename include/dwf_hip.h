@@ -54,7 +54,12 @@ enum dwf_method {
      * stops on the recursive residual, as the reference does. */
     DWF_SOLVE_CG64 = 0,
     /* FP32 CG with FP64 reliable updates; stops on the FP64 true residual. */
-    DWF_SOLVE_MIXED = 1
+    DWF_SOLVE_MIXED = 1,
+    /* dwf_solve_dirac only: even-odd (4D checkerboard) Schur complement of the DWF
+     * operator, FP32 CGNR with FP64 reliable updates on the odd half-lattice and FP64
+     * defect correction; certified on the reference's full system.  Needs even Nt, Nx
+     * and Ls in {4, 8, 12, 16}; otherwise it runs as DWF_SOLVE_MIXED. */
+    DWF_SOLVE_EO_MIXED = 2
 };
 
 typedef struct dwf_ctx dwf_ctx;
@@ -65,6 +70,8 @@ typedef struct dwf_solve_stats {
     int converged;         /* 1 if the stopping rule was met                           */
     int reliable_updates;  /* FP64 true-residual recomputations (mixed only)           */
     int fp64_iterations;   /* FP64 iterations (CG64, or the mixed solver's fallback)   */
+    int passes;            /* even-odd defect-correction passes (0 for the other solvers) */
+    int pad;
     double relres;         /* the solver's own ||r||/||b|| at exit (recursive for CG64) */
     double true_relres;    /* FP64 ||b - A x|| / ||b|| recomputed at exit              */
 } dwf_solve_stats;
@@ -127,6 +134,18 @@ int dwf_solve(dwf_ctx* ctx, int normal_op, int method, const double* b, const do
 int dwf_solve_device(dwf_ctx* ctx, int normal_op, int method, const double* b_device,
                      const double* x0_device, double rtol, int maxiter, int nrhs,
                      double* x_device, dwf_solve_stats* stats);
+
+/* x = D^-1 src for nrhs sources, the propagator pattern of the analyzers: rhs =
+ * D^dagger src, conjugate_gradient(D^dagger D, rhs) (correlator.hpp:77-79, 139-142,
+ * 213-220; wilson/include/cg_solver.hpp:19-51).  CG64 stops on the recursive residual as
+ * the reference; MIXED and EO_MIXED certify on the FP64 true residual of that same
+ * system, ||D^dagger src - D^dagger D x|| / ||D^dagger src|| <= rtol.  stats[i].relres
+ * is the solver's own residual, stats[i].true_relres the certified one. */
+int dwf_solve_dirac(dwf_ctx* ctx, int method, const double* src, const double* x0, double rtol,
+                    int maxiter, int nrhs, double* x, dwf_solve_stats* stats);
+int dwf_solve_dirac_device(dwf_ctx* ctx, int method, const double* src_device,
+                           const double* x0_device, double rtol, int maxiter, int nrhs,
+                           double* x_device, dwf_solve_stats* stats);
 
 /* ---- pseudofermion force, gauge action, HMC ---------------------------------------- */
 

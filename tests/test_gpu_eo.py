@@ -1,0 +1,79 @@
+"""GPU tests of the even-odd Schur-complement solver (dwf_solve_dirac, EO_MIXED):
+x = D^-1 src certified by the oracle's own FP64 operator on the reference's system
+||D^dag src - D^dag D x|| / ||D^dag src||, without falling back to the full-lattice
+solver (passes >= 1), and equal to the oracle's CG solution."""
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import print_case
+from dense_ref import rand_spinor, rand_theta
+
+import three_part_dwf_study_amd as dwf
+
+pytestmark = pytest.mark.gpu
+
+M5, MF = 1.0, 0.05
+
+
+def normal_relres(th, x, src, Nt, Nx, Ls, mf=MF):
+    B = oracle.dwf_apply(1, th, src, Nt, Nx, Ls, M5, mf)
+    r = B - oracle.dwf_apply(3, th, x, Nt, Nx, Ls, M5, mf)
+    return np.linalg.norm(r) / np.linalg.norm(B)
+
+
+@pytest.mark.parametrize("Nt,Nx,Ls,nrhs", [(8, 12, 4, 2), (16, 16, 8, 1), (12, 8, 12, 3), (96, 96, 16, 2)])
+def test_eo_solve_certified(Nt, Nx, Ls, nrhs):
+    print_case(test="eo_solve", Nt=Nt, Nx=Nx, Ls=Ls, nrhs=nrhs)
+    rng = np.random.default_rng(Nt * 100 + Ls)
+    th = rand_theta(rng, Nt, Nx)
+    src = rand_spinor(rng, nrhs * Ls * Nt * Nx * 2).reshape(nrhs, -1)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        x, st, rc = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, method=dwf.DWF_SOLVE_EO_MIXED)
+        x2, st2, rc2 = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, method=dwf.DWF_SOLVE_MIXED)
+    assert rc == 0
+    for i in range(nrhs):
+        assert st[i]["converged"] and 1 <= st[i]["passes"] <= 3, st[i]
+        tr = normal_relres(th, x[i], src[i], Nt, Nx, Ls)
+        assert tr <= 1e-10, tr
+        assert abs(tr - st[i]["true_relres"]) <= 1e-12
+        assert np.linalg.norm(x[i] - x2[i]) / np.linalg.norm(x2[i]) < 1e-6
+        # the even-odd system needs fewer iterations than the full normal equations
+        assert st[i]["iterations"] < st2[i]["iterations"], (st[i]["iterations"], st2[i]["iterations"])
+
+
+def test_eo_matches_oracle_cg():
+    Nt, Nx, Ls = 8, 8, 8
+    print_case(test="eo_vs_oracle")
+    rng = np.random.default_rng(42)
+    th = rand_theta(rng, Nt, Nx)
+    src = rand_spinor(rng, Ls * Nt * Nx * 2)
+    B = oracle.dwf_apply(1, th, src, Nt, Nx, Ls, M5, MF)
+    xo, it, rr, conv, rc = oracle.dwf_cg(3, th, B, Nt, Nx, Ls, M5, MF, 1e-12, 20000)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        x, st, rc = ctx.solve_dirac(src, rtol=1e-12, maxiter=20000)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-9
+
+
+def test_eo_falls_back_on_odd_extent_and_guess():
+    Nt, Nx, Ls = 7, 6, 4
+    print_case(test="eo_fallback")
+    rng = np.random.default_rng(3)
+    th = rand_theta(rng, Nt, Nx)
+    src = rand_spinor(rng, Ls * Nt * Nx * 2)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        x, st, rc = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000)
+        assert rc == 0 and st[0]["passes"] == 0 and normal_relres(th, x, src, Nt, Nx, Ls) <= 1e-10
+    with dwf.DwfContext.dwf(8, 6, 4, M5, MF) as ctx:
+        th = rand_theta(rng, 8, 6)
+        src = rand_spinor(rng, 4 * 8 * 6 * 2)
+        ctx.set_gauge(th)
+        x, st, rc = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000)
+        g = x + 1e-4 * rand_spinor(rng, x.size)
+        x2, st2, rc = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, x0=g)
+        assert rc == 0 and normal_relres(th, x2, src, 8, 6, 4) <= 1e-10
+        assert st2[0]["iterations"] < st[0]["iterations"]

@@ -15,7 +15,11 @@ from .capi import (  # noqa: F401
     DWF_PREC_FP32,
     DWF_PREC_FP64,
     DWF_SOLVE_CG64,
+    DWF_SOLVE_EO_MIXED,
     DWF_SOLVE_MIXED,
+    DWF_TUNE_RELIABLE_DELTA,
+    DWF_TUNE_SCHUNKS,
+    DWF_TUNE_XCD_REMAP,
     DwfContext,
     DwfError,
     build_library,

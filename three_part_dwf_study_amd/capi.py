@@ -36,6 +36,7 @@ DWF_PREC_FP32 = 1
 
 DWF_SOLVE_CG64 = 0
 DWF_SOLVE_MIXED = 1
+DWF_SOLVE_EO_MIXED = 2
 
 DWF_TUNE_SCHUNKS = 0
 DWF_TUNE_XCD_REMAP = 1
@@ -56,6 +57,8 @@ class SolveStats(ctypes.Structure):
         ("converged", ctypes.c_int),
         ("reliable_updates", ctypes.c_int),
         ("fp64_iterations", ctypes.c_int),
+        ("passes", ctypes.c_int),
+        ("pad", ctypes.c_int),
         ("relres", ctypes.c_double),
         ("true_relres", ctypes.c_double),
     ]
@@ -153,6 +156,8 @@ PROTOTYPES = {
     "dwf_apply_device": (_I, [_P, _I, _I, _P, _P, _I]),
     "dwf_solve": (_I, [_P, _I, _I, _P, _P, _D, _I, _I, _P, _P]),
     "dwf_solve_device": (_I, [_P, _I, _I, _P, _P, _D, _I, _I, _P, _P]),
+    "dwf_solve_dirac": (_I, [_P, _I, _P, _P, _D, _I, _I, _P, _P]),
+    "dwf_solve_dirac_device": (_I, [_P, _I, _P, _P, _D, _I, _I, _P, _P]),
     "dwf_pf_action_force": (_I, [_P, _I, _P, _P, _D, _I, _P, _P, _P, _P]),
     "dwf_force_from_eta": (_I, [_P, _P, _P]),
     "dwf_gauge_action_force": (_I, [_P, _D, _P, _P]),
@@ -330,6 +335,31 @@ class DwfContext:
         if raise_on_fail:
             _check(self._lib, rc, "dwf_solve")
         return x.reshape(np.shape(b)), [s.as_dict() for s in stats], rc
+
+    def solve_dirac(self, src: np.ndarray, rtol: float = 1e-10, maxiter: int = 10000,
+                    method: int = DWF_SOLVE_EO_MIXED, x0: np.ndarray | None = None, raise_on_fail: bool = True):
+        """x = D^-1 src (certified on the D^dag D system); returns (x, [stats], rc)."""
+        bb, nrhs = self._as_batch(src)
+        x = np.empty_like(bb)
+        x0p = None
+        if x0 is not None:
+            x0a, _ = self._as_batch(x0)
+            x0p = _ptr(x0a)
+        stats = (SolveStats * nrhs)()
+        rc = self._lib.dwf_solve_dirac(self._h, method, _ptr(bb), x0p, rtol, maxiter, nrhs, _ptr(x),
+                                       ctypes.cast(stats, ctypes.c_void_p))
+        if raise_on_fail:
+            _check(self._lib, rc, "dwf_solve_dirac")
+        return x.reshape(np.shape(src)), [s.as_dict() for s in stats], rc
+
+    def solve_dirac_device(self, method: int, src_ptr: int, x0_ptr: int | None, rtol: float, maxiter: int,
+                           nrhs: int, x_ptr: int, raise_on_fail: bool = True):
+        stats = (SolveStats * nrhs)()
+        rc = self._lib.dwf_solve_dirac_device(self._h, method, src_ptr, x0_ptr, rtol, maxiter, nrhs, x_ptr,
+                                              ctypes.cast(stats, ctypes.c_void_p))
+        if raise_on_fail:
+            _check(self._lib, rc, "dwf_solve_dirac_device")
+        return [s.as_dict() for s in stats], rc
 
     def solve_device(self, normal: int, method: int, b_ptr: int, x0_ptr: int | None, rtol: float,
                      maxiter: int, nrhs: int, x_ptr: int, raise_on_fail: bool = True):

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "dwf_internal.hpp"
+#include "../../include/dwf_hip.h"
 
 #include <functional>
 #include <stdexcept>
@@ -44,6 +45,21 @@ enum BufId : int {
     B_THETA0,  // HMC: configuration at the start of the trajectory
     B_SCAL,    // FP64 scalar results
     B_RPART,   // reduction partials
+    B_TQ64,    // CG64 direction image
+    B_STATE2,  // certification states
+    B_PR2,     // certification partials
+    B_EO_B,    // D^dag src
+    B_EO_RSRC, // Dirac residual of the current iterate
+    B_EO_BE,
+    B_EO_BO,
+    B_EO_UE,
+    B_EO_BH,
+    B_EO_BHAT,
+    B_EO_XO,
+    B_EO_XE,
+    B_EO_Q,
+    B_EO_TE32,
+    B_EO_DELTA,
     B_COUNT
 };
 
@@ -80,5 +96,7 @@ void dwf_apply_device_impl(dwf_ctx* c, int op, int prec, const void* in, void* o
 // Scalar slots in the B_SCAL buffer.
 enum ScalarSlot : int { S_GAUGE = 0, S_PLAQ, S_PF, S_KIN, S_COUNT = 8 };
 
+int dwf_solve_dirac_impl(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter,
+                         int nrhs, double* x, dwf_solve_stats* stats);
 int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0,
                           double rtol, int maxiter, int nrhs, double* x);

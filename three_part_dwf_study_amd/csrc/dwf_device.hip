@@ -504,34 +504,31 @@ template void launch_hop<float>(bool, int, const HopArgs<float>&, int, hipStream
 template void launch_hop<double>(bool, int, const HopArgs<double>&, int, hipStream_t);
 
 template <typename R>
-void launch_cg_p(const Lat& L, const HopCfg& cfg, R* r, R* p, const double* partA, const double* partB,
-                 CgState* st, int slot, int mode, int nrhs, hipStream_t s)
+void launch_cg_p(long long sites, int nblk, R* r, R* p, const double* partA, const double* partB, CgState* st,
+                 int slot, int mode, int nrhs, hipStream_t s)
 {
-    const long long V5 = L.V5();
-    const long long per = (V5 + 4LL * kBlock - 1) / (4LL * kBlock);  // 4 sites per thread
+    const long long per = (sites + 4LL * kBlock - 1) / (4LL * kBlock);  // 4 sites per thread
     const dim3 grid((unsigned)per, 1, nrhs);
-    k_cg_p<R><<<grid, kBlock, 0, s>>>(V5, cfg.nblk(), r, p, partA, partB, st, slot, mode);
+    k_cg_p<R><<<grid, kBlock, 0, s>>>(sites, nblk, r, p, partA, partB, st, slot, mode);
 }
-template void launch_cg_p<float>(const Lat&, const HopCfg&, float*, float*, const double*, const double*,
-                                 CgState*, int, int, int, hipStream_t);
-template void launch_cg_p<double>(const Lat&, const HopCfg&, double*, double*, const double*,
-                                  const double*, CgState*, int, int, int, hipStream_t);
+template void launch_cg_p<float>(long long, int, float*, float*, const double*, const double*, CgState*, int,
+                                 int, int, hipStream_t);
+template void launch_cg_p<double>(long long, int, double*, double*, const double*, const double*, CgState*, int,
+                                  int, int, hipStream_t);
 
-void launch_cg_start(const HopCfg& cfg, const double* partR, CgState* st, double tol, double delta,
-                     int nrhs, hipStream_t s)
+void launch_cg_start(int nblk, const double* partR, CgState* st, double tol, double delta, int nrhs, hipStream_t s)
 {
-    k_cg_start<<<nrhs, 64, 0, s>>>(cfg.nblk(), partR, st, tol, delta);
-}
-
-void launch_ru_commit(const HopCfg& cfg, const double* partR, CgState* st, int slot, int nrhs,
-                      hipStream_t s)
-{
-    k_ru_commit<<<nrhs, 64, 0, s>>>(cfg.nblk(), partR, st, slot);
+    k_cg_start<<<nrhs, 64, 0, s>>>(nblk, partR, st, tol, delta);
 }
 
-void launch_true_resid(const HopCfg& cfg, const double* partR, CgState* st, int nrhs, hipStream_t s)
+void launch_ru_commit(int nblk, const double* partR, CgState* st, int slot, int nrhs, hipStream_t s)
 {
-    k_true_resid<<<nrhs, 64, 0, s>>>(cfg.nblk(), partR, st);
+    k_ru_commit<<<nrhs, 64, 0, s>>>(nblk, partR, st, slot);
+}
+
+void launch_true_resid(int nblk, const double* partR, CgState* st, int nrhs, hipStream_t s)
+{
+    k_true_resid<<<nrhs, 64, 0, s>>>(nblk, partR, st);
 }
 
 void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s)

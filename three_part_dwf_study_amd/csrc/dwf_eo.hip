@@ -1,0 +1,499 @@
+// dwf_eo.hip -- even-odd (4D checkerboard) kernels of the Schur-complement DWF solver.
+//
+// The Shamir operator splits as D = A - 1/2 H, with A = (3-M5) + D5 local in 4D (it
+// couples only the s-slices of one 4D site) and H the 4D hopping term, which connects
+// the two 4D parities (dwf_operator.hpp:15-92).  With Nt, Nx even:
+//
+//   D x = b  <=>  Dhat x_o = b_o + 1/2 H_oe A^-1 b_e,   x_e = A^-1 (b_e + 1/2 H_eo x_o),
+//   Dhat = A - 1/4 H_oe A^-1 H_eo,   Dhat^dag = A^dag - 1/4 H^dag_oe A^-dag H^dag_eo,
+//
+// where H^dag is the hopping term of D^dag (dwf_operator.hpp:94-171).
+//
+// A^-1 has a closed form per spin: spin 0 of A is d + S_dn (lower bidiagonal with the
+// wall term +mf in the corner), spin 1 is d + S_up (upper), with d = 3-M5; A^dag swaps
+// them.  Each is solved by one O(Ls) recurrence plus the wall correction
+//   lower: beta_0 = 0, beta_s = (v_s + beta_{s-1})/d, y_0 = (v_0 - mf beta_{L-1})/(d + mf d^{1-L}),
+//          y_s = d^-s y_0 + beta_s;
+//   upper: the mirror image from s = L-1 downwards,
+// so a thread that owns one 4D site keeps the whole s-column in registers (Ls is a
+// template parameter).
+//
+// Half-lattice layout: parity p = (t+x)&1 holds V4/2 sites, row-major (t, xh) with
+// x = 2 xh + ((t+p)&1); a vector is [rhs][s][t][xh] spinors.  Links stay lexicographic.
+
+#include "dwf_internal.hpp"
+#include "dwf_eo.hpp"
+
+#include <cmath>
+
+namespace dwfk {
+namespace {
+
+template <typename R> struct V4T;
+template <> struct V4T<float> { using T = float4; };
+template <> struct V4T<double> { using T = double4; };
+template <typename R> struct C2T;
+template <> struct C2T<float> { using T = float2; };
+template <> struct C2T<double> { using T = double2; };
+
+template <typename R> struct Sp {
+    R ar, ai, br, bi;
+};
+
+template <typename R> __device__ __forceinline__ Sp<R> ld(const R* base, long long i)
+{
+    const typename V4T<R>::T v = reinterpret_cast<const typename V4T<R>::T*>(base)[i];
+    return Sp<R>{v.x, v.y, v.z, v.w};
+}
+template <typename R> __device__ __forceinline__ void st(R* base, long long i, const Sp<R>& s)
+{
+    typename V4T<R>::T v;
+    v.x = s.ar;
+    v.y = s.ai;
+    v.z = s.br;
+    v.w = s.bi;
+    reinterpret_cast<typename V4T<R>::T*>(base)[i] = v;
+}
+template <typename R> __device__ __forceinline__ double n2(const Sp<R>& s)
+{
+    const double a = s.ar, b = s.ai, c = s.br, d = s.bi;
+    return a * a + b * b + c * c + d * d;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* sh)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[w] = a;
+        sh[4 + w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+        b = (sh[4] + sh[5]) + (sh[6] + sh[7]);
+    }
+}
+__device__ __forceinline__ double sum_partials(const double* p, int n, double* sh)
+{
+    if (threadIdx.x < 64) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < n; i += 64) v += p[2 * i];
+        v = wave_sum(v);
+        if (threadIdx.x == 0) sh[0] = v;
+    }
+    __syncthreads();
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Geometry of one target-parity site: its lexicographic (t, x), the half-lattice
+// indices of its four neighbours (other parity), links and boundary signs.
+template <typename R> struct EoSite {
+    int t, x;
+    int ntp, ntm, nxp, nxm;
+    typename C2T<R>::T ut, ux, utm, uxm;
+    R stp, stm, sxp, sxm;
+};
+
+template <typename R>
+__device__ __forceinline__ EoSite<R> eo_site(const Lat& L, int Nxh, int tpar, int hs, const void* links)
+{
+    EoSite<R> g;
+    const int Nt = L.Nt, Nx = L.Nx, V4 = L.V4;
+    const int t = hs / Nxh, xh = hs - t * Nxh;
+    const int x = 2 * xh + ((t + tpar) & 1);
+    const int tp = (t + 1 == Nt) ? 0 : t + 1, tm = (t == 0) ? Nt - 1 : t - 1;
+    const int xp = (x + 1 == Nx) ? 0 : x + 1, xm = (x == 0) ? Nx - 1 : x - 1;
+    g.t = t;
+    g.x = x;
+    g.ntp = tp * Nxh + (x >> 1);
+    g.ntm = tm * Nxh + (x >> 1);
+    g.nxp = t * Nxh + (xp >> 1);
+    g.nxm = t * Nxh + (xm >> 1);
+    using C = typename C2T<R>::T;
+    const C* lk = static_cast<const C*>(links);
+    g.ut = lk[t * Nx + x];
+    g.ux = lk[V4 + t * Nx + x];
+    g.utm = lk[tm * Nx + x];
+    g.uxm = lk[V4 + t * Nx + xm];
+    g.stp = (t == Nt - 1) ? (R)L.bct : (R)1;
+    g.stm = (t == 0) ? (R)L.bct : (R)1;
+    g.sxp = (x == Nx - 1) ? (R)L.bcx : (R)1;
+    g.sxm = (x == 0) ? (R)L.bcx : (R)1;
+    return g;
+}
+
+// o += c * H psi at one slice (H: the bracket of D's hop, or D^dag's when DAG).
+template <typename R, bool DAG>
+__device__ __forceinline__ void add_hop(Sp<R>& o, R c, const EoSite<R>& g, const R* in, long long sl)
+{
+    constexpr R sg = DAG ? (R)-1 : (R)1;
+    const Sp<R> vtp = ld(in, sl + g.ntp), vtm = ld(in, sl + g.ntm);
+    const Sp<R> vxp = ld(in, sl + g.nxp), vxm = ld(in, sl + g.nxm);
+    {
+        const R ar = g.stp * (vtp.ar - sg * vtp.br), ai = g.stp * (vtp.ai - sg * vtp.bi);
+        const R hr = g.ut.x * ar - g.ut.y * ai, hi = g.ut.x * ai + g.ut.y * ar;
+        o.ar += c * hr;
+        o.ai += c * hi;
+        o.br -= c * sg * hr;
+        o.bi -= c * sg * hi;
+    }
+    {
+        const R br = g.stm * (vtm.ar + sg * vtm.br), bi = g.stm * (vtm.ai + sg * vtm.bi);
+        const R hr = g.utm.x * br + g.utm.y * bi, hi = g.utm.x * bi - g.utm.y * br;
+        o.ar += c * hr;
+        o.ai += c * hi;
+        o.br += c * sg * hr;
+        o.bi += c * sg * hi;
+    }
+    {
+        const R cr = g.sxp * (vxp.ar - sg * vxp.bi), ci = g.sxp * (vxp.ai + sg * vxp.br);
+        const R hr = g.ux.x * cr - g.ux.y * ci, hi = g.ux.x * ci + g.ux.y * cr;
+        o.ar += c * hr;
+        o.ai += c * hi;
+        o.br += c * sg * hi;
+        o.bi -= c * sg * hr;
+    }
+    {
+        const R dr = g.sxm * (vxm.ar + sg * vxm.bi), di = g.sxm * (vxm.ai - sg * vxm.br);
+        const R hr = g.uxm.x * dr + g.uxm.y * di, hi = g.uxm.x * di - g.uxm.y * dr;
+        o.ar += c * hr;
+        o.ai += c * hi;
+        o.br -= c * sg * hi;
+        o.bi += c * sg * hr;
+    }
+}
+
+// In-register solve along s.  LOWER: d y_s - y_{s-1} = v_s (s>0), d y_0 + mf y_{L-1} = v_0.
+// UPPER: d y_s - y_{s+1} = v_s (s<L-1), d y_{L-1} + mf y_0 = v_{L-1}.  re/im of one spin
+// component are passed as two arrays.
+template <typename R, int LS, bool UPPER>
+__device__ __forceinline__ void wall_solve(R (&vr)[LS], R (&vi)[LS], R d, R mf)
+{
+    const R inv = (R)1 / d;
+    R dpow = (R)1;  // d^{-(L-1)}
+#pragma unroll
+    for (int k = 1; k < LS; ++k) dpow *= inv;
+    const R den = (R)1 / (d + mf * dpow);
+    if constexpr (!UPPER) {
+        R br = 0, bi = 0;
+#pragma unroll
+        for (int s = 1; s < LS; ++s) {
+            br = (vr[s] + br) * inv;
+            bi = (vi[s] + bi) * inv;
+            vr[s] = br;
+            vi[s] = bi;
+        }
+        const R y0r = (vr[0] - mf * br) * den, y0i = (vi[0] - mf * bi) * den;
+        vr[0] = y0r;
+        vi[0] = y0i;
+        R a = (R)1;
+#pragma unroll
+        for (int s = 1; s < LS; ++s) {
+            a *= inv;
+            vr[s] += a * y0r;
+            vi[s] += a * y0i;
+        }
+    } else {
+        R br = 0, bi = 0;
+#pragma unroll
+        for (int s = LS - 2; s >= 0; --s) {
+            br = (vr[s] + br) * inv;
+            bi = (vi[s] + bi) * inv;
+            vr[s] = br;
+            vi[s] = bi;
+        }
+        const R ylr = (vr[LS - 1] - mf * br) * den, yli = (vi[LS - 1] - mf * bi) * den;
+        vr[LS - 1] = ylr;
+        vi[LS - 1] = yli;
+        R a = (R)1;
+#pragma unroll
+        for (int s = LS - 2; s >= 0; --s) {
+            a *= inv;
+            vr[s] += a * ylr;
+            vi[s] += a * yli;
+        }
+    }
+}
+
+// out = [A^-1 | A^-dag | 1] ( cb b + ch H psi_other ) on the target parity, whole
+// s-column per thread.  b and the hop are optional (cb == 0 / ch == 0 skip them).
+template <typename R, bool DAG, int LS, bool AINV>
+__global__ __launch_bounds__(kBlockA) void k_eo_a(const EoArgs<R> a)
+{
+    const int total = gridDim.x;
+    const int lb = a.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
+    const int rhs = lb / a.bxa;
+    const int hs = (lb - rhs * a.bxa) * kBlockA + threadIdx.x;
+    if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;
+    if (hs >= a.V4h) return;
+    const long long V5h = (long long)a.V4h * LS;
+    const long long rb = (long long)rhs * V5h * 4;
+    const EoSite<R> g = eo_site<R>(a.L, a.Nxh, a.tpar, hs, a.links);
+    R v0r[LS], v0i[LS], v1r[LS], v1i[LS];
+#pragma unroll
+    for (int s = 0; s < LS; ++s) {
+        const long long sl = (long long)s * a.V4h;
+        Sp<R> o{0, 0, 0, 0};
+        if (a.cb != (R)0) {
+            const Sp<R> bv = ld(a.b + rb, sl + hs);
+            o = Sp<R>{a.cb * bv.ar, a.cb * bv.ai, a.cb * bv.br, a.cb * bv.bi};
+        }
+        if (a.ch != (R)0) add_hop<R, DAG>(o, a.ch, g, a.hin + rb, sl);
+        v0r[s] = o.ar;
+        v0i[s] = o.ai;
+        v1r[s] = o.br;
+        v1i[s] = o.bi;
+    }
+    if constexpr (AINV) {
+        const R d = (R)a.L.diag, mf = (R)a.L.mf;
+        // A: spin 0 lower, spin 1 upper; A^dag: spin 0 upper, spin 1 lower
+        wall_solve<R, LS, DAG>(v0r, v0i, d, mf);
+        wall_solve<R, LS, !DAG>(v1r, v1i, d, mf);
+    }
+#pragma unroll
+    for (int s = 0; s < LS; ++s) st(a.out + rb, (long long)s * a.V4h + hs, Sp<R>{v0r[s], v0i[s], v1r[s], v1i[s]});
+}
+
+// out = ca [A | A^dag | 1] psi + ch H hin on the target parity, with the CG epilogues
+// of the full-lattice stencil (store / norm / CG update / FP64 residual).
+template <typename R, bool DAG, bool AOP, int EPI>
+__global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
+{
+    const int nblk = a.bx * a.chunks;
+    const int total = gridDim.x;
+    const int lb = a.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
+    const int rhs = lb / nblk;
+    const int bid = lb - rhs * nblk;
+    const int chunk = bid / a.bx;
+    const int hs = (bid - chunk * a.bx) * kBlock + threadIdx.x;
+    if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;
+    __shared__ double sh[8];
+    R alpha = 0;
+    if constexpr (EPI == EPI_CGUPD) {
+        const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, sh);
+        if (pAp == 0.0) return;
+        alpha = (R)(a.st[rhs].rr[a.slot] / pAp);
+    }
+    const int Ls = a.L.Ls, V4h = a.V4h;
+    const long long V5h = (long long)V4h * Ls;
+    const long long rb = (long long)rhs * V5h * 4;
+    double acc0 = 0.0, acc1 = 0.0;
+    const int s0 = chunk * a.spt, s1 = min(Ls, s0 + a.spt);
+    if (hs < V4h && s0 < s1) {
+        const EoSite<R> g = eo_site<R>(a.L, a.Nxh, a.tpar, hs, a.links);
+        const R d = (R)a.L.diag, mf = (R)a.L.mf;
+        const R* in = a.in + rb;
+        Sp<R> cur = ld(in, (long long)s0 * V4h + hs);
+        Sp<R> dn{0, 0, 0, 0};
+        if constexpr (AOP) dn = ld(in, (long long)((s0 == 0) ? Ls - 1 : s0 - 1) * V4h + hs);
+        for (int s = s0; s < s1; ++s) {
+            const long long sl = (long long)s * V4h;
+            Sp<R> up{0, 0, 0, 0};
+            if constexpr (AOP) up = ld(in, (long long)((s + 1 == Ls) ? 0 : s + 1) * V4h + hs);
+            Sp<R> o;
+            if constexpr (AOP) {
+                // A psi: d psi + D5 (D: spin1 <- s+1, spin0 <- s-1; D^dag swapped)
+                const R cup = (s + 1 == Ls) ? mf : (R)-1;
+                const R cdn = (s == 0) ? mf : (R)-1;
+                o = Sp<R>{d * cur.ar, d * cur.ai, d * cur.br, d * cur.bi};
+                if constexpr (!DAG) {
+                    o.br += cup * up.br;
+                    o.bi += cup * up.bi;
+                    o.ar += cdn * dn.ar;
+                    o.ai += cdn * dn.ai;
+                } else {
+                    o.br += cdn * dn.br;
+                    o.bi += cdn * dn.bi;
+                    o.ar += cup * up.ar;
+                    o.ai += cup * up.ai;
+                }
+            } else {
+                o = cur;
+            }
+            o.ar *= a.ca;
+            o.ai *= a.ca;
+            o.br *= a.ca;
+            o.bi *= a.ca;
+            add_hop<R, DAG>(o, a.ch, g, a.hin + rb, sl);
+            const long long idx = sl + hs;
+            if constexpr (EPI == EPI_STORE) {
+                st(a.out + rb, idx, o);
+            } else if constexpr (EPI == EPI_NORM) {
+                st(a.out + rb, idx, o);
+                acc0 += n2(o);
+            } else if constexpr (EPI == EPI_CGUPD) {
+                const Sp<R> pv = ld(a.p + rb, idx);
+                Sp<R> xv = ld(a.x + rb, idx);
+                Sp<R> rv = ld(a.r + rb, idx);
+                xv.ar += alpha * pv.ar;
+                xv.ai += alpha * pv.ai;
+                xv.br += alpha * pv.br;
+                xv.bi += alpha * pv.bi;
+                rv.ar -= alpha * o.ar;
+                rv.ai -= alpha * o.ai;
+                rv.br -= alpha * o.br;
+                rv.bi -= alpha * o.bi;
+                st(a.x + rb, idx, xv);
+                st(a.r + rb, idx, rv);
+                acc0 += n2(rv);
+            } else {
+                if constexpr (sizeof(R) == 8) {
+                    const Sp<double> bv = ld(a.b + rb, idx);
+                    const Sp<double> rv{bv.ar - o.ar, bv.ai - o.ai, bv.br - o.br, bv.bi - o.bi};
+                    st(a.r64 + rb, idx, rv);
+                    if (a.p64) st(a.p64 + rb, idx, rv);
+                    if (a.r32 || a.p32) {
+                        const Sp<float> fv{(float)rv.ar, (float)rv.ai, (float)rv.br, (float)rv.bi};
+                        if (a.r32) st(a.r32 + rb, idx, fv);
+                        if (a.p32) st(a.p32 + rb, idx, fv);
+                    }
+                    acc0 += n2(rv);
+                    acc1 += n2(bv);
+                }
+            }
+            if constexpr (AOP) {
+                dn = cur;
+                cur = up;
+            } else if (s + 1 < s1) {
+                cur = ld(in, sl + V4h + hs);
+            }
+        }
+    }
+    if constexpr (EPI != EPI_STORE) {
+        block_sum2(acc0, acc1, sh);
+        if (threadIdx.x == 0) {
+            double* pp = a.part + ((size_t)rhs * nblk + bid) * 2;
+            pp[0] = acc0;
+            pp[1] = acc1;
+        }
+    }
+}
+
+// full lexicographic [rhs][s][t][x] <-> halves [rhs][s][t][xh] (FP64)
+__global__ void k_eo_split(const int Nt, const int Nx, const int Ls, const double* full, double* ev, double* od)
+{
+    const int V4 = Nt * Nx, V4h = V4 / 2;
+    const long long V5 = (long long)V4 * Ls;
+    const int rhs = blockIdx.y;
+    const double4* F = reinterpret_cast<const double4*>(full) + rhs * V5;
+    double4* E = reinterpret_cast<double4*>(ev) + rhs * (V5 / 2);
+    double4* O = reinterpret_cast<double4*>(od) + rhs * (V5 / 2);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < V5; i += (long long)gridDim.x * blockDim.x) {
+        const int s = (int)(i / V4), site = (int)(i - (long long)s * V4);
+        const int t = site / Nx, x = site - t * Nx;
+        const long long h = (long long)s * V4h + t * (Nx / 2) + (x >> 1);
+        if (((t + x) & 1) == 0) E[h] = F[i];
+        else O[h] = F[i];
+    }
+}
+
+__global__ void k_eo_merge(const int Nt, const int Nx, const int Ls, const double* ev, const double* od, double* full)
+{
+    const int V4 = Nt * Nx, V4h = V4 / 2;
+    const long long V5 = (long long)V4 * Ls;
+    const int rhs = blockIdx.y;
+    double4* F = reinterpret_cast<double4*>(full) + rhs * V5;
+    const double4* E = reinterpret_cast<const double4*>(ev) + rhs * (V5 / 2);
+    const double4* O = reinterpret_cast<const double4*>(od) + rhs * (V5 / 2);
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < V5; i += (long long)gridDim.x * blockDim.x) {
+        const int s = (int)(i / V4), site = (int)(i - (long long)s * V4);
+        const int t = site / Nx, x = site - t * Nx;
+        const long long h = (long long)s * V4h + t * (Nx / 2) + (x >> 1);
+        F[i] = (((t + x) & 1) == 0) ? E[h] : O[h];
+    }
+}
+
+template <typename R, bool DAG, int LS>
+void dispatch_a(bool ainv, const EoArgs<R>& a, unsigned grid, hipStream_t s)
+{
+    if (ainv)
+        k_eo_a<R, DAG, LS, true><<<grid, kBlockA, 0, s>>>(a);
+    else
+        k_eo_a<R, DAG, LS, false><<<grid, kBlockA, 0, s>>>(a);
+}
+
+template <typename R, bool DAG, bool AOP>
+void dispatch_b(int epi, const EoArgs<R>& a, unsigned grid, hipStream_t s)
+{
+    switch (epi) {
+        case EPI_STORE: k_eo_b<R, DAG, AOP, EPI_STORE><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_NORM: k_eo_b<R, DAG, AOP, EPI_NORM><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_CGUPD: k_eo_b<R, DAG, AOP, EPI_CGUPD><<<grid, kBlock, 0, s>>>(a); break;
+        default:
+            if constexpr (sizeof(R) == 8) k_eo_b<R, DAG, AOP, EPI_RESID><<<grid, kBlock, 0, s>>>(a);
+            break;
+    }
+}
+
+}  // namespace
+
+bool eo_supported(const Lat& L)
+{
+    if (!L.d5 || (L.Nt & 1) || (L.Nx & 1) || L.Nt < 2 || L.Nx < 2) return false;
+    return L.Ls == 4 || L.Ls == 8 || L.Ls == 12 || L.Ls == 16;
+}
+
+template <typename R> void launch_eo_a(bool dagger, bool ainv, const EoArgs<R>& a, int nrhs, hipStream_t s)
+{
+    const unsigned grid = (unsigned)(a.bxa * nrhs);
+    const int Ls = a.L.Ls;
+#define DWF_EO_A(LSV)                                      \
+    if (Ls == LSV) {                                       \
+        if (dagger)                                        \
+            dispatch_a<R, true, LSV>(ainv, a, grid, s);    \
+        else                                               \
+            dispatch_a<R, false, LSV>(ainv, a, grid, s);   \
+        return;                                            \
+    }
+    DWF_EO_A(4)
+    DWF_EO_A(8)
+    DWF_EO_A(12)
+    DWF_EO_A(16)
+#undef DWF_EO_A
+}
+template void launch_eo_a<float>(bool, bool, const EoArgs<float>&, int, hipStream_t);
+template void launch_eo_a<double>(bool, bool, const EoArgs<double>&, int, hipStream_t);
+
+template <typename R> void launch_eo_b(bool dagger, bool aop, int epi, const EoArgs<R>& a, int nrhs, hipStream_t s)
+{
+    const unsigned grid = (unsigned)(a.bx * a.chunks * nrhs);
+    if (dagger) {
+        if (aop)
+            dispatch_b<R, true, true>(epi, a, grid, s);
+        else
+            dispatch_b<R, true, false>(epi, a, grid, s);
+    } else {
+        if (aop)
+            dispatch_b<R, false, true>(epi, a, grid, s);
+        else
+            dispatch_b<R, false, false>(epi, a, grid, s);
+    }
+}
+template void launch_eo_b<float>(bool, bool, int, const EoArgs<float>&, int, hipStream_t);
+template void launch_eo_b<double>(bool, bool, int, const EoArgs<double>&, int, hipStream_t);
+
+void launch_eo_split(const Lat& L, const double* full, double* ev, double* od, int nrhs, hipStream_t s)
+{
+    const long long V5 = L.V5();
+    const unsigned bx = (unsigned)std::min<long long>((V5 + 255) / 256, 4096);
+    k_eo_split<<<dim3(bx, nrhs), 256, 0, s>>>(L.Nt, L.Nx, L.Ls, full, ev, od);
+}
+
+void launch_eo_merge(const Lat& L, const double* ev, const double* od, double* full, int nrhs, hipStream_t s)
+{
+    const long long V5 = L.V5();
+    const unsigned bx = (unsigned)std::min<long long>((V5 + 255) / 256, 4096);
+    k_eo_merge<<<dim3(bx, nrhs), 256, 0, s>>>(L.Nt, L.Nx, L.Ls, ev, od, full);
+}
+
+}  // namespace dwfk

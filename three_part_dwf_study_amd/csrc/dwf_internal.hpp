@@ -98,13 +98,11 @@ void launch_pack_links(const Lat& L, const double* theta, double2* l64, float2* 
 template <typename R>
 void launch_hop(bool dagger, int epi, const HopArgs<R>& a, int nrhs, hipStream_t s);
 template <typename R>
-void launch_cg_p(const Lat& L, const HopCfg& cfg, R* r, R* p, const double* partA,
-                 const double* partB, CgState* st, int slot, int mode, int nrhs, hipStream_t s);
-void launch_cg_start(const HopCfg& cfg, const double* partR, CgState* st, double tol, double delta,
-                     int nrhs, hipStream_t s);
-void launch_ru_commit(const HopCfg& cfg, const double* partR, CgState* st, int slot, int nrhs,
-                      hipStream_t s);
-void launch_true_resid(const HopCfg& cfg, const double* partR, CgState* st, int nrhs, hipStream_t s);
+void launch_cg_p(long long sites, int nblk, R* r, R* p, const double* partA, const double* partB, CgState* st,
+                 int slot, int mode, int nrhs, hipStream_t s);
+void launch_cg_start(int nblk, const double* partR, CgState* st, double tol, double delta, int nrhs, hipStream_t s);
+void launch_ru_commit(int nblk, const double* partR, CgState* st, int slot, int nrhs, hipStream_t s);
+void launch_true_resid(int nblk, const double* partR, CgState* st, int nrhs, hipStream_t s);
 void launch_accum_x(long long n4, double* x64, float* x32, const CgState* st, int slot, int nrhs,
                     hipStream_t s);
 template <typename R> void launch_zero(R* p, long long n, hipStream_t s);

@@ -230,18 +230,16 @@ extern "C" int dwf_measure(dwf_ctx* c, const dwf_measure_spec* spec, dwf_measure
             c->check(hipStreamSynchronize(s), "noise upload");
         }
         double* src = (double*)c->buf(B_AUX0, nv * nrhs * 8);
-        double* rhs = (double*)c->buf(B_AUX1, nv * nrhs * 8);
         double* X = (double*)c->buf(B_AUX2, nv * nrhs * 8);
         k_sources<<<dim3((V4 + 255) / 256, nrhs), 256, 0, s>>>(Nt, Nx, Ls, nwall, npoint, nnoise, dnoise, src);
         c->check_launch("sources");
-        dwf_apply_device_impl(c, DWF_OP_DDAG, DWF_PREC_FP64, src, rhs, nrhs);  // rhs = D^dag src
-        dwf_solve_device_impl(c, DWF_OP_DDAGD, spec->method, rhs, nullptr, spec->rtol, spec->maxiter, nrhs, X);
-        std::vector<CgState> st(c->st_host, c->st_host + nrhs);
-        for (int k = 0; k < nrhs; ++k) {
-            if (st[k].status == 4) throw DwfError(DWF_ERR_ZERO_DENOM, "CG encountered a zero denominator");
-            if (out->iterations) out->iterations[k] = st[k].iters;
-        }
-        auto converged = [&](int k) { return (st[k].flags[c->last_slot] & F_DONE) != 0; };
+        std::vector<dwf_solve_stats> stv(nrhs);
+        const int rc = dwf_solve_dirac_impl(c, spec->method, src, nullptr, spec->rtol, spec->maxiter, nrhs, X,
+                                            stv.data());
+        if (rc == DWF_ERR_ZERO_DENOM) throw DwfError(DWF_ERR_ZERO_DENOM, "CG encountered a zero denominator");
+        for (int k = 0; k < nrhs; ++k)
+            if (out->iterations) out->iterations[k] = stv[k].iterations + (spec->method == DWF_SOLVE_CG64 ? 0 : stv[k].fp64_iterations);
+        auto converged = [&](int k) { return stv[k].converged != 0; };
         for (int k = nwall; k < nwall + npoint; ++k)
             if (!converged(k)) throw DwfError(DWF_ERR_NOT_CONVERGED, "Momentum propagator CGNR failed to converge");
         for (int k = nwall + npoint; k < nrhs; ++k)

@@ -1,0 +1,686 @@
+// dwf_solver.hip -- conjugate-gradient drivers of libdwfhip.
+//
+// One driver (cg_run) runs CG on any operator A = Op2 Op1 whose first half yields
+// p^dag A p = |Op1 p|^2 (the normal operators D^dag D, D D^dag and the even-odd Schur
+// normal operator Dhat^dag Dhat).  Every vector and per-RHS scalar stays on the device;
+// per iteration three launches (Op1 + norm partials, Op2 fused with the x/r update and
+// |r|^2 partials, direction update) for the full operator, five for the Schur one.
+//   CG64:  FP64 throughout, the reference's recursive-residual stop
+//          (domain_wall/include/cg_solver.hpp:17-64);
+//   MIXED: FP32 iterations, FP64 reliable updates certifying on the FP64 true residual.
+// dwf_solve_dirac adds the propagator pattern x = D^-1 src of correlator.hpp (rhs = D^dag
+// src, CG on D^dag D), optionally through the even-odd Schur complement with defect
+// correction, certified on the reference's own system ||D^dag src - D^dag D x||.
+
+#include "../../include/dwf_hip.h"
+#include "dwf_context.hpp"
+#include "dwf_eo.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <string>
+#include <vector>
+
+using namespace dwfk;
+
+namespace {
+
+template <typename R> HopArgs<R> hop_base(const dwf_ctx* c, const HopCfg& cfg)
+{
+    HopArgs<R> a{};
+    a.L = c->L;
+    a.cfg = cfg;
+    a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
+    return a;
+}
+
+// ----------------------------------------------------------------- operators
+struct CgOperator {
+    long long sites = 0;  // spinor sites per right-hand side of x, b, r, p
+    int nblk = 0;         // block partials per right-hand side
+    virtual ~CgOperator() = default;
+    // t = Op1 p, partials |t|^2
+    virtual void first(bool f64, const void* p, void* t, double* part, const CgState* st, int slot, int skip) = 0;
+    // x += alpha p, r -= alpha Op2 t, partials |r|^2 (alpha = rr / sum partA)
+    virtual void second(bool f64, const void* t, void* x, void* r, const void* p, const double* partA, double* part,
+                        const CgState* st, int slot, int skip) = 0;
+    // r64 = b - A x in FP64 (copies into p64/r32/p32 if given), partials |r|^2, |b|^2
+    virtual void residual(const double* x, const double* b, double* r64, double* p64, float* r32, float* p32,
+                          double* part, const CgState* st, int slot, int skip) = 0;
+};
+
+// A = Op2 Op1 = D^dag D (first_dag = false) or D D^dag (first_dag = true).
+struct FullOp : CgOperator {
+    dwf_ctx* c;
+    HopCfg cfg;
+    bool first_dag;
+    int nrhs;
+    double* t64;
+    FullOp(dwf_ctx* ctx, bool fd, int n, double* tmp64) : c(ctx), first_dag(fd), nrhs(n), t64(tmp64)
+    {
+        cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap);
+        sites = c->L.V5();
+        nblk = cfg.nblk();
+    }
+    template <typename R> void first_t(const R* p, R* t, double* part, const CgState* st, int slot, int skip)
+    {
+        HopArgs<R> a = hop_base<R>(c, cfg);
+        a.in = p;
+        a.out = t;
+        a.part = part;
+        a.st = st;
+        a.slot = slot;
+        a.skip = skip;
+        launch_hop<R>(first_dag, EPI_NORM, a, nrhs, c->stream);
+    }
+    template <typename R>
+    void second_t(const R* t, R* x, R* r, const R* p, const double* partA, double* part, const CgState* st, int slot,
+                  int skip)
+    {
+        HopArgs<R> a = hop_base<R>(c, cfg);
+        a.in = t;
+        a.x = x;
+        a.r = r;
+        a.p = p;
+        a.partA = partA;
+        a.part = part;
+        a.st = st;
+        a.slot = slot;
+        a.skip = skip;
+        launch_hop<R>(!first_dag, EPI_CGUPD, a, nrhs, c->stream);
+    }
+    void first(bool f64, const void* p, void* t, double* part, const CgState* st, int slot, int skip) override
+    {
+        if (f64)
+            first_t<double>((const double*)p, (double*)t, part, st, slot, skip);
+        else
+            first_t<float>((const float*)p, (float*)t, part, st, slot, skip);
+    }
+    void second(bool f64, const void* t, void* x, void* r, const void* p, const double* partA, double* part,
+                const CgState* st, int slot, int skip) override
+    {
+        if (f64)
+            second_t<double>((const double*)t, (double*)x, (double*)r, (const double*)p, partA, part, st, slot, skip);
+        else
+            second_t<float>((const float*)t, (float*)x, (float*)r, (const float*)p, partA, part, st, slot, skip);
+    }
+    void residual(const double* x, const double* b, double* r64, double* p64, float* r32, float* p32, double* part,
+                  const CgState* st, int slot, int skip) override
+    {
+        HopArgs<double> a = hop_base<double>(c, cfg);
+        a.in = x;
+        a.out = t64;
+        a.st = st;
+        a.slot = slot;
+        a.skip = skip;
+        launch_hop<double>(first_dag, EPI_STORE, a, nrhs, c->stream);
+        HopArgs<double> e = hop_base<double>(c, cfg);
+        e.in = t64;
+        e.b = b;
+        e.r64 = r64;
+        e.p64 = p64;
+        e.r32 = r32;
+        e.p32 = p32;
+        e.part = part;
+        e.st = st;
+        e.slot = slot;
+        e.skip = skip;
+        launch_hop<double>(!first_dag, EPI_RESID, e, nrhs, c->stream);
+    }
+};
+
+// A = Dhat^dag Dhat on the odd half-lattice.
+struct SchurOp : CgOperator {
+    dwf_ctx* c;
+    int nrhs;
+    int bx, chunks, spt;
+    float* te32;
+    double* te64;
+    double* q64;
+    SchurOp(dwf_ctx* ctx, int n, float* t32, double* t64, double* q) : c(ctx), nrhs(n), te32(t32), te64(t64), q64(q)
+    {
+        const Lat& L = c->L;
+        const int V4h = L.V4 / 2;
+        bx = (V4h + kBlock - 1) / kBlock;
+        int ch = 1;
+        if (c->chunks_override > 0)
+            ch = c->chunks_override;
+        else
+            while (ch < L.Ls && (long long)bx * ch * nrhs < 2048) ch *= 2;
+        ch = std::min(ch, L.Ls);
+        spt = (L.Ls + ch - 1) / ch;
+        chunks = (L.Ls + spt - 1) / spt;
+        sites = (long long)V4h * L.Ls;
+        nblk = bx * chunks;
+    }
+    template <typename R> EoArgs<R> base(int tpar) const
+    {
+        EoArgs<R> a{};
+        a.L = c->L;
+        a.Nxh = c->L.Nx / 2;
+        a.V4h = c->L.V4 / 2;
+        a.bx = bx;
+        a.bxa = (a.V4h + kBlockA - 1) / kBlockA;
+        a.chunks = chunks;
+        a.spt = spt;
+        a.remap = c->xcd_remap;
+        a.tpar = tpar;
+        a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
+        return a;
+    }
+    // out_o = [Dhat | Dhat^dag] in_o with epilogue `epi` (args in the returned struct)
+    template <typename R> EoArgs<R> apply_args(bool dag, const R* in, R* te, const CgState* st, int slot, int skip)
+    {
+        EoArgs<R> a = base<R>(0);
+        a.hin = in;
+        a.ch = (R)1;
+        a.out = te;
+        a.st = st;
+        a.slot = slot;
+        a.skip = skip;
+        launch_eo_a<R>(dag, true, a, nrhs, c->stream);  // te = A^-1 H_eo in   (A^-dag H^dag when dag)
+        EoArgs<R> b = base<R>(1);
+        b.in = in;
+        b.ca = (R)1;
+        b.hin = te;
+        b.ch = (R)-0.25;
+        b.st = st;
+        b.slot = slot;
+        b.skip = skip;
+        return b;
+    }
+    void first(bool f64, const void* p, void* t, double* part, const CgState* st, int slot, int skip) override
+    {
+        if (f64) {
+            EoArgs<double> b = apply_args<double>(false, (const double*)p, te64, st, slot, skip);
+            b.out = (double*)t;
+            b.part = part;
+            launch_eo_b<double>(false, true, EPI_NORM, b, nrhs, c->stream);
+        } else {
+            EoArgs<float> b = apply_args<float>(false, (const float*)p, te32, st, slot, skip);
+            b.out = (float*)t;
+            b.part = part;
+            launch_eo_b<float>(false, true, EPI_NORM, b, nrhs, c->stream);
+        }
+    }
+    void second(bool f64, const void* t, void* x, void* r, const void* p, const double* partA, double* part,
+                const CgState* st, int slot, int skip) override
+    {
+        if (f64) {
+            EoArgs<double> b = apply_args<double>(true, (const double*)t, te64, st, slot, skip);
+            b.x = (double*)x;
+            b.r = (double*)r;
+            b.p = (const double*)p;
+            b.partA = partA;
+            b.part = part;
+            launch_eo_b<double>(true, true, EPI_CGUPD, b, nrhs, c->stream);
+        } else {
+            EoArgs<float> b = apply_args<float>(true, (const float*)t, te32, st, slot, skip);
+            b.x = (float*)x;
+            b.r = (float*)r;
+            b.p = (const float*)p;
+            b.partA = partA;
+            b.part = part;
+            launch_eo_b<float>(true, true, EPI_CGUPD, b, nrhs, c->stream);
+        }
+    }
+    void residual(const double* x, const double* b64, double* r64, double* p64, float* r32, float* p32, double* part,
+                  const CgState* st, int slot, int skip) override
+    {
+        EoArgs<double> b = apply_args<double>(false, x, te64, st, slot, skip);
+        b.out = q64;
+        launch_eo_b<double>(false, true, EPI_STORE, b, nrhs, c->stream);
+        EoArgs<double> e = apply_args<double>(true, q64, te64, st, slot, skip);
+        e.b = b64;
+        e.r64 = r64;
+        e.p64 = p64;
+        e.r32 = r32;
+        e.p32 = p32;
+        e.part = part;
+        launch_eo_b<double>(true, true, EPI_RESID, e, nrhs, c->stream);
+    }
+};
+
+// ------------------------------------------------------------------- driver
+void read_states(dwf_ctx* c, const CgState* st, int nrhs)
+{
+    CgState* h = c->host_states(nrhs);
+    c->check(hipMemcpyAsync(h, st, sizeof(CgState) * nrhs, hipMemcpyDeviceToHost, c->stream), "state readback");
+    c->check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+bool all_done(const CgState* h, int slot, int nrhs)
+{
+    for (int i = 0; i < nrhs; ++i)
+        if (!(h[i].flags[slot] & F_DONE)) return false;
+    return true;
+}
+
+__global__ void k_mark_done(CgState* st, const CgState* pre, int nrhs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrhs && (pre[i].flags[0] & F_DONE)) {
+        st[i].flags[0] |= F_DONE;
+        st[i].flags[1] |= F_DONE;
+    }
+}
+
+__global__ void k_axpy_masked(double* x, const double* d, long long n_per, const CgState* pre)
+{
+    const int rhs = blockIdx.y;
+    if (pre && (pre[rhs].flags[0] & F_DONE)) return;
+    double* xr = x + rhs * n_per;
+    const double* dr = d + rhs * n_per;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_per; i += (long long)gridDim.x * blockDim.x)
+        xr[i] += dr[i];
+}
+
+__global__ void k_sub_into(const double* a, const double* b, double* out, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        out[i] = a[i] - b[i];
+}
+
+// CG on op.  Leaves the per-RHS states in c->st_host (slot c->last_slot).  `pre`
+// (nullable) marks right-hand sides that are already finished.
+void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const double* x0, double rtol, int maxiter,
+            int nrhs, double* x, const CgState* pre = nullptr)
+{
+    const long long n4 = op.sites * 4;
+    const size_t nv = (size_t)n4 * nrhs;
+    const size_t npart = (size_t)nrhs * op.nblk * 2;
+    double* r64 = (double*)c->buf(B_R64, nv * 8);
+    double* partA = (double*)c->buf(B_PA, npart * 8);
+    double* partB = (double*)c->buf(B_PB, npart * 8);
+    double* partR = (double*)c->buf(B_PR, npart * 8);
+    CgState* st = (CgState*)c->buf(B_STATE, sizeof(CgState) * nrhs);
+    double *p64 = nullptr, *t64 = nullptr;
+    float *r32 = nullptr, *t32 = nullptr, *p32 = nullptr, *x32 = nullptr;
+    if (mixed) {
+        r32 = (float*)c->buf(B_R32, nv * 4);
+        t32 = (float*)c->buf(B_T32, nv * 4);
+        p32 = (float*)c->buf(B_P32, nv * 4);
+        x32 = (float*)c->buf(B_X32, nv * 4);
+    } else {
+        p64 = (double*)c->buf(B_P64, nv * 8);
+        t64 = (double*)c->buf(B_TQ64, nv * 8);
+    }
+    hipStream_t s = c->stream;
+    if (x0) {
+        if (x0 != x) c->check(hipMemcpyAsync(x, x0, nv * 8, hipMemcpyDeviceToDevice, s), "copy x0");
+    } else {
+        launch_zero<double>(x, (long long)nv, s);
+    }
+    // r = b - A x0 (the reference applies A even to a zero guess, cg_solver.hpp:24-27)
+    op.residual(x, b, r64, mixed ? nullptr : p64, mixed ? r32 : nullptr, mixed ? p32 : nullptr, partR, nullptr, 0, 0);
+    const double delta = c->reliable_delta;
+    launch_cg_start(op.nblk, partR, st, rtol, delta, nrhs, s);
+    if (pre) k_mark_done<<<(nrhs + 63) / 64, 64, 0, s>>>(st, pre, nrhs);
+    if (mixed) launch_zero<float>(x32, (long long)nv, s);
+    c->check_launch("solver start");
+
+    int it = 0;
+    int chunk = mixed ? 16 : 32;
+    read_states(c, st, nrhs);
+    std::vector<double> last_true(nrhs);
+    std::vector<int> last_iters(nrhs, 0);
+    for (int i = 0; i < nrhs; ++i) last_true[i] = c->st_host[i].true_relres;
+    const int mode = mixed ? MODE_MIXED : MODE_CG64;
+    while (!all_done(c->st_host, it & 1, nrhs)) {
+        const int n = std::min(chunk, maxiter - it);
+        if (n <= 0) break;
+        for (int j = 0; j < n; ++j, ++it) {
+            const int slot = it & 1, skip = F_DONE | F_FROZEN;
+            if (mixed) {
+                op.first(false, p32, t32, partA, st, slot, skip);
+                op.second(false, t32, x32, r32, p32, partA, partB, st, slot, skip);
+                launch_cg_p<float>(op.sites, op.nblk, r32, p32, partA, partB, st, slot, mode, nrhs, s);
+            } else {
+                op.first(true, p64, t64, partA, st, slot, skip);
+                op.second(true, t64, x, r64, p64, partA, partB, st, slot, skip);
+                launch_cg_p<double>(op.sites, op.nblk, r64, p64, partA, partB, st, slot, mode, nrhs, s);
+            }
+        }
+        c->check_launch("CG iteration");
+        if (mixed) {
+            // Reliable update: x64 += x32, FP64 true residual, certify or restart the cycle.
+            const int slot = it & 1;
+            launch_accum_x(n4, x, x32, st, slot, nrhs, s);
+            op.residual(x, b, r64, nullptr, r32, nullptr, partR, st, slot, F_DONE);
+            launch_ru_commit(op.nblk, partR, st, slot, nrhs, s);
+            c->check_launch("reliable update");
+            read_states(c, st, nrhs);
+            // Next cycle: iterations the fastest active RHS needs to shrink its residual
+            // by delta (or to reach the tolerance), from the last cycle's rate.
+            double best = 1e30;
+            for (int i = 0; i < nrhs; ++i) {
+                const CgState& S = c->st_host[i];
+                if (S.flags[slot] & F_DONE) continue;
+                const int di = S.iters - last_iters[i];
+                if (di > 0 && S.true_relres > 0.0 && last_true[i] > 0.0 && S.true_relres < last_true[i]) {
+                    const double rate = std::log(S.true_relres / last_true[i]) / di;
+                    const double need_delta = std::log(delta) / rate;
+                    const double need_tol = std::log(rtol / S.true_relres) / rate + 2.0;
+                    best = std::min(best, std::min(need_delta, need_tol));
+                }
+                last_true[i] = S.true_relres;
+                last_iters[i] = S.iters;
+            }
+            if (best < 1e29) chunk = std::max(8, std::min(1024, (int)std::ceil(best)));
+        } else {
+            read_states(c, st, nrhs);
+            chunk = 32;
+        }
+    }
+    // FP64 true residual of the final x
+    op.residual(x, b, r64, nullptr, nullptr, nullptr, partR, nullptr, 0, 0);
+    launch_true_resid(op.nblk, partR, st, nrhs, s);
+    c->check_launch("true residual");
+    read_states(c, st, nrhs);
+    c->last_slot = it & 1;
+}
+
+void to_stats(const CgState& S, int slot, int method, dwf_solve_stats& s)
+{
+    s = dwf_solve_stats{};
+    s.iterations = S.iters;
+    s.reliable_updates = S.reliable;
+    s.relres = S.relres;
+    s.true_relres = S.true_relres;
+    s.fp64_iterations = method == DWF_SOLVE_CG64 ? S.iters : 0;
+    s.converged = ((S.flags[slot] & F_DONE) && S.status == 0) ? 1 : 0;
+}
+
+int collect(dwf_ctx* c, int method, int nrhs, dwf_solve_stats* stats)
+{
+    int rc = DWF_OK;
+    for (int i = 0; i < nrhs; ++i) {
+        dwf_solve_stats s;
+        to_stats(c->st_host[i], c->last_slot, method, s);
+        if (c->st_host[i].status == 4) {
+            dwf_set_error("CG encountered a zero denominator");
+            rc = DWF_ERR_ZERO_DENOM;
+        } else if (!s.converged && rc == DWF_OK) {
+            dwf_set_error("CG did not converge within maxiter");
+            rc = DWF_ERR_NOT_CONVERGED;
+        }
+        if (stats) stats[i] = s;
+    }
+    return rc;
+}
+
+}  // namespace
+
+// Normal-equation solve A x = b with A = D^dag D or D D^dag (device buffers).
+int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0, double rtol,
+                          int maxiter, int nrhs, double* x)
+{
+    const size_t nv = (size_t)c->L.V5() * 4 * nrhs;
+    FullOp op(c, normal == DWF_OP_DDDAG, nrhs, (double*)c->buf(B_T64, nv * 8));
+    cg_run(c, op, method == DWF_SOLVE_MIXED, b, x0, rtol, maxiter, nrhs, x);
+    return DWF_OK;
+}
+
+namespace {
+
+int solve_normal(dwf_ctx* c, int normal, int method, const double* b, const double* x0, double rtol, int maxiter,
+                 int nrhs, double* x, dwf_solve_stats* stats)
+{
+    if (!c || !b || !x || nrhs < 1 || maxiter < 0 || !(rtol >= 0.0) ||
+        (normal != DWF_OP_DDDAG && normal != DWF_OP_DDAGD) || (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED))
+        throw DwfError(DWF_ERR_ARG, "dwf_solve: bad argument");
+    if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve: no gauge field set");
+    dwf_solve_device_impl(c, normal, method, b, x0, rtol, maxiter, nrhs, x);
+    int rc = collect(c, method, nrhs, stats);
+    if (method == DWF_SOLVE_MIXED && rc == DWF_ERR_NOT_CONVERGED) {
+        // The FP32 cycle stalled: plain FP64 CG from the current iterate decides.
+        std::vector<dwf_solve_stats> first(nrhs);
+        collect(c, method, nrhs, first.data());
+        dwf_solve_device_impl(c, normal, DWF_SOLVE_CG64, b, x, rtol, maxiter, nrhs, x);
+        rc = collect(c, DWF_SOLVE_CG64, nrhs, stats);
+        if (stats)
+            for (int i = 0; i < nrhs; ++i) {
+                stats[i].fp64_iterations = stats[i].iterations;
+                stats[i].iterations = first[i].iterations;
+                stats[i].reliable_updates = first[i].reliable_updates;
+            }
+        if (rc == DWF_ERR_NOT_CONVERGED) rc = DWF_ERR_CERTIFY;
+    }
+    return rc;
+}
+
+// x = D^-1 src through the even-odd Schur complement with defect correction; certified
+// on ||D^dag src - D^dag D x|| / ||D^dag src|| <= rtol in FP64.
+int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol, int maxiter, int nrhs, double* x,
+                   dwf_solve_stats* stats)
+{
+    const Lat& L = c->L;
+    hipStream_t s = c->stream;
+    const long long nfull = L.V5() * 4;           // doubles per full spinor
+    const long long nhalf = nfull / 2;
+    const size_t nvf = (size_t)nfull * nrhs, nvh = (size_t)nhalf * nrhs;
+    double* B = (double*)c->buf(B_EO_B, nvf * 8);        // D^dag src (certification rhs)
+    double* rsrc = (double*)c->buf(B_EO_RSRC, nvf * 8);  // current Dirac residual src - D x
+    double* be = (double*)c->buf(B_EO_BE, nvh * 8);
+    double* bo = (double*)c->buf(B_EO_BO, nvh * 8);
+    double* ue = (double*)c->buf(B_EO_UE, nvh * 8);
+    double* bh = (double*)c->buf(B_EO_BH, nvh * 8);
+    double* Bh = (double*)c->buf(B_EO_BHAT, nvh * 8);
+    double* xo = (double*)c->buf(B_EO_XO, nvh * 8);
+    double* xe = (double*)c->buf(B_EO_XE, nvh * 8);
+    double* q64 = (double*)c->buf(B_EO_Q, nvh * 8);
+    float* te32 = (float*)c->buf(B_EO_TE32, nvh * 4);
+    double* delta = (double*)c->buf(B_EO_DELTA, nvf * 8);
+    double* t64 = (double*)c->buf(B_T64, nvf * 8);
+    CgState* cert = (CgState*)c->buf(B_STATE2, sizeof(CgState) * nrhs);
+
+    FullOp full(c, false, nrhs, t64);
+    double* partC = (double*)c->buf(B_PR2, (size_t)nrhs * full.nblk * 2 * 8);
+    SchurOp schur(c, nrhs, te32, ue, q64);
+    // certification rhs B = D^dag src
+    {
+        HopArgs<double> a = hop_base<double>(c, full.cfg);
+        a.in = src;
+        a.out = B;
+        launch_hop<double>(true, EPI_STORE, a, nrhs, s);
+    }
+    if (x0) {
+        if (x0 != x) c->check(hipMemcpyAsync(x, x0, nvf * 8, hipMemcpyDeviceToDevice, s), "copy x0");
+    } else {
+        launch_zero<double>(x, (long long)nvf, s);
+    }
+    std::vector<int> iters(nrhs, 0), reliable(nrhs, 0);
+    std::vector<double> srelres(nrhs, 0.0);
+    bool have_cert = false;
+    double worst = 1.0;
+    const int passes = 4;
+    int used = 0;
+    for (int pass = 0; pass < passes; ++pass) {
+        // Dirac residual of the current iterate
+        const double* r_in = src;
+        if (pass > 0 || x0) {
+            HopArgs<double> a = hop_base<double>(c, full.cfg);
+            a.in = x;
+            a.out = t64;
+            launch_hop<double>(false, EPI_STORE, a, nrhs, s);
+            k_sub_into<<<2048, 256, 0, s>>>(src, t64, rsrc, (long long)nvf);
+            r_in = rsrc;
+        }
+        // Schur right-hand side: bh = b_o + 1/2 H_oe A^-1 b_e, Bh = Dhat^dag bh
+        launch_eo_split(L, r_in, be, bo, nrhs, s);
+        {
+            EoArgs<double> a = schur.base<double>(0);
+            a.b = be;
+            a.cb = 1.0;
+            a.out = ue;
+            launch_eo_a<double>(false, true, a, nrhs, s);
+            EoArgs<double> b = schur.base<double>(1);
+            b.in = bo;
+            b.ca = 1.0;
+            b.hin = ue;
+            b.ch = 0.5;
+            b.out = bh;
+            launch_eo_b<double>(false, false, EPI_STORE, b, nrhs, s);
+            EoArgs<double> d = schur.apply_args<double>(true, bh, ue, nullptr, 0, 0);
+            d.out = Bh;
+            launch_eo_b<double>(true, true, EPI_STORE, d, nrhs, s);
+        }
+        c->check_launch("Schur right-hand side");
+        const double rtol_pass = std::min(0.5, 0.25 * rtol / std::max(worst, 1e-300));
+        cg_run(c, schur, true, Bh, nullptr, rtol_pass, maxiter, nrhs, xo, have_cert ? cert : nullptr);
+        used = pass + 1;
+        for (int i = 0; i < nrhs; ++i) {
+            iters[i] += c->st_host[i].iters;
+            reliable[i] += c->st_host[i].reliable;
+            srelres[i] = c->st_host[i].true_relres;
+        }
+        // x_e = A^-1 (b_e + 1/2 H_eo x_o), assemble the correction and add it
+        {
+            EoArgs<double> a = schur.base<double>(0);
+            a.b = be;
+            a.cb = 1.0;
+            a.hin = xo;
+            a.ch = 0.5;
+            a.out = xe;
+            launch_eo_a<double>(false, true, a, nrhs, s);
+        }
+        launch_eo_merge(L, xe, xo, delta, nrhs, s);
+        k_axpy_masked<<<dim3(1024, nrhs), 256, 0, s>>>(x, delta, nfull, have_cert ? cert : nullptr);
+        // certify on the reference's system (delta is free again)
+        full.residual(x, B, delta, nullptr, nullptr, nullptr, partC, nullptr, 0, 0);
+        launch_cg_start(full.nblk, partC, cert, rtol, c->reliable_delta, nrhs, s);
+        c->check_launch("certification");
+        have_cert = true;
+        read_states(c, cert, nrhs);
+        worst = 0.0;
+        bool all = true;
+        for (int i = 0; i < nrhs; ++i) {
+            if (!(c->st_host[i].flags[0] & F_DONE)) {
+                all = false;
+                worst = std::max(worst, c->st_host[i].relres);
+            }
+        }
+        if (all) break;
+    }
+    int rc = DWF_OK;
+    for (int i = 0; i < nrhs; ++i) {
+        const CgState& S = c->st_host[i];
+        dwf_solve_stats st{};
+        st.iterations = iters[i];
+        st.passes = used;
+        st.reliable_updates = reliable[i];
+        st.relres = srelres[i];
+        st.true_relres = S.relres;
+        st.converged = (S.flags[0] & F_DONE) ? 1 : 0;
+        if (!st.converged && rc == DWF_OK) {
+            dwf_set_error("even-odd solve did not certify within maxiter");
+            rc = DWF_ERR_NOT_CONVERGED;
+        }
+        if (stats) stats[i] = st;
+    }
+    return rc;
+}
+
+int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter, int nrhs,
+                double* x, dwf_solve_stats* stats)
+{
+    if (!c || !src || !x || nrhs < 1 || maxiter < 0 || !(rtol >= 0.0) ||
+        (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_EO_MIXED))
+        throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: bad argument");
+    if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: no gauge field set");
+    if (method == DWF_SOLVE_EO_MIXED && eo_supported(c->L)) {
+        const int rc = solve_dirac_eo(c, src, x0, rtol, maxiter, nrhs, x, stats);
+        if (rc != DWF_ERR_NOT_CONVERGED) return rc;
+        method = DWF_SOLVE_MIXED;  // fall back to the full-lattice solver from this iterate
+        x0 = x;
+    } else if (method == DWF_SOLVE_EO_MIXED) {
+        method = DWF_SOLVE_MIXED;  // odd extents or an Ls without an even-odd kernel
+    }
+    // the reference's pattern: rhs = D^dag src, CG on D^dag D
+    const size_t nv = (size_t)c->L.V5() * 4 * nrhs;
+    double* B = (double*)c->buf(B_EO_B, nv * 8);
+    {
+        HopArgs<double> a = hop_base<double>(c, hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap));
+        a.in = src;
+        a.out = B;
+        launch_hop<double>(true, EPI_STORE, a, nrhs, c->stream);
+    }
+    return solve_normal(c, DWF_OP_DDAGD, method, B, x0, rtol, maxiter, nrhs, x, stats);
+}
+
+int guard(const std::function<int()>& f)
+{
+    try {
+        return f();
+    } catch (const DwfError& e) {
+        dwf_set_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        dwf_set_error(e.what());
+        return DWF_ERR_ARG;
+    }
+}
+
+}  // namespace
+
+int dwf_solve_dirac_impl(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter,
+                         int nrhs, double* x, dwf_solve_stats* stats)
+{
+    return solve_dirac(c, method, src, x0, rtol, maxiter, nrhs, x, stats);
+}
+
+extern "C" {
+
+int dwf_solve_device(dwf_ctx* c, int normal_op, int method, const double* b, const double* x0, double rtol,
+                     int maxiter, int nrhs, double* x, dwf_solve_stats* stats)
+{
+    return guard([&]() -> int { return solve_normal(c, normal_op, method, b, x0, rtol, maxiter, nrhs, x, stats); });
+}
+
+int dwf_solve(dwf_ctx* c, int normal_op, int method, const double* b, const double* x0, double rtol, int maxiter,
+              int nrhs, double* x, dwf_solve_stats* stats)
+{
+    return guard([&]() -> int {
+        if (!c || !b || !x || nrhs < 1) throw DwfError(DWF_ERR_ARG, "dwf_solve: bad argument");
+        const size_t n = (size_t)c->L.V5() * 4 * nrhs;
+        double* db = (double*)c->buf(B_HOST_IN, n * 8);
+        double* dx = (double*)c->buf(B_HOST_OUT, n * 8);
+        c->check(hipMemcpyAsync(db, b, n * 8, hipMemcpyHostToDevice, c->stream), "solve upload b");
+        if (x0) c->check(hipMemcpyAsync(dx, x0, n * 8, hipMemcpyHostToDevice, c->stream), "solve upload x0");
+        const int rc = solve_normal(c, normal_op, method, db, x0 ? dx : nullptr, rtol, maxiter, nrhs, dx, stats);
+        if (rc == DWF_OK || rc == DWF_ERR_NOT_CONVERGED || rc == DWF_ERR_CERTIFY) {
+            c->check(hipMemcpyAsync(x, dx, n * 8, hipMemcpyDeviceToHost, c->stream), "solve download");
+            c->check(hipStreamSynchronize(c->stream), "solve sync");
+        }
+        return rc;
+    });
+}
+
+int dwf_solve_dirac_device(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter,
+                           int nrhs, double* x, dwf_solve_stats* stats)
+{
+    return guard([&]() -> int { return solve_dirac(c, method, src, x0, rtol, maxiter, nrhs, x, stats); });
+}
+
+int dwf_solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter, int nrhs,
+                    double* x, dwf_solve_stats* stats)
+{
+    return guard([&]() -> int {
+        if (!c || !src || !x || nrhs < 1)
+            throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: bad argument");
+        const size_t n = (size_t)c->L.V5() * 4 * nrhs;
+        double* db = (double*)c->buf(B_HOST_IN, n * 8);
+        double* dx = (double*)c->buf(B_HOST_OUT, n * 8);
+        c->check(hipMemcpyAsync(db, src, n * 8, hipMemcpyHostToDevice, c->stream), "solve upload src");
+        if (x0) c->check(hipMemcpyAsync(dx, x0, n * 8, hipMemcpyHostToDevice, c->stream), "solve upload x0");
+        const int rc = solve_dirac(c, method, db, x0 ? dx : nullptr, rtol, maxiter, nrhs, dx, stats);
+        if (rc == DWF_OK || rc == DWF_ERR_NOT_CONVERGED || rc == DWF_ERR_CERTIFY) {
+            c->check(hipMemcpyAsync(x, dx, n * 8, hipMemcpyDeviceToHost, c->stream), "solve download");
+            c->check(hipStreamSynchronize(c->stream), "solve sync");
+        }
+        return rc;
+    });
+}
+
+}  // extern "C"
