@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--dslash-rhs", type=int, default=128)
     ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the roofline launches (batched FP32 D apply and the copy kernel), for counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent "
@@ -126,6 +128,17 @@ def main():
 
     rng = np.random.default_rng(1234)
     theta = rng.normal(0.0, 1.0 / np.sqrt(BETA), size=2 * NT * NX)
+    if args.kernels_only:
+        ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
+        ctx.set_gauge(theta)
+        ms32, gbs32 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, args.dslash_rhs, 5)
+        cb = dslash_bytes(4, args.dslash_rhs) // 2 // 16 * 16
+        msc, gbsc = time_copy(ctx, capi, cb, 5)
+        print(json.dumps({"dslash_fp32_us": ms32 * 1e3, "dslash_fp32_gbs": gbs32,
+                          "dslash_bytes": dslash_bytes(4, args.dslash_rhs), "copy_us": msc * 1e3,
+                          "copy_gbs": gbsc, "copy_bytes_each_way": cb}), flush=True)
+        ctx.close()
+        return
     nrhs = args.nrhs
     src = boundary_sources(nrhs, 918273)
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
