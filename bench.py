@@ -88,6 +88,19 @@ def time_copy(ctx, capi, nbytes, reps):
     return ms, 2 * nbytes / (ms * 1e-3) / 1e9
 
 
+def measured_traffic():
+    """HBM bytes per launch of the roofline kernel from the committed counter summary
+    (two rocprofv3 --pmc runs calibrated on the copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "k_hop_fp32_dwf96_batch128_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return int(line.split("=")[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(theta, b1, judge_iters, sample_iters):
     import oracle
 
@@ -227,7 +240,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(gbs32 / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic() if args.dslash_rhs == 128 else None,
             "kernel": "k_hop<float,D,DWF,EPI_STORE>",
             "avg_launch_us": round(ms32 * 1e3, 2),
             "bytes_per_launch": dslash_bytes(4, args.dslash_rhs),
