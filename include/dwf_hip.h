@@ -97,8 +97,13 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * autotuner or by hand).  DWF_TUNE_SCHUNKS: s-chunks per 4D site of the stencils
  * (0 = heuristic, 1..Ls); DWF_TUNE_XCD_REMAP: 0/1 XCD-aware block order;
  * DWF_TUNE_RELIABLE_DELTA: residual drop (0 < delta < 1) that triggers a reliable update
- * of the mixed solver. */
-enum dwf_tune_param { DWF_TUNE_SCHUNKS = 0, DWF_TUNE_XCD_REMAP = 1, DWF_TUNE_RELIABLE_DELTA = 2 };
+ * of the mixed solver; DWF_TUNE_PREFETCH: 0/1 software-pipelined stencil slice loop. */
+enum dwf_tune_param {
+    DWF_TUNE_SCHUNKS = 0,
+    DWF_TUNE_XCD_REMAP = 1,
+    DWF_TUNE_RELIABLE_DELTA = 2,
+    DWF_TUNE_PREFETCH = 3
+};
 int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 
 /* Upload gauge angles theta[2][Nt][Nx] (host).  The library keeps them on the device in

@@ -41,6 +41,7 @@ DWF_SOLVE_EO_MIXED = 2
 DWF_TUNE_SCHUNKS = 0
 DWF_TUNE_XCD_REMAP = 1
 DWF_TUNE_RELIABLE_DELTA = 2
+DWF_TUNE_PREFETCH = 3
 
 
 class DwfError(RuntimeError):

@@ -103,7 +103,7 @@ __device__ __forceinline__ double sum_partials(const double* p, int n, int which
 }
 
 // ------------------------------------------------------------------ stencil kernel
-template <typename R, bool DAG, bool D5, int EPI>
+template <typename R, bool DAG, bool D5, int EPI, bool PF>
 __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
 {
     // 1D grid over (tile, s-chunk, rhs), optionally in XCD-aware order
@@ -157,18 +157,42 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
         constexpr R sg = DAG ? (R)-1 : (R)1;
 
         const R* in = a.in + rb;
+        // PF (software pipeline): the loads of slice s+1 (four neighbours) and of the
+        // centre of slice s+2 are issued before slice s is computed, so every wave keeps
+        // two slices of memory traffic in flight, at the price of more registers.
+        const int Ls = L.Ls;
+        auto wrap = [Ls](int s) { return s >= Ls ? s - Ls : (s < 0 ? s + Ls : s); };
         Sp<R> cur = ldsp(in, (long long)s0 * V4 + site);
-        Sp<R> dn{0, 0, 0, 0};
-        if constexpr (D5) dn = ldsp(in, (long long)((s0 == 0) ? L.Ls - 1 : s0 - 1) * V4 + site);
+        Sp<R> dn{0, 0, 0, 0}, up{0, 0, 0, 0};
+        if constexpr (D5) dn = ldsp(in, (long long)wrap(s0 - 1) * V4 + site);
+        if (D5 || s0 + 1 < s1) up = ldsp(in, (long long)wrap(s0 + 1) * V4 + site);
+        long long sl0 = (long long)s0 * V4;
+        Sp<R> vtp{0, 0, 0, 0}, vtm{0, 0, 0, 0}, vxp{0, 0, 0, 0}, vxm{0, 0, 0, 0};
+        if constexpr (PF) {
+            vtp = ldsp(in, sl0 + ntp);
+            vtm = ldsp(in, sl0 + ntm);
+            vxp = ldsp(in, sl0 + nxp);
+            vxm = ldsp(in, sl0 + nxm);
+        }
 
         for (int s = s0; s < s1; ++s) {
             const long long sl = (long long)s * V4;
-            Sp<R> up{0, 0, 0, 0};
-            if constexpr (D5) up = ldsp(in, (long long)((s + 1 == L.Ls) ? 0 : s + 1) * V4 + site);
-            const Sp<R> vtp = ldsp(in, sl + ntp);
-            const Sp<R> vtm = ldsp(in, sl + ntm);
-            const Sp<R> vxp = ldsp(in, sl + nxp);
-            const Sp<R> vxm = ldsp(in, sl + nxm);
+            const bool more = s + 1 < s1;
+            Sp<R> up2{0, 0, 0, 0}, ntp_{0, 0, 0, 0}, ntm_{0, 0, 0, 0}, nxp_{0, 0, 0, 0}, nxm_{0, 0, 0, 0};
+            if constexpr (!PF) {
+                vtp = ldsp(in, sl + ntp);
+                vtm = ldsp(in, sl + ntm);
+                vxp = ldsp(in, sl + nxp);
+                vxm = ldsp(in, sl + nxm);
+                if (D5 || s + 2 < s1) up2 = ldsp(in, (long long)wrap(s + 2) * V4 + site);
+            } else if (more) {
+                const long long sn = sl + V4;
+                ntp_ = ldsp(in, sn + ntp);
+                ntm_ = ldsp(in, sn + ntm);
+                nxp_ = ldsp(in, sn + nxp);
+                nxm_ = ldsp(in, sn + nxm);
+                if (D5 || s + 2 < s1) up2 = ldsp(in, (long long)wrap(s + 2) * V4 + site);
+            }
 
             Sp<R> o{diag * cur.ar, diag * cur.ai, diag * cur.br, diag * cur.bi};
             {   // forward t: U_t(x) (1 - g sigma1) psi(x+t): upper a = v0 - g v1, lower -g a
@@ -258,11 +282,14 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
                 }
             }
 
-            if constexpr (D5) {
-                dn = cur;
-                cur = up;
-            } else if (s + 1 < s1) {
-                cur = ldsp(in, sl + V4 + site);
+            dn = cur;
+            cur = up;
+            up = up2;
+            if constexpr (PF) {
+                vtp = ntp_;
+                vtm = ntm_;
+                vxp = nxp_;
+                vxm = nxm_;
             }
         }
     }
@@ -450,25 +477,35 @@ __global__ void k_pack_links(const int n, const double* theta, double2* l64, flo
     l32[i] = make_float2((float)c, (float)s);
 }
 
-template <typename R, bool DAG, bool D5>
-void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
+template <typename R, bool DAG, bool D5, bool PF>
+void dispatch_epi_pf(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 {
     switch (epi) {
-        case EPI_STORE: k_hop<R, DAG, D5, EPI_STORE><<<grid, kBlock, 0, s>>>(a); break;
-        case EPI_NORM: k_hop<R, DAG, D5, EPI_NORM><<<grid, kBlock, 0, s>>>(a); break;
-        case EPI_CGUPD: k_hop<R, DAG, D5, EPI_CGUPD><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_STORE: k_hop<R, DAG, D5, EPI_STORE, PF><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_NORM: k_hop<R, DAG, D5, EPI_NORM, PF><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_CGUPD: k_hop<R, DAG, D5, EPI_CGUPD, PF><<<grid, kBlock, 0, s>>>(a); break;
         default:
-            if constexpr (sizeof(R) == 8) k_hop<R, DAG, D5, EPI_RESID><<<grid, kBlock, 0, s>>>(a);
+            if constexpr (sizeof(R) == 8) k_hop<R, DAG, D5, EPI_RESID, PF><<<grid, kBlock, 0, s>>>(a);
             break;
     }
 }
 
+template <typename R, bool DAG, bool D5>
+void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
+{
+    if (a.cfg.prefetch)
+        dispatch_epi_pf<R, DAG, D5, true>(epi, a, grid, s);
+    else
+        dispatch_epi_pf<R, DAG, D5, false>(epi, a, grid, s);
+}
+
 }  // namespace
 
-HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap)
+HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefetch)
 {
     HopCfg c{};
     c.remap = remap;
+    c.prefetch = prefetch;
     c.bx = (L.V4 + kBlock - 1) / kBlock;
     int chunks = 1;
     if (chunks_override > 0) {
