@@ -97,3 +97,43 @@ def test_copy_kernel_and_timer():
         t.stop(ctx)
         assert t.elapsed_ms() >= 0.0
         assert np.array_equal(dst.download(np.float32), a)
+
+
+@pytest.mark.parametrize("tile_x,prefetch,chunks", [(0, 1, 0), (32, 0, 0), (32, 1, 2), (16, 1, 3), (48, 0, 16)])
+def test_tuned_launch_shapes_keep_parity(tile_x, prefetch, chunks):
+    Nt, Nx, Ls = 12, 96, 6
+    print_case(test="tuned_shapes", tile_x=tile_x, prefetch=prefetch, chunks=chunks)
+    rng = np.random.default_rng(tile_x + prefetch + chunks)
+    th = rand_theta(rng, Nt, Nx)
+    psi = rand_spinor(rng, 2 * Ls * Nt * Nx * 2).reshape(2, -1)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_TILE_X, tile_x)
+        ctx.tune(dwf.DWF_TUNE_PREFETCH, prefetch)
+        ctx.tune(dwf.DWF_TUNE_SCHUNKS, min(chunks, Ls))
+        for op in (0, 1, 3):
+            got = ctx.apply(op, psi)
+            for i in range(2):
+                want = oracle.dwf_apply(op, th, psi[i], Nt, Nx, Ls, M5, MF)
+                assert rel(got[i], want) <= 1e-12, (op, rel(got[i], want))
+        b = ctx.apply(1, psi[0])
+        x, st, rc = ctx.solve(3, b, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_MIXED)
+        Ax = oracle.dwf_apply(3, th, x, Nt, Nx, Ls, M5, MF)
+        assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= 1e-10
+
+
+def test_autotune_and_cache(tmp_path):
+    Nt, Nx, Ls = 16, 32, 8
+    print_case(test="autotune")
+    rng = np.random.default_rng(8)
+    th = rand_theta(rng, Nt, Nx)
+    psi = rand_spinor(rng, Ls * Nt * Nx * 2)
+    cache = str(tmp_path / "tune.txt")
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        r1 = ctx.autotune(4, dwf.DWF_PREC_FP32, cache)
+        r2 = ctx.autotune(4, dwf.DWF_PREC_FP32, cache)
+        assert r1["from_cache"] == 0 and r2["from_cache"] == 1 and r1["microseconds"] > 0
+        assert (r1["tile_x"], r1["prefetch"], r1["chunks"]) == (r2["tile_x"], r2["prefetch"], r2["chunks"])
+        got = ctx.apply(0, psi)
+        assert rel(got, oracle.dwf_apply(0, th, psi, Nt, Nx, Ls, M5, MF)) <= 1e-12

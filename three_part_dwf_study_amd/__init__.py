@@ -20,6 +20,7 @@ from .capi import (  # noqa: F401
     DWF_TUNE_PREFETCH,
     DWF_TUNE_RELIABLE_DELTA,
     DWF_TUNE_SCHUNKS,
+    DWF_TUNE_TILE_X,
     DWF_TUNE_XCD_REMAP,
     DwfContext,
     DwfError,

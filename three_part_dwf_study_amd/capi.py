@@ -42,6 +42,7 @@ DWF_TUNE_SCHUNKS = 0
 DWF_TUNE_XCD_REMAP = 1
 DWF_TUNE_RELIABLE_DELTA = 2
 DWF_TUNE_PREFETCH = 3
+DWF_TUNE_TILE_X = 4
 
 
 class DwfError(RuntimeError):
@@ -101,6 +102,16 @@ class HmcResult(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("tile_x", ctypes.c_int),
+        ("prefetch", ctypes.c_int),
+        ("chunks", ctypes.c_int),
+        ("from_cache", ctypes.c_int),
+        ("microseconds", ctypes.c_double),
+    ]
+
+
 class MeasureSpec(ctypes.Structure):
     _fields_ = [
         ("method", ctypes.c_int),
@@ -151,6 +162,7 @@ PROTOTYPES = {
     "dwf_ctx_destroy": (_I, [_P]),
     "dwf_ctx_stream": (_P, [_P]),
     "dwf_ctx_tune": (_I, [_P, _I, _D]),
+    "dwf_autotune": (_I, [_P, _I, _I, ctypes.c_char_p, _P]),
     "dwf_set_gauge": (_I, [_P, _P]),
     "dwf_set_gauge_device": (_I, [_P, _P]),
     "dwf_apply": (_I, [_P, _I, _P, _P, _I]),
@@ -283,6 +295,13 @@ class DwfContext:
 
     def tune(self, param: int, value: float):
         _check(self._lib, self._lib.dwf_ctx_tune(self._h, param, value), "dwf_ctx_tune")
+
+    def autotune(self, nrhs: int, precision: int = DWF_PREC_FP32, cache_path: str | None = None) -> dict:
+        r = TuneResult()
+        _check(self._lib, self._lib.dwf_autotune(self._h, nrhs, precision,
+                                                 cache_path.encode() if cache_path else None, ctypes.byref(r)),
+               "dwf_autotune")
+        return {name: getattr(r, name) for name, _ in r._fields_}
 
     def synchronize(self):
         _check(self._lib, self._lib.dwf_ctx_synchronize(self._h), "dwf_ctx_synchronize")

@@ -1,0 +1,142 @@
+// dwf_autotune.hip -- launch-shape autotuner of the Dirac stencil.
+//
+// For one (Nt, Nx, Ls, nrhs, precision) it times the D apply over the candidate shapes
+// -- block mapping (consecutive sites or 2D tiles of width 16/32/64 dividing Nx),
+// software-pipelined slice loop on/off, s-chunks per 4D site -- with HIP events on the
+// context's stream, keeps the fastest and applies it to the context.  Results can be
+// cached in a text file (one line per key), so a production run tunes once.
+
+#include "../../include/dwf_hip.h"
+#include "dwf_context.hpp"
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+using namespace dwfk;
+
+namespace {
+
+struct Shape {
+    int tile_x, prefetch, chunks;
+};
+
+std::string key_of(const dwf_ctx* c, int nrhs, int prec)
+{
+    std::ostringstream k;
+    k << (c->kind == 0 ? "dwf" : "wilson") << ' ' << c->L.Nt << ' ' << c->L.Nx << ' ' << c->L.Ls << ' ' << nrhs << ' '
+      << (prec == DWF_PREC_FP32 ? "fp32" : "fp64");
+    return k.str();
+}
+
+bool load_cached(const char* path, const std::string& key, Shape& s, double& us)
+{
+    if (!path) return false;
+    std::ifstream f(path);
+    std::string line;
+    bool found = false;
+    while (std::getline(f, line)) {
+        if (line.compare(0, key.size(), key) != 0 || line.size() <= key.size() || line[key.size()] != ' ') continue;
+        std::istringstream rest(line.substr(key.size()));
+        Shape t{};
+        double u = 0;
+        if (rest >> t.tile_x >> t.prefetch >> t.chunks >> u) {
+            s = t;
+            us = u;
+            found = true;  // the last line for a key wins
+        }
+    }
+    return found;
+}
+
+void apply_shape(dwf_ctx* c, const Shape& s)
+{
+    c->tile_x = s.tile_x;
+    c->prefetch = s.prefetch;
+    c->chunks_override = s.chunks;
+}
+
+int guard(const std::function<int()>& f)
+{
+    try {
+        return f();
+    } catch (const DwfError& e) {
+        dwf_set_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        dwf_set_error(e.what());
+        return DWF_ERR_ARG;
+    }
+}
+
+}  // namespace
+
+extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cache_path, dwf_tune_result* out)
+{
+    return guard([&]() -> int {
+        if (!c || nrhs < 1 || (precision != DWF_PREC_FP32 && precision != DWF_PREC_FP64))
+            throw DwfError(DWF_ERR_ARG, "dwf_autotune: bad argument");
+        if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_autotune: no gauge field set");
+        const std::string key = key_of(c, nrhs, precision);
+        Shape best{};
+        double best_us = 0;
+        if (load_cached(cache_path, key, best, best_us)) {
+            apply_shape(c, best);
+            if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, 1, best_us};
+            return DWF_OK;
+        }
+        const Lat& L = c->L;
+        std::vector<Shape> cand;
+        std::vector<int> tiles = {0};
+        for (int tx : {16, 32, 64})
+            if (L.Nx % tx == 0 && L.Nx > tx / 2) tiles.push_back(tx);
+        std::vector<int> chunks = {0};
+        for (int ch : {1, 2, 4, 8, 16})
+            if (ch <= L.Ls) chunks.push_back(ch);
+        for (int tx : tiles)
+            for (int pf : {0, 1})
+                for (int ch : chunks) cand.push_back(Shape{tx, pf, ch});
+
+        const size_t rb = precision == DWF_PREC_FP32 ? 4 : 8;
+        const size_t n = (size_t)L.V5() * 4 * nrhs;
+        void* in = c->buf(B_AUX0, n * rb);
+        void* o = c->buf(B_AUX1, n * rb);
+        c->check(hipMemsetAsync(in, 0, n * rb, c->stream), "autotune fill");  // zeros time like data
+        hipEvent_t e0, e1;
+        c->check(hipEventCreate(&e0), "event");
+        c->check(hipEventCreate(&e1), "event");
+        const int saved_tx = c->tile_x, saved_pf = c->prefetch, saved_ch = c->chunks_override;
+        best_us = 1e30;
+        const int reps = nrhs >= 8 ? 5 : 20;
+        for (const Shape& s : cand) {
+            apply_shape(c, s);
+            dwf_apply_device_impl(c, DWF_OP_D, precision, in, o, nrhs);  // warm-up
+            c->check(hipEventRecord(e0, c->stream), "event");
+            for (int r = 0; r < reps; ++r) dwf_apply_device_impl(c, DWF_OP_D, precision, in, o, nrhs);
+            c->check(hipEventRecord(e1, c->stream), "event");
+            c->check(hipEventSynchronize(e1), "event sync");
+            float ms = 0;
+            c->check(hipEventElapsedTime(&ms, e0, e1), "event time");
+            const double us = 1e3 * ms / reps;
+            if (us < best_us) {
+                best_us = us;
+                best = s;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)saved_tx;
+        (void)saved_pf;
+        (void)saved_ch;
+        apply_shape(c, best);
+        if (cache_path) {
+            std::ofstream f(cache_path, std::ios::app);
+            if (f) f << key << ' ' << best.tile_x << ' ' << best.prefetch << ' ' << best.chunks << ' ' << best_us << '\n';
+        }
+        if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, 0, best_us};
+        return DWF_OK;
+    });
+}

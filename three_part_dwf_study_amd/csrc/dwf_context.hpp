@@ -75,6 +75,7 @@ struct dwf_ctx {
     int chunks_override = 0;      // s-chunks per 4D site for the stencils (0: heuristic)
     int xcd_remap = 1;            // XCD-aware block order for the stencils
     int prefetch = 0;             // software-pipelined stencil slice loop
+    int tile_x = 0;               // stencil 2D tile width (0: linear blocks)
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
 
     struct Buf {

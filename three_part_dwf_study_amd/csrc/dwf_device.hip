@@ -129,15 +129,24 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
     const int V4 = L.V4;
     const long long V5 = L.V5();
     const long long rb = (long long)rhs * V5 * 4;  // offset in reals
-    const int site = tile * kBlock + threadIdx.x;
+    int t, x;
+    if (a.cfg.tx) {  // 2D tile: (kBlock/tx) rows x tx columns
+        const int tt = tile / a.cfg.tiles_x, tc = tile - tt * a.cfg.tiles_x;
+        const int lt = threadIdx.x / a.cfg.tx;
+        t = tt * (kBlock / a.cfg.tx) + lt;
+        x = tc * a.cfg.tx + (threadIdx.x - lt * a.cfg.tx);
+    } else {
+        const int lin = tile * kBlock + threadIdx.x;
+        t = lin / L.Nx;
+        x = lin - t * L.Nx;
+    }
+    const int site = t * L.Nx + x;
     double acc0 = 0.0, acc1 = 0.0;
 
     const int s0 = chunk * a.cfg.spt;
     const int s1 = min(L.Ls, s0 + a.cfg.spt);
-    if (site < V4 && s0 < s1) {
+    if (t < L.Nt && s0 < s1) {
         const int Nx = L.Nx, Nt = L.Nt;
-        const int t = site / Nx;
-        const int x = site - t * Nx;
         const int tp = (t + 1 == Nt) ? 0 : t + 1;
         const int tm = (t == 0) ? Nt - 1 : t - 1;
         const int xp = (x + 1 == Nx) ? 0 : x + 1;
@@ -501,12 +510,21 @@ void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 
 }  // namespace
 
-HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefetch)
+HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefetch, int tile_x)
 {
     HopCfg c{};
     c.remap = remap;
     c.prefetch = prefetch;
-    c.bx = (L.V4 + kBlock - 1) / kBlock;
+    if (tile_x > 0 && tile_x <= kBlock && kBlock % tile_x == 0 && L.Nx % tile_x == 0) {
+        c.tx = tile_x;
+        c.tiles_x = L.Nx / tile_x;
+        const int rows = kBlock / tile_x;
+        c.bx = c.tiles_x * ((L.Nt + rows - 1) / rows);
+    } else {
+        c.tx = 0;
+        c.tiles_x = 0;
+        c.bx = (L.V4 + kBlock - 1) / kBlock;
+    }
     int chunks = 1;
     if (chunks_override > 0) {
         chunks = chunks_override;
