@@ -99,23 +99,25 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * DWF_TUNE_RELIABLE_DELTA: residual drop (0 < delta < 1) that triggers a reliable update
  * of the mixed solver; DWF_TUNE_PREFETCH: 0/1 software-pipelined stencil slice loop;
  * DWF_TUNE_TILE_X: 0 for blocks of consecutive 4D sites, or the width (dividing Nx and
- * 256) of 2D stencil tiles of 256/width rows. */
+ * 256) of 2D stencil tiles of 256/width rows; DWF_TUNE_NT_STORE: 0/1 streaming
+ * (non-temporal) stores of stencil outputs. */
 enum dwf_tune_param {
     DWF_TUNE_SCHUNKS = 0,
     DWF_TUNE_XCD_REMAP = 1,
     DWF_TUNE_RELIABLE_DELTA = 2,
     DWF_TUNE_PREFETCH = 3,
-    DWF_TUNE_TILE_X = 4
+    DWF_TUNE_TILE_X = 4,
+    DWF_TUNE_NT_STORE = 5
 };
 int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 
 /* Autotuner (no counterpart in the reference): times the D stencil over its launch
- * shapes (block mapping, software pipelining, s-chunks) for nrhs right-hand sides in
+ * shapes (block mapping, software pipelining, s-chunks, streaming stores) for nrhs right-hand sides in
  * the given precision, applies the fastest to the context and reports it.  cache_path
  * (nullable) is a text file of earlier results: a cached key is reused without timing,
  * a new one is appended.  Needs the gauge field set. */
 typedef struct dwf_tune_result {
-    int tile_x, prefetch, chunks;
+    int tile_x, prefetch, chunks, nt_store;
     int from_cache;
     double microseconds;  /* one D apply of nrhs right-hand sides */
 } dwf_tune_result;

@@ -110,6 +110,7 @@ def test_tuned_launch_shapes_keep_parity(tile_x, prefetch, chunks):
         ctx.set_gauge(th)
         ctx.tune(dwf.DWF_TUNE_TILE_X, tile_x)
         ctx.tune(dwf.DWF_TUNE_PREFETCH, prefetch)
+        ctx.tune(dwf.DWF_TUNE_NT_STORE, prefetch)
         ctx.tune(dwf.DWF_TUNE_SCHUNKS, min(chunks, Ls))
         for op in (0, 1, 3):
             got = ctx.apply(op, psi)
@@ -134,6 +135,6 @@ def test_autotune_and_cache(tmp_path):
         r1 = ctx.autotune(4, dwf.DWF_PREC_FP32, cache)
         r2 = ctx.autotune(4, dwf.DWF_PREC_FP32, cache)
         assert r1["from_cache"] == 0 and r2["from_cache"] == 1 and r1["microseconds"] > 0
-        assert (r1["tile_x"], r1["prefetch"], r1["chunks"]) == (r2["tile_x"], r2["prefetch"], r2["chunks"])
+        assert all(r1[k] == r2[k] for k in ("tile_x", "prefetch", "chunks", "nt_store"))
         got = ctx.apply(0, psi)
         assert rel(got, oracle.dwf_apply(0, th, psi, Nt, Nx, Ls, M5, MF)) <= 1e-12

@@ -43,6 +43,7 @@ DWF_TUNE_XCD_REMAP = 1
 DWF_TUNE_RELIABLE_DELTA = 2
 DWF_TUNE_PREFETCH = 3
 DWF_TUNE_TILE_X = 4
+DWF_TUNE_NT_STORE = 5
 
 
 class DwfError(RuntimeError):
@@ -107,6 +108,7 @@ class TuneResult(ctypes.Structure):
         ("tile_x", ctypes.c_int),
         ("prefetch", ctypes.c_int),
         ("chunks", ctypes.c_int),
+        ("nt_store", ctypes.c_int),
         ("from_cache", ctypes.c_int),
         ("microseconds", ctypes.c_double),
     ]

@@ -2,7 +2,8 @@
 //
 // For one (Nt, Nx, Ls, nrhs, precision) it times the D apply over the candidate shapes
 // -- block mapping (consecutive sites or 2D tiles of width 16/32/64 dividing Nx),
-// software-pipelined slice loop on/off, s-chunks per 4D site -- with HIP events on the
+// software-pipelined slice loop on/off, s-chunks per 4D site, streaming stores on/off
+// -- with HIP events on the
 // context's stream, keeps the fastest and applies it to the context.  Results can be
 // cached in a text file (one line per key), so a production run tunes once.
 
@@ -21,7 +22,7 @@ using namespace dwfk;
 namespace {
 
 struct Shape {
-    int tile_x, prefetch, chunks;
+    int tile_x, prefetch, chunks, nt_store;
 };
 
 std::string key_of(const dwf_ctx* c, int nrhs, int prec)
@@ -43,7 +44,7 @@ bool load_cached(const char* path, const std::string& key, Shape& s, double& us)
         std::istringstream rest(line.substr(key.size()));
         Shape t{};
         double u = 0;
-        if (rest >> t.tile_x >> t.prefetch >> t.chunks >> u) {
+        if (rest >> t.tile_x >> t.prefetch >> t.chunks >> t.nt_store >> u) {
             s = t;
             us = u;
             found = true;  // the last line for a key wins
@@ -57,6 +58,7 @@ void apply_shape(dwf_ctx* c, const Shape& s)
     c->tile_x = s.tile_x;
     c->prefetch = s.prefetch;
     c->chunks_override = s.chunks;
+    c->nt_store = s.nt_store;
 }
 
 int guard(const std::function<int()>& f)
@@ -85,7 +87,7 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
         double best_us = 0;
         if (load_cached(cache_path, key, best, best_us)) {
             apply_shape(c, best);
-            if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, 1, best_us};
+            if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, best.nt_store, 1, best_us};
             return DWF_OK;
         }
         const Lat& L = c->L;
@@ -98,7 +100,8 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
             if (ch <= L.Ls) chunks.push_back(ch);
         for (int tx : tiles)
             for (int pf : {0, 1})
-                for (int ch : chunks) cand.push_back(Shape{tx, pf, ch});
+                for (int ch : chunks)
+                    for (int nt : {0, 1}) cand.push_back(Shape{tx, pf, ch, nt});
 
         const size_t rb = precision == DWF_PREC_FP32 ? 4 : 8;
         const size_t n = (size_t)L.V5() * 4 * nrhs;
@@ -134,9 +137,11 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
         apply_shape(c, best);
         if (cache_path) {
             std::ofstream f(cache_path, std::ios::app);
-            if (f) f << key << ' ' << best.tile_x << ' ' << best.prefetch << ' ' << best.chunks << ' ' << best_us << '\n';
+            if (f)
+                f << key << ' ' << best.tile_x << ' ' << best.prefetch << ' ' << best.chunks << ' ' << best.nt_store << ' '
+                  << best_us << '\n';
         }
-        if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, 0, best_us};
+        if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, best.nt_store, 0, best_us};
         return DWF_OK;
     });
 }

@@ -76,6 +76,7 @@ struct dwf_ctx {
     int xcd_remap = 1;            // XCD-aware block order for the stencils
     int prefetch = 0;             // software-pipelined stencil slice loop
     int tile_x = 0;               // stencil 2D tile width (0: linear blocks)
+    int nt_store = 0;             // streaming stores of stencil outputs
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
 
     struct Buf {

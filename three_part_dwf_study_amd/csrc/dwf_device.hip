@@ -56,6 +56,18 @@ template <typename R> __device__ __forceinline__ void stsp(R* base, long long i,
     reinterpret_cast<typename Vec4<R>::T*>(base)[i] = v;
 }
 
+// Streaming store (no L2 allocation on the way to HBM) for outputs nobody re-reads soon.
+template <typename R> __device__ __forceinline__ void stsp_nt(R* base, long long i, const Sp<R>& s)
+{
+    typedef R nv4 __attribute__((ext_vector_type(4)));
+    nv4 v;
+    v.x = s.ar;
+    v.y = s.ai;
+    v.z = s.br;
+    v.w = s.bi;
+    __builtin_nontemporal_store(v, reinterpret_cast<nv4*>(base) + i);
+}
+
 template <typename R> __device__ __forceinline__ double norm2(const Sp<R>& s)
 {
     const double a = s.ar, b = s.ai, c = s.br, d = s.bi;
@@ -256,9 +268,15 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
 
             const long long idx = sl + site;
             if constexpr (EPI == EPI_STORE) {
-                stsp(a.out + rb, idx, o);
+                if (a.cfg.nt_store)
+                    stsp_nt(a.out + rb, idx, o);
+                else
+                    stsp(a.out + rb, idx, o);
             } else if constexpr (EPI == EPI_NORM) {
-                stsp(a.out + rb, idx, o);
+                if (a.cfg.nt_store)
+                    stsp_nt(a.out + rb, idx, o);
+                else
+                    stsp(a.out + rb, idx, o);
                 acc0 += norm2(o);
             } else if constexpr (EPI == EPI_CGUPD) {
                 const Sp<R> pv = ldsp(a.p + rb, idx);
@@ -510,9 +528,10 @@ void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 
 }  // namespace
 
-HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefetch, int tile_x)
+HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefetch, int tile_x, int nt_store)
 {
     HopCfg c{};
+    c.nt_store = nt_store;
     c.remap = remap;
     c.prefetch = prefetch;
     if (tile_x > 0 && tile_x <= kBlock && kBlock % tile_x == 0 && L.Nx % tile_x == 0) {
