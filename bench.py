@@ -144,6 +144,7 @@ def main():
     if args.kernels_only:
         ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
         ctx.set_gauge(theta)
+        ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32)
         ms32, gbs32 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, args.dslash_rhs, 5)
         cb = dslash_bytes(4, args.dslash_rhs) // 2 // 16 * 16
         msc, gbsc = time_copy(ctx, capi, cb, 5)
@@ -157,6 +158,8 @@ def main():
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
     ctx.set_gauge(theta)
 
+    tuned = {"solves": ctx.autotune(nrhs, dwf.DWF_PREC_FP32)}
+    log(f"autotuned stencil shape for {nrhs} RHS: {tuned['solves']}")
     n = nrhs * LS * NT * NX * 4
     d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
     d_x = capi.DeviceArray(n * 8)
@@ -188,8 +191,11 @@ def main():
                              max_true_relres=tr)
         log(f"{name}: {results[name]}")
 
+    tuned["fp32_batch"] = ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32)
     ms32, gbs32 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, args.dslash_rhs, 20)
+    tuned["fp64_batch"] = ctx.autotune(args.dslash_rhs // 2, dwf.DWF_PREC_FP64)
     ms64, gbs64 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP64, args.dslash_rhs // 2, 20)
+    tuned["fp32_single"] = ctx.autotune(1, dwf.DWF_PREC_FP32)
     ms1, gbs1 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, 1, 200)
     msc, gbsc = time_copy(ctx, capi, dslash_bytes(4, args.dslash_rhs) // 2 // 16 * 16, 20)
     log(f"copy kernel ({dslash_bytes(4, args.dslash_rhs) // 2 >> 20} MiB each way): {msc:.4f} ms {gbsc:.0f} GB/s")
@@ -246,6 +252,8 @@ def main():
             "bytes_per_launch": dslash_bytes(4, args.dslash_rhs),
         },
         "cpu_baseline": cpu,
+        "autotuned_shapes": {k: {kk: v[kk] for kk in ("tile_x", "prefetch", "chunks", "nt_store")}
+                             for k, v in tuned.items()},
     }
     print(json.dumps(line), flush=True)
     ctx.close()
