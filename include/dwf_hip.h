@@ -129,6 +129,14 @@ int dwf_autotune(dwf_ctx* ctx, int nrhs, int precision, const char* cache_path, 
 int dwf_set_gauge(dwf_ctx* ctx, const double* theta);
 int dwf_set_gauge_device(dwf_ctx* ctx, const double* theta_device);
 
+/* Hold ncfg configurations theta[ncfg][2][Nt][Nx] of the same lattice at once (no
+ * counterpart in the reference, which measures one configuration per OpenMP thread,
+ * domain_wall/analyze_configs.cpp:45-76).  Afterwards the operator, solver and
+ * measurement entry points take nrhs = ncfg * k right-hand sides, and right-hand side r
+ * belongs to configuration r / k: one batched launch serves many configurations.
+ * The force, gauge and HMC entry points need a single configuration. */
+int dwf_set_gauge_batch(dwf_ctx* ctx, int ncfg, const double* theta);
+
 /* ---- operators --------------------------------------------------------------------- */
 
 /* out = op(in) for nrhs vectors in FP64 (host buffers).  Replaces apply_D,
@@ -269,6 +277,13 @@ typedef struct dwf_measure_out {
  * "Eta loop CGNR failed to converge", "analysis.max_momentum is outside 0..Nx/2".  Wall
  * solves that miss rtol are counted, not fatal, as in the reference. */
 int dwf_measure(dwf_ctx* ctx, const dwf_measure_spec* spec, dwf_measure_out* out);
+
+/* The same for the ncfg configurations a context holds (dwf_set_gauge_batch): all
+ * ncfg x (8 + noise) sources go through one batched solve.  seeds[k] (nullable: every
+ * configuration uses spec->noise_seed) is configuration k's Z4 seed, outs[k] its outputs.
+ * Replaces the OpenMP loop over configurations of domain_wall/analyze_configs.cpp:45-76. */
+int dwf_measure_batch(dwf_ctx* ctx, const dwf_measure_spec* spec, int ncfg,
+                      const unsigned long long* seeds, dwf_measure_out* outs);
 
 /* ---- host random numbers of the callers (reference draw order) --------------------- */
 

@@ -251,6 +251,43 @@ inline Measurements measure(Context& c, const Gauge& theta, double rtol, int max
     return m;
 }
 
+// The same for ncfg configurations at once (thetas: ncfg consecutive fields, seeds: one
+// Z4 seed each): every configuration's sources go through one batched solve.
+inline std::vector<Measurements> measure_batch(Context& c, const double* thetas, int ncfg,
+                                               const std::vector<unsigned long long>& seeds, double rtol,
+                                               int maxiter, int max_momentum, int noise_vectors,
+                                               int method = DWF_SOLVE_CG64)
+{
+    if (ncfg < 1 || (int)seeds.size() != ncfg) throw std::runtime_error("measure_batch: one seed per configuration");
+    const int Nt = c.Nt(), Nx = c.Nx();
+    check(dwf_set_gauge_batch(c.get(), ncfg, thetas));
+    c.invalidate_gauge();
+    std::vector<Measurements> ms(ncfg);
+    std::vector<std::vector<double>> mom(ncfg, std::vector<double>((size_t)(max_momentum + 1) * Nt, 0.0));
+    std::vector<std::vector<double>> loop(ncfg, std::vector<double>((size_t)Nt * Nx * 2, 0.0));
+    std::vector<dwf_measure_out> outs(ncfg);
+    for (int k = 0; k < ncfg; ++k) {
+        ms[k].pair.CPP.assign(Nt, 0.0);
+        ms[k].pair.CJ5.assign(Nt, 0.0);
+        outs[k] = dwf_measure_out{ms[k].pair.CPP.data(), ms[k].pair.CJ5.data(), mom[k].data(), loop[k].data(),
+                                  nullptr, nullptr, 0, 0};
+    }
+    dwf_measure_spec spec{method, maxiter, rtol, 1, 1, max_momentum, noise_vectors, 0};
+    const int rc = dwf_measure_batch(c.get(), &spec, ncfg, seeds.data(), outs.data());
+    check(dwf_set_gauge_batch(c.get(), 1, thetas));  // leave the context with one configuration
+    check(rc);
+    for (int k = 0; k < ncfg; ++k) {
+        Measurements& m = ms[k];
+        m.wall_nonconverged = outs[k].wall_nonconverged;
+        m.momentum_CPP.assign(max_momentum + 1, Corr(Nt, 0.0));
+        for (int n = 0; n <= max_momentum; ++n)
+            for (int t = 0; t < Nt; ++t) m.momentum_CPP[n][t] = mom[k][(size_t)n * Nt + t];
+        m.loop.resize((size_t)Nt * Nx);
+        for (size_t i = 0; i < m.loop.size(); ++i) m.loop[i] = Complex(loop[k][2 * i], loop[k][2 * i + 1]);
+    }
+    return ms;
+}
+
 inline CorrelatorPair compute_CPP_CJ5(Context& c, const Gauge& theta, double rtol, int maxiter,
                                       int method = DWF_SOLVE_CG64)
 {

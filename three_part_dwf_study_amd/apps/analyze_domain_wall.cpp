@@ -1,7 +1,8 @@
 // analyze_domain_wall -- drop-in for the reference's domain_wall/analyze_configs.cpp.
-// Per configuration, one device call solves all 8 + eta.noise_vectors propagators in a
-// batch and contracts C_PP, C_J5, the momentum-projected point correlator and the Z4
-// loop density in FP64 (dwf_measure).  The ensemble contractions -- averages, m_res,
+// Configurations are measured in groups (gpu.analysis_batch, automatic when absent): one
+// device call solves all 8 + eta.noise_vectors propagators of every configuration in the
+// group as one batch and contracts C_PP, C_J5, the momentum-projected point correlator
+// and the Z4 loop density in FP64 (dwf_measure_batch).  The ensemble contractions -- averages, m_res,
 // the vacuum-subtracted disconnected loop, eta, momentum projection -- stay on the host
 // in FP64 (analyze_configs.cpp:78-271) and the three CSV files keep the reference's
 // layouts: correlator.csv, channels_by_config.csv, momentum_correlators.csv.
@@ -9,6 +10,7 @@
 #include "dwf_hip.hpp"
 #include "dwf_params.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <iomanip>
@@ -37,17 +39,27 @@ int main()
         std::vector<Corr> CPP_all(ncfg), CJ5_all(ncfg);
         std::vector<std::vector<Complex>> loops_all(ncfg);
         std::vector<std::vector<Corr>> mom_all(ncfg);
-        for (int c = 0; c < ncfg; ++c) {
-            std::cout << "measuring config " << c + 1 << "/" << ncfg << '\n';
-            std::vector<double> theta(arr.data.begin() + (long)(c * ngauge), arr.data.begin() + (long)((c + 1) * ngauge));
-            const dwfhip::Measurements m = dwfhip::measure(
-                ctx, theta, P.propagator_rtol, P.propagator_maxiter, P.max_momentum, P.eta_noise_vectors,
-                P.eta_random_seed + (unsigned long long)c * 104729ULL, P.propagator_method);
-            for (int k = 0; k < m.wall_nonconverged; ++k) std::cout << "    CGNR info = nonzero\n";
-            CPP_all[c] = m.pair.CPP;
-            CJ5_all[c] = m.pair.CJ5;
-            loops_all[c] = m.loop;
-            mom_all[c] = m.momentum_CPP;
+        // about 512 right-hand sides per batched solve unless gpu.analysis_batch says otherwise
+        const int per_cfg = 8 + P.eta_noise_vectors;
+        const int group = P.analysis_batch > 0 ? P.analysis_batch : std::max(1, 512 / per_cfg);
+        for (int c0 = 0; c0 < ncfg; c0 += group) {
+            const int n = std::min(group, ncfg - c0);
+            std::vector<unsigned long long> seeds(n);
+            for (int k = 0; k < n; ++k) {
+                std::cout << "measuring config " << c0 + k + 1 << "/" << ncfg << '\n';
+                seeds[k] = P.eta_random_seed + (unsigned long long)(c0 + k) * 104729ULL;  // analyze_configs.cpp:67-68
+            }
+            const std::vector<dwfhip::Measurements> ms =
+                dwfhip::measure_batch(ctx, arr.data.data() + (size_t)c0 * ngauge, n, seeds, P.propagator_rtol,
+                                      P.propagator_maxiter, P.max_momentum, P.eta_noise_vectors, P.propagator_method);
+            for (int k = 0; k < n; ++k) {
+                const dwfhip::Measurements& m = ms[k];
+                for (int j = 0; j < m.wall_nonconverged; ++j) std::cout << "    CGNR info = nonzero\n";
+                CPP_all[c0 + k] = m.pair.CPP;
+                CJ5_all[c0 + k] = m.pair.CJ5;
+                loops_all[c0 + k] = m.loop;
+                mom_all[c0 + k] = m.momentum_CPP;
+            }
         }
 
         // ensemble averages and m_res (analyze_configs.cpp:78-113)

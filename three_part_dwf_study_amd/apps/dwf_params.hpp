@@ -20,6 +20,7 @@ struct DwfParams {
     int max_momentum;
     unsigned long long random_seed;
     int force_method, propagator_method;  // gpu.* (additive keys)
+    int analysis_batch;                   // configurations per batched solve (0 = automatic)
 
     explicit DwfParams(const Parameters& p)
     {
@@ -46,6 +47,8 @@ struct DwfParams {
         random_seed = p.unsigned_integer("dwf.random_seed");
         force_method = solver_method(p, "gpu.force_solver", 0);
         propagator_method = solver_method(p, "gpu.propagator_solver", 0);
+        analysis_batch = std::stoi(p.text_or("gpu.analysis_batch", "0"));
+        if (analysis_batch < 0) throw std::runtime_error("gpu.analysis_batch must be >= 0");
     }
 };
 

@@ -167,6 +167,7 @@ PROTOTYPES = {
     "dwf_autotune": (_I, [_P, _I, _I, ctypes.c_char_p, _P]),
     "dwf_set_gauge": (_I, [_P, _P]),
     "dwf_set_gauge_device": (_I, [_P, _P]),
+    "dwf_set_gauge_batch": (_I, [_P, _I, _P]),
     "dwf_apply": (_I, [_P, _I, _P, _P, _I]),
     "dwf_apply_device": (_I, [_P, _I, _I, _P, _P, _I]),
     "dwf_solve": (_I, [_P, _I, _I, _P, _P, _D, _I, _I, _P, _P]),
@@ -180,6 +181,7 @@ PROTOTYPES = {
     "dwf_get_gauge": (_I, [_P, _P]),
     "dwf_hmc_trajectory": (_I, [_P, _P, _P, _P, _D, _P]),
     "dwf_measure": (_I, [_P, _P, _P]),
+    "dwf_measure_batch": (_I, [_P, _P, _I, _P, _P]),
     "dwf_device_alloc": (_P, [ctypes.c_ulonglong]),
     "dwf_device_free": (_I, [_P]),
     "dwf_memcpy_htod": (_I, [_P, _P, ctypes.c_ulonglong]),
@@ -317,6 +319,13 @@ class DwfContext:
         if th.size != 2 * self.geom.V4:
             raise ValueError("theta must hold 2*Nt*Nx angles")
         _check(self._lib, self._lib.dwf_set_gauge(self._h, _ptr(th)), "dwf_set_gauge")
+
+    def set_gauge_batch(self, thetas: np.ndarray):
+        th = np.ascontiguousarray(thetas, dtype=np.float64).reshape(-1)
+        if th.size % (2 * self.geom.V4):
+            raise ValueError("thetas must hold ncfg x 2*Nt*Nx angles")
+        n = th.size // (2 * self.geom.V4)
+        _check(self._lib, self._lib.dwf_set_gauge_batch(self._h, n, _ptr(th)), "dwf_set_gauge_batch")
 
     def set_gauge_device(self, theta_ptr: int):
         _check(self._lib, self._lib.dwf_set_gauge_device(self._h, theta_ptr), "dwf_set_gauge_device")
@@ -461,6 +470,32 @@ class DwfContext:
         _check(self._lib, self._lib.dwf_measure(self._h, ctypes.byref(spec), ctypes.byref(out)), "dwf_measure")
         return dict(CPP=cpp, CJ5=cj5, momentum_CPP=mom.reshape(max_momentum + 1, Nt), loop=loop.reshape(Nt, Nx),
                     iterations=iters[:nrhs], dirac_residual=dres, wall_nonconverged=out.wall_nonconverged)
+
+    def measure_batch(self, ncfg: int, rtol: float, maxiter: int, wall: bool = True, point: bool = True,
+                      max_momentum: int = 0, noise_vectors: int = 0, seeds=None,
+                      method: int = DWF_SOLVE_CG64) -> list:
+        """dwf_measure for the ncfg configurations held (set_gauge_batch); one dict each."""
+        Nt, Nx = self.geom.Nt, self.geom.Nx
+        K = (4 if wall else 0) + (4 if point else 0) + noise_vectors
+        keep, outs = [], (MeasureOut * ncfg)()
+        for k in range(ncfg):
+            d = dict(CPP=np.zeros(Nt), CJ5=np.zeros(Nt), momentum_CPP=np.zeros((max_momentum + 1) * Nt),
+                     loop=np.zeros(Nt * Nx, np.complex128), iterations=np.zeros(max(K, 1), np.int32),
+                     dirac_residual=np.zeros(4))
+            keep.append(d)
+            outs[k] = MeasureOut(*(_ptr(d[n]) for n in ("CPP", "CJ5", "momentum_CPP", "loop", "iterations",
+                                                        "dirac_residual")), 0, 0)
+        sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.uint64)
+        spec = MeasureSpec(method, maxiter, rtol, int(wall), int(point), max_momentum, noise_vectors, 0)
+        _check(self._lib, self._lib.dwf_measure_batch(self._h, ctypes.byref(spec), ncfg,
+                                                      None if sd is None else _ptr(sd),
+                                                      ctypes.cast(outs, ctypes.c_void_p)), "dwf_measure_batch")
+        for k, d in enumerate(keep):
+            d["momentum_CPP"] = d["momentum_CPP"].reshape(max_momentum + 1, Nt)
+            d["loop"] = d["loop"].reshape(Nt, Nx)
+            d["iterations"] = d["iterations"][:K]
+            d["wall_nonconverged"] = outs[k].wall_nonconverged
+        return keep
 
 
 class HostRng:

@@ -72,6 +72,12 @@ struct dwf_ctx {
     double2* l64 = nullptr;   // [2][V4] (cos, sin) FP64
     float2* l32 = nullptr;    // [2][V4] (cos, sin) FP32
     bool have_gauge = false;
+    int ncfg = 1;                 // configurations held (same lattice), see dwf_set_gauge_batch
+    int cfg_cap = 1;              // configurations the link buffers can hold
+    int rhs_per_cfg = 1 << 30;    // set per call: right-hand side r uses configuration r / rhs_per_cfg
+    void bind_rhs(int nrhs);      // checks nrhs against ncfg and sets rhs_per_cfg
+    void require_single_config(const char* what) const;
+    void reserve_configs(int n);  // grow the gauge buffers to n configurations
     int chunks_override = 0;      // s-chunks per 4D site for the stencils (0: heuristic)
     int xcd_remap = 1;            // XCD-aware block order for the stencils
     int prefetch = 0;             // software-pipelined stencil slice loop

@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
         const R sxp = (x == Nx - 1) ? (R)L.bcx : (R)1;
         const R sxm = (x == 0) ? (R)L.bcx : (R)1;
         using C = typename Cx<R>::T;
-        const C* lk = static_cast<const C*>(a.links);
+        const C* lk = static_cast<const C*>(a.links) + (long long)(rhs / a.rhs_per_cfg) * a.link_stride;
         const C ut = lk[site];
         const C ux = lk[V4 + site];
         const C utm = lk[tm * Nx + x];
@@ -626,9 +626,9 @@ template <typename R> void launch_zero(R* p, long long n, hipStream_t s)
 template void launch_zero<float>(float*, long long, hipStream_t);
 template void launch_zero<double>(double*, long long, hipStream_t);
 
-void launch_pack_links(const Lat& L, const double* theta, double2* l64, float2* l32, hipStream_t s)
+void launch_pack_links(long long nlinks, const double* theta, double2* l64, float2* l32, hipStream_t s)
 {
-    const int n = 2 * L.V4;
+    const int n = (int)nlinks;
     k_pack_links<<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(n, theta, l64, l32);
 }
 

@@ -132,6 +132,12 @@ __device__ __forceinline__ EoSite<R> eo_site(const Lat& L, int Nxh, int tpar, in
     return g;
 }
 
+// Links of the configuration that right-hand side `rhs` belongs to.
+template <typename R> __device__ __forceinline__ const void* cfg_links(const EoArgs<R>& a, int rhs)
+{
+    return static_cast<const typename C2T<R>::T*>(a.links) + (long long)(rhs / a.rhs_per_cfg) * a.link_stride;
+}
+
 // o += c * H psi at one slice (H: the bracket of D's hop, or D^dag's when DAG).
 template <typename R, bool DAG>
 __device__ __forceinline__ void add_hop(Sp<R>& o, R c, const EoSite<R>& g, const R* in, long long sl)
@@ -238,7 +244,7 @@ __global__ __launch_bounds__(kBlockA) void k_eo_a(const EoArgs<R> a)
     if (hs >= a.V4h) return;
     const long long V5h = (long long)a.V4h * LS;
     const long long rb = (long long)rhs * V5h * 4;
-    const EoSite<R> g = eo_site<R>(a.L, a.Nxh, a.tpar, hs, a.links);
+    const EoSite<R> g = eo_site<R>(a.L, a.Nxh, a.tpar, hs, cfg_links(a, rhs));
     R v0r[LS], v0i[LS], v1r[LS], v1i[LS];
 #pragma unroll
     for (int s = 0; s < LS; ++s) {
@@ -290,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
     double acc0 = 0.0, acc1 = 0.0;
     const int s0 = chunk * a.spt, s1 = min(Ls, s0 + a.spt);
     if (hs < V4h && s0 < s1) {
-        const EoSite<R> g = eo_site<R>(a.L, a.Nxh, a.tpar, hs, a.links);
+        const EoSite<R> g = eo_site<R>(a.L, a.Nxh, a.tpar, hs, cfg_links(a, rhs));
         const R d = (R)a.L.diag, mf = (R)a.L.mf;
         const R* in = a.in + rb;
         Sp<R> cur = ld(in, (long long)s0 * V4h + hs);

@@ -15,7 +15,9 @@ template <typename R> struct EoArgs {
     int chunks, spt;     // s-chunks (k_eo_b only)
     int remap;           // XCD-aware block order
     int tpar;            // target parity (0 even, 1 odd)
-    const void* links;   // [2][V4] complex R
+    const void* links;   // [ncfg][2][V4] complex R
+    long long link_stride;  // complex entries per configuration (2 V4)
+    int rhs_per_cfg;     // right-hand side r uses configuration r / rhs_per_cfg
     const R* in;         // k_eo_b: psi on the target parity (A term / identity term)
     const R* hin;        // the other-parity vector the hop reads
     const R* b;          // k_eo_a: optional target-parity source; k_eo_b EPI_RESID: rhs

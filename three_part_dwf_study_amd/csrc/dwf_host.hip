@@ -81,6 +81,41 @@ void dwf_ctx::check(hipError_t e, const char* what)
 
 void dwf_ctx::check_launch(const char* what) { check(hipGetLastError(), what); }
 
+void dwf_ctx::bind_rhs(int nrhs)
+{
+    if (ncfg <= 1) {
+        rhs_per_cfg = 1 << 30;
+        return;
+    }
+    if (nrhs % ncfg != 0)
+        throw DwfError(DWF_ERR_ARG, "the number of right-hand sides must be a multiple of the configurations held (" +
+                                        std::to_string(ncfg) + ")");
+    rhs_per_cfg = nrhs / ncfg;
+}
+
+void dwf_ctx::require_single_config(const char* what) const
+{
+    if (ncfg != 1) throw DwfError(DWF_ERR_ARG, std::string(what) + " needs a context holding one configuration");
+}
+
+void dwf_ctx::reserve_configs(int n)
+{
+    if (n <= cfg_cap) return;
+    const size_t nl = (size_t)2 * L.V4 * n;
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(theta);
+    (void)hipFree(l64);
+    (void)hipFree(l32);
+    theta = nullptr;
+    l64 = nullptr;
+    l32 = nullptr;
+    cfg_cap = 0;
+    check(hipMalloc(&theta, nl * sizeof(double)), "gauge allocation");
+    check(hipMalloc(&l64, nl * sizeof(double2)), "gauge allocation");
+    check(hipMalloc(&l32, nl * sizeof(float2)), "gauge allocation");
+    cfg_cap = n;
+}
+
 // ------------------------------------------------------------------- operators
 namespace {
 
@@ -90,6 +125,8 @@ template <typename R> HopArgs<R> hop_base(const dwf_ctx* c, const HopCfg& cfg)
     a.L = c->L;
     a.cfg = cfg;
     a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
+    a.link_stride = 2LL * c->L.V4;
+    a.rhs_per_cfg = c->rhs_per_cfg;
     return a;
 }
 
@@ -267,22 +304,27 @@ int dwf_set_gauge_device(dwf_ctx* c, const double* theta_device)
         const size_t nl = (size_t)2 * c->L.V4;
         if (theta_device != c->theta)
             c->check(hipMemcpyAsync(c->theta, theta_device, nl * 8, hipMemcpyDeviceToDevice, c->stream), "gauge copy");
-        launch_pack_links(c->L, c->theta, c->l64, c->l32, c->stream);
+        launch_pack_links((long long)nl, c->theta, c->l64, c->l32, c->stream);
         c->check_launch("pack links");
+        c->ncfg = 1;
         c->have_gauge = true;
         return DWF_OK;
     });
 }
 
-int dwf_set_gauge(dwf_ctx* c, const double* theta)
+int dwf_set_gauge(dwf_ctx* c, const double* theta) { return dwf_set_gauge_batch(c, 1, theta); }
+
+int dwf_set_gauge_batch(dwf_ctx* c, int ncfg, const double* theta)
 {
     return guard([&] {
-        if (!c || !theta) throw DwfError(DWF_ERR_ARG, "dwf_set_gauge: null argument");
+        if (!c || !theta || ncfg < 1) throw DwfError(DWF_ERR_ARG, "dwf_set_gauge: bad argument");
         const size_t nl = (size_t)2 * c->L.V4;
-        c->check(hipMemcpyAsync(c->theta, theta, nl * 8, hipMemcpyHostToDevice, c->stream), "gauge upload");
-        launch_pack_links(c->L, c->theta, c->l64, c->l32, c->stream);
+        c->reserve_configs(ncfg);
+        c->check(hipMemcpyAsync(c->theta, theta, nl * ncfg * 8, hipMemcpyHostToDevice, c->stream), "gauge upload");
+        launch_pack_links((long long)(nl * ncfg), c->theta, c->l64, c->l32, c->stream);
         c->check_launch("pack links");
         c->check(hipStreamSynchronize(c->stream), "gauge upload sync");
+        c->ncfg = ncfg;
         c->have_gauge = true;
         return DWF_OK;
     });
@@ -295,6 +337,7 @@ int dwf_apply_device(dwf_ctx* c, int op, int precision, const void* in, void* ou
             (precision != DWF_PREC_FP64 && precision != DWF_PREC_FP32) || in == out)
             throw DwfError(DWF_ERR_ARG, "dwf_apply_device: bad argument");
         if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_apply_device: no gauge field set");
+        c->bind_rhs(nrhs);
         dwf_apply_device_impl(c, op, precision, in, out, nrhs);
         return DWF_OK;
     });
@@ -306,6 +349,7 @@ int dwf_apply(dwf_ctx* c, int op, const double* in, double* out, int nrhs)
         if (!c || !in || !out || nrhs < 1 || op < DWF_OP_D || op > DWF_OP_DDAGD)
             throw DwfError(DWF_ERR_ARG, "dwf_apply: bad argument");
         if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_apply: no gauge field set");
+        c->bind_rhs(nrhs);
         const size_t n = (size_t)c->L.V5() * 4 * nrhs;
         double* din = (double*)c->buf(B_HOST_IN, n * 8);
         double* dout = (double*)c->buf(B_HOST_OUT, n * 8);

@@ -118,6 +118,8 @@ void require(dwf_ctx* c)
 {
     if (!c) throw DwfError(DWF_ERR_ARG, "null context");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "no gauge field set");
+    c->require_single_config("the force, gauge and HMC entry points");
+    c->bind_rhs(1);
 }
 
 }  // namespace

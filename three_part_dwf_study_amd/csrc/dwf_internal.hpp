@@ -79,7 +79,9 @@ enum Epi : int {
 template <typename R> struct HopArgs {
     Lat L;
     HopCfg cfg;
-    const void* links;   // [2][V4] complex R: (cos theta, sin theta)
+    const void* links;   // [ncfg][2][V4] complex R: (cos theta, sin theta)
+    long long link_stride;  // complex entries per configuration (2 V4)
+    int rhs_per_cfg;     // right-hand side r uses configuration r / rhs_per_cfg
     const R* in;         // [nrhs][V5] spinors of 4 R
     R* out;              // EPI_STORE / EPI_NORM
     R* x;                // EPI_CGUPD
@@ -98,7 +100,7 @@ template <typename R> struct HopArgs {
 };
 
 // Host launchers (dwf_device.hip).
-void launch_pack_links(const Lat& L, const double* theta, double2* l64, float2* l32, hipStream_t s);
+void launch_pack_links(long long nlinks, const double* theta, double2* l64, float2* l32, hipStream_t s);
 template <typename R>
 void launch_hop(bool dagger, int epi, const HopArgs<R>& a, int nrhs, hipStream_t s);
 template <typename R>

@@ -46,12 +46,14 @@ __device__ __forceinline__ double bsum(double v, double* sh)
 // The 4 wall / 4 point right-hand sides are ordered (t_src=0, ss=0), (0, 1),
 // (Nt/2, 0), (Nt/2, 1).  Noise: src = P_L eta at s=0, + P_R eta at s=Ls-1 (:60-75).
 __global__ void k_sources(const int Nt, const int Nx, const int Ls, const int nwall, const int npoint,
-                          const int nnoise, const double* noise, double* src)
+                          const int nnoise, const double* noise_all, double* src)
 {
     const int V4 = Nt * Nx;
     const long long V5 = (long long)V4 * Ls;
-    const int rhs = blockIdx.y;
-    double4* out = reinterpret_cast<double4*>(src) + rhs * V5;
+    const int K = nwall + npoint + nnoise;       // right-hand sides per configuration
+    const int cfg = blockIdx.y / K, rhs = blockIdx.y - cfg * K;
+    const double* noise = noise_all ? noise_all + (long long)cfg * nnoise * V4 * 4 : nullptr;
+    double4* out = reinterpret_cast<double4*>(src) + (long long)blockIdx.y * V5;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V4; i += gridDim.x * blockDim.x) {
         const int t = i / Nx, x = i - t * Nx;
         double4 lo = make_double4(0, 0, 0, 0), hi = make_double4(0, 0, 0, 0);  // s = 0, s = Ls-1
@@ -84,14 +86,16 @@ __global__ void k_sources(const int Nt, const int Nx, const int Ls, const int nw
 
 // C_PP / C_J5 of the wall solutions (correlator.hpp:258-300): block per (t, source time),
 // outputs per source time k: pp[k][t], j5[k][t] summed over x and source spin.
-__global__ void k_contract_wall(const int Nt, const int Nx, const int Ls, const double* X, double* pp,
-                                double* j5)
+__global__ void k_contract_wall(const int Nt, const int Nx, const int Ls, const int K, const double* X, double* pp_all,
+                                double* j5_all)
 {
     __shared__ double sh[8];
-    const int t = blockIdx.x, k = blockIdx.y;
+    const int t = blockIdx.x, k = blockIdx.y, cfg = blockIdx.z;
     const int V4 = Nt * Nx;
     const long long V5 = (long long)V4 * Ls;
-    const double4* P = reinterpret_cast<const double4*>(X);
+    const double4* P = reinterpret_cast<const double4*>(X) + (long long)cfg * K * V5;
+    double* pp = pp_all + (long long)cfg * 2 * Nt;
+    double* j5 = j5_all + (long long)cfg * 2 * Nt;
     double a = 0.0, b = 0.0;
     for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
         const int i = t * Nx + x;
@@ -113,14 +117,15 @@ __global__ void k_contract_wall(const int Nt, const int Nx, const int Ls, const 
 
 // Momentum-projected point correlator (correlator.hpp:149-174): block per (t, source
 // time); out[k][n][t] = sum_x cos(2 pi n x / Nx) sum_ss |q|^2.
-__global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const int nmom, const double* X,
-                                 double* out)
+__global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const int nmom, const int K, const double* X,
+                                 double* out_all)
 {
     __shared__ double sh[8];
-    const int t = blockIdx.x, k = blockIdx.y;
+    const int t = blockIdx.x, k = blockIdx.y, cfg = blockIdx.z;
     const int V4 = Nt * Nx;
     const long long V5 = (long long)V4 * Ls;
-    const double4* P = reinterpret_cast<const double4*>(X);
+    const double4* P = reinterpret_cast<const double4*>(X) + (long long)cfg * K * V5;
+    double* out = out_all + (long long)cfg * 2 * nmom * Nt;
     const double pi = 3.141592653589793238462643383279502884;
     for (int n = 0; n < nmom; ++n) {
         const double p = 2.0 * pi * n / Nx;
@@ -141,12 +146,15 @@ __global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const
 }
 
 // Loop density (correlator.hpp:83-107): L(x) = (1/N) sum_noise eta(x)^dag sigma3 q(x).
-__global__ void k_contract_loop(const int Nt, const int Nx, const int Ls, const int nnoise, const double* noise,
-                                const double* X, double* loop)
+__global__ void k_contract_loop(const int Nt, const int Nx, const int Ls, const int nnoise, const int K,
+                                const double* noise_all, const double* X, double* loop_all)
 {
     const int V4 = Nt * Nx;
     const long long V5 = (long long)V4 * Ls;
-    const double4* P = reinterpret_cast<const double4*>(X);
+    const int cfg = blockIdx.y;
+    const double4* P = reinterpret_cast<const double4*>(X) + (long long)cfg * K * V5;
+    const double* noise = noise_all + (long long)cfg * nnoise * V4 * 4;
+    double* loop = loop_all + (long long)cfg * V4 * 2;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V4; i += gridDim.x * blockDim.x) {
         double lr = 0.0, li = 0.0;
         for (int n = 0; n < nnoise; ++n) {
@@ -195,103 +203,141 @@ int guard(const std::function<int()>& f)
 
 }  // namespace
 
+namespace {
+
+int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsigned long long* seeds,
+                 dwf_measure_out* outs)
+{
+    if (!c || !spec || !outs) throw DwfError(DWF_ERR_ARG, "dwf_measure: null argument");
+    if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_measure: no gauge field set");
+    if (c->kind != 0) throw DwfError(DWF_ERR_ARG, "dwf_measure: needs a domain-wall context");
+    if (ncfg != c->ncfg) throw DwfError(DWF_ERR_ARG, "dwf_measure: ncfg differs from the configurations held");
+    const Lat& L = c->L;
+    const int Nt = L.Nt, Nx = L.Nx, Ls = L.Ls, V4 = L.V4;
+    const int nwall = spec->wall ? 4 : 0;
+    const int npoint = spec->point ? 4 : 0;
+    const int nnoise = spec->noise_vectors;
+    if (nnoise < 0) throw DwfError(DWF_ERR_ARG, "eta.noise_vectors must be positive");
+    if (spec->point && (spec->max_momentum < 0 || spec->max_momentum > Nx / 2))
+        throw DwfError(DWF_ERR_ARG, "analysis.max_momentum is outside 0..Nx/2");
+    for (int k = 0; k < ncfg; ++k) {
+        const dwf_measure_out& o = outs[k];
+        if ((spec->wall && (!o.CPP || !o.CJ5)) || (spec->point && !o.momentum_CPP) || (nnoise > 0 && !o.loop))
+            throw DwfError(DWF_ERR_ARG, "dwf_measure: missing output buffer");
+    }
+    if (spec->maxiter < 0 || (spec->method != DWF_SOLVE_CG64 && spec->method != DWF_SOLVE_MIXED &&
+                              spec->method != DWF_SOLVE_EO_MIXED))
+        throw DwfError(DWF_ERR_ARG, "dwf_measure: bad argument");
+    const int K = nwall + npoint + nnoise;  // right-hand sides per configuration
+    for (int k = 0; k < ncfg; ++k) outs[k].wall_nonconverged = 0;
+    if (K == 0) return DWF_OK;
+    const int nrhs = K * ncfg;
+    c->bind_rhs(nrhs);
+    const long long V5 = L.V5();
+    const size_t nv = (size_t)V5 * 4;  // doubles per spinor
+    hipStream_t s = c->stream;
+
+    // Z4 noise from the host stream of correlator.hpp:35-58, one engine per configuration
+    double* dnoise = nullptr;
+    if (nnoise > 0) {
+        const size_t per = (size_t)nnoise * V4 * 4;
+        std::vector<double> h(per * ncfg);
+        for (int k = 0; k < ncfg; ++k) dwf_z4_noise(seeds ? seeds[k] : spec->noise_seed, nnoise, V4 * 2, h.data() + per * k);
+        dnoise = (double*)c->buf(B_AUX3, h.size() * 8);
+        c->check(hipMemcpyAsync(dnoise, h.data(), h.size() * 8, hipMemcpyHostToDevice, s), "noise upload");
+        c->check(hipStreamSynchronize(s), "noise upload");
+    }
+    double* src = (double*)c->buf(B_AUX0, nv * nrhs * 8);
+    double* X = (double*)c->buf(B_AUX2, nv * nrhs * 8);
+    k_sources<<<dim3((V4 + 255) / 256, nrhs), 256, 0, s>>>(Nt, Nx, Ls, nwall, npoint, nnoise, dnoise, src);
+    c->check_launch("sources");
+    std::vector<dwf_solve_stats> stv(nrhs);
+    const int rc = dwf_solve_dirac_impl(c, spec->method, src, nullptr, spec->rtol, spec->maxiter, nrhs, X, stv.data());
+    if (rc == DWF_ERR_ZERO_DENOM) throw DwfError(DWF_ERR_ZERO_DENOM, "CG encountered a zero denominator");
+    for (int k = 0; k < ncfg; ++k) {
+        for (int j = 0; j < K; ++j) {
+            const dwf_solve_stats& st = stv[(size_t)k * K + j];
+            if (outs[k].iterations) outs[k].iterations[j] = st.iterations + (spec->method == DWF_SOLVE_CG64 ? 0 : st.fp64_iterations);
+            if (j >= nwall && j < nwall + npoint && !st.converged)
+                throw DwfError(DWF_ERR_NOT_CONVERGED, "Momentum propagator CGNR failed to converge");
+            if (j >= nwall + npoint && !st.converged)
+                throw DwfError(DWF_ERR_NOT_CONVERGED, "Eta loop CGNR failed to converge");
+            if (j < nwall && !st.converged) outs[k].wall_nonconverged += 1;  // "CGNR info = nonzero" (:222-223)
+        }
+    }
+    double* res = (double*)c->buf(B_SCAL, (size_t)std::max(64, 8 * ncfg) * 8);
+    if (nwall) {
+        // Dirac residual ||D x - src|| / ||src|| of the wall solves (:225-227)
+        double* dx = (double*)c->buf(B_APPLY_TMP, nv * nrhs * 8);
+        double* diff = (double*)c->buf(B_T64, nv * nrhs * 8);
+        dwf_apply_device_impl(c, DWF_OP_D, DWF_PREC_FP64, X, dx, nrhs);
+        k_sub<<<1024, 256, 0, s>>>(dx, src, diff, (long long)nv * nrhs);
+        std::vector<double> hr(2 * ncfg * 4);
+        double* nd = (double*)c->buf(B_AUX1, (size_t)2 * nrhs * 8);
+        k_norms<<<nrhs, 256, 0, s>>>(diff, (long long)nv, nd);
+        k_norms<<<nrhs, 256, 0, s>>>(src, (long long)nv, nd + nrhs);
+        double* pp = (double*)c->buf(B_PA, (size_t)2 * Nt * ncfg * 8);
+        double* j5 = (double*)c->buf(B_PB, (size_t)2 * Nt * ncfg * 8);
+        k_contract_wall<<<dim3(Nt, 2, ncfg), 256, 0, s>>>(Nt, Nx, Ls, K, X, pp, j5);
+        c->check_launch("wall contraction");
+        std::vector<double> hpp((size_t)2 * Nt * ncfg), hj5((size_t)2 * Nt * ncfg), hn((size_t)2 * nrhs);
+        c->check(hipMemcpyAsync(hpp.data(), pp, hpp.size() * 8, hipMemcpyDeviceToHost, s), "readback");
+        c->check(hipMemcpyAsync(hj5.data(), j5, hj5.size() * 8, hipMemcpyDeviceToHost, s), "readback");
+        c->check(hipMemcpyAsync(hn.data(), nd, hn.size() * 8, hipMemcpyDeviceToHost, s), "readback");
+        c->check(hipStreamSynchronize(s), "readback");
+        for (int k = 0; k < ncfg; ++k) {
+            const double* a = hpp.data() + (size_t)k * 2 * Nt;
+            const double* b = hj5.data() + (size_t)k * 2 * Nt;
+            for (int dt = 0; dt < Nt; ++dt) {
+                outs[k].CPP[dt] = (a[dt] + a[Nt + (dt + Nt / 2) % Nt]) / 2.0;
+                outs[k].CJ5[dt] = (b[dt] + b[Nt + (dt + Nt / 2) % Nt]) / 2.0;
+            }
+            if (outs[k].dirac_residual)
+                for (int j = 0; j < nwall; ++j)
+                    outs[k].dirac_residual[j] = std::sqrt(hn[(size_t)k * K + j]) / std::sqrt(hn[nrhs + (size_t)k * K + j]);
+        }
+    }
+    if (npoint) {
+        const int nmom = spec->max_momentum + 1;
+        double* pm = (double*)c->buf(B_PR, (size_t)2 * nmom * Nt * ncfg * 8);
+        k_contract_point<<<dim3(Nt, 2, ncfg), 256, 0, s>>>(Nt, Nx, Ls, nmom, K, X + nv * nwall, pm);
+        c->check_launch("point contraction");
+        std::vector<double> h((size_t)2 * nmom * Nt * ncfg);
+        c->check(hipMemcpyAsync(h.data(), pm, h.size() * 8, hipMemcpyDeviceToHost, s), "readback");
+        c->check(hipStreamSynchronize(s), "readback");
+        for (int k = 0; k < ncfg; ++k) {
+            const double* a = h.data() + (size_t)k * 2 * nmom * Nt;
+            for (int n = 0; n < nmom; ++n)
+                for (int dt = 0; dt < Nt; ++dt)
+                    outs[k].momentum_CPP[n * Nt + dt] = (a[n * Nt + dt] + a[(nmom + n) * Nt + (dt + Nt / 2) % Nt]) / 2.0;
+        }
+    }
+    if (nnoise) {
+        double* lp = (double*)c->buf(B_PR, (size_t)V4 * 2 * ncfg * 8);
+        k_contract_loop<<<dim3((V4 + 255) / 256, ncfg), 256, 0, s>>>(Nt, Nx, Ls, nnoise, K, dnoise,
+                                                                    X + nv * (nwall + npoint), lp);
+        c->check_launch("loop contraction");
+        for (int k = 0; k < ncfg; ++k)
+            c->check(hipMemcpyAsync(outs[k].loop, lp + (size_t)k * V4 * 2, (size_t)V4 * 2 * 8, hipMemcpyDeviceToHost, s),
+                     "readback");
+        c->check(hipStreamSynchronize(s), "readback");
+    }
+    (void)res;
+    return DWF_OK;
+}
+
+}  // namespace
+
 extern "C" int dwf_measure(dwf_ctx* c, const dwf_measure_spec* spec, dwf_measure_out* out)
 {
     return guard([&]() -> int {
-        if (!c || !spec || !out) throw DwfError(DWF_ERR_ARG, "dwf_measure: null argument");
-        if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_measure: no gauge field set");
-        if (c->kind != 0) throw DwfError(DWF_ERR_ARG, "dwf_measure: needs a domain-wall context");
-        const Lat& L = c->L;
-        const int Nt = L.Nt, Nx = L.Nx, Ls = L.Ls, V4 = L.V4;
-        const int nwall = spec->wall ? 4 : 0;
-        const int npoint = spec->point ? 4 : 0;
-        const int nnoise = spec->noise_vectors;
-        if (nnoise < 0) throw DwfError(DWF_ERR_ARG, "eta.noise_vectors must be positive");
-        if (spec->point && (spec->max_momentum < 0 || spec->max_momentum > Nx / 2))
-            throw DwfError(DWF_ERR_ARG, "analysis.max_momentum is outside 0..Nx/2");
-        if ((spec->wall && (!out->CPP || !out->CJ5)) || (spec->point && !out->momentum_CPP) ||
-            (nnoise > 0 && !out->loop) || spec->maxiter < 0 ||
-            (spec->method != DWF_SOLVE_CG64 && spec->method != DWF_SOLVE_MIXED))
-            throw DwfError(DWF_ERR_ARG, "dwf_measure: bad argument");
-        const int nrhs = nwall + npoint + nnoise;
-        out->wall_nonconverged = 0;
-        if (nrhs == 0) return DWF_OK;
-        const long long V5 = L.V5();
-        const size_t nv = (size_t)V5 * 4;  // doubles per spinor
-        hipStream_t s = c->stream;
-
-        // Z4 noise from the host stream of correlator.hpp:35-58 (seed per configuration)
-        double* dnoise = nullptr;
-        if (nnoise > 0) {
-            std::vector<double> h((size_t)nnoise * V4 * 4);
-            dwf_z4_noise(spec->noise_seed, nnoise, V4 * 2, h.data());
-            dnoise = (double*)c->buf(B_AUX3, h.size() * 8);
-            c->check(hipMemcpyAsync(dnoise, h.data(), h.size() * 8, hipMemcpyHostToDevice, s), "noise upload");
-            c->check(hipStreamSynchronize(s), "noise upload");
-        }
-        double* src = (double*)c->buf(B_AUX0, nv * nrhs * 8);
-        double* X = (double*)c->buf(B_AUX2, nv * nrhs * 8);
-        k_sources<<<dim3((V4 + 255) / 256, nrhs), 256, 0, s>>>(Nt, Nx, Ls, nwall, npoint, nnoise, dnoise, src);
-        c->check_launch("sources");
-        std::vector<dwf_solve_stats> stv(nrhs);
-        const int rc = dwf_solve_dirac_impl(c, spec->method, src, nullptr, spec->rtol, spec->maxiter, nrhs, X,
-                                            stv.data());
-        if (rc == DWF_ERR_ZERO_DENOM) throw DwfError(DWF_ERR_ZERO_DENOM, "CG encountered a zero denominator");
-        for (int k = 0; k < nrhs; ++k)
-            if (out->iterations) out->iterations[k] = stv[k].iterations + (spec->method == DWF_SOLVE_CG64 ? 0 : stv[k].fp64_iterations);
-        auto converged = [&](int k) { return stv[k].converged != 0; };
-        for (int k = nwall; k < nwall + npoint; ++k)
-            if (!converged(k)) throw DwfError(DWF_ERR_NOT_CONVERGED, "Momentum propagator CGNR failed to converge");
-        for (int k = nwall + npoint; k < nrhs; ++k)
-            if (!converged(k)) throw DwfError(DWF_ERR_NOT_CONVERGED, "Eta loop CGNR failed to converge");
-
-        double* res = (double*)c->buf(B_SCAL, 64 * 8);
-        if (nwall) {
-            for (int k = 0; k < nwall; ++k)
-                if (!converged(k)) out->wall_nonconverged += 1;  // "CGNR info = nonzero" (:222-223)
-            // Dirac residual ||D x - src|| / ||src|| of the wall solves (:225-227)
-            double* dx = (double*)c->buf(B_APPLY_TMP, nv * nwall * 8);
-            double* diff = (double*)c->buf(B_T64, nv * nwall * 8);
-            dwf_apply_device_impl(c, DWF_OP_D, DWF_PREC_FP64, X, dx, nwall);
-            k_sub<<<1024, 256, 0, s>>>(dx, src, diff, (long long)nv * nwall);
-            k_norms<<<nwall, 256, 0, s>>>(diff, (long long)nv, res);
-            k_norms<<<nwall, 256, 0, s>>>(src, (long long)nv, res + 8);
-            double* pp = (double*)c->buf(B_PA, 4 * Nt * 8);
-            double* j5 = (double*)c->buf(B_PB, 4 * Nt * 8);
-            k_contract_wall<<<dim3(Nt, 2), 256, 0, s>>>(Nt, Nx, Ls, X, pp, j5);
-            c->check_launch("wall contraction");
-            std::vector<double> hpp(2 * Nt), hj5(2 * Nt), hr(16);
-            c->check(hipMemcpyAsync(hpp.data(), pp, 2 * Nt * 8, hipMemcpyDeviceToHost, s), "readback");
-            c->check(hipMemcpyAsync(hj5.data(), j5, 2 * Nt * 8, hipMemcpyDeviceToHost, s), "readback");
-            c->check(hipMemcpyAsync(hr.data(), res, 16 * 8, hipMemcpyDeviceToHost, s), "readback");
-            c->check(hipStreamSynchronize(s), "readback");
-            for (int dt = 0; dt < Nt; ++dt) {
-                out->CPP[dt] = (hpp[dt] + hpp[Nt + (dt + Nt / 2) % Nt]) / 2.0;
-                out->CJ5[dt] = (hj5[dt] + hj5[Nt + (dt + Nt / 2) % Nt]) / 2.0;
-            }
-            if (out->dirac_residual)
-                for (int k = 0; k < nwall; ++k) out->dirac_residual[k] = std::sqrt(hr[k]) / std::sqrt(hr[8 + k]);
-        }
-        if (npoint) {
-            const int nmom = spec->max_momentum + 1;
-            double* pm = (double*)c->buf(B_PR, 2 * nmom * Nt * 8);
-            k_contract_point<<<dim3(Nt, 2), 256, 0, s>>>(Nt, Nx, Ls, nmom, X + nv * nwall, pm);
-            c->check_launch("point contraction");
-            std::vector<double> h(2 * nmom * Nt);
-            c->check(hipMemcpyAsync(h.data(), pm, h.size() * 8, hipMemcpyDeviceToHost, s), "readback");
-            c->check(hipStreamSynchronize(s), "readback");
-            for (int n = 0; n < nmom; ++n)
-                for (int dt = 0; dt < Nt; ++dt)
-                    out->momentum_CPP[n * Nt + dt] =
-                        (h[n * Nt + dt] + h[(nmom + n) * Nt + (dt + Nt / 2) % Nt]) / 2.0;
-        }
-        if (nnoise) {
-            double* lp = (double*)c->buf(B_PR, (size_t)V4 * 2 * 8 + 2 * 64 * Nt * 8);
-            k_contract_loop<<<(V4 + 255) / 256, 256, 0, s>>>(Nt, Nx, Ls, nnoise, dnoise, X + nv * (nwall + npoint), lp);
-            c->check_launch("loop contraction");
-            c->check(hipMemcpyAsync(out->loop, lp, (size_t)V4 * 2 * 8, hipMemcpyDeviceToHost, s), "readback");
-            c->check(hipStreamSynchronize(s), "readback");
-        }
-        return DWF_OK;
+        if (c && c->ncfg != 1) throw DwfError(DWF_ERR_ARG, "dwf_measure: the context holds several configurations; use dwf_measure_batch");
+        return measure_impl(c, spec, 1, nullptr, out);
     });
+}
+
+extern "C" int dwf_measure_batch(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsigned long long* seeds,
+                                 dwf_measure_out* outs)
+{
+    return guard([&]() -> int { return measure_impl(c, spec, ncfg, seeds, outs); });
 }

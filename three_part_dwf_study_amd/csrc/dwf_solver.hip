@@ -32,6 +32,8 @@ template <typename R> HopArgs<R> hop_base(const dwf_ctx* c, const HopCfg& cfg)
     a.L = c->L;
     a.cfg = cfg;
     a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
+    a.link_stride = 2LL * c->L.V4;
+    a.rhs_per_cfg = c->rhs_per_cfg;
     return a;
 }
 
@@ -167,6 +169,8 @@ struct SchurOp : CgOperator {
         a.remap = c->xcd_remap;
         a.tpar = tpar;
         a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
+        a.link_stride = 2LL * c->L.V4;
+        a.rhs_per_cfg = c->rhs_per_cfg;
         return a;
     }
     // out_o = [Dhat | Dhat^dag] in_o with epilogue `epi` (args in the returned struct)
@@ -431,6 +435,7 @@ int solve_normal(dwf_ctx* c, int normal, int method, const double* b, const doub
         (normal != DWF_OP_DDDAG && normal != DWF_OP_DDAGD) || (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED))
         throw DwfError(DWF_ERR_ARG, "dwf_solve: bad argument");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve: no gauge field set");
+    c->bind_rhs(nrhs);
     dwf_solve_device_impl(c, normal, method, b, x0, rtol, maxiter, nrhs, x);
     int rc = collect(c, method, nrhs, stats);
     if (method == DWF_SOLVE_MIXED && rc == DWF_ERR_NOT_CONVERGED) {
@@ -589,6 +594,7 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
         (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_EO_MIXED))
         throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: bad argument");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: no gauge field set");
+    c->bind_rhs(nrhs);
     if (method == DWF_SOLVE_EO_MIXED && eo_supported(c->L)) {
         const int rc = solve_dirac_eo(c, src, x0, rtol, maxiter, nrhs, x, stats);
         if (rc != DWF_ERR_NOT_CONVERGED) return rc;
