@@ -14,7 +14,15 @@ FP32 Dirac stencil, measured with HIP events on the library's stream over a batc
 128 right-hand sides (604 MB, larger than the 256 MB MALL).  The CPU baseline is the
 oracle's FP64 CG (the reference's algorithm and loop order) on one host core.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+With --workload analyzer a step is instead the batched analyzer pass: 128 seeded
+configurations, 32 each at Nx = Nt in {32, 56, 80, 96} with Ls=16, each with 2 point
+sources (one per spin) at a seeded site and 32 Z4 noise vectors -- 4,352 propagator
+solves (FP32 CG + FP64 reliable updates on the even-odd Schur complement, FP64 true
+residual <= 1e-10) and their FP64 contractions (point C_PP, C_J5, loop density).  The
+four lattice sizes run concurrently, one context and stream each; each size's 1,088
+right-hand sides are one batched solve.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload solves|analyzer]
 """
 
 import argparse
@@ -119,6 +127,129 @@ def cpu_baseline(theta, b1, judge_iters, sample_iters):
     }
 
 
+ANALYZER_SIZES = (32, 56, 80, 96)
+
+
+def analyzer_inputs(per_size):
+    """Per lattice size: thetas [per][2*N*N] ~ N(0, 1/4) with seed 1234 + 104729 i, point
+    sites drawn from seed 555 + i, Z4 seeds 918273 + 104729 i (i = global config index)."""
+    out = []
+    for g, N in enumerate(ANALYZER_SIZES):
+        idx = [g * per_size + k for k in range(per_size)]
+        thetas = np.stack([np.random.default_rng(1234 + 104729 * i).normal(0.0, 1.0 / np.sqrt(BETA), 2 * N * N)
+                           for i in idx])
+        sites = np.array([np.random.default_rng(555 + i).integers(0, N, size=2) for i in idx], np.int32)
+        seeds = [918273 + 104729 * i for i in idx]
+        out.append((N, thetas, sites, seeds))
+    return out
+
+
+def analyzer_cpu_baseline(inputs, sample_iters):
+    """Oracle FP64 CG (the reference's algorithm, 1 core): one point-source solve to
+    1e-10 at the smallest size gives the iteration count; `sample_iters` iterations per
+    size give the time per iteration; the whole job is scaled from those."""
+    import oracle
+
+    total, parts = 0.0, []
+    iters = None
+    for N, thetas, sites, _ in inputs:
+        src = np.zeros((LS, N, N, 2), np.complex128)
+        t0_, x0_ = int(sites[0][0]), int(sites[0][1])
+        src[LS - 1, t0_, x0_, 0] = 1.0  # P_R e_0 at s = Ls-1
+        b = oracle.dwf_apply(1, thetas[0], src.reshape(-1), N, N, LS, M5, MF)
+        if iters is None:
+            t0 = time.perf_counter()
+            _, iters, _, _, _ = oracle.dwf_cg(3, thetas[0], b, N, N, LS, M5, MF, TOL, 100000)
+            parts.append(f"N={N}: full solve {iters} iterations in {time.perf_counter() - t0:.1f} s")
+        t0 = time.perf_counter()
+        _, it, _, _, _ = oracle.dwf_cg(3, thetas[0], b, N, N, LS, M5, MF, TOL, sample_iters)
+        per_iter = (time.perf_counter() - t0) / max(1, it)
+        total += len(thetas) * 34 * iters * per_iter
+        parts.append(f"N={N}: {per_iter * 1e3:.2f} ms/iteration")
+    nsolves = sum(len(t) * 34 for _, t, _, _ in inputs)
+    return {
+        "value": nsolves / total,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": ("oracle FP64 CG (reference algorithm, 1 core), contractions excluded; "
+                   + "; ".join(parts)
+                   + f"; every solve assumed to take the {iters} iterations of the full one"),
+    }
+
+
+def run_analyzer(args, dwf):
+    import threading
+
+    per = args.analyzer_configs
+    inputs = analyzer_inputs(per)
+    ctxs = [dwf.DwfContext.dwf(N, N, LS, M5, MF) for N, _, _, _ in inputs]
+    method = dwf.DWF_SOLVE_EO_MIXED
+
+    def one(g, out):
+        N, thetas, sites, seeds = inputs[g]
+        t0 = time.perf_counter()
+        ctxs[g].set_gauge_batch(thetas)
+        m = ctxs[g].measure_batch(per, TOL, 20000, wall=False, point=True, max_momentum=0, noise_vectors=32,
+                                  seeds=seeds, method=method, point_sites=sites)
+        out[g] = (time.perf_counter() - t0, m)
+
+    def step():
+        out = [None] * len(inputs)
+        th = [threading.Thread(target=one, args=(g, out)) for g in range(len(inputs))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if any(o is None for o in out):
+            raise SystemExit("analyzer step failed")
+        return out
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    dt = time.perf_counter() - t0
+    nsolves = per * 34 * len(inputs)
+    groups = {}
+    for (N, _, _, _), (sec, m) in zip(inputs, out):
+        its = np.concatenate([d["iterations"] for d in m])
+        groups[f"N{N}"] = {"seconds": round(sec, 4), "solves": per * 34,
+                           "mean_iterations": round(float(its.mean()), 1), "max_iterations": int(its.max())}
+        log(f"N={N}: {groups[f'N{N}']}")
+    cpu = None if args.no_cpu else analyzer_cpu_baseline(inputs, args.cpu_iters)
+    if cpu:
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "DWF analyzer propagator solves/s (batched, 4 lattice sizes) @1e-10",
+        "value": round(nsolves * args.steps / dt, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched analyzer: {per * len(inputs)} configurations, {per} each at Nx=Nt in "
+                        f"{list(ANALYZER_SIZES)}, Ls=16, theta~N(0,1/4); per configuration 2 point sources at a "
+                        "seeded site + 32 Z4 noise vectors; point C_PP, C_J5 and loop density in FP64",
+            "solves_per_step": nsolves,
+            "solver": "even-odd Schur complement, FP32 CGNR + FP64 reliable updates, certified at 1e-10",
+        },
+        "groups": groups,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    for c in ctxs:
+        c.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +259,8 @@ def main():
     ap.add_argument("--dslash-rhs", type=int, default=128)
     ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("solves", "analyzer"), default="solves")
+    ap.add_argument("--analyzer-configs", type=int, default=32, help="configurations per lattice size")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the roofline launches (batched FP32 D apply and the copy kernel), for counter runs")
     args = ap.parse_args()
@@ -139,6 +272,9 @@ def main():
     import three_part_dwf_study_amd as dwf
     from three_part_dwf_study_amd import capi
 
+    if args.workload == "analyzer":
+        run_analyzer(args, dwf)
+        return
     rng = np.random.default_rng(1234)
     theta = rng.normal(0.0, 1.0 / np.sqrt(BETA), size=2 * NT * NX)
     if args.kernels_only:

@@ -100,14 +100,17 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * of the mixed solver; DWF_TUNE_PREFETCH: 0/1 software-pipelined stencil slice loop;
  * DWF_TUNE_TILE_X: 0 for blocks of consecutive 4D sites, or the width (dividing Nx and
  * 256) of 2D stencil tiles of 256/width rows; DWF_TUNE_NT_STORE: 0/1 streaming
- * (non-temporal) stores of stencil outputs. */
+ * (non-temporal) stores of stencil outputs; DWF_TUNE_GRAPHS: 0/1 (default 1) replay
+ * blocks of 8 CG iterations as instantiated HIP graphs instead of launching their
+ * kernels one by one (same kernels, same order, same results). */
 enum dwf_tune_param {
     DWF_TUNE_SCHUNKS = 0,
     DWF_TUNE_XCD_REMAP = 1,
     DWF_TUNE_RELIABLE_DELTA = 2,
     DWF_TUNE_PREFETCH = 3,
     DWF_TUNE_TILE_X = 4,
-    DWF_TUNE_NT_STORE = 5
+    DWF_TUNE_NT_STORE = 5,
+    DWF_TUNE_GRAPHS = 6
 };
 int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 
@@ -258,6 +261,11 @@ typedef struct dwf_measure_spec {
     int max_momentum;              /* analysis.max_momentum (0..Nx/2)                    */
     int noise_vectors;             /* eta.noise_vectors (0 skips the loop density)       */
     unsigned long long noise_seed; /* eta.random_seed + 104729 * config                  */
+    const int* point_site;         /* nullable: the reference's 4 point sources at
+                                      (t0 in {0, Nt/2}, x=0).  Otherwise [ncfg][2] = (t0, x0):
+                                      2 point sources (one per spin) at that site per
+                                      configuration; correlators are relative to it.  No
+                                      reference counterpart (the batched-analyzer workload) */
 } dwf_measure_spec;
 
 typedef struct dwf_measure_out {
@@ -265,10 +273,13 @@ typedef struct dwf_measure_out {
     double* CJ5;             /* [Nt]  C_J5(dt)                                          */
     double* momentum_CPP;    /* [max_momentum+1][Nt] (correlator.hpp:111-181), if point   */
     double* loop;            /* [Nt][Nx] complex L(t,x) (correlator.hpp:27-109), if noise */
-    int* iterations;         /* nullable: [4 wall][4 point][noise] CG iterations           */
+    int* iterations;         /* nullable: [4 wall][4 or 2 point][noise] CG iterations      */
     double* dirac_residual;  /* nullable: [4] ||D x - src|| / ||src|| of the wall solves  */
     int wall_nonconverged;   /* wall solves that missed rtol ("CGNR info = nonzero")      */
     int pad;
+    double* point_CJ5;       /* nullable: [Nt] C_J5(dt) of the point sources (midpoint
+                                s = Ls/2 as for the wall sources), averaged like
+                                momentum_CPP; no reference counterpart                  */
 } dwf_measure_out;
 
 /* One configuration's measurements (the loop body of domain_wall/analyze_configs.cpp:57-75:

@@ -11,6 +11,7 @@ from conftest import print_case
 from dense_ref import rand_spinor, rand_theta
 
 import three_part_dwf_study_amd as dwf
+from three_part_dwf_study_amd import capi
 
 pytestmark = pytest.mark.gpu
 
@@ -40,8 +41,8 @@ def test_dwf_apply_batch_per_config(ncfg, per):
                 want = oracle.dwf_apply(op, ths[r // per], psis[r], Nt, Nx, Ls, M5, MF)
                 assert rel(got[r], want) <= 1e-12, (op, r)
         # FP32 device path: same configuration selection
-        a = dwf.DeviceArray.from_host(psis.astype(np.complex128).view(np.float64).astype(np.float32))
-        b = dwf.DeviceArray(a.nbytes)
+        a = capi.DeviceArray.from_host(psis.astype(np.complex128).view(np.float64).astype(np.float32))
+        b = capi.DeviceArray(a.nbytes)
         ctx.apply_device(0, dwf.DWF_PREC_FP32, a.ptr, b.ptr, ncfg * per)
         got32 = b.download(np.float32).astype(np.float64).view(np.complex128).reshape(ncfg * per, -1)
         for r in range(ncfg * per):
@@ -127,3 +128,33 @@ def test_batch_argument_errors():
             ctx.average_plaquette()
         with pytest.raises(dwf.DwfError, match="ncfg differs"):
             ctx.measure_batch(3, 1e-8, 100)
+
+
+def test_point_site_sources():
+    """Point sources at chosen sites (the batched-analyzer workload).  Checks: the two
+    single-time-slice results at t0 = 0 and Nt/2 average to the reference's folded
+    momentum correlator (pinned by the oracle); translating the gauge field in x by x0
+    and moving the source with it leaves every correlator unchanged (x is periodic)."""
+    Nt, Nx, Ls, mm, x0 = 8, 6, 4, 3, 2
+    print_case(test="point_site", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(9)
+    th = rand_theta(rng, Nt, Nx)
+    th_shift = np.roll(th.reshape(2, Nt, Nx), x0, axis=2).reshape(-1)
+    rtol, maxit = 1e-12, 20000
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge_batch(np.stack([th, th, th_shift]))
+        m = ctx.measure_batch(3, rtol, maxit, wall=False, point=True, max_momentum=mm,
+                              point_sites=[[0, 0], [Nt // 2, 0], [Nt // 2, x0]])
+        with pytest.raises(dwf.DwfError, match="outside the lattice"):
+            ctx.measure_batch(3, rtol, maxit, wall=False, point=True, point_sites=[[0, 0], [Nt, 0], [0, 0]])
+        ctx.set_gauge(th)
+        ref = ctx.measure(rtol, maxit, wall=False, point=True, max_momentum=mm)
+    want = oracle.dwf_momentum_cpp(th, Nt, Nx, Ls, M5, MF, mm, rtol, maxit)
+    assert np.allclose(ref["momentum_CPP"], want, rtol=1e-8, atol=1e-11 * np.max(np.abs(want)))
+    avg = (m[0]["momentum_CPP"] + m[1]["momentum_CPP"]) / 2
+    assert np.allclose(avg, want, rtol=1e-8, atol=1e-11 * np.max(np.abs(want)))
+    pj = (m[0]["point_CJ5"] + m[1]["point_CJ5"]) / 2
+    assert np.allclose(pj, ref["point_CJ5"], rtol=1e-9, atol=1e-12 * np.max(np.abs(ref["point_CJ5"])))
+    for key in ("momentum_CPP", "point_CJ5"):
+        scale = np.max(np.abs(m[1][key]))
+        assert np.allclose(m[2][key], m[1][key], rtol=1e-8, atol=1e-10 * scale), key

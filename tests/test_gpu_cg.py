@@ -98,3 +98,27 @@ def test_wilson_cgnr_matches_oracle():
         rhs = ctx.apply(dwf.DWF_OP_DDAG, src)
         x, st, rc = ctx.solve(dwf.DWF_OP_DDAGD, rhs, rtol=1e-12, maxiter=5000)
     assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-9
+
+
+@pytest.mark.parametrize("method", [dwf.DWF_SOLVE_CG64, dwf.DWF_SOLVE_MIXED, dwf.DWF_SOLVE_EO_MIXED])
+def test_graph_replay_matches_eager(method):
+    """CG iteration blocks replayed as HIP graphs give the eager launches' results bit
+    for bit, over repeated calls (cached graphs) and a change of batch size."""
+    Nt, Nx, Ls = 16, 12, 8
+    print_case(test="graph_replay", method=method)
+    rng = np.random.default_rng(123)
+    th = rand_theta(rng, Nt, Nx)
+    src = rand_spinor(rng, 3 * Ls * Nt * Nx * 2).reshape(3, -1)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
+        eager3, st3, _ = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, method=method)
+        eager1, st1, _ = ctx.solve_dirac(src[:1], rtol=1e-10, maxiter=20000, method=method)
+        ctx.tune(dwf.DWF_TUNE_GRAPHS, 1)
+        for rep in range(3):
+            g3, gs3, _ = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, method=method)
+            assert np.array_equal(g3, eager3), rep
+            assert [s["iterations"] for s in gs3] == [s["iterations"] for s in st3]
+            g1, gs1, _ = ctx.solve_dirac(src[:1], rtol=1e-10, maxiter=20000, method=method)
+            assert np.array_equal(g1, eager1), rep
+            assert gs1[0]["iterations"] == st1[0]["iterations"]

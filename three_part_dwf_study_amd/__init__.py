@@ -18,6 +18,7 @@ from .capi import (  # noqa: F401
     DWF_SOLVE_EO_MIXED,
     DWF_SOLVE_MIXED,
     DWF_TUNE_NT_STORE,
+    DWF_TUNE_GRAPHS,
     DWF_TUNE_PREFETCH,
     DWF_TUNE_RELIABLE_DELTA,
     DWF_TUNE_SCHUNKS,

@@ -44,6 +44,7 @@ DWF_TUNE_RELIABLE_DELTA = 2
 DWF_TUNE_PREFETCH = 3
 DWF_TUNE_TILE_X = 4
 DWF_TUNE_NT_STORE = 5
+DWF_TUNE_GRAPHS = 6
 
 
 class DwfError(RuntimeError):
@@ -124,6 +125,7 @@ class MeasureSpec(ctypes.Structure):
         ("max_momentum", ctypes.c_int),
         ("noise_vectors", ctypes.c_int),
         ("noise_seed", ctypes.c_ulonglong),
+        ("point_site", ctypes.c_void_p),
     ]
 
 
@@ -137,6 +139,7 @@ class MeasureOut(ctypes.Structure):
         ("dirac_residual", ctypes.c_void_p),
         ("wall_nonconverged", ctypes.c_int),
         ("pad", ctypes.c_int),
+        ("point_CJ5", ctypes.c_void_p),
     ]
 
 
@@ -464,29 +467,35 @@ class DwfContext:
         mom = np.zeros((max_momentum + 1) * Nt)
         loop = np.zeros(Nt * Nx, np.complex128)
         iters = np.zeros(max(nrhs, 1), np.int32)
-        dres = np.zeros(4)
-        spec = MeasureSpec(method, maxiter, rtol, int(wall), int(point), max_momentum, noise_vectors, noise_seed)
-        out = MeasureOut(_ptr(cpp), _ptr(cj5), _ptr(mom), _ptr(loop), _ptr(iters), _ptr(dres), 0, 0)
+        dres, pj5 = np.zeros(4), np.zeros(Nt)
+        spec = MeasureSpec(method, maxiter, rtol, int(wall), int(point), max_momentum, noise_vectors, noise_seed,
+                           None)
+        out = MeasureOut(_ptr(cpp), _ptr(cj5), _ptr(mom), _ptr(loop), _ptr(iters), _ptr(dres), 0, 0, _ptr(pj5))
         _check(self._lib, self._lib.dwf_measure(self._h, ctypes.byref(spec), ctypes.byref(out)), "dwf_measure")
         return dict(CPP=cpp, CJ5=cj5, momentum_CPP=mom.reshape(max_momentum + 1, Nt), loop=loop.reshape(Nt, Nx),
-                    iterations=iters[:nrhs], dirac_residual=dres, wall_nonconverged=out.wall_nonconverged)
+                    iterations=iters[:nrhs], dirac_residual=dres, wall_nonconverged=out.wall_nonconverged,
+                    point_CJ5=pj5)
 
     def measure_batch(self, ncfg: int, rtol: float, maxiter: int, wall: bool = True, point: bool = True,
                       max_momentum: int = 0, noise_vectors: int = 0, seeds=None,
-                      method: int = DWF_SOLVE_CG64) -> list:
-        """dwf_measure for the ncfg configurations held (set_gauge_batch); one dict each."""
+                      method: int = DWF_SOLVE_CG64, point_sites=None) -> list:
+        """dwf_measure for the ncfg configurations held (set_gauge_batch); one dict each.
+        point_sites: None (the reference's 4 point sources) or [ncfg][2] = (t0, x0)."""
         Nt, Nx = self.geom.Nt, self.geom.Nx
-        K = (4 if wall else 0) + (4 if point else 0) + noise_vectors
+        npoint = 0 if not point else (4 if point_sites is None else 2)
+        K = (4 if wall else 0) + npoint + noise_vectors
         keep, outs = [], (MeasureOut * ncfg)()
         for k in range(ncfg):
             d = dict(CPP=np.zeros(Nt), CJ5=np.zeros(Nt), momentum_CPP=np.zeros((max_momentum + 1) * Nt),
                      loop=np.zeros(Nt * Nx, np.complex128), iterations=np.zeros(max(K, 1), np.int32),
-                     dirac_residual=np.zeros(4))
+                     dirac_residual=np.zeros(4), point_CJ5=np.zeros(Nt))
             keep.append(d)
             outs[k] = MeasureOut(*(_ptr(d[n]) for n in ("CPP", "CJ5", "momentum_CPP", "loop", "iterations",
-                                                        "dirac_residual")), 0, 0)
+                                                        "dirac_residual")), 0, 0, _ptr(d["point_CJ5"]))
         sd = None if seeds is None else np.ascontiguousarray(seeds, dtype=np.uint64)
-        spec = MeasureSpec(method, maxiter, rtol, int(wall), int(point), max_momentum, noise_vectors, 0)
+        ps = None if point_sites is None else np.ascontiguousarray(point_sites, dtype=np.int32).reshape(-1)
+        spec = MeasureSpec(method, maxiter, rtol, int(wall), int(point), max_momentum, noise_vectors, 0,
+                           None if ps is None else _ptr(ps))
         _check(self._lib, self._lib.dwf_measure_batch(self._h, ctypes.byref(spec), ncfg,
                                                       None if sd is None else _ptr(sd),
                                                       ctypes.cast(outs, ctypes.c_void_p)), "dwf_measure_batch")

@@ -5,8 +5,10 @@
 #include "../../include/dwf_hip.h"
 
 #include <functional>
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 // Error carrying a C ABI status code; caught at the ABI boundary.
 struct DwfError : std::runtime_error {
@@ -36,6 +38,7 @@ enum BufId : int {
     B_AUX1,
     B_AUX2,
     B_AUX3,
+    B_AUX4,
     B_PHI,     // HMC: pseudofermion field
     B_ETA,     // HMC: (D D^dag)^-1 phi (also the chronological guess)
     B_XI,      // HMC: D^dag eta
@@ -84,6 +87,11 @@ struct dwf_ctx {
     int tile_x = 0;               // stencil 2D tile width (0: linear blocks)
     int nt_store = 0;             // streaming stores of stencil outputs
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
+    int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs
+
+    // Instantiated CG iteration blocks, keyed by every argument their kernels capture.
+    std::map<std::vector<unsigned long long>, hipGraphExec_t> graphs;
+    void drop_graphs();
 
     struct Buf {
         void* p = nullptr;

@@ -29,9 +29,16 @@ int dwf_hip_fail(hipError_t e, const char* what)
     return DWF_ERR_HIP;
 }
 
+void dwf_ctx::drop_graphs()
+{
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+}
+
 dwf_ctx::~dwf_ctx()
 {
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
     for (auto& b : bufs)
         if (b.p) (void)hipFree(b.p);
     if (theta) (void)hipFree(theta);
@@ -289,6 +296,9 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
         case DWF_TUNE_RELIABLE_DELTA:
             if (!(value > 0.0 && value < 1.0)) break;
             ctx->reliable_delta = value;
+            return DWF_OK;
+        case DWF_TUNE_GRAPHS:
+            ctx->use_graphs = value != 0.0 ? 1 : 0;
             return DWF_OK;
         default:
             break;

@@ -41,12 +41,14 @@ __device__ __forceinline__ double bsum(double v, double* sh)
     return r;
 }
 
-// Sources.  kind 0: wall at t_src (all x), 1: point at (t_src, x=0); the source spin ss
-// fills P_L e_ss at s=0 and P_R e_ss at s=Ls-1 (correlator.hpp:131-138, 199-211).
-// The 4 wall / 4 point right-hand sides are ordered (t_src=0, ss=0), (0, 1),
-// (Nt/2, 0), (Nt/2, 1).  Noise: src = P_L eta at s=0, + P_R eta at s=Ls-1 (:60-75).
+// Sources.  Wall: all x at t_src; point: the single site (t_src, x_src).  The source
+// spin ss fills P_L e_ss at s=0 and P_R e_ss at s=Ls-1 (correlator.hpp:131-138, 199-211).
+// The 4 wall / 4 reference point right-hand sides are ordered (t_src=0, ss=0), (0, 1),
+// (Nt/2, 0), (Nt/2, 1), point ones at x_src = 0.  With psite ([cfg][2] = (t0, x0)) the
+// point block is 2 right-hand sides (ss = 0, 1) at that site.  Noise: src = P_L eta at
+// s=0, + P_R eta at s=Ls-1 (:60-75).
 __global__ void k_sources(const int Nt, const int Nx, const int Ls, const int nwall, const int npoint,
-                          const int nnoise, const double* noise_all, double* src)
+                          const int nnoise, const double* noise_all, const int* psite, double* src)
 {
     const int V4 = Nt * Nx;
     const long long V5 = (long long)V4 * Ls;
@@ -58,10 +60,17 @@ __global__ void k_sources(const int Nt, const int Nx, const int Ls, const int nw
         const int t = i / Nx, x = i - t * Nx;
         double4 lo = make_double4(0, 0, 0, 0), hi = make_double4(0, 0, 0, 0);  // s = 0, s = Ls-1
         if (rhs < nwall + npoint) {
-            const int k = rhs < nwall ? rhs : rhs - nwall;
-            const int t_src = (k >> 1) ? Nt / 2 : 0;
-            const int ss = k & 1;
-            const bool on = (t == t_src) && (rhs < nwall || x == 0);
+            int t_src, x_src = 0, ss;
+            if (rhs < nwall || !psite) {
+                const int k = rhs < nwall ? rhs : rhs - nwall;
+                t_src = (k >> 1) ? Nt / 2 : 0;
+                ss = k & 1;
+            } else {
+                t_src = psite[2 * cfg];
+                x_src = psite[2 * cfg + 1];
+                ss = rhs - nwall;
+            }
+            const bool on = (t == t_src) && (rhs < nwall || x == x_src);
             if (on) {
                 if (ss == 1) lo.z = 1.0;  // P_L e_1 = (0, 1) at s = 0
                 else hi.x = 1.0;          // P_R e_0 = (1, 0) at s = Ls-1
@@ -115,18 +124,34 @@ __global__ void k_contract_wall(const int Nt, const int Nx, const int Ls, const 
     }
 }
 
-// Momentum-projected point correlator (correlator.hpp:149-174): block per (t, source
-// time); out[k][n][t] = sum_x cos(2 pi n x / Nx) sum_ss |q|^2.
-__global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const int nmom, const int K, const double* X,
-                                 double* out_all)
+// Point correlators (correlator.hpp:149-174): block per (t, source time k, configuration);
+// out[k][n][t] = sum_x cos(2 pi n (x - x0) / Nx) sum_ss |q|^2 and j5[k][t] the midpoint
+// correlator of k_contract_wall.  nsrc_t source times of 2 spins each follow one another
+// in the point block; x0 = psite[cfg][1] (0 without psite).
+__global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const int nmom, const int K,
+                                 const int nsrc_t, const int* psite, const double* X, double* out_all,
+                                 double* j5_all)
 {
     __shared__ double sh[8];
     const int t = blockIdx.x, k = blockIdx.y, cfg = blockIdx.z;
     const int V4 = Nt * Nx;
     const long long V5 = (long long)V4 * Ls;
     const double4* P = reinterpret_cast<const double4*>(X) + (long long)cfg * K * V5;
-    double* out = out_all + (long long)cfg * 2 * nmom * Nt;
+    double* out = out_all + (long long)cfg * nsrc_t * nmom * Nt;
+    double* j5 = j5_all + (long long)cfg * nsrc_t * Nt;
+    const int x0 = psite ? psite[2 * cfg + 1] : 0;
     const double pi = 3.141592653589793238462643383279502884;
+    double b = 0.0;
+    for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
+        const int i = t * Nx + x;
+        for (int ss = 0; ss < 2; ++ss) {
+            const double4* Y = P + (2 * k + ss) * V5;
+            const double4 l = Y[i], r = Y[(long long)(Ls - 1) * V4 + i], m = Y[(long long)(Ls / 2) * V4 + i];
+            b += m.x * r.x + m.y * r.y + m.z * l.z + m.w * l.w;
+        }
+    }
+    b = bsum(b, sh);
+    if (threadIdx.x == 0) j5[(long long)k * Nt + t] = b;
     for (int n = 0; n < nmom; ++n) {
         const double p = 2.0 * pi * n / Nx;
         double v = 0.0;
@@ -138,7 +163,7 @@ __global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const
                 const double4 l = Y[i], r = Y[(long long)(Ls - 1) * V4 + i];
                 local += r.x * r.x + r.y * r.y + l.z * l.z + l.w * l.w;
             }
-            v += cos(p * x) * local;
+            v += cos(p * (x - x0)) * local;
         }
         v = bsum(v, sh);
         if (threadIdx.x == 0) out[((long long)k * nmom + n) * Nt + t] = v;
@@ -215,11 +240,17 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
     const Lat& L = c->L;
     const int Nt = L.Nt, Nx = L.Nx, Ls = L.Ls, V4 = L.V4;
     const int nwall = spec->wall ? 4 : 0;
-    const int npoint = spec->point ? 4 : 0;
+    const int* hsite = spec->point ? spec->point_site : nullptr;
+    const int nsrc_t = hsite ? 1 : 2;  // point source times per configuration
+    const int npoint = spec->point ? 2 * nsrc_t : 0;
     const int nnoise = spec->noise_vectors;
     if (nnoise < 0) throw DwfError(DWF_ERR_ARG, "eta.noise_vectors must be positive");
     if (spec->point && (spec->max_momentum < 0 || spec->max_momentum > Nx / 2))
         throw DwfError(DWF_ERR_ARG, "analysis.max_momentum is outside 0..Nx/2");
+    if (hsite)
+        for (int k = 0; k < ncfg; ++k)
+            if (hsite[2 * k] < 0 || hsite[2 * k] >= Nt || hsite[2 * k + 1] < 0 || hsite[2 * k + 1] >= Nx)
+                throw DwfError(DWF_ERR_ARG, "dwf_measure: point_site outside the lattice");
     for (int k = 0; k < ncfg; ++k) {
         const dwf_measure_out& o = outs[k];
         if ((spec->wall && (!o.CPP || !o.CJ5)) || (spec->point && !o.momentum_CPP) || (nnoise > 0 && !o.loop))
@@ -247,9 +278,15 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
         c->check(hipMemcpyAsync(dnoise, h.data(), h.size() * 8, hipMemcpyHostToDevice, s), "noise upload");
         c->check(hipStreamSynchronize(s), "noise upload");
     }
+    int* dsite = nullptr;
+    if (hsite) {
+        dsite = (int*)c->buf(B_AUX4, (size_t)ncfg * 2 * sizeof(int));
+        c->check(hipMemcpyAsync(dsite, hsite, (size_t)ncfg * 2 * sizeof(int), hipMemcpyHostToDevice, s), "site upload");
+        c->check(hipStreamSynchronize(s), "site upload");
+    }
     double* src = (double*)c->buf(B_AUX0, nv * nrhs * 8);
     double* X = (double*)c->buf(B_AUX2, nv * nrhs * 8);
-    k_sources<<<dim3((V4 + 255) / 256, nrhs), 256, 0, s>>>(Nt, Nx, Ls, nwall, npoint, nnoise, dnoise, src);
+    k_sources<<<dim3((V4 + 255) / 256, nrhs), 256, 0, s>>>(Nt, Nx, Ls, nwall, npoint, nnoise, dnoise, dsite, src);
     c->check_launch("sources");
     std::vector<dwf_solve_stats> stv(nrhs);
     const int rc = dwf_solve_dirac_impl(c, spec->method, src, nullptr, spec->rtol, spec->maxiter, nrhs, X, stv.data());
@@ -299,17 +336,31 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
     }
     if (npoint) {
         const int nmom = spec->max_momentum + 1;
-        double* pm = (double*)c->buf(B_PR, (size_t)2 * nmom * Nt * ncfg * 8);
-        k_contract_point<<<dim3(Nt, 2, ncfg), 256, 0, s>>>(Nt, Nx, Ls, nmom, K, X + nv * nwall, pm);
+        double* pm = (double*)c->buf(B_PR, (size_t)nsrc_t * nmom * Nt * ncfg * 8);
+        double* pj = (double*)c->buf(B_PA, (size_t)nsrc_t * Nt * ncfg * 8);
+        k_contract_point<<<dim3(Nt, nsrc_t, ncfg), 256, 0, s>>>(Nt, Nx, Ls, nmom, K, nsrc_t, dsite, X + nv * nwall,
+                                                                pm, pj);
         c->check_launch("point contraction");
-        std::vector<double> h((size_t)2 * nmom * Nt * ncfg);
+        std::vector<double> h((size_t)nsrc_t * nmom * Nt * ncfg), hj((size_t)nsrc_t * Nt * ncfg);
         c->check(hipMemcpyAsync(h.data(), pm, h.size() * 8, hipMemcpyDeviceToHost, s), "readback");
+        c->check(hipMemcpyAsync(hj.data(), pj, hj.size() * 8, hipMemcpyDeviceToHost, s), "readback");
         c->check(hipStreamSynchronize(s), "readback");
         for (int k = 0; k < ncfg; ++k) {
-            const double* a = h.data() + (size_t)k * 2 * nmom * Nt;
-            for (int n = 0; n < nmom; ++n)
-                for (int dt = 0; dt < Nt; ++dt)
-                    outs[k].momentum_CPP[n * Nt + dt] = (a[n * Nt + dt] + a[(nmom + n) * Nt + (dt + Nt / 2) % Nt]) / 2.0;
+            const double* a = h.data() + (size_t)k * nsrc_t * nmom * Nt;
+            const double* b = hj.data() + (size_t)k * nsrc_t * Nt;
+            // relative to the source time: folded over t0 = 0, Nt/2 (:158-174) or a single t0
+            for (int dt = 0; dt < Nt; ++dt) {
+                if (!hsite) {
+                    for (int n = 0; n < nmom; ++n)
+                        outs[k].momentum_CPP[n * Nt + dt] =
+                            (a[n * Nt + dt] + a[(nmom + n) * Nt + (dt + Nt / 2) % Nt]) / 2.0;
+                    if (outs[k].point_CJ5) outs[k].point_CJ5[dt] = (b[dt] + b[Nt + (dt + Nt / 2) % Nt]) / 2.0;
+                } else {
+                    const int t = (hsite[2 * k] + dt) % Nt;
+                    for (int n = 0; n < nmom; ++n) outs[k].momentum_CPP[n * Nt + dt] = a[n * Nt + t];
+                    if (outs[k].point_CJ5) outs[k].point_CJ5[dt] = b[t];
+                }
+            }
         }
     }
     if (nnoise) {

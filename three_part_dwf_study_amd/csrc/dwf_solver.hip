@@ -17,6 +17,7 @@
 #include "dwf_eo.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <cmath>
 #include <functional>
 #include <string>
@@ -50,7 +51,20 @@ struct CgOperator {
     // r64 = b - A x in FP64 (copies into p64/r32/p32 if given), partials |r|^2, |b|^2
     virtual void residual(const double* x, const double* b, double* r64, double* p64, float* r32, float* p32,
                           double* part, const CgState* st, int slot, int skip) = 0;
+    // Every value first()/second() pass to their kernels beyond their own arguments.
+    virtual void signature(std::vector<unsigned long long>& k) const = 0;
 };
+
+using Sig = std::vector<unsigned long long>;
+inline unsigned long long sig_ptr(const void* p) { return (unsigned long long)(uintptr_t)p; }
+
+void sig_ctx(const dwf_ctx* c, Sig& k)
+{
+    k.push_back(sig_ptr(c->l64));
+    k.push_back(sig_ptr(c->l32));
+    k.push_back((unsigned long long)c->rhs_per_cfg);
+    k.push_back((unsigned long long)c->xcd_remap);
+}
 
 // A = Op2 Op1 = D^dag D (first_dag = false) or D D^dag (first_dag = true).
 struct FullOp : CgOperator {
@@ -129,6 +143,13 @@ struct FullOp : CgOperator {
         e.slot = slot;
         e.skip = skip;
         launch_hop<double>(!first_dag, EPI_RESID, e, nrhs, c->stream);
+    }
+    void signature(Sig& k) const override
+    {
+        sig_ctx(c, k);
+        k.insert(k.end(), {1ULL, (unsigned long long)first_dag, (unsigned long long)nrhs});
+        const int v[] = {cfg.bx, cfg.chunks, cfg.spt, cfg.remap, cfg.prefetch, cfg.tx, cfg.tiles_x, cfg.nt_store};
+        for (int x : v) k.push_back((unsigned long long)(unsigned)x);
     }
 };
 
@@ -244,6 +265,12 @@ struct SchurOp : CgOperator {
         e.part = part;
         launch_eo_b<double>(true, true, EPI_RESID, e, nrhs, c->stream);
     }
+    void signature(Sig& k) const override
+    {
+        sig_ctx(c, k);
+        k.insert(k.end(), {2ULL, (unsigned long long)nrhs, (unsigned long long)bx, (unsigned long long)chunks,
+                           (unsigned long long)spt, sig_ptr(te32), sig_ptr(te64), sig_ptr(q64)});
+    }
 };
 
 // ------------------------------------------------------------------- driver
@@ -284,6 +311,31 @@ __global__ void k_sub_into(const double* a, const double* b, double* out, long l
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         out[i] = a[i] - b[i];
+}
+
+// Instantiated graph of `body` (kernel launches on c->stream only), cached under `key`,
+// which must hold every argument the captured kernels receive.
+hipGraphExec_t cg_graph(dwf_ctx* c, const Sig& key, const std::function<void()>& body)
+{
+    auto f = c->graphs.find(key);
+    if (f != c->graphs.end()) return f->second;
+    if (c->graphs.size() >= 64) c->drop_graphs();
+    c->check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipGraph_t g = nullptr;
+    try {
+        body();
+    } catch (...) {
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    c->check(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    c->check(e, "hipGraphInstantiate");
+    c->graphs.emplace(key, ex);
+    return ex;
 }
 
 // CG on op.  Leaves the per-RHS states in c->st_host (slot c->last_slot).  `pre`
@@ -331,21 +383,39 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     std::vector<int> last_iters(nrhs, 0);
     for (int i = 0; i < nrhs; ++i) last_true[i] = c->st_host[i].true_relres;
     const int mode = mixed ? MODE_MIXED : MODE_CG64;
+    auto iteration = [&](int i) {
+        const int slot = i & 1, skip = F_DONE | F_FROZEN;
+        if (mixed) {
+            op.first(false, p32, t32, partA, st, slot, skip);
+            op.second(false, t32, x32, r32, p32, partA, partB, st, slot, skip);
+            launch_cg_p<float>(op.sites, op.nblk, r32, p32, partA, partB, st, slot, mode, nrhs, s);
+        } else {
+            op.first(true, p64, t64, partA, st, slot, skip);
+            op.second(true, t64, x, r64, p64, partA, partB, st, slot, skip);
+            launch_cg_p<double>(op.sites, op.nblk, r64, p64, partA, partB, st, slot, mode, nrhs, s);
+        }
+    };
+    // Blocks of kGraphIters iterations starting on an even iteration are one graph launch.
+    constexpr int kGraphIters = 8;
+    hipGraphExec_t block = nullptr;
+    if (c->use_graphs && maxiter >= kGraphIters) {
+        Sig key{(unsigned long long)mixed, (unsigned long long)nrhs, (unsigned long long)op.sites,
+                (unsigned long long)op.nblk, (unsigned long long)mode, sig_ptr(x),   sig_ptr(r64),  sig_ptr(p64),
+                sig_ptr(t64),            sig_ptr(r32),            sig_ptr(t32),  sig_ptr(p32), sig_ptr(x32),
+                sig_ptr(partA),          sig_ptr(partB),          sig_ptr(st)};
+        op.signature(key);
+        block = cg_graph(c, key, [&] {
+            for (int i = 0; i < kGraphIters; ++i) iteration(i);
+        });
+    }
     while (!all_done(c->st_host, it & 1, nrhs)) {
         const int n = std::min(chunk, maxiter - it);
         if (n <= 0) break;
-        for (int j = 0; j < n; ++j, ++it) {
-            const int slot = it & 1, skip = F_DONE | F_FROZEN;
-            if (mixed) {
-                op.first(false, p32, t32, partA, st, slot, skip);
-                op.second(false, t32, x32, r32, p32, partA, partB, st, slot, skip);
-                launch_cg_p<float>(op.sites, op.nblk, r32, p32, partA, partB, st, slot, mode, nrhs, s);
-            } else {
-                op.first(true, p64, t64, partA, st, slot, skip);
-                op.second(true, t64, x, r64, p64, partA, partB, st, slot, skip);
-                launch_cg_p<double>(op.sites, op.nblk, r64, p64, partA, partB, st, slot, mode, nrhs, s);
-            }
-        }
+        int j = 0;
+        if (block && (it & 1) == 0)
+            for (; j + kGraphIters <= n; j += kGraphIters, it += kGraphIters)
+                c->check(hipGraphLaunch(block, s), "hipGraphLaunch");
+        for (; j < n; ++j, ++it) iteration(it);
         c->check_launch("CG iteration");
         if (mixed) {
             // Reliable update: x64 += x32, FP64 true residual, certify or restart the cycle.
