@@ -59,7 +59,11 @@ enum dwf_method {
      * operator, FP32 CGNR with FP64 reliable updates on the odd half-lattice and FP64
      * defect correction; certified on the reference's full system.  Needs even Nt, Nx
      * and Ls in {4, 8, 12, 16}; otherwise it runs as DWF_SOLVE_MIXED. */
-    DWF_SOLVE_EO_MIXED = 2
+    DWF_SOLVE_EO_MIXED = 2,
+    /* Domain-wall D^dag D only: flexible CG (FCG(1)) in FP64 preconditioned by the
+     * learned network attached with dwf_ctx_set_preconditioner; certified on the FP64
+     * true residual.  No counterpart in the reference. */
+    DWF_SOLVE_LEARNED = 3
 };
 
 typedef struct dwf_ctx dwf_ctx;
@@ -295,6 +299,51 @@ int dwf_measure(dwf_ctx* ctx, const dwf_measure_spec* spec, dwf_measure_out* out
  * Replaces the OpenMP loop over configurations of domain_wall/analyze_configs.cpp:45-76. */
 int dwf_measure_batch(dwf_ctx* ctx, const dwf_measure_spec* spec, int ncfg,
                       const unsigned long long* seeds, dwf_measure_out* outs);
+
+/* ---- learned preconditioner (no counterpart in the reference) -------------------- */
+
+/* A gauge-equivariant linear convolutional network M ~ (D^dag D)^-1: three layers of
+ * parallel-transported 7-point stencils in (t, x, s) with s-dependent complex weights,
+ * 2 -> 8 -> 8 -> 2 complex channels, run in BF16 on the matrix cores with FP32
+ * accumulation.  Weights (host arrays of float): per layer [Ls][7][C_out][C_in]
+ * (re, im), layers concatenated; points self, +t, -t, +x, -x, +s, -s.  The network
+ * lives on the current device and can serve any context with the same Ls and Nt*Nx a
+ * multiple of 16. */
+typedef struct dwf_net dwf_net;
+long long dwf_net_num_weights(int Ls);
+int dwf_net_create(dwf_net** out, int Ls, unsigned long long seed);
+int dwf_net_destroy(dwf_net* net);
+int dwf_net_set_weights(dwf_net* net, const float* w);
+int dwf_net_get_weights(const dwf_net* net, float* w);
+
+/* out = M in (adjoint = 0) or M^dag in (adjoint = 1) for nrhs FP64 spinors on the
+ * device, with the context's configurations. */
+int dwf_net_apply_device(dwf_ctx* ctx, const dwf_net* net, int adjoint, const double* in, double* out, int nrhs);
+
+/* Training pairs: b[npairs] (sources of D^dag D x = b) and x[npairs] (their solutions),
+ * FP32 spinors on the device; pair i uses configuration pair_cfg[i] (host array) of the
+ * configurations the context holds.  loss 0: mean ||M b - x||^2 / ||x||^2;
+ * loss 1: mean ||D^dag D M b - b||^2 / ||b||^2 (x unused, may be null). */
+typedef struct dwf_train_params {
+    int steps;        /* Adam steps                                   */
+    int batch;        /* pairs per step (reshuffled every epoch)      */
+    int loss;         /* 0 or 1, as above                              */
+    int pad;
+    double lr, beta1, beta2, eps;
+    unsigned long long seed;  /* batch order                          */
+} dwf_train_params;
+
+/* Loss over all npairs and its gradient (host float array in the weight layout). */
+int dwf_net_loss_grad_device(dwf_ctx* ctx, dwf_net* net, int loss, const float* b, const float* x,
+                             const int* pair_cfg, int npairs, double* loss_out, float* grad);
+
+/* p->steps Adam steps; loss_history (nullable) receives each step's batch loss. */
+int dwf_net_train_device(dwf_ctx* ctx, dwf_net* net, const float* b, const float* x, const int* pair_cfg,
+                         int npairs, const dwf_train_params* p, double* loss_history);
+
+/* The network DWF_SOLVE_LEARNED uses (null detaches).  The context keeps the pointer:
+ * destroy the network only after detaching it or destroying the context. */
+int dwf_ctx_set_preconditioner(dwf_ctx* ctx, const dwf_net* net);
 
 /* ---- host random numbers of the callers (reference draw order) --------------------- */
 

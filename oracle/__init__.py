@@ -43,6 +43,9 @@ _PROTOS = {
     "orc_wilson_pseudoscalar_correlator": (_I, [_I, _I, _D, _D, _D, _P, _I, _I, _D, _I, _P]),
     "orc_mt19937_64_nth": (_U, [_U, _I]),
     "orc_dwf_heatbath_draws": (None, [_I, _I, _U, _P, _P, _P]),
+    "orc_net_num_weights": (ctypes.c_longlong, [_I]),
+    "orc_net_apply": (_I, [_I, _I, _I, _P, _P, _P, _P, _I, _I, _I]),
+    "orc_net_loss_grad": (_I, [_I, _I, _I, _D, _D, _P, _P, _P, _P, _I, _I, _I, _P, _P]),
 }
 
 
@@ -259,3 +262,39 @@ def dwf_heatbath_draws(ndof, ngauge, seed):
     u = ctypes.c_double()
     lib().orc_dwf_heatbath_draws(ndof, ngauge, seed, _p(chi), _p(mom), ctypes.byref(u))
     return chi, mom, u.value
+
+
+# ------------------------------------------------------------ learned preconditioner
+
+def net_num_weights(Ls):
+    """Real parameters of the 2 -> 8 -> 8 -> 2 network (oracle_net.cpp)."""
+    return int(lib().orc_net_num_weights(Ls))
+
+
+def net_apply(theta, w, v, Nt, Nx, Ls, adj=False, bf16=False):
+    """z = M v (or M^dag v); v [nrhs][ndof] complex (or one spinor)."""
+    v = _c(v)
+    one = v.ndim == 1
+    vb = v.reshape(-1, Ls * Nt * Nx * 2)
+    z = np.empty_like(vb)
+    w = _c(w, np.float64)
+    rc = lib().orc_net_apply(Nt, Nx, Ls, _p(_c(theta, np.float64)), _p(w), _p(vb), _p(z), vb.shape[0],
+                             int(adj), int(bf16))
+    if rc:
+        raise OracleError(rc, "orc_net_apply: bad argument")
+    return z[0] if one else z
+
+
+def net_loss_grad(theta, w, b, x, Nt, Nx, Ls, bf16=False, loss=0, M5=1.0, mf=0.05):
+    """(loss, grad): loss 0 = mean_i ||M b_i - x_i||^2 / ||x_i||^2, loss 1 =
+    mean_i ||D^dag D M b_i - b_i||^2 / ||b_i||^2 (x unused)."""
+    b = _c(b).reshape(-1, Ls * Nt * Nx * 2)
+    xx = None if x is None else _c(x).reshape(b.shape)
+    w = _c(w, np.float64)
+    g = np.zeros_like(w)
+    L = ctypes.c_double()
+    rc = lib().orc_net_loss_grad(Nt, Nx, Ls, M5, mf, _p(_c(theta, np.float64)), _p(w), _p(b), _p(xx), b.shape[0],
+                                 loss, int(bf16), ctypes.byref(L), _p(g))
+    if rc:
+        raise OracleError(rc, "orc_net_loss_grad: bad argument")
+    return L.value, g

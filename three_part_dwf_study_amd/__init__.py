@@ -16,6 +16,7 @@ from .capi import (  # noqa: F401
     DWF_PREC_FP64,
     DWF_SOLVE_CG64,
     DWF_SOLVE_EO_MIXED,
+    DWF_SOLVE_LEARNED,
     DWF_SOLVE_MIXED,
     DWF_TUNE_NT_STORE,
     DWF_TUNE_GRAPHS,
@@ -26,6 +27,7 @@ from .capi import (  # noqa: F401
     DWF_TUNE_XCD_REMAP,
     DwfContext,
     DwfError,
+    Network,
     build_library,
     library_path,
     load_library,

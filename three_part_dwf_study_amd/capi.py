@@ -37,6 +37,7 @@ DWF_PREC_FP32 = 1
 DWF_SOLVE_CG64 = 0
 DWF_SOLVE_MIXED = 1
 DWF_SOLVE_EO_MIXED = 2
+DWF_SOLVE_LEARNED = 3
 
 DWF_TUNE_SCHUNKS = 0
 DWF_TUNE_XCD_REMAP = 1
@@ -185,6 +186,15 @@ PROTOTYPES = {
     "dwf_hmc_trajectory": (_I, [_P, _P, _P, _P, _D, _P]),
     "dwf_measure": (_I, [_P, _P, _P]),
     "dwf_measure_batch": (_I, [_P, _P, _I, _P, _P]),
+    "dwf_net_num_weights": (ctypes.c_longlong, [_I]),
+    "dwf_net_create": (_I, [_P, _I, ctypes.c_ulonglong]),
+    "dwf_net_destroy": (_I, [_P]),
+    "dwf_net_set_weights": (_I, [_P, _P]),
+    "dwf_net_get_weights": (_I, [_P, _P]),
+    "dwf_net_apply_device": (_I, [_P, _P, _I, _P, _P, _I]),
+    "dwf_net_loss_grad_device": (_I, [_P, _P, _I, _P, _P, _P, _I, _P, _P]),
+    "dwf_net_train_device": (_I, [_P, _P, _P, _P, _P, _I, _P, _P]),
+    "dwf_ctx_set_preconditioner": (_I, [_P, _P]),
     "dwf_device_alloc": (_P, [ctypes.c_ulonglong]),
     "dwf_device_free": (_I, [_P]),
     "dwf_memcpy_htod": (_I, [_P, _P, ctypes.c_ulonglong]),
@@ -322,6 +332,12 @@ class DwfContext:
         if th.size != 2 * self.geom.V4:
             raise ValueError("theta must hold 2*Nt*Nx angles")
         _check(self._lib, self._lib.dwf_set_gauge(self._h, _ptr(th)), "dwf_set_gauge")
+
+    def set_preconditioner(self, net):
+        """Attach the network DWF_SOLVE_LEARNED uses (None detaches); keeps a reference."""
+        self._net = net
+        _check(self._lib, self._lib.dwf_ctx_set_preconditioner(self._h, net.handle if net else None),
+               "dwf_ctx_set_preconditioner")
 
     def set_gauge_batch(self, thetas: np.ndarray):
         th = np.ascontiguousarray(thetas, dtype=np.float64).reshape(-1)
@@ -582,6 +598,107 @@ class DeviceArray:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class TrainParams(ctypes.Structure):
+    _fields_ = [
+        ("steps", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("loss", ctypes.c_int),
+        ("pad", ctypes.c_int),
+        ("lr", ctypes.c_double),
+        ("beta1", ctypes.c_double),
+        ("beta2", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("seed", ctypes.c_ulonglong),
+    ]
+
+
+class Network:
+    """The learned preconditioner (dwf_net_*): weights on the device."""
+
+    def __init__(self, Ls: int, seed: int = 0):
+        self._lib = load_library()
+        self.Ls = Ls
+        h = ctypes.c_void_p()
+        _check(self._lib, self._lib.dwf_net_create(ctypes.byref(h), Ls, seed), "dwf_net_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_weights(self) -> int:
+        return int(self._lib.dwf_net_num_weights(self.Ls))
+
+    def get_weights(self) -> np.ndarray:
+        w = np.empty(self.num_weights(), np.float32)
+        _check(self._lib, self._lib.dwf_net_get_weights(self._h, _ptr(w)), "dwf_net_get_weights")
+        return w
+
+    def set_weights(self, w: np.ndarray):
+        w = np.ascontiguousarray(w, np.float32)
+        if w.size != self.num_weights():
+            raise ValueError("wrong number of weights")
+        _check(self._lib, self._lib.dwf_net_set_weights(self._h, _ptr(w)), "dwf_net_set_weights")
+
+    def apply(self, ctx: "DwfContext", v: np.ndarray, adjoint: bool = False) -> np.ndarray:
+        """M v (or M^dag v) for host complex spinors [nrhs][ndof] (or one spinor)."""
+        vb, n = ctx._as_batch(v)
+        d_in = DeviceArray.from_host(vb.view(np.float64))
+        d_out = DeviceArray(d_in.nbytes)
+        _check(self._lib, self._lib.dwf_net_apply_device(ctx.handle, self._h, int(adjoint), d_in.ptr, d_out.ptr, n),
+               "dwf_net_apply_device")
+        out = d_out.download(np.float64).view(np.complex128)
+        return out.reshape(np.shape(v))
+
+    @staticmethod
+    def _pairs(b, x):
+        bb = np.ascontiguousarray(b, np.complex128).reshape(np.shape(b)[0], -1)
+        db = DeviceArray.from_host(bb.view(np.float64).astype(np.float32))
+        dx = None
+        if x is not None:
+            xx = np.ascontiguousarray(x, np.complex128).reshape(bb.shape)
+            dx = DeviceArray.from_host(xx.view(np.float64).astype(np.float32))
+        return db, dx, bb.shape[0]
+
+    def loss_grad(self, ctx: "DwfContext", b, x, pair_cfg=None, loss: int = 0):
+        """(loss, grad) over host pairs b[i], x[i] (complex spinors)."""
+        db, dx, n = self._pairs(b, x)
+        pc = np.ascontiguousarray(np.zeros(n) if pair_cfg is None else pair_cfg, np.int32)
+        g = np.empty(self.num_weights(), np.float32)
+        L = ctypes.c_double()
+        _check(self._lib, self._lib.dwf_net_loss_grad_device(ctx.handle, self._h, loss, db.ptr,
+                                                             dx.ptr if dx else None, _ptr(pc), n,
+                                                             ctypes.byref(L), _ptr(g)), "dwf_net_loss_grad_device")
+        return L.value, g
+
+    def train_device(self, ctx: "DwfContext", b_ptr: int, x_ptr: int | None, pair_cfg, npairs: int, steps: int,
+                     batch: int, loss: int = 1, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999,
+                     eps: float = 1e-8, seed: int = 0) -> np.ndarray:
+        """Adam steps on FP32 device pairs; returns the per-step batch losses."""
+        pc = np.ascontiguousarray(pair_cfg, np.int32)
+        hist = np.zeros(steps)
+        p = TrainParams(steps, batch, loss, 0, lr, beta1, beta2, eps, seed)
+        _check(self._lib, self._lib.dwf_net_train_device(ctx.handle, self._h, b_ptr, x_ptr, _ptr(pc), npairs,
+                                                         ctypes.byref(p), _ptr(hist)), "dwf_net_train_device")
+        return hist
+
+    def train(self, ctx: "DwfContext", b, x=None, pair_cfg=None, **kw) -> np.ndarray:
+        db, dx, n = self._pairs(b, x)
+        pc = np.zeros(n, np.int32) if pair_cfg is None else pair_cfg
+        return self.train_device(ctx, db.ptr, dx.ptr if dx else None, pc, n, **kw)
+
+    def close(self):
+        if self._h:
+            self._lib.dwf_net_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

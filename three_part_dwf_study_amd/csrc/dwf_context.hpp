@@ -63,6 +63,26 @@ enum BufId : int {
     B_EO_Q,
     B_EO_TE32,
     B_EO_DELTA,
+    B_NET_H1,
+    B_NET_H2,
+    B_NET_Y,
+    B_NET_GY,
+    B_NET_T,
+    B_NET_ROW,
+    B_NET_CFG,
+    B_NET_GB,
+    B_NET_WPART,
+    B_NET_PART,
+    B_NET_N2,
+    B_NET_LOSS,
+    B_NET_STAGE,
+    B_NET_GRAD,
+    B_FCG_R,
+    B_FCG_P,
+    B_FCG_Q,
+    B_FCG_Z,
+    B_FCG_PART,
+    B_FCG_SCAL,
     B_COUNT
 };
 
@@ -88,10 +108,14 @@ struct dwf_ctx {
     int nt_store = 0;             // streaming stores of stencil outputs
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs
+    const dwf_net* net = nullptr; // learned preconditioner of DWF_SOLVE_LEARNED (not owned)
 
     // Instantiated CG iteration blocks, keyed by every argument their kernels capture.
     std::map<std::vector<unsigned long long>, hipGraphExec_t> graphs;
     void drop_graphs();
+    // The instantiated graph of `body` (kernel launches on `stream` only), cached under
+    // `key`, which must hold every argument the captured kernels receive.
+    hipGraphExec_t cached_graph(const std::vector<unsigned long long>& key, const std::function<void()>& body);
 
     struct Buf {
         void* p = nullptr;
@@ -115,5 +139,7 @@ enum ScalarSlot : int { S_GAUGE = 0, S_PLAQ, S_PF, S_KIN, S_COUNT = 8 };
 
 int dwf_solve_dirac_impl(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter,
                          int nrhs, double* x, dwf_solve_stats* stats);
+int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, double rtol, int maxiter, int nrhs,
+                  dwf_solve_stats* stats);
 int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0,
                           double rtol, int maxiter, int nrhs, double* x);

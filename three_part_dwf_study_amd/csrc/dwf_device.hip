@@ -168,7 +168,8 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
         const R sxp = (x == Nx - 1) ? (R)L.bcx : (R)1;
         const R sxm = (x == 0) ? (R)L.bcx : (R)1;
         using C = typename Cx<R>::T;
-        const C* lk = static_cast<const C*>(a.links) + (long long)(rhs / a.rhs_per_cfg) * a.link_stride;
+        const int cfg = a.cfg_map ? a.cfg_map[rhs] : rhs / a.rhs_per_cfg;
+        const C* lk = static_cast<const C*>(a.links) + (long long)cfg * a.link_stride;
         const C ut = lk[site];
         const C ux = lk[V4 + site];
         const C utm = lk[tm * Nx + x];

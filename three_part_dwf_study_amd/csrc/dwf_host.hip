@@ -35,6 +35,29 @@ void dwf_ctx::drop_graphs()
     graphs.clear();
 }
 
+hipGraphExec_t dwf_ctx::cached_graph(const std::vector<unsigned long long>& key, const std::function<void()>& body)
+{
+    auto f = graphs.find(key);
+    if (f != graphs.end()) return f->second;
+    if (graphs.size() >= 64) drop_graphs();
+    check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipGraph_t g = nullptr;
+    try {
+        body();
+    } catch (...) {
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    check(e, "hipGraphInstantiate");
+    graphs.emplace(key, ex);
+    return ex;
+}
+
 dwf_ctx::~dwf_ctx()
 {
     if (stream) (void)hipStreamSynchronize(stream);

@@ -81,7 +81,8 @@ template <typename R> struct HopArgs {
     HopCfg cfg;
     const void* links;   // [ncfg][2][V4] complex R: (cos theta, sin theta)
     long long link_stride;  // complex entries per configuration (2 V4)
-    int rhs_per_cfg;     // right-hand side r uses configuration r / rhs_per_cfg
+    int rhs_per_cfg;     // right-hand side r uses configuration r / rhs_per_cfg ...
+    const int* cfg_map;  // ... or cfg_map[r] when not null
     const R* in;         // [nrhs][V5] spinors of 4 R
     R* out;              // EPI_STORE / EPI_NORM
     R* x;                // EPI_CGUPD

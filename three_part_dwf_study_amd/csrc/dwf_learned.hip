@@ -1,0 +1,1401 @@
+// dwf_learned.hip -- the learned preconditioner of libdwfhip (no counterpart in the
+// reference, which solves with unpreconditioned CG: domain_wall/include/cg_solver.hpp).
+//
+// Network.  Three linear parallel-transport convolution layers, 2 -> 8 -> 8 -> 2
+// complex channels per 5D site.  Layer output
+//     y(s,t,x) = sum_p W_p(s) U_p(t,x) h(neighbour_p(s,t,x)),
+// p over the 7 points self, +t, -t, +x, -x, +s, -s; U_p the Dirac operator's transporter
+// (antiperiodic time boundary), +-s periodic without link; W_p(s) complex C_out x C_in,
+// s-dependent.  Linear and gauge covariant; M ~ (D^dag D)^-1 after training.
+//
+// Matrix cores.  Per layer the work is a GEMM [sites] x [7 * 2 C_in] x [2 C_out] in the
+// real form of the complex products (re/im interleaved per channel: row k = 2c + im,
+// column n = 2o + im).  One wavefront owns a tile of 16 consecutive 4D sites at fixed
+// (rhs, s) and issues v_mfma_f32_16x16x16_bf16 per stencil point (C_in = 8) or per 4
+// stencil points (C_in = 2): the A operand is the neighbour field transported in FP32
+// registers and rounded to BF16, the B operand the BF16 weight fragment of that s (kept
+// in registers while the wave walks tiles of one slice), FP32 accumulation.  Hidden
+// activations are stored as BF16 (32 bytes per site), inputs and outputs are the
+// solver's spinors.
+//
+// Lane layout of v_mfma_f32_16x16x16_bf16 (64 lanes, lane = 16 g + m):
+//   A[i][k]: lane supplies A[m][4g + j], j = 0..3;  B[k][n]: lane supplies B[4g + j][m];
+//   D[i][n]: lane holds D[4g + j][m].
+//
+// Training (dwf_net_train_device): forward, loss gradient, adjoint layers (the same
+// kernel with conjugate-transposed, point-reversed weight fragments), weight gradients
+// as MFMA outer-product sums G = sum_sites a^T g with deterministic block partials,
+// Adam on FP32 master weights, re-packing of the BF16 fragments -- all on the device.
+//
+// Solver (DWF_SOLVE_LEARNED): flexible CG (Notay's FCG(1)) on D^dag D in FP64 with
+// z = M r, certified by the FP64 true residual.
+
+#include "../../include/dwf_hip.h"
+#include "dwf_context.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+using namespace dwfk;
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+struct dwf_net {
+    int Ls = 0;
+    int device = 0;
+    float2* w[3] = {nullptr, nullptr, nullptr};   // FP32 master weights [Ls][7][Cout][Cin]
+    float2* m1[3] = {nullptr, nullptr, nullptr};  // Adam first moments
+    float2* m2[3] = {nullptr, nullptr, nullptr};  // Adam second moments
+    unsigned short* wf[3] = {nullptr, nullptr, nullptr};  // BF16 B fragments, forward
+    unsigned short* wa[3] = {nullptr, nullptr, nullptr};  // BF16 B fragments, adjoint
+    int adam_t = 0;
+    hipStream_t pack_stream = nullptr;  // for set_weights outside a context
+    ~dwf_net();
+};
+
+namespace {
+
+constexpr int kPts = 7;
+constexpr int kCinL[3] = {2, 8, 8};
+constexpr int kCoutL[3] = {8, 8, 2};
+
+__host__ __device__ constexpr int nstep_of(int cin) { return cin == 8 ? 7 : 2; }
+long long layer_cw(int l, int Ls) { return (long long)Ls * kPts * kCoutL[l] * kCinL[l]; }  // complex weights
+long long frag_shorts(int cin, int Ls) { return (long long)Ls * nstep_of(cin) * 64 * 4; }
+
+__device__ __forceinline__ unsigned short f2bf(float f)
+{
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite values)
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+struct PtcArgs {
+    int Nt, Nx, Ls, V4;
+    int tps;                   // 16-site tiles per (rhs, s) slice: V4 / 16
+    long long ntiles;          // nrhs * Ls * tps
+    const void* in;            // layer input
+    void* out;                 // layer output (forward kernel)
+    const int* in_row;         // nullable: right-hand side -> row of `in`
+    const int* cfg;            // right-hand side -> configuration of the links
+    const float2* links;       // [cfg][2][V4] (cos, sin)
+    long long link_stride;     // entries per configuration
+    const unsigned short* wpk; // B fragments [Ls][nstep][64][4]
+    const CgState* st;         // nullable: skip right-hand sides with flags[0] & F_DONE
+    int tpw;                   // tiles per wavefront
+};
+
+// Neighbour (slice, 4D site) and FP32 transporter of stencil point p at (s, t, x).
+struct Nb {
+    int s, i;
+    float c, sn;
+};
+
+__device__ __forceinline__ Nb neighbour(const PtcArgs& a, const float2* L, int p, int s, int t, int x)
+{
+    Nb r{s, t * a.Nx + x, 1.f, 0.f};
+    switch (p) {
+        case 1: {
+            const int t2 = t + 1 == a.Nt ? 0 : t + 1;
+            const float2 u = L[t * a.Nx + x];
+            const float sg = t == a.Nt - 1 ? -1.f : 1.f;
+            r.i = t2 * a.Nx + x;
+            r.c = sg * u.x;
+            r.sn = sg * u.y;
+            break;
+        }
+        case 2: {
+            const int t2 = t == 0 ? a.Nt - 1 : t - 1;
+            const float2 u = L[t2 * a.Nx + x];
+            const float sg = t == 0 ? -1.f : 1.f;
+            r.i = t2 * a.Nx + x;
+            r.c = sg * u.x;
+            r.sn = sg * -u.y;
+            break;
+        }
+        case 3: {
+            const int x2 = x + 1 == a.Nx ? 0 : x + 1;
+            const float2 u = L[a.V4 + t * a.Nx + x];
+            r.i = t * a.Nx + x2;
+            r.c = u.x;
+            r.sn = u.y;
+            break;
+        }
+        case 4: {
+            const int x2 = x == 0 ? a.Nx - 1 : x - 1;
+            const float2 u = L[a.V4 + t * a.Nx + x2];
+            r.i = t * a.Nx + x2;
+            r.c = u.x;
+            r.sn = -u.y;
+            break;
+        }
+        case 5: r.s = s + 1 == a.Ls ? 0 : s + 1; break;
+        case 6: r.s = s == 0 ? a.Ls - 1 : s - 1; break;
+        default: break;
+    }
+    return r;
+}
+
+// (hr + i hi)(c + i sn) in the explicit fma form the oracle mirrors
+__device__ __forceinline__ float tr_re(const Nb& n, float hr, float hi) { return fmaf(n.c, hr, -(n.sn * hi)); }
+__device__ __forceinline__ float tr_im(const Nb& n, float hr, float hi) { return fmaf(n.sn, hr, n.c * hi); }
+
+__device__ __forceinline__ float4 ld_spinor(const double* p, long long site)
+{
+    const double4 v = reinterpret_cast<const double4*>(p)[site];
+    return make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+}
+__device__ __forceinline__ float4 ld_spinor(const float* p, long long site)
+{
+    return reinterpret_cast<const float4*>(p)[site];
+}
+__device__ __forceinline__ void st_spinor(double* p, long long site, float a, float b, float c, float d)
+{
+    reinterpret_cast<double4*>(p)[site] = make_double4(a, b, c, d);
+}
+__device__ __forceinline__ void st_spinor(float* p, long long site, float a, float b, float c, float d)
+{
+    reinterpret_cast<float4*>(p)[site] = make_float4(a, b, c, d);
+}
+
+__device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d)
+{
+    bf16x4_t v;
+    v[0] = (short)f2bf(a);
+    v[1] = (short)f2bf(b);
+    v[2] = (short)f2bf(c);
+    v[3] = (short)f2bf(d);
+    return v;
+}
+
+// ------------------------------------------------------------------ forward layer
+// TIN/TOUT: double or float for spinors (2 channels), unsigned short for 8 BF16 channels.
+template <int CIN, int COUT, typename TIN, typename TOUT>
+__global__ __launch_bounds__(256) void k_ptc_fwd(const PtcArgs a)
+{
+    constexpr int NSTEP = nstep_of(CIN);
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+    const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long long T0 = wave * a.tpw;
+    const long long T1 = T0 + a.tpw < a.ntiles ? T0 + a.tpw : a.ntiles;
+    const long long per_rhs = (long long)a.Ls * a.tps;
+    int cur_s = -1;
+    bf16x4_t B[NSTEP];
+    for (long long T = T0; T < T1; ++T) {
+        const int rhs = (int)(T / per_rhs);
+        const int rem = (int)(T - rhs * per_rhs);
+        const int s = rem / a.tps;
+        const int i0 = (rem - s * a.tps) * 16;
+        if (a.st && (a.st[rhs].flags[0] & F_DONE)) continue;  // uniform per wavefront
+        if (s != cur_s) {
+            const bf16x4_t* wp = reinterpret_cast<const bf16x4_t*>(a.wpk) + (long long)s * NSTEP * 64 + lane;
+#pragma unroll
+            for (int q = 0; q < NSTEP; ++q) B[q] = wp[q * 64];
+            cur_s = s;
+        }
+        const int row = a.in_row ? a.in_row[rhs] : rhs;
+        const float2* L = a.links + (long long)a.cfg[rhs] * a.link_stride;
+        const int i = i0 + m, t = i / a.Nx, x = i - t * a.Nx;
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        if constexpr (CIN == 8) {
+            const unsigned short* inb =
+                static_cast<const unsigned short*>(a.in) + (long long)row * a.Ls * a.V4 * 16 + 4 * g;
+#pragma unroll
+            for (int p = 0; p < kPts; ++p) {
+                const Nb n = neighbour(a, L, p, s, t, x);
+                const ushort4 v = *reinterpret_cast<const ushort4*>(inb + ((long long)n.s * a.V4 + n.i) * 16);
+                const float h0r = bf2f(v.x), h0i = bf2f(v.y), h1r = bf2f(v.z), h1i = bf2f(v.w);
+                const bf16x4_t A = pack4(tr_re(n, h0r, h0i), tr_im(n, h0r, h0i), tr_re(n, h1r, h1i),
+                                         tr_im(n, h1r, h1i));
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B[p], acc, 0, 0, 0);
+            }
+        } else {
+            const TIN* inb = static_cast<const TIN*>(a.in) + (long long)row * a.Ls * a.V4 * 4;
+#pragma unroll
+            for (int q = 0; q < NSTEP; ++q) {
+                const int p = 4 * q + g;  // this lane group's stencil point
+                bf16x4_t A = {0, 0, 0, 0};
+                if (p < kPts) {
+                    const Nb n = neighbour(a, L, p, s, t, x);
+                    const float4 v = ld_spinor(inb, (long long)n.s * a.V4 + n.i);
+                    A = pack4(tr_re(n, v.x, v.y), tr_im(n, v.x, v.y), tr_re(n, v.z, v.w), tr_im(n, v.z, v.w));
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B[q], acc, 0, 0, 0);
+            }
+        }
+        const long long obase = ((long long)rhs * a.Ls + s) * a.V4 + i0;
+        if constexpr (COUT == 8) {
+            unsigned short* out = static_cast<unsigned short*>(a.out) + obase * 16;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float v = acc[j];
+                const float o = __shfl_xor(v, 1, 64);
+                if ((m & 1) == 0)
+                    *reinterpret_cast<unsigned*>(out + (4 * g + j) * 16 + m) =
+                        (unsigned)f2bf(v) | ((unsigned)f2bf(o) << 16);
+            }
+        } else {
+            TOUT* out = static_cast<TOUT*>(a.out);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float v0 = __shfl(acc[j], 16 * g + 0, 64), v1 = __shfl(acc[j], 16 * g + 1, 64);
+                const float v2 = __shfl(acc[j], 16 * g + 2, 64), v3 = __shfl(acc[j], 16 * g + 3, 64);
+                if (m == j) st_spinor(out, obase + 4 * g + j, v0, v1, v2, v3);
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------- weight gradient
+// G_s[step][k][n] = sum over the sites of slice s (all right-hand sides) of
+// a_p(site)[k] * g(site)[n], a_p the transported layer input, g the gradient of the
+// layer output.  Grid (blocks per slice, Ls); block partials [Ls][gridDim.x][NSTEP][256].
+template <int CIN, int COUT, typename TIN, typename TG>
+__global__ __launch_bounds__(256) void k_ptc_wgrad(const PtcArgs a, const TG* grad, float* part, int nrhs)
+{
+    constexpr int NSTEP = nstep_of(CIN);
+    __shared__ float red[4][NSTEP * 256];
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4, w = threadIdx.x >> 6;
+    const int s = blockIdx.y;
+    const long long per_s = (long long)nrhs * a.tps;
+    f32x4_t acc[NSTEP];
+#pragma unroll
+    for (int q = 0; q < NSTEP; ++q) acc[q] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    for (long long q = (long long)blockIdx.x * 4 + w; q < per_s; q += (long long)gridDim.x * 4) {
+        const int rhs = (int)(q / a.tps);
+        const int i0 = (int)(q - (long long)rhs * a.tps) * 16;
+        const int row = a.in_row ? a.in_row[rhs] : rhs;
+        const float2* L = a.links + (long long)a.cfg[rhs] * a.link_stride;
+        const long long gbase = ((long long)rhs * a.Ls + s) * a.V4 + i0;
+        // B operand: gradient component n = m at sites 4g + j
+        bf16x4_t Bv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if constexpr (COUT == 8) {
+                Bv[j] = (short)reinterpret_cast<const unsigned short*>(grad)[(gbase + 4 * g + j) * 16 + m];
+            } else {
+                float v = 0.f;
+                if (m < 4) {
+                    const float4 gv = ld_spinor(grad, gbase + 4 * g + j);
+                    v = m == 0 ? gv.x : m == 1 ? gv.y : m == 2 ? gv.z : gv.w;
+                }
+                Bv[j] = (short)f2bf(v);
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < NSTEP; ++st) {
+            bf16x4_t A;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int i = i0 + 4 * g + j, t = i / a.Nx, x = i - t * a.Nx;
+                float v = 0.f;
+                if constexpr (CIN == 8) {
+                    const Nb n = neighbour(a, L, st, s, t, x);
+                    const unsigned short* hp = static_cast<const unsigned short*>(a.in) +
+                                               (((long long)row * a.Ls + n.s) * a.V4 + n.i) * 16 + 2 * (m >> 1);
+                    const ushort2 h = *reinterpret_cast<const ushort2*>(hp);
+                    const float hr = bf2f(h.x), hi = bf2f(h.y);
+                    v = (m & 1) ? tr_im(n, hr, hi) : tr_re(n, hr, hi);
+                } else {
+                    const int p = 4 * st + (m >> 2), comp = m & 3;
+                    if (p < kPts) {
+                        const Nb n = neighbour(a, L, p, s, t, x);
+                        const float4 h = ld_spinor(static_cast<const TIN*>(a.in) + (long long)row * a.Ls * a.V4 * 4,
+                                                   (long long)n.s * a.V4 + n.i);
+                        const float hr = comp < 2 ? h.x : h.z, hi = comp < 2 ? h.y : h.w;
+                        v = (comp & 1) ? tr_im(n, hr, hi) : tr_re(n, hr, hi);
+                    }
+                }
+                A[j] = (short)f2bf(v);
+            }
+            acc[st] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, Bv, acc[st], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int st = 0; st < NSTEP; ++st)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[w][st * 256 + (4 * g + j) * 16 + m] = acc[st][j];
+    __syncthreads();
+    float* out = part + ((long long)s * gridDim.x + blockIdx.x) * NSTEP * 256;
+    for (int e = threadIdx.x; e < NSTEP * 256; e += 256) out[e] = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+}
+
+// GB[s][e] = sum_b part[s][b][e] (fixed order)
+__global__ void k_wgrad_reduce(const float* part, int nb, int per, int Ls, double* GB)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)Ls * per) return;
+    const int s = (int)(idx / per), e = (int)(idx - (long long)s * per);
+    double v = 0.0;
+    for (int b = 0; b < nb; ++b) v += part[((long long)s * nb + b) * per + e];
+    GB[idx] = v;
+}
+
+// Complex weight gradient from GB and (optionally) one Adam step on the master weights.
+__global__ void k_fold_adam(const double* GB, int cin, int cout, int Ls, float2* w, float2* m1, float2* m2,
+                            float* grad_out, int adam, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int n = Ls * kPts * cout * cin;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int c = idx % cin, o = (idx / cin) % cout, p = (idx / (cin * cout)) % kPts, s = idx / (cin * cout * kPts);
+    const int NSTEP = nstep_of(cin);
+    const int step = cin == 8 ? p : p / 4;
+    const int k0 = cin == 8 ? 2 * c : 4 * (p % 4) + 2 * c;
+    const double* G = GB + ((long long)s * NSTEP + step) * 256;
+    const int n0 = 2 * o;
+    const float gr = (float)(G[k0 * 16 + n0] + G[(k0 + 1) * 16 + n0 + 1]);
+    const float gi = (float)(G[k0 * 16 + n0 + 1] - G[(k0 + 1) * 16 + n0]);
+    if (grad_out) {
+        grad_out[2 * idx] = gr;
+        grad_out[2 * idx + 1] = gi;
+    }
+    if (!adam) return;
+    float2 a = m1[idx], v = m2[idx], x = w[idx];
+    a.x = b1 * a.x + (1.f - b1) * gr;
+    a.y = b1 * a.y + (1.f - b1) * gi;
+    v.x = b2 * v.x + (1.f - b2) * gr * gr;
+    v.y = b2 * v.y + (1.f - b2) * gi * gi;
+    x.x -= lr * (a.x * c1) / (sqrtf(v.x * c2) + eps);
+    x.y -= lr * (a.y * c1) / (sqrtf(v.y * c2) + eps);
+    m1[idx] = a;
+    m2[idx] = v;
+    w[idx] = x;
+}
+
+// BF16 B fragments of layer (cin_l -> cout_l): forward, or the adjoint kernel's
+// (cout_l -> cin_l) with W'_p(s) = W_{-p}(s + ds_p)^H.
+__global__ void k_pack(const float2* w, int cin_l, int cout_l, int Ls, int adj, unsigned short* out)
+{
+    const int kin = adj ? cout_l : cin_l, kout = adj ? cin_l : cout_l;
+    const int NSTEP = nstep_of(kin);
+    const long long total = (long long)Ls * NSTEP * 256;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int jj = (int)(idx & 3), lane = (int)((idx >> 2) & 63);
+    const int step = (int)((idx >> 8) % NSTEP), s = (int)(idx / (256LL * NSTEP));
+    const int g = lane >> 4, n = lane & 15, k = 4 * g + jj;
+    int p, c, kr;
+    if (kin == 8) {
+        p = step;
+        c = k >> 1;
+        kr = k & 1;
+    } else {
+        p = 4 * step + g;
+        c = jj >> 1;
+        kr = jj & 1;
+    }
+    const int o = n >> 1, nr = n & 1;
+    float val = 0.f;
+    if (p < kPts && o < kout) {
+        float wr, wi;
+        if (!adj) {
+            const float2 W = w[(((long long)s * kPts + p) * cout_l + o) * cin_l + c];
+            wr = W.x;
+            wi = W.y;
+        } else {
+            const int neg[kPts] = {0, 2, 1, 4, 3, 6, 5};
+            const int ds = p == 5 ? 1 : p == 6 ? -1 : 0;
+            const int sn = (s + ds + Ls) % Ls;
+            const float2 W = w[(((long long)sn * kPts + neg[p]) * cout_l + c) * cin_l + o];
+            wr = W.x;
+            wi = -W.y;
+        }
+        val = kr == 0 ? (nr == 0 ? wr : wi) : (nr == 0 ? -wi : wr);
+    }
+    out[idx] = f2bf(val);
+}
+
+// ----------------------------------------------------------------- reductions
+__device__ __forceinline__ double wsum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sums of a and b (256 threads) into part[0..1] (thread 0 writes).
+__device__ __forceinline__ void block_part2(double a, double b, double* part)
+{
+    __shared__ double sh[8];
+    a = wsum(a);
+    b = wsum(b);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sh[2 * w] = a;
+        sh[2 * w + 1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[0] = sh[0] + sh[2] + sh[4] + sh[6];
+        part[1] = sh[1] + sh[3] + sh[5] + sh[7];
+    }
+}
+
+// Sum of element `which` of n partial pairs (64-thread block, every thread gets it).
+__device__ __forceinline__ double fold64(const double* p, int n, int which)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) v += p[2 * i + which];
+    return wsum(v);
+}
+
+// |x_r|^2 of float spinor rows (one block per row)
+__global__ void k_row_norms(const float* x, long long sites, double* out)
+{
+    double a = 0.0;
+    const float4* v = reinterpret_cast<const float4*>(x) + (long long)blockIdx.x * sites;
+    for (long long i = threadIdx.x; i < sites; i += 256) {
+        const float4 q = v[i];
+        a += (double)q.x * q.x + (double)q.y * q.y + (double)q.z * q.z + (double)q.w * q.w;
+    }
+    __shared__ double tmp[2];
+    block_part2(a, 0.0, tmp);
+    if (threadIdx.x == 0) out[blockIdx.x] = tmp[0];
+}
+
+// Solution loss: d = y - x_row, gy = d * 2 / (npairs |x_row|^2), partial |d|^2.
+__global__ __launch_bounds__(256) void k_loss_sol(const float* y, const float* x, const int* row,
+                                                  const double* xn2, long long sites, int npairs, float* gy,
+                                                  double* part)
+{
+    const int rhs = blockIdx.y, r = row[rhs];
+    const float sc = (float)(2.0 / ((double)npairs * xn2[r]));
+    const float4* Y = reinterpret_cast<const float4*>(y) + (long long)rhs * sites;
+    const float4* X = reinterpret_cast<const float4*>(x) + (long long)r * sites;
+    float4* G = reinterpret_cast<float4*>(gy) + (long long)rhs * sites;
+    double a = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < sites; i += (long long)gridDim.x * 256) {
+        const float4 u = Y[i], v = X[i];
+        const float4 d = make_float4(u.x - v.x, u.y - v.y, u.z - v.z, u.w - v.w);
+        a += (double)d.x * d.x + (double)d.y * d.y + (double)d.z * d.z + (double)d.w * d.w;
+        G[i] = make_float4(d.x * sc, d.y * sc, d.z * sc, d.w * sc);
+    }
+    block_part2(a, 0.0, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
+}
+
+// Residual loss, first half: d = u - b_row (u = A y), partial |d|^2; d overwrites u.
+__global__ __launch_bounds__(256) void k_loss_res_d(float* u, const float* b, const int* row, long long sites,
+                                                    double* part)
+{
+    const int rhs = blockIdx.y, r = row[rhs];
+    float4* U = reinterpret_cast<float4*>(u) + (long long)rhs * sites;
+    const float4* Bv = reinterpret_cast<const float4*>(b) + (long long)r * sites;
+    double a = 0.0;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < sites; i += (long long)gridDim.x * 256) {
+        const float4 p = U[i], q = Bv[i];
+        const float4 d = make_float4(p.x - q.x, p.y - q.y, p.z - q.z, p.w - q.w);
+        a += (double)d.x * d.x + (double)d.y * d.y + (double)d.z * d.z + (double)d.w * d.w;
+        U[i] = d;
+    }
+    block_part2(a, 0.0, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
+}
+
+// gy *= 2 / (npairs |b_row|^2)  (residual loss: gy = A d scaled)
+__global__ void k_scale_rows(float* v, const int* row, const double* n2, long long sites, int npairs)
+{
+    const int rhs = blockIdx.y;
+    const float sc = (float)(2.0 / ((double)npairs * n2[row[rhs]]));
+    float4* V = reinterpret_cast<float4*>(v) + (long long)rhs * sites;
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < sites; i += (long long)gridDim.x * 256) {
+        float4 q = V[i];
+        q.x *= sc;
+        q.y *= sc;
+        q.z *= sc;
+        q.w *= sc;
+        V[i] = q;
+    }
+}
+
+// loss = (1/npairs) sum_rhs (sum_b part) / n2[row]  -> *out
+__global__ void k_loss_fold(const double* part, int nb, const int* row, const double* n2, int nrhs, int npairs,
+                            double* out)
+{
+    double L = 0.0;
+    for (int r = 0; r < nrhs; ++r) {
+        const double d = fold64(part + (long long)r * nb * 2, nb, 0);
+        L += d / n2[row[r]];
+    }
+    if (threadIdx.x == 0) *out = L / npairs;
+}
+
+// ------------------------------------------------------------- flexible CG (FP64)
+struct FcgScal {
+    double rz, pq, alpha, beta;
+};
+
+template <typename F> __device__ __forceinline__ void for_sites(long long n, F f)
+{
+    for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) f(i);
+}
+
+__device__ __forceinline__ double dotr(const double4& a, const double4& b)
+{
+    return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+}
+
+// r = b - q; partials |r|^2, |b|^2
+__global__ __launch_bounds__(256) void k_fcg_init(const double* b, const double* q, double* r, long long sites,
+                                                  const CgState* st, double* part)
+{
+    const int rhs = blockIdx.y;
+    const long long o = (long long)rhs * sites;
+    const double4* B = reinterpret_cast<const double4*>(b) + o;
+    const double4* Q = reinterpret_cast<const double4*>(q) + o;
+    double4* R = reinterpret_cast<double4*>(r) + o;
+    double a = 0.0, c = 0.0;
+    for_sites(sites, [&](long long i) {
+        const double4 bb = B[i], qq = Q[i];
+        const double4 rr = make_double4(bb.x - qq.x, bb.y - qq.y, bb.z - qq.z, bb.w - qq.w);
+        R[i] = rr;
+        a += dotr(rr, rr);
+        c += dotr(bb, bb);
+    });
+    (void)st;
+    block_part2(a, c, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
+}
+
+// Start of a pass: |b|^2, |r|^2; done if converged (or b = 0).
+__global__ void k_fcg_start(const double* part, int nb, CgState* st, double tol, int reset)
+{
+    const int rhs = blockIdx.x;
+    const double rr = fold64(part + (long long)rhs * nb * 2, nb, 0);
+    const double bb = fold64(part + (long long)rhs * nb * 2, nb, 1);
+    if (threadIdx.x == 0) {
+        CgState& S = st[rhs];
+        S.bnorm2 = bb;
+        S.rr[0] = rr;
+        S.tol = tol;
+        S.relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+        S.status = 0;
+        S.flags[0] = (bb == 0.0 || S.relres <= tol) ? F_DONE : 0;
+        S.flags[1] = S.flags[0];
+        if (reset) {
+            S.iters = 0;
+            S.reliable = 0;
+            S.fp64_iters = 0;
+        }
+    }
+}
+
+// p = z; partial (z, r)
+__global__ __launch_bounds__(256) void k_fcg_p0(const double* z, const double* r, double* p, long long sites,
+                                                const CgState* st, double* part)
+{
+    const int rhs = blockIdx.y;
+    if (st[rhs].flags[0] & F_DONE) return;
+    const long long o = (long long)rhs * sites;
+    const double4* Z = reinterpret_cast<const double4*>(z) + o;
+    const double4* R = reinterpret_cast<const double4*>(r) + o;
+    double4* P = reinterpret_cast<double4*>(p) + o;
+    double a = 0.0;
+    for_sites(sites, [&](long long i) {
+        const double4 zz = Z[i];
+        P[i] = zz;
+        a += dotr(zz, R[i]);
+    });
+    block_part2(a, 0.0, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
+}
+
+__global__ void k_fcg_rz(const double* part, int nb, CgState* st, FcgScal* fs)
+{
+    const int rhs = blockIdx.x;
+    if (st[rhs].flags[0] & F_DONE) return;
+    const double zr = fold64(part + (long long)rhs * nb * 2, nb, 0);
+    if (threadIdx.x == 0) {
+        fs[rhs].rz = zr;
+        if (!(zr > 0.0)) {  // preconditioner not positive on this residual
+            st[rhs].status = 5;
+            st[rhs].flags[0] |= F_DONE;
+        }
+    }
+}
+
+// alpha = (z, r) / (p, A p), (p, A p) = |D p|^2 from the stencil's partials
+__global__ void k_fcg_alpha(const double* hpart, int hnb, CgState* st, FcgScal* fs)
+{
+    const int rhs = blockIdx.x;
+    if (st[rhs].flags[0] & F_DONE) return;
+    const double pq = fold64(hpart + (long long)rhs * hnb * 2, hnb, 0);
+    if (threadIdx.x == 0) {
+        fs[rhs].pq = pq;
+        if (!(pq > 0.0)) {
+            st[rhs].status = 4;
+            st[rhs].flags[0] |= F_DONE;
+        } else {
+            fs[rhs].alpha = fs[rhs].rz / pq;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha q; partial |r|^2
+__global__ __launch_bounds__(256) void k_fcg_xr(double* x, double* r, const double* p, const double* q,
+                                                long long sites, const CgState* st, const FcgScal* fs, double* part)
+{
+    const int rhs = blockIdx.y;
+    if (st[rhs].flags[0] & F_DONE) return;
+    const double al = fs[rhs].alpha;
+    const long long o = (long long)rhs * sites;
+    double4* X = reinterpret_cast<double4*>(x) + o;
+    double4* R = reinterpret_cast<double4*>(r) + o;
+    const double4* P = reinterpret_cast<const double4*>(p) + o;
+    const double4* Q = reinterpret_cast<const double4*>(q) + o;
+    double a = 0.0;
+    for_sites(sites, [&](long long i) {
+        const double4 pp = P[i], qq = Q[i];
+        double4 xx = X[i], rr = R[i];
+        xx.x += al * pp.x;
+        xx.y += al * pp.y;
+        xx.z += al * pp.z;
+        xx.w += al * pp.w;
+        rr.x -= al * qq.x;
+        rr.y -= al * qq.y;
+        rr.z -= al * qq.z;
+        rr.w -= al * qq.w;
+        X[i] = xx;
+        R[i] = rr;
+        a += dotr(rr, rr);
+    });
+    block_part2(a, 0.0, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
+}
+
+__global__ void k_fcg_conv(const double* part, int nb, CgState* st)
+{
+    const int rhs = blockIdx.x;
+    if (st[rhs].flags[0] & F_DONE) return;
+    const double rr = fold64(part + (long long)rhs * nb * 2, nb, 0);
+    if (threadIdx.x == 0) {
+        CgState& S = st[rhs];
+        S.rr[0] = rr;
+        S.iters += 1;
+        S.relres = std::sqrt(rr / S.bnorm2);
+        if (S.relres <= S.tol) S.flags[0] |= F_DONE;
+    }
+}
+
+// partials (z, r), (z, q)
+__global__ __launch_bounds__(256) void k_fcg_dots(const double* z, const double* r, const double* q,
+                                                  long long sites, const CgState* st, double* part)
+{
+    const int rhs = blockIdx.y;
+    if (st[rhs].flags[0] & F_DONE) return;
+    const long long o = (long long)rhs * sites;
+    const double4* Z = reinterpret_cast<const double4*>(z) + o;
+    const double4* R = reinterpret_cast<const double4*>(r) + o;
+    const double4* Q = reinterpret_cast<const double4*>(q) + o;
+    double a = 0.0, c = 0.0;
+    for_sites(sites, [&](long long i) {
+        const double4 zz = Z[i];
+        a += dotr(zz, R[i]);
+        c += dotr(zz, Q[i]);
+    });
+    block_part2(a, c, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
+}
+
+// beta = -(z, A p) / (p, A p) (A-orthogonal to the previous direction); rz = (z, r)
+__global__ void k_fcg_beta(const double* part, int nb, CgState* st, FcgScal* fs)
+{
+    const int rhs = blockIdx.x;
+    if (st[rhs].flags[0] & F_DONE) return;
+    const double zr = fold64(part + (long long)rhs * nb * 2, nb, 0);
+    const double zq = fold64(part + (long long)rhs * nb * 2, nb, 1);
+    if (threadIdx.x == 0) {
+        if (!(zr > 0.0)) {
+            st[rhs].status = 5;
+            st[rhs].flags[0] |= F_DONE;
+        } else {
+            fs[rhs].rz = zr;
+            fs[rhs].beta = -zq / fs[rhs].pq;
+        }
+    }
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(256) void k_fcg_p(const double* z, double* p, long long sites, const CgState* st,
+                                               const FcgScal* fs)
+{
+    const int rhs = blockIdx.y;
+    if (st[rhs].flags[0] & F_DONE) return;
+    const double be = fs[rhs].beta;
+    const long long o = (long long)rhs * sites;
+    const double4* Z = reinterpret_cast<const double4*>(z) + o;
+    double4* P = reinterpret_cast<double4*>(p) + o;
+    for_sites(sites, [&](long long i) {
+        const double4 zz = Z[i];
+        double4 pp = P[i];
+        pp.x = zz.x + be * pp.x;
+        pp.y = zz.y + be * pp.y;
+        pp.z = zz.z + be * pp.z;
+        pp.w = zz.w + be * pp.w;
+        P[i] = pp;
+    });
+}
+
+// true residual of the pass: |b - A x| / |b|
+__global__ void k_fcg_true(const double* part, int nb, CgState* st)
+{
+    const int rhs = blockIdx.x;
+    const double rr = fold64(part + (long long)rhs * nb * 2, nb, 0);
+    if (threadIdx.x == 0) {
+        CgState& S = st[rhs];
+        S.true_relres = S.bnorm2 > 0.0 ? std::sqrt(rr / S.bnorm2) : 0.0;
+    }
+}
+
+__global__ void k_iota_cfg(int* cfg, int nrhs, int rhs_per_cfg)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrhs) cfg[i] = i / rhs_per_cfg;
+}
+
+// ------------------------------------------------------------------- host side
+int guard(const std::function<int()>& f)
+{
+    try {
+        return f();
+    } catch (const DwfError& e) {
+        dwf_set_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        dwf_set_error(e.what());
+        return DWF_ERR_ARG;
+    }
+}
+
+void hchk(hipError_t e, const char* what) { dwf_ctx::check(e, what); }
+
+int nb_vec(long long sites) { return (int)std::max(1LL, std::min(256LL, (sites + 1023) / 1024)); }
+
+// Launch one layer: forward weights (adj = 0) or the adjoint kernel of layer l.
+struct LayerIO {
+    const void* in;
+    const int* in_row;
+    void* out;
+    int in_kind;   // 0 double spinor, 1 float spinor, 2 bf16 channels
+    int out_kind;  // 0 double spinor, 1 float spinor, 2 bf16 channels
+};
+
+PtcArgs ptc_args(const dwf_ctx* c, const dwf_net* net, int l, bool adj, const LayerIO& io, const int* cfg, int nrhs,
+                 const CgState* st)
+{
+    const Lat& L = c->L;
+    PtcArgs a{};
+    a.Nt = L.Nt;
+    a.Nx = L.Nx;
+    a.Ls = L.Ls;
+    a.V4 = L.V4;
+    a.tps = L.V4 / 16;
+    a.ntiles = (long long)nrhs * L.Ls * a.tps;
+    a.in = io.in;
+    a.out = io.out;
+    a.in_row = io.in_row;
+    a.cfg = cfg;
+    a.links = c->l32;
+    a.link_stride = 2LL * L.V4;
+    a.wpk = adj ? net->wa[l] : net->wf[l];
+    a.st = st;
+    a.tpw = (int)std::max(1LL, std::min(16LL, a.ntiles / 4096));
+    return a;
+}
+
+void launch_layer(dwf_ctx* c, const dwf_net* net, int l, bool adj, const LayerIO& io, const int* cfg, int nrhs,
+                  const CgState* st)
+{
+    const PtcArgs a = ptc_args(c, net, l, adj, io, cfg, nrhs, st);
+    const long long waves = (a.ntiles + a.tpw - 1) / a.tpw;
+    const unsigned blocks = (unsigned)((waves + 3) / 4);
+    const int cin = adj ? kCoutL[l] : kCinL[l], cout = adj ? kCinL[l] : kCoutL[l];
+    hipStream_t s = c->stream;
+    if (cin == 2 && cout == 8) {
+        if (io.in_kind == 0)
+            k_ptc_fwd<2, 8, double, unsigned short><<<blocks, 256, 0, s>>>(a);
+        else
+            k_ptc_fwd<2, 8, float, unsigned short><<<blocks, 256, 0, s>>>(a);
+    } else if (cin == 8 && cout == 8) {
+        k_ptc_fwd<8, 8, unsigned short, unsigned short><<<blocks, 256, 0, s>>>(a);
+    } else {  // 8 -> 2
+        if (io.out_kind == 0)
+            k_ptc_fwd<8, 2, unsigned short, double><<<blocks, 256, 0, s>>>(a);
+        else
+            k_ptc_fwd<8, 2, unsigned short, float><<<blocks, 256, 0, s>>>(a);
+    }
+}
+
+void require_net(const dwf_ctx* c, const dwf_net* net)
+{
+    if (!c || !net) throw DwfError(DWF_ERR_ARG, "learned preconditioner: null argument");
+    if (c->kind != 0) throw DwfError(DWF_ERR_ARG, "learned preconditioner: needs a domain-wall context");
+    if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "learned preconditioner: no gauge field set");
+    if (net->Ls != c->L.Ls) throw DwfError(DWF_ERR_ARG, "learned preconditioner: network Ls differs from the lattice's");
+    if (c->L.V4 % 16 != 0)
+        throw DwfError(DWF_ERR_ARG, "learned preconditioner: Nt*Nx must be a multiple of 16");
+}
+
+void repack(const dwf_net* net, hipStream_t s)
+{
+    for (int l = 0; l < 3; ++l) {
+        const long long nf = frag_shorts(kCinL[l], net->Ls), na = frag_shorts(kCoutL[l], net->Ls);
+        k_pack<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(net->w[l], kCinL[l], kCoutL[l], net->Ls, 0, net->wf[l]);
+        k_pack<<<(unsigned)((na + 255) / 256), 256, 0, s>>>(net->w[l], kCinL[l], kCoutL[l], net->Ls, 1, net->wa[l]);
+    }
+    dwf_ctx::check_launch("pack weights");
+}
+
+// z = M in for nrhs right-hand sides (double spinors), configuration per cfg[rhs].
+void net_forward(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind, const int* in_row, void* out,
+                 int out_kind, const int* cfg, int nrhs, const CgState* st, unsigned short* h1, unsigned short* h2)
+{
+    launch_layer(c, net, 0, false, LayerIO{in, in_row, h1, in_kind, 2}, cfg, nrhs, st);
+    launch_layer(c, net, 1, false, LayerIO{h1, nullptr, h2, 2, 2}, cfg, nrhs, st);
+    launch_layer(c, net, 2, false, LayerIO{h2, nullptr, out, 2, out_kind}, cfg, nrhs, st);
+}
+
+void net_adjoint(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind, void* out, int out_kind,
+                 const int* cfg, int nrhs, unsigned short* h1, unsigned short* h2)
+{
+    launch_layer(c, net, 2, true, LayerIO{in, nullptr, h2, in_kind, 2}, cfg, nrhs, nullptr);
+    launch_layer(c, net, 1, true, LayerIO{h2, nullptr, h1, 2, 2}, cfg, nrhs, nullptr);
+    launch_layer(c, net, 0, true, LayerIO{h1, nullptr, out, 2, out_kind}, cfg, nrhs, nullptr);
+}
+
+int* cfg_identity_map(dwf_ctx* c, int nrhs)
+{
+    int* cfg = (int*)c->buf(B_NET_CFG, (size_t)nrhs * sizeof(int));
+    k_iota_cfg<<<(nrhs + 63) / 64, 64, 0, c->stream>>>(cfg, nrhs, c->ncfg <= 1 ? (1 << 30) : c->rhs_per_cfg);
+    return cfg;
+}
+
+template <typename R> HopArgs<R> hop_args(const dwf_ctx* c, int nrhs, const int* cfg_map)
+{
+    HopArgs<R> a{};
+    a.L = c->L;
+    a.cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store);
+    a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
+    a.link_stride = 2LL * c->L.V4;
+    a.rhs_per_cfg = c->ncfg <= 1 ? (1 << 30) : c->rhs_per_cfg;
+    a.cfg_map = cfg_map;
+    return a;
+}
+
+// Weight gradients of all layers from the stored activations (training step core).
+// Buffers: h0 the network input rows, h1/h2 the hidden activations (overwritten by the
+// activation gradients), gy the output gradient (float spinors).
+void backward(dwf_ctx* c, dwf_net* net, const float* h0, const int* row, const int* cfg, int nrhs, unsigned short* h1,
+              unsigned short* h2, const float* gy, double* GB[3])
+{
+    const Lat& L = c->L;
+    hipStream_t s = c->stream;
+    const long long per_s = (long long)nrhs * (L.V4 / 16);
+    const int nbx = (int)std::max(1LL, std::min(128LL, per_s / 8));
+    float* part = (float*)c->buf(B_NET_WPART, (size_t)L.Ls * nbx * 7 * 256 * sizeof(float));
+    auto reduce = [&](int cin, double* out) {
+        const int per = nstep_of(cin) * 256;
+        const long long n = (long long)L.Ls * per;
+        k_wgrad_reduce<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(part, nbx, per, L.Ls, out);
+    };
+    const dim3 grid(nbx, L.Ls);
+    // layer 3 (8 -> 2): input h2, output gradient gy
+    {
+        const PtcArgs a = ptc_args(c, net, 2, false, LayerIO{h2, nullptr, nullptr, 2, 1}, cfg, nrhs, nullptr);
+        k_ptc_wgrad<8, 2, unsigned short, float><<<grid, 256, 0, s>>>(a, gy, part, nrhs);
+        reduce(8, GB[2]);
+    }
+    // g2 = L3^dag gy  (into h2)
+    launch_layer(c, net, 2, true, LayerIO{gy, nullptr, h2, 1, 2}, cfg, nrhs, nullptr);
+    // layer 2 (8 -> 8): input h1, output gradient g2
+    {
+        const PtcArgs a = ptc_args(c, net, 1, false, LayerIO{h1, nullptr, nullptr, 2, 2}, cfg, nrhs, nullptr);
+        k_ptc_wgrad<8, 8, unsigned short, unsigned short><<<grid, 256, 0, s>>>(a, h2, part, nrhs);
+        reduce(8, GB[1]);
+    }
+    // g1 = L2^dag g2  (into h1)
+    launch_layer(c, net, 1, true, LayerIO{h2, nullptr, h1, 2, 2}, cfg, nrhs, nullptr);
+    // layer 1 (2 -> 8): input h0 rows, output gradient g1
+    {
+        const PtcArgs a = ptc_args(c, net, 0, false, LayerIO{h0, row, nullptr, 1, 2}, cfg, nrhs, nullptr);
+        k_ptc_wgrad<2, 8, float, unsigned short><<<grid, 256, 0, s>>>(a, h1, part, nrhs);
+        reduce(2, GB[0]);
+    }
+    dwf_ctx::check_launch("network backward");
+}
+
+// Loss and output gradient for one batch; leaves gy and the loss value in *loss_dev.
+void loss_and_grad(dwf_ctx* c, int loss, const float* y, const float* b, const float* x, const int* row,
+                   const int* cfg, const double* n2, int nrhs, int npairs, float* gy, double* loss_dev)
+{
+    const Lat& L = c->L;
+    const long long sites = L.V5();
+    hipStream_t s = c->stream;
+    const int nb = nb_vec(sites);
+    double* part = (double*)c->buf(B_NET_PART, (size_t)nrhs * nb * 2 * 8);
+    if (loss == 0) {
+        k_loss_sol<<<dim3(nb, nrhs), 256, 0, s>>>(y, x, row, n2, sites, npairs, gy, part);
+    } else {
+        // u = A y (A = D^dag D, FP32 stencils), d = u - b, gy = 2 A d / (npairs |b|^2)
+        float* t = (float*)c->buf(B_NET_T, (size_t)nrhs * sites * 16);
+        HopArgs<float> a = hop_args<float>(c, nrhs, cfg);
+        a.in = y;
+        a.out = t;
+        launch_hop<float>(false, EPI_STORE, a, nrhs, s);
+        a.in = t;
+        a.out = gy;
+        launch_hop<float>(true, EPI_STORE, a, nrhs, s);
+        k_loss_res_d<<<dim3(nb, nrhs), 256, 0, s>>>(gy, b, row, sites, part);
+        a.in = gy;
+        a.out = t;
+        launch_hop<float>(false, EPI_STORE, a, nrhs, s);
+        a.in = t;
+        a.out = gy;
+        launch_hop<float>(true, EPI_STORE, a, nrhs, s);
+        k_scale_rows<<<dim3(nb, nrhs), 256, 0, s>>>(gy, row, n2, sites, npairs);
+    }
+    k_loss_fold<<<1, 64, 0, s>>>(part, nb, row, n2, nrhs, npairs, loss_dev);
+    dwf_ctx::check_launch("loss");
+}
+
+struct TrainBufs {
+    unsigned short *h1, *h2;
+    float *y, *gy;
+    int *row, *cfg;
+    double* GB[3];
+};
+
+TrainBufs train_bufs(dwf_ctx* c, int nrhs)
+{
+    const long long sites = c->L.V5();
+    TrainBufs t;
+    t.h1 = (unsigned short*)c->buf(B_NET_H1, (size_t)nrhs * sites * 32);
+    t.h2 = (unsigned short*)c->buf(B_NET_H2, (size_t)nrhs * sites * 32);
+    t.y = (float*)c->buf(B_NET_Y, (size_t)nrhs * sites * 16);
+    t.gy = (float*)c->buf(B_NET_GY, (size_t)nrhs * sites * 16);
+    t.row = (int*)c->buf(B_NET_ROW, (size_t)nrhs * sizeof(int));
+    t.cfg = (int*)c->buf(B_NET_CFG, (size_t)nrhs * sizeof(int));
+    double* gb = (double*)c->buf(B_NET_GB, (size_t)c->L.Ls * (2 + 7 + 7) * 256 * 8);
+    t.GB[0] = gb;
+    t.GB[1] = gb + (size_t)c->L.Ls * 2 * 256;
+    t.GB[2] = gb + (size_t)c->L.Ls * 9 * 256;
+    return t;
+}
+
+void check_pairs(const dwf_ctx* c, const float* b, const float* x, const int* pair_cfg, int npairs, int loss)
+{
+    if (!b || (loss == 0 && !x) || !pair_cfg || npairs < 1)
+        throw DwfError(DWF_ERR_ARG, "learned preconditioner: bad training pairs");
+    if (loss != 0 && loss != 1) throw DwfError(DWF_ERR_ARG, "learned preconditioner: loss must be 0 or 1");
+    for (int i = 0; i < npairs; ++i)
+        if (pair_cfg[i] < 0 || pair_cfg[i] >= c->ncfg)
+            throw DwfError(DWF_ERR_ARG, "learned preconditioner: pair configuration outside the configurations held");
+}
+
+}  // namespace
+
+dwf_net::~dwf_net()
+{
+    for (int l = 0; l < 3; ++l) {
+        (void)hipFree(w[l]);
+        (void)hipFree(m1[l]);
+        (void)hipFree(m2[l]);
+        (void)hipFree(wf[l]);
+        (void)hipFree(wa[l]);
+    }
+    if (pack_stream) (void)hipStreamDestroy(pack_stream);
+}
+
+// ---------------------------------------------------------------- C ABI
+extern "C" long long dwf_net_num_weights(int Ls)
+{
+    if (Ls < 1) return 0;
+    long long n = 0;
+    for (int l = 0; l < 3; ++l) n += layer_cw(l, Ls);
+    return 2 * n;
+}
+
+extern "C" int dwf_net_create(dwf_net** out, int Ls, unsigned long long seed)
+{
+    return guard([&]() -> int {
+        if (!out || Ls < 2 || Ls > 64) throw DwfError(DWF_ERR_ARG, "dwf_net_create: Ls must be 2..64");
+        *out = nullptr;
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+            throw DwfError(DWF_ERR_HIP, "no HIP device available: libdwfhip needs an MI355X (gfx950) GPU");
+        std::unique_ptr<dwf_net> net(new dwf_net);
+        net->Ls = Ls;
+        hchk(hipGetDevice(&net->device), "hipGetDevice");
+        hchk(hipStreamCreateWithFlags(&net->pack_stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (int l = 0; l < 3; ++l) {
+            const size_t nw = (size_t)layer_cw(l, Ls) * sizeof(float2);
+            hchk(hipMalloc(&net->w[l], nw), "hipMalloc weights");
+            hchk(hipMalloc(&net->m1[l], nw), "hipMalloc weights");
+            hchk(hipMalloc(&net->m2[l], nw), "hipMalloc weights");
+            hchk(hipMalloc(&net->wf[l], (size_t)frag_shorts(kCinL[l], Ls) * 2), "hipMalloc weights");
+            hchk(hipMalloc(&net->wa[l], (size_t)frag_shorts(kCoutL[l], Ls) * 2), "hipMalloc weights");
+        }
+        // Initial weights: identity embeddings on the self point (layer 3 scaled by 1/4,
+        // about the inverse of D^dag D's typical eigenvalue), small random elsewhere.
+        std::vector<float> w((size_t)dwf_net_num_weights(Ls));
+        std::mt19937_64 rng(seed);
+        std::normal_distribution<float> nd(0.0f, 1e-3f);
+        size_t off = 0;
+        for (int l = 0; l < 3; ++l) {
+            const int ci = kCinL[l], co = kCoutL[l];
+            for (int s = 0; s < Ls; ++s)
+                for (int p = 0; p < kPts; ++p)
+                    for (int o = 0; o < co; ++o)
+                        for (int cc = 0; cc < ci; ++cc) {
+                            float re = nd(rng), im = nd(rng);
+                            if (p == 0 && o == cc && o < 2) re += (l == 2) ? 0.25f : 1.0f;
+                            if (p == 0 && l == 1 && o == cc && o >= 2) re += 1.0f;
+                            w[off++] = re;
+                            w[off++] = im;
+                        }
+        }
+        dwf_net* raw = net.release();
+        const int rc = dwf_net_set_weights(raw, w.data());
+        if (rc != DWF_OK) {
+            delete raw;
+            return rc;
+        }
+        *out = raw;
+        return DWF_OK;
+    });
+}
+
+extern "C" int dwf_net_destroy(dwf_net* net)
+{
+    if (net) {
+        (void)hipDeviceSynchronize();
+        delete net;
+    }
+    return DWF_OK;
+}
+
+extern "C" int dwf_net_set_weights(dwf_net* net, const float* w)
+{
+    return guard([&]() -> int {
+        if (!net || !w) throw DwfError(DWF_ERR_ARG, "dwf_net_set_weights: null argument");
+        size_t off = 0;
+        for (int l = 0; l < 3; ++l) {
+            const size_t n = (size_t)layer_cw(l, net->Ls);
+            hchk(hipMemcpyAsync(net->w[l], w + off, n * 8, hipMemcpyHostToDevice, net->pack_stream), "weights upload");
+            hchk(hipMemsetAsync(net->m1[l], 0, n * 8, net->pack_stream), "weights");
+            hchk(hipMemsetAsync(net->m2[l], 0, n * 8, net->pack_stream), "weights");
+            off += 2 * n;
+        }
+        net->adam_t = 0;
+        repack(net, net->pack_stream);
+        hchk(hipStreamSynchronize(net->pack_stream), "weights upload");
+        return DWF_OK;
+    });
+}
+
+extern "C" int dwf_net_get_weights(const dwf_net* net, float* w)
+{
+    return guard([&]() -> int {
+        if (!net || !w) throw DwfError(DWF_ERR_ARG, "dwf_net_get_weights: null argument");
+        hchk(hipDeviceSynchronize(), "weights download");
+        size_t off = 0;
+        for (int l = 0; l < 3; ++l) {
+            const size_t n = (size_t)layer_cw(l, net->Ls);
+            hchk(hipMemcpy(w + off, net->w[l], n * 8, hipMemcpyDeviceToHost), "weights download");
+            off += 2 * n;
+        }
+        return DWF_OK;
+    });
+}
+
+extern "C" int dwf_net_apply_device(dwf_ctx* c, const dwf_net* net, int adjoint, const double* in, double* out,
+                                    int nrhs)
+{
+    return guard([&]() -> int {
+        require_net(c, net);
+        if (!in || !out || nrhs < 1) throw DwfError(DWF_ERR_ARG, "dwf_net_apply_device: bad argument");
+        c->bind_rhs(nrhs);
+        const long long sites = c->L.V5();
+        unsigned short* h1 = (unsigned short*)c->buf(B_NET_H1, (size_t)nrhs * sites * 32);
+        unsigned short* h2 = (unsigned short*)c->buf(B_NET_H2, (size_t)nrhs * sites * 32);
+        const int* cfg = cfg_identity_map(c, nrhs);
+        if (adjoint)
+            net_adjoint(c, net, in, 0, out, 0, cfg, nrhs, h1, h2);
+        else
+            net_forward(c, net, in, 0, nullptr, out, 0, cfg, nrhs, nullptr, h1, h2);
+        c->check_launch("network apply");
+        return DWF_OK;
+    });
+}
+
+extern "C" int dwf_net_loss_grad_device(dwf_ctx* c, dwf_net* net, int loss, const float* b, const float* x,
+                                        const int* pair_cfg, int npairs, double* loss_out, float* grad)
+{
+    return guard([&]() -> int {
+        require_net(c, net);
+        check_pairs(c, b, x, pair_cfg, npairs, loss);
+        if (!loss_out || !grad) throw DwfError(DWF_ERR_ARG, "dwf_net_loss_grad_device: null output");
+        const long long sites = c->L.V5();
+        hipStream_t s = c->stream;
+        TrainBufs tb = train_bufs(c, npairs);
+        std::vector<int> rows(npairs);
+        std::iota(rows.begin(), rows.end(), 0);
+        hchk(hipMemcpyAsync(tb.row, rows.data(), npairs * sizeof(int), hipMemcpyHostToDevice, s), "upload");
+        hchk(hipMemcpyAsync(tb.cfg, pair_cfg, npairs * sizeof(int), hipMemcpyHostToDevice, s), "upload");
+        double* n2 = (double*)c->buf(B_NET_N2, (size_t)npairs * 8);
+        k_row_norms<<<npairs, 256, 0, s>>>(loss == 0 ? x : b, sites, n2);
+        net_forward(c, net, b, 1, tb.row, tb.y, 1, tb.cfg, npairs, nullptr, tb.h1, tb.h2);
+        double* ld = (double*)c->buf(B_NET_LOSS, 8);
+        loss_and_grad(c, loss, tb.y, b, x, tb.row, tb.cfg, n2, npairs, npairs, tb.gy, ld);
+        backward(c, net, b, tb.row, tb.cfg, npairs, tb.h1, tb.h2, tb.gy, tb.GB);
+        float* gdev = (float*)c->buf(B_NET_GRAD, (size_t)dwf_net_num_weights(net->Ls) * 4);
+        size_t off = 0;
+        for (int l = 0; l < 3; ++l) {
+            const int n = (int)layer_cw(l, net->Ls);
+            k_fold_adam<<<(n + 255) / 256, 256, 0, s>>>(tb.GB[l], kCinL[l], kCoutL[l], net->Ls, net->w[l], net->m1[l],
+                                                         net->m2[l], gdev + off, 0, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f);
+            off += 2 * (size_t)n;
+        }
+        c->check_launch("loss gradient");
+        hchk(hipMemcpyAsync(grad, gdev, off * 4, hipMemcpyDeviceToHost, s), "gradient download");
+        hchk(hipMemcpyAsync(loss_out, ld, 8, hipMemcpyDeviceToHost, s), "loss download");
+        hchk(hipStreamSynchronize(s), "loss gradient");
+        return DWF_OK;
+    });
+}
+
+extern "C" int dwf_net_train_device(dwf_ctx* c, dwf_net* net, const float* b, const float* x, const int* pair_cfg,
+                                    int npairs, const dwf_train_params* p, double* loss_history)
+{
+    return guard([&]() -> int {
+        require_net(c, net);
+        if (!p || p->steps < 1 || p->batch < 1 || !(p->lr > 0.0))
+            throw DwfError(DWF_ERR_ARG, "dwf_net_train_device: bad training parameters");
+        check_pairs(c, b, x, pair_cfg, npairs, p->loss);
+        const int B = std::min(p->batch, npairs);
+        const long long sites = c->L.V5();
+        hipStream_t s = c->stream;
+        TrainBufs tb = train_bufs(c, B);
+        double* n2 = (double*)c->buf(B_NET_N2, (size_t)npairs * 8);
+        k_row_norms<<<npairs, 256, 0, s>>>(p->loss == 0 ? x : b, sites, n2);
+        double* hist = (double*)c->buf(B_NET_LOSS, (size_t)p->steps * 8);
+        // the batch tables (rows, configurations) of every step, uploaded once
+        int* stage = (int*)c->buf(B_NET_STAGE, (size_t)p->steps * 2 * B * sizeof(int));
+        std::vector<int> order(npairs), tab((size_t)p->steps * 2 * B);
+        std::iota(order.begin(), order.end(), 0);
+        std::mt19937_64 rng(p->seed);
+        int pos = npairs;
+        for (int st = 0; st < p->steps; ++st)
+            for (int j = 0; j < B; ++j) {
+                if (pos == npairs) {
+                    std::shuffle(order.begin(), order.end(), rng);
+                    pos = 0;
+                }
+                const int r = order[pos++];
+                tab[(size_t)st * 2 * B + j] = r;
+                tab[(size_t)st * 2 * B + B + j] = pair_cfg[r];
+            }
+        hchk(hipMemcpyAsync(stage, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, s), "upload");
+        const float b1 = (float)p->beta1, b2 = (float)p->beta2;
+        for (int st = 0; st < p->steps; ++st) {
+            const int* row = stage + (size_t)st * 2 * B;
+            const int* cfg = row + B;
+            net_forward(c, net, b, 1, row, tb.y, 1, cfg, B, nullptr, tb.h1, tb.h2);
+            loss_and_grad(c, p->loss, tb.y, b, x, row, cfg, n2, B, B, tb.gy, hist + st);
+            backward(c, net, b, row, cfg, B, tb.h1, tb.h2, tb.gy, tb.GB);
+            net->adam_t += 1;
+            const float c1 = 1.f / (1.f - std::pow(b1, (float)net->adam_t));
+            const float c2 = 1.f / (1.f - std::pow(b2, (float)net->adam_t));
+            for (int l = 0; l < 3; ++l) {
+                const int n = (int)layer_cw(l, net->Ls);
+                k_fold_adam<<<(n + 255) / 256, 256, 0, s>>>(tb.GB[l], kCinL[l], kCoutL[l], net->Ls, net->w[l],
+                                                             net->m1[l], net->m2[l], nullptr, 1, (float)p->lr, b1, b2,
+                                                             (float)p->eps, c1, c2);
+            }
+            repack(net, s);
+            c->check_launch("training step");
+        }
+        if (loss_history)
+            hchk(hipMemcpyAsync(loss_history, hist, (size_t)p->steps * 8, hipMemcpyDeviceToHost, s), "loss download");
+        hchk(hipStreamSynchronize(s), "training");
+        return DWF_OK;
+    });
+}
+
+extern "C" int dwf_ctx_set_preconditioner(dwf_ctx* c, const dwf_net* net)
+{
+    return guard([&]() -> int {
+        if (!c) throw DwfError(DWF_ERR_ARG, "dwf_ctx_set_preconditioner: null context");
+        if (net && net->Ls != c->L.Ls)
+            throw DwfError(DWF_ERR_ARG, "learned preconditioner: network Ls differs from the lattice's");
+        c->net = net;
+        return DWF_OK;
+    });
+}
+
+// Flexible CG on D^dag D x = b (FP64) with z = M r; passes restart from the current x
+// until the FP64 true residual meets rtol (at most 3).  Leaves states in c->st_host.
+int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, double rtol, int maxiter, int nrhs,
+                  dwf_solve_stats* stats)
+{
+    const dwf_net* net = c->net;
+    if (!net) throw DwfError(DWF_ERR_ARG, "DWF_SOLVE_LEARNED: no preconditioner attached (dwf_ctx_set_preconditioner)");
+    require_net(c, net);
+    const long long sites = c->L.V5();
+    const size_t nv = (size_t)sites * 4 * nrhs;
+    hipStream_t s = c->stream;
+    double* r = (double*)c->buf(B_FCG_R, nv * 8);
+    double* p = (double*)c->buf(B_FCG_P, nv * 8);
+    double* q = (double*)c->buf(B_FCG_Q, nv * 8);
+    double* z = (double*)c->buf(B_FCG_Z, nv * 8);
+    double* t = (double*)c->buf(B_T64, nv * 8);
+    unsigned short* h1 = (unsigned short*)c->buf(B_NET_H1, (size_t)nrhs * sites * 32);
+    unsigned short* h2 = (unsigned short*)c->buf(B_NET_H2, (size_t)nrhs * sites * 32);
+    const int nb = nb_vec(sites);
+    double* part = (double*)c->buf(B_FCG_PART, (size_t)nrhs * nb * 2 * 8);
+    CgState* st = (CgState*)c->buf(B_STATE, sizeof(CgState) * nrhs);
+    FcgScal* fs = (FcgScal*)c->buf(B_FCG_SCAL, sizeof(FcgScal) * nrhs);
+    const int* cfg = cfg_identity_map(c, nrhs);
+    HopArgs<double> ha = hop_args<double>(c, nrhs, nullptr);
+    const int hnb = ha.cfg.nblk();
+    double* hpart = (double*)c->buf(B_PA, (size_t)nrhs * hnb * 2 * 8);
+    ha.st = st;
+    ha.slot = 0;
+    ha.skip = F_DONE;
+    const dim3 vgrid(nb, nrhs);
+
+    if (!have_guess) launch_zero<double>(x, (long long)nv, s);
+    std::vector<int> total_iters(nrhs, 0);
+    int passes = 0;
+    std::vector<CgState> h(nrhs);
+    auto read = [&]() {
+        hchk(hipMemcpyAsync(h.data(), st, sizeof(CgState) * nrhs, hipMemcpyDeviceToHost, s), "state readback");
+        hchk(hipStreamSynchronize(s), "state readback");
+    };
+    int pass_no = 0;
+    auto residual = [&](bool start) {  // r = b - A x (t scratch); start: states for a new pass
+        HopArgs<double> a = hop_args<double>(c, nrhs, nullptr);
+        a.in = x;
+        a.out = t;
+        launch_hop<double>(false, EPI_STORE, a, nrhs, s);
+        a.in = t;
+        a.out = q;
+        launch_hop<double>(true, EPI_STORE, a, nrhs, s);
+        k_fcg_init<<<vgrid, 256, 0, s>>>(b, q, r, sites, st, part);
+        if (start)
+            k_fcg_start<<<nrhs, 64, 0, s>>>(part, nb, st, rtol, pass_no == 0 ? 1 : 0);
+        else
+            k_fcg_true<<<nrhs, 64, 0, s>>>(part, nb, st);
+    };
+    auto iteration = [&]() {
+        HopArgs<double> a = ha;
+        a.in = p;
+        a.out = t;
+        a.part = hpart;
+        launch_hop<double>(false, EPI_NORM, a, nrhs, s);
+        a.in = t;
+        a.out = q;
+        a.part = nullptr;
+        launch_hop<double>(true, EPI_STORE, a, nrhs, s);
+        k_fcg_alpha<<<nrhs, 64, 0, s>>>(hpart, hnb, st, fs);
+        k_fcg_xr<<<vgrid, 256, 0, s>>>(x, r, p, q, sites, st, fs, part);
+        k_fcg_conv<<<nrhs, 64, 0, s>>>(part, nb, st);
+        net_forward(c, net, r, 0, nullptr, z, 0, cfg, nrhs, st, h1, h2);
+        k_fcg_dots<<<vgrid, 256, 0, s>>>(z, r, q, sites, st, part);
+        k_fcg_beta<<<nrhs, 64, 0, s>>>(part, nb, st, fs);
+        k_fcg_p<<<vgrid, 256, 0, s>>>(z, p, sites, st, fs);
+    };
+    constexpr int kBlockIters = 8;
+    hipGraphExec_t graph = nullptr;
+    if (c->use_graphs) {
+        std::vector<unsigned long long> key{0xFC6ULL,
+                                            (unsigned long long)nrhs,
+                                            (unsigned long long)(uintptr_t)x,
+                                            (unsigned long long)(uintptr_t)r,
+                                            (unsigned long long)(uintptr_t)p,
+                                            (unsigned long long)(uintptr_t)q,
+                                            (unsigned long long)(uintptr_t)z,
+                                            (unsigned long long)(uintptr_t)t,
+                                            (unsigned long long)(uintptr_t)h1,
+                                            (unsigned long long)(uintptr_t)h2,
+                                            (unsigned long long)(uintptr_t)part,
+                                            (unsigned long long)(uintptr_t)hpart,
+                                            (unsigned long long)(uintptr_t)st,
+                                            (unsigned long long)(uintptr_t)fs,
+                                            (unsigned long long)(uintptr_t)cfg,
+                                            (unsigned long long)(uintptr_t)net,
+                                            (unsigned long long)(uintptr_t)net->wf[0],
+                                            (unsigned long long)(uintptr_t)c->l64,
+                                            (unsigned long long)(uintptr_t)c->l32,
+                                            (unsigned long long)(uintptr_t)ha.rhs_per_cfg};
+        const int v[] = {ha.cfg.bx, ha.cfg.chunks, ha.cfg.spt, ha.cfg.remap, ha.cfg.prefetch, ha.cfg.tx,
+                         ha.cfg.tiles_x, ha.cfg.nt_store};
+        for (int e : v) key.push_back((unsigned long long)(unsigned)e);
+        graph = c->cached_graph(key, [&] {
+            for (int i = 0; i < kBlockIters; ++i) iteration();
+        });
+    }
+    std::vector<double> true_rel(nrhs, 0.0);
+    int worst_status = 0;
+    for (pass_no = 0; pass_no < 3; ++pass_no) {
+        passes = pass_no + 1;
+        residual(true);
+        net_forward(c, net, r, 0, nullptr, z, 0, cfg, nrhs, st, h1, h2);
+        k_fcg_p0<<<vgrid, 256, 0, s>>>(z, r, p, sites, st, part);
+        k_fcg_rz<<<nrhs, 64, 0, s>>>(part, nb, st, fs);
+        c->check_launch("FCG start");
+        read();
+        // iterations of this pass, counted from zero in the states
+        std::vector<int> base(nrhs);
+        for (int i = 0; i < nrhs; ++i) base[i] = h[i].iters;
+        int it = 0;
+        auto all_done = [&]() {
+            for (int i = 0; i < nrhs; ++i)
+                if (!(h[i].flags[0] & F_DONE)) return false;
+            return true;
+        };
+        const int budget = std::max(0, maxiter - *std::max_element(total_iters.begin(), total_iters.end()));
+        while (!all_done() && it < budget) {
+            const int n = std::min(32, budget - it);
+            int j = 0;
+            if (graph)
+                for (; j + kBlockIters <= n; j += kBlockIters) hchk(hipGraphLaunch(graph, s), "hipGraphLaunch");
+            for (; j < n; ++j) iteration();
+            it += n;
+            c->check_launch("FCG iteration");
+            read();
+        }
+        for (int i = 0; i < nrhs; ++i) total_iters[i] += h[i].iters - base[i];
+        residual(false);
+        c->check_launch("FCG true residual");
+        read();
+        bool ok = true;
+        for (int i = 0; i < nrhs; ++i) {
+            true_rel[i] = h[i].true_relres;
+            worst_status = std::max(worst_status, h[i].status);
+            if (h[i].true_relres > rtol) ok = false;
+        }
+        if (ok || worst_status != 0 || !all_done()) break;
+    }
+    int rc = DWF_OK;
+    for (int i = 0; i < nrhs; ++i) {
+        dwf_solve_stats& S = stats[i];
+        S = dwf_solve_stats{};
+        S.iterations = total_iters[i];
+        S.fp64_iterations = total_iters[i];
+        S.passes = passes;
+        S.relres = h[i].relres;
+        S.true_relres = true_rel[i];
+        S.converged = (h[i].status == 0 && true_rel[i] <= rtol) ? 1 : 0;
+        if (!S.converged && rc == DWF_OK) rc = h[i].status ? DWF_ERR_ZERO_DENOM : DWF_ERR_NOT_CONVERGED;
+    }
+    if (rc == DWF_ERR_ZERO_DENOM)
+        dwf_set_error("flexible CG broke down: the preconditioner is not positive on a residual");
+    else if (rc == DWF_ERR_NOT_CONVERGED)
+        dwf_set_error("CG did not converge within maxiter");
+    return rc;
+}

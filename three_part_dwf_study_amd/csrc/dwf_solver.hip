@@ -313,31 +313,6 @@ __global__ void k_sub_into(const double* a, const double* b, double* out, long l
         out[i] = a[i] - b[i];
 }
 
-// Instantiated graph of `body` (kernel launches on c->stream only), cached under `key`,
-// which must hold every argument the captured kernels receive.
-hipGraphExec_t cg_graph(dwf_ctx* c, const Sig& key, const std::function<void()>& body)
-{
-    auto f = c->graphs.find(key);
-    if (f != c->graphs.end()) return f->second;
-    if (c->graphs.size() >= 64) c->drop_graphs();
-    c->check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    hipGraph_t g = nullptr;
-    try {
-        body();
-    } catch (...) {
-        (void)hipStreamEndCapture(c->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-    }
-    c->check(hipStreamEndCapture(c->stream, &g), "hipStreamEndCapture");
-    hipGraphExec_t ex = nullptr;
-    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    c->check(e, "hipGraphInstantiate");
-    c->graphs.emplace(key, ex);
-    return ex;
-}
-
 // CG on op.  Leaves the per-RHS states in c->st_host (slot c->last_slot).  `pre`
 // (nullable) marks right-hand sides that are already finished.
 void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const double* x0, double rtol, int maxiter,
@@ -404,7 +379,7 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
                 sig_ptr(t64),            sig_ptr(r32),            sig_ptr(t32),  sig_ptr(p32), sig_ptr(x32),
                 sig_ptr(partA),          sig_ptr(partB),          sig_ptr(st)};
         op.signature(key);
-        block = cg_graph(c, key, [&] {
+        block = c->cached_graph(key, [&] {
             for (int i = 0; i < kGraphIters; ++i) iteration(i);
         });
     }
@@ -498,14 +473,43 @@ int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, c
 
 namespace {
 
+// D^dag D x = b by flexible CG with the learned preconditioner.  If it breaks down (the
+// network is not positive on some residual) or misses rtol, plain FP64 CG continues from
+// its iterate and decides; the stats then count both.
+int solve_learned(dwf_ctx* c, const double* b, const double* x0, double rtol, int maxiter, int nrhs, double* x,
+                  dwf_solve_stats* stats)
+{
+    const size_t nv = (size_t)c->L.V5() * 4 * nrhs;
+    if (x0 && x0 != x) c->check(hipMemcpyAsync(x, x0, nv * 8, hipMemcpyDeviceToDevice, c->stream), "copy x0");
+    std::vector<dwf_solve_stats> first(nrhs);
+    int rc = dwf_fcg_solve(c, b, x, x0 != nullptr, rtol, maxiter, nrhs, first.data());
+    if (rc == DWF_ERR_ZERO_DENOM || rc == DWF_ERR_NOT_CONVERGED) {
+        dwf_solve_device_impl(c, DWF_OP_DDAGD, DWF_SOLVE_CG64, b, x, rtol, maxiter, nrhs, x);
+        std::vector<dwf_solve_stats> second(nrhs);
+        rc = collect(c, DWF_SOLVE_CG64, nrhs, second.data());
+        for (int i = 0; i < nrhs; ++i) {
+            second[i].fp64_iterations += first[i].iterations;
+            second[i].iterations = first[i].iterations;
+            second[i].passes = first[i].passes;
+        }
+        first = second;
+        if (rc == DWF_ERR_NOT_CONVERGED) rc = DWF_ERR_CERTIFY;
+    }
+    if (stats) std::copy(first.begin(), first.end(), stats);
+    return rc;
+}
+
 int solve_normal(dwf_ctx* c, int normal, int method, const double* b, const double* x0, double rtol, int maxiter,
                  int nrhs, double* x, dwf_solve_stats* stats)
 {
     if (!c || !b || !x || nrhs < 1 || maxiter < 0 || !(rtol >= 0.0) ||
-        (normal != DWF_OP_DDDAG && normal != DWF_OP_DDAGD) || (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED))
+        (normal != DWF_OP_DDDAG && normal != DWF_OP_DDAGD) ||
+        (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED &&
+         !(method == DWF_SOLVE_LEARNED && normal == DWF_OP_DDAGD)))
         throw DwfError(DWF_ERR_ARG, "dwf_solve: bad argument");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve: no gauge field set");
     c->bind_rhs(nrhs);
+    if (method == DWF_SOLVE_LEARNED) return solve_learned(c, b, x0, rtol, maxiter, nrhs, x, stats);
     dwf_solve_device_impl(c, normal, method, b, x0, rtol, maxiter, nrhs, x);
     int rc = collect(c, method, nrhs, stats);
     if (method == DWF_SOLVE_MIXED && rc == DWF_ERR_NOT_CONVERGED) {
@@ -661,7 +665,8 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
                 double* x, dwf_solve_stats* stats)
 {
     if (!c || !src || !x || nrhs < 1 || maxiter < 0 || !(rtol >= 0.0) ||
-        (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_EO_MIXED))
+        (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_EO_MIXED &&
+         method != DWF_SOLVE_LEARNED))
         throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: bad argument");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: no gauge field set");
     c->bind_rhs(nrhs);
@@ -682,6 +687,7 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
         a.out = B;
         launch_hop<double>(true, EPI_STORE, a, nrhs, c->stream);
     }
+    if (method == DWF_SOLVE_LEARNED) return solve_learned(c, B, x0, rtol, maxiter, nrhs, x, stats);
     return solve_normal(c, DWF_OP_DDAGD, method, B, x0, rtol, maxiter, nrhs, x, stats);
 }
 
