@@ -22,7 +22,14 @@ residual <= 1e-10) and their FP64 contractions (point C_PP, C_J5, loop density).
 four lattice sizes run concurrently, one context and stream each; each size's 1,088
 right-hand sides are one batched solve.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload solves|analyzer]
+With --workload learned: the learned preconditioner at 96x96, Ls=16.  Training pairs
+(source, solution) come from --learn-configs seeded configurations (seeds 1234 + 104729 i)
+with --learn-pairs full-volume Gaussian sources each, solved to 1e-10; the network is
+trained on the device (Adam, --learn-steps steps of --learn-batch pairs); a step is then
+one batch of 16 boundary-source solves on a held-out configuration by flexible CG with
+the network, beside plain FP64 CG and the even-odd mixed solver on the same batch.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload solves|analyzer|learned]
 """
 
 import argparse
@@ -250,6 +257,116 @@ def run_analyzer(args, dwf):
         c.close()
 
 
+def run_learned(args, dwf, capi):
+    V5 = NT * NX * LS
+    ntrain, per = args.learn_configs, args.learn_pairs
+    npairs = ntrain * per
+    thetas = np.stack([np.random.default_rng(1234 + 104729 * i).normal(0.0, 1.0 / np.sqrt(BETA), 2 * NT * NX)
+                       for i in range(ntrain + 1)])
+    ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
+    ctx.set_gauge_batch(thetas[:ntrain])
+    # pairs: chunk j holds pair j of every configuration (right-hand side c uses config c)
+    d_b32 = capi.DeviceArray(npairs * V5 * 16)
+    d_x32 = capi.DeviceArray(npairs * V5 * 16)
+    d_src = capi.DeviceArray(ntrain * V5 * 32)
+    d_b = capi.DeviceArray(ntrain * V5 * 32)
+    d_x = capi.DeviceArray(ntrain * V5 * 32)
+    rng = np.random.default_rng(918273)
+    t0 = time.perf_counter()
+    its = []
+    for j in range(per):
+        eta = rng.normal(size=2 * ntrain * V5 * 2)
+        d_src.upload(eta)
+        ctx.apply_device(dwf.DWF_OP_DDAG, dwf.DWF_PREC_FP64, d_src.ptr, d_b.ptr, ntrain)
+        st, _ = ctx.solve_dirac_device(dwf.DWF_SOLVE_EO_MIXED, d_src.ptr, None, TOL, 20000, ntrain, d_x.ptr)
+        its.append(np.mean([q["iterations"] for q in st]))
+        n = ntrain * V5 * 4
+        ctx.convert_f64_f32(d_b.ptr, d_b32.ptr + j * n * 4, n)
+        ctx.convert_f64_f32(d_x.ptr, d_x32.ptr + j * n * 4, n)
+    ctx.synchronize()
+    t_pairs = time.perf_counter() - t0
+    log(f"{npairs} training pairs in {t_pairs:.1f} s")
+    pair_cfg = np.tile(np.arange(ntrain), per)
+    net = dwf.Network(LS, seed=11)
+    t0 = time.perf_counter()
+    hist = net.train_device(ctx, d_b32.ptr, d_x32.ptr, pair_cfg, npairs, steps=args.learn_steps,
+                            batch=args.learn_batch, loss=args.learn_loss, lr=args.learn_lr, seed=5)
+    t_train = time.perf_counter() - t0
+    log(f"training: {args.learn_steps} steps in {t_train:.1f} s, loss {hist[0]:.4f} -> "
+        f"{float(np.mean(hist[-20:])):.4f}")
+    for d in (d_b32, d_x32, d_src, d_b, d_x):
+        d.free()
+    ctx.close()
+
+    # held-out configuration: 16 boundary sources per step
+    ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
+    ctx.set_gauge(thetas[ntrain])
+    ctx.set_preconditioner(net)
+    nrhs = args.nrhs
+    src = boundary_sources(nrhs, 918273 + 1)
+    d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
+    d_x = capi.DeviceArray(nrhs * V5 * 32)
+    results = {}
+    for name, method in (("learned", dwf.DWF_SOLVE_LEARNED), ("fp64_cg", dwf.DWF_SOLVE_CG64),
+                         ("eo_mixed", dwf.DWF_SOLVE_EO_MIXED)):
+        for _ in range(args.warmup):
+            ctx.solve_dirac_device(method, d_src.ptr, None, TOL, 20000, nrhs, d_x.ptr)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st, _ = ctx.solve_dirac_device(method, d_src.ptr, None, TOL, 20000, nrhs, d_x.ptr)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        tr = max(q["true_relres"] for q in st)
+        if tr > TOL or not all(q["converged"] for q in st):
+            raise SystemExit(f"{name} solve failed certification: max true residual {tr:.3e}")
+        results[name] = dict(solves_per_s=args.steps * nrhs / dt, ms_per_step=dt / args.steps * 1e3,
+                             iterations=float(np.mean([q["iterations"] for q in st])),
+                             fp64_iterations=float(np.mean([q["fp64_iterations"] for q in st])),
+                             max_true_relres=tr)
+        log(f"{name}: {results[name]}")
+    fast, judge = results["learned"], results["fp64_cg"]
+    line = {
+        "metric": "DWF solves/s 96²,Ls=16 @1e-10: learned preconditioner (flexible CG) vs plain FP64 CG",
+        "value": round(fast["solves_per_s"], 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(fast["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"learned preconditioner: trained on {npairs} (source, solution) pairs from {ntrain} "
+                        "configurations (full-volume Gaussian sources), tested on a held-out configuration, "
+                        "16 boundary sources per step, FP64 true residual <= 1e-10",
+            "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs,
+            "network": "3 linear parallel-transport conv layers 2-8-8-2 complex channels, BF16 MFMA",
+            "loss": ["||M b - x||^2/||x||^2", "||A M b - b||^2/||b||^2"][args.learn_loss],
+            "training_steps": args.learn_steps, "batch": args.learn_batch, "lr": args.learn_lr,
+        },
+        "pairs_seconds": round(t_pairs, 2),
+        "training_seconds": round(t_train, 2),
+        "loss_first": float(hist[0]),
+        "loss_last": float(np.mean(hist[-20:])),
+        "fp64_cg_solves_per_s": round(judge["solves_per_s"], 3),
+        "eo_mixed_solves_per_s": round(results["eo_mixed"]["solves_per_s"], 3),
+        "speedup_vs_fp64_cg": round(fast["solves_per_s"] / judge["solves_per_s"], 3),
+        "iterations": {k: v["iterations"] for k, v in results.items()},
+        "iteration_ratio_vs_fp64_cg": round(fast["iterations"] / judge["iterations"], 3),
+        "max_true_relres": {k: v["max_true_relres"] for k, v in results.items()},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.set_preconditioner(None)
+    ctx.close()
+    net.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,7 +376,13 @@ def main():
     ap.add_argument("--dslash-rhs", type=int, default=128)
     ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("solves", "analyzer"), default="solves")
+    ap.add_argument("--workload", choices=("solves", "analyzer", "learned"), default="solves")
+    ap.add_argument("--learn-configs", type=int, default=64)
+    ap.add_argument("--learn-pairs", type=int, default=64, help="training pairs per configuration")
+    ap.add_argument("--learn-steps", type=int, default=1000)
+    ap.add_argument("--learn-batch", type=int, default=16)
+    ap.add_argument("--learn-loss", type=int, default=1, choices=(0, 1))
+    ap.add_argument("--learn-lr", type=float, default=3e-3)
     ap.add_argument("--analyzer-configs", type=int, default=32, help="configurations per lattice size")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the roofline launches (batched FP32 D apply and the copy kernel), for counter runs")
@@ -274,6 +397,9 @@ def main():
 
     if args.workload == "analyzer":
         run_analyzer(args, dwf)
+        return
+    if args.workload == "learned":
+        run_learned(args, dwf, capi)
         return
     rng = np.random.default_rng(1234)
     theta = rng.normal(0.0, 1.0 / np.sqrt(BETA), size=2 * NT * NX)

@@ -390,6 +390,10 @@ double dwf_timer_elapsed_ms(dwf_timer* timer);  /* waits for the stop event; < 0
  * stream: the achievable-bandwidth reference for the stencils' roofline. */
 int dwf_copy_device(dwf_ctx* ctx, void* dst, const void* src, unsigned long long bytes);
 
+/* dst[i] = (float)src[i] for n doubles on the device, on the context's stream (training
+ * pairs for the learned preconditioner are stored in FP32). */
+int dwf_convert_f64_f32_device(dwf_ctx* ctx, const double* src, float* dst, unsigned long long n);
+
 /* Message of the calling thread's last failure ("" if none). */
 const char* dwf_last_error(void);
 

@@ -206,6 +206,7 @@ PROTOTYPES = {
     "dwf_timer_stop": (_I, [_P, _P]),
     "dwf_timer_elapsed_ms": (_D, [_P]),
     "dwf_copy_device": (_I, [_P, _P, _P, ctypes.c_ulonglong]),
+    "dwf_convert_f64_f32_device": (_I, [_P, _P, _P, ctypes.c_ulonglong]),
     "dwf_rng_create": (_P, [ctypes.c_ulonglong]),
     "dwf_rng_destroy": (None, [_P]),
     "dwf_rng_dwf_heatbath": (None, [_P, _I, _I, _P, _P]),
@@ -325,6 +326,10 @@ class DwfContext:
 
     def copy_device(self, dst_ptr: int, src_ptr: int, nbytes: int):
         _check(self._lib, self._lib.dwf_copy_device(self._h, dst_ptr, src_ptr, nbytes), "dwf_copy_device")
+
+    def convert_f64_f32(self, src_ptr: int, dst_ptr: int, n: int):
+        _check(self._lib, self._lib.dwf_convert_f64_f32_device(self._h, src_ptr, dst_ptr, n),
+               "dwf_convert_f64_f32_device")
 
     # gauge --------------------------------------------------------------------------
     def set_gauge(self, theta: np.ndarray):

@@ -9,6 +9,12 @@ struct dwf_timer {
 
 namespace {
 
+__global__ void k_f64_to_f32(const double* a, float* b, const long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
 __global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst,
                                                 long long n)
 {
@@ -102,6 +108,18 @@ double dwf_timer_elapsed_ms(dwf_timer* t)
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, t->a, t->b) != hipSuccess) return -1.0;
     return ms;
+}
+
+int dwf_convert_f64_f32_device(dwf_ctx* ctx, const double* src, float* dst, unsigned long long n)
+{
+    if (!ctx || !dst || !src) {
+        dwf_set_error("dwf_convert_f64_f32_device: bad argument");
+        return DWF_ERR_ARG;
+    }
+    const long long blocks = std::min<long long>(((long long)n + 255) / 256, 256LL * 64);
+    if (n) k_f64_to_f32<<<(unsigned)blocks, 256, 0, ctx->stream>>>(src, dst, (long long)n);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? DWF_OK : fail(e, "dwf_convert_f64_f32_device");
 }
 
 int dwf_copy_device(dwf_ctx* ctx, void* dst, const void* src, unsigned long long bytes)
