@@ -29,7 +29,13 @@ trained on the device (Adam, --learn-steps steps of --learn-batch pairs); a step
 one batch of 16 boundary-source solves on a held-out configuration by flexible CG with
 the network, beside plain FP64 CG and the even-odd mixed solver on the same batch.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload solves|analyzer|learned]
+With --workload hmc a step is one Nf=2 domain-wall HMC trajectory at 96x96, Ls=16,
+beta=4, mf=0.05, M5=1, tau=1 with 20 leapfrog steps (epsilon=0.05), from a hot start
+theta ~ U[-pi, pi) (seed 1234), the host random stream seeded 1234, force solves to 1e-10
+with chronological initial guesses; the gauge update, forces and Delta H stay in FP64 on
+the device.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload solves|analyzer|learned|hmc]
 """
 
 import argparse
@@ -367,6 +373,72 @@ def run_learned(args, dwf, capi):
     net.close()
 
 
+def run_hmc(args, dwf, capi):
+    theta0 = np.random.default_rng(1234).uniform(-np.pi, np.pi, 2 * NT * NX)
+    ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
+    ctx.set_gauge(theta0)
+    rng = capi.HostRng(1234)
+    ndof, ngauge = LS * NT * NX * 2, 2 * NT * NX
+    method = {"cg64": dwf.DWF_SOLVE_CG64, "mixed": dwf.DWF_SOLVE_MIXED}[args.hmc_solver]
+    eps, nsteps = 0.05, 20
+
+    def trajectory():
+        chi, mom = rng.dwf_heatbath(ndof, ngauge)
+        u = rng.uniform()
+        return ctx.hmc_trajectory(chi, mom, u, BETA, eps, nsteps, TOL, 20000, method=method)
+
+    for _ in range(args.warmup):
+        trajectory()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    hist = [trajectory() for _ in range(args.steps)]
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    for i, h in enumerate(hist):
+        log(f"trajectory {i}: {h}")
+    cg_per_traj = float(np.mean([h["cg_iterations"] for h in hist]))
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        src = boundary_sources(1, 918273)[0].reshape(-1)
+        t1 = time.perf_counter()
+        _, it, _, _, _ = oracle.dwf_cg(2, theta0, src, NT, NX, LS, M5, MF, TOL, args.cpu_iters)
+        per_iter = (time.perf_counter() - t1) / max(1, it)
+        cpu = {"value": 1.0 / (per_iter * cg_per_traj), "unit": "trajectories/s", "cores": 1, "kind": "port",
+               "sample": (f"oracle FP64 CG on D D^dag (reference algorithm, 1 core): {it} iterations at 96x96x16 "
+                          f"timed ({per_iter * 1e3:.1f} ms/iteration), scaled by the {cg_per_traj:.0f} CG "
+                          "iterations one device trajectory needs; forces and gauge update excluded")}
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "DWF HMC trajectories/s 96²,Ls=16 (tau=1, 20 leapfrog steps)",
+        "value": round(args.steps / dt, 4),
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.hmc_solver == "cg64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "Nf=2 Shamir DWF pseudofermion HMC, 96x96 Ls=16 beta=4 mf=0.05 M5=1, tau=1, 20 leapfrog "
+                        "steps, hot start seed 1234, force solves to 1e-10 with chronological guesses",
+            "force_solver": args.hmc_solver,
+        },
+        "cg_iterations_per_trajectory": cg_per_traj,
+        "delta_h": [h["delta_h"] for h in hist],
+        "plaquette": [h["plaquette"] for h in hist],
+        "accepted": [h["accepted"] for h in hist],
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,7 +448,8 @@ def main():
     ap.add_argument("--dslash-rhs", type=int, default=128)
     ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("solves", "analyzer", "learned"), default="solves")
+    ap.add_argument("--workload", choices=("solves", "analyzer", "learned", "hmc"), default="solves")
+    ap.add_argument("--hmc-solver", choices=("cg64", "mixed"), default="mixed")
     ap.add_argument("--learn-configs", type=int, default=64)
     ap.add_argument("--learn-pairs", type=int, default=64, help="training pairs per configuration")
     ap.add_argument("--learn-steps", type=int, default=1000)
@@ -400,6 +473,9 @@ def main():
         return
     if args.workload == "learned":
         run_learned(args, dwf, capi)
+        return
+    if args.workload == "hmc":
+        run_hmc(args, dwf, capi)
         return
     rng = np.random.default_rng(1234)
     theta = rng.normal(0.0, 1.0 / np.sqrt(BETA), size=2 * NT * NX)
