@@ -601,7 +601,7 @@ def main():
             "bytes_per_launch": dslash_bytes(4, args.dslash_rhs),
         },
         "cpu_baseline": cpu,
-        "autotuned_shapes": {k: {kk: v[kk] for kk in ("tile_x", "prefetch", "chunks", "nt_store")}
+        "autotuned_shapes": {k: {kk: v[kk] for kk in ("tile_x", "prefetch", "chunks", "nt_store", "lds")}
                              for k, v in tuned.items()},
     }
     print(json.dumps(line), flush=True)

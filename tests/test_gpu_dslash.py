@@ -123,6 +123,40 @@ def test_tuned_launch_shapes_keep_parity(tile_x, prefetch, chunks):
         assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= 1e-10
 
 
+@pytest.mark.parametrize("tile_x,chunks", [(8, 0), (16, 2), (32, 6), (64, 1)])
+def test_lds_tiles_keep_parity(tile_x, chunks):
+    """The LDS-staged stencil (full 2D tiles, one halo per slice) against the oracle for
+    every operator, and inside the FP64 and mixed solvers (its NORM/CGUPD/RESID paths)."""
+    Nt, Nx, Ls = 32, 64, 6
+    print_case(test="lds_tiles", tile_x=tile_x, chunks=chunks)
+    rng = np.random.default_rng(100 + tile_x + chunks)
+    th = rand_theta(rng, Nt, Nx)
+    psi = rand_spinor(rng, 2 * Ls * Nt * Nx * 2).reshape(2, -1)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_LDS, 1)
+        ctx.tune(dwf.DWF_TUNE_TILE_X, tile_x)
+        ctx.tune(dwf.DWF_TUNE_NT_STORE, tile_x % 16 == 0)
+        ctx.tune(dwf.DWF_TUNE_SCHUNKS, chunks)
+        for op in (0, 1, 2, 3):
+            got = ctx.apply(op, psi)
+            for i in range(2):
+                want = oracle.dwf_apply(op, th, psi[i], Nt, Nx, Ls, M5, MF)
+                assert rel(got[i], want) <= 1e-12, (op, rel(got[i], want))
+        b = ctx.apply(1, psi[0])
+        for method in (dwf.DWF_SOLVE_CG64, dwf.DWF_SOLVE_MIXED):
+            x, st, rc = ctx.solve(3, b, rtol=1e-10, maxiter=5000, method=method)
+            Ax = oracle.dwf_apply(3, th, x, Nt, Nx, Ls, M5, MF)
+            assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= 1.01e-10
+    with dwf.DwfContext.wilson(Nt, Nx, 0.1, -1.0, 1.0) as ctx:
+        w = rand_spinor(rng, Nt * Nx * 2)
+        ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_LDS, 1)
+        ctx.tune(dwf.DWF_TUNE_TILE_X, tile_x)
+        for op in (0, 1):
+            assert rel(ctx.apply(op, w), oracle.wilson_apply(op, th, w, Nt, Nx, 0.1, -1.0, 1.0)) <= 1e-12
+
+
 def test_autotune_and_cache(tmp_path):
     Nt, Nx, Ls = 16, 32, 8
     print_case(test="autotune")
@@ -135,6 +169,6 @@ def test_autotune_and_cache(tmp_path):
         r1 = ctx.autotune(4, dwf.DWF_PREC_FP32, cache)
         r2 = ctx.autotune(4, dwf.DWF_PREC_FP32, cache)
         assert r1["from_cache"] == 0 and r2["from_cache"] == 1 and r1["microseconds"] > 0
-        assert all(r1[k] == r2[k] for k in ("tile_x", "prefetch", "chunks", "nt_store"))
+        assert all(r1[k] == r2[k] for k in ("tile_x", "prefetch", "chunks", "nt_store", "lds"))
         got = ctx.apply(0, psi)
         assert rel(got, oracle.dwf_apply(0, th, psi, Nt, Nx, Ls, M5, MF)) <= 1e-12

@@ -46,6 +46,7 @@ DWF_TUNE_PREFETCH = 3
 DWF_TUNE_TILE_X = 4
 DWF_TUNE_NT_STORE = 5
 DWF_TUNE_GRAPHS = 6
+DWF_TUNE_LDS = 7
 
 
 class DwfError(RuntimeError):
@@ -113,6 +114,8 @@ class TuneResult(ctypes.Structure):
         ("nt_store", ctypes.c_int),
         ("from_cache", ctypes.c_int),
         ("microseconds", ctypes.c_double),
+        ("lds", ctypes.c_int),
+        ("pad", ctypes.c_int),
     ]
 
 

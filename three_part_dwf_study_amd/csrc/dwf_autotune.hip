@@ -2,7 +2,8 @@
 //
 // For one (Nt, Nx, Ls, nrhs, precision) it times the D apply over the candidate shapes
 // -- block mapping (consecutive sites or 2D tiles of width 16/32/64 dividing Nx),
-// software-pipelined slice loop on/off, s-chunks per 4D site, streaming stores on/off
+// software-pipelined slice loop on/off, s-chunks per 4D site, streaming stores on/off,
+// and the LDS-staged tile kernel (tiles of width 8/16/32/64 that cover the lattice)
 // -- with HIP events on the
 // context's stream, keeps the fastest and applies it to the context.  Results can be
 // cached in a text file (one line per key), so a production run tunes once.
@@ -22,13 +23,13 @@ using namespace dwfk;
 namespace {
 
 struct Shape {
-    int tile_x, prefetch, chunks, nt_store;
+    int tile_x, prefetch, chunks, nt_store, lds;
 };
 
 std::string key_of(const dwf_ctx* c, int nrhs, int prec)
 {
     std::ostringstream k;
-    k << (c->kind == 0 ? "dwf" : "wilson") << ' ' << c->L.Nt << ' ' << c->L.Nx << ' ' << c->L.Ls << ' ' << nrhs << ' '
+    k << "v2 " << (c->kind == 0 ? "dwf" : "wilson") << ' ' << c->L.Nt << ' ' << c->L.Nx << ' ' << c->L.Ls << ' ' << nrhs << ' '
       << (prec == DWF_PREC_FP32 ? "fp32" : "fp64");
     return k.str();
 }
@@ -44,7 +45,7 @@ bool load_cached(const char* path, const std::string& key, Shape& s, double& us)
         std::istringstream rest(line.substr(key.size()));
         Shape t{};
         double u = 0;
-        if (rest >> t.tile_x >> t.prefetch >> t.chunks >> t.nt_store >> u) {
+        if (rest >> t.tile_x >> t.prefetch >> t.chunks >> t.nt_store >> t.lds >> u) {
             s = t;
             us = u;
             found = true;  // the last line for a key wins
@@ -59,6 +60,7 @@ void apply_shape(dwf_ctx* c, const Shape& s)
     c->prefetch = s.prefetch;
     c->chunks_override = s.chunks;
     c->nt_store = s.nt_store;
+    c->lds = s.lds;
 }
 
 int guard(const std::function<int()>& f)
@@ -87,7 +89,7 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
         double best_us = 0;
         if (load_cached(cache_path, key, best, best_us)) {
             apply_shape(c, best);
-            if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, best.nt_store, 1, best_us};
+            if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, best.nt_store, 1, best_us, best.lds, 0};
             return DWF_OK;
         }
         const Lat& L = c->L;
@@ -101,7 +103,11 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
         for (int tx : tiles)
             for (int pf : {0, 1})
                 for (int ch : chunks)
-                    for (int nt : {0, 1}) cand.push_back(Shape{tx, pf, ch, nt});
+                    for (int nt : {0, 1}) cand.push_back(Shape{tx, pf, ch, nt, 0});
+        for (int tx : {8, 16, 32, 64})
+            if (L.Nx % tx == 0 && L.Nt % (kBlock / tx) == 0)
+                for (int ch : chunks)
+                    for (int nt : {0, 1}) cand.push_back(Shape{tx, 0, ch, nt, 1});
 
         const size_t rb = precision == DWF_PREC_FP32 ? 4 : 8;
         const size_t n = (size_t)L.V5() * 4 * nrhs;
@@ -139,9 +145,9 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
             std::ofstream f(cache_path, std::ios::app);
             if (f)
                 f << key << ' ' << best.tile_x << ' ' << best.prefetch << ' ' << best.chunks << ' ' << best.nt_store << ' '
-                  << best_us << '\n';
+                  << best.lds << ' ' << best_us << '\n';
         }
-        if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, best.nt_store, 0, best_us};
+        if (out) *out = dwf_tune_result{best.tile_x, best.prefetch, best.chunks, best.nt_store, 0, best_us, best.lds, 0};
         return DWF_OK;
     });
 }

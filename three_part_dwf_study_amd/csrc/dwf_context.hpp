@@ -106,6 +106,7 @@ struct dwf_ctx {
     int prefetch = 0;             // software-pipelined stencil slice loop
     int tile_x = 0;               // stencil 2D tile width (0: linear blocks)
     int nt_store = 0;             // streaming stores of stencil outputs
+    int lds = 0;                  // LDS-staged stencil tiles
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs
     const dwf_net* net = nullptr; // learned preconditioner of DWF_SOLVE_LEARNED (not owned)

@@ -114,7 +114,155 @@ __device__ __forceinline__ double sum_partials(const double* p, int n, int which
     return r;
 }
 
-// ------------------------------------------------------------------ stencil kernel
+// ------------------------------------------------------------------ stencil kernels
+// Per-thread constants of one 4D site: its four links and boundary signs.
+template <typename R> struct SiteK {
+    typename Cx<R>::T ut, ux, utm, uxm;
+    R stp, stm, sxp, sxm;
+};
+
+template <typename R>
+__device__ __forceinline__ SiteK<R> site_consts(const HopArgs<R>& a, int rhs, int t, int x)
+{
+    const Lat& L = a.L;
+    const int Nx = L.Nx, Nt = L.Nt, V4 = L.V4;
+    const int tm = (t == 0) ? Nt - 1 : t - 1;
+    const int xm = (x == 0) ? Nx - 1 : x - 1;
+    using C = typename Cx<R>::T;
+    const int cfg = a.cfg_map ? a.cfg_map[rhs] : rhs / a.rhs_per_cfg;
+    const C* lk = static_cast<const C*>(a.links) + (long long)cfg * a.link_stride;
+    const int site = t * Nx + x;
+    SiteK<R> k;
+    k.ut = lk[site];
+    k.ux = lk[V4 + site];
+    k.utm = lk[tm * Nx + x];
+    k.uxm = lk[V4 + t * Nx + xm];
+    k.stp = (t == Nt - 1) ? (R)L.bct : (R)1;
+    k.stm = (t == 0) ? (R)L.bct : (R)1;
+    k.sxp = (x == Nx - 1) ? (R)L.bcx : (R)1;
+    k.sxm = (x == 0) ? (R)L.bcx : (R)1;
+    return k;
+}
+
+// The operator at one site of slice s from the centre, its four 4D neighbours and (D5)
+// its fifth-dimension neighbours.
+template <typename R, bool DAG, bool D5>
+__device__ __forceinline__ Sp<R> hop_site(const Lat& L, const SiteK<R>& k, const Sp<R>& cur, const Sp<R>& vtp,
+                                          const Sp<R>& vtm, const Sp<R>& vxp, const Sp<R>& vxm, const Sp<R>& dn,
+                                          const Sp<R>& up, int s)
+{
+    const R diag = (R)L.diag, mf = (R)L.mf, h = (R)-0.5;
+    constexpr R sg = DAG ? (R)-1 : (R)1;
+    const R stp = k.stp, stm = k.stm, sxp = k.sxp, sxm = k.sxm;
+    const auto ut = k.ut, ux = k.ux, utm = k.utm, uxm = k.uxm;
+    (void)dn;
+    (void)up;
+    (void)mf;
+    Sp<R> o{diag * cur.ar, diag * cur.ai, diag * cur.br, diag * cur.bi};
+    {   // forward t: U_t(x) (1 - g sigma1) psi(x+t): upper a = v0 - g v1, lower -g a
+        const R ar = stp * (vtp.ar - sg * vtp.br), ai = stp * (vtp.ai - sg * vtp.bi);
+        const R hr = ut.x * ar - ut.y * ai, hi = ut.x * ai + ut.y * ar;
+        o.ar += h * hr;
+        o.ai += h * hi;
+        o.br -= h * sg * hr;
+        o.bi -= h * sg * hi;
+    }
+    {   // backward t: U_t(x-t)^* (1 + g sigma1) psi(x-t): upper b = v0 + g v1, lower g b
+        const R br = stm * (vtm.ar + sg * vtm.br), bi = stm * (vtm.ai + sg * vtm.bi);
+        const R hr = utm.x * br + utm.y * bi, hi = utm.x * bi - utm.y * br;
+        o.ar += h * hr;
+        o.ai += h * hi;
+        o.br += h * sg * hr;
+        o.bi += h * sg * hi;
+    }
+    {   // forward x: U_x(x) (1 - g sigma2) psi(x+x): upper c = v0 + g i v1, lower -g i c
+        const R cr = sxp * (vxp.ar - sg * vxp.bi), ci = sxp * (vxp.ai + sg * vxp.br);
+        const R hr = ux.x * cr - ux.y * ci, hi = ux.x * ci + ux.y * cr;
+        o.ar += h * hr;
+        o.ai += h * hi;
+        o.br += h * sg * hi;   // -g i (hr + i hi) = g hi - i g hr
+        o.bi -= h * sg * hr;
+    }
+    {   // backward x: U_x(x-x)^* (1 + g sigma2) psi(x-x): upper d = v0 - g i v1, lower g i d
+        const R dr = sxm * (vxm.ar + sg * vxm.bi), di = sxm * (vxm.ai - sg * vxm.br);
+        const R hr = uxm.x * dr + uxm.y * di, hi = uxm.x * di - uxm.y * dr;
+        o.ar += h * hr;
+        o.ai += h * hi;
+        o.br -= h * sg * hi;   // g i (hr + i hi) = -g hi + i g hr
+        o.bi += h * sg * hr;
+    }
+    if constexpr (D5) {
+        // D:   spin1 <- s+1 (wall: +mf psi(0)),    spin0 <- s-1 (wall: +mf psi(Ls-1))
+        // D^+: spin1 <- s-1 (wall: +mf psi(Ls-1)), spin0 <- s+1 (wall: +mf psi(0))
+        const R cup = (s + 1 == L.Ls) ? mf : (R)-1;
+        const R cdn = (s == 0) ? mf : (R)-1;
+        if constexpr (!DAG) {
+            o.br += cup * up.br;
+            o.bi += cup * up.bi;
+            o.ar += cdn * dn.ar;
+            o.ai += cdn * dn.ai;
+        } else {
+            o.br += cdn * dn.br;
+            o.bi += cdn * dn.bi;
+            o.ar += cup * up.ar;
+            o.ai += cup * up.ai;
+        }
+    }
+
+    return o;
+}
+
+// Epilogue at spinor index idx of right-hand side offset rb.
+template <typename R, int EPI>
+__device__ __forceinline__ void hop_epi(const HopArgs<R>& a, long long rb, long long idx, const Sp<R>& o, R alpha,
+                                        double& acc0, double& acc1)
+{
+    (void)alpha;
+    (void)acc1;
+    if constexpr (EPI == EPI_STORE) {
+        if (a.cfg.nt_store)
+            stsp_nt(a.out + rb, idx, o);
+        else
+            stsp(a.out + rb, idx, o);
+    } else if constexpr (EPI == EPI_NORM) {
+        if (a.cfg.nt_store)
+            stsp_nt(a.out + rb, idx, o);
+        else
+            stsp(a.out + rb, idx, o);
+        acc0 += norm2(o);
+    } else if constexpr (EPI == EPI_CGUPD) {
+        const Sp<R> pv = ldsp(a.p + rb, idx);
+        Sp<R> xv = ldsp(a.x + rb, idx);
+        Sp<R> rv = ldsp(a.r + rb, idx);
+        xv.ar += alpha * pv.ar;
+        xv.ai += alpha * pv.ai;
+        xv.br += alpha * pv.br;
+        xv.bi += alpha * pv.bi;
+        rv.ar -= alpha * o.ar;
+        rv.ai -= alpha * o.ai;
+        rv.br -= alpha * o.br;
+        rv.bi -= alpha * o.bi;
+        stsp(a.x + rb, idx, xv);
+        stsp(a.r + rb, idx, rv);
+        acc0 += norm2(rv);
+    } else {  // EPI_RESID (FP64 only)
+        if constexpr (sizeof(R) == 8) {
+            const Sp<double> bv = ldsp(a.b + rb, idx);
+            const Sp<double> rv{bv.ar - o.ar, bv.ai - o.ai, bv.br - o.br, bv.bi - o.bi};
+            stsp(a.r64 + rb, idx, rv);
+            if (a.p64) stsp(a.p64 + rb, idx, rv);
+            if (a.r32 || a.p32) {
+                const Sp<float> fv{(float)rv.ar, (float)rv.ai, (float)rv.br, (float)rv.bi};
+                if (a.r32) stsp(a.r32 + rb, idx, fv);
+                if (a.p32) stsp(a.p32 + rb, idx, fv);
+            }
+            acc0 += norm2(rv);
+            acc1 += norm2(bv);
+        }
+    }
+
+}
+
 template <typename R, bool DAG, bool D5, int EPI, bool PF>
 __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
 {
@@ -163,20 +311,8 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
         const int tm = (t == 0) ? Nt - 1 : t - 1;
         const int xp = (x + 1 == Nx) ? 0 : x + 1;
         const int xm = (x == 0) ? Nx - 1 : x - 1;
-        const R stp = (t == Nt - 1) ? (R)L.bct : (R)1;
-        const R stm = (t == 0) ? (R)L.bct : (R)1;
-        const R sxp = (x == Nx - 1) ? (R)L.bcx : (R)1;
-        const R sxm = (x == 0) ? (R)L.bcx : (R)1;
-        using C = typename Cx<R>::T;
-        const int cfg = a.cfg_map ? a.cfg_map[rhs] : rhs / a.rhs_per_cfg;
-        const C* lk = static_cast<const C*>(a.links) + (long long)cfg * a.link_stride;
-        const C ut = lk[site];
-        const C ux = lk[V4 + site];
-        const C utm = lk[tm * Nx + x];
-        const C uxm = lk[V4 + t * Nx + xm];
+        const SiteK<R> k = site_consts<R>(a, rhs, t, x);
         const int ntp = tp * Nx + x, ntm = tm * Nx + x, nxp = t * Nx + xp, nxm = t * Nx + xm;
-        const R diag = (R)L.diag, mf = (R)L.mf, h = (R)-0.5;
-        constexpr R sg = DAG ? (R)-1 : (R)1;
 
         const R* in = a.in + rb;
         // PF (software pipeline): the loads of slice s+1 (four neighbours) and of the
@@ -216,99 +352,8 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
                 if (D5 || s + 2 < s1) up2 = ldsp(in, (long long)wrap(s + 2) * V4 + site);
             }
 
-            Sp<R> o{diag * cur.ar, diag * cur.ai, diag * cur.br, diag * cur.bi};
-            {   // forward t: U_t(x) (1 - g sigma1) psi(x+t): upper a = v0 - g v1, lower -g a
-                const R ar = stp * (vtp.ar - sg * vtp.br), ai = stp * (vtp.ai - sg * vtp.bi);
-                const R hr = ut.x * ar - ut.y * ai, hi = ut.x * ai + ut.y * ar;
-                o.ar += h * hr;
-                o.ai += h * hi;
-                o.br -= h * sg * hr;
-                o.bi -= h * sg * hi;
-            }
-            {   // backward t: U_t(x-t)^* (1 + g sigma1) psi(x-t): upper b = v0 + g v1, lower g b
-                const R br = stm * (vtm.ar + sg * vtm.br), bi = stm * (vtm.ai + sg * vtm.bi);
-                const R hr = utm.x * br + utm.y * bi, hi = utm.x * bi - utm.y * br;
-                o.ar += h * hr;
-                o.ai += h * hi;
-                o.br += h * sg * hr;
-                o.bi += h * sg * hi;
-            }
-            {   // forward x: U_x(x) (1 - g sigma2) psi(x+x): upper c = v0 + g i v1, lower -g i c
-                const R cr = sxp * (vxp.ar - sg * vxp.bi), ci = sxp * (vxp.ai + sg * vxp.br);
-                const R hr = ux.x * cr - ux.y * ci, hi = ux.x * ci + ux.y * cr;
-                o.ar += h * hr;
-                o.ai += h * hi;
-                o.br += h * sg * hi;   // -g i (hr + i hi) = g hi - i g hr
-                o.bi -= h * sg * hr;
-            }
-            {   // backward x: U_x(x-x)^* (1 + g sigma2) psi(x-x): upper d = v0 - g i v1, lower g i d
-                const R dr = sxm * (vxm.ar + sg * vxm.bi), di = sxm * (vxm.ai - sg * vxm.br);
-                const R hr = uxm.x * dr + uxm.y * di, hi = uxm.x * di - uxm.y * dr;
-                o.ar += h * hr;
-                o.ai += h * hi;
-                o.br -= h * sg * hi;   // g i (hr + i hi) = -g hi + i g hr
-                o.bi += h * sg * hr;
-            }
-            if constexpr (D5) {
-                // D:   spin1 <- s+1 (wall: +mf psi(0)),    spin0 <- s-1 (wall: +mf psi(Ls-1))
-                // D^+: spin1 <- s-1 (wall: +mf psi(Ls-1)), spin0 <- s+1 (wall: +mf psi(0))
-                const R cup = (s + 1 == L.Ls) ? mf : (R)-1;
-                const R cdn = (s == 0) ? mf : (R)-1;
-                if constexpr (!DAG) {
-                    o.br += cup * up.br;
-                    o.bi += cup * up.bi;
-                    o.ar += cdn * dn.ar;
-                    o.ai += cdn * dn.ai;
-                } else {
-                    o.br += cdn * dn.br;
-                    o.bi += cdn * dn.bi;
-                    o.ar += cup * up.ar;
-                    o.ai += cup * up.ai;
-                }
-            }
-
-            const long long idx = sl + site;
-            if constexpr (EPI == EPI_STORE) {
-                if (a.cfg.nt_store)
-                    stsp_nt(a.out + rb, idx, o);
-                else
-                    stsp(a.out + rb, idx, o);
-            } else if constexpr (EPI == EPI_NORM) {
-                if (a.cfg.nt_store)
-                    stsp_nt(a.out + rb, idx, o);
-                else
-                    stsp(a.out + rb, idx, o);
-                acc0 += norm2(o);
-            } else if constexpr (EPI == EPI_CGUPD) {
-                const Sp<R> pv = ldsp(a.p + rb, idx);
-                Sp<R> xv = ldsp(a.x + rb, idx);
-                Sp<R> rv = ldsp(a.r + rb, idx);
-                xv.ar += alpha * pv.ar;
-                xv.ai += alpha * pv.ai;
-                xv.br += alpha * pv.br;
-                xv.bi += alpha * pv.bi;
-                rv.ar -= alpha * o.ar;
-                rv.ai -= alpha * o.ai;
-                rv.br -= alpha * o.br;
-                rv.bi -= alpha * o.bi;
-                stsp(a.x + rb, idx, xv);
-                stsp(a.r + rb, idx, rv);
-                acc0 += norm2(rv);
-            } else {  // EPI_RESID (FP64 only)
-                if constexpr (sizeof(R) == 8) {
-                    const Sp<double> bv = ldsp(a.b + rb, idx);
-                    const Sp<double> rv{bv.ar - o.ar, bv.ai - o.ai, bv.br - o.br, bv.bi - o.bi};
-                    stsp(a.r64 + rb, idx, rv);
-                    if (a.p64) stsp(a.p64 + rb, idx, rv);
-                    if (a.r32 || a.p32) {
-                        const Sp<float> fv{(float)rv.ar, (float)rv.ai, (float)rv.br, (float)rv.bi};
-                        if (a.r32) stsp(a.r32 + rb, idx, fv);
-                        if (a.p32) stsp(a.p32 + rb, idx, fv);
-                    }
-                    acc0 += norm2(rv);
-                    acc1 += norm2(bv);
-                }
-            }
+            const Sp<R> o = hop_site<R, DAG, D5>(L, k, cur, vtp, vtm, vxp, vxm, dn, up, s);
+            hop_epi<R, EPI>(a, rb, sl + site, o, alpha, acc0, acc1);
 
             dn = cur;
             cur = up;
@@ -320,6 +365,124 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
                 vxm = nxm_;
             }
         }
+    }
+
+    if constexpr (EPI != EPI_STORE) {
+        block_sum2(acc0, acc1, sh);
+        if (threadIdx.x == 0) {
+            double* pp = a.part + ((size_t)rhs * nblk + bid) * 2;
+            pp[0] = acc0;
+            pp[1] = acc1;
+        }
+    }
+}
+
+
+// LDS-tiled variant: a block owns a full 2D tile of (kBlock/tx) x tx sites of one
+// right-hand side and walks its s-chunk.  Each slice of the tile plus its one-site halo
+// is staged in LDS once (double-buffered, one barrier per slice): a site costs one
+// global load of its own spinor plus (2 tx + 2 ty)/256 halo loads instead of five
+// loads.  The fifth-dimension window stays in registers; the next slice's halo and
+// the slice after next's centre are in flight while the current slice is computed.
+// Needs Nx % tx == 0 and Nt % (kBlock/tx) == 0 (every thread owns a site).
+constexpr int kLdsMax = 400;  // (ty+2)(tx+2) for tx in {8, 16, 32, 64}
+
+template <typename R, bool DAG, bool D5, int EPI>
+__global__ __launch_bounds__(kBlock) void k_hop_lds(const HopArgs<R> a)
+{
+    const int nblk = a.cfg.nblk();
+    const int total = gridDim.x;
+    const int lb = a.cfg.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
+    const int rhs = lb / nblk;
+    const int bid = lb - rhs * nblk;
+    const int chunk = bid / a.cfg.bx;
+    const int tile = bid - chunk * a.cfg.bx;
+    const Lat& L = a.L;
+    if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
+
+    __shared__ double sh[8];
+    using V = typename Vec4<R>::T;
+    __shared__ V lds[2][kLdsMax];
+
+    R alpha = 0;
+    if constexpr (EPI == EPI_CGUPD) {
+        const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
+        if (pAp == 0.0) return;  // block-uniform; the p-update kernel records the failure
+        alpha = (R)(a.st[rhs].rr[a.slot] / pAp);
+    }
+
+    const int V4 = L.V4, Nx = L.Nx, Nt = L.Nt, Ls = L.Ls;
+    const long long rb = (long long)a.L.V5() * 4 * rhs;
+    const int tx = a.cfg.tx, ty = kBlock / tx, W = tx + 2;
+    const int tt = tile / a.cfg.tiles_x, tc = tile - tt * a.cfg.tiles_x;
+    const int lt = threadIdx.x / tx, lx = threadIdx.x - lt * tx;
+    const int t0 = tt * ty, x0 = tc * tx;
+    const int t = t0 + lt, x = x0 + lx;
+    const int site = t * Nx + x;
+    const int own = (lt + 1) * W + lx + 1;
+    // this thread's halo site (2 tx + 2 ty <= kBlock of them), or none
+    int hsite = -1, hslot = 0;
+    {
+        const int hh = threadIdx.x;
+        const int tb = t0 == 0 ? Nt - 1 : t0 - 1, ta = t0 + ty == Nt ? 0 : t0 + ty;
+        const int xl = x0 == 0 ? Nx - 1 : x0 - 1, xr = x0 + tx == Nx ? 0 : x0 + tx;
+        if (hh < tx) {
+            hsite = tb * Nx + x0 + hh;
+            hslot = hh + 1;
+        } else if (hh < 2 * tx) {
+            hsite = ta * Nx + x0 + (hh - tx);
+            hslot = (ty + 1) * W + (hh - tx) + 1;
+        } else if (hh < 2 * tx + ty) {
+            hsite = (t0 + hh - 2 * tx) * Nx + xl;
+            hslot = (hh - 2 * tx + 1) * W;
+        } else if (hh < 2 * tx + 2 * ty) {
+            hsite = (t0 + hh - 2 * tx - ty) * Nx + xr;
+            hslot = (hh - 2 * tx - ty + 1) * W + tx + 1;
+        }
+    }
+    const SiteK<R> k = site_consts<R>(a, rhs, t, x);
+    double acc0 = 0.0, acc1 = 0.0;
+    const int s0 = chunk * a.cfg.spt;
+    const int s1 = min(Ls, s0 + a.cfg.spt);
+    auto wrap = [Ls](int s) { return s >= Ls ? s - Ls : (s < 0 ? s + Ls : s); };
+    auto put = [&](int b, int slot, const Sp<R>& v) {
+        V q;
+        q.x = v.ar;
+        q.y = v.ai;
+        q.z = v.br;
+        q.w = v.bi;
+        lds[b][slot] = q;
+    };
+    auto get = [&](int b, int slot) {
+        const V q = lds[b][slot];
+        return Sp<R>{q.x, q.y, q.z, q.w};
+    };
+    const R* in = a.in + rb;
+    Sp<R> cur = ldsp(in, (long long)s0 * V4 + site);
+    Sp<R> dn{0, 0, 0, 0}, up{0, 0, 0, 0};
+    if constexpr (D5) dn = ldsp(in, (long long)wrap(s0 - 1) * V4 + site);
+    if (D5 || s0 + 1 < s1) up = ldsp(in, (long long)wrap(s0 + 1) * V4 + site);
+    put(0, own, cur);
+    if (hsite >= 0) put(0, hslot, ldsp(in, (long long)s0 * V4 + hsite));
+    int buf = 0;
+    for (int s = s0; s < s1; ++s) {
+        __syncthreads();  // slice s staged in lds[buf]; nobody still reads lds[buf ^ 1]
+        const bool more = s + 1 < s1;
+        Sp<R> up2{0, 0, 0, 0}, hn{0, 0, 0, 0};
+        if (D5 || s + 2 < s1) up2 = ldsp(in, (long long)wrap(s + 2) * V4 + site);
+        if (more && hsite >= 0) hn = ldsp(in, (long long)(s + 1) * V4 + hsite);
+        const Sp<R> vtp = get(buf, own + W), vtm = get(buf, own - W);
+        const Sp<R> vxp = get(buf, own + 1), vxm = get(buf, own - 1);
+        const Sp<R> o = hop_site<R, DAG, D5>(L, k, cur, vtp, vtm, vxp, vxm, dn, up, s);
+        hop_epi<R, EPI>(a, rb, (long long)s * V4 + site, o, alpha, acc0, acc1);
+        if (more) {
+            put(buf ^ 1, own, up);
+            if (hsite >= 0) put(buf ^ 1, hslot, hn);
+        }
+        dn = cur;
+        cur = up;
+        up = up2;
+        buf ^= 1;
     }
 
     if constexpr (EPI != EPI_STORE) {
@@ -519,9 +682,24 @@ void dispatch_epi_pf(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 }
 
 template <typename R, bool DAG, bool D5>
+void dispatch_lds(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
+{
+    switch (epi) {
+        case EPI_STORE: k_hop_lds<R, DAG, D5, EPI_STORE><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_NORM: k_hop_lds<R, DAG, D5, EPI_NORM><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_CGUPD: k_hop_lds<R, DAG, D5, EPI_CGUPD><<<grid, kBlock, 0, s>>>(a); break;
+        default:
+            if constexpr (sizeof(R) == 8) k_hop_lds<R, DAG, D5, EPI_RESID><<<grid, kBlock, 0, s>>>(a);
+            break;
+    }
+}
+
+template <typename R, bool DAG, bool D5>
 void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 {
-    if (a.cfg.prefetch)
+    if (a.cfg.lds)
+        dispatch_lds<R, DAG, D5>(epi, a, grid, s);
+    else if (a.cfg.prefetch)
         dispatch_epi_pf<R, DAG, D5, true>(epi, a, grid, s);
     else
         dispatch_epi_pf<R, DAG, D5, false>(epi, a, grid, s);
@@ -529,7 +707,8 @@ void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 
 }  // namespace
 
-HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefetch, int tile_x, int nt_store)
+HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefetch, int tile_x, int nt_store,
+              int lds)
 {
     HopCfg c{};
     c.nt_store = nt_store;
@@ -545,6 +724,7 @@ HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefe
         c.tiles_x = 0;
         c.bx = (L.V4 + kBlock - 1) / kBlock;
     }
+    c.lds = (lds && (c.tx == 8 || c.tx == 16 || c.tx == 32 || c.tx == 64) && L.Nt % (kBlock / c.tx) == 0) ? 1 : 0;
     int chunks = 1;
     if (chunks_override > 0) {
         chunks = chunks_override;

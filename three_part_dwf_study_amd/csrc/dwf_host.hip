@@ -163,7 +163,7 @@ template <typename R> HopArgs<R> hop_base(const dwf_ctx* c, const HopCfg& cfg)
 // out = op(in) on device buffers; tmp is needed for the normal operators.
 template <typename R> void apply_op(dwf_ctx* c, int op, const R* in, R* out, R* tmp, int nrhs)
 {
-    const HopCfg cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store);
+    const HopCfg cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store, c->lds);
     HopArgs<R> a = hop_base<R>(c, cfg);
     switch (op) {
         case DWF_OP_D:
@@ -322,6 +322,9 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             return DWF_OK;
         case DWF_TUNE_GRAPHS:
             ctx->use_graphs = value != 0.0 ? 1 : 0;
+            return DWF_OK;
+        case DWF_TUNE_LDS:
+            ctx->lds = value != 0.0 ? 1 : 0;
             return DWF_OK;
         default:
             break;

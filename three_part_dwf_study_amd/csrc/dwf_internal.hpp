@@ -53,6 +53,7 @@ struct HopCfg {
     int tx;        // 0: block = 256 consecutive 4D sites; else a 2D tile of (256/tx) rows x tx columns
     int tiles_x;   // Nx / tx
     int nt_store;  // 1: streaming (non-temporal) stores of the stencil output
+    int lds;       // 1: LDS-staged slice tiles (needs full 2D tiles)
     __host__ __device__ int nblk() const { return bx * chunks; }
 };
 
@@ -116,7 +117,7 @@ template <typename R> void launch_zero(R* p, long long n, hipStream_t s);
 void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s);
 
 HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override = 0, int remap = 1, int prefetch = 0, int tile_x = 0,
-              int nt_store = 0);
+              int nt_store = 0, int lds = 0);
 
 // HMC kernels (dwf_hmc.hip).  `scalars` is a small device array of FP64 results; each
 // reduction folds its block partials in index order into scalars[slot].

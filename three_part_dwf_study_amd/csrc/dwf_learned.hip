@@ -878,7 +878,7 @@ template <typename R> HopArgs<R> hop_args(const dwf_ctx* c, int nrhs, const int*
 {
     HopArgs<R> a{};
     a.L = c->L;
-    a.cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store);
+    a.cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store, c->lds);
     a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
     a.link_stride = 2LL * c->L.V4;
     a.rhs_per_cfg = c->ncfg <= 1 ? (1 << 30) : c->rhs_per_cfg;

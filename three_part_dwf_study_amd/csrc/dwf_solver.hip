@@ -75,7 +75,7 @@ struct FullOp : CgOperator {
     double* t64;
     FullOp(dwf_ctx* ctx, bool fd, int n, double* tmp64) : c(ctx), first_dag(fd), nrhs(n), t64(tmp64)
     {
-        cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store);
+        cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store, c->lds);
         sites = c->L.V5();
         nblk = cfg.nblk();
     }
@@ -682,7 +682,7 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
     const size_t nv = (size_t)c->L.V5() * 4 * nrhs;
     double* B = (double*)c->buf(B_EO_B, nv * 8);
     {
-        HopArgs<double> a = hop_base<double>(c, hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store));
+        HopArgs<double> a = hop_base<double>(c, hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store, c->lds));
         a.in = src;
         a.out = B;
         launch_hop<double>(true, EPI_STORE, a, nrhs, c->stream);
