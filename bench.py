@@ -457,6 +457,7 @@ def main():
     ap.add_argument("--learn-loss", type=int, default=1, choices=(0, 1))
     ap.add_argument("--learn-lr", type=float, default=3e-3)
     ap.add_argument("--analyzer-configs", type=int, default=32, help="configurations per lattice size")
+    ap.add_argument("--tune-cache", default=None, help="autotuner result file (reused when present)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the roofline launches (batched FP32 D apply and the copy kernel), for counter runs")
     args = ap.parse_args()
@@ -482,7 +483,7 @@ def main():
     if args.kernels_only:
         ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
         ctx.set_gauge(theta)
-        ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32)
+        ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32, args.tune_cache)
         ms32, gbs32 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, args.dslash_rhs, 5)
         cb = dslash_bytes(4, args.dslash_rhs) // 2 // 16 * 16
         msc, gbsc = time_copy(ctx, capi, cb, 5)
@@ -496,7 +497,7 @@ def main():
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
     ctx.set_gauge(theta)
 
-    tuned = {"solves": ctx.autotune(nrhs, dwf.DWF_PREC_FP32)}
+    tuned = {"solves": ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)}
     log(f"autotuned stencil shape for {nrhs} RHS: {tuned['solves']}")
     n = nrhs * LS * NT * NX * 4
     d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
@@ -539,11 +540,11 @@ def main():
     pcie_rate = args.steps * nrhs / (time.perf_counter() - t0)
     log(f"eo_mixed with host arrays (PCIe-inclusive): {pcie_rate:.1f} solves/s")
 
-    tuned["fp32_batch"] = ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32)
+    tuned["fp32_batch"] = ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32, args.tune_cache)
     ms32, gbs32 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, args.dslash_rhs, 20)
-    tuned["fp64_batch"] = ctx.autotune(args.dslash_rhs // 2, dwf.DWF_PREC_FP64)
+    tuned["fp64_batch"] = ctx.autotune(args.dslash_rhs // 2, dwf.DWF_PREC_FP64, args.tune_cache)
     ms64, gbs64 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP64, args.dslash_rhs // 2, 20)
-    tuned["fp32_single"] = ctx.autotune(1, dwf.DWF_PREC_FP32)
+    tuned["fp32_single"] = ctx.autotune(1, dwf.DWF_PREC_FP32, args.tune_cache)
     ms1, gbs1 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, 1, 200)
     msc, gbsc = time_copy(ctx, capi, dslash_bytes(4, args.dslash_rhs) // 2 // 16 * 16, 20)
     log(f"copy kernel ({dslash_bytes(4, args.dslash_rhs) // 2 >> 20} MiB each way): {msc:.4f} ms {gbsc:.0f} GB/s")

@@ -15,11 +15,15 @@ __global__ void k_f64_to_f32(const double* a, float* b, const long long n)
         b[i] = (float)a[i];
 }
 
+// Plain copy of n 16-byte words with streaming (non-temporal) loads and stores: the
+// achievable-bandwidth reference the stencil is compared with.
 __global__ __launch_bounds__(256) void k_copy16(const float4* __restrict__ src, float4* __restrict__ dst,
                                                 long long n)
 {
+    typedef float f4 __attribute__((ext_vector_type(4)));
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        dst[i] = src[i];
+        __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<const f4*>(src) + i),
+                                    reinterpret_cast<f4*>(dst) + i);
 }
 
 int fail(hipError_t e, const char* what)
@@ -129,7 +133,7 @@ int dwf_copy_device(dwf_ctx* ctx, void* dst, const void* src, unsigned long long
         return DWF_ERR_ARG;
     }
     const long long n = (long long)(bytes / 16);
-    const long long blocks = std::min<long long>((n + 255) / 256, 256LL * 64);
+    const long long blocks = std::min<long long>((n + 255) / 256, 65536LL);
     k_copy16<<<(unsigned)blocks, 256, 0, ctx->stream>>>((const float4*)src, (float4*)dst, n);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? DWF_OK : fail(e, "dwf_copy_device");
