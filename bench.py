@@ -458,6 +458,8 @@ def main():
     ap.add_argument("--learn-lr", type=float, default=3e-3)
     ap.add_argument("--analyzer-configs", type=int, default=32, help="configurations per lattice size")
     ap.add_argument("--tune-cache", default=None, help="autotuner result file (reused when present)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch CG iterations one by one instead of replaying HIP graphs (for tracing tools)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the roofline launches (batched FP32 D apply and the copy kernel), for counter runs")
     args = ap.parse_args()
@@ -496,6 +498,8 @@ def main():
     src = boundary_sources(nrhs, 918273)
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
     ctx.set_gauge(theta)
+    if args.no_graphs:
+        ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
 
     tuned = {"solves": ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)}
     log(f"autotuned stencil shape for {nrhs} RHS: {tuned['solves']}")

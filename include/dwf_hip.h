@@ -106,9 +106,10 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * 256) of 2D stencil tiles of 256/width rows; DWF_TUNE_NT_STORE: 0/1 streaming
  * (non-temporal) stores of stencil outputs; DWF_TUNE_GRAPHS: 0/1 (default 1) replay
  * blocks of 8 CG iterations as instantiated HIP graphs instead of launching their
- * kernels one by one (same kernels, same order, same results); DWF_TUNE_LDS: 0/1 stage
- * each slice of a 2D stencil tile and its halo in LDS (with DWF_TUNE_TILE_X in
- * {8, 16, 32, 64} dividing Nx and 256/width dividing Nt; ignored otherwise). */
+ * kernels one by one (same kernels, same order, same results); DWF_TUNE_LDS: 0, or 1 / 2
+ * to stage each slice of a 2D stencil tile of 256 / 512 sites and its halo in LDS
+ * (with DWF_TUNE_TILE_X in {8, 16, 32, 64} dividing Nx and sites/width dividing Nt;
+ * ignored otherwise). */
 enum dwf_tune_param {
     DWF_TUNE_SCHUNKS = 0,
     DWF_TUNE_XCD_REMAP = 1,
@@ -130,7 +131,7 @@ typedef struct dwf_tune_result {
     int tile_x, prefetch, chunks, nt_store;
     int from_cache;
     double microseconds;  /* one D apply of nrhs right-hand sides */
-    int lds;              /* 1: the LDS-staged tile kernel won */
+    int lds;              /* 1 / 2: the LDS-staged tile kernel of 256 / 512 threads won */
     int pad;
 } dwf_tune_result;
 int dwf_autotune(dwf_ctx* ctx, int nrhs, int precision, const char* cache_path, dwf_tune_result* result);

@@ -123,18 +123,19 @@ def test_tuned_launch_shapes_keep_parity(tile_x, prefetch, chunks):
         assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= 1e-10
 
 
-@pytest.mark.parametrize("tile_x,chunks", [(8, 0), (16, 2), (32, 6), (64, 1)])
-def test_lds_tiles_keep_parity(tile_x, chunks):
+@pytest.mark.parametrize("tile_x,chunks,mode", [(8, 0, 1), (16, 2, 1), (32, 6, 1), (64, 1, 1), (32, 1, 2),
+                                                (16, 3, 2), (64, 2, 2)])
+def test_lds_tiles_keep_parity(tile_x, chunks, mode):
     """The LDS-staged stencil (full 2D tiles, one halo per slice) against the oracle for
     every operator, and inside the FP64 and mixed solvers (its NORM/CGUPD/RESID paths)."""
     Nt, Nx, Ls = 32, 64, 6
-    print_case(test="lds_tiles", tile_x=tile_x, chunks=chunks)
-    rng = np.random.default_rng(100 + tile_x + chunks)
+    print_case(test="lds_tiles", tile_x=tile_x, chunks=chunks, mode=mode)
+    rng = np.random.default_rng(100 + tile_x + chunks + mode)
     th = rand_theta(rng, Nt, Nx)
     psi = rand_spinor(rng, 2 * Ls * Nt * Nx * 2).reshape(2, -1)
     with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
         ctx.set_gauge(th)
-        ctx.tune(dwf.DWF_TUNE_LDS, 1)
+        ctx.tune(dwf.DWF_TUNE_LDS, mode)
         ctx.tune(dwf.DWF_TUNE_TILE_X, tile_x)
         ctx.tune(dwf.DWF_TUNE_NT_STORE, tile_x % 16 == 0)
         ctx.tune(dwf.DWF_TUNE_SCHUNKS, chunks)
@@ -151,7 +152,7 @@ def test_lds_tiles_keep_parity(tile_x, chunks):
     with dwf.DwfContext.wilson(Nt, Nx, 0.1, -1.0, 1.0) as ctx:
         w = rand_spinor(rng, Nt * Nx * 2)
         ctx.set_gauge(th)
-        ctx.tune(dwf.DWF_TUNE_LDS, 1)
+        ctx.tune(dwf.DWF_TUNE_LDS, mode)
         ctx.tune(dwf.DWF_TUNE_TILE_X, tile_x)
         for op in (0, 1):
             assert rel(ctx.apply(op, w), oracle.wilson_apply(op, th, w, Nt, Nx, 0.1, -1.0, 1.0)) <= 1e-12

@@ -104,10 +104,11 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
             for (int pf : {0, 1})
                 for (int ch : chunks)
                     for (int nt : {0, 1}) cand.push_back(Shape{tx, pf, ch, nt, 0});
-        for (int tx : {8, 16, 32, 64})
-            if (L.Nx % tx == 0 && L.Nt % (kBlock / tx) == 0)
-                for (int ch : chunks)
-                    for (int nt : {0, 1}) cand.push_back(Shape{tx, 0, ch, nt, 1});
+        for (int mode : {1, 2})
+            for (int tx : {8, 16, 32, 64})
+                if (L.Nx % tx == 0 && L.Nt % ((mode == 2 ? 512 : kBlock) / tx) == 0)
+                    for (int ch : chunks)
+                        for (int nt : {0, 1}) cand.push_back(Shape{tx, 0, ch, nt, mode});
 
         const size_t rb = precision == DWF_PREC_FP32 ? 4 : 8;
         const size_t n = (size_t)L.V5() * 4 * nrhs;

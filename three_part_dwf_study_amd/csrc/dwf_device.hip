@@ -68,6 +68,12 @@ template <typename R> __device__ __forceinline__ void stsp_nt(R* base, long long
     __builtin_nontemporal_store(v, reinterpret_cast<nv4*>(base) + i);
 }
 
+// c ? a : b field by field (a select of whole structs can put both in scratch memory)
+template <typename R> __device__ __forceinline__ Sp<R> pick(bool c, const Sp<R>& a, const Sp<R>& b)
+{
+    return Sp<R>{c ? a.ar : b.ar, c ? a.ai : b.ai, c ? a.br : b.br, c ? a.bi : b.bi};
+}
+
 template <typename R> __device__ __forceinline__ double norm2(const Sp<R>& s)
 {
     const double a = s.ar, b = s.ai, c = s.br, d = s.bi;
@@ -112,6 +118,30 @@ __device__ __forceinline__ double sum_partials(const double* p, int n, int which
     const double r = sh[0];
     __syncthreads();
     return r;
+}
+
+// Block sum of two accumulators over NT threads (NT/64 wavefronts, fixed order); the
+// result is valid in thread 0.  sh holds 2 * NT / 64 doubles.
+template <int NT> __device__ __forceinline__ void block_sum2_n(double& a, double& b, double* sh)
+{
+    constexpr int nw = NT / 64;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+        sh[w] = a;
+        sh[nw + w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0;
+        for (int i = 0; i < nw; ++i) {
+            x += sh[i];
+            y += sh[nw + i];
+        }
+        a = x;
+        b = y;
+    }
 }
 
 // ------------------------------------------------------------------ stencil kernels
@@ -342,14 +372,14 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
                 vtm = ldsp(in, sl + ntm);
                 vxp = ldsp(in, sl + nxp);
                 vxm = ldsp(in, sl + nxm);
-                if (D5 || s + 2 < s1) up2 = ldsp(in, (long long)wrap(s + 2) * V4 + site);
+                if (D5 ? s + 2 <= s1 : s + 2 < s1) up2 = ldsp(in, (long long)wrap(s + 2) * V4 + site);
             } else if (more) {
                 const long long sn = sl + V4;
                 ntp_ = ldsp(in, sn + ntp);
                 ntm_ = ldsp(in, sn + ntm);
                 nxp_ = ldsp(in, sn + nxp);
                 nxm_ = ldsp(in, sn + nxm);
-                if (D5 || s + 2 < s1) up2 = ldsp(in, (long long)wrap(s + 2) * V4 + site);
+                if (D5 ? s + 2 <= s1 : s + 2 < s1) up2 = ldsp(in, (long long)wrap(s + 2) * V4 + site);
             }
 
             const Sp<R> o = hop_site<R, DAG, D5>(L, k, cur, vtp, vtm, vxp, vxm, dn, up, s);
@@ -382,13 +412,17 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
 // right-hand side and walks its s-chunk.  Each slice of the tile plus its one-site halo
 // is staged in LDS once (double-buffered, one barrier per slice): a site costs one
 // global load of its own spinor plus (2 tx + 2 ty)/256 halo loads instead of five
-// loads.  The fifth-dimension window stays in registers; the next slice's halo and
-// the slice after next's centre are in flight while the current slice is computed.
+// loads.  The fifth-dimension window stays in registers, the next slice's halo and the
+// slice after next's centre are in flight while a slice is computed, and each chunk
+// fetches its spt slices plus the two boundary slices of the fifth-dimension hop, and
+// no more; odd chunks walk downwards so that neighbouring chunks fetch their shared
+// boundary slices at the same time (the second fetch hits L2).
 // Needs Nx % tx == 0 and Nt % (kBlock/tx) == 0 (every thread owns a site).
-constexpr int kLdsMax = 400;  // (ty+2)(tx+2) for tx in {8, 16, 32, 64}
+// (ty+2)(tx+2) for tx in {8, 16, 32, 64} and ty = NT / tx
+constexpr int lds_max(int nt) { return nt == 256 ? 400 : 680; }
 
-template <typename R, bool DAG, bool D5, int EPI>
-__global__ __launch_bounds__(kBlock) void k_hop_lds(const HopArgs<R> a)
+template <typename R, bool DAG, bool D5, int EPI, int NT>
+__global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
 {
     const int nblk = a.cfg.nblk();
     const int total = gridDim.x;
@@ -400,9 +434,9 @@ __global__ __launch_bounds__(kBlock) void k_hop_lds(const HopArgs<R> a)
     const Lat& L = a.L;
     if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
 
-    __shared__ double sh[8];
+    __shared__ double sh[16];
     using V = typename Vec4<R>::T;
-    __shared__ V lds[2][kLdsMax];
+    __shared__ V lds[2][lds_max(NT)];
 
     R alpha = 0;
     if constexpr (EPI == EPI_CGUPD) {
@@ -413,14 +447,14 @@ __global__ __launch_bounds__(kBlock) void k_hop_lds(const HopArgs<R> a)
 
     const int V4 = L.V4, Nx = L.Nx, Nt = L.Nt, Ls = L.Ls;
     const long long rb = (long long)a.L.V5() * 4 * rhs;
-    const int tx = a.cfg.tx, ty = kBlock / tx, W = tx + 2;
+    const int tx = a.cfg.tx, ty = NT / tx, W = tx + 2;
     const int tt = tile / a.cfg.tiles_x, tc = tile - tt * a.cfg.tiles_x;
     const int lt = threadIdx.x / tx, lx = threadIdx.x - lt * tx;
     const int t0 = tt * ty, x0 = tc * tx;
     const int t = t0 + lt, x = x0 + lx;
     const int site = t * Nx + x;
     const int own = (lt + 1) * W + lx + 1;
-    // this thread's halo site (2 tx + 2 ty <= kBlock of them), or none
+    // this thread's halo site (2 tx + 2 ty <= NT of them), or none
     int hsite = -1, hslot = 0;
     {
         const int hh = threadIdx.x;
@@ -457,36 +491,59 @@ __global__ __launch_bounds__(kBlock) void k_hop_lds(const HopArgs<R> a)
         const V q = lds[b][slot];
         return Sp<R>{q.x, q.y, q.z, q.w};
     };
+    // Odd chunks walk their slices downwards, so the slice on each chunk boundary (and
+    // the wall pair Ls-1 / 0) is fetched by both neighbouring blocks at the same moment
+    // and the second fetch hits L2.  Window: prev = psi(s - dir), cur = psi(s),
+    // next = psi(s + dir).
+    const int n = s1 - s0;
+    const int dir = (chunk & 1) ? -1 : 1;
+    const int sa = dir > 0 ? s0 : s1 - 1;
     const R* in = a.in + rb;
-    Sp<R> cur = ldsp(in, (long long)s0 * V4 + site);
-    Sp<R> dn{0, 0, 0, 0}, up{0, 0, 0, 0};
-    if constexpr (D5) dn = ldsp(in, (long long)wrap(s0 - 1) * V4 + site);
-    if (D5 || s0 + 1 < s1) up = ldsp(in, (long long)wrap(s0 + 1) * V4 + site);
+    // One chunk spanning all of s (then dir = +1, sa = 0): the wall hop needs psi(Ls-1)
+    // first and psi(0) last, the slices the walk starts from -- keep them instead of
+    // fetching them twice.
+    const bool whole = D5 && n == Ls;
+    Sp<R> cur = ldsp(in, (long long)sa * V4 + site);
+    Sp<R> prev{0, 0, 0, 0}, next{0, 0, 0, 0};
+    if constexpr (D5) prev = ldsp(in, (long long)wrap(sa - dir) * V4 + site);
+    if (D5 || n > 1) next = ldsp(in, (long long)wrap(sa + dir) * V4 + site);
+    const Sp<R> cur0 = cur, prev0 = prev;
+    const int s_prev0 = wrap(sa - dir);
     put(0, own, cur);
-    if (hsite >= 0) put(0, hslot, ldsp(in, (long long)s0 * V4 + hsite));
+    if (hsite >= 0) put(0, hslot, ldsp(in, (long long)sa * V4 + hsite));
     int buf = 0;
-    for (int s = s0; s < s1; ++s) {
+    for (int k2 = 0; k2 < n; ++k2) {
+        const int s = sa + k2 * dir;
         __syncthreads();  // slice s staged in lds[buf]; nobody still reads lds[buf ^ 1]
-        const bool more = s + 1 < s1;
-        Sp<R> up2{0, 0, 0, 0}, hn{0, 0, 0, 0};
-        if (D5 || s + 2 < s1) up2 = ldsp(in, (long long)wrap(s + 2) * V4 + site);
-        if (more && hsite >= 0) hn = ldsp(in, (long long)(s + 1) * V4 + hsite);
+        const bool more = k2 + 1 < n;
+        // window of the next step: psi(s + 2 dir), a kept wall slice, or nothing
+        Sp<R> next2{0, 0, 0, 0}, hn{0, 0, 0, 0};
+        bool r0 = false, r1 = false;
+        if (more && (D5 || k2 + 2 < n)) {
+            const int sn = wrap(s + 2 * dir);
+            r0 = whole && sn == s_prev0;
+            r1 = whole && sn == sa;
+            if (!r0 && !r1) next2 = ldsp(in, (long long)sn * V4 + site);
+        }
+        next2 = pick(r0, prev0, pick(r1, cur0, next2));
+        if (more && hsite >= 0) hn = ldsp(in, (long long)(s + dir) * V4 + hsite);
         const Sp<R> vtp = get(buf, own + W), vtm = get(buf, own - W);
         const Sp<R> vxp = get(buf, own + 1), vxm = get(buf, own - 1);
+        const Sp<R> dn = pick(dir > 0, prev, next), up = pick(dir > 0, next, prev);
         const Sp<R> o = hop_site<R, DAG, D5>(L, k, cur, vtp, vtm, vxp, vxm, dn, up, s);
         hop_epi<R, EPI>(a, rb, (long long)s * V4 + site, o, alpha, acc0, acc1);
         if (more) {
-            put(buf ^ 1, own, up);
+            put(buf ^ 1, own, next);
             if (hsite >= 0) put(buf ^ 1, hslot, hn);
         }
-        dn = cur;
-        cur = up;
-        up = up2;
+        prev = cur;
+        cur = next;
+        next = next2;
         buf ^= 1;
     }
 
     if constexpr (EPI != EPI_STORE) {
-        block_sum2(acc0, acc1, sh);
+        block_sum2_n<NT>(acc0, acc1, sh);
         if (threadIdx.x == 0) {
             double* pp = a.part + ((size_t)rhs * nblk + bid) * 2;
             pp[0] = acc0;
@@ -681,17 +738,26 @@ void dispatch_epi_pf(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
     }
 }
 
+template <typename R, bool DAG, bool D5, int NT>
+void dispatch_lds_nt(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
+{
+    switch (epi) {
+        case EPI_STORE: k_hop_lds<R, DAG, D5, EPI_STORE, NT><<<grid, NT, 0, s>>>(a); break;
+        case EPI_NORM: k_hop_lds<R, DAG, D5, EPI_NORM, NT><<<grid, NT, 0, s>>>(a); break;
+        case EPI_CGUPD: k_hop_lds<R, DAG, D5, EPI_CGUPD, NT><<<grid, NT, 0, s>>>(a); break;
+        default:
+            if constexpr (sizeof(R) == 8) k_hop_lds<R, DAG, D5, EPI_RESID, NT><<<grid, NT, 0, s>>>(a);
+            break;
+    }
+}
+
 template <typename R, bool DAG, bool D5>
 void dispatch_lds(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 {
-    switch (epi) {
-        case EPI_STORE: k_hop_lds<R, DAG, D5, EPI_STORE><<<grid, kBlock, 0, s>>>(a); break;
-        case EPI_NORM: k_hop_lds<R, DAG, D5, EPI_NORM><<<grid, kBlock, 0, s>>>(a); break;
-        case EPI_CGUPD: k_hop_lds<R, DAG, D5, EPI_CGUPD><<<grid, kBlock, 0, s>>>(a); break;
-        default:
-            if constexpr (sizeof(R) == 8) k_hop_lds<R, DAG, D5, EPI_RESID><<<grid, kBlock, 0, s>>>(a);
-            break;
-    }
+    if (a.cfg.lds == 2)
+        dispatch_lds_nt<R, DAG, D5, 512>(epi, a, grid, s);
+    else
+        dispatch_lds_nt<R, DAG, D5, 256>(epi, a, grid, s);
 }
 
 template <typename R, bool DAG, bool D5>
@@ -724,7 +790,14 @@ HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefe
         c.tiles_x = 0;
         c.bx = (L.V4 + kBlock - 1) / kBlock;
     }
-    c.lds = (lds && (c.tx == 8 || c.tx == 16 || c.tx == 32 || c.tx == 64) && L.Nt % (kBlock / c.tx) == 0) ? 1 : 0;
+    c.lds = 0;
+    if (lds == 1 || lds == 2) {
+        const int nt = lds == 2 ? 512 : kBlock;
+        if ((c.tx == 8 || c.tx == 16 || c.tx == 32 || c.tx == 64) && L.Nt % (nt / c.tx) == 0) {
+            c.lds = lds;
+            c.bx = c.tiles_x * (L.Nt / (nt / c.tx));
+        }
+    }
     int chunks = 1;
     if (chunks_override > 0) {
         chunks = chunks_override;
