@@ -601,7 +601,8 @@ def main():
             "unit": "GB/s",
             "frac": round(gbs32 / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic() if args.dslash_rhs == 128 else None,
-            "kernel": "k_hop<float,D,DWF,EPI_STORE>",
+            "kernel": (f"k_hop_lds<float,D,DWF,EPI_STORE,{256 * tuned['fp32_batch']['lds']}>"
+                       if tuned["fp32_batch"]["lds"] else "k_hop<float,D,DWF,EPI_STORE>"),
             "avg_launch_us": round(ms32 * 1e3, 2),
             "bytes_per_launch": dslash_bytes(4, args.dslash_rhs),
         },
