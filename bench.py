@@ -197,6 +197,9 @@ def run_analyzer(args, dwf):
     per = args.analyzer_configs
     inputs = analyzer_inputs(per)
     ctxs = [dwf.DwfContext.dwf(N, N, LS, M5, MF) for N, _, _, _ in inputs]
+    if args.no_graphs:
+        for c in ctxs:
+            c.tune(dwf.DWF_TUNE_GRAPHS, 0)
     method = dwf.DWF_SOLVE_EO_MIXED
 
     def one(g, out):
@@ -307,6 +310,8 @@ def run_learned(args, dwf, capi):
     # held-out configuration: 16 boundary sources per step
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
     ctx.set_gauge(thetas[ntrain])
+    if args.no_graphs:
+        ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
     ctx.set_preconditioner(net)
     nrhs = args.nrhs
     src = boundary_sources(nrhs, 918273 + 1)
@@ -377,6 +382,8 @@ def run_hmc(args, dwf, capi):
     theta0 = np.random.default_rng(1234).uniform(-np.pi, np.pi, 2 * NT * NX)
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
     ctx.set_gauge(theta0)
+    if args.no_graphs:
+        ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
     rng = capi.HostRng(1234)
     ndof, ngauge = LS * NT * NX * 2, 2 * NT * NX
     method = {"cg64": dwf.DWF_SOLVE_CG64, "mixed": dwf.DWF_SOLVE_MIXED}[args.hmc_solver]
