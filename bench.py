@@ -338,7 +338,7 @@ def run_learned(args, dwf, capi):
         log(f"{name}: {results[name]}")
     fast, judge = results["learned"], results["fp64_cg"]
     line = {
-        "metric": "DWF solves/s 96²,Ls=16 @1e-10: learned preconditioner (flexible CG) vs plain FP64 CG",
+        "metric": "DWF solves/s 96²,Ls=16 @1e-10: learned preconditioner + FP32 (flexible CG) vs plain FP64 CG",
         "value": round(fast["solves_per_s"], 3),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -348,7 +348,7 @@ def run_learned(args, dwf, capi):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16+f64",
+        "dtype": "bf16+f32+f64",
         "data": "synthetic",
         "config": {
             "workload": f"learned preconditioner: trained on {npairs} (source, solution) pairs from {ntrain} "
@@ -356,6 +356,7 @@ def run_learned(args, dwf, capi):
                         "16 boundary sources per step, FP64 true residual <= 1e-10",
             "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs,
             "network": "3 linear parallel-transport conv layers 2-8-8-2 complex channels, BF16 MFMA",
+            "solver": "flexible CG (FCG(1)) in FP32 with FP64 reliable updates, certified at 1e-10",
             "loss": ["||M b - x||^2/||x||^2", "||A M b - b||^2/||b||^2"][args.learn_loss],
             "training_steps": args.learn_steps, "batch": args.learn_batch, "lr": args.learn_lr,
         },

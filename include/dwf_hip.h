@@ -60,9 +60,11 @@ enum dwf_method {
      * defect correction; certified on the reference's full system.  Needs even Nt, Nx
      * and Ls in {4, 8, 12, 16}; otherwise it runs as DWF_SOLVE_MIXED. */
     DWF_SOLVE_EO_MIXED = 2,
-    /* Domain-wall D^dag D only: flexible CG (FCG(1)) in FP64 preconditioned by the
-     * learned network attached with dwf_ctx_set_preconditioner; certified on the FP64
-     * true residual.  No counterpart in the reference. */
+    /* Domain-wall D^dag D only: flexible CG (FCG(1)) preconditioned by the learned
+     * network attached with dwf_ctx_set_preconditioner, FP32 cycles between FP64
+     * reliable updates, certified on the FP64 true residual; plain FP64 CG continues
+     * if the network breaks down (stats: fp64_iterations > 0).  No counterpart in the
+     * reference. */
     DWF_SOLVE_LEARNED = 3
 };
 

@@ -83,6 +83,12 @@ enum BufId : int {
     B_FCG_Z,
     B_FCG_PART,
     B_FCG_SCAL,
+    B_FCG_R32,
+    B_FCG_P32,
+    B_FCG_Q32,
+    B_FCG_Z32,
+    B_FCG_T32,
+    B_FCG_X32,
     B_COUNT
 };
 

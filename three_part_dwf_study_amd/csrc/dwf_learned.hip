@@ -77,6 +77,16 @@ __device__ __forceinline__ unsigned short f2bf(float f)
     u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite values)
     return (unsigned short)(u >> 16);
 }
+
+// Two floats to packed bf16 (low, high) with gfx950's v_cvt_pk_bf16_f32 (round to
+// nearest even, the same rounding as f2bf for finite values).
+typedef __bf16 bf16x2_hw __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ unsigned pk_bf16(float lo, float hi)
+{
+    const f32x2_t v = {lo, hi};
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2_hw));
+}
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
 struct PtcArgs {
@@ -169,16 +179,67 @@ __device__ __forceinline__ void st_spinor(float* p, long long site, float a, flo
 
 __device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d)
 {
+    const unsigned lo = pk_bf16(a, b), hi = pk_bf16(c, d);
     bf16x4_t v;
-    v[0] = (short)f2bf(a);
-    v[1] = (short)f2bf(b);
-    v[2] = (short)f2bf(c);
-    v[3] = (short)f2bf(d);
+    v[0] = (short)(lo & 0xffffu);
+    v[1] = (short)(lo >> 16);
+    v[2] = (short)(hi & 0xffffu);
+    v[3] = (short)(hi >> 16);
     return v;
 }
 
 // ------------------------------------------------------------------ forward layer
+// A operand of MFMA step q for the lane's site (t, x) of slice s: the transported
+// neighbour channels of stencil point q (C_in = 8: channels 2g, 2g+1) or of point
+// 4q + g (C_in = 2: both channels), rounded to bf16.
+template <int CIN, typename TIN>
+__device__ __forceinline__ bf16x4_t build_A(const PtcArgs& a, const float2* L, const void* inb, int q, int s, int t,
+                                            int x, int g)
+{
+    if constexpr (CIN == 8) {
+        const Nb n = neighbour(a, L, q, s, t, x);
+        const ushort4 v = *reinterpret_cast<const ushort4*>(static_cast<const unsigned short*>(inb) +
+                                                            ((long long)n.s * a.V4 + n.i) * 16 + 4 * g);
+        const float h0r = bf2f(v.x), h0i = bf2f(v.y), h1r = bf2f(v.z), h1i = bf2f(v.w);
+        return pack4(tr_re(n, h0r, h0i), tr_im(n, h0r, h0i), tr_re(n, h1r, h1i), tr_im(n, h1r, h1i));
+    } else {
+        const int p = 4 * q + g;  // this lane group's stencil point
+        bf16x4_t A = {0, 0, 0, 0};
+        if (p < kPts) {
+            const Nb n = neighbour(a, L, p, s, t, x);
+            const float4 v = ld_spinor(static_cast<const TIN*>(inb), (long long)n.s * a.V4 + n.i);
+            A = pack4(tr_re(n, v.x, v.y), tr_im(n, v.x, v.y), tr_re(n, v.z, v.w), tr_im(n, v.z, v.w));
+        }
+        return A;
+    }
+}
+
+// D[4g + j][m] of one 16-site tile at obase (sites obase .. obase + 15)
+template <int COUT, typename TOUT>
+__device__ __forceinline__ void store_tile(const PtcArgs& a, const f32x4_t& acc, long long obase, int m, int g)
+{
+    if constexpr (COUT == 8) {
+        unsigned short* out = static_cast<unsigned short*>(a.out) + obase * 16;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float v = acc[j];
+            const float o = __shfl_xor(v, 1, 64);
+            if ((m & 1) == 0) *reinterpret_cast<unsigned*>(out + (4 * g + j) * 16 + m) = pk_bf16(v, o);
+        }
+    } else {
+        TOUT* out = static_cast<TOUT*>(a.out);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float v0 = __shfl(acc[j], 16 * g + 0, 64), v1 = __shfl(acc[j], 16 * g + 1, 64);
+            const float v2 = __shfl(acc[j], 16 * g + 2, 64), v3 = __shfl(acc[j], 16 * g + 3, 64);
+            if (m == j) st_spinor(out, obase + 4 * g + j, v0, v1, v2, v3);
+        }
+    }
+}
+
 // TIN/TOUT: double or float for spinors (2 channels), unsigned short for 8 BF16 channels.
+// Two adjacent tiles of one slice are processed together when they exist, so the loads
+// of both are in flight before either MFMA chain starts.
 template <int CIN, int COUT, typename TIN, typename TOUT>
 __global__ __launch_bounds__(256) void k_ptc_fwd(const PtcArgs a)
 {
@@ -188,69 +249,58 @@ __global__ __launch_bounds__(256) void k_ptc_fwd(const PtcArgs a)
     const long long T0 = wave * a.tpw;
     const long long T1 = T0 + a.tpw < a.ntiles ? T0 + a.tpw : a.ntiles;
     const long long per_rhs = (long long)a.Ls * a.tps;
-    int cur_s = -1;
+    constexpr int ch = CIN == 8 ? 16 : 4;  // input values per site
+    int cur_s = -1, cur_rhs = -1;
+    bool skip = false;
+    const float2* L = a.links;
+    const void* inb = a.in;
     bf16x4_t B[NSTEP];
-    for (long long T = T0; T < T1; ++T) {
+    for (long long T = T0; T < T1;) {
         const int rhs = (int)(T / per_rhs);
         const int rem = (int)(T - rhs * per_rhs);
         const int s = rem / a.tps;
-        const int i0 = (rem - s * a.tps) * 16;
-        if (a.st && (a.st[rhs].flags[0] & F_DONE)) continue;  // uniform per wavefront
+        const int tl = rem - s * a.tps;
+        const int i0 = tl * 16;
+        const bool two = T + 1 < T1 && tl + 1 < a.tps;  // the next tile is in the same slice
+        if (rhs != cur_rhs) {  // per right-hand side lookups, once per run of its tiles
+            skip = a.st && (a.st[rhs].flags[0] & (F_DONE | F_FROZEN));
+            const int row = a.in_row ? a.in_row[rhs] : rhs;
+            L = a.links + (long long)a.cfg[rhs] * a.link_stride;
+            if constexpr (CIN == 8)
+                inb = static_cast<const unsigned short*>(a.in) + (long long)row * a.Ls * a.V4 * ch;
+            else
+                inb = static_cast<const TIN*>(a.in) + (long long)row * a.Ls * a.V4 * ch;
+            cur_rhs = rhs;
+        }
+        T += two ? 2 : 1;
+        if (skip) continue;  // uniform per wavefront
         if (s != cur_s) {
             const bf16x4_t* wp = reinterpret_cast<const bf16x4_t*>(a.wpk) + (long long)s * NSTEP * 64 + lane;
 #pragma unroll
             for (int q = 0; q < NSTEP; ++q) B[q] = wp[q * 64];
             cur_s = s;
         }
-        const int row = a.in_row ? a.in_row[rhs] : rhs;
-        const float2* L = a.links + (long long)a.cfg[rhs] * a.link_stride;
-        const int i = i0 + m, t = i / a.Nx, x = i - t * a.Nx;
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-        if constexpr (CIN == 8) {
-            const unsigned short* inb =
-                static_cast<const unsigned short*>(a.in) + (long long)row * a.Ls * a.V4 * 16 + 4 * g;
-#pragma unroll
-            for (int p = 0; p < kPts; ++p) {
-                const Nb n = neighbour(a, L, p, s, t, x);
-                const ushort4 v = *reinterpret_cast<const ushort4*>(inb + ((long long)n.s * a.V4 + n.i) * 16);
-                const float h0r = bf2f(v.x), h0i = bf2f(v.y), h1r = bf2f(v.z), h1i = bf2f(v.w);
-                const bf16x4_t A = pack4(tr_re(n, h0r, h0i), tr_im(n, h0r, h0i), tr_re(n, h1r, h1i),
-                                         tr_im(n, h1r, h1i));
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B[p], acc, 0, 0, 0);
-            }
-        } else {
-            const TIN* inb = static_cast<const TIN*>(a.in) + (long long)row * a.Ls * a.V4 * 4;
+        const long long obase = ((long long)rhs * a.Ls + s) * a.V4 + i0;
+        const int ia = i0 + m, ta = ia / a.Nx, xa = ia - ta * a.Nx;
+        if (two) {
+            const int ib = ia + 16, tb = ib / a.Nx, xb = ib - tb * a.Nx;
+            f32x4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < NSTEP; ++q) {
-                const int p = 4 * q + g;  // this lane group's stencil point
-                bf16x4_t A = {0, 0, 0, 0};
-                if (p < kPts) {
-                    const Nb n = neighbour(a, L, p, s, t, x);
-                    const float4 v = ld_spinor(inb, (long long)n.s * a.V4 + n.i);
-                    A = pack4(tr_re(n, v.x, v.y), tr_im(n, v.x, v.y), tr_re(n, v.z, v.w), tr_im(n, v.z, v.w));
-                }
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, B[q], acc, 0, 0, 0);
+                const bf16x4_t A0 = build_A<CIN, TIN>(a, L, inb, q, s, ta, xa, g);
+                const bf16x4_t A1 = build_A<CIN, TIN>(a, L, inb, q, s, tb, xb, g);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A0, B[q], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1, B[q], acc1, 0, 0, 0);
             }
-        }
-        const long long obase = ((long long)rhs * a.Ls + s) * a.V4 + i0;
-        if constexpr (COUT == 8) {
-            unsigned short* out = static_cast<unsigned short*>(a.out) + obase * 16;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float v = acc[j];
-                const float o = __shfl_xor(v, 1, 64);
-                if ((m & 1) == 0)
-                    *reinterpret_cast<unsigned*>(out + (4 * g + j) * 16 + m) =
-                        (unsigned)f2bf(v) | ((unsigned)f2bf(o) << 16);
-            }
+            store_tile<COUT, TOUT>(a, acc0, obase, m, g);
+            store_tile<COUT, TOUT>(a, acc1, obase + 16, m, g);
         } else {
-            TOUT* out = static_cast<TOUT*>(a.out);
+            f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float v0 = __shfl(acc[j], 16 * g + 0, 64), v1 = __shfl(acc[j], 16 * g + 1, 64);
-                const float v2 = __shfl(acc[j], 16 * g + 2, 64), v3 = __shfl(acc[j], 16 * g + 3, 64);
-                if (m == j) st_spinor(out, obase + 4 * g + j, v0, v1, v2, v3);
-            }
+            for (int q = 0; q < NSTEP; ++q)
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(build_A<CIN, TIN>(a, L, inb, q, s, ta, xa, g), B[q],
+                                                                acc, 0, 0, 0);
+            store_tile<COUT, TOUT>(a, acc, obase, m, g);
         }
     }
 }
@@ -270,30 +320,39 @@ __global__ __launch_bounds__(256) void k_ptc_wgrad(const PtcArgs a, const TG* gr
     f32x4_t acc[NSTEP];
 #pragma unroll
     for (int q = 0; q < NSTEP; ++q) acc[q] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    int cur_rhs = -1, row = 0;
+    const float2* L = a.links;
     for (long long q = (long long)blockIdx.x * 4 + w; q < per_s; q += (long long)gridDim.x * 4) {
         const int rhs = (int)(q / a.tps);
         const int i0 = (int)(q - (long long)rhs * a.tps) * 16;
-        const int row = a.in_row ? a.in_row[rhs] : rhs;
-        const float2* L = a.links + (long long)a.cfg[rhs] * a.link_stride;
+        if (rhs != cur_rhs) {
+            row = a.in_row ? a.in_row[rhs] : rhs;
+            L = a.links + (long long)a.cfg[rhs] * a.link_stride;
+            cur_rhs = rhs;
+        }
         const long long gbase = ((long long)rhs * a.Ls + s) * a.V4 + i0;
         // B operand: gradient component n = m at sites 4g + j
         bf16x4_t Bv;
+        if constexpr (COUT == 8) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if constexpr (COUT == 8) {
+            for (int j = 0; j < 4; ++j)
                 Bv[j] = (short)reinterpret_cast<const unsigned short*>(grad)[(gbase + 4 * g + j) * 16 + m];
-            } else {
+        } else {
+            float gv4[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
                 float v = 0.f;
                 if (m < 4) {
                     const float4 gv = ld_spinor(grad, gbase + 4 * g + j);
                     v = m == 0 ? gv.x : m == 1 ? gv.y : m == 2 ? gv.z : gv.w;
                 }
-                Bv[j] = (short)f2bf(v);
+                gv4[j] = v;
             }
+            Bv = pack4(gv4[0], gv4[1], gv4[2], gv4[3]);
         }
 #pragma unroll
         for (int st = 0; st < NSTEP; ++st) {
-            bf16x4_t A;
+            float av[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int i = i0 + 4 * g + j, t = i / a.Nx, x = i - t * a.Nx;
@@ -315,8 +374,9 @@ __global__ __launch_bounds__(256) void k_ptc_wgrad(const PtcArgs a, const TG* gr
                         v = (comp & 1) ? tr_im(n, hr, hi) : tr_re(n, hr, hi);
                     }
                 }
-                A[j] = (short)f2bf(v);
+                av[j] = v;
             }
+            const bf16x4_t A = pack4(av[0], av[1], av[2], av[3]);
             acc[st] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A, Bv, acc[st], 0, 0, 0);
         }
     }
@@ -528,24 +588,30 @@ __global__ void k_loss_fold(const double* part, int nb, const int* row, const do
     if (threadIdx.x == 0) *out = L / npairs;
 }
 
-// ------------------------------------------------------------- flexible CG (FP64)
+// ------------------------------------------- flexible CG (FP32 + FP64 reliable updates)
 struct FcgScal {
     double rz, pq, alpha, beta;
 };
+
+template <typename R> struct V4R;
+template <> struct V4R<float> { using T = float4; };
+template <> struct V4R<double> { using T = double4; };
 
 template <typename F> __device__ __forceinline__ void for_sites(long long n, F f)
 {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) f(i);
 }
 
-__device__ __forceinline__ double dotr(const double4& a, const double4& b)
+template <typename T4> __device__ __forceinline__ double dotr(const T4& a, const T4& b)
 {
-    return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w;
+    return (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
 }
 
-// r = b - q; partials |r|^2, |b|^2
-__global__ __launch_bounds__(256) void k_fcg_init(const double* b, const double* q, double* r, long long sites,
-                                                  const CgState* st, double* part)
+constexpr int kSkip = F_DONE | F_FROZEN;
+
+// FP64: r = b - q; partials |r|^2, |b|^2
+__global__ __launch_bounds__(256) void k_fcg_resid64(const double* b, const double* q, double* r, long long sites,
+                                                     double* part)
 {
     const int rhs = blockIdx.y;
     const long long o = (long long)rhs * sites;
@@ -560,48 +626,81 @@ __global__ __launch_bounds__(256) void k_fcg_init(const double* b, const double*
         a += dotr(rr, rr);
         c += dotr(bb, bb);
     });
-    (void)st;
     block_part2(a, c, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
 }
 
-// Start of a pass: |b|^2, |r|^2; done if converged (or b = 0).
-__global__ void k_fcg_start(const double* part, int nb, CgState* st, double tol, int reset)
+// Reliable update: the FP64 true residual certifies a right-hand side or restarts its
+// FP32 cycle (first: the start of the solve).
+__global__ void k_fcg_start(const double* part, int nb, CgState* st, double tol, double delta, int first)
 {
     const int rhs = blockIdx.x;
     const double rr = fold64(part + (long long)rhs * nb * 2, nb, 0);
     const double bb = fold64(part + (long long)rhs * nb * 2, nb, 1);
     if (threadIdx.x == 0) {
         CgState& S = st[rhs];
-        S.bnorm2 = bb;
-        S.rr[0] = rr;
-        S.tol = tol;
-        S.relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
-        S.status = 0;
-        S.flags[0] = (bb == 0.0 || S.relres <= tol) ? F_DONE : 0;
-        S.flags[1] = S.flags[0];
-        if (reset) {
+        if (first) {
             S.iters = 0;
             S.reliable = 0;
             S.fp64_iters = 0;
+            S.status = 0;
+        } else if (!(S.flags[0] & F_DONE)) {
+            S.reliable += 1;
         }
+        S.bnorm2 = bb;
+        S.tol = tol;
+        S.delta = delta;
+        S.rr_ref = rr;
+        S.rr[0] = rr;
+        S.true_relres = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+        S.relres = S.true_relres;
+        const bool done = (S.status != 0) || bb == 0.0 || S.true_relres <= tol;
+        S.flags[0] = done ? F_DONE : 0;
+        S.flags[1] = S.flags[0];
     }
 }
 
-// p = z; partial (z, r)
-__global__ __launch_bounds__(256) void k_fcg_p0(const double* z, const double* r, double* p, long long sites,
-                                                const CgState* st, double* part)
+// r = (R) r64 and x = 0 for the right-hand sides still running
+template <typename R>
+__global__ __launch_bounds__(256) void k_fcg_restart(const double* r64, R* r, R* x, long long sites, const CgState* st)
 {
     const int rhs = blockIdx.y;
-    if (st[rhs].flags[0] & F_DONE) return;
+    if (st[rhs].flags[0] & kSkip) return;
+    using T4 = typename V4R<R>::T;
     const long long o = (long long)rhs * sites;
-    const double4* Z = reinterpret_cast<const double4*>(z) + o;
-    const double4* R = reinterpret_cast<const double4*>(r) + o;
-    double4* P = reinterpret_cast<double4*>(p) + o;
+    const double4* S = reinterpret_cast<const double4*>(r64) + o;
+    T4* Rr = reinterpret_cast<T4*>(r) + o;
+    T4* X = reinterpret_cast<T4*>(x) + o;
+    for_sites(sites, [&](long long i) {
+        const double4 v = S[i];
+        T4 w;
+        w.x = (R)v.x;
+        w.y = (R)v.y;
+        w.z = (R)v.z;
+        w.w = (R)v.w;
+        Rr[i] = w;
+        T4 zero;
+        zero.x = zero.y = zero.z = zero.w = (R)0;
+        X[i] = zero;
+    });
+}
+
+// p = z; partial (z, r)
+template <typename R>
+__global__ __launch_bounds__(256) void k_fcg_p0(const R* z, const R* r, R* p, long long sites, const CgState* st,
+                                                double* part)
+{
+    const int rhs = blockIdx.y;
+    if (st[rhs].flags[0] & kSkip) return;
+    using T4 = typename V4R<R>::T;
+    const long long o = (long long)rhs * sites;
+    const T4* Z = reinterpret_cast<const T4*>(z) + o;
+    const T4* Rr = reinterpret_cast<const T4*>(r) + o;
+    T4* P = reinterpret_cast<T4*>(p) + o;
     double a = 0.0;
     for_sites(sites, [&](long long i) {
-        const double4 zz = Z[i];
+        const T4 zz = Z[i];
         P[i] = zz;
-        a += dotr(zz, R[i]);
+        a += dotr(zz, Rr[i]);
     });
     block_part2(a, 0.0, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
 }
@@ -609,7 +708,7 @@ __global__ __launch_bounds__(256) void k_fcg_p0(const double* z, const double* r
 __global__ void k_fcg_rz(const double* part, int nb, CgState* st, FcgScal* fs)
 {
     const int rhs = blockIdx.x;
-    if (st[rhs].flags[0] & F_DONE) return;
+    if (st[rhs].flags[0] & kSkip) return;
     const double zr = fold64(part + (long long)rhs * nb * 2, nb, 0);
     if (threadIdx.x == 0) {
         fs[rhs].rz = zr;
@@ -624,7 +723,7 @@ __global__ void k_fcg_rz(const double* part, int nb, CgState* st, FcgScal* fs)
 __global__ void k_fcg_alpha(const double* hpart, int hnb, CgState* st, FcgScal* fs)
 {
     const int rhs = blockIdx.x;
-    if (st[rhs].flags[0] & F_DONE) return;
+    if (st[rhs].flags[0] & kSkip) return;
     const double pq = fold64(hpart + (long long)rhs * hnb * 2, hnb, 0);
     if (threadIdx.x == 0) {
         fs[rhs].pq = pq;
@@ -638,21 +737,23 @@ __global__ void k_fcg_alpha(const double* hpart, int hnb, CgState* st, FcgScal* 
 }
 
 // x += alpha p; r -= alpha q; partial |r|^2
-__global__ __launch_bounds__(256) void k_fcg_xr(double* x, double* r, const double* p, const double* q,
-                                                long long sites, const CgState* st, const FcgScal* fs, double* part)
+template <typename R>
+__global__ __launch_bounds__(256) void k_fcg_xr(R* x, R* r, const R* p, const R* q, long long sites,
+                                                const CgState* st, const FcgScal* fs, double* part)
 {
     const int rhs = blockIdx.y;
-    if (st[rhs].flags[0] & F_DONE) return;
-    const double al = fs[rhs].alpha;
+    if (st[rhs].flags[0] & kSkip) return;
+    using T4 = typename V4R<R>::T;
+    const R al = (R)fs[rhs].alpha;
     const long long o = (long long)rhs * sites;
-    double4* X = reinterpret_cast<double4*>(x) + o;
-    double4* R = reinterpret_cast<double4*>(r) + o;
-    const double4* P = reinterpret_cast<const double4*>(p) + o;
-    const double4* Q = reinterpret_cast<const double4*>(q) + o;
+    T4* X = reinterpret_cast<T4*>(x) + o;
+    T4* Rr = reinterpret_cast<T4*>(r) + o;
+    const T4* P = reinterpret_cast<const T4*>(p) + o;
+    const T4* Q = reinterpret_cast<const T4*>(q) + o;
     double a = 0.0;
     for_sites(sites, [&](long long i) {
-        const double4 pp = P[i], qq = Q[i];
-        double4 xx = X[i], rr = R[i];
+        const T4 pp = P[i], qq = Q[i];
+        T4 xx = X[i], rr = Rr[i];
         xx.x += al * pp.x;
         xx.y += al * pp.y;
         xx.z += al * pp.z;
@@ -662,40 +763,44 @@ __global__ __launch_bounds__(256) void k_fcg_xr(double* x, double* r, const doub
         rr.z -= al * qq.z;
         rr.w -= al * qq.w;
         X[i] = xx;
-        R[i] = rr;
+        Rr[i] = rr;
         a += dotr(rr, rr);
     });
     block_part2(a, 0.0, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
 }
 
+// |r|^2 of the cycle: freeze for a reliable update when it met the tolerance or fell by
+// delta since the last reliable update
 __global__ void k_fcg_conv(const double* part, int nb, CgState* st)
 {
     const int rhs = blockIdx.x;
-    if (st[rhs].flags[0] & F_DONE) return;
+    if (st[rhs].flags[0] & kSkip) return;
     const double rr = fold64(part + (long long)rhs * nb * 2, nb, 0);
     if (threadIdx.x == 0) {
         CgState& S = st[rhs];
         S.rr[0] = rr;
         S.iters += 1;
         S.relres = std::sqrt(rr / S.bnorm2);
-        if (S.relres <= S.tol) S.flags[0] |= F_DONE;
+        if (S.relres <= S.tol || rr <= S.delta * S.delta * S.rr_ref) S.flags[0] |= F_FROZEN;
     }
 }
 
 // partials (z, r), (z, q)
-__global__ __launch_bounds__(256) void k_fcg_dots(const double* z, const double* r, const double* q,
-                                                  long long sites, const CgState* st, double* part)
+template <typename R>
+__global__ __launch_bounds__(256) void k_fcg_dots(const R* z, const R* r, const R* q, long long sites,
+                                                  const CgState* st, double* part)
 {
     const int rhs = blockIdx.y;
-    if (st[rhs].flags[0] & F_DONE) return;
+    if (st[rhs].flags[0] & kSkip) return;
+    using T4 = typename V4R<R>::T;
     const long long o = (long long)rhs * sites;
-    const double4* Z = reinterpret_cast<const double4*>(z) + o;
-    const double4* R = reinterpret_cast<const double4*>(r) + o;
-    const double4* Q = reinterpret_cast<const double4*>(q) + o;
+    const T4* Z = reinterpret_cast<const T4*>(z) + o;
+    const T4* Rr = reinterpret_cast<const T4*>(r) + o;
+    const T4* Q = reinterpret_cast<const T4*>(q) + o;
     double a = 0.0, c = 0.0;
     for_sites(sites, [&](long long i) {
-        const double4 zz = Z[i];
-        a += dotr(zz, R[i]);
+        const T4 zz = Z[i];
+        a += dotr(zz, Rr[i]);
         c += dotr(zz, Q[i]);
     });
     block_part2(a, c, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
@@ -705,7 +810,7 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const double* z, const double*
 __global__ void k_fcg_beta(const double* part, int nb, CgState* st, FcgScal* fs)
 {
     const int rhs = blockIdx.x;
-    if (st[rhs].flags[0] & F_DONE) return;
+    if (st[rhs].flags[0] & kSkip) return;
     const double zr = fold64(part + (long long)rhs * nb * 2, nb, 0);
     const double zq = fold64(part + (long long)rhs * nb * 2, nb, 1);
     if (threadIdx.x == 0) {
@@ -720,35 +825,26 @@ __global__ void k_fcg_beta(const double* part, int nb, CgState* st, FcgScal* fs)
 }
 
 // p = z + beta p
-__global__ __launch_bounds__(256) void k_fcg_p(const double* z, double* p, long long sites, const CgState* st,
+template <typename R>
+__global__ __launch_bounds__(256) void k_fcg_p(const R* z, R* p, long long sites, const CgState* st,
                                                const FcgScal* fs)
 {
     const int rhs = blockIdx.y;
-    if (st[rhs].flags[0] & F_DONE) return;
-    const double be = fs[rhs].beta;
+    if (st[rhs].flags[0] & kSkip) return;
+    using T4 = typename V4R<R>::T;
+    const R be = (R)fs[rhs].beta;
     const long long o = (long long)rhs * sites;
-    const double4* Z = reinterpret_cast<const double4*>(z) + o;
-    double4* P = reinterpret_cast<double4*>(p) + o;
+    const T4* Z = reinterpret_cast<const T4*>(z) + o;
+    T4* P = reinterpret_cast<T4*>(p) + o;
     for_sites(sites, [&](long long i) {
-        const double4 zz = Z[i];
-        double4 pp = P[i];
+        const T4 zz = Z[i];
+        T4 pp = P[i];
         pp.x = zz.x + be * pp.x;
         pp.y = zz.y + be * pp.y;
         pp.z = zz.z + be * pp.z;
         pp.w = zz.w + be * pp.w;
         P[i] = pp;
     });
-}
-
-// true residual of the pass: |b - A x| / |b|
-__global__ void k_fcg_true(const double* part, int nb, CgState* st)
-{
-    const int rhs = blockIdx.x;
-    const double rr = fold64(part + (long long)rhs * nb * 2, nb, 0);
-    if (threadIdx.x == 0) {
-        CgState& S = st[rhs];
-        S.true_relres = S.bnorm2 > 0.0 ? std::sqrt(rr / S.bnorm2) : 0.0;
-    }
 }
 
 __global__ void k_iota_cfg(int* cfg, int nrhs, int rhs_per_cfg)
@@ -803,7 +899,7 @@ PtcArgs ptc_args(const dwf_ctx* c, const dwf_net* net, int l, bool adj, const La
     a.link_stride = 2LL * L.V4;
     a.wpk = adj ? net->wa[l] : net->wf[l];
     a.st = st;
-    a.tpw = (int)std::max(1LL, std::min(16LL, a.ntiles / 4096));
+    a.tpw = 2 * (int)std::max(1LL, std::min(8LL, a.ntiles / 8192));  // even: tiles go in pairs
     return a;
 }
 
@@ -1237,8 +1333,10 @@ extern "C" int dwf_ctx_set_preconditioner(dwf_ctx* c, const dwf_net* net)
     });
 }
 
-// Flexible CG on D^dag D x = b (FP64) with z = M r; passes restart from the current x
-// until the FP64 true residual meets rtol (at most 3).  Leaves states in c->st_host.
+// Flexible CG on D^dag D x = b with z = M r: FP32 cycles (stencils, vectors and the
+// network's FP32 inputs/outputs) between FP64 reliable updates, which accumulate the
+// FP32 correction into x, recompute the FP64 true residual and either certify the
+// right-hand side or restart its cycle from that residual.  Leaves states in st.
 int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, double rtol, int maxiter, int nrhs,
                   dwf_solve_stats* stats)
 {
@@ -1248,11 +1346,15 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     const long long sites = c->L.V5();
     const size_t nv = (size_t)sites * 4 * nrhs;
     hipStream_t s = c->stream;
-    double* r = (double*)c->buf(B_FCG_R, nv * 8);
-    double* p = (double*)c->buf(B_FCG_P, nv * 8);
-    double* q = (double*)c->buf(B_FCG_Q, nv * 8);
-    double* z = (double*)c->buf(B_FCG_Z, nv * 8);
-    double* t = (double*)c->buf(B_T64, nv * 8);
+    double* r64 = (double*)c->buf(B_FCG_R, nv * 8);
+    double* q64 = (double*)c->buf(B_FCG_Q, nv * 8);
+    double* t64 = (double*)c->buf(B_T64, nv * 8);
+    float* r32 = (float*)c->buf(B_FCG_R32, nv * 4);
+    float* p32 = (float*)c->buf(B_FCG_P32, nv * 4);
+    float* q32 = (float*)c->buf(B_FCG_Q32, nv * 4);
+    float* z32 = (float*)c->buf(B_FCG_Z32, nv * 4);
+    float* t32 = (float*)c->buf(B_FCG_T32, nv * 4);
+    float* x32 = (float*)c->buf(B_FCG_X32, nv * 4);
     unsigned short* h1 = (unsigned short*)c->buf(B_NET_H1, (size_t)nrhs * sites * 32);
     unsigned short* h2 = (unsigned short*)c->buf(B_NET_H2, (size_t)nrhs * sites * 32);
     const int nb = nb_vec(sites);
@@ -1260,137 +1362,116 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     CgState* st = (CgState*)c->buf(B_STATE, sizeof(CgState) * nrhs);
     FcgScal* fs = (FcgScal*)c->buf(B_FCG_SCAL, sizeof(FcgScal) * nrhs);
     const int* cfg = cfg_identity_map(c, nrhs);
-    HopArgs<double> ha = hop_args<double>(c, nrhs, nullptr);
+    HopArgs<float> ha = hop_args<float>(c, nrhs, nullptr);
     const int hnb = ha.cfg.nblk();
     double* hpart = (double*)c->buf(B_PA, (size_t)nrhs * hnb * 2 * 8);
     ha.st = st;
     ha.slot = 0;
-    ha.skip = F_DONE;
+    ha.skip = kSkip;
     const dim3 vgrid(nb, nrhs);
 
     if (!have_guess) launch_zero<double>(x, (long long)nv, s);
-    std::vector<int> total_iters(nrhs, 0);
-    int passes = 0;
     std::vector<CgState> h(nrhs);
     auto read = [&]() {
         hchk(hipMemcpyAsync(h.data(), st, sizeof(CgState) * nrhs, hipMemcpyDeviceToHost, s), "state readback");
         hchk(hipStreamSynchronize(s), "state readback");
     };
-    int pass_no = 0;
-    auto residual = [&](bool start) {  // r = b - A x (t scratch); start: states for a new pass
+    auto reliable_update = [&](bool first) {  // r64 = b - A x (FP64), certify or restart
         HopArgs<double> a = hop_args<double>(c, nrhs, nullptr);
         a.in = x;
-        a.out = t;
+        a.out = t64;
         launch_hop<double>(false, EPI_STORE, a, nrhs, s);
-        a.in = t;
-        a.out = q;
+        a.in = t64;
+        a.out = q64;
         launch_hop<double>(true, EPI_STORE, a, nrhs, s);
-        k_fcg_init<<<vgrid, 256, 0, s>>>(b, q, r, sites, st, part);
-        if (start)
-            k_fcg_start<<<nrhs, 64, 0, s>>>(part, nb, st, rtol, pass_no == 0 ? 1 : 0);
-        else
-            k_fcg_true<<<nrhs, 64, 0, s>>>(part, nb, st);
+        k_fcg_resid64<<<vgrid, 256, 0, s>>>(b, q64, r64, sites, part);
+        k_fcg_start<<<nrhs, 64, 0, s>>>(part, nb, st, rtol, c->reliable_delta, first ? 1 : 0);
     };
     auto iteration = [&]() {
-        HopArgs<double> a = ha;
-        a.in = p;
-        a.out = t;
+        HopArgs<float> a = ha;
+        a.in = p32;
+        a.out = t32;
         a.part = hpart;
-        launch_hop<double>(false, EPI_NORM, a, nrhs, s);
-        a.in = t;
-        a.out = q;
+        launch_hop<float>(false, EPI_NORM, a, nrhs, s);
+        a.in = t32;
+        a.out = q32;
         a.part = nullptr;
-        launch_hop<double>(true, EPI_STORE, a, nrhs, s);
+        launch_hop<float>(true, EPI_STORE, a, nrhs, s);
         k_fcg_alpha<<<nrhs, 64, 0, s>>>(hpart, hnb, st, fs);
-        k_fcg_xr<<<vgrid, 256, 0, s>>>(x, r, p, q, sites, st, fs, part);
+        k_fcg_xr<float><<<vgrid, 256, 0, s>>>(x32, r32, p32, q32, sites, st, fs, part);
         k_fcg_conv<<<nrhs, 64, 0, s>>>(part, nb, st);
-        net_forward(c, net, r, 0, nullptr, z, 0, cfg, nrhs, st, h1, h2);
-        k_fcg_dots<<<vgrid, 256, 0, s>>>(z, r, q, sites, st, part);
+        net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
+        k_fcg_dots<float><<<vgrid, 256, 0, s>>>(z32, r32, q32, sites, st, part);
         k_fcg_beta<<<nrhs, 64, 0, s>>>(part, nb, st, fs);
-        k_fcg_p<<<vgrid, 256, 0, s>>>(z, p, sites, st, fs);
+        k_fcg_p<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs);
     };
     constexpr int kBlockIters = 8;
     hipGraphExec_t graph = nullptr;
     if (c->use_graphs) {
-        std::vector<unsigned long long> key{0xFC6ULL,
-                                            (unsigned long long)nrhs,
-                                            (unsigned long long)(uintptr_t)x,
-                                            (unsigned long long)(uintptr_t)r,
-                                            (unsigned long long)(uintptr_t)p,
-                                            (unsigned long long)(uintptr_t)q,
-                                            (unsigned long long)(uintptr_t)z,
-                                            (unsigned long long)(uintptr_t)t,
-                                            (unsigned long long)(uintptr_t)h1,
-                                            (unsigned long long)(uintptr_t)h2,
-                                            (unsigned long long)(uintptr_t)part,
-                                            (unsigned long long)(uintptr_t)hpart,
-                                            (unsigned long long)(uintptr_t)st,
-                                            (unsigned long long)(uintptr_t)fs,
-                                            (unsigned long long)(uintptr_t)cfg,
-                                            (unsigned long long)(uintptr_t)net,
-                                            (unsigned long long)(uintptr_t)net->wf[0],
-                                            (unsigned long long)(uintptr_t)c->l64,
-                                            (unsigned long long)(uintptr_t)c->l32,
-                                            (unsigned long long)(uintptr_t)ha.rhs_per_cfg};
+        std::vector<unsigned long long> key{0xFC7ULL, (unsigned long long)nrhs};
+        for (const void* ptr : {(const void*)r32, (const void*)p32, (const void*)q32, (const void*)z32,
+                                (const void*)t32, (const void*)x32, (const void*)h1, (const void*)h2,
+                                (const void*)part, (const void*)hpart, (const void*)st, (const void*)fs,
+                                (const void*)cfg, (const void*)net, (const void*)net->wf[0], (const void*)c->l32})
+            key.push_back((unsigned long long)(uintptr_t)ptr);
+        key.push_back((unsigned long long)ha.rhs_per_cfg);
         const int v[] = {ha.cfg.bx, ha.cfg.chunks, ha.cfg.spt, ha.cfg.remap, ha.cfg.prefetch, ha.cfg.tx,
-                         ha.cfg.tiles_x, ha.cfg.nt_store};
+                         ha.cfg.tiles_x, ha.cfg.nt_store, ha.cfg.lds};
         for (int e : v) key.push_back((unsigned long long)(unsigned)e);
         graph = c->cached_graph(key, [&] {
             for (int i = 0; i < kBlockIters; ++i) iteration();
         });
     }
-    std::vector<double> true_rel(nrhs, 0.0);
-    int worst_status = 0;
-    for (pass_no = 0; pass_no < 3; ++pass_no) {
-        passes = pass_no + 1;
-        residual(true);
-        net_forward(c, net, r, 0, nullptr, z, 0, cfg, nrhs, st, h1, h2);
-        k_fcg_p0<<<vgrid, 256, 0, s>>>(z, r, p, sites, st, part);
+    auto max_iters = [&]() {
+        int m = 0;
+        for (const CgState& S : h) m = std::max(m, S.iters);
+        return m;
+    };
+    auto running = [&]() {
+        for (const CgState& S : h)
+            if (!(S.flags[0] & kSkip)) return true;
+        return false;
+    };
+    reliable_update(true);
+    read();
+    int cycles = 0;
+    while (true) {
+        bool all_done = true;
+        for (const CgState& S : h)
+            if (!(S.flags[0] & F_DONE)) all_done = false;
+        if (all_done || max_iters() >= maxiter || cycles >= 1000) break;
+        ++cycles;
+        k_fcg_restart<float><<<vgrid, 256, 0, s>>>(r64, r32, x32, sites, st);
+        net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
+        k_fcg_p0<float><<<vgrid, 256, 0, s>>>(z32, r32, p32, sites, st, part);
         k_fcg_rz<<<nrhs, 64, 0, s>>>(part, nb, st, fs);
-        c->check_launch("FCG start");
+        c->check_launch("FCG cycle start");
         read();
-        // iterations of this pass, counted from zero in the states
-        std::vector<int> base(nrhs);
-        for (int i = 0; i < nrhs; ++i) base[i] = h[i].iters;
-        int it = 0;
-        auto all_done = [&]() {
-            for (int i = 0; i < nrhs; ++i)
-                if (!(h[i].flags[0] & F_DONE)) return false;
-            return true;
-        };
-        const int budget = std::max(0, maxiter - *std::max_element(total_iters.begin(), total_iters.end()));
-        while (!all_done() && it < budget) {
-            const int n = std::min(32, budget - it);
+        while (running() && max_iters() < maxiter) {
+            const int n = std::min(32, maxiter - max_iters());
             int j = 0;
             if (graph)
                 for (; j + kBlockIters <= n; j += kBlockIters) hchk(hipGraphLaunch(graph, s), "hipGraphLaunch");
             for (; j < n; ++j) iteration();
-            it += n;
             c->check_launch("FCG iteration");
             read();
         }
-        for (int i = 0; i < nrhs; ++i) total_iters[i] += h[i].iters - base[i];
-        residual(false);
-        c->check_launch("FCG true residual");
+        launch_accum_x((long long)sites * 4, x, x32, st, 0, nrhs, s);  // x += x32 (all not done)
+        reliable_update(false);
+        c->check_launch("FCG reliable update");
         read();
-        bool ok = true;
-        for (int i = 0; i < nrhs; ++i) {
-            true_rel[i] = h[i].true_relres;
-            worst_status = std::max(worst_status, h[i].status);
-            if (h[i].true_relres > rtol) ok = false;
-        }
-        if (ok || worst_status != 0 || !all_done()) break;
     }
     int rc = DWF_OK;
     for (int i = 0; i < nrhs; ++i) {
         dwf_solve_stats& S = stats[i];
         S = dwf_solve_stats{};
-        S.iterations = total_iters[i];
-        S.fp64_iterations = total_iters[i];
-        S.passes = passes;
+        S.iterations = h[i].iters;
+        S.fp64_iterations = 0;
+        S.reliable_updates = h[i].reliable;
+        S.passes = 1;
         S.relres = h[i].relres;
-        S.true_relres = true_rel[i];
-        S.converged = (h[i].status == 0 && true_rel[i] <= rtol) ? 1 : 0;
+        S.true_relres = h[i].true_relres;
+        S.converged = (h[i].status == 0 && (h[i].flags[0] & F_DONE) && h[i].true_relres <= rtol) ? 1 : 0;
         if (!S.converged && rc == DWF_OK) rc = h[i].status ? DWF_ERR_ZERO_DENOM : DWF_ERR_NOT_CONVERGED;
     }
     if (rc == DWF_ERR_ZERO_DENOM)

@@ -473,9 +473,10 @@ int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, c
 
 namespace {
 
-// D^dag D x = b by flexible CG with the learned preconditioner.  If it breaks down (the
-// network is not positive on some residual) or misses rtol, plain FP64 CG continues from
-// its iterate and decides; the stats then count both.
+// D^dag D x = b by flexible CG with the learned preconditioner (FP32 cycles, FP64
+// reliable updates).  If it breaks down (the network is not positive on some residual)
+// or misses rtol, plain FP64 CG continues from its iterate and decides; the stats then
+// count both (fp64_iterations = the fallback's iterations, 0 without fallback).
 int solve_learned(dwf_ctx* c, const double* b, const double* x0, double rtol, int maxiter, int nrhs, double* x,
                   dwf_solve_stats* stats)
 {
@@ -487,9 +488,10 @@ int solve_learned(dwf_ctx* c, const double* b, const double* x0, double rtol, in
         dwf_solve_device_impl(c, DWF_OP_DDAGD, DWF_SOLVE_CG64, b, x, rtol, maxiter, nrhs, x);
         std::vector<dwf_solve_stats> second(nrhs);
         rc = collect(c, DWF_SOLVE_CG64, nrhs, second.data());
-        for (int i = 0; i < nrhs; ++i) {
-            second[i].fp64_iterations += first[i].iterations;
+        for (int i = 0; i < nrhs; ++i) {  // iterations: the FCG ones; fp64_iterations: the fallback's
+            second[i].fp64_iterations = second[i].iterations;
             second[i].iterations = first[i].iterations;
+            second[i].reliable_updates = first[i].reliable_updates;
             second[i].passes = first[i].passes;
         }
         first = second;
