@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the DWF fermion hot path on one MI355X.
 
-Workload (one "step"): solve D^dag D x = D^dag b for a batch of boundary sources on a
+Workload (one "step"): solve D^dag D x = D^dag b for a batch of 32 boundary sources on a
 96x96 lattice with Ls=16, M5=1.0, m_f=0.05 and links theta ~ N(0, 1/beta), beta=4, to an
 FP64 true residual ||D^dag b - D^dag D x|| / ||D^dag b|| <= 1e-10.  Sources are complex
 Gaussian eta(t,x,spin) placed as P_L eta at s=0 and P_R eta at s=Ls-1, as
@@ -26,7 +26,7 @@ With --workload learned: the learned preconditioner at 96x96, Ls=16.  Training p
 (source, solution) come from --learn-configs seeded configurations (seeds 1234 + 104729 i)
 with --learn-pairs full-volume Gaussian sources each, solved to 1e-10; the network is
 trained on the device (Adam, --learn-steps steps of --learn-batch pairs); a step is then
-one batch of 16 boundary-source solves on a held-out configuration by flexible CG with
+one batch of --nrhs (32) boundary-source solves on a held-out configuration by flexible CG with
 the network, beside plain FP64 CG and the even-odd mixed solver on the same batch.
 
 With --workload hmc a step is one Nf=2 domain-wall HMC trajectory at 96x96, Ls=16,
@@ -353,7 +353,7 @@ def run_learned(args, dwf, capi):
         "config": {
             "workload": f"learned preconditioner: trained on {npairs} (source, solution) pairs from {ntrain} "
                         "configurations (full-volume Gaussian sources), tested on a held-out configuration, "
-                        "16 boundary sources per step, FP64 true residual <= 1e-10",
+                        f"{nrhs} boundary sources per step, FP64 true residual <= 1e-10",
             "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs,
             "network": "3 linear parallel-transport conv layers 2-8-8-2 complex channels, BF16 MFMA",
             "solver": "flexible CG (FCG(1)) in FP32 with FP64 reliable updates, certified at 1e-10",
@@ -452,7 +452,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--nrhs", type=int, default=16, help="right-hand sides per step")
+    ap.add_argument("--nrhs", type=int, default=32, help="right-hand sides per step")
     ap.add_argument("--dslash-rhs", type=int, default=128)
     ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
