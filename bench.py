@@ -307,7 +307,7 @@ def run_learned(args, dwf, capi):
         d.free()
     ctx.close()
 
-    # held-out configuration: 16 boundary sources per step
+    # held-out configuration: --nrhs boundary sources per step
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
     ctx.set_gauge(thetas[ntrain])
     if args.no_graphs:
