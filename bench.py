@@ -273,6 +273,8 @@ def run_learned(args, dwf, capi):
     thetas = np.stack([np.random.default_rng(1234 + 104729 * i).normal(0.0, 1.0 / np.sqrt(BETA), 2 * NT * NX)
                        for i in range(ntrain + 1)])
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
+    if args.no_graphs:
+        ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
     ctx.set_gauge_batch(thetas[:ntrain])
     # pairs: chunk j holds pair j of every configuration (right-hand side c uses config c)
     d_b32 = capi.DeviceArray(npairs * V5 * 16)
@@ -312,8 +314,10 @@ def run_learned(args, dwf, capi):
     ctx.set_gauge(thetas[ntrain])
     if args.no_graphs:
         ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
-    ctx.set_preconditioner(net)
     nrhs = args.nrhs
+    tuned = ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)
+    log(f"autotuned stencil shape for {nrhs} RHS: {tuned}")
+    ctx.set_preconditioner(net)
     src = boundary_sources(nrhs, 918273 + 1)
     d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
     d_x = capi.DeviceArray(nrhs * V5 * 32)

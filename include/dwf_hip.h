@@ -111,7 +111,9 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * kernels one by one (same kernels, same order, same results); DWF_TUNE_LDS: 0, or 1 / 2
  * to stage each slice of a 2D stencil tile of 256 / 512 sites and its halo in LDS
  * (with DWF_TUNE_TILE_X in {8, 16, 32, 64} dividing Nx and sites/width dividing Nt;
- * ignored otherwise). */
+ * ignored otherwise); DWF_TUNE_NET_FUSED: 0/1 (default 1) run the learned
+ * preconditioner's three layers as one fused kernel (LDS-resident hidden layers) instead
+ * of layer by layer; both give bit-identical results. */
 enum dwf_tune_param {
     DWF_TUNE_SCHUNKS = 0,
     DWF_TUNE_XCD_REMAP = 1,
@@ -120,7 +122,8 @@ enum dwf_tune_param {
     DWF_TUNE_TILE_X = 4,
     DWF_TUNE_NT_STORE = 5,
     DWF_TUNE_GRAPHS = 6,
-    DWF_TUNE_LDS = 7
+    DWF_TUNE_LDS = 7,
+    DWF_TUNE_NET_FUSED = 8
 };
 int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 

@@ -26,10 +26,10 @@
 // Weights: per layer [Ls][7][C_out][C_in] complex, layers concatenated, (re, im) pairs.
 //
 // Quantized mode (bf16 = 1) rounds at exactly the points the device kernels round:
-// inputs to float, links to float, the transported field (float fma form) to bf16, the
-// weights to bf16, stored hidden activations and gradients to bf16, with FP64
-// accumulation instead of the device's FP32 -- so device and oracle agree to
-// accumulation-order error.
+// layer inputs to float and then bf16, links to float and then bf16, the transported
+// field (exact bf16 products, one rounding of their sum to float) to bf16, the weights to
+// bf16, stored hidden activations and gradients to bf16, with FP64 accumulation instead
+// of the device's FP32 -- so device and oracle agree to accumulation-order error.
 
 #include <cmath>
 #include <complex>
@@ -133,12 +133,15 @@ Hop hop(const Geo& g, const double* th, int p, int s, int t, int x)
     return h;
 }
 
-// Transported value U v as the device computes it: float fma form, then bf16.
+// Transported value U v as the device computes it with bf16 dot products: v rounded to
+// float then bf16, the transporter's float (cos, sin) rounded to bf16, the two products of
+// each component exact, their sum rounded once to float, then to bf16.
 cplx transport_q(const Hop& h, cplx v)
 {
-    const float hr = (float)v.real(), hi = (float)v.imag();
-    const float ar = std::fma(h.cf, hr, -(h.sf * hi));
-    const float ai = std::fma(h.sf, hr, h.cf * hi);
+    const double hr = bf16_round((float)v.real()), hi = bf16_round((float)v.imag());
+    const double c = bf16_round(h.cf), sn = bf16_round(h.sf);
+    const float ar = (float)(c * hr - sn * hi);
+    const float ai = (float)(sn * hr + c * hi);
     return cplx(bf16_round(ar), bf16_round(ai));
 }
 

@@ -58,6 +58,29 @@ def test_net_apply_matches_oracle(Nt, Nx, Ls):
     assert abs(lhs - rhs) <= 1e-2 * np.linalg.norm(u) * np.linalg.norm(z[0])
 
 
+@pytest.mark.parametrize("Nt,Nx,Ls", [(8, 6, 4), (16, 12, 8), (12, 20, 3), (16, 32, 5)])
+def test_fused_network_matches_layers(Nt, Nx, Ls):
+    """The fused three-layer kernel (LDS-resident hidden layers, output tiles of up to
+    8 x 16 sites, wrapped fifth dimension) against the layer-by-layer kernels: the same
+    bf16 operands, transports and MFMA order, so bit-identical; and against the oracle."""
+    print_case(test="fused_network", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(3 * Nt + Nx + Ls)
+    th = rand_theta(rng, Nt, Nx)
+    w = weights32(rng, Ls)
+    v = rand_spinor(rng, 3 * Ls * Nt * Nx * 2).reshape(3, -1)
+    net = dwf.Network(Ls, seed=1)
+    net.set_weights(w)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        zf = net.apply(ctx, v)
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, 0)
+        zl = net.apply(ctx, v)
+    assert np.array_equal(zf, zl)
+    zq = oracle.net_apply(th, w.astype(np.float64), v, Nt, Nx, Ls, bf16=True)
+    for r in range(3):
+        assert rel(zf[r], zq[r]) <= 1e-3, rel(zf[r], zq[r])
+
+
 def test_net_apply_per_configuration():
     Nt, Nx, Ls = 8, 6, 4
     print_case(test="net_apply_batch")
@@ -140,6 +163,31 @@ def test_learned_solve_certified_untrained():
         lhs = oracle.dwf_apply(1, th, src[r] - Dx, Nt, Nx, Ls, M5, MF)
         rhs = oracle.dwf_apply(1, th, src[r], Nt, Nx, Ls, M5, MF)
         assert np.linalg.norm(lhs) / np.linalg.norm(rhs) <= 1.01e-10
+
+
+def test_learned_solve_fused_and_layers():
+    """FCG with the fused network (dot products in its epilogue) and with the layer
+    kernels (separate dot-product pass): both certified, iteration counts within 2
+    (the two sum the dot products in different orders)."""
+    Nt, Nx, Ls = 16, 16, 8
+    print_case(test="learned_solve_fused", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(12)
+    th = rand_theta(rng, Nt, Nx)
+    src = boundary_sources(rng, 3, Nt, Nx, Ls)
+    net = dwf.Network(Ls, seed=3)
+    res = []
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.set_preconditioner(net)
+        for fused in (1, 0):
+            ctx.tune(dwf.DWF_TUNE_NET_FUSED, fused)
+            res.append(ctx.solve_dirac(src, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_LEARNED))
+    for (x, st, _) in res:
+        for r in range(3):
+            assert st[r]["converged"] and st[r]["true_relres"] <= 1e-10 and st[r]["fp64_iterations"] == 0
+    for r in range(3):
+        assert abs(res[0][1][r]["iterations"] - res[1][1][r]["iterations"]) <= 2
+        assert rel(res[0][0][r], res[1][0][r]) <= 1e-8
 
 
 def volume_sources(rng, n, Nt, Nx, Ls):

@@ -21,6 +21,7 @@ from .capi import (  # noqa: F401
     DWF_TUNE_NT_STORE,
     DWF_TUNE_GRAPHS,
     DWF_TUNE_LDS,
+    DWF_TUNE_NET_FUSED,
     DWF_TUNE_PREFETCH,
     DWF_TUNE_RELIABLE_DELTA,
     DWF_TUNE_SCHUNKS,

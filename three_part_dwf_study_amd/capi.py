@@ -47,6 +47,7 @@ DWF_TUNE_TILE_X = 4
 DWF_TUNE_NT_STORE = 5
 DWF_TUNE_GRAPHS = 6
 DWF_TUNE_LDS = 7
+DWF_TUNE_NET_FUSED = 8
 
 
 class DwfError(RuntimeError):

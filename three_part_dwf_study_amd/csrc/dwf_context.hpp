@@ -89,6 +89,7 @@ enum BufId : int {
     B_FCG_Z32,
     B_FCG_T32,
     B_FCG_X32,
+    B_FCG_NPART,
     B_COUNT
 };
 
@@ -113,6 +114,7 @@ struct dwf_ctx {
     int tile_x = 0;               // stencil 2D tile width (0: linear blocks)
     int nt_store = 0;             // streaming stores of stencil outputs
     int lds = 0;                  // LDS-staged stencil tiles
+    int net_fused = 1;            // learned preconditioner: fused three-layer kernel when the shape allows
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs
     const dwf_net* net = nullptr; // learned preconditioner of DWF_SOLVE_LEARNED (not owned)

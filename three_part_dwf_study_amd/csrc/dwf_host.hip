@@ -327,6 +327,9 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             if (!(value == 0.0 || value == 1.0 || value == 2.0)) break;
             ctx->lds = (int)value;
             return DWF_OK;
+        case DWF_TUNE_NET_FUSED:
+            ctx->net_fused = value != 0.0 ? 1 : 0;
+            return DWF_OK;
         default:
             break;
     }

@@ -155,9 +155,22 @@ __device__ __forceinline__ Nb neighbour(const PtcArgs& a, const float2* L, int p
     return r;
 }
 
-// (hr + i hi)(c + i sn) in the explicit fma form the oracle mirrors
-__device__ __forceinline__ float tr_re(const Nb& n, float hr, float hi) { return fmaf(n.c, hr, -(n.sn * hi)); }
-__device__ __forceinline__ float tr_im(const Nb& n, float hr, float hi) { return fmaf(n.sn, hr, n.c * hi); }
+// Transport in bf16 dot products (v_dot2_f32_bf16): the field value h = (hr, hi) and the
+// transporter u = (c, sn) are bf16, the two products are exact in FP32 and their sum is
+// rounded once.  lk.x = (c, -sn), lk.y = (sn, c) as bf16 pairs (low half first), so
+// Re(u h) = dot2(h, lk.x) and Im(u h) = dot2(h, lk.y).  A point without a link has
+// u = 1, for which both products are exact: the value passes through unchanged.
+__device__ __forceinline__ uint2 link_ops(const Nb& n)
+{
+    return make_uint2(pk_bf16(n.c, -n.sn), pk_bf16(n.sn, n.c));
+}
+__device__ __forceinline__ float dot2(unsigned h, unsigned l)
+{
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2_hw, h), __builtin_bit_cast(bf16x2_hw, l), 0.0f,
+                                           false);
+}
+// u h for one complex channel (bf16 pair in, bf16 pair out)
+__device__ __forceinline__ unsigned transport(unsigned h, const uint2& lk) { return pk_bf16(dot2(h, lk.x), dot2(h, lk.y)); }
 
 __device__ __forceinline__ float4 ld_spinor(const double* p, long long site)
 {
@@ -189,51 +202,58 @@ __device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d)
 }
 
 // ------------------------------------------------------------------ forward layer
-// A operand of MFMA step q for the lane's site (t, x) of slice s: the transported
+// Each MFMA computes D' = W^T H^T for one 16-site tile: the A operand is the weight
+// fragment (lane (m, g) holds W[in 4g + j][out m], j = 0..3), the B operand the
+// transported input (lane (m, g) holds input components 4g .. 4g + 3 of site m), and the
+// result leaves lane (m, g) with output components 4g .. 4g + 3 of site m.
+
+// bf16 pairs (re, im) of two complex values
+__device__ __forceinline__ bf16x4_t pack_pairs(unsigned lo, unsigned hi)
+{
+    bf16x4_t v;
+    v[0] = (short)(lo & 0xffffu);
+    v[1] = (short)(lo >> 16);
+    v[2] = (short)(hi & 0xffffu);
+    v[3] = (short)(hi >> 16);
+    return v;
+}
+
+// B operand of MFMA step q for the lane's site (t, x) of slice s: the transported
 // neighbour channels of stencil point q (C_in = 8: channels 2g, 2g+1) or of point
-// 4q + g (C_in = 2: both channels), rounded to bf16.
+// 4q + g (C_in = 2: both channels), in bf16.
 template <int CIN, typename TIN>
-__device__ __forceinline__ bf16x4_t build_A(const PtcArgs& a, const float2* L, const void* inb, int q, int s, int t,
+__device__ __forceinline__ bf16x4_t build_B(const PtcArgs& a, const float2* L, const void* inb, int q, int s, int t,
                                             int x, int g)
 {
     if constexpr (CIN == 8) {
         const Nb n = neighbour(a, L, q, s, t, x);
-        const ushort4 v = *reinterpret_cast<const ushort4*>(static_cast<const unsigned short*>(inb) +
-                                                            ((long long)n.s * a.V4 + n.i) * 16 + 4 * g);
-        const float h0r = bf2f(v.x), h0i = bf2f(v.y), h1r = bf2f(v.z), h1i = bf2f(v.w);
-        return pack4(tr_re(n, h0r, h0i), tr_im(n, h0r, h0i), tr_re(n, h1r, h1i), tr_im(n, h1r, h1i));
+        const uint2 v = *reinterpret_cast<const uint2*>(static_cast<const unsigned short*>(inb) +
+                                                        ((long long)n.s * a.V4 + n.i) * 16 + 4 * g);
+        if (q == 0 || q >= 5) return pack_pairs(v.x, v.y);  // no link: the stored values
+        const uint2 lk = link_ops(n);
+        return pack_pairs(transport(v.x, lk), transport(v.y, lk));
     } else {
         const int p = 4 * q + g;  // this lane group's stencil point
-        bf16x4_t A = {0, 0, 0, 0};
+        bf16x4_t B = {0, 0, 0, 0};
         if (p < kPts) {
             const Nb n = neighbour(a, L, p, s, t, x);
             const float4 v = ld_spinor(static_cast<const TIN*>(inb), (long long)n.s * a.V4 + n.i);
-            A = pack4(tr_re(n, v.x, v.y), tr_im(n, v.x, v.y), tr_re(n, v.z, v.w), tr_im(n, v.z, v.w));
+            const uint2 lk = link_ops(n);
+            B = pack_pairs(transport(pk_bf16(v.x, v.y), lk), transport(pk_bf16(v.z, v.w), lk));
         }
-        return A;
+        return B;
     }
 }
 
-// D[4g + j][m] of one 16-site tile at obase (sites obase .. obase + 15)
+// Output components 4g .. 4g + 3 of site m of one 16-site tile at obase.
 template <int COUT, typename TOUT>
 __device__ __forceinline__ void store_tile(const PtcArgs& a, const f32x4_t& acc, long long obase, int m, int g)
 {
     if constexpr (COUT == 8) {
-        unsigned short* out = static_cast<unsigned short*>(a.out) + obase * 16;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float v = acc[j];
-            const float o = __shfl_xor(v, 1, 64);
-            if ((m & 1) == 0) *reinterpret_cast<unsigned*>(out + (4 * g + j) * 16 + m) = pk_bf16(v, o);
-        }
+        unsigned short* out = static_cast<unsigned short*>(a.out) + (obase + m) * 16 + 4 * g;
+        *reinterpret_cast<uint2*>(out) = make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
     } else {
-        TOUT* out = static_cast<TOUT*>(a.out);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const float v0 = __shfl(acc[j], 16 * g + 0, 64), v1 = __shfl(acc[j], 16 * g + 1, 64);
-            const float v2 = __shfl(acc[j], 16 * g + 2, 64), v3 = __shfl(acc[j], 16 * g + 3, 64);
-            if (m == j) st_spinor(out, obase + 4 * g + j, v0, v1, v2, v3);
-        }
+        if (g == 0) st_spinor(static_cast<TOUT*>(a.out), obase + m, acc[0], acc[1], acc[2], acc[3]);
     }
 }
 
@@ -254,7 +274,7 @@ __global__ __launch_bounds__(256) void k_ptc_fwd(const PtcArgs a)
     bool skip = false;
     const float2* L = a.links;
     const void* inb = a.in;
-    bf16x4_t B[NSTEP];
+    bf16x4_t W[NSTEP];
     for (long long T = T0; T < T1;) {
         const int rhs = (int)(T / per_rhs);
         const int rem = (int)(T - rhs * per_rhs);
@@ -277,7 +297,7 @@ __global__ __launch_bounds__(256) void k_ptc_fwd(const PtcArgs a)
         if (s != cur_s) {
             const bf16x4_t* wp = reinterpret_cast<const bf16x4_t*>(a.wpk) + (long long)s * NSTEP * 64 + lane;
 #pragma unroll
-            for (int q = 0; q < NSTEP; ++q) B[q] = wp[q * 64];
+            for (int q = 0; q < NSTEP; ++q) W[q] = wp[q * 64];
             cur_s = s;
         }
         const long long obase = ((long long)rhs * a.Ls + s) * a.V4 + i0;
@@ -287,10 +307,10 @@ __global__ __launch_bounds__(256) void k_ptc_fwd(const PtcArgs a)
             f32x4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < NSTEP; ++q) {
-                const bf16x4_t A0 = build_A<CIN, TIN>(a, L, inb, q, s, ta, xa, g);
-                const bf16x4_t A1 = build_A<CIN, TIN>(a, L, inb, q, s, tb, xb, g);
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A0, B[q], acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1, B[q], acc1, 0, 0, 0);
+                const bf16x4_t H0 = build_B<CIN, TIN>(a, L, inb, q, s, ta, xa, g);
+                const bf16x4_t H1 = build_B<CIN, TIN>(a, L, inb, q, s, tb, xb, g);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[q], H0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[q], H1, acc1, 0, 0, 0);
             }
             store_tile<COUT, TOUT>(a, acc0, obase, m, g);
             store_tile<COUT, TOUT>(a, acc1, obase + 16, m, g);
@@ -298,7 +318,7 @@ __global__ __launch_bounds__(256) void k_ptc_fwd(const PtcArgs a)
             f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < NSTEP; ++q)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(build_A<CIN, TIN>(a, L, inb, q, s, ta, xa, g), B[q],
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[q], build_B<CIN, TIN>(a, L, inb, q, s, ta, xa, g),
                                                                 acc, 0, 0, 0);
             store_tile<COUT, TOUT>(a, acc, obase, m, g);
         }
@@ -359,19 +379,20 @@ __global__ __launch_bounds__(256) void k_ptc_wgrad(const PtcArgs a, const TG* gr
                 float v = 0.f;
                 if constexpr (CIN == 8) {
                     const Nb n = neighbour(a, L, st, s, t, x);
-                    const unsigned short* hp = static_cast<const unsigned short*>(a.in) +
-                                               (((long long)row * a.Ls + n.s) * a.V4 + n.i) * 16 + 2 * (m >> 1);
-                    const ushort2 h = *reinterpret_cast<const ushort2*>(hp);
-                    const float hr = bf2f(h.x), hi = bf2f(h.y);
-                    v = (m & 1) ? tr_im(n, hr, hi) : tr_re(n, hr, hi);
+                    const unsigned* hp = reinterpret_cast<const unsigned*>(
+                        static_cast<const unsigned short*>(a.in) + (((long long)row * a.Ls + n.s) * a.V4 + n.i) * 16) +
+                        (m >> 1);
+                    const uint2 lk = link_ops(n);
+                    v = dot2(*hp, (m & 1) ? lk.y : lk.x);  // rounded to bf16 by pack4
                 } else {
                     const int p = 4 * st + (m >> 2), comp = m & 3;
                     if (p < kPts) {
                         const Nb n = neighbour(a, L, p, s, t, x);
                         const float4 h = ld_spinor(static_cast<const TIN*>(a.in) + (long long)row * a.Ls * a.V4 * 4,
                                                    (long long)n.s * a.V4 + n.i);
-                        const float hr = comp < 2 ? h.x : h.z, hi = comp < 2 ? h.y : h.w;
-                        v = (comp & 1) ? tr_im(n, hr, hi) : tr_re(n, hr, hi);
+                        const unsigned hb = comp < 2 ? pk_bf16(h.x, h.y) : pk_bf16(h.z, h.w);
+                        const uint2 lk = link_ops(n);
+                        v = dot2(hb, (comp & 1) ? lk.y : lk.x);
                     }
                 }
                 av[j] = v;
@@ -507,6 +528,332 @@ __device__ __forceinline__ double fold64(const double* p, int n, int which)
     double v = 0.0;
     for (int i = threadIdx.x; i < n; i += 64) v += p[2 * i + which];
     return wsum(v);
+}
+
+// ------------------------------------------------------- fused network forward
+// z = M r in one launch: the three layers of one output tile of TY x TX sites of one
+// right-hand side, walking the fifth dimension.  A block stages, for the slices it
+// needs, the tile's input with a 3-site halo (bf16), the first hidden layer with a
+// 2-site halo and the second with a 1-site halo in LDS rings of 4 slices, so HBM sees
+// the input once (plus the halo and the wrapped slices, mostly from L2) and the output
+// once instead of every layer's activations.  The +-s hops are periodic, so the walk
+// covers the extended range: input slices -3 .. Ls+2, layer 1 at -2 .. Ls+1, layer 2
+// at -1 .. Ls, output at 0 .. Ls-1 (extended slices are recomputed from the same data
+// and weights as their periodic images, so the result equals the layer-by-layer one).
+//
+// Step j (one barrier each): every wave loads its layers' weight fragments for the
+// step, then input slice j-2 into registers (written to LDS at the end of step j+1, so
+// its latency spans two steps); layer 1 runs at slice j-5, layer 2 at j-7, the output
+// at j-9.  Waves have fixed roles and fixed 16-site tiles (no loop over tiles, so the
+// compiler waits for exactly the weights before the first MFMA and never for the
+// input in flight): waves 0-3 run layer 2 (3 tiles each), waves 4-7 the output layer
+// (2 tiles each) and layer 1 (4 tiles each).  The transport operands of a wave's tiles
+// and, for layers 2 and 3, the fifth-dimension window (the s-1 and s values at the
+// tile's sites) stay in registers across steps.  The arithmetic (bf16 operands, dot2
+// transport, MFMA order over stencil points) is that of k_ptc_fwd, so both give
+// bit-identical results.  Optionally the flexible-CG dot products (z, r) and (z, q) per
+// block, in a pass over the block's output after the walk.
+constexpr int kFuTy = 8, kFuTx = 16;  // largest output tile
+constexpr int kFuR0 = (kFuTy + 6) * (kFuTx + 6), kFuR1 = (kFuTy + 4) * (kFuTx + 4), kFuR2 = (kFuTy + 2) * (kFuTx + 2);
+constexpr int kFuThreads = 512, kFuWaves = kFuThreads / 64;
+constexpr int kFuL2Tiles = 3, kFuL3Tiles = 2, kFuL1Tiles = 4;  // tiles per wave of its role
+static_assert(4 * kFuL2Tiles * 16 >= kFuR2 && 4 * kFuL3Tiles * 16 >= kFuTy * kFuTx && 4 * kFuL1Tiles * 16 >= kFuR1,
+              "the waves' tiles must cover every region of the largest output tile");
+
+struct NetFusedArgs {
+    int Nt, Nx, Ls, V4;
+    int TY, TX;            // output tile, dividing Nt and Nx
+    int tiles_x, tiles;    // tiles along x, tiles per lattice
+    const void* in;        // [nrhs][Ls][V4] spinors (TIN)
+    void* out;             // [nrhs][Ls][V4] spinors (TOUT)
+    const float2* links;   // [cfg][2][V4] (cos, sin)
+    long long link_stride;
+    const int* cfg;        // right-hand side -> configuration
+    const unsigned short* wf[3];
+    const CgState* st;     // nullable: skip right-hand sides with flags[0] & (F_DONE | F_FROZEN)
+    const float* q;        // DOTS: q = A p of the flexible CG
+    double* part;          // DOTS: [nrhs][tiles][2] partials of (z, r), (z, q)
+};
+
+// i = k / w for 0 <= k < 2^16 / w (w <= 22 here): one multiply and a shift
+__device__ __forceinline__ int div_small(int k, unsigned inv) { return (int)(((unsigned)k * inv) >> 16); }
+
+// Transport operands of hop p (1 +t, 2 -t, 3 +x, 4 -x; anything else: none) at lattice
+// site (t, x): the Dirac operator's transporters with the antiperiodic time boundary.
+__device__ __forceinline__ uint2 hop_link(const float2* L, int Nt, int Nx, int V4, int p, int t, int x)
+{
+    float c = 1.f, sn = 0.f;
+    if (p == 1) {
+        const float2 u = L[t * Nx + x];
+        const float sg = t == Nt - 1 ? -1.f : 1.f;
+        c = sg * u.x;
+        sn = sg * u.y;
+    } else if (p == 2) {
+        const int tm = t == 0 ? Nt - 1 : t - 1;
+        const float2 u = L[tm * Nx + x];
+        const float sg = t == 0 ? -1.f : 1.f;
+        c = sg * u.x;
+        sn = -sg * u.y;
+    } else if (p == 3) {
+        const float2 u = L[V4 + t * Nx + x];
+        c = u.x;
+        sn = u.y;
+    } else if (p == 4) {
+        const float2 u = L[V4 + t * Nx + (x == 0 ? Nx - 1 : x - 1)];
+        c = u.x;
+        sn = -u.y;
+    }
+    return link_ops(Nb{0, 0, c, sn});
+}
+
+template <typename TIN, typename TOUT, bool DOTS>
+__global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_net_fused(const NetFusedArgs a)
+{
+    __shared__ uint2 ring0[4][kFuR0];     // input, bf16 (comp 0, comp 1)
+    __shared__ uint2 ring1[4][kFuR1][4];  // layer 1 output, 8 channels bf16
+    __shared__ uint2 ring2[4][kFuR2][4];  // layer 2 output
+    __shared__ uint4 wrec[2][kFuThreads]; // weight fragments of a step: layer 1 (2), 2 (7), 3 (7) x 64 lanes
+    __shared__ double red[2 * kFuWaves];
+
+    const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+    const int rhs = lb / a.tiles, tile = lb - rhs * a.tiles;
+    if (a.st && (a.st[rhs].flags[0] & (F_DONE | F_FROZEN))) return;  // block-uniform
+    const int TY = a.TY, TX = a.TX;
+    const int W0 = TX + 6, W1 = TX + 4, W2 = TX + 2;
+    const int n0 = (TY + 6) * W0, n1 = (TY + 4) * W1, n2 = (TY + 2) * W2, n3 = TY * TX;
+    const unsigned inv0 = (65535u + W0) / W0, inv1 = (65535u + W1) / W1, inv2 = (65535u + W2) / W2,
+                   inv3 = (65535u + TX) / TX;
+    const int ty = tile / a.tiles_x, tx = tile - ty * a.tiles_x;
+    const int t0 = ty * TY, x0 = tx * TX;
+    const int Nt = a.Nt, Nx = a.Nx, Ls = a.Ls, V4 = a.V4;
+    auto wrap_t = [Nt](int t) { return t < 0 ? t + Nt : (t >= Nt ? t - Nt : t); };
+    auto wrap_x = [Nx](int x) { return x < 0 ? x + Nx : (x >= Nx ? x - Nx : x); };
+    const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: scalar control
+    const float2* L = a.links + (long long)a.cfg[rhs] * a.link_stride;
+
+    // this thread's input site of R0 (origin t0-3, x0-3)
+    int gsite = -1;
+    if (tid < n0) {
+        const int i = div_small(tid, inv0), jx = tid - i * W0;
+        gsite = wrap_t(t0 - 3 + i) * Nx + wrap_x(x0 - 3 + jx);
+    }
+    const TIN* inr = static_cast<const TIN*>(a.in) + (long long)rhs * Ls * V4 * 4;
+    TOUT* outr = static_cast<TOUT*>(a.out) + (long long)rhs * Ls * V4 * 4;
+    auto smod = [Ls](int s) { return ((s % Ls) + Ls) % Ls; };
+    auto slot = [](int s) { return (s + 16) & 3; };
+    // this thread's 16 bytes of a step's weight record: layer 1 (threads 0..63), layer 2
+    // (64..287), layer 3 (288..511); the layers run 5, 7 and 9 steps behind the step index
+    const int wl = tid < 64 ? 0 : (tid < 288 ? 1 : 2);
+    const int wbase = tid < 64 ? tid : (tid < 288 ? tid - 64 : tid - 288);
+    const int wlag = wl == 0 ? 5 : (wl == 1 ? 7 : 9), wsize = wl == 0 ? 2 * 64 * 4 : 7 * 64 * 4;  // shorts per s
+    const unsigned short* wsrc = a.wf[wl] + wbase * 8;
+
+    // Global loads are issued at the end of a step (after its output stores) and stored
+    // to LDS at the end of the next one, so their latency spans a step; the two register
+    // sets alternate between steps, so no copy waits for a load in flight.  Every thread
+    // issues both loads on every step (clamped), so the count in flight never depends on
+    // the path.  Step j loads input slice j-2 and the weights of step j+2, and stores
+    // input slice j-3 (its ring slot held slice j-7, last read by layer 1 in step j-1) and
+    // the weights of step j+1 (their buffer was last read in step j-1).
+    auto loads = [&](int j, float4& in_new, uint4& w_new) {
+        in_new = ld_spinor(inr, (long long)smod(min(j - 2, Ls + 2)) * V4 + max(gsite, 0));
+        w_new = *reinterpret_cast<const uint4*>(wsrc + (long long)smod(j + 2 - wlag) * wsize);
+    };
+    auto stores = [&](int j, const float4& in_old, const uint4& w_old) {
+        if (j - 3 >= -3 && j - 3 <= Ls + 2 && gsite >= 0)
+            ring0[slot(j - 3)][tid] = make_uint2(pk_bf16(in_old.x, in_old.y), pk_bf16(in_old.z, in_old.w));
+        wrec[(j + 1) & 1][tid] = w_old;
+        __syncthreads();
+    };
+    // fragment q of layer l (0, 1, 2) in step j's record
+    auto frag = [&](int j, int l, int q) {
+        const uint2* r = reinterpret_cast<const uint2*>(wrec[j & 1]);
+        const uint2 v = r[(l == 0 ? 0 : (l == 1 ? 128 : 576)) + q * 64 + lane];
+        return pack_pairs(v.x, v.y);
+    };
+    // steps j = -3 .. Ls + 8 (in pairs; one idle step when Ls is odd): nothing runs
+    // before step 3, whose layer 1 needs input slices -3 .. -1 (loaded in steps -1 .. 1)
+    const int jend = -3 + 2 * ((Ls + 13) / 2);
+
+    // One 16-site tile of a C_in = 8 layer at slice s: b is the lane's site in the input
+    // region (width wi); lk the transport operands of the four hops; win[0], win[1] the
+    // values at s-1 and s (loaded here when `first`), advanced to s, s+1 on return.
+    auto hop8 = [&](const auto& rin, int j, int l, int s, int b, int wi, const uint2 (&lk)[4], uint2 (&win)[2],
+                    bool first) {
+        if (first) {
+            win[0] = rin[slot(s - 1)][b][g];
+            win[1] = rin[slot(s)][b][g];
+        }
+        bf16x4_t B[7];
+        const uint2 up = rin[slot(s + 1)][b][g];
+#pragma unroll
+        for (int q = 1; q <= 4; ++q) {
+            const int nb = q == 1 ? b + wi : q == 2 ? b - wi : q == 3 ? b + 1 : b - 1;
+            const uint2 v = rin[slot(s)][nb][g];
+            B[q] = pack_pairs(transport(v.x, lk[q - 1]), transport(v.y, lk[q - 1]));
+        }
+        B[0] = pack_pairs(win[1].x, win[1].y);
+        B[5] = pack_pairs(up.x, up.y);
+        B[6] = pack_pairs(win[0].x, win[0].y);
+        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < 7; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(j, l, q), B[q], acc, 0, 0, 0);
+        win[0] = win[1];
+        win[1] = up;
+        return acc;
+    };
+
+    if (wv < 4) {
+        // ---- layer 2: R2 tiles wv*3 + e (origin t0-1, x0-1), input ring1 (width W1)
+        int bsite[kFuL2Tiles], osite[kFuL2Tiles];
+        bool valid[kFuL2Tiles];
+        uint2 lk[kFuL2Tiles][4], win[kFuL2Tiles][2];
+#pragma unroll
+        for (int e = 0; e < kFuL2Tiles; ++e) {
+            const int k = wv * kFuL2Tiles + e;
+            const int sg = min(16 * k + m, n2 - 1);
+            valid[e] = 16 * k + m < n2;
+            const int i = div_small(sg, inv2), jx = sg - i * W2;
+            bsite[e] = (i + 1) * W1 + jx + 1;
+            osite[e] = sg;
+            const int t = wrap_t(t0 - 1 + i), x = wrap_x(x0 - 1 + jx);
+#pragma unroll
+            for (int p = 1; p <= 4; ++p) lk[e][p - 1] = hop_link(L, Nt, Nx, V4, p, t, x);
+            win[e][0] = win[e][1] = make_uint2(0u, 0u);
+        }
+        const int nt = __builtin_amdgcn_readfirstlane(min(kFuL2Tiles, ((n2 + 15) >> 4) - wv * kFuL2Tiles));
+        auto step = [&](int j, float4& in_new, uint4& w_new, const float4& in_old, const uint4& w_old) {
+            const int s2 = j - 7;
+            if (s2 >= -1 && s2 <= Ls) {
+#pragma unroll
+                for (int e = 0; e < kFuL2Tiles; ++e)
+                    if (e < nt) {
+                        const f32x4_t acc = hop8(ring1, j, 1, s2, bsite[e], W1, lk[e], win[e], s2 == -1);
+                        if (valid[e])
+                            ring2[slot(s2)][osite[e]][g] = make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
+                    }
+            }
+            loads(j, in_new, w_new);
+            stores(j, in_old, w_old);
+        };
+        float4 inX = make_float4(0.f, 0.f, 0.f, 0.f), inY = inX;
+        uint4 wX = make_uint4(0u, 0u, 0u, 0u), wY = wX;
+        for (int j = -3; j < jend; j += 2) {
+            step(j, inX, wX, inY, wY);
+            step(j + 1, inY, wY, inX, wX);
+        }
+    } else {
+        // ---- layer 1: R1 tiles (wv-4)*4 + e (origin t0-2, x0-2), input ring0 (width W0)
+        // ---- output layer: tiles (wv-4)*2 + e of the TY x TX tile, input ring2 (width W2)
+        const int wr = wv - 4;
+        int b3[kFuL3Tiles], o3[kFuL3Tiles];
+        bool v3[kFuL3Tiles];
+        uint2 lk3[kFuL3Tiles][4], win3[kFuL3Tiles][2];
+#pragma unroll
+        for (int e = 0; e < kFuL3Tiles; ++e) {
+            const int k = wr * kFuL3Tiles + e;
+            const int sg = min(16 * k + m, n3 - 1);
+            v3[e] = 16 * k + m < n3 && g == 0;
+            const int i = div_small(sg, inv3), jx = sg - i * TX;
+            b3[e] = (i + 1) * W2 + jx + 1;
+            o3[e] = (t0 + i) * Nx + x0 + jx;
+#pragma unroll
+            for (int p = 1; p <= 4; ++p) lk3[e][p - 1] = hop_link(L, Nt, Nx, V4, p, t0 + i, x0 + jx);
+            win3[e][0] = win3[e][1] = make_uint2(0u, 0u);
+        }
+        int b1[kFuL1Tiles], o1[kFuL1Tiles];
+        bool v1[kFuL1Tiles];
+        uint2 lk1[kFuL1Tiles][2];
+#pragma unroll
+        for (int e = 0; e < kFuL1Tiles; ++e) {
+            const int k = wr * kFuL1Tiles + e;
+            const int sg = min(16 * k + m, n1 - 1);
+            v1[e] = 16 * k + m < n1;
+            const int i = div_small(sg, inv1), jx = sg - i * W1;
+            b1[e] = (i + 1) * W0 + jx + 1;
+            o1[e] = sg;
+            const int t = wrap_t(t0 - 2 + i), x = wrap_x(x0 - 2 + jx);
+            lk1[e][0] = hop_link(L, Nt, Nx, V4, g, t, x);      // point g (0: none)
+            lk1[e][1] = hop_link(L, Nt, Nx, V4, 4 + g, t, x);  // point 4 + g (4: -x; 5, 6: none)
+        }
+        const int nt3 = __builtin_amdgcn_readfirstlane(min(kFuL3Tiles, ((n3 + 15) >> 4) - wr * kFuL3Tiles));
+        const int nt1 = __builtin_amdgcn_readfirstlane(min(kFuL1Tiles, ((n1 + 15) >> 4) - wr * kFuL1Tiles));
+        auto step = [&](int j, float4& in_new, uint4& w_new, const float4& in_old, const uint4& w_old) {
+            const int s1 = j - 5, s3 = j - 9;
+            if (s1 >= -2 && s1 <= Ls + 1) {
+                const int sc = slot(s1), sp = slot(s1 + 1), sm = slot(s1 - 1);
+                const bf16x4_t Wa0 = frag(j, 0, 0), Wa1 = frag(j, 0, 1);
+#pragma unroll
+                for (int e = 0; e < kFuL1Tiles; ++e)
+                    if (e < nt1) {
+                        const int b0 = b1[e];
+                        bf16x4_t B[2];
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            const int p = 4 * q + g;
+                            const int nb = p == 1 ? b0 + W0 : p == 2 ? b0 - W0 : p == 3 ? b0 + 1 : p == 4 ? b0 - 1 : b0;
+                            const int ss = p == 5 ? sp : p == 6 ? sm : sc;
+                            const uint2 v = ring0[ss][nb];
+                            B[q] = p < kPts ? pack_pairs(transport(v.x, lk1[e][q]), transport(v.y, lk1[e][q]))
+                                            : bf16x4_t{0, 0, 0, 0};
+                        }
+                        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wa0, B[0], acc, 0, 0, 0);
+                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wa1, B[1], acc, 0, 0, 0);
+                        if (v1[e]) ring1[sc][o1[e]][g] = make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
+                    }
+            }
+            if (s3 >= 0 && s3 <= Ls - 1) {
+#pragma unroll
+                for (int e = 0; e < kFuL3Tiles; ++e)
+                    if (e < nt3) {
+                        const f32x4_t acc = hop8(ring2, j, 2, s3, b3[e], W2, lk3[e], win3[e], s3 == 0);
+                        if (v3[e]) st_spinor(outr, (long long)s3 * V4 + o3[e], acc[0], acc[1], acc[2], acc[3]);
+                    }
+            }
+            loads(j, in_new, w_new);
+            stores(j, in_old, w_old);
+        };
+        float4 inX = make_float4(0.f, 0.f, 0.f, 0.f), inY = inX;
+        uint4 wX = make_uint4(0u, 0u, 0u, 0u), wY = wX;
+        for (int j = -3; j < jend; j += 2) {
+            step(j, inX, wX, inY, wY);
+            step(j + 1, inY, wY, inX, wX);
+        }
+    }
+    if constexpr (DOTS) {
+        // (z, r) and (z, q) over the block's output (z written above by this block)
+        const float4* Z = reinterpret_cast<const float4*>(outr);
+        const float4* Rv = reinterpret_cast<const float4*>(inr);
+        const float4* Q = reinterpret_cast<const float4*>(a.q) + (long long)rhs * Ls * V4;
+        double dzr = 0.0, dzq = 0.0;
+        for (int e = tid; e < n3 * Ls; e += kFuThreads) {
+            const int s = e / n3, k = e - s * n3;
+            const int i = div_small(k, inv3), jx = k - i * TX;
+            const long long site = (long long)s * V4 + (t0 + i) * Nx + x0 + jx;
+            const float4 z = Z[site], r = Rv[site], qv = Q[site];
+            dzr += (double)z.x * r.x + (double)z.y * r.y + (double)z.z * r.z + (double)z.w * r.w;
+            dzq += (double)z.x * qv.x + (double)z.y * qv.y + (double)z.z * qv.z + (double)z.w * qv.w;
+        }
+        dzr = wsum(dzr);
+        dzq = wsum(dzq);
+        if (lane == 0) {
+            red[2 * wv] = dzr;
+            red[2 * wv + 1] = dzq;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double zr = 0.0, zq = 0.0;
+            for (int w = 0; w < kFuWaves; ++w) {
+                zr += red[2 * w];
+                zq += red[2 * w + 1];
+            }
+            double* pp = a.part + ((long long)rhs * a.tiles + tile) * 2;
+            pp[0] = zr;
+            pp[1] = zq;
+        }
+    }
 }
 
 // |x_r|^2 of float spinor rows (one block per row)
@@ -946,10 +1293,67 @@ void repack(const dwf_net* net, hipStream_t s)
     dwf_ctx::check_launch("pack weights");
 }
 
-// z = M in for nrhs right-hand sides (double spinors), configuration per cfg[rhs].
+// Output tile of the fused network kernel: the largest divisors of Nt (<= 8) and Nx (<= 16).
+bool fused_shape(const Lat& L, int& TY, int& TX)
+{
+    if (L.Nt < 3 || L.Nx < 3) return false;
+    TY = 1;
+    TX = 1;
+    for (int d = 1; d <= kFuTy; ++d)
+        if (L.Nt % d == 0) TY = d;
+    for (int d = 1; d <= kFuTx; ++d)
+        if (L.Nx % d == 0) TX = d;
+    return true;
+}
+
+// Fused z = M in (k_net_fused); with q and part also the (z, in), (z, q) partials, part
+// [nrhs][tiles][2].  Returns the number of tiles per lattice, or 0 if the shape does not
+// allow the fused kernel (then nothing is launched).
+int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind, void* out, int out_kind,
+                     const int* cfg, int nrhs, const CgState* st, const float* q, double* part)
+{
+    const Lat& L = c->L;
+    int TY, TX;
+    if (!c->net_fused || !fused_shape(L, TY, TX)) return 0;
+    NetFusedArgs a{};
+    a.Nt = L.Nt;
+    a.Nx = L.Nx;
+    a.Ls = L.Ls;
+    a.V4 = L.V4;
+    a.TY = TY;
+    a.TX = TX;
+    a.tiles_x = L.Nx / TX;
+    a.tiles = (L.Nt / TY) * a.tiles_x;
+    a.in = in;
+    a.out = out;
+    a.links = c->l32;
+    a.link_stride = 2LL * L.V4;
+    a.cfg = cfg;
+    for (int l = 0; l < 3; ++l) a.wf[l] = net->wf[l];
+    a.st = st;
+    a.q = q;
+    a.part = part;
+    const unsigned grid = (unsigned)(a.tiles * nrhs);
+    hipStream_t s = c->stream;
+    if (q && part) {
+        if (in_kind != 1 || out_kind != 1) throw DwfError(DWF_ERR_ARG, "fused network: dot products need FP32 spinors");
+        k_net_fused<float, float, true><<<grid, kFuThreads, 0, s>>>(a);
+    } else if (in_kind == 0 && out_kind == 0) {
+        k_net_fused<double, double, false><<<grid, kFuThreads, 0, s>>>(a);
+    } else if (in_kind == 1 && out_kind == 1) {
+        k_net_fused<float, float, false><<<grid, kFuThreads, 0, s>>>(a);
+    } else {
+        throw DwfError(DWF_ERR_ARG, "fused network: unsupported spinor types");
+    }
+    return a.tiles;
+}
+
+// z = M in for nrhs right-hand sides, configuration per cfg[rhs]: the fused kernel when the
+// shape allows it and the input is not gathered by rows, else layer by layer through h1, h2.
 void net_forward(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind, const int* in_row, void* out,
                  int out_kind, const int* cfg, int nrhs, const CgState* st, unsigned short* h1, unsigned short* h2)
 {
+    if (!in_row && launch_net_fused(c, net, in, in_kind, out, out_kind, cfg, nrhs, st, nullptr, nullptr)) return;
     launch_layer(c, net, 0, false, LayerIO{in, in_row, h1, in_kind, 2}, cfg, nrhs, st);
     launch_layer(c, net, 1, false, LayerIO{h1, nullptr, h2, 2, 2}, cfg, nrhs, st);
     launch_layer(c, net, 2, false, LayerIO{h2, nullptr, out, 2, out_kind}, cfg, nrhs, st);
@@ -1369,6 +1773,9 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     ha.slot = 0;
     ha.skip = kSkip;
     const dim3 vgrid(nb, nrhs);
+    int fty = 0, ftx = 0;
+    const int ntiles = fused_shape(c->L, fty, ftx) ? (c->L.Nt / fty) * (c->L.Nx / ftx) : 1;
+    double* npart = (double*)c->buf(B_FCG_NPART, (size_t)nrhs * ntiles * 2 * 8);
 
     if (!have_guess) launch_zero<double>(x, (long long)nv, s);
     std::vector<CgState> h(nrhs);
@@ -1400,9 +1807,15 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         k_fcg_alpha<<<nrhs, 64, 0, s>>>(hpart, hnb, st, fs);
         k_fcg_xr<float><<<vgrid, 256, 0, s>>>(x32, r32, p32, q32, sites, st, fs, part);
         k_fcg_conv<<<nrhs, 64, 0, s>>>(part, nb, st);
-        net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
-        k_fcg_dots<float><<<vgrid, 256, 0, s>>>(z32, r32, q32, sites, st, part);
-        k_fcg_beta<<<nrhs, 64, 0, s>>>(part, nb, st, fs);
+        // z = M r with (z, r) and (z, q) accumulated by the fused network kernel, or
+        // layer by layer and a separate dot-product pass
+        if (launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, q32, npart)) {
+            k_fcg_beta<<<nrhs, 64, 0, s>>>(npart, ntiles, st, fs);
+        } else {
+            net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
+            k_fcg_dots<float><<<vgrid, 256, 0, s>>>(z32, r32, q32, sites, st, part);
+            k_fcg_beta<<<nrhs, 64, 0, s>>>(part, nb, st, fs);
+        }
         k_fcg_p<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs);
     };
     constexpr int kBlockIters = 8;
@@ -1412,9 +1825,11 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         for (const void* ptr : {(const void*)r32, (const void*)p32, (const void*)q32, (const void*)z32,
                                 (const void*)t32, (const void*)x32, (const void*)h1, (const void*)h2,
                                 (const void*)part, (const void*)hpart, (const void*)st, (const void*)fs,
+                                (const void*)npart,
                                 (const void*)cfg, (const void*)net, (const void*)net->wf[0], (const void*)c->l32})
             key.push_back((unsigned long long)(uintptr_t)ptr);
         key.push_back((unsigned long long)ha.rhs_per_cfg);
+        key.push_back((unsigned long long)c->net_fused);
         const int v[] = {ha.cfg.bx, ha.cfg.chunks, ha.cfg.spt, ha.cfg.remap, ha.cfg.prefetch, ha.cfg.tx,
                          ha.cfg.tiles_x, ha.cfg.nt_store, ha.cfg.lds};
         for (int e : v) key.push_back((unsigned long long)(unsigned)e);
