@@ -25,11 +25,14 @@
 //
 // Weights: per layer [Ls][7][C_out][C_in] complex, layers concatenated, (re, im) pairs.
 //
-// Quantized mode (bf16 = 1) rounds at exactly the points the device kernels round:
-// layer inputs to float and then bf16, links to float and then bf16, the transported
-// field (exact bf16 products, one rounding of their sum to float) to bf16, the weights to
-// bf16, stored hidden activations and gradients to bf16, with FP64 accumulation instead
-// of the device's FP32 -- so device and oracle agree to accumulation-order error.
+// Quantized mode (bf16 = 1) rounds at exactly the points the device kernels round: the
+// weights to bf16; stored hidden activations and gradients to bf16; in layers with two
+// input channels (and in the weight gradients) the layer input to float and then bf16,
+// the transporter to float and then bf16, and the transported value (exact bf16 products,
+// one rounding of their sum to float) to bf16; in layers with eight input channels the
+// transporter to float, applied without rounding to the product of the bf16 input.  The
+// oracle accumulates in FP64 where the device accumulates in FP32, so device and oracle
+// agree to accumulation-order error.
 
 #include <cmath>
 #include <complex>
@@ -175,7 +178,9 @@ void layer(const Geo& g, const double* th, const double* w, int l, bool adj, con
                 const int sw = adj ? (int)(h.site / g.V4()) : s;
                 const cplx* W = wptr(w, l, g.Ls, sw, adj ? neg[p] : p);
                 for (int c = 0; c < ci; ++c) {
-                    const cplx a = q ? transport_q(h, hn[c]) : h.u * hn[c];
+                    // quantized: C_in = 2 layers transport the bf16 input in bf16 before the
+                    // product; C_in = 8 layers multiply the product by the FP32 transporter
+                    const cplx a = !q ? h.u * hn[c] : (ci == 2 ? transport_q(h, hn[c]) : cplx(h.cf, h.sf) * q16(hn[c]));
                     for (int o = 0; o < co; ++o) {
                         cplx wv = adj ? std::conj(W[c * kCin[l] + o]) : W[o * kCin[l] + c];
                         if (q) wv = qw(wv);

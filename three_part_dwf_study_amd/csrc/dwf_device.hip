@@ -261,6 +261,12 @@ __device__ __forceinline__ void hop_epi(const HopArgs<R>& a, long long rb, long 
             stsp(a.out + rb, idx, o);
         acc0 += norm2(o);
     } else if constexpr (EPI == EPI_CGUPD) {
+        if (a.out) {
+            if (a.cfg.nt_store)
+                stsp_nt(a.out + rb, idx, o);
+            else
+                stsp(a.out + rb, idx, o);
+        }
         const Sp<R> pv = ldsp(a.p + rb, idx);
         Sp<R> xv = ldsp(a.x + rb, idx);
         Sp<R> rv = ldsp(a.r + rb, idx);
@@ -313,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
     if constexpr (EPI == EPI_CGUPD) {
         const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
         if (pAp == 0.0) return;  // block-uniform; the p-update kernel records the failure
-        alpha = (R)(a.st[rhs].rr[a.slot] / pAp);
+        alpha = (R)((a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp);
     }
 
     const int V4 = L.V4;
@@ -442,7 +448,7 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     if constexpr (EPI == EPI_CGUPD) {
         const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
         if (pAp == 0.0) return;  // block-uniform; the p-update kernel records the failure
-        alpha = (R)(a.st[rhs].rr[a.slot] / pAp);
+        alpha = (R)((a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp);
     }
 
     const int V4 = L.V4, Nx = L.Nx, Nt = L.Nt, Ls = L.Ls;

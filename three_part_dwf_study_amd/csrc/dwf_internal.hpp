@@ -73,7 +73,7 @@ __device__ __forceinline__ int xcd_logical_block(int hw, int total)
 enum Epi : int {
     EPI_STORE = 0,  // out = Op in
     EPI_NORM = 1,   // out = Op in, partial |out|^2
-    EPI_CGUPD = 2,  // alpha = rr/sum(partA); x += alpha p; r -= alpha (Op in); partial |r|^2
+    EPI_CGUPD = 2,  // alpha = rr/sum(partA); x += alpha p; r -= alpha (Op in); partial |r|^2; optionally out = Op in
     EPI_RESID = 3   // r64 = b - Op in (+ copies to p64/r32/p32); partials |r|^2, |b|^2
 };
 
@@ -90,6 +90,8 @@ template <typename R> struct HopArgs {
     R* r;                // EPI_CGUPD
     const R* p;          // EPI_CGUPD
     const double* partA; // EPI_CGUPD: partials of p^dag A p from the first stencil
+    const double* anum;  // EPI_CGUPD (nullable): alpha's numerator is anum[rhs * anum_stride] instead of
+    int anum_stride;     //   st[rhs].rr[slot] (the flexible CG's (z, r)); out (nullable) receives Op in
     const double* b;     // EPI_RESID
     double* r64;         // EPI_RESID
     double* p64;         // EPI_RESID (nullable)

@@ -41,6 +41,7 @@
 #include <numeric>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace dwfk;
@@ -203,46 +204,83 @@ __device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d)
 
 // ------------------------------------------------------------------ forward layer
 // Each MFMA computes D' = W^T H^T for one 16-site tile: the A operand is the weight
-// fragment (lane (m, g) holds W[in 4g + j][out m], j = 0..3), the B operand the
-// transported input (lane (m, g) holds input components 4g .. 4g + 3 of site m), and the
-// result leaves lane (m, g) with output components 4g .. 4g + 3 of site m.
+// fragment (lane (m, g) holds W[in 4g + j][out m], j = 0..3), the B operand the input
+// (lane (m, g) holds input components 4g .. 4g + 3 of site m), and the result leaves
+// lane (m, g) with output components 4g .. 4g + 3 of site m, i.e. the (re, im) pairs of
+// output channels 2g and 2g + 1.
+//
+// C_in = 8 layers transport after the product: W_p (U_p h) = U_p (W_p h) because U_p is a
+// complex scalar per site, so each hop is one MFMA of the stored bf16 values followed by
+// the FP32 complex multiply-accumulate by U_p of the output site.  The order is fixed
+// (it makes the layer kernel and the fused network bit-identical): the MFMA chain of the
+// points without a link (self, +s, -s), then the hops +t, -t, +x, -x.
+// C_in = 2 layers pack four stencil points into one MFMA, so they transport before the
+// product, in bf16 dot products (transport()).
 
-// bf16 pairs (re, im) of two complex values
+// bf16 pairs (re, im) of two complex values, as an MFMA operand
 __device__ __forceinline__ bf16x4_t pack_pairs(unsigned lo, unsigned hi)
 {
-    bf16x4_t v;
-    v[0] = (short)(lo & 0xffffu);
-    v[1] = (short)(lo >> 16);
-    v[2] = (short)(hi & 0xffffu);
-    v[3] = (short)(hi >> 16);
-    return v;
+    return __builtin_bit_cast(bf16x4_t, make_uint2(lo, hi));
 }
 
-// B operand of MFMA step q for the lane's site (t, x) of slice s: the transported
-// neighbour channels of stencil point q (C_in = 8: channels 2g, 2g+1) or of point
-// 4q + g (C_in = 2: both channels), in bf16.
-template <int CIN, typename TIN>
-__device__ __forceinline__ bf16x4_t build_B(const PtcArgs& a, const float2* L, const void* inb, int q, int s, int t,
-                                            int x, int g)
+// y += u G for the two complex values of a lane (FP32, fixed fma order)
+__device__ __forceinline__ void cmac2(f32x4_t& y, const f32x4_t& G, float c, float sn)
 {
-    if constexpr (CIN == 8) {
+    y[0] = fmaf(c, G[0], y[0]);
+    y[0] = fmaf(-sn, G[1], y[0]);
+    y[1] = fmaf(sn, G[0], y[1]);
+    y[1] = fmaf(c, G[1], y[1]);
+    y[2] = fmaf(c, G[2], y[2]);
+    y[2] = fmaf(-sn, G[3], y[2]);
+    y[3] = fmaf(sn, G[2], y[3]);
+    y[3] = fmaf(c, G[3], y[3]);
+}
+
+// One 16-site tile of a C_in = 8 layer: h[q] the stored values at stencil point q's
+// neighbour, u[p - 1] the transporter (cos, sin) of hop p = 1..4 at the output site.
+__device__ __forceinline__ f32x4_t tile8(const bf16x4_t (&W)[7], const bf16x4_t (&h)[7], const float2 (&u)[4])
+{
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    f32x4_t y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[0], h[0], zero, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[5], h[5], y, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[6], h[6], y, 0, 0, 0);
+    f32x4_t G[4];
+#pragma unroll
+    for (int p = 1; p <= 4; ++p) G[p - 1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[p], h[p], zero, 0, 0, 0);
+#pragma unroll
+    for (int p = 1; p <= 4; ++p) cmac2(y, G[p - 1], u[p - 1].x, u[p - 1].y);
+    return y;
+}
+
+// Operands of a C_in = 8 tile at the lane's site (t, x) of slice s (channels 2g, 2g+1).
+__device__ __forceinline__ void gather8(const PtcArgs& a, const float2* L, const void* inb, int s, int t, int x, int g,
+                                        bf16x4_t (&h)[7], float2 (&u)[4])
+{
+#pragma unroll
+    for (int q = 0; q < kPts; ++q) {
         const Nb n = neighbour(a, L, q, s, t, x);
         const uint2 v = *reinterpret_cast<const uint2*>(static_cast<const unsigned short*>(inb) +
                                                         ((long long)n.s * a.V4 + n.i) * 16 + 4 * g);
-        if (q == 0 || q >= 5) return pack_pairs(v.x, v.y);  // no link: the stored values
-        const uint2 lk = link_ops(n);
-        return pack_pairs(transport(v.x, lk), transport(v.y, lk));
-    } else {
-        const int p = 4 * q + g;  // this lane group's stencil point
-        bf16x4_t B = {0, 0, 0, 0};
-        if (p < kPts) {
-            const Nb n = neighbour(a, L, p, s, t, x);
-            const float4 v = ld_spinor(static_cast<const TIN*>(inb), (long long)n.s * a.V4 + n.i);
-            const uint2 lk = link_ops(n);
-            B = pack_pairs(transport(pk_bf16(v.x, v.y), lk), transport(pk_bf16(v.z, v.w), lk));
-        }
-        return B;
+        h[q] = pack_pairs(v.x, v.y);
+        if (q >= 1 && q <= 4) u[q - 1] = make_float2(n.c, n.sn);
     }
+}
+
+// B operand of MFMA step q of a C_in = 2 layer for the lane's site (t, x) of slice s:
+// stencil point 4q + g's two channels, transported in bf16.
+template <typename TIN>
+__device__ __forceinline__ bf16x4_t build_B2(const PtcArgs& a, const float2* L, const void* inb, int q, int s, int t,
+                                             int x, int g)
+{
+    const int p = 4 * q + g;  // this lane group's stencil point
+    bf16x4_t B = {0, 0, 0, 0};
+    if (p < kPts) {
+        const Nb n = neighbour(a, L, p, s, t, x);
+        const float4 v = ld_spinor(static_cast<const TIN*>(inb), (long long)n.s * a.V4 + n.i);
+        const uint2 lk = link_ops(n);
+        B = pack_pairs(transport(pk_bf16(v.x, v.y), lk), transport(pk_bf16(v.z, v.w), lk));
+    }
+    return B;
 }
 
 // Output components 4g .. 4g + 3 of site m of one 16-site tile at obase.
@@ -302,13 +340,20 @@ __global__ __launch_bounds__(256) void k_ptc_fwd(const PtcArgs a)
         }
         const long long obase = ((long long)rhs * a.Ls + s) * a.V4 + i0;
         const int ia = i0 + m, ta = ia / a.Nx, xa = ia - ta * a.Nx;
-        if (two) {
-            const int ib = ia + 16, tb = ib / a.Nx, xb = ib - tb * a.Nx;
+        const int ib = ia + 16, tb = ib / a.Nx, xb = ib - tb * a.Nx;
+        if constexpr (CIN == 8) {
+            bf16x4_t h0[7], h1[7];
+            float2 u0[4], u1[4];
+            gather8(a, L, inb, s, ta, xa, g, h0, u0);
+            if (two) gather8(a, L, inb, s, tb, xb, g, h1, u1);
+            store_tile<COUT, TOUT>(a, tile8(W, h0, u0), obase, m, g);
+            if (two) store_tile<COUT, TOUT>(a, tile8(W, h1, u1), obase + 16, m, g);
+        } else if (two) {
             f32x4_t acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < NSTEP; ++q) {
-                const bf16x4_t H0 = build_B<CIN, TIN>(a, L, inb, q, s, ta, xa, g);
-                const bf16x4_t H1 = build_B<CIN, TIN>(a, L, inb, q, s, tb, xb, g);
+                const bf16x4_t H0 = build_B2<TIN>(a, L, inb, q, s, ta, xa, g);
+                const bf16x4_t H1 = build_B2<TIN>(a, L, inb, q, s, tb, xb, g);
                 acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[q], H0, acc0, 0, 0, 0);
                 acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[q], H1, acc1, 0, 0, 0);
             }
@@ -318,8 +363,8 @@ __global__ __launch_bounds__(256) void k_ptc_fwd(const PtcArgs a)
             f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int q = 0; q < NSTEP; ++q)
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[q], build_B<CIN, TIN>(a, L, inb, q, s, ta, xa, g),
-                                                                acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[q], build_B2<TIN>(a, L, inb, q, s, ta, xa, g), acc,
+                                                                0, 0, 0);
             store_tile<COUT, TOUT>(a, acc, obase, m, g);
         }
     }
@@ -547,7 +592,7 @@ __device__ __forceinline__ double fold64(const double* p, int n, int which)
 // at j-9.  Waves have fixed roles and fixed 16-site tiles (no loop over tiles, so the
 // compiler waits for exactly the weights before the first MFMA and never for the
 // input in flight): waves 0-3 run layer 2 (3 tiles each), waves 4-7 the output layer
-// (2 tiles each) and layer 1 (4 tiles each).  The transport operands of a wave's tiles
+// (2 tiles each), and every wave 2 tiles of layer 1.  The transport operands of a wave's tiles
 // and, for layers 2 and 3, the fifth-dimension window (the s-1 and s values at the
 // tile's sites) stay in registers across steps.  The arithmetic (bf16 operands, dot2
 // transport, MFMA order over stencil points) is that of k_ptc_fwd, so both give
@@ -556,8 +601,11 @@ __device__ __forceinline__ double fold64(const double* p, int n, int which)
 constexpr int kFuTy = 8, kFuTx = 16;  // largest output tile
 constexpr int kFuR0 = (kFuTy + 6) * (kFuTx + 6), kFuR1 = (kFuTy + 4) * (kFuTx + 4), kFuR2 = (kFuTy + 2) * (kFuTx + 2);
 constexpr int kFuThreads = 512, kFuWaves = kFuThreads / 64;
-constexpr int kFuL2Tiles = 3, kFuL3Tiles = 2, kFuL1Tiles = 4;  // tiles per wave of its role
-static_assert(4 * kFuL2Tiles * 16 >= kFuR2 && 4 * kFuL3Tiles * 16 >= kFuTy * kFuTx && 4 * kFuL1Tiles * 16 >= kFuR1,
+// tiles per wave: waves 0-3 run 3 layer-2 tiles, waves 4-7 2 output tiles, and every wave
+// 2 layer-1 tiles (wave w takes layer-1 tiles 2w, 2w + 1)
+constexpr int kFuL2Tiles = 3, kFuL3Tiles = 2, kFuL1Tiles = 2;
+static_assert(4 * kFuL2Tiles * 16 >= kFuR2 && 4 * kFuL3Tiles * 16 >= kFuTy * kFuTx &&
+                  kFuWaves * kFuL1Tiles * 16 >= kFuR1,
               "the waves' tiles must cover every region of the largest output tile");
 
 struct NetFusedArgs {
@@ -578,9 +626,10 @@ struct NetFusedArgs {
 // i = k / w for 0 <= k < 2^16 / w (w <= 22 here): one multiply and a shift
 __device__ __forceinline__ int div_small(int k, unsigned inv) { return (int)(((unsigned)k * inv) >> 16); }
 
-// Transport operands of hop p (1 +t, 2 -t, 3 +x, 4 -x; anything else: none) at lattice
-// site (t, x): the Dirac operator's transporters with the antiperiodic time boundary.
-__device__ __forceinline__ uint2 hop_link(const float2* L, int Nt, int Nx, int V4, int p, int t, int x)
+// Transporter (cos, sin) of hop p (1 +t, 2 -t, 3 +x, 4 -x; anything else: none, 1) at
+// lattice site (t, x): the Dirac operator's transporters with the antiperiodic time
+// boundary, as neighbour() forms them.
+__device__ __forceinline__ float2 hop_u(const float2* L, int Nt, int Nx, int V4, int p, int t, int x)
 {
     float c = 1.f, sn = 0.f;
     if (p == 1) {
@@ -593,7 +642,7 @@ __device__ __forceinline__ uint2 hop_link(const float2* L, int Nt, int Nx, int V
         const float2 u = L[tm * Nx + x];
         const float sg = t == 0 ? -1.f : 1.f;
         c = sg * u.x;
-        sn = -sg * u.y;
+        sn = sg * -u.y;
     } else if (p == 3) {
         const float2 u = L[V4 + t * Nx + x];
         c = u.x;
@@ -603,8 +652,17 @@ __device__ __forceinline__ uint2 hop_link(const float2* L, int Nt, int Nx, int V
         c = u.x;
         sn = -u.y;
     }
-    return link_ops(Nb{0, 0, c, sn});
+    return make_float2(c, sn);
 }
+
+// bf16 dot-product operands of the same transporter
+__device__ __forceinline__ uint2 hop_link(const float2* L, int Nt, int Nx, int V4, int p, int t, int x)
+{
+    const float2 u = hop_u(L, Nt, Nx, V4, p, t, x);
+    return link_ops(Nb{0, 0, u.x, u.y});
+}
+
+template <int V> using IC = std::integral_constant<int, V>;
 
 template <typename TIN, typename TOUT, bool DOTS>
 __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_net_fused(const NetFusedArgs a)
@@ -612,7 +670,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     __shared__ uint2 ring0[4][kFuR0];     // input, bf16 (comp 0, comp 1)
     __shared__ uint2 ring1[4][kFuR1][4];  // layer 1 output, 8 channels bf16
     __shared__ uint2 ring2[4][kFuR2][4];  // layer 2 output
-    __shared__ uint4 wrec[2][kFuThreads]; // weight fragments of a step: layer 1 (2), 2 (7), 3 (7) x 64 lanes
+    __shared__ uint2 wrec[2][2 * kFuThreads]; // weight fragments of a step: layer 1 (2), 2 (7), 3 (7) x 64 lanes
     __shared__ double red[2 * kFuWaves];
 
     const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -641,14 +699,16 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     const TIN* inr = static_cast<const TIN*>(a.in) + (long long)rhs * Ls * V4 * 4;
     TOUT* outr = static_cast<TOUT*>(a.out) + (long long)rhs * Ls * V4 * 4;
     auto smod = [Ls](int s) { return ((s % Ls) + Ls) % Ls; };
-    auto slot = [](int s) { return (s + 16) & 3; };
     // this thread's 16 bytes of a step's weight record: layer 1 (threads 0..63), layer 2
     // (64..287), layer 3 (288..511); the layers run 5, 7 and 9 steps behind the step index
     const int wl = tid < 64 ? 0 : (tid < 288 ? 1 : 2);
-    const int wbase = tid < 64 ? tid : (tid < 288 ? tid - 64 : tid - 288);
-    const int wlag = wl == 0 ? 5 : (wl == 1 ? 7 : 9), wsize = wl == 0 ? 2 * 64 * 4 : 7 * 64 * 4;  // shorts per s
-    const unsigned short* wsrc = a.wf[wl] + wbase * 8;
+    const int wbase = (tid < 64 ? tid : (tid < 288 ? tid - 64 : tid - 288)) * 8;  // shorts
 
+    // The walk runs steps j = -3 .. Ls + 8 four at a time; the position K = (j + 3) mod 4
+    // is a compile-time constant in each copy of the step, and so is every ring slot
+    // (slot(s) = s mod 4): slot(j - lag + d) = (K + 13 - lag + d) mod 4.  Nothing runs
+    // before step 3, whose layer 1 needs input slices -3 .. -1 (loaded in steps -1 .. 1).
+    //
     // Global loads are issued at the end of a step (after its output stores) and stored
     // to LDS at the end of the next one, so their latency spans a step; the two register
     // sets alternate between steps, so no copy waits for a load in flight.  Every thread
@@ -656,170 +716,203 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     // the path.  Step j loads input slice j-2 and the weights of step j+2, and stores
     // input slice j-3 (its ring slot held slice j-7, last read by layer 1 in step j-1) and
     // the weights of step j+1 (their buffer was last read in step j-1).
+    const int jend = -3 + 4 * ((Ls + 15) / 4);
     auto loads = [&](int j, float4& in_new, uint4& w_new) {
         in_new = ld_spinor(inr, (long long)smod(min(j - 2, Ls + 2)) * V4 + max(gsite, 0));
-        w_new = *reinterpret_cast<const uint4*>(wsrc + (long long)smod(j + 2 - wlag) * wsize);
+        // the three layers' slices are wave-uniform; the thread picks its layer's record
+        const unsigned short* w0 = a.wf[0] + smod(j - 3) * (2 * 64 * 4);
+        const unsigned short* w1 = a.wf[1] + smod(j - 5) * (7 * 64 * 4);
+        const unsigned short* w2 = a.wf[2] + smod(j - 7) * (7 * 64 * 4);
+        w_new = *reinterpret_cast<const uint4*>((wl == 0 ? w0 : (wl == 1 ? w1 : w2)) + wbase);
     };
-    auto stores = [&](int j, const float4& in_old, const uint4& w_old) {
+    auto stores = [&](auto Kc, int j, const float4& in_old, const uint4& w_old) {
+        constexpr int K = decltype(Kc)::value;
         if (j - 3 >= -3 && j - 3 <= Ls + 2 && gsite >= 0)
-            ring0[slot(j - 3)][tid] = make_uint2(pk_bf16(in_old.x, in_old.y), pk_bf16(in_old.z, in_old.w));
-        wrec[(j + 1) & 1][tid] = w_old;
+            ring0[(K + 2) & 3][tid] = make_uint2(pk_bf16(in_old.x, in_old.y), pk_bf16(in_old.z, in_old.w));
+        reinterpret_cast<uint4*>(wrec[K & 1])[tid] = w_old;
         __syncthreads();
     };
-    // fragment q of layer l (0, 1, 2) in step j's record
-    auto frag = [&](int j, int l, int q) {
-        const uint2* r = reinterpret_cast<const uint2*>(wrec[j & 1]);
-        const uint2 v = r[(l == 0 ? 0 : (l == 1 ? 128 : 576)) + q * 64 + lane];
+    // fragment q of layer l (0, 1, 2) in the weight record of a step at position K
+    auto frag = [&](auto Kc, int l, int q) {
+        constexpr int K = decltype(Kc)::value;
+        const uint2 v = wrec[(K + 1) & 1][(l == 0 ? 0 : (l == 1 ? 128 : 576)) + q * 64 + lane];
         return pack_pairs(v.x, v.y);
     };
-    // steps j = -3 .. Ls + 8 (in pairs; one idle step when Ls is odd): nothing runs
-    // before step 3, whose layer 1 needs input slices -3 .. -1 (loaded in steps -1 .. 1)
-    const int jend = -3 + 2 * ((Ls + 13) / 2);
 
-    // One 16-site tile of a C_in = 8 layer at slice s: b is the lane's site in the input
-    // region (width wi); lk the transport operands of the four hops; win[0], win[1] the
-    // values at s-1 and s (loaded here when `first`), advanced to s, s+1 on return.
-    auto hop8 = [&](const auto& rin, int j, int l, int s, int b, int wi, const uint2 (&lk)[4], uint2 (&win)[2],
-                    bool first) {
+    // One 16-site tile of a C_in = 8 layer: rin the input ring, SM / S0 / SP the slots of
+    // slices s-1, s, s+1, e0 the lane's element (site * 4 + g) in the input region and
+    // etp / etm those of its +t / -t neighbours (the +-x ones are e0 -+ 4), u the
+    // transporters at the output site; win[0], win[1] the values at s-1 and s (loaded
+    // here when `first`), advanced to s, s+1.
+    auto hop8 = [&](auto Kc, auto Lc, auto SMc, auto S0c, auto SPc, const auto& rin, int e0, int etp, int etm,
+                    const float2 (&u)[4], uint2 (&win)[2], bool first) {
+        constexpr int SM = decltype(SMc)::value, S0 = decltype(S0c)::value, SP = decltype(SPc)::value;
+        const uint2* r0 = &rin[SM][0][0];
+        const uint2* rs = &rin[S0][0][0];
+        const uint2* rp = &rin[SP][0][0];
         if (first) {
-            win[0] = rin[slot(s - 1)][b][g];
-            win[1] = rin[slot(s)][b][g];
+            win[0] = r0[e0];
+            win[1] = rs[e0];
         }
-        bf16x4_t B[7];
-        const uint2 up = rin[slot(s + 1)][b][g];
+        bf16x4_t h[7], W[7];
+        const uint2 up = rp[e0];
+        const uint2 vtp = rs[etp], vtm = rs[etm], vxp = rs[e0 + 4], vxm = rs[e0 - 4];
+        h[0] = pack_pairs(win[1].x, win[1].y);
+        h[1] = pack_pairs(vtp.x, vtp.y);
+        h[2] = pack_pairs(vtm.x, vtm.y);
+        h[3] = pack_pairs(vxp.x, vxp.y);
+        h[4] = pack_pairs(vxm.x, vxm.y);
+        h[5] = pack_pairs(up.x, up.y);
+        h[6] = pack_pairs(win[0].x, win[0].y);
 #pragma unroll
-        for (int q = 1; q <= 4; ++q) {
-            const int nb = q == 1 ? b + wi : q == 2 ? b - wi : q == 3 ? b + 1 : b - 1;
-            const uint2 v = rin[slot(s)][nb][g];
-            B[q] = pack_pairs(transport(v.x, lk[q - 1]), transport(v.y, lk[q - 1]));
-        }
-        B[0] = pack_pairs(win[1].x, win[1].y);
-        B[5] = pack_pairs(up.x, up.y);
-        B[6] = pack_pairs(win[0].x, win[0].y);
-        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int q = 0; q < 7; ++q) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(frag(j, l, q), B[q], acc, 0, 0, 0);
+        for (int q = 0; q < 7; ++q) W[q] = frag(Kc, decltype(Lc)::value, q);
         win[0] = win[1];
         win[1] = up;
-        return acc;
+        return tile8(W, h, u);
+    };
+
+    // ---- layer 1 (every wave): R1 tiles wv*2 + e (origin t0-2, x0-2), input ring0 (width W0).
+    // Lane group g supplies stencil point g (MFMA step 0: self, +t, -t, +x, all in slice
+    // s) and point 4 + g (step 1: -x in slice s, +s, -s, and none with zero transporter).
+    int e1a[kFuL1Tiles], e1b[kFuL1Tiles], o1[kFuL1Tiles];
+    bool v1[kFuL1Tiles];
+    uint2 lk1[kFuL1Tiles][2];
+    const int dsl = g == 1 ? 1 : (g == 2 ? -1 : 0);  // step 1: slice offset of the lane's point
+#pragma unroll
+    for (int e = 0; e < kFuL1Tiles; ++e) {
+        const int k = wv * kFuL1Tiles + e;
+        const int sg = min(16 * k + m, n1 - 1);
+        v1[e] = 16 * k + m < n1;
+        const int i = div_small(sg, inv1), jx = sg - i * W1;
+        const int b0 = (i + 1) * W0 + jx + 1;
+        e1a[e] = g == 1 ? b0 + W0 : (g == 2 ? b0 - W0 : (g == 3 ? b0 + 1 : b0));
+        e1b[e] = g == 0 ? b0 - 1 : b0;
+        o1[e] = sg * 4 + g;
+        const int t = wrap_t(t0 - 2 + i), x = wrap_x(x0 - 2 + jx);
+        lk1[e][0] = hop_link(L, Nt, Nx, V4, g, t, x);  // point g (0: none)
+        lk1[e][1] = g == 3 ? make_uint2(0u, 0u) : hop_link(L, Nt, Nx, V4, 4 + g, t, x);  // -x, none, none, zero
+    }
+    const int nt1 = __builtin_amdgcn_readfirstlane(min(kFuL1Tiles, ((n1 + 15) >> 4) - wv * kFuL1Tiles));
+    auto layer1 = [&](auto Kc, int j) {
+        constexpr int K = decltype(Kc)::value;
+        const int s1 = j - 5;
+        if (s1 < -2 || s1 > Ls + 1) return;
+        constexpr int S0 = K & 3;
+        const int sb = (K + dsl) & 3;  // slot of the lane's step-1 point
+        const bf16x4_t Wa0 = frag(Kc, 0, 0), Wa1 = frag(Kc, 0, 1);
+#pragma unroll
+        for (int e = 0; e < kFuL1Tiles; ++e)
+            if (e < nt1) {
+                const uint2 va = ring0[S0][e1a[e]];
+                const uint2 vb = ring0[sb][e1b[e]];
+                const bf16x4_t B0 = pack_pairs(transport(va.x, lk1[e][0]), transport(va.y, lk1[e][0]));
+                const bf16x4_t B1 = pack_pairs(transport(vb.x, lk1[e][1]), transport(vb.y, lk1[e][1]));
+                f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wa0, B0, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wa1, B1, acc, 0, 0, 0);
+                if (v1[e])
+                    (&ring1[S0][0][0])[o1[e]] = make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
+            }
     };
 
     if (wv < 4) {
         // ---- layer 2: R2 tiles wv*3 + e (origin t0-1, x0-1), input ring1 (width W1)
-        int bsite[kFuL2Tiles], osite[kFuL2Tiles];
+        int e0[kFuL2Tiles], etp[kFuL2Tiles], etm[kFuL2Tiles], eo[kFuL2Tiles];
         bool valid[kFuL2Tiles];
-        uint2 lk[kFuL2Tiles][4], win[kFuL2Tiles][2];
+        float2 lk[kFuL2Tiles][4];
+        uint2 win[kFuL2Tiles][2];
 #pragma unroll
         for (int e = 0; e < kFuL2Tiles; ++e) {
             const int k = wv * kFuL2Tiles + e;
             const int sg = min(16 * k + m, n2 - 1);
             valid[e] = 16 * k + m < n2;
             const int i = div_small(sg, inv2), jx = sg - i * W2;
-            bsite[e] = (i + 1) * W1 + jx + 1;
-            osite[e] = sg;
+            const int b = (i + 1) * W1 + jx + 1;
+            e0[e] = b * 4 + g;
+            etp[e] = (b + W1) * 4 + g;
+            etm[e] = (b - W1) * 4 + g;
+            eo[e] = sg * 4 + g;
             const int t = wrap_t(t0 - 1 + i), x = wrap_x(x0 - 1 + jx);
 #pragma unroll
-            for (int p = 1; p <= 4; ++p) lk[e][p - 1] = hop_link(L, Nt, Nx, V4, p, t, x);
+            for (int p = 1; p <= 4; ++p) lk[e][p - 1] = hop_u(L, Nt, Nx, V4, p, t, x);
             win[e][0] = win[e][1] = make_uint2(0u, 0u);
         }
         const int nt = __builtin_amdgcn_readfirstlane(min(kFuL2Tiles, ((n2 + 15) >> 4) - wv * kFuL2Tiles));
-        auto step = [&](int j, float4& in_new, uint4& w_new, const float4& in_old, const uint4& w_old) {
+        auto step = [&](auto Kc, int jv, float4& in_new, uint4& w_new, const float4& in_old, const uint4& w_old) {
+            constexpr int K = decltype(Kc)::value;
+            const int j = __builtin_amdgcn_readfirstlane(jv);  // keep the walk's arithmetic scalar
             const int s2 = j - 7;
+            layer1(Kc, j);
             if (s2 >= -1 && s2 <= Ls) {
 #pragma unroll
                 for (int e = 0; e < kFuL2Tiles; ++e)
                     if (e < nt) {
-                        const f32x4_t acc = hop8(ring1, j, 1, s2, bsite[e], W1, lk[e], win[e], s2 == -1);
+                        const f32x4_t acc = hop8(Kc, IC<1>{}, IC<(K + 1) & 3>{}, IC<(K + 2) & 3>{},
+                                                 IC<(K + 3) & 3>{}, ring1, e0[e], etp[e], etm[e], lk[e], win[e],
+                                                 s2 == -1);
                         if (valid[e])
-                            ring2[slot(s2)][osite[e]][g] = make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
+                            (&ring2[(K + 2) & 3][0][0])[eo[e]] =
+                                make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
                     }
             }
             loads(j, in_new, w_new);
-            stores(j, in_old, w_old);
+            stores(Kc, j, in_old, w_old);
         };
         float4 inX = make_float4(0.f, 0.f, 0.f, 0.f), inY = inX;
         uint4 wX = make_uint4(0u, 0u, 0u, 0u), wY = wX;
-        for (int j = -3; j < jend; j += 2) {
-            step(j, inX, wX, inY, wY);
-            step(j + 1, inY, wY, inX, wX);
+        for (int j = -3; j < jend; j += 4) {
+            step(IC<0>{}, j, inX, wX, inY, wY);
+            step(IC<1>{}, j + 1, inY, wY, inX, wX);
+            step(IC<2>{}, j + 2, inX, wX, inY, wY);
+            step(IC<3>{}, j + 3, inY, wY, inX, wX);
         }
     } else {
-        // ---- layer 1: R1 tiles (wv-4)*4 + e (origin t0-2, x0-2), input ring0 (width W0)
         // ---- output layer: tiles (wv-4)*2 + e of the TY x TX tile, input ring2 (width W2)
         const int wr = wv - 4;
-        int b3[kFuL3Tiles], o3[kFuL3Tiles];
+        int e3[kFuL3Tiles], e3p[kFuL3Tiles], e3m[kFuL3Tiles], o3[kFuL3Tiles];
         bool v3[kFuL3Tiles];
-        uint2 lk3[kFuL3Tiles][4], win3[kFuL3Tiles][2];
+        float2 lk3[kFuL3Tiles][4];
+        uint2 win3[kFuL3Tiles][2];
 #pragma unroll
         for (int e = 0; e < kFuL3Tiles; ++e) {
             const int k = wr * kFuL3Tiles + e;
             const int sg = min(16 * k + m, n3 - 1);
             v3[e] = 16 * k + m < n3 && g == 0;
             const int i = div_small(sg, inv3), jx = sg - i * TX;
-            b3[e] = (i + 1) * W2 + jx + 1;
+            const int b = (i + 1) * W2 + jx + 1;
+            e3[e] = b * 4 + g;
+            e3p[e] = (b + W2) * 4 + g;
+            e3m[e] = (b - W2) * 4 + g;
             o3[e] = (t0 + i) * Nx + x0 + jx;
 #pragma unroll
-            for (int p = 1; p <= 4; ++p) lk3[e][p - 1] = hop_link(L, Nt, Nx, V4, p, t0 + i, x0 + jx);
+            for (int p = 1; p <= 4; ++p) lk3[e][p - 1] = hop_u(L, Nt, Nx, V4, p, t0 + i, x0 + jx);
             win3[e][0] = win3[e][1] = make_uint2(0u, 0u);
         }
-        int b1[kFuL1Tiles], o1[kFuL1Tiles];
-        bool v1[kFuL1Tiles];
-        uint2 lk1[kFuL1Tiles][2];
-#pragma unroll
-        for (int e = 0; e < kFuL1Tiles; ++e) {
-            const int k = wr * kFuL1Tiles + e;
-            const int sg = min(16 * k + m, n1 - 1);
-            v1[e] = 16 * k + m < n1;
-            const int i = div_small(sg, inv1), jx = sg - i * W1;
-            b1[e] = (i + 1) * W0 + jx + 1;
-            o1[e] = sg;
-            const int t = wrap_t(t0 - 2 + i), x = wrap_x(x0 - 2 + jx);
-            lk1[e][0] = hop_link(L, Nt, Nx, V4, g, t, x);      // point g (0: none)
-            lk1[e][1] = hop_link(L, Nt, Nx, V4, 4 + g, t, x);  // point 4 + g (4: -x; 5, 6: none)
-        }
         const int nt3 = __builtin_amdgcn_readfirstlane(min(kFuL3Tiles, ((n3 + 15) >> 4) - wr * kFuL3Tiles));
-        const int nt1 = __builtin_amdgcn_readfirstlane(min(kFuL1Tiles, ((n1 + 15) >> 4) - wr * kFuL1Tiles));
-        auto step = [&](int j, float4& in_new, uint4& w_new, const float4& in_old, const uint4& w_old) {
-            const int s1 = j - 5, s3 = j - 9;
-            if (s1 >= -2 && s1 <= Ls + 1) {
-                const int sc = slot(s1), sp = slot(s1 + 1), sm = slot(s1 - 1);
-                const bf16x4_t Wa0 = frag(j, 0, 0), Wa1 = frag(j, 0, 1);
-#pragma unroll
-                for (int e = 0; e < kFuL1Tiles; ++e)
-                    if (e < nt1) {
-                        const int b0 = b1[e];
-                        bf16x4_t B[2];
-#pragma unroll
-                        for (int q = 0; q < 2; ++q) {
-                            const int p = 4 * q + g;
-                            const int nb = p == 1 ? b0 + W0 : p == 2 ? b0 - W0 : p == 3 ? b0 + 1 : p == 4 ? b0 - 1 : b0;
-                            const int ss = p == 5 ? sp : p == 6 ? sm : sc;
-                            const uint2 v = ring0[ss][nb];
-                            B[q] = p < kPts ? pack_pairs(transport(v.x, lk1[e][q]), transport(v.y, lk1[e][q]))
-                                            : bf16x4_t{0, 0, 0, 0};
-                        }
-                        f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wa0, B[0], acc, 0, 0, 0);
-                        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wa1, B[1], acc, 0, 0, 0);
-                        if (v1[e]) ring1[sc][o1[e]][g] = make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
-                    }
-            }
+        auto step = [&](auto Kc, int jv, float4& in_new, uint4& w_new, const float4& in_old, const uint4& w_old) {
+            constexpr int K = decltype(Kc)::value;
+            const int j = __builtin_amdgcn_readfirstlane(jv);  // keep the walk's arithmetic scalar
+            const int s3 = j - 9;
+            layer1(Kc, j);
             if (s3 >= 0 && s3 <= Ls - 1) {
 #pragma unroll
                 for (int e = 0; e < kFuL3Tiles; ++e)
                     if (e < nt3) {
-                        const f32x4_t acc = hop8(ring2, j, 2, s3, b3[e], W2, lk3[e], win3[e], s3 == 0);
+                        const f32x4_t acc = hop8(Kc, IC<2>{}, IC<(K + 3) & 3>{}, IC<K & 3>{}, IC<(K + 1) & 3>{},
+                                                 ring2, e3[e], e3p[e], e3m[e], lk3[e], win3[e], s3 == 0);
                         if (v3[e]) st_spinor(outr, (long long)s3 * V4 + o3[e], acc[0], acc[1], acc[2], acc[3]);
                     }
             }
             loads(j, in_new, w_new);
-            stores(j, in_old, w_old);
+            stores(Kc, j, in_old, w_old);
         };
         float4 inX = make_float4(0.f, 0.f, 0.f, 0.f), inY = inX;
         uint4 wX = make_uint4(0u, 0u, 0u, 0u), wY = wX;
-        for (int j = -3; j < jend; j += 2) {
-            step(j, inX, wX, inY, wY);
-            step(j + 1, inY, wY, inX, wX);
+        for (int j = -3; j < jend; j += 4) {
+            step(IC<0>{}, j, inX, wX, inY, wY);
+            step(IC<1>{}, j + 1, inY, wY, inX, wX);
+            step(IC<2>{}, j + 2, inX, wX, inY, wY);
+            step(IC<3>{}, j + 3, inY, wY, inX, wX);
         }
     }
     if constexpr (DOTS) {
@@ -1081,39 +1174,6 @@ __global__ void k_fcg_alpha(const double* hpart, int hnb, CgState* st, FcgScal* 
             fs[rhs].alpha = fs[rhs].rz / pq;
         }
     }
-}
-
-// x += alpha p; r -= alpha q; partial |r|^2
-template <typename R>
-__global__ __launch_bounds__(256) void k_fcg_xr(R* x, R* r, const R* p, const R* q, long long sites,
-                                                const CgState* st, const FcgScal* fs, double* part)
-{
-    const int rhs = blockIdx.y;
-    if (st[rhs].flags[0] & kSkip) return;
-    using T4 = typename V4R<R>::T;
-    const R al = (R)fs[rhs].alpha;
-    const long long o = (long long)rhs * sites;
-    T4* X = reinterpret_cast<T4*>(x) + o;
-    T4* Rr = reinterpret_cast<T4*>(r) + o;
-    const T4* P = reinterpret_cast<const T4*>(p) + o;
-    const T4* Q = reinterpret_cast<const T4*>(q) + o;
-    double a = 0.0;
-    for_sites(sites, [&](long long i) {
-        const T4 pp = P[i], qq = Q[i];
-        T4 xx = X[i], rr = Rr[i];
-        xx.x += al * pp.x;
-        xx.y += al * pp.y;
-        xx.z += al * pp.z;
-        xx.w += al * pp.w;
-        rr.x -= al * qq.x;
-        rr.y -= al * qq.y;
-        rr.z -= al * qq.z;
-        rr.w -= al * qq.w;
-        X[i] = xx;
-        Rr[i] = rr;
-        a += dotr(rr, rr);
-    });
-    block_part2(a, 0.0, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
 }
 
 // |r|^2 of the cycle: freeze for a reliable update when it met the tolerance or fell by
@@ -1769,6 +1829,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     HopArgs<float> ha = hop_args<float>(c, nrhs, nullptr);
     const int hnb = ha.cfg.nblk();
     double* hpart = (double*)c->buf(B_PA, (size_t)nrhs * hnb * 2 * 8);
+    double* rpart = (double*)c->buf(B_PB, (size_t)nrhs * hnb * 2 * 8);
     ha.st = st;
     ha.slot = 0;
     ha.skip = kSkip;
@@ -1800,13 +1861,20 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         a.out = t32;
         a.part = hpart;
         launch_hop<float>(false, EPI_NORM, a, nrhs, s);
+        // (p, A p) and the breakdown check, then q = D^dag t with x += alpha p,
+        // r -= alpha q and |r|^2 in the stencil's epilogue (alpha = (z, r) / (p, A p))
+        k_fcg_alpha<<<nrhs, 64, 0, s>>>(hpart, hnb, st, fs);
         a.in = t32;
         a.out = q32;
-        a.part = nullptr;
-        launch_hop<float>(true, EPI_STORE, a, nrhs, s);
-        k_fcg_alpha<<<nrhs, 64, 0, s>>>(hpart, hnb, st, fs);
-        k_fcg_xr<float><<<vgrid, 256, 0, s>>>(x32, r32, p32, q32, sites, st, fs, part);
-        k_fcg_conv<<<nrhs, 64, 0, s>>>(part, nb, st);
+        a.x = x32;
+        a.r = r32;
+        a.p = p32;
+        a.partA = hpart;
+        a.anum = &fs[0].rz;
+        a.anum_stride = (int)(sizeof(FcgScal) / sizeof(double));
+        a.part = rpart;
+        launch_hop<float>(true, EPI_CGUPD, a, nrhs, s);
+        k_fcg_conv<<<nrhs, 64, 0, s>>>(rpart, hnb, st);
         // z = M r with (z, r) and (z, q) accumulated by the fused network kernel, or
         // layer by layer and a separate dot-product pass
         if (launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, q32, npart)) {
@@ -1825,7 +1893,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         for (const void* ptr : {(const void*)r32, (const void*)p32, (const void*)q32, (const void*)z32,
                                 (const void*)t32, (const void*)x32, (const void*)h1, (const void*)h2,
                                 (const void*)part, (const void*)hpart, (const void*)st, (const void*)fs,
-                                (const void*)npart,
+                                (const void*)npart, (const void*)rpart,
                                 (const void*)cfg, (const void*)net, (const void*)net->wf[0], (const void*)c->l32})
             key.push_back((unsigned long long)(uintptr_t)ptr);
         key.push_back((unsigned long long)ha.rhs_per_cfg);
