@@ -317,6 +317,9 @@ def run_learned(args, dwf, capi):
     nrhs = args.nrhs
     tuned = ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)
     log(f"autotuned stencil shape for {nrhs} RHS: {tuned}")
+    ctx.tune(dwf.DWF_TUNE_FCG_BETA, args.fcg_beta)
+    if args.reliable_delta:
+        ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA, args.reliable_delta)
     ctx.set_preconditioner(net)
     src = boundary_sources(nrhs, 918273 + 1)
     d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
@@ -360,7 +363,8 @@ def run_learned(args, dwf, capi):
                         f"{nrhs} boundary sources per step, FP64 true residual <= 1e-10",
             "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs,
             "network": "3 linear parallel-transport conv layers 2-8-8-2 complex channels, BF16 MFMA",
-            "solver": "flexible CG (FCG(1)) in FP32 with FP64 reliable updates, certified at 1e-10",
+            "solver": ["flexible CG (FCG(1))", "preconditioned CG (Fletcher-Reeves beta)"][args.fcg_beta] +
+                      " in FP32 with FP64 reliable updates, certified at 1e-10",
             "loss": ["||M b - x||^2/||x||^2", "||A M b - b||^2/||b||^2"][args.learn_loss],
             "training_steps": args.learn_steps, "batch": args.learn_batch, "lr": args.learn_lr,
         },
@@ -468,6 +472,10 @@ def main():
     ap.add_argument("--learn-batch", type=int, default=16)
     ap.add_argument("--learn-loss", type=int, default=1, choices=(0, 1))
     ap.add_argument("--learn-lr", type=float, default=3e-3)
+    ap.add_argument("--reliable-delta", type=float, default=None,
+                    help="learned solver: residual drop between FP64 reliable updates (library default 0.01)")
+    ap.add_argument("--fcg-beta", type=int, default=0, choices=(0, 1),
+                    help="learned solver direction update: 0 flexible CG (FCG(1)), 1 Fletcher-Reeves")
     ap.add_argument("--analyzer-configs", type=int, default=32, help="configurations per lattice size")
     ap.add_argument("--tune-cache", default=None, help="autotuner result file (reused when present)")
     ap.add_argument("--no-graphs", action="store_true",

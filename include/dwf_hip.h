@@ -113,7 +113,10 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * (with DWF_TUNE_TILE_X in {8, 16, 32, 64} dividing Nx and sites/width dividing Nt;
  * ignored otherwise); DWF_TUNE_NET_FUSED: 0/1 (default 1) run the learned
  * preconditioner's three layers as one fused kernel (LDS-resident hidden layers) instead
- * of layer by layer; both give bit-identical results. */
+ * of layer by layer; both give bit-identical results; DWF_TUNE_FCG_BETA: the direction
+ * update of DWF_SOLVE_LEARNED, 0 (default) flexible CG's beta = -(z, A p) / (p, A p)
+ * (Notay's FCG(1), for any preconditioner), 1 Fletcher-Reeves beta = (z, r) / (z, r)_prev
+ * (exact for a symmetric preconditioner; one vector pass less per iteration). */
 enum dwf_tune_param {
     DWF_TUNE_SCHUNKS = 0,
     DWF_TUNE_XCD_REMAP = 1,
@@ -123,7 +126,8 @@ enum dwf_tune_param {
     DWF_TUNE_NT_STORE = 5,
     DWF_TUNE_GRAPHS = 6,
     DWF_TUNE_LDS = 7,
-    DWF_TUNE_NET_FUSED = 8
+    DWF_TUNE_NET_FUSED = 8,
+    DWF_TUNE_FCG_BETA = 9
 };
 int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 

@@ -165,6 +165,33 @@ def test_learned_solve_certified_untrained():
         assert np.linalg.norm(lhs) / np.linalg.norm(rhs) <= 1.01e-10
 
 
+def test_learned_solve_fletcher_reeves():
+    """The Fletcher-Reeves direction update (no q = A p pass) converges and is certified
+    with the untrained network (a multiple of the identity, symmetric), with CG's
+    iteration count within 2x."""
+    Nt, Nx, Ls = 16, 16, 8
+    print_case(test="learned_solve_fr", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(13)
+    th = rand_theta(rng, Nt, Nx)
+    src = boundary_sources(rng, 3, Nt, Nx, Ls)
+    net = dwf.Network(Ls, seed=3)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.set_preconditioner(net)
+        ctx.tune(dwf.DWF_TUNE_FCG_BETA, 1)
+        x, st, _ = ctx.solve_dirac(src, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_LEARNED)
+        _, stc, _ = ctx.solve_dirac(src, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_CG64)
+        with pytest.raises(dwf.DwfError):
+            ctx.tune(dwf.DWF_TUNE_FCG_BETA, 2)
+    for r in range(3):
+        assert st[r]["converged"] and st[r]["true_relres"] <= 1e-10 and st[r]["fp64_iterations"] == 0
+        assert st[r]["iterations"] <= 2 * stc[r]["iterations"]
+        Dx = oracle.dwf_apply(0, th, x[r], Nt, Nx, Ls, M5, MF)
+        lhs = oracle.dwf_apply(1, th, src[r] - Dx, Nt, Nx, Ls, M5, MF)
+        rhs = oracle.dwf_apply(1, th, src[r], Nt, Nx, Ls, M5, MF)
+        assert np.linalg.norm(lhs) / np.linalg.norm(rhs) <= 1.01e-10
+
+
 def test_learned_solve_fused_and_layers():
     """FCG with the fused network (dot products in its epilogue) and with the layer
     kernels (separate dot-product pass): both certified, iteration counts within 2

@@ -48,6 +48,7 @@ DWF_TUNE_NT_STORE = 5
 DWF_TUNE_GRAPHS = 6
 DWF_TUNE_LDS = 7
 DWF_TUNE_NET_FUSED = 8
+DWF_TUNE_FCG_BETA = 9
 
 
 class DwfError(RuntimeError):

@@ -115,6 +115,7 @@ struct dwf_ctx {
     int nt_store = 0;             // streaming stores of stencil outputs
     int lds = 0;                  // LDS-staged stencil tiles
     int net_fused = 1;            // learned preconditioner: fused three-layer kernel when the shape allows
+    int fcg_beta = 0;             // learned solver: 0 flexible CG (Notay's FCG(1)), 1 Fletcher-Reeves
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs
     const dwf_net* net = nullptr; // learned preconditioner of DWF_SOLVE_LEARNED (not owned)

@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
     R alpha = 0;
     if constexpr (EPI == EPI_CGUPD) {
         const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
-        if (pAp == 0.0) return;  // block-uniform; the p-update kernel records the failure
+        if (pAp == 0.0 || (a.anum && !(pAp > 0.0))) return;  // block-uniform; a later kernel records the failure
         alpha = (R)((a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp);
     }
 
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     R alpha = 0;
     if constexpr (EPI == EPI_CGUPD) {
         const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
-        if (pAp == 0.0) return;  // block-uniform; the p-update kernel records the failure
+        if (pAp == 0.0 || (a.anum && !(pAp > 0.0))) return;  // block-uniform; a later kernel records the failure
         alpha = (R)((a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp);
     }
 

@@ -330,6 +330,10 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
         case DWF_TUNE_NET_FUSED:
             ctx->net_fused = value != 0.0 ? 1 : 0;
             return DWF_OK;
+        case DWF_TUNE_FCG_BETA:
+            if (!(value == 0.0 || value == 1.0)) break;
+            ctx->fcg_beta = (int)value;
+            return DWF_OK;
         default:
             break;
     }

@@ -608,6 +608,12 @@ static_assert(4 * kFuL2Tiles * 16 >= kFuR2 && 4 * kFuL3Tiles * 16 >= kFuTy * kFu
                   kFuWaves * kFuL1Tiles * 16 >= kFuR1,
               "the waves' tiles must cover every region of the largest output tile");
 
+// Element of channel group g (channels 2g, 2g+1) of a hidden-layer site in an LDS ring
+// slot.  A half-wave of an MFMA operand read covers 16 sites x groups {0, 1} (or {2, 3});
+// with 32-byte sites, sites m and m + 8 would share banks, so the group index is XOR-ed
+// with 2 * bit 3 of the site: 16 consecutive sites then cover the 64 banks once.
+__device__ __forceinline__ int hel(int site, int g) { return site * 4 + (g ^ ((site >> 2) & 2)); }
+
 struct NetFusedArgs {
     int Nt, Nx, Ls, V4;
     int TY, TX;            // output tile, dividing Nt and Nx
@@ -622,6 +628,9 @@ struct NetFusedArgs {
     const float* q;        // DOTS: q = A p of the flexible CG
     double* part;          // DOTS: [nrhs][tiles][2] partials of (z, r), (z, q)
 };
+
+// s_waitcnt immediate (gfx9 encoding): vmcnt(0), expcnt and lgkmcnt unconstrained
+constexpr int kWaitVmcnt0 = 0x0F70;
 
 // i = k / w for 0 <= k < 2^16 / w (w <= 22 here): one multiply and a shift
 __device__ __forceinline__ int div_small(int k, unsigned inv) { return (int)(((unsigned)k * inv) >> 16); }
@@ -740,12 +749,12 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     };
 
     // One 16-site tile of a C_in = 8 layer: rin the input ring, SM / S0 / SP the slots of
-    // slices s-1, s, s+1, e0 the lane's element (site * 4 + g) in the input region and
-    // etp / etm those of its +t / -t neighbours (the +-x ones are e0 -+ 4), u the
-    // transporters at the output site; win[0], win[1] the values at s-1 and s (loaded
-    // here when `first`), advanced to s, s+1.
+    // slices s-1, s, s+1, e0 the lane's element (hel) in the input region and etp / etm /
+    // exp / exm those of its +t / -t / +x / -x neighbours, u the transporters at the
+    // output site; win[0], win[1] the values at s-1 and s (loaded here when `first`),
+    // advanced to s, s+1.
     auto hop8 = [&](auto Kc, auto Lc, auto SMc, auto S0c, auto SPc, const auto& rin, int e0, int etp, int etm,
-                    const float2 (&u)[4], uint2 (&win)[2], bool first) {
+                    int exp, int exm, const float2 (&u)[4], uint2 (&win)[2], bool first) {
         constexpr int SM = decltype(SMc)::value, S0 = decltype(S0c)::value, SP = decltype(SPc)::value;
         const uint2* r0 = &rin[SM][0][0];
         const uint2* rs = &rin[S0][0][0];
@@ -756,7 +765,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         }
         bf16x4_t h[7], W[7];
         const uint2 up = rp[e0];
-        const uint2 vtp = rs[etp], vtm = rs[etm], vxp = rs[e0 + 4], vxm = rs[e0 - 4];
+        const uint2 vtp = rs[etp], vtm = rs[etm], vxp = rs[exp], vxm = rs[exm];
         h[0] = pack_pairs(win[1].x, win[1].y);
         h[1] = pack_pairs(vtp.x, vtp.y);
         h[2] = pack_pairs(vtm.x, vtm.y);
@@ -787,7 +796,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         const int b0 = (i + 1) * W0 + jx + 1;
         e1a[e] = g == 1 ? b0 + W0 : (g == 2 ? b0 - W0 : (g == 3 ? b0 + 1 : b0));
         e1b[e] = g == 0 ? b0 - 1 : b0;
-        o1[e] = sg * 4 + g;
+        o1[e] = hel(sg, g);
         const int t = wrap_t(t0 - 2 + i), x = wrap_x(x0 - 2 + jx);
         lk1[e][0] = hop_link(L, Nt, Nx, V4, g, t, x);  // point g (0: none)
         lk1[e][1] = g == 3 ? make_uint2(0u, 0u) : hop_link(L, Nt, Nx, V4, 4 + g, t, x);  // -x, none, none, zero
@@ -817,7 +826,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
 
     if (wv < 4) {
         // ---- layer 2: R2 tiles wv*3 + e (origin t0-1, x0-1), input ring1 (width W1)
-        int e0[kFuL2Tiles], etp[kFuL2Tiles], etm[kFuL2Tiles], eo[kFuL2Tiles];
+        int e0[kFuL2Tiles], etp[kFuL2Tiles], etm[kFuL2Tiles], exp[kFuL2Tiles], exm[kFuL2Tiles], eo[kFuL2Tiles];
         bool valid[kFuL2Tiles];
         float2 lk[kFuL2Tiles][4];
         uint2 win[kFuL2Tiles][2];
@@ -828,10 +837,12 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             valid[e] = 16 * k + m < n2;
             const int i = div_small(sg, inv2), jx = sg - i * W2;
             const int b = (i + 1) * W1 + jx + 1;
-            e0[e] = b * 4 + g;
-            etp[e] = (b + W1) * 4 + g;
-            etm[e] = (b - W1) * 4 + g;
-            eo[e] = sg * 4 + g;
+            e0[e] = hel(b, g);
+            etp[e] = hel(b + W1, g);
+            etm[e] = hel(b - W1, g);
+            exp[e] = hel(b + 1, g);
+            exm[e] = hel(b - 1, g);
+            eo[e] = hel(sg, g);
             const int t = wrap_t(t0 - 1 + i), x = wrap_x(x0 - 1 + jx);
 #pragma unroll
             for (int p = 1; p <= 4; ++p) lk[e][p - 1] = hop_u(L, Nt, Nx, V4, p, t, x);
@@ -848,7 +859,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
                 for (int e = 0; e < kFuL2Tiles; ++e)
                     if (e < nt) {
                         const f32x4_t acc = hop8(Kc, IC<1>{}, IC<(K + 1) & 3>{}, IC<(K + 2) & 3>{},
-                                                 IC<(K + 3) & 3>{}, ring1, e0[e], etp[e], etm[e], lk[e], win[e],
+                                                 IC<(K + 3) & 3>{}, ring1, e0[e], etp[e], etm[e], exp[e], exm[e], lk[e], win[e],
                                                  s2 == -1);
                         if (valid[e])
                             (&ring2[(K + 2) & 3][0][0])[eo[e]] =
@@ -860,6 +871,9 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         };
         float4 inX = make_float4(0.f, 0.f, 0.f, 0.f), inY = inX;
         uint4 wX = make_uint4(0u, 0u, 0u, 0u), wY = wX;
+        // the set-up's global loads (transporters) are complete before the walk, so no
+        // wait inside it is for them (such a wait would also wait for the input in flight)
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         for (int j = -3; j < jend; j += 4) {
             step(IC<0>{}, j, inX, wX, inY, wY);
             step(IC<1>{}, j + 1, inY, wY, inX, wX);
@@ -869,7 +883,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     } else {
         // ---- output layer: tiles (wv-4)*2 + e of the TY x TX tile, input ring2 (width W2)
         const int wr = wv - 4;
-        int e3[kFuL3Tiles], e3p[kFuL3Tiles], e3m[kFuL3Tiles], o3[kFuL3Tiles];
+        int e3[kFuL3Tiles], e3p[kFuL3Tiles], e3m[kFuL3Tiles], e3xp[kFuL3Tiles], e3xm[kFuL3Tiles], o3[kFuL3Tiles];
         bool v3[kFuL3Tiles];
         float2 lk3[kFuL3Tiles][4];
         uint2 win3[kFuL3Tiles][2];
@@ -880,9 +894,11 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             v3[e] = 16 * k + m < n3 && g == 0;
             const int i = div_small(sg, inv3), jx = sg - i * TX;
             const int b = (i + 1) * W2 + jx + 1;
-            e3[e] = b * 4 + g;
-            e3p[e] = (b + W2) * 4 + g;
-            e3m[e] = (b - W2) * 4 + g;
+            e3[e] = hel(b, g);
+            e3p[e] = hel(b + W2, g);
+            e3m[e] = hel(b - W2, g);
+            e3xp[e] = hel(b + 1, g);
+            e3xm[e] = hel(b - 1, g);
             o3[e] = (t0 + i) * Nx + x0 + jx;
 #pragma unroll
             for (int p = 1; p <= 4; ++p) lk3[e][p - 1] = hop_u(L, Nt, Nx, V4, p, t0 + i, x0 + jx);
@@ -899,7 +915,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
                 for (int e = 0; e < kFuL3Tiles; ++e)
                     if (e < nt3) {
                         const f32x4_t acc = hop8(Kc, IC<2>{}, IC<(K + 3) & 3>{}, IC<K & 3>{}, IC<(K + 1) & 3>{},
-                                                 ring2, e3[e], e3p[e], e3m[e], lk3[e], win3[e], s3 == 0);
+                                                 ring2, e3[e], e3p[e], e3m[e], e3xp[e], e3xm[e], lk3[e], win3[e], s3 == 0);
                         if (v3[e]) st_spinor(outr, (long long)s3 * V4 + o3[e], acc[0], acc[1], acc[2], acc[3]);
                     }
             }
@@ -908,6 +924,9 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         };
         float4 inX = make_float4(0.f, 0.f, 0.f, 0.f), inY = inX;
         uint4 wX = make_uint4(0u, 0u, 0u, 0u), wY = wX;
+        // the set-up's global loads (transporters) are complete before the walk, so no
+        // wait inside it is for them (such a wait would also wait for the input in flight)
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
         for (int j = -3; j < jend; j += 4) {
             step(IC<0>{}, j, inX, wX, inY, wY);
             step(IC<1>{}, j + 1, inY, wY, inX, wX);
@@ -916,18 +935,21 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         }
     }
     if constexpr (DOTS) {
-        // (z, r) and (z, q) over the block's output (z written above by this block)
+        // (z, r) and, with q, (z, q) over the block's output (z written above by this block)
         const float4* Z = reinterpret_cast<const float4*>(outr);
         const float4* Rv = reinterpret_cast<const float4*>(inr);
-        const float4* Q = reinterpret_cast<const float4*>(a.q) + (long long)rhs * Ls * V4;
+        const float4* Q = a.q ? reinterpret_cast<const float4*>(a.q) + (long long)rhs * Ls * V4 : nullptr;
         double dzr = 0.0, dzq = 0.0;
         for (int e = tid; e < n3 * Ls; e += kFuThreads) {
             const int s = e / n3, k = e - s * n3;
             const int i = div_small(k, inv3), jx = k - i * TX;
             const long long site = (long long)s * V4 + (t0 + i) * Nx + x0 + jx;
-            const float4 z = Z[site], r = Rv[site], qv = Q[site];
+            const float4 z = Z[site], r = Rv[site];
             dzr += (double)z.x * r.x + (double)z.y * r.y + (double)z.z * r.z + (double)z.w * r.w;
-            dzq += (double)z.x * qv.x + (double)z.y * qv.y + (double)z.z * qv.z + (double)z.w * qv.w;
+            if (Q) {
+                const float4 qv = Q[site];
+                dzq += (double)z.x * qv.x + (double)z.y * qv.y + (double)z.z * qv.z + (double)z.w * qv.w;
+            }
         }
         dzr = wsum(dzr);
         dzq = wsum(dzq);
@@ -1031,6 +1053,7 @@ __global__ void k_loss_fold(const double* part, int nb, const int* row, const do
 // ------------------------------------------- flexible CG (FP32 + FP64 reliable updates)
 struct FcgScal {
     double rz, pq, alpha, beta;
+    double rz_prev;  // (z, r) of the previous iteration (Fletcher-Reeves beta)
 };
 
 template <typename R> struct V4R;
@@ -1159,31 +1182,26 @@ __global__ void k_fcg_rz(const double* part, int nb, CgState* st, FcgScal* fs)
     }
 }
 
-// alpha = (z, r) / (p, A p), (p, A p) = |D p|^2 from the stencil's partials
-__global__ void k_fcg_alpha(const double* hpart, int hnb, CgState* st, FcgScal* fs)
+// After the update stencil: (p, A p) from the first stencil's partials (a breakdown if
+// not positive: the update stencil skipped the right-hand side), then |r|^2 from the
+// update stencil's partials: freeze the right-hand side for a reliable update when it
+// met the tolerance or fell by delta since the last reliable update.
+__global__ void k_fcg_check(const double* hpart, const double* rpart, int nb, CgState* st, FcgScal* fs)
 {
     const int rhs = blockIdx.x;
     if (st[rhs].flags[0] & kSkip) return;
-    const double pq = fold64(hpart + (long long)rhs * hnb * 2, hnb, 0);
-    if (threadIdx.x == 0) {
-        fs[rhs].pq = pq;
-        if (!(pq > 0.0)) {
+    const double pq = fold64(hpart + (long long)rhs * nb * 2, nb, 0);
+    if (!(pq > 0.0)) {
+        if (threadIdx.x == 0) {
             st[rhs].status = 4;
             st[rhs].flags[0] |= F_DONE;
-        } else {
-            fs[rhs].alpha = fs[rhs].rz / pq;
         }
+        return;
     }
-}
-
-// |r|^2 of the cycle: freeze for a reliable update when it met the tolerance or fell by
-// delta since the last reliable update
-__global__ void k_fcg_conv(const double* part, int nb, CgState* st)
-{
-    const int rhs = blockIdx.x;
-    if (st[rhs].flags[0] & kSkip) return;
-    const double rr = fold64(part + (long long)rhs * nb * 2, nb, 0);
+    const double rr = fold64(rpart + (long long)rhs * nb * 2, nb, 0);
     if (threadIdx.x == 0) {
+        fs[rhs].pq = pq;
+        fs[rhs].rz_prev = fs[rhs].rz;  // consumed by this iteration's alpha
         CgState& S = st[rhs];
         S.rr[0] = rr;
         S.iters += 1;
@@ -1213,33 +1231,41 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const R* z, const R* r, const 
     block_part2(a, c, part + ((long long)rhs * gridDim.x + blockIdx.x) * 2);
 }
 
-// beta = -(z, A p) / (p, A p) (A-orthogonal to the previous direction); rz = (z, r)
-__global__ void k_fcg_beta(const double* part, int nb, CgState* st, FcgScal* fs)
-{
-    const int rhs = blockIdx.x;
-    if (st[rhs].flags[0] & kSkip) return;
-    const double zr = fold64(part + (long long)rhs * nb * 2, nb, 0);
-    const double zq = fold64(part + (long long)rhs * nb * 2, nb, 1);
-    if (threadIdx.x == 0) {
-        if (!(zr > 0.0)) {
-            st[rhs].status = 5;
-            st[rhs].flags[0] |= F_DONE;
-        } else {
-            fs[rhs].rz = zr;
-            fs[rhs].beta = -zq / fs[rhs].pq;
-        }
-    }
-}
-
-// p = z + beta p
+// beta = -(z, A p) / (p, A p) (flexible CG, Notay's FCG(1)) or, Fletcher-Reeves, the
+// ratio of the new and previous (z, r), from the (z, r), (z, q) partials folded by every
+// block in the same order; then p = z + beta p.  Block 0 records rz = (z, r) for the next alpha
+// (read by the next update stencil) or the breakdown when (z, r) is not positive.
 template <typename R>
-__global__ __launch_bounds__(256) void k_fcg_p(const R* z, R* p, long long sites, const CgState* st,
-                                               const FcgScal* fs)
+__global__ __launch_bounds__(256) void k_fcg_pbeta(const R* z, R* p, long long sites, CgState* st, FcgScal* fs,
+                                                   const double* part, int nb, int fletcher_reeves)
 {
     const int rhs = blockIdx.y;
     if (st[rhs].flags[0] & kSkip) return;
+    __shared__ double sh[2];
+    if (threadIdx.x < 64) {
+        const double zr = fold64(part + (long long)rhs * nb * 2, nb, 0);
+        const double zq = fold64(part + (long long)rhs * nb * 2, nb, 1);
+        if (threadIdx.x == 0) {
+            sh[0] = zr;
+            sh[1] = zq;
+        }
+    }
+    __syncthreads();
+    const double zr = sh[0], zq = sh[1];
+    if (!(zr > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st[rhs].status = 5;
+            st[rhs].flags[0] |= F_DONE;
+        }
+        return;
+    }
+    const double beta = fletcher_reeves ? zr / fs[rhs].rz_prev : -zq / fs[rhs].pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        fs[rhs].rz = zr;
+        fs[rhs].beta = beta;
+    }
     using T4 = typename V4R<R>::T;
-    const R be = (R)fs[rhs].beta;
+    const R be = (R)beta;
     const long long o = (long long)rhs * sites;
     const T4* Z = reinterpret_cast<const T4*>(z) + o;
     T4* P = reinterpret_cast<T4*>(p) + o;
@@ -1366,8 +1392,8 @@ bool fused_shape(const Lat& L, int& TY, int& TX)
     return true;
 }
 
-// Fused z = M in (k_net_fused); with q and part also the (z, in), (z, q) partials, part
-// [nrhs][tiles][2].  Returns the number of tiles per lattice, or 0 if the shape does not
+// Fused z = M in (k_net_fused); with part also the (z, in) and, with q, (z, q) partials,
+// part [nrhs][tiles][2].  Returns the number of tiles per lattice, or 0 if the shape does not
 // allow the fused kernel (then nothing is launched).
 int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind, void* out, int out_kind,
                      const int* cfg, int nrhs, const CgState* st, const float* q, double* part)
@@ -1395,7 +1421,7 @@ int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind
     a.part = part;
     const unsigned grid = (unsigned)(a.tiles * nrhs);
     hipStream_t s = c->stream;
-    if (q && part) {
+    if (part) {
         if (in_kind != 1 || out_kind != 1) throw DwfError(DWF_ERR_ARG, "fused network: dot products need FP32 spinors");
         k_net_fused<float, float, true><<<grid, kFuThreads, 0, s>>>(a);
     } else if (in_kind == 0 && out_kind == 0) {
@@ -1834,6 +1860,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     ha.slot = 0;
     ha.skip = kSkip;
     const dim3 vgrid(nb, nrhs);
+    const int fr = c->fcg_beta;
     int fty = 0, ftx = 0;
     const int ntiles = fused_shape(c->L, fty, ftx) ? (c->L.Nt / fty) * (c->L.Nx / ftx) : 1;
     double* npart = (double*)c->buf(B_FCG_NPART, (size_t)nrhs * ntiles * 2 * 8);
@@ -1861,11 +1888,10 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         a.out = t32;
         a.part = hpart;
         launch_hop<float>(false, EPI_NORM, a, nrhs, s);
-        // (p, A p) and the breakdown check, then q = D^dag t with x += alpha p,
-        // r -= alpha q and |r|^2 in the stencil's epilogue (alpha = (z, r) / (p, A p))
-        k_fcg_alpha<<<nrhs, 64, 0, s>>>(hpart, hnb, st, fs);
+        // q = D^dag t with x += alpha p, r -= alpha q and |r|^2 in the stencil's epilogue
+        // (alpha = (z, r) / (p, A p)), then the breakdown and convergence checks
         a.in = t32;
-        a.out = q32;
+        a.out = fr ? nullptr : q32;  // Fletcher-Reeves beta needs no q
         a.x = x32;
         a.r = r32;
         a.p = p32;
@@ -1874,17 +1900,16 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         a.anum_stride = (int)(sizeof(FcgScal) / sizeof(double));
         a.part = rpart;
         launch_hop<float>(true, EPI_CGUPD, a, nrhs, s);
-        k_fcg_conv<<<nrhs, 64, 0, s>>>(rpart, hnb, st);
+        k_fcg_check<<<nrhs, 64, 0, s>>>(hpart, rpart, hnb, st, fs);
         // z = M r with (z, r) and (z, q) accumulated by the fused network kernel, or
-        // layer by layer and a separate dot-product pass
-        if (launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, q32, npart)) {
-            k_fcg_beta<<<nrhs, 64, 0, s>>>(npart, ntiles, st, fs);
+        // layer by layer and a separate dot-product pass; then beta and p = z + beta p
+        if (launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, fr ? nullptr : q32, npart)) {
+            k_fcg_pbeta<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs, npart, ntiles, fr);
         } else {
             net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
             k_fcg_dots<float><<<vgrid, 256, 0, s>>>(z32, r32, q32, sites, st, part);
-            k_fcg_beta<<<nrhs, 64, 0, s>>>(part, nb, st, fs);
+            k_fcg_pbeta<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs, part, nb, fr);
         }
-        k_fcg_p<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs);
     };
     constexpr int kBlockIters = 8;
     hipGraphExec_t graph = nullptr;
@@ -1898,6 +1923,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
             key.push_back((unsigned long long)(uintptr_t)ptr);
         key.push_back((unsigned long long)ha.rhs_per_cfg);
         key.push_back((unsigned long long)c->net_fused);
+        key.push_back((unsigned long long)fr);
         const int v[] = {ha.cfg.bx, ha.cfg.chunks, ha.cfg.spt, ha.cfg.remap, ha.cfg.prefetch, ha.cfg.tx,
                          ha.cfg.tiles_x, ha.cfg.nt_store, ha.cfg.lds};
         for (int e : v) key.push_back((unsigned long long)(unsigned)e);
