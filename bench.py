@@ -92,6 +92,19 @@ def time_kernel(ctx, capi, launch, reps):
     return t.elapsed_ms() / reps
 
 
+def apply_shape(ctx, dwf, shape):
+    """Sets the context's stencil launch shape to an autotuner result."""
+    ctx.tune(dwf.DWF_TUNE_TILE_X, shape["tile_x"])
+    ctx.tune(dwf.DWF_TUNE_PREFETCH, shape["prefetch"])
+    ctx.tune(dwf.DWF_TUNE_SCHUNKS, shape["chunks"])
+    ctx.tune(dwf.DWF_TUNE_NT_STORE, shape["nt_store"])
+    ctx.tune(dwf.DWF_TUNE_LDS, shape["lds"])
+
+
+# Threads per block of each LDS tile mode of the stencil (DWF_TUNE_LDS 1..7).
+LDS_THREADS = {1: 256, 2: 512, 3: 64, 4: 128, 5: 384, 6: 768, 7: 1024}
+
+
 def time_dslash(ctx, dwf, capi, prec, nrhs, reps):
     n = nrhs * LS * NT * NX * 4
     dt = np.float32 if prec == dwf.DWF_PREC_FP32 else np.float64
@@ -104,7 +117,9 @@ def time_dslash(ctx, dwf, capi, prec, nrhs, reps):
 
 
 def time_copy(ctx, capi, nbytes, reps):
-    src = capi.DeviceArray(nbytes)
+    """Streaming copy of random data (HBM moves zeros measurably faster than random
+    bits, and the stencil is timed on random data too)."""
+    src = capi.DeviceArray.from_host(np.random.default_rng(78).normal(size=nbytes // 4).astype(np.float32))
     dst = capi.DeviceArray(nbytes)
     ms = time_kernel(ctx, capi, lambda: ctx.copy_device(dst.ptr, src.ptr, nbytes), reps)
     return ms, 2 * nbytes / (ms * 1e-3) / 1e9
@@ -343,8 +358,9 @@ def run_learned(args, dwf, capi):
     if args.no_graphs:
         ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
     nrhs = args.nrhs
+    tuned64 = ctx.autotune(nrhs, dwf.DWF_PREC_FP64, args.tune_cache)
     tuned = ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)
-    log(f"autotuned stencil shape for {nrhs} RHS: {tuned}")
+    log(f"autotuned stencil shapes for {nrhs} RHS: FP32 {tuned}, FP64 {tuned64}")
     solver = learned_solver_config(args, dwf, ctx, net)
     src = boundary_sources(nrhs, 918273)
     d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
@@ -354,6 +370,7 @@ def run_learned(args, dwf, capi):
                          ("eo_mixed", dwf.DWF_SOLVE_EO_MIXED)):
         ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA,
                  args.reliable_delta if method == dwf.DWF_SOLVE_LEARNED else LIB_RELIABLE_DELTA)
+        apply_shape(ctx, dwf, tuned64 if method == dwf.DWF_SOLVE_CG64 else tuned)
         for _ in range(args.warmup):
             ctx.solve_dirac_device(method, d_src.ptr, None, TOL, 20000, nrhs, d_x.ptr)
         ctx.synchronize()
@@ -542,8 +559,10 @@ def main():
     if args.no_graphs:
         ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
 
-    tuned = {"solves": ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)}
-    log(f"autotuned stencil shape for {nrhs} RHS: {tuned['solves']}")
+    # each solver runs with the stencil shape tuned for the precision its iterations use
+    tuned = {"solves_fp64": ctx.autotune(nrhs, dwf.DWF_PREC_FP64, args.tune_cache),
+             "solves": ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)}
+    log(f"autotuned stencil shapes for {nrhs} RHS: FP32 {tuned['solves']}, FP64 {tuned['solves_fp64']}")
     learned_solver = learned_solver_config(args, dwf, ctx, net)
     n = nrhs * LS * NT * NX * 4
     d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
@@ -558,6 +577,7 @@ def main():
                          ("mixed", dwf.DWF_SOLVE_MIXED), ("fp64_cg", dwf.DWF_SOLVE_CG64)):
         ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA,
                  args.reliable_delta if method == dwf.DWF_SOLVE_LEARNED else LIB_RELIABLE_DELTA)
+        apply_shape(ctx, dwf, tuned["solves_fp64" if method == dwf.DWF_SOLVE_CG64 else "solves"])
         steps = args.steps
         for _ in range(args.warmup):
             solve(method)
@@ -578,6 +598,7 @@ def main():
                              max_true_relres=tr)
         log(f"{name}: {results[name]}")
 
+    apply_shape(ctx, dwf, tuned["solves"])
     # the same solves through the host-array entry point (sources in and solutions out over
     # PCIe, as the reference's call sites hand over host vectors)
     host_src = src.reshape(nrhs, -1)
@@ -653,7 +674,7 @@ def main():
             "unit": "GB/s",
             "frac": round(gbs32 / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic() if args.dslash_rhs == 128 else None,
-            "kernel": (f"k_hop_lds<float,D,DWF,EPI_STORE,{256 * tuned['fp32_batch']['lds']}>"
+            "kernel": (f"k_hop_lds<float,D,DWF,EPI_STORE,{LDS_THREADS[tuned['fp32_batch']['lds']]}>"
                        if tuned["fp32_batch"]["lds"] else "k_hop<float,D,DWF,EPI_STORE>"),
             "avg_launch_us": round(ms32 * 1e3, 2),
             "bytes_per_launch": dslash_bytes(4, args.dslash_rhs),

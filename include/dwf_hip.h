@@ -106,11 +106,12 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * of the mixed solver; DWF_TUNE_PREFETCH: 0/1 software-pipelined stencil slice loop;
  * DWF_TUNE_TILE_X: 0 for blocks of consecutive 4D sites, or the width (dividing Nx and
  * 256) of 2D stencil tiles of 256/width rows; DWF_TUNE_NT_STORE: 0/1 streaming
- * (non-temporal) stores of stencil outputs; DWF_TUNE_GRAPHS: 0/1 (default 1) replay
+ * (non-temporal) stores of stencil outputs (the LDS-staged tiles always stream); DWF_TUNE_GRAPHS: 0/1 (default 1) replay
  * blocks of 8 CG iterations as instantiated HIP graphs instead of launching their
- * kernels one by one (same kernels, same order, same results); DWF_TUNE_LDS: 0, or 1 / 2
- * to stage each slice of a 2D stencil tile of 256 / 512 sites and its halo in LDS
- * (with DWF_TUNE_TILE_X in {8, 16, 32, 64} dividing Nx and sites/width dividing Nt;
+ * kernels one by one (same kernels, same order, same results); DWF_TUNE_LDS: 0, or
+ * 1 / 2 / 3 / 4 / 5 / 6 / 7 to stage each slice of a 2D stencil tile of 256 / 512 / 64 /
+ * 128 / 384 / 768 / 1024 sites and its halo in LDS (with DWF_TUNE_TILE_X between 8 and 128 dividing
+ * Nx and the tile size, sites/width dividing Nt, and 2 (width + sites/width) <= sites;
  * ignored otherwise); DWF_TUNE_NET_FUSED: 0/1 (default 1) run the learned
  * preconditioner's three layers as one fused kernel (LDS-resident hidden layers) instead
  * of layer by layer; both give bit-identical results; DWF_TUNE_FCG_BETA: the direction

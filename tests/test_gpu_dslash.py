@@ -124,12 +124,23 @@ def test_tuned_launch_shapes_keep_parity(tile_x, prefetch, chunks):
 
 
 @pytest.mark.parametrize("tile_x,chunks,mode", [(8, 0, 1), (16, 2, 1), (32, 6, 1), (64, 1, 1), (32, 1, 2),
-                                                (16, 3, 2), (64, 2, 2)])
+                                                (16, 3, 2), (64, 2, 2), (16, 0, 3), (8, 2, 3), (32, 0, 4),
+                                                (16, 1, 4), (32, 0, 7), (64, 1, 7)])
 def test_lds_tiles_keep_parity(tile_x, chunks, mode):
     """The LDS-staged stencil (full 2D tiles, one halo per slice) against the oracle for
     every operator, and inside the FP64 and mixed solvers (its NORM/CGUPD/RESID paths)."""
-    Nt, Nx, Ls = 32, 64, 6
-    print_case(test="lds_tiles", tile_x=tile_x, chunks=chunks, mode=mode)
+    lds_parity(32, 64, 6, tile_x, chunks, mode)
+
+
+@pytest.mark.parametrize("tile_x,chunks,mode", [(96, 0, 5), (96, 2, 6), (32, 1, 5), (48, 0, 5)])
+def test_lds_full_row_tiles_keep_parity(tile_x, chunks, mode):
+    """Tiles of 384 / 768 threads, among them tiles as wide as the lattice (96): the x
+    halo is the tile's own wrapped edge columns."""
+    lds_parity(24, 96, 4, tile_x, chunks, mode)
+
+
+def lds_parity(Nt, Nx, Ls, tile_x, chunks, mode):
+    print_case(test="lds_tiles", Nt=Nt, Nx=Nx, Ls=Ls, tile_x=tile_x, chunks=chunks, mode=mode)
     rng = np.random.default_rng(100 + tile_x + chunks + mode)
     th = rand_theta(rng, Nt, Nx)
     psi = rand_spinor(rng, 2 * Ls * Nt * Nx * 2).reshape(2, -1)

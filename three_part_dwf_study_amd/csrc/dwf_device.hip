@@ -24,9 +24,12 @@
 #include "dwf_internal.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace dwfk {
 namespace {
+
+template <int V> using IC = std::integral_constant<int, V>;
 
 template <typename R> struct Vec4;
 template <> struct Vec4<float> { using T = float4; };
@@ -414,18 +417,91 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
 }
 
 
-// LDS-tiled variant: a block owns a full 2D tile of (kBlock/tx) x tx sites of one
+// Raw buffer access.  A lane whose byte offset lies past the end of the buffer reads
+// zeros and writes nothing, so optional loads and stores need no branch (a branch around
+// a load makes the compiler wait for every load in flight where the paths join).
+constexpr int kBufOff = 0x7ffffff0;  // past the end of every buffer (all are < 2 GiB)
+constexpr int kRsrcWord3 = 0x00020000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, long long bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, p ? (int)bytes : 0, kRsrcWord3);
+}
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+template <typename R> __device__ __forceinline__ Sp<R> bld(__amdgpu_buffer_rsrc_t r, int off)
+{
+    if constexpr (sizeof(R) == 4) {
+        const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        return Sp<R>{__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w)};
+    } else {
+        const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16, 0, 0);
+        auto d = [](unsigned lo, unsigned hi) {
+            return __hiloint2double((int)hi, (int)lo);
+        };
+        return Sp<R>{(R)d(v.x, v.y), (R)d(v.z, v.w), (R)d(w.x, w.y), (R)d(w.z, w.w)};
+    }
+}
+// AUX 2: streaming (non-temporal) store
+template <typename R, int AUX = 0> __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, int off, const Sp<R>& s)
+{
+    if constexpr (sizeof(R) == 4) {
+        const u32x4 v = {__float_as_uint(s.ar), __float_as_uint(s.ai), __float_as_uint(s.br), __float_as_uint(s.bi)};
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, AUX);
+    } else {
+        auto lo = [](double x) { return (unsigned)__double2loint(x); };
+        auto hi = [](double x) { return (unsigned)__double2hiint(x); };
+        const u32x4 v = {lo(s.ar), hi(s.ar), lo(s.ai), hi(s.ai)};
+        const u32x4 w = {lo(s.br), hi(s.br), lo(s.bi), hi(s.bi)};
+        __builtin_amdgcn_raw_buffer_store_b128(v, r, off, 0, AUX);
+        __builtin_amdgcn_raw_buffer_store_b128(w, r, off + 16, 0, AUX);
+    }
+}
+
+// LDS-tiled variant: a block owns a full 2D tile of ty x tx sites (ty = NT / tx) of one
 // right-hand side and walks its s-chunk.  Each slice of the tile plus its one-site halo
 // is staged in LDS once (double-buffered, one barrier per slice): a site costs one
-// global load of its own spinor plus (2 tx + 2 ty)/256 halo loads instead of five
-// loads.  The fifth-dimension window stays in registers, the next slice's halo and the
-// slice after next's centre are in flight while a slice is computed, and each chunk
-// fetches its spt slices plus the two boundary slices of the fifth-dimension hop, and
-// no more; odd chunks walk downwards so that neighbouring chunks fetch their shared
-// boundary slices at the same time (the second fetch hits L2).
-// Needs Nx % tx == 0 and Nt % (kBlock/tx) == 0 (every thread owns a site).
-// (ty+2)(tx+2) for tx in {8, 16, 32, 64} and ty = NT / tx
-constexpr int lds_max(int nt) { return nt == 256 ? 400 : 680; }
+// global load of its own spinor plus (2 tx + 2 ty) / NT halo loads instead of five.  The
+// fifth-dimension window (previous, current, next slice) stays in registers.  Pipeline:
+// at the step of slice s the block loads its own sites of slice s + 2 and the halo of
+// slice s + 2, and stages slice s + 1 (loaded one step earlier) into LDS, so every load
+// has a full step to arrive.  The walk is unrolled four times, so the window's four
+// register sets and the two halo sets change roles at compile time (no copies of
+// registers a load is still writing), and every load and store is unconditional: steps
+// past the chunk's end and lanes without a halo site use out-of-range buffer offsets.
+// The output is always stored streaming (non-temporal): a choice between two store
+// kinds would be a branch again.
+// Each chunk fetches its spt slices plus the two boundary slices of the fifth-dimension
+// hop, and no more; odd chunks walk downwards so that neighbouring chunks fetch their
+// shared boundary slices at the same time (the second fetch hits L2).
+// Needs 8 <= tx <= 128, tx dividing NT and Nx, Nt % (NT/tx) == 0 (every thread owns a
+// site) and 2 tx + 2 NT/tx <= NT (every halo site has a loader).  A tile as wide as the
+// lattice (tx = Nx) has no x halo of its own: the wrapped columns are its own edges.
+constexpr bool lds_tile_ok(int nt, int tx)
+{
+    return tx >= 8 && tx <= 128 && nt % tx == 0 && 2 * tx + 2 * (nt / tx) <= nt;
+}
+// Largest (ty+2)(tx+2) over the tile widths lds_tile_ok accepts.
+constexpr int lds_max(int nt)
+{
+    int m = 0;
+    for (int tx = 8; tx <= 128; ++tx)
+        if (lds_tile_ok(nt, tx)) {
+            const int v = (nt / tx + 2) * (tx + 2);
+            m = v > m ? v : m;
+        }
+    return m;
+}
+// Threads per block of LDS tile mode 1..7.
+constexpr int lds_threads(int mode)
+{
+    return mode == 2   ? 512
+           : mode == 3 ? 64
+           : mode == 4 ? 128
+           : mode == 5 ? 384
+           : mode == 6 ? 768
+           : mode == 7 ? 1024
+                       : kBlock;
+}
 
 template <typename R, bool DAG, bool D5, int EPI, int NT>
 __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
@@ -440,9 +516,10 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     const Lat& L = a.L;
     if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
 
-    __shared__ double sh[16];
+    __shared__ double sh[2 * NT / 64 > 16 ? 2 * NT / 64 : 16];
     using V = typename Vec4<R>::T;
-    __shared__ V lds[2][lds_max(NT)];
+    constexpr int kSlots = lds_max(NT) + 1;  // + one sink slot for lanes without a halo site
+    __shared__ V lds[2][kSlots];
 
     R alpha = 0;
     if constexpr (EPI == EPI_CGUPD) {
@@ -452,7 +529,11 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     }
 
     const int V4 = L.V4, Nx = L.Nx, Nt = L.Nt, Ls = L.Ls;
-    const long long rb = (long long)a.L.V5() * 4 * rhs;
+    const long long V5 = L.V5();
+    const long long rb = V5 * 4 * rhs;
+    constexpr int VB = (int)sizeof(V);
+    const long long nbytes = V5 * VB;
+    const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.in + rb, nbytes);
     const int tx = a.cfg.tx, ty = NT / tx, W = tx + 2;
     const int tt = tile / a.cfg.tiles_x, tc = tile - tt * a.cfg.tiles_x;
     const int lt = threadIdx.x / tx, lx = threadIdx.x - lt * tx;
@@ -460,8 +541,9 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     const int t = t0 + lt, x = x0 + lx;
     const int site = t * Nx + x;
     const int own = (lt + 1) * W + lx + 1;
-    // this thread's halo site (2 tx + 2 ty <= NT of them), or none
-    int hsite = -1, hslot = 0;
+    // this thread's halo site (2 tx + 2 ty <= NT of them), or none (then its halo
+    // loads are out of range and it stages them into the sink slot)
+    int hsite = -1, hslot = kSlots - 1;
     {
         const int hh = threadIdx.x;
         const int tb = t0 == 0 ? Nt - 1 : t0 - 1, ta = t0 + ty == Nt ? 0 : t0 + ty;
@@ -497,55 +579,137 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
         const V q = lds[b][slot];
         return Sp<R>{q.x, q.y, q.z, q.w};
     };
-    // Odd chunks walk their slices downwards, so the slice on each chunk boundary (and
-    // the wall pair Ls-1 / 0) is fetched by both neighbouring blocks at the same moment
-    // and the second fetch hits L2.  Window: prev = psi(s - dir), cur = psi(s),
-    // next = psi(s + dir).
+    auto off_own = [&](int s) { return (s * V4 + site) * VB; };
+    auto off_halo = [&](int s) { return hsite >= 0 ? (s * V4 + hsite) * VB : kBufOff; };
+    // Odd chunks walk their slices downwards.  Step k computes slice s_k = sa + k dir.
     const int n = s1 - s0;
     const int dir = (chunk & 1) ? -1 : 1;
     const int sa = dir > 0 ? s0 : s1 - 1;
-    const R* in = a.in + rb;
     // One chunk spanning all of s (then dir = +1, sa = 0): the wall hop needs psi(Ls-1)
     // first and psi(0) last, the slices the walk starts from -- keep them instead of
     // fetching them twice.
     const bool whole = D5 && n == Ls;
-    Sp<R> cur = ldsp(in, (long long)sa * V4 + site);
-    Sp<R> prev{0, 0, 0, 0}, next{0, 0, 0, 0};
-    if constexpr (D5) prev = ldsp(in, (long long)wrap(sa - dir) * V4 + site);
-    if (D5 || n > 1) next = ldsp(in, (long long)wrap(sa + dir) * V4 + site);
-    const Sp<R> cur0 = cur, prev0 = prev;
     const int s_prev0 = wrap(sa - dir);
-    put(0, own, cur);
-    if (hsite >= 0) put(0, hslot, ldsp(in, (long long)sa * V4 + hsite));
-    int buf = 0;
-    for (int k2 = 0; k2 < n; ++k2) {
-        const int s = sa + k2 * dir;
-        __syncthreads();  // slice s staged in lds[buf]; nobody still reads lds[buf ^ 1]
-        const bool more = k2 + 1 < n;
-        // window of the next step: psi(s + 2 dir), a kept wall slice, or nothing
-        Sp<R> next2{0, 0, 0, 0}, hn{0, 0, 0, 0};
-        bool r0 = false, r1 = false;
-        if (more && (D5 || k2 + 2 < n)) {
+
+    // window: at step k (d = k mod 4) win[d] = psi(s_k), win[d+3] = psi(s_k - dir),
+    // win[d+1] = psi(s_k + dir), win[d+2] receives psi(s_k + 2 dir); halo: hreg[d & 1]
+    // holds the halo of slice s_k + dir, hreg[(d + 1) & 1] receives that of s_k + 2 dir
+    Sp<R> win[4], hreg[2];
+    win[0] = bld<R>(rin, off_own(sa));
+    win[3] = bld<R>(rin, D5 ? off_own(s_prev0) : kBufOff);
+    win[1] = bld<R>(rin, (D5 || n > 1) ? off_own(wrap(sa + dir)) : kBufOff);
+    win[2] = Sp<R>{0, 0, 0, 0};
+    const Sp<R> h0 = bld<R>(rin, off_halo(sa));
+    hreg[0] = bld<R>(rin, n > 1 ? off_halo(sa + dir) : kBufOff);
+    hreg[1] = Sp<R>{0, 0, 0, 0};
+    const Sp<R> cur0 = win[0], prev0 = win[3];
+    put(0, own, win[0]);
+    put(0, hslot, h0);
+
+    // epilogue buffers of this right-hand side (null pointers give empty buffers)
+    constexpr bool kOut = EPI != EPI_RESID, kUpd = EPI == EPI_CGUPD, kRes = EPI == EPI_RESID;
+    const __amdgpu_buffer_rsrc_t rout = make_rsrc(kOut && a.out ? a.out + rb : nullptr, nbytes);
+    const __amdgpu_buffer_rsrc_t rx = make_rsrc(kUpd ? a.x + rb : nullptr, nbytes);
+    const __amdgpu_buffer_rsrc_t rr = make_rsrc(kUpd ? a.r + rb : nullptr, nbytes);
+    const __amdgpu_buffer_rsrc_t rp = make_rsrc(kUpd ? a.p + rb : nullptr, nbytes);
+    const __amdgpu_buffer_rsrc_t rbv = make_rsrc(kRes ? a.b + rb : nullptr, V5 * 32);
+    const __amdgpu_buffer_rsrc_t rr64 = make_rsrc(kRes ? a.r64 + rb : nullptr, V5 * 32);
+    const __amdgpu_buffer_rsrc_t rp64 = make_rsrc(kRes && a.p64 ? a.p64 + rb : nullptr, V5 * 32);
+    const __amdgpu_buffer_rsrc_t rr32 = make_rsrc(kRes && a.r32 ? a.r32 + rb : nullptr, V5 * 16);
+    const __amdgpu_buffer_rsrc_t rp32 = make_rsrc(kRes && a.p32 ? a.p32 + rb : nullptr, V5 * 16);
+
+    // FP32 CG update: p, x, r of a slice are loaded one step before their epilogue
+    // (ep[d & 1] serves step d, ep[(d + 1) & 1] receives the next slice's)
+    constexpr bool kPre = kUpd && sizeof(R) == 4;
+    Sp<R> ep[2][3];
+    if constexpr (kPre) {
+        ep[0][0] = bld<R>(rp, off_own(sa));
+        ep[0][1] = bld<R>(rx, off_own(sa));
+        ep[0][2] = bld<R>(rr, off_own(sa));
+    }
+
+    auto step = [&](const int kk, auto D) {
+        constexpr int d = decltype(D)::value;
+        constexpr int b = d & 1;
+        const bool active = kk < n;
+        const int s = sa + kk * dir;
+        __syncthreads();  // slice s staged in lds[b]; nobody still reads lds[b ^ 1]
+        // own sites of slice s + 2 dir (unless past the end or a kept wall slice)
+        int oo = kBufOff;
+        if (kk + 1 < n && (D5 || kk + 2 < n)) {
             const int sn = wrap(s + 2 * dir);
-            r0 = whole && sn == s_prev0;
-            r1 = whole && sn == sa;
-            if (!r0 && !r1) next2 = ldsp(in, (long long)sn * V4 + site);
+            if (!(whole && (sn == s_prev0 || sn == sa))) oo = off_own(sn);
         }
-        next2 = pick(r0, prev0, pick(r1, cur0, next2));
-        if (more && hsite >= 0) hn = ldsp(in, (long long)(s + dir) * V4 + hsite);
-        const Sp<R> vtp = get(buf, own + W), vtm = get(buf, own - W);
-        const Sp<R> vxp = get(buf, own + 1), vxm = get(buf, own - 1);
-        const Sp<R> dn = pick(dir > 0, prev, next), up = pick(dir > 0, next, prev);
+        win[(d + 2) & 3] = bld<R>(rin, oo);
+        hreg[(d + 1) & 1] = bld<R>(rin, kk + 2 < n ? off_halo(s + 2 * dir) : kBufOff);
+        if constexpr (kPre) {
+            const int en = kk + 1 < n ? off_own(s + dir) : kBufOff;
+            ep[(d + 1) & 1][0] = bld<R>(rp, en);
+            ep[(d + 1) & 1][1] = bld<R>(rx, en);
+            ep[(d + 1) & 1][2] = bld<R>(rr, en);
+        }
+        // psi(s + dir), loaded one step ago, or the kept wall slice it stands for
+        {
+            const int sn = wrap(s + dir);
+            win[(d + 1) & 3] = pick(whole && sn == s_prev0, prev0, pick(whole && sn == sa, cur0, win[(d + 1) & 3]));
+        }
+        const Sp<R>& cur = win[d];
+        const Sp<R>& nxt = win[(d + 1) & 3];
+        const Sp<R>& prv = win[(d + 3) & 3];
+        const Sp<R> vtp = get(b, own + W), vtm = get(b, own - W);
+        const Sp<R> vxp = get(b, own + 1), vxm = get(b, own - 1);
+        const Sp<R> dn = pick(dir > 0, prv, nxt), up = pick(dir > 0, nxt, prv);
         const Sp<R> o = hop_site<R, DAG, D5>(L, k, cur, vtp, vtm, vxp, vxm, dn, up, s);
-        hop_epi<R, EPI>(a, rb, (long long)s * V4 + site, o, alpha, acc0, acc1);
-        if (more) {
-            put(buf ^ 1, own, next);
-            if (hsite >= 0) put(buf ^ 1, hslot, hn);
+        const int eo = active ? off_own(s) : kBufOff;
+        if constexpr (EPI == EPI_STORE || EPI == EPI_NORM) {
+            bst<R, 2>(rout, eo, o);
+            if constexpr (EPI == EPI_NORM) acc0 += active ? norm2(o) : 0.0;
+        } else if constexpr (EPI == EPI_CGUPD) {
+            bst<R, 2>(rout, eo, o);
+            Sp<R> pv, xv, rv;
+            if constexpr (kPre) {
+                pv = ep[d & 1][0];
+                xv = ep[d & 1][1];
+                rv = ep[d & 1][2];
+            } else {
+                pv = bld<R>(rp, eo);
+                xv = bld<R>(rx, eo);
+                rv = bld<R>(rr, eo);
+            }
+            xv.ar += alpha * pv.ar;
+            xv.ai += alpha * pv.ai;
+            xv.br += alpha * pv.br;
+            xv.bi += alpha * pv.bi;
+            rv.ar -= alpha * o.ar;
+            rv.ai -= alpha * o.ai;
+            rv.br -= alpha * o.br;
+            rv.bi -= alpha * o.bi;
+            bst<R>(rx, eo, xv);
+            bst<R>(rr, eo, rv);
+            acc0 += active ? norm2(rv) : 0.0;
+        } else {  // EPI_RESID (FP64 only)
+            if constexpr (sizeof(R) == 8) {
+                const int eo32 = active ? (s * V4 + site) * 16 : kBufOff;
+                const Sp<double> bv = bld<double>(rbv, eo);
+                const Sp<double> rv{bv.ar - o.ar, bv.ai - o.ai, bv.br - o.br, bv.bi - o.bi};
+                bst<double>(rr64, eo, rv);
+                bst<double>(rp64, eo, rv);
+                const Sp<float> fv{(float)rv.ar, (float)rv.ai, (float)rv.br, (float)rv.bi};
+                bst<float>(rr32, eo32, fv);
+                bst<float>(rp32, eo32, fv);
+                acc0 += active ? norm2(rv) : 0.0;
+                acc1 += active ? norm2(bv) : 0.0;
+            }
         }
-        prev = cur;
-        cur = next;
-        next = next2;
-        buf ^= 1;
+        // stage slice s + dir for the next step
+        put(b ^ 1, own, nxt);
+        put(b ^ 1, hslot, hreg[d & 1]);
+    };
+    for (int k0 = 0; k0 < n; k0 += 4) {
+        step(k0, IC<0>{});
+        step(k0 + 1, IC<1>{});
+        step(k0 + 2, IC<2>{});
+        step(k0 + 3, IC<3>{});
     }
 
     if constexpr (EPI != EPI_STORE) {
@@ -760,10 +924,15 @@ void dispatch_lds_nt(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 template <typename R, bool DAG, bool D5>
 void dispatch_lds(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 {
-    if (a.cfg.lds == 2)
-        dispatch_lds_nt<R, DAG, D5, 512>(epi, a, grid, s);
-    else
-        dispatch_lds_nt<R, DAG, D5, 256>(epi, a, grid, s);
+    switch (a.cfg.lds) {
+        case 2: dispatch_lds_nt<R, DAG, D5, 512>(epi, a, grid, s); break;
+        case 3: dispatch_lds_nt<R, DAG, D5, 64>(epi, a, grid, s); break;
+        case 4: dispatch_lds_nt<R, DAG, D5, 128>(epi, a, grid, s); break;
+        case 5: dispatch_lds_nt<R, DAG, D5, 384>(epi, a, grid, s); break;
+        case 6: dispatch_lds_nt<R, DAG, D5, 768>(epi, a, grid, s); break;
+        case 7: dispatch_lds_nt<R, DAG, D5, 1024>(epi, a, grid, s); break;
+        default: dispatch_lds_nt<R, DAG, D5, 256>(epi, a, grid, s); break;
+    }
 }
 
 template <typename R, bool DAG, bool D5>
@@ -786,23 +955,26 @@ HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefe
     c.nt_store = nt_store;
     c.remap = remap;
     c.prefetch = prefetch;
-    if (tile_x > 0 && tile_x <= kBlock && kBlock % tile_x == 0 && L.Nx % tile_x == 0) {
+    c.lds = 0;
+    c.tx = 0;
+    c.tiles_x = 0;
+    c.bx = (L.V4 + kBlock - 1) / kBlock;
+    // (the LDS kernel addresses one right-hand side's spinors with 32-bit byte offsets)
+    if (lds >= 1 && lds <= 7 && L.V5() * 32 < kBufOff) {
+        const int nt = lds_threads(lds);
+        if (lds_tile_ok(nt, tile_x) && L.Nx % tile_x == 0 && L.Nt % (nt / tile_x) == 0) {
+            c.lds = lds;
+            c.tx = tile_x;
+            c.tiles_x = L.Nx / tile_x;
+            c.bx = c.tiles_x * (L.Nt / (nt / tile_x));
+            c.nt_store = 1;  // the LDS kernel always streams its output
+        }
+    }
+    if (!c.lds && tile_x > 0 && tile_x <= kBlock && kBlock % tile_x == 0 && L.Nx % tile_x == 0) {
         c.tx = tile_x;
         c.tiles_x = L.Nx / tile_x;
         const int rows = kBlock / tile_x;
         c.bx = c.tiles_x * ((L.Nt + rows - 1) / rows);
-    } else {
-        c.tx = 0;
-        c.tiles_x = 0;
-        c.bx = (L.V4 + kBlock - 1) / kBlock;
-    }
-    c.lds = 0;
-    if (lds == 1 || lds == 2) {
-        const int nt = lds == 2 ? 512 : kBlock;
-        if ((c.tx == 8 || c.tx == 16 || c.tx == 32 || c.tx == 64) && L.Nt % (nt / c.tx) == 0) {
-            c.lds = lds;
-            c.bx = c.tiles_x * (L.Nt / (nt / c.tx));
-        }
     }
     int chunks = 1;
     if (chunks_override > 0) {

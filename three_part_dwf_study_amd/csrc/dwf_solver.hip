@@ -148,7 +148,8 @@ struct FullOp : CgOperator {
     {
         sig_ctx(c, k);
         k.insert(k.end(), {1ULL, (unsigned long long)first_dag, (unsigned long long)nrhs});
-        const int v[] = {cfg.bx, cfg.chunks, cfg.spt, cfg.remap, cfg.prefetch, cfg.tx, cfg.tiles_x, cfg.nt_store};
+        const int v[] = {cfg.bx,       cfg.chunks,  cfg.spt,      cfg.remap, cfg.prefetch,
+                         cfg.tx,       cfg.tiles_x, cfg.nt_store, cfg.lds};
         for (int x : v) k.push_back((unsigned long long)(unsigned)x);
     }
 };
