@@ -125,17 +125,25 @@ def time_copy(ctx, capi, nbytes, reps):
     return ms, 2 * nbytes / (ms * 1e-3) / 1e9
 
 
-def measured_traffic():
-    """HBM bytes per launch of the roofline kernel from the committed counter summary
-    (two rocprofv3 --pmc runs calibrated on the copy kernel), or None."""
+def measured_traffic(shape):
+    """HBM bytes per launch of the roofline kernel in this launch shape, from the
+    committed counter summary (two rocprofv3 --pmc runs calibrated on the copy kernel),
+    or None when that shape was not measured."""
     path = os.path.join(ROOT, "profiles", "k_hop_fp32_dwf96_batch128_traffic.md")
+    key = f"traffic_bytes_per_launch lds={shape['lds']} tile_x={shape['tile_x']} ="
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch"):
-                return int(line.split("=")[1].strip())
+            if line.startswith(key):
+                return int(line[len(key):].strip())
     except OSError:
         pass
     return None
+
+
+def parse_shape(text):
+    """'lds,tile_x' -> a launch shape of the LDS stencil with one s-chunk."""
+    lds, tx = (int(v) for v in text.split(","))
+    return {"tile_x": tx, "prefetch": 0, "chunks": 1, "nt_store": 1, "lds": lds}
 
 
 def cpu_baseline(theta, b1, judge_iters, sample_iters):
@@ -500,6 +508,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nrhs", type=int, default=32, help="right-hand sides per step")
     ap.add_argument("--dslash-rhs", type=int, default=128)
+    ap.add_argument("--dslash-shape", default=None,
+                    help="'lds,tile_x': time the FP32 D apply in this LDS tile shape instead of the autotuned one")
     ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("solves", "analyzer", "learned", "hmc"), default="solves")
@@ -542,13 +552,18 @@ def main():
     if args.kernels_only:
         ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
         ctx.set_gauge(theta)
-        ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32, args.tune_cache)
+        if args.dslash_shape:
+            shape = parse_shape(args.dslash_shape)
+            apply_shape(ctx, dwf, shape)
+        else:
+            shape = ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32, args.tune_cache)
         ms32, gbs32 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, args.dslash_rhs, 5)
         cb = dslash_bytes(4, args.dslash_rhs) // 2 // 16 * 16
         msc, gbsc = time_copy(ctx, capi, cb, 5)
         print(json.dumps({"dslash_fp32_us": ms32 * 1e3, "dslash_fp32_gbs": gbs32,
                           "dslash_bytes": dslash_bytes(4, args.dslash_rhs), "copy_us": msc * 1e3,
-                          "copy_gbs": gbsc, "copy_bytes_each_way": cb}), flush=True)
+                          "copy_gbs": gbsc, "copy_bytes_each_way": cb,
+                          "shape": {k: shape[k] for k in ("lds", "tile_x", "chunks")}}), flush=True)
         ctx.close()
         return
     nrhs = args.nrhs
@@ -613,7 +628,11 @@ def main():
         pcie[name] = args.steps * nrhs / (time.perf_counter() - t0)
         log(f"{name} with host arrays (PCIe-inclusive): {pcie[name]:.1f} solves/s")
 
-    tuned["fp32_batch"] = ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32, args.tune_cache)
+    if args.dslash_shape:
+        tuned["fp32_batch"] = parse_shape(args.dslash_shape)
+        apply_shape(ctx, dwf, tuned["fp32_batch"])
+    else:
+        tuned["fp32_batch"] = ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32, args.tune_cache)
     ms32, gbs32 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, args.dslash_rhs, 20)
     tuned["fp64_batch"] = ctx.autotune(args.dslash_rhs // 2, dwf.DWF_PREC_FP64, args.tune_cache)
     ms64, gbs64 = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP64, args.dslash_rhs // 2, 20)
@@ -673,7 +692,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(gbs32 / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic() if args.dslash_rhs == 128 else None,
+            "traffic": measured_traffic(tuned["fp32_batch"]) if args.dslash_rhs == 128 else None,
             "kernel": (f"k_hop_lds<float,D,DWF,EPI_STORE,{LDS_THREADS[tuned['fp32_batch']['lds']]}>"
                        if tuned["fp32_batch"]["lds"] else "k_hop<float,D,DWF,EPI_STORE>"),
             "avg_launch_us": round(ms32 * 1e3, 2),
