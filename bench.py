@@ -350,6 +350,8 @@ def learned_solver_config(args, dwf, ctx, net):
     """Attach the network to ctx with the learned solver's direction update; the learned
     solver's reliable-update threshold is set around its solves (solve_with)."""
     ctx.tune(dwf.DWF_TUNE_FCG_BETA, args.fcg_beta)
+    if args.net_fused is not None:
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, args.net_fused)
     ctx.set_preconditioner(net)
     return ["flexible CG (FCG(1))", "preconditioned CG (Fletcher-Reeves beta)"][args.fcg_beta] + \
         f" in FP32 with FP64 reliable updates (residual drop {args.reliable_delta:g}), certified at 1e-10"
@@ -522,6 +524,9 @@ def main():
     ap.add_argument("--learn-lr", type=float, default=3e-3)
     ap.add_argument("--reliable-delta", type=float, default=1e-4,
                     help="learned solver: residual drop between FP64 reliable updates (library default 0.01)")
+    ap.add_argument("--net-fused", type=int, default=None, choices=(0, 1, 2),
+                    help="learned network kernel: 0 layer by layer, 1 fused, 2 fused in a gauge per block "
+                         "(default: the library's)")
     ap.add_argument("--fcg-beta", type=int, default=1, choices=(0, 1),
                     help="learned solver direction update: 0 flexible CG (FCG(1)), 1 Fletcher-Reeves")
     ap.add_argument("--analyzer-configs", type=int, default=32, help="configurations per lattice size")

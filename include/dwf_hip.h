@@ -112,9 +112,11 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * 1 / 2 / 3 / 4 / 5 / 6 / 7 to stage each slice of a 2D stencil tile of 256 / 512 / 64 /
  * 128 / 384 / 768 / 1024 sites and its halo in LDS (with DWF_TUNE_TILE_X between 8 and 128 dividing
  * Nx and the tile size, sites/width dividing Nt, and 2 (width + sites/width) <= sites;
- * ignored otherwise); DWF_TUNE_NET_FUSED: 0/1 (default 1) run the learned
- * preconditioner's three layers as one fused kernel (LDS-resident hidden layers) instead
- * of layer by layer; both give bit-identical results; DWF_TUNE_FCG_BETA: the direction
+ * ignored otherwise); DWF_TUNE_NET_FUSED: the learned preconditioner's kernels, 0
+ * layer by layer, 1 the three layers as one fused kernel (LDS-resident hidden layers;
+ * bit-identical to 0), 2 (default) the fused kernel with each block computing in a
+ * temporal gauge of its own (no transport along t; the same network up to the bf16
+ * rounding of the gauge-transformed values); DWF_TUNE_FCG_BETA: the direction
  * update of DWF_SOLVE_LEARNED, 0 (default) flexible CG's beta = -(z, A p) / (p, A p)
  * (Notay's FCG(1), for any preconditioner), 1 Fletcher-Reeves beta = (z, r) / (z, r)_prev
  * (exact for a symmetric preconditioner; one vector pass less per iteration). */

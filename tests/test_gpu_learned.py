@@ -42,6 +42,7 @@ def test_net_apply_matches_oracle(Nt, Nx, Ls):
     assert np.array_equal(net.get_weights(), w)
     with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
         ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, 1)  # the fused kernel in the lattice's own gauge
         z = net.apply(ctx, v)
         za = net.apply(ctx, v, adjoint=True)
         zu = net.apply(ctx, u, adjoint=True)
@@ -72,6 +73,7 @@ def test_fused_network_matches_layers(Nt, Nx, Ls):
     net.set_weights(w)
     with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
         ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, 1)
         zf = net.apply(ctx, v)
         ctx.tune(dwf.DWF_TUNE_NET_FUSED, 0)
         zl = net.apply(ctx, v)
@@ -79,6 +81,90 @@ def test_fused_network_matches_layers(Nt, Nx, Ls):
     zq = oracle.net_apply(th, w.astype(np.float64), v, Nt, Nx, Ls, bf16=True)
     for r in range(3):
         assert rel(zf[r], zq[r]) <= 1e-3, rel(zf[r], zq[r])
+
+
+def block_gauge(th, Nt, Nx, t_start):
+    """Temporal gauge starting at row t_start: Om = 1 on that row, Om(t + 1) = Om(t) T_t(t)
+    with T_t the +t transporter (antiperiodic sign on the last row), continued around the
+    lattice.  Returns Om [Nt, Nx] and the angles theta' (same layout as theta) whose
+    transporters are Om(x) T(x) Om(x + mu)^*: 1 along t except into row t_start."""
+    sg = np.ones(Nt)
+    sg[-1] = -1.0
+    Tt = sg[:, None] * np.exp(1j * th[: Nt * Nx].reshape(Nt, Nx))
+    Ux = np.exp(1j * th[Nt * Nx:].reshape(Nt, Nx))
+    om = np.ones((Nt, Nx), complex)
+    o = np.ones(Nx, complex)
+    for k in range(Nt):
+        t = (t_start + k) % Nt
+        om[t] = o
+        o = o * Tt[t]
+    Tt2 = om * Tt * np.conj(np.roll(om, -1, axis=0))
+    Tx2 = om * Ux * np.conj(np.roll(om, -1, axis=1))
+    return om, np.concatenate([np.angle(Tt2 / sg[:, None]).ravel(), np.angle(Tx2).ravel()])
+
+
+@pytest.mark.parametrize("Nt,Nx,Ls", [(16, 32, 5), (24, 22, 4)])
+def test_fused_network_block_gauge_matches_oracle(Nt, Nx, Ls):
+    """DWF_TUNE_NET_FUSED 2: each block of the fused kernel computes in a temporal gauge
+    of its own (no transport along t inside the block).  Pinned per block against the
+    bf16-quantized oracle run on the whole lattice in that block's gauge (Om v in,
+    Om^* z' out): the block's output region sees the same transporters and inputs.  The
+    shapes are large enough that a block's input region (tile + 3-site halo) does not
+    wrap around the lattice."""
+    print_case(test="fused_network_block_gauge", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(5 * Nt + Nx + Ls)
+    th = rand_theta(rng, Nt, Nx)
+    w = weights32(rng, Ls)
+    v = rand_spinor(rng, 2 * Ls * Nt * Nx * 2).reshape(2, -1)
+    net = dwf.Network(Ls, seed=1)
+    net.set_weights(w)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, 2)
+        zg = net.apply(ctx, v)
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, 1)
+        zf = net.apply(ctx, v)
+    TY = max(d for d in range(1, 9) if Nt % d == 0)
+    TX = max(d for d in range(1, 17) if Nx % d == 0)
+    assert Nt >= TY + 6 and Nx >= TX + 6
+    w64 = w.astype(np.float64)
+    zg4 = zg.reshape(2, Ls, Nt, Nx, 2)
+    for t0 in range(0, Nt, TY):
+        om, th2 = block_gauge(th, Nt, Nx, (t0 - 3) % Nt)
+        vg = (om[None, None, :, :, None] * v.reshape(2, Ls, Nt, Nx, 2)).reshape(2, -1)
+        zq = oracle.net_apply(th2, w64, vg, Nt, Nx, Ls, bf16=True).reshape(2, Ls, Nt, Nx, 2)
+        zq = np.conj(om)[None, None, :, :, None] * zq
+        for x0 in range(0, Nx, TX):
+            blk = (slice(None), slice(None), slice(t0, t0 + TY), slice(x0, x0 + TX))
+            assert rel(zg4[blk].ravel(), zq[blk].ravel()) <= 1e-3, (t0, x0, rel(zg4[blk].ravel(), zq[blk].ravel()))
+    # and the gauge-covariant network gives the same field as without the gauge change,
+    # up to the bf16 rounding of the transformed values
+    z64 = oracle.net_apply(th, w64, v, Nt, Nx, Ls)
+    for r in range(2):
+        assert rel(zg[r], zf[r]) <= 2e-2 and rel(zg[r], z64[r]) <= 2e-2
+
+
+@pytest.mark.parametrize("Nt,Nx,Ls", [(8, 6, 4), (12, 20, 3)])
+def test_fused_network_block_gauge_small_lattices(Nt, Nx, Ls):
+    """Block gauge on lattices smaller than a block's input region (the region is a patch
+    of the covering lattice): same field as the exact fused kernel and the FP64 oracle up
+    to bf16 rounding."""
+    print_case(test="fused_network_block_gauge_small", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(7 * Nt + Nx + Ls)
+    th = rand_theta(rng, Nt, Nx)
+    w = weights32(rng, Ls)
+    v = rand_spinor(rng, 2 * Ls * Nt * Nx * 2).reshape(2, -1)
+    net = dwf.Network(Ls, seed=1)
+    net.set_weights(w)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, 2)
+        zg = net.apply(ctx, v)
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, 1)
+        zf = net.apply(ctx, v)
+    z64 = oracle.net_apply(th, w.astype(np.float64), v, Nt, Nx, Ls)
+    for r in range(2):
+        assert rel(zg[r], zf[r]) <= 2e-2 and rel(zg[r], z64[r]) <= 2e-2
 
 
 def test_net_apply_per_configuration():
@@ -92,6 +178,7 @@ def test_net_apply_per_configuration():
     net.set_weights(w)
     with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
         ctx.set_gauge_batch(ths)
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, 1)
         z = net.apply(ctx, v)
     for r in range(4):
         want = oracle.net_apply(ths[r // 2], w.astype(np.float64), v[r], Nt, Nx, Ls, bf16=True)
@@ -206,7 +293,7 @@ def test_learned_solve_fused_and_layers():
     with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
         ctx.set_gauge(th)
         ctx.set_preconditioner(net)
-        for fused in (1, 0):
+        for fused in (1, 0, 2):
             ctx.tune(dwf.DWF_TUNE_NET_FUSED, fused)
             res.append(ctx.solve_dirac(src, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_LEARNED))
     for (x, st, _) in res:
@@ -215,6 +302,9 @@ def test_learned_solve_fused_and_layers():
     for r in range(3):
         assert abs(res[0][1][r]["iterations"] - res[1][1][r]["iterations"]) <= 2
         assert rel(res[0][0][r], res[1][0][r]) <= 1e-8
+        # the block-gauge network is the same preconditioner up to bf16 rounding
+        assert abs(res[2][1][r]["iterations"] - res[0][1][r]["iterations"]) <= 0.1 * res[0][1][r]["iterations"] + 2
+        assert rel(res[2][0][r], res[0][0][r]) <= 1e-8
 
 
 def volume_sources(rng, n, Nt, Nx, Ls):

@@ -114,7 +114,7 @@ struct dwf_ctx {
     int tile_x = 0;               // stencil 2D tile width (0: linear blocks)
     int nt_store = 0;             // streaming stores of stencil outputs
     int lds = 0;                  // LDS-staged stencil tiles
-    int net_fused = 1;            // learned preconditioner: fused three-layer kernel when the shape allows
+    int net_fused = 2;            // learned preconditioner: 0 layer by layer, 1 fused, 2 fused in a gauge per block
     int fcg_beta = 0;             // learned solver: 0 flexible CG (Notay's FCG(1)), 1 Fletcher-Reeves
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs

@@ -328,7 +328,8 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             ctx->lds = (int)value;
             return DWF_OK;
         case DWF_TUNE_NET_FUSED:
-            ctx->net_fused = value != 0.0 ? 1 : 0;
+            if (!(value == 0.0 || value == 1.0 || value == 2.0)) break;
+            ctx->net_fused = (int)value;
             return DWF_OK;
         case DWF_TUNE_FCG_BETA:
             if (!(value == 0.0 || value == 1.0)) break;

@@ -57,6 +57,7 @@ struct dwf_net {
     float2* m2[3] = {nullptr, nullptr, nullptr};  // Adam second moments
     unsigned short* wf[3] = {nullptr, nullptr, nullptr};  // BF16 B fragments, forward
     unsigned short* wa[3] = {nullptr, nullptr, nullptr};  // BF16 B fragments, adjoint
+    unsigned short* wg0 = nullptr;  // layer 1's fragments in the gauge-fixed fused kernel's point order
     int adam_t = 0;
     hipStream_t pack_stream = nullptr;  // for set_weights outside a context
     ~dwf_net();
@@ -165,6 +166,7 @@ __device__ __forceinline__ uint2 link_ops(const Nb& n)
 {
     return make_uint2(pk_bf16(n.c, -n.sn), pk_bf16(n.sn, n.c));
 }
+__device__ __forceinline__ uint2 link_ops_f2(float2 u) { return make_uint2(pk_bf16(u.x, -u.y), pk_bf16(u.y, u.x)); }
 __device__ __forceinline__ float dot2(unsigned h, unsigned l)
 {
     return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2_hw, h), __builtin_bit_cast(bf16x2_hw, l), 0.0f,
@@ -251,6 +253,26 @@ __device__ __forceinline__ f32x4_t tile8(const bf16x4_t (&W)[7], const bf16x4_t 
     for (int p = 1; p <= 4; ++p) cmac2(y, G[p - 1], u[p - 1].x, u[p - 1].y);
     return y;
 }
+
+// tile8 in a gauge where the +-t transporters are 1: those points chain into the
+// accumulator like self and +-s; u[0], u[1] are the +x, -x transporters.
+__device__ __forceinline__ f32x4_t tile8_tg(const bf16x4_t (&W)[7], const bf16x4_t (&h)[7], const float2 (&u)[2])
+{
+    const f32x4_t zero = {0.f, 0.f, 0.f, 0.f};
+    f32x4_t y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[0], h[0], zero, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[5], h[5], y, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[6], h[6], y, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[1], h[1], y, 0, 0, 0);
+    y = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[2], h[2], y, 0, 0, 0);
+    const f32x4_t G3 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[3], h[3], zero, 0, 0, 0);
+    const f32x4_t G4 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(W[4], h[4], zero, 0, 0, 0);
+    cmac2(y, G3, u[0].x, u[0].y);
+    cmac2(y, G4, u[1].x, u[1].y);
+    return y;
+}
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
 // Operands of a C_in = 8 tile at the lane's site (t, x) of slice s (channels 2g, 2g+1).
 __device__ __forceinline__ void gather8(const PtcArgs& a, const float2* L, const void* inb, int s, int t, int x, int g,
@@ -500,8 +522,11 @@ __global__ void k_fold_adam(const double* GB, int cin, int cout, int Ls, float2*
 
 // BF16 B fragments of layer (cin_l -> cout_l): forward, or the adjoint kernel's
 // (cout_l -> cin_l) with W'_p(s) = W_{-p}(s + ds_p)^H.
-__global__ void k_pack(const float2* w, int cin_l, int cout_l, int Ls, int adj, unsigned short* out)
+// adj: 0 forward, 1 adjoint, 2 forward C_in = 2 fragments in the point order of the
+// gauge-fixed fused kernel (MFMA step 0: self, +t, -t, +s; step 1: -s, +x, -x, none).
+__global__ void k_pack(const float2* w, int cin_l, int cout_l, int Ls, int mode, unsigned short* out)
 {
+    const bool adj = mode == 1;
     const int kin = adj ? cout_l : cin_l, kout = adj ? cin_l : cout_l;
     const int NSTEP = nstep_of(kin);
     const long long total = (long long)Ls * NSTEP * 256;
@@ -516,7 +541,8 @@ __global__ void k_pack(const float2* w, int cin_l, int cout_l, int Ls, int adj, 
         c = k >> 1;
         kr = k & 1;
     } else {
-        p = 4 * step + g;
+        constexpr int gf[2][4] = {{0, 1, 2, 5}, {6, 3, 4, kPts}};
+        p = mode == 2 ? gf[step][g] : 4 * step + g;
         c = jj >> 1;
         kr = jj & 1;
     }
@@ -673,7 +699,7 @@ __device__ __forceinline__ uint2 hop_link(const float2* L, int Nt, int Nx, int V
 
 template <int V> using IC = std::integral_constant<int, V>;
 
-template <typename TIN, typename TOUT, bool DOTS>
+template <typename TIN, typename TOUT, bool DOTS, bool TG>
 __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_net_fused(const NetFusedArgs a)
 {
     __shared__ uint2 ring0[4][kFuR0];     // input, bf16 (comp 0, comp 1)
@@ -705,6 +731,34 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         const int i = div_small(tid, inv0), jx = tid - i * W0;
         gsite = wrap_t(t0 - 3 + i) * Nx + wrap_x(x0 - 3 + jx);
     }
+    // TG: the block computes in a gauge of its own (temporal gauge on its input region
+    // R0, rows t0-3 .. t0+TY+2, columns x0-3 .. x0+TX+2, in patch coordinates, so a region
+    // that wraps around a small lattice is a patch of the covering lattice): Om = 1 on the
+    // region's first row and Om(i + 1, jx) = Om(i, jx) U_+t(i, jx), so every +-t transporter
+    // inside the region is 1 and no layer transports along t; the +-x transporters become
+    // Om(a) U Om(b)^*, the input is multiplied by Om and the output by Om^*.  The network
+    // is gauge covariant, so the output is the same up to the bf16 rounding of the
+    // transformed values.  Om is kept in the first bytes of ring2 until the walk (ring2 is
+    // first written in step 6, after the set-up's reads).
+    float2 om_in = make_float2(1.f, 0.f);
+    const float2* om = reinterpret_cast<const float2*>(&ring2[0][0][0]);
+    if constexpr (TG) {
+        float2* omw = reinterpret_cast<float2*>(&ring2[0][0][0]);
+        if (tid < W0) {
+            float2 o = make_float2(1.f, 0.f);
+            const int x = wrap_x(x0 - 3 + tid);
+            for (int i = 0; i < TY + 6; ++i) {
+                omw[i * W0 + tid] = o;
+                o = cmulf(o, hop_u(L, Nt, Nx, V4, 1, wrap_t(t0 - 3 + i), x));
+            }
+        }
+        __syncthreads();
+        if (tid < n0) om_in = om[tid];
+    }
+    // +x (p = 3) or -x (p = 4) transporter at the site (t, x) with R0 index b, in the block's gauge
+    auto tg_u = [&](int p, int b, int t, int x) {
+        return cmulf(cmulf(om[b], hop_u(L, Nt, Nx, V4, p, t, x)), conjf2(om[p == 3 ? b + 1 : b - 1]));
+    };
     const TIN* inr = static_cast<const TIN*>(a.in) + (long long)rhs * Ls * V4 * 4;
     TOUT* outr = static_cast<TOUT*>(a.out) + (long long)rhs * Ls * V4 * 4;
     auto smod = [Ls](int s) { return ((s % Ls) + Ls) % Ls; };
@@ -726,18 +780,37 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     // input slice j-3 (its ring slot held slice j-7, last read by layer 1 in step j-1) and
     // the weights of step j+1 (their buffer was last read in step j-1).
     const int jend = -3 + 4 * ((Ls + 15) / 4);
-    auto loads = [&](int j, float4& in_new, uint4& w_new) {
-        in_new = ld_spinor(inr, (long long)smod(min(j - 2, Ls + 2)) * V4 + max(gsite, 0));
+    // The slices a step loads, smod(j - 2) (input; past slice Ls + 2 the loaded value is
+    // not used) and smod(j - 3), smod(j - 5), smod(j - 7) (weights of layers 1-3), are
+    // carried from step to step as scalar counters: a modulo per step costs ~25 scalar
+    // instructions.
+    struct Slices {
+        int c2, c3, c5, c7;
+    };
+    auto slices_at = [&](int j) { return Slices{smod(j - 2), smod(j - 3), smod(j - 5), smod(j - 7)}; };
+    auto loads = [&](Slices& c, float4& in_new, uint4& w_new) {
+        in_new = ld_spinor(inr, (long long)c.c2 * V4 + max(gsite, 0));
         // the three layers' slices are wave-uniform; the thread picks its layer's record
-        const unsigned short* w0 = a.wf[0] + smod(j - 3) * (2 * 64 * 4);
-        const unsigned short* w1 = a.wf[1] + smod(j - 5) * (7 * 64 * 4);
-        const unsigned short* w2 = a.wf[2] + smod(j - 7) * (7 * 64 * 4);
+        const unsigned short* w0 = a.wf[0] + c.c3 * (2 * 64 * 4);
+        const unsigned short* w1 = a.wf[1] + c.c5 * (7 * 64 * 4);
+        const unsigned short* w2 = a.wf[2] + c.c7 * (7 * 64 * 4);
         w_new = *reinterpret_cast<const uint4*>((wl == 0 ? w0 : (wl == 1 ? w1 : w2)) + wbase);
+        auto inc = [Ls](int& v) { v = v + 1 == Ls ? 0 : v + 1; };
+        inc(c.c2);
+        inc(c.c3);
+        inc(c.c5);
+        inc(c.c7);
     };
     auto stores = [&](auto Kc, int j, const float4& in_old, const uint4& w_old) {
         constexpr int K = decltype(Kc)::value;
-        if (j - 3 >= -3 && j - 3 <= Ls + 2 && gsite >= 0)
-            ring0[(K + 2) & 3][tid] = make_uint2(pk_bf16(in_old.x, in_old.y), pk_bf16(in_old.z, in_old.w));
+        if (j - 3 >= -3 && j - 3 <= Ls + 2 && gsite >= 0) {
+            float4 v = in_old;
+            if constexpr (TG) {
+                const float2 c0 = cmulf(om_in, make_float2(v.x, v.y)), c1 = cmulf(om_in, make_float2(v.z, v.w));
+                v = make_float4(c0.x, c0.y, c1.x, c1.y);
+            }
+            ring0[(K + 2) & 3][tid] = make_uint2(pk_bf16(v.x, v.y), pk_bf16(v.z, v.w));
+        }
         reinterpret_cast<uint4*>(wrec[K & 1])[tid] = w_old;
         __syncthreads();
     };
@@ -764,6 +837,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             win[1] = rs[e0];
         }
         bf16x4_t h[7], W[7];
+        (void)u;
         const uint2 up = rp[e0];
         const uint2 vtp = rs[etp], vtm = rs[etm], vxp = rs[exp], vxm = rs[exm];
         h[0] = pack_pairs(win[1].x, win[1].y);
@@ -777,16 +851,24 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         for (int q = 0; q < 7; ++q) W[q] = frag(Kc, decltype(Lc)::value, q);
         win[0] = win[1];
         win[1] = up;
-        return tile8(W, h, u);
+        if constexpr (TG) {
+            const float2 ux[2] = {u[2], u[3]};
+            return tile8_tg(W, h, ux);
+        } else {
+            return tile8(W, h, u);
+        }
     };
 
     // ---- layer 1 (every wave): R1 tiles wv*2 + e (origin t0-2, x0-2), input ring0 (width W0).
     // Lane group g supplies stencil point g (MFMA step 0: self, +t, -t, +x, all in slice
     // s) and point 4 + g (step 1: -x in slice s, +s, -s, and none with zero transporter).
+    // TG: step 0 self, +t, -t, +s (no transport at all), step 1 -s, +x, -x, none.
     int e1a[kFuL1Tiles], e1b[kFuL1Tiles], o1[kFuL1Tiles];
     bool v1[kFuL1Tiles];
     uint2 lk1[kFuL1Tiles][2];
-    const int dsl = g == 1 ? 1 : (g == 2 ? -1 : 0);  // step 1: slice offset of the lane's point
+    // slice offsets of the lane's points in steps 0 and 1
+    const int dsa = TG && g == 3 ? 1 : 0;
+    const int dsb = TG ? (g == 0 ? -1 : 0) : (g == 1 ? 1 : (g == 2 ? -1 : 0));
 #pragma unroll
     for (int e = 0; e < kFuL1Tiles; ++e) {
         const int k = wv * kFuL1Tiles + e;
@@ -794,12 +876,20 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         v1[e] = 16 * k + m < n1;
         const int i = div_small(sg, inv1), jx = sg - i * W1;
         const int b0 = (i + 1) * W0 + jx + 1;
-        e1a[e] = g == 1 ? b0 + W0 : (g == 2 ? b0 - W0 : (g == 3 ? b0 + 1 : b0));
-        e1b[e] = g == 0 ? b0 - 1 : b0;
         o1[e] = hel(sg, g);
         const int t = wrap_t(t0 - 2 + i), x = wrap_x(x0 - 2 + jx);
-        lk1[e][0] = hop_link(L, Nt, Nx, V4, g, t, x);  // point g (0: none)
-        lk1[e][1] = g == 3 ? make_uint2(0u, 0u) : hop_link(L, Nt, Nx, V4, 4 + g, t, x);  // -x, none, none, zero
+        if constexpr (TG) {
+            e1a[e] = g == 1 ? b0 + W0 : (g == 2 ? b0 - W0 : b0);
+            e1b[e] = g == 1 ? b0 + 1 : (g == 2 ? b0 - 1 : b0);
+            lk1[e][0] = make_uint2(0u, 0u);
+            lk1[e][1] = g == 0 ? link_ops(Nb{0, 0, 1.f, 0.f})
+                                : (g == 3 ? make_uint2(0u, 0u) : link_ops_f2(tg_u(g == 1 ? 3 : 4, b0, t, x)));
+        } else {
+            e1a[e] = g == 1 ? b0 + W0 : (g == 2 ? b0 - W0 : (g == 3 ? b0 + 1 : b0));
+            e1b[e] = g == 0 ? b0 - 1 : b0;
+            lk1[e][0] = hop_link(L, Nt, Nx, V4, g, t, x);  // point g (0: none)
+            lk1[e][1] = g == 3 ? make_uint2(0u, 0u) : hop_link(L, Nt, Nx, V4, 4 + g, t, x);  // -x, none, none, zero
+        }
     }
     const int nt1 = __builtin_amdgcn_readfirstlane(min(kFuL1Tiles, ((n1 + 15) >> 4) - wv * kFuL1Tiles));
     auto layer1 = [&](auto Kc, int j) {
@@ -807,14 +897,18 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         const int s1 = j - 5;
         if (s1 < -2 || s1 > Ls + 1) return;
         constexpr int S0 = K & 3;
-        const int sb = (K + dsl) & 3;  // slot of the lane's step-1 point
+        const int sa = (K + dsa) & 3, sb = (K + dsb) & 3;  // slots of the lane's step-0 / step-1 points
         const bf16x4_t Wa0 = frag(Kc, 0, 0), Wa1 = frag(Kc, 0, 1);
 #pragma unroll
         for (int e = 0; e < kFuL1Tiles; ++e)
             if (e < nt1) {
-                const uint2 va = ring0[S0][e1a[e]];
+                const uint2 va = ring0[sa][e1a[e]];
                 const uint2 vb = ring0[sb][e1b[e]];
-                const bf16x4_t B0 = pack_pairs(transport(va.x, lk1[e][0]), transport(va.y, lk1[e][0]));
+                bf16x4_t B0;
+                if constexpr (TG)
+                    B0 = pack_pairs(va.x, va.y);
+                else
+                    B0 = pack_pairs(transport(va.x, lk1[e][0]), transport(va.y, lk1[e][0]));
                 const bf16x4_t B1 = pack_pairs(transport(vb.x, lk1[e][1]), transport(vb.y, lk1[e][1]));
                 f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wa0, B0, acc, 0, 0, 0);
@@ -844,12 +938,20 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             exm[e] = hel(b - 1, g);
             eo[e] = hel(sg, g);
             const int t = wrap_t(t0 - 1 + i), x = wrap_x(x0 - 1 + jx);
+            if constexpr (TG) {
+                const int pb = (i + 2) * W0 + jx + 2;
+                lk[e][0] = lk[e][1] = make_float2(1.f, 0.f);
+                lk[e][2] = tg_u(3, pb, t, x);
+                lk[e][3] = tg_u(4, pb, t, x);
+            } else {
 #pragma unroll
-            for (int p = 1; p <= 4; ++p) lk[e][p - 1] = hop_u(L, Nt, Nx, V4, p, t, x);
+                for (int p = 1; p <= 4; ++p) lk[e][p - 1] = hop_u(L, Nt, Nx, V4, p, t, x);
+            }
             win[e][0] = win[e][1] = make_uint2(0u, 0u);
         }
         const int nt = __builtin_amdgcn_readfirstlane(min(kFuL2Tiles, ((n2 + 15) >> 4) - wv * kFuL2Tiles));
-        auto step = [&](auto Kc, int jv, float4& in_new, uint4& w_new, const float4& in_old, const uint4& w_old) {
+        auto step = [&](auto Kc, int jv, Slices& sl, float4& in_new, uint4& w_new, const float4& in_old,
+                        const uint4& w_old) {
             constexpr int K = decltype(Kc)::value;
             const int j = __builtin_amdgcn_readfirstlane(jv);  // keep the walk's arithmetic scalar
             const int s2 = j - 7;
@@ -866,7 +968,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
                                 make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
                     }
             }
-            loads(j, in_new, w_new);
+            loads(sl, in_new, w_new);
             stores(Kc, j, in_old, w_old);
         };
         float4 inX = make_float4(0.f, 0.f, 0.f, 0.f), inY = inX;
@@ -874,11 +976,12 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         // the set-up's global loads (transporters) are complete before the walk, so no
         // wait inside it is for them (such a wait would also wait for the input in flight)
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        Slices sl = slices_at(-3);
         for (int j = -3; j < jend; j += 4) {
-            step(IC<0>{}, j, inX, wX, inY, wY);
-            step(IC<1>{}, j + 1, inY, wY, inX, wX);
-            step(IC<2>{}, j + 2, inX, wX, inY, wY);
-            step(IC<3>{}, j + 3, inY, wY, inX, wX);
+            step(IC<0>{}, j, sl, inX, wX, inY, wY);
+            step(IC<1>{}, j + 1, sl, inY, wY, inX, wX);
+            step(IC<2>{}, j + 2, sl, inX, wX, inY, wY);
+            step(IC<3>{}, j + 3, sl, inY, wY, inX, wX);
         }
     } else {
         // ---- output layer: tiles (wv-4)*2 + e of the TY x TX tile, input ring2 (width W2)
@@ -886,6 +989,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         int e3[kFuL3Tiles], e3p[kFuL3Tiles], e3m[kFuL3Tiles], e3xp[kFuL3Tiles], e3xm[kFuL3Tiles], o3[kFuL3Tiles];
         bool v3[kFuL3Tiles];
         float2 lk3[kFuL3Tiles][4];
+        float2 om3[kFuL3Tiles];  // TG: Om^* at the output site
         uint2 win3[kFuL3Tiles][2];
 #pragma unroll
         for (int e = 0; e < kFuL3Tiles; ++e) {
@@ -900,12 +1004,22 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             e3xp[e] = hel(b + 1, g);
             e3xm[e] = hel(b - 1, g);
             o3[e] = (t0 + i) * Nx + x0 + jx;
+            if constexpr (TG) {
+                const int pb = (i + 3) * W0 + jx + 3;
+                lk3[e][0] = lk3[e][1] = make_float2(1.f, 0.f);
+                lk3[e][2] = tg_u(3, pb, t0 + i, x0 + jx);
+                lk3[e][3] = tg_u(4, pb, t0 + i, x0 + jx);
+                om3[e] = conjf2(om[pb]);
+            } else {
 #pragma unroll
-            for (int p = 1; p <= 4; ++p) lk3[e][p - 1] = hop_u(L, Nt, Nx, V4, p, t0 + i, x0 + jx);
+                for (int p = 1; p <= 4; ++p) lk3[e][p - 1] = hop_u(L, Nt, Nx, V4, p, t0 + i, x0 + jx);
+                om3[e] = make_float2(1.f, 0.f);
+            }
             win3[e][0] = win3[e][1] = make_uint2(0u, 0u);
         }
         const int nt3 = __builtin_amdgcn_readfirstlane(min(kFuL3Tiles, ((n3 + 15) >> 4) - wr * kFuL3Tiles));
-        auto step = [&](auto Kc, int jv, float4& in_new, uint4& w_new, const float4& in_old, const uint4& w_old) {
+        auto step = [&](auto Kc, int jv, Slices& sl, float4& in_new, uint4& w_new, const float4& in_old,
+                        const uint4& w_old) {
             constexpr int K = decltype(Kc)::value;
             const int j = __builtin_amdgcn_readfirstlane(jv);  // keep the walk's arithmetic scalar
             const int s3 = j - 9;
@@ -916,10 +1030,18 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
                     if (e < nt3) {
                         const f32x4_t acc = hop8(Kc, IC<2>{}, IC<(K + 3) & 3>{}, IC<K & 3>{}, IC<(K + 1) & 3>{},
                                                  ring2, e3[e], e3p[e], e3m[e], e3xp[e], e3xm[e], lk3[e], win3[e], s3 == 0);
-                        if (v3[e]) st_spinor(outr, (long long)s3 * V4 + o3[e], acc[0], acc[1], acc[2], acc[3]);
+                        if (v3[e]) {
+                            if constexpr (TG) {
+                                const float2 c0 = cmulf(om3[e], make_float2(acc[0], acc[1]));
+                                const float2 c1 = cmulf(om3[e], make_float2(acc[2], acc[3]));
+                                st_spinor(outr, (long long)s3 * V4 + o3[e], c0.x, c0.y, c1.x, c1.y);
+                            } else {
+                                st_spinor(outr, (long long)s3 * V4 + o3[e], acc[0], acc[1], acc[2], acc[3]);
+                            }
+                        }
                     }
             }
-            loads(j, in_new, w_new);
+            loads(sl, in_new, w_new);
             stores(Kc, j, in_old, w_old);
         };
         float4 inX = make_float4(0.f, 0.f, 0.f, 0.f), inY = inX;
@@ -927,11 +1049,12 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         // the set-up's global loads (transporters) are complete before the walk, so no
         // wait inside it is for them (such a wait would also wait for the input in flight)
         __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        Slices sl = slices_at(-3);
         for (int j = -3; j < jend; j += 4) {
-            step(IC<0>{}, j, inX, wX, inY, wY);
-            step(IC<1>{}, j + 1, inY, wY, inX, wX);
-            step(IC<2>{}, j + 2, inX, wX, inY, wY);
-            step(IC<3>{}, j + 3, inY, wY, inX, wX);
+            step(IC<0>{}, j, sl, inX, wX, inY, wY);
+            step(IC<1>{}, j + 1, sl, inY, wY, inX, wX);
+            step(IC<2>{}, j + 2, sl, inX, wX, inY, wY);
+            step(IC<3>{}, j + 3, sl, inY, wY, inX, wX);
         }
     }
     if constexpr (DOTS) {
@@ -1376,6 +1499,8 @@ void repack(const dwf_net* net, hipStream_t s)
         k_pack<<<(unsigned)((nf + 255) / 256), 256, 0, s>>>(net->w[l], kCinL[l], kCoutL[l], net->Ls, 0, net->wf[l]);
         k_pack<<<(unsigned)((na + 255) / 256), 256, 0, s>>>(net->w[l], kCinL[l], kCoutL[l], net->Ls, 1, net->wa[l]);
     }
+    const long long ng = frag_shorts(kCinL[0], net->Ls);
+    k_pack<<<(unsigned)((ng + 255) / 256), 256, 0, s>>>(net->w[0], kCinL[0], kCoutL[0], net->Ls, 2, net->wg0);
     dwf_ctx::check_launch("pack weights");
 }
 
@@ -1416,6 +1541,8 @@ int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind
     a.link_stride = 2LL * L.V4;
     a.cfg = cfg;
     for (int l = 0; l < 3; ++l) a.wf[l] = net->wf[l];
+    const bool tg = c->net_fused == 2;
+    if (tg) a.wf[0] = net->wg0;
     a.st = st;
     a.q = q;
     a.part = part;
@@ -1423,11 +1550,20 @@ int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind
     hipStream_t s = c->stream;
     if (part) {
         if (in_kind != 1 || out_kind != 1) throw DwfError(DWF_ERR_ARG, "fused network: dot products need FP32 spinors");
-        k_net_fused<float, float, true><<<grid, kFuThreads, 0, s>>>(a);
+        if (tg)
+            k_net_fused<float, float, true, true><<<grid, kFuThreads, 0, s>>>(a);
+        else
+            k_net_fused<float, float, true, false><<<grid, kFuThreads, 0, s>>>(a);
     } else if (in_kind == 0 && out_kind == 0) {
-        k_net_fused<double, double, false><<<grid, kFuThreads, 0, s>>>(a);
+        if (tg)
+            k_net_fused<double, double, false, true><<<grid, kFuThreads, 0, s>>>(a);
+        else
+            k_net_fused<double, double, false, false><<<grid, kFuThreads, 0, s>>>(a);
     } else if (in_kind == 1 && out_kind == 1) {
-        k_net_fused<float, float, false><<<grid, kFuThreads, 0, s>>>(a);
+        if (tg)
+            k_net_fused<float, float, false, true><<<grid, kFuThreads, 0, s>>>(a);
+        else
+            k_net_fused<float, float, false, false><<<grid, kFuThreads, 0, s>>>(a);
     } else {
         throw DwfError(DWF_ERR_ARG, "fused network: unsupported spinor types");
     }
@@ -1592,6 +1728,7 @@ dwf_net::~dwf_net()
         (void)hipFree(m2[l]);
         (void)hipFree(wf[l]);
         (void)hipFree(wa[l]);
+        if (l == 0) (void)hipFree(wg0);
     }
     if (pack_stream) (void)hipStreamDestroy(pack_stream);
 }
@@ -1624,6 +1761,7 @@ extern "C" int dwf_net_create(dwf_net** out, int Ls, unsigned long long seed)
             hchk(hipMalloc(&net->m2[l], nw), "hipMalloc weights");
             hchk(hipMalloc(&net->wf[l], (size_t)frag_shorts(kCinL[l], Ls) * 2), "hipMalloc weights");
             hchk(hipMalloc(&net->wa[l], (size_t)frag_shorts(kCoutL[l], Ls) * 2), "hipMalloc weights");
+            if (l == 0) hchk(hipMalloc(&net->wg0, (size_t)frag_shorts(kCinL[0], Ls) * 2), "hipMalloc weights");
         }
         // Initial weights: identity embeddings on the self point (layer 3 scaled by 1/4,
         // about the inverse of D^dag D's typical eigenvalue), small random elsewhere.
