@@ -503,6 +503,18 @@ constexpr int lds_threads(int mode)
                        : kBlock;
 }
 
+// s_waitcnt lgkmcnt(0) (gfx9 encoding; vmcnt and expcnt unconstrained)
+constexpr int kWaitLgkm0 = 0xC07F;
+// Block barrier that first completes this wave's LDS writes.  __syncthreads() alone is
+// not enough at the top of a loop: the compiler drops the LDS wait of its fence on the
+// path that enters the loop header over the back-edge, so a wave could pass the barrier
+// while another wave's writes of the previous step are still in flight.
+__device__ __forceinline__ void lds_barrier()
+{
+    __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+    __syncthreads();
+}
+
 template <typename R, bool DAG, bool D5, int EPI, int NT>
 __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
 {
@@ -633,7 +645,7 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
         constexpr int b = d & 1;
         const bool active = kk < n;
         const int s = sa + kk * dir;
-        __syncthreads();  // slice s staged in lds[b]; nobody still reads lds[b ^ 1]
+        lds_barrier();  // slice s staged in lds[b]; nobody still reads lds[b ^ 1]
         // own sites of slice s + 2 dir (unless past the end or a kept wall slice)
         int oo = kBufOff;
         if (kk + 1 < n && (D5 || kk + 2 < n)) {
