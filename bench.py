@@ -290,16 +290,21 @@ def run_analyzer(args, dwf):
         c.close()
 
 
-def train_network(args, dwf, capi):
+def train_network(args, dwf, capi, hot=False):
     """Training pairs (D^dag eta, solution of D^dag D x = D^dag eta to 1e-10) from
     --learn-configs seeded configurations (seeds 1234 + 104729 i, i = 1 .. n: the test
-    configuration, seed 1234, is held out) with --learn-pairs full-volume Gaussian sources
+    configuration, seed 1234, is held out; theta ~ N(0, 1/beta), or uniform on [-pi, pi)
+    with hot=True, the HMC's hot start) with --learn-pairs full-volume Gaussian sources
     each, then device training.  Returns (network, statistics)."""
     V5 = NT * NX * LS
     ntrain, per = args.learn_configs, args.learn_pairs
     npairs = ntrain * per
-    thetas = np.stack([np.random.default_rng(1234 + 104729 * i).normal(0.0, 1.0 / np.sqrt(BETA), 2 * NT * NX)
-                       for i in range(1, ntrain + 1)])
+
+    def config(i):
+        g = np.random.default_rng(1234 + 104729 * i)
+        return g.uniform(-np.pi, np.pi, 2 * NT * NX) if hot else g.normal(0.0, 1.0 / np.sqrt(BETA), 2 * NT * NX)
+
+    thetas = np.stack([config(i) for i in range(1, ntrain + 1)])
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
     if args.no_graphs:
         ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
@@ -443,8 +448,15 @@ def run_hmc(args, dwf, capi):
         ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
     rng = capi.HostRng(1234)
     ndof, ngauge = LS * NT * NX * 2, 2 * NT * NX
-    method = {"cg64": dwf.DWF_SOLVE_CG64, "mixed": dwf.DWF_SOLVE_MIXED}[args.hmc_solver]
+    method = {"cg64": dwf.DWF_SOLVE_CG64, "mixed": dwf.DWF_SOLVE_MIXED,
+              "learned": dwf.DWF_SOLVE_LEARNED}[args.hmc_solver]
     eps, nsteps = 0.05, 20
+    net, tstats = None, None
+    if method == dwf.DWF_SOLVE_LEARNED:
+        # trained on hot-start configurations (held out: the trajectory's own seed)
+        net, tstats = train_network(args, dwf, capi, hot=True)
+        learned_solver_config(args, dwf, ctx, net)
+        ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA, args.reliable_delta)
 
     def trajectory():
         chi, mom = rng.dwf_heatbath(ndof, ngauge)
@@ -485,7 +497,7 @@ def run_hmc(args, dwf, capi):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.hmc_solver == "cg64" else "f32+f64",
+        "dtype": {"cg64": "f64", "mixed": "f32+f64", "learned": "bf16+f32+f64"}[args.hmc_solver],
         "data": "synthetic",
         "config": {
             "workload": "Nf=2 Shamir DWF pseudofermion HMC, 96x96 Ls=16 beta=4 mf=0.05 M5=1, tau=1, 20 leapfrog "
@@ -493,6 +505,7 @@ def run_hmc(args, dwf, capi):
             "force_solver": args.hmc_solver,
         },
         "cg_iterations_per_trajectory": cg_per_traj,
+        "training": tstats,
         "delta_h": [h["delta_h"] for h in hist],
         "plaquette": [h["plaquette"] for h in hist],
         "accepted": [h["accepted"] for h in hist],
@@ -501,6 +514,8 @@ def run_hmc(args, dwf, capi):
     }
     print(json.dumps(line), flush=True)
     ctx.close()
+    if net is not None:
+        net.close()
 
 
 def main():
@@ -515,7 +530,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("solves", "analyzer", "learned", "hmc"), default="solves")
-    ap.add_argument("--hmc-solver", choices=("cg64", "mixed"), default="mixed")
+    ap.add_argument("--hmc-solver", choices=("cg64", "mixed", "learned"), default="mixed")
     ap.add_argument("--learn-configs", type=int, default=64)
     ap.add_argument("--learn-pairs", type=int, default=64, help="training pairs per configuration")
     ap.add_argument("--learn-steps", type=int, default=1000)

@@ -60,11 +60,13 @@ enum dwf_method {
      * defect correction; certified on the reference's full system.  Needs even Nt, Nx
      * and Ls in {4, 8, 12, 16}; otherwise it runs as DWF_SOLVE_MIXED. */
     DWF_SOLVE_EO_MIXED = 2,
-    /* Domain-wall D^dag D only: flexible CG (FCG(1)) preconditioned by the learned
-     * network attached with dwf_ctx_set_preconditioner, FP32 cycles between FP64
-     * reliable updates, certified on the FP64 true residual; plain FP64 CG continues
-     * if the network breaks down (stats: fp64_iterations > 0).  No counterpart in the
-     * reference. */
+    /* Domain-wall contexts only: CG preconditioned by the learned network attached with
+     * dwf_ctx_set_preconditioner (DWF_TUNE_FCG_BETA picks flexible CG or
+     * Fletcher-Reeves), FP32 cycles between FP64 reliable updates, certified on the FP64
+     * true residual; plain FP64 CG continues if the network breaks down (stats:
+     * fp64_iterations > 0).  D^dag D directly; D D^dag (the pseudofermion force and HMC
+     * solves) as the same solve of Gamma b, Gamma = sigma3 R, since D D^dag =
+     * Gamma D^dag D Gamma.  No counterpart in the reference. */
     DWF_SOLVE_LEARNED = 3
 };
 
@@ -242,7 +244,7 @@ typedef struct dwf_hmc_params {
     double epsilon;   /* leapfrog step                                     */
     int steps;        /* leapfrog steps                                    */
     int integrator;   /* enum dwf_integrator                               */
-    int method;       /* force solver: DWF_SOLVE_CG64 or DWF_SOLVE_MIXED   */
+    int method;       /* force solver: DWF_SOLVE_CG64, _MIXED or _LEARNED  */
     int maxiter;      /* force CG iteration limit                          */
     double rtol;      /* force CG tolerance                                */
 } dwf_hmc_params;

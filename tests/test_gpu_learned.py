@@ -14,6 +14,7 @@ from conftest import print_case
 from dense_ref import rand_spinor, rand_theta
 
 import three_part_dwf_study_amd as dwf
+from three_part_dwf_study_amd import capi
 
 pytestmark = pytest.mark.gpu
 
@@ -342,3 +343,43 @@ def test_training_cuts_iterations():
             assert all(s["converged"] and s["true_relres"] <= 1e-10 for s in sl)
             assert all(s["fp64_iterations"] == 0 for s in sl)  # no fallback to plain CG
             assert il.mean() <= 0.6 * ic.mean()
+
+
+def test_learned_dddag_solve_force_and_hmc():
+    """DWF_SOLVE_LEARNED on D D^dag (as the rotated D^dag D solve): certified against the
+    oracle's D D^dag, with and without an initial guess; the pseudofermion action and
+    force agree with FP64 CG's, and a short HMC trajectory with learned force solves has
+    FP64 CG's Delta H."""
+    Nt, Nx, Ls = 16, 12, 8
+    print_case(test="learned_dddag", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(31)
+    th = rand_theta(rng, Nt, Nx)
+    b = rand_spinor(rng, Ls * Nt * Nx * 2)
+    net = dwf.Network(Ls, seed=3)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.set_preconditioner(net)
+        x, st, rc = ctx.solve(dwf.DWF_OP_DDDAG, b, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_LEARNED)
+        r = b - oracle.dwf_apply(dwf.DWF_OP_DDDAG, th, x, Nt, Nx, Ls, M5, MF)
+        assert st[0]["converged"] and np.linalg.norm(r) / np.linalg.norm(b) <= 1.01e-10
+        x0 = x + 1e-6 * rand_spinor(rng, x.size)
+        x2, st2, _ = ctx.solve(dwf.DWF_OP_DDDAG, b, x0=x0, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_LEARNED)
+        r2 = b - oracle.dwf_apply(dwf.DWF_OP_DDDAG, th, x2, Nt, Nx, Ls, M5, MF)
+        assert st2[0]["converged"] and np.linalg.norm(r2) / np.linalg.norm(b) <= 1.01e-10
+        assert st2[0]["iterations"] < st[0]["iterations"]
+        S, F, *_ = ctx.pf_action_force(b, 1e-12, 5000)
+        Sl, Fl, *_ = ctx.pf_action_force(b, 1e-12, 5000, method=dwf.DWF_SOLVE_LEARNED)
+        assert abs(Sl - S) <= 1e-10 * abs(S) and rel(Fl, F) <= 1e-9
+    hist = []
+    for method in (dwf.DWF_SOLVE_CG64, dwf.DWF_SOLVE_LEARNED):
+        r = capi.HostRng(77)
+        with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+            ctx.set_gauge(np.ones(2 * Nt * Nx))
+            ctx.set_preconditioner(net)
+            dh = []
+            for _ in range(2):
+                chi, mom = r.dwf_heatbath(Ls * Nt * Nx * 2, 2 * Nt * Nx)
+                res = ctx.hmc_trajectory(chi, mom, r.uniform(), 3.0, 0.05, 6, 1e-11, 10000, method=method)
+                dh.append(res["delta_h"])
+            hist.append(np.array(dh))
+    assert np.max(np.abs(hist[0] - hist[1])) <= 1e-8

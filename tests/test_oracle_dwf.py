@@ -84,6 +84,17 @@ def test_gamma5_R_hermiticity(rng):
     assert np.allclose(_apply(1, th, psi), G @ _apply(0, th, G @ psi), atol=1e-13)
 
 
+def test_normal_operators_related_by_gamma5_R(rng):
+    """D D^dag = (R gamma5) D^dag D (R gamma5): the learned solver runs D D^dag solves
+    (pseudofermion force, HMC) as D^dag D solves of the rotated source."""
+    th = rand_theta(rng, NT, NX, 0.8)
+    psi = rand_spinor(rng, LS * NT * NX * 2)
+    G = reflect_gamma5(NT, NX, LS)
+    lhs = _apply(2, th, psi)
+    assert np.allclose(lhs, G @ _apply(3, th, G @ psi), atol=1e-13 * np.abs(lhs).max())
+    assert not np.allclose(lhs, _apply(3, th, psi), atol=1e-3)  # the two operators differ
+
+
 def test_gauge_covariance(rng):
     th = rand_theta(rng, NT, NX, 0.9).reshape(2, NT, NX)
     alpha = rng.uniform(-np.pi, np.pi, size=(NT, NX))

@@ -90,6 +90,7 @@ enum BufId : int {
     B_FCG_T32,
     B_FCG_X32,
     B_FCG_NPART,
+    B_GAMMA_B,   // learned D D^dag solves: Gamma b
     B_COUNT
 };
 
@@ -151,5 +152,9 @@ int dwf_solve_dirac_impl(dwf_ctx* c, int method, const double* src, const double
                          int nrhs, double* x, dwf_solve_stats* stats);
 int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, double rtol, int maxiter, int nrhs,
                   dwf_solve_stats* stats);
+// Learned-preconditioner solve of A x = b, A = D^dag D or D D^dag (device buffers), with
+// the plain FP64 CG fallback; D D^dag = Gamma D^dag D Gamma (Gamma = sigma3 R).
+int dwf_solve_learned_impl(dwf_ctx* c, int normal, const double* b, const double* x0, double rtol, int maxiter,
+                           int nrhs, double* x, dwf_solve_stats* stats);
 int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0,
                           double rtol, int maxiter, int nrhs, double* x);

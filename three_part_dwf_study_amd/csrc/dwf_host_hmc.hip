@@ -65,13 +65,40 @@ struct PfOut {
     int rc = DWF_OK;
 };
 
+PfOut check_cg(dwf_ctx* c, int method, double rtol);
+
 // eta = (D D^dag)^-1 phi (guess nullable, may alias eta), S -> scal[S_PF], force.
 PfOut pf_dev(Dev& d, int method, const double* phi, const double* guess, double rtol, int maxiter, double* eta,
              double* force)
 {
     dwf_ctx* c = d.c;
     PfOut o;
-    dwf_solve_device_impl(c, DWF_OP_DDDAG, method, phi, guess, rtol, maxiter, 1, eta);
+    if (method == DWF_SOLVE_LEARNED) {
+        // learned preconditioner with the certified FP64 CG fallback (certified or failed)
+        const int rc = dwf_solve_learned_impl(c, DWF_OP_DDDAG, phi, guess, rtol, maxiter, 1, eta, &o.stats);
+        o.iterations = o.stats.iterations + o.stats.fp64_iterations;
+        if (rc != DWF_OK || !o.stats.converged) {
+            o.rc = DWF_ERR_NOT_CONVERGED;  // pseudofermion.hpp:39-41
+            dwf_set_error("Pseudofermion CG failed to converge");
+            return o;
+        }
+    } else {
+        dwf_solve_device_impl(c, DWF_OP_DDDAG, method, phi, guess, rtol, maxiter, 1, eta);
+        o = check_cg(c, method, rtol);
+        if (o.rc != DWF_OK) return o;
+    }
+    launch_redot(phi, eta, (long long)d.ncx, d.rpart, d.scal, S_PF, c->stream);  // S = Re phi^dag eta
+    double* xi = d.vec(B_XI);
+    dwf_apply_device_impl(c, DWF_OP_DDAG, DWF_PREC_FP64, eta, xi, 1);
+    launch_pf_force(c->L, c->l64, eta, xi, force, c->stream);
+    c->check_launch("pseudofermion force");
+    return o;
+}
+
+// State of the CG64 / MIXED solve just run: stats, and the reference's failure rules.
+PfOut check_cg(dwf_ctx* c, int method, double rtol)
+{
+    PfOut o;
     const CgState& S = c->st_host[0];
     const bool done = (S.flags[c->last_slot] & F_DONE) && S.status == 0;
     o.stats.iterations = S.iters;
@@ -96,11 +123,6 @@ PfOut pf_dev(Dev& d, int method, const double* phi, const double* guess, double 
         dwf_set_error(msg.str());
         return o;
     }
-    launch_redot(phi, eta, (long long)d.ncx, d.rpart, d.scal, S_PF, c->stream);  // S = Re phi^dag eta
-    double* xi = d.vec(B_XI);
-    dwf_apply_device_impl(c, DWF_OP_DDAG, DWF_PREC_FP64, eta, xi, 1);
-    launch_pf_force(c->L, c->l64, eta, xi, force, c->stream);
-    c->check_launch("pseudofermion force");
     return o;
 }
 
@@ -131,7 +153,8 @@ int dwf_pf_action_force(dwf_ctx* c, int method, const double* phi, const double*
 {
     return guard([&]() -> int {
         require(c);
-        if (!phi || !action || !force || maxiter < 0 || (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED))
+        if (!phi || !action || !force || maxiter < 0 ||
+            (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_LEARNED))
             throw DwfError(DWF_ERR_ARG, "dwf_pf_action_force: bad argument");
         Dev d(c);
         double* dphi = d.vec(B_PHI);
@@ -216,7 +239,7 @@ int dwf_hmc_trajectory(dwf_ctx* c, const dwf_hmc_params* P, const double* chi, c
         require(c);
         if (!P || !chi || !momentum || !res || P->steps < 0 || P->maxiter < 0 ||
             (P->integrator != DWF_HMC_DOMAIN_WALL && P->integrator != DWF_HMC_WILSON) ||
-            (P->method != DWF_SOLVE_CG64 && P->method != DWF_SOLVE_MIXED))
+            (P->method != DWF_SOLVE_CG64 && P->method != DWF_SOLVE_MIXED && P->method != DWF_SOLVE_LEARNED))
             throw DwfError(DWF_ERR_ARG, "dwf_hmc_trajectory: bad argument");
         Dev d(c);
         const int ng = d.ng;

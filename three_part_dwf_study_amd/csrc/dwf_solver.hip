@@ -474,15 +474,50 @@ int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, c
 
 namespace {
 
+// v <- Gamma v in place, Gamma = sigma3 R (R: s -> Ls-1-s; sigma3 = diag(1, -1) on the
+// two spin components), for nrhs spinors.  Gamma^2 = 1 and D^dag = Gamma D Gamma
+// (dwf_operator.hpp:94-171), so D D^dag = Gamma (D^dag D) Gamma.
+__global__ void k_gamma_r(double* v, long long V4, int Ls, int nrhs)
+{
+    const long long V5 = V4 * Ls, half = V4 * ((Ls + 1) / 2);
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int rhs = blockIdx.y;
+    if (i >= half) return;
+    const int s = (int)(i / V4);
+    const long long site = i - (long long)s * V4;
+    double4* p = reinterpret_cast<double4*>(v + (long long)rhs * V5 * 4);
+    const long long a = (long long)s * V4 + site, b = (long long)(Ls - 1 - s) * V4 + site;
+    const double4 u = p[a], w = p[b];
+    p[a] = make_double4(w.x, w.y, -w.z, -w.w);
+    if (b != a) p[b] = make_double4(u.x, u.y, -u.z, -u.w);
+}
+
+void gamma_r(dwf_ctx* c, double* v, int nrhs)
+{
+    const long long half = (long long)c->L.V4 * ((c->L.Ls + 1) / 2);
+    const dim3 grid((unsigned)((half + kBlock - 1) / kBlock), (unsigned)nrhs);
+    k_gamma_r<<<grid, kBlock, 0, c->stream>>>(v, c->L.V4, c->L.Ls, nrhs);
+}
+
 // D^dag D x = b by flexible CG with the learned preconditioner (FP32 cycles, FP64
 // reliable updates).  If it breaks down (the network is not positive on some residual)
 // or misses rtol, plain FP64 CG continues from its iterate and decides; the stats then
 // count both (fp64_iterations = the fallback's iterations, 0 without fallback).
-int solve_learned(dwf_ctx* c, const double* b, const double* x0, double rtol, int maxiter, int nrhs, double* x,
-                  dwf_solve_stats* stats)
+// D D^dag x = b is the same solve of the Gamma-rotated system: b' = Gamma b,
+// x = Gamma x' (the residual norms are the same, Gamma being unitary).
+int solve_learned(dwf_ctx* c, int normal, const double* b, const double* x0, double rtol, int maxiter, int nrhs,
+                  double* x, dwf_solve_stats* stats)
 {
     const size_t nv = (size_t)c->L.V5() * 4 * nrhs;
     if (x0 && x0 != x) c->check(hipMemcpyAsync(x, x0, nv * 8, hipMemcpyDeviceToDevice, c->stream), "copy x0");
+    const bool rot = normal == DWF_OP_DDDAG;
+    if (rot) {
+        double* gb = (double*)c->buf(B_GAMMA_B, nv * 8);
+        c->check(hipMemcpyAsync(gb, b, nv * 8, hipMemcpyDeviceToDevice, c->stream), "copy b");
+        gamma_r(c, gb, nrhs);
+        if (x0) gamma_r(c, x, nrhs);
+        b = gb;
+    }
     std::vector<dwf_solve_stats> first(nrhs);
     int rc = dwf_fcg_solve(c, b, x, x0 != nullptr, rtol, maxiter, nrhs, first.data());
     if (rc == DWF_ERR_ZERO_DENOM || rc == DWF_ERR_NOT_CONVERGED) {
@@ -498,6 +533,8 @@ int solve_learned(dwf_ctx* c, const double* b, const double* x0, double rtol, in
         first = second;
         if (rc == DWF_ERR_NOT_CONVERGED) rc = DWF_ERR_CERTIFY;
     }
+    if (rot) gamma_r(c, x, nrhs);
+    c->check_launch("learned solve");
     if (stats) std::copy(first.begin(), first.end(), stats);
     return rc;
 }
@@ -507,12 +544,11 @@ int solve_normal(dwf_ctx* c, int normal, int method, const double* b, const doub
 {
     if (!c || !b || !x || nrhs < 1 || maxiter < 0 || !(rtol >= 0.0) ||
         (normal != DWF_OP_DDDAG && normal != DWF_OP_DDAGD) ||
-        (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED &&
-         !(method == DWF_SOLVE_LEARNED && normal == DWF_OP_DDAGD)))
+        (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_LEARNED))
         throw DwfError(DWF_ERR_ARG, "dwf_solve: bad argument");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve: no gauge field set");
     c->bind_rhs(nrhs);
-    if (method == DWF_SOLVE_LEARNED) return solve_learned(c, b, x0, rtol, maxiter, nrhs, x, stats);
+    if (method == DWF_SOLVE_LEARNED) return solve_learned(c, normal, b, x0, rtol, maxiter, nrhs, x, stats);
     dwf_solve_device_impl(c, normal, method, b, x0, rtol, maxiter, nrhs, x);
     int rc = collect(c, method, nrhs, stats);
     if (method == DWF_SOLVE_MIXED && rc == DWF_ERR_NOT_CONVERGED) {
@@ -690,7 +726,7 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
         a.out = B;
         launch_hop<double>(true, EPI_STORE, a, nrhs, c->stream);
     }
-    if (method == DWF_SOLVE_LEARNED) return solve_learned(c, B, x0, rtol, maxiter, nrhs, x, stats);
+    if (method == DWF_SOLVE_LEARNED) return solve_learned(c, DWF_OP_DDAGD, B, x0, rtol, maxiter, nrhs, x, stats);
     return solve_normal(c, DWF_OP_DDAGD, method, B, x0, rtol, maxiter, nrhs, x, stats);
 }
 
@@ -708,6 +744,12 @@ int guard(const std::function<int()>& f)
 }
 
 }  // namespace
+
+int dwf_solve_learned_impl(dwf_ctx* c, int normal, const double* b, const double* x0, double rtol, int maxiter,
+                           int nrhs, double* x, dwf_solve_stats* stats)
+{
+    return solve_learned(c, normal, b, x0, rtol, maxiter, nrhs, x, stats);
+}
 
 int dwf_solve_dirac_impl(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter,
                          int nrhs, double* x, dwf_solve_stats* stats)
