@@ -79,6 +79,16 @@ def dslash_bytes(prec_bytes, nrhs):
     return nrhs * 2 * Bv + BL
 
 
+def cg_iteration_bytes(prec_bytes):
+    """Algorithmic bytes of one CG iteration on D^dag D for one right-hand side: two
+    stencils (2 B_v + B_L each) and nine vector passes (x, r update with |r|^2: 4 reads,
+    2 writes; p update: 2 reads, 1 write)."""
+    V4 = NT * NX
+    Bv = V4 * LS * 4 * prec_bytes
+    BL = V4 * 4 * prec_bytes
+    return 2 * (2 * Bv + BL) + 9 * Bv
+
+
 def time_kernel(ctx, capi, launch, reps):
     """Average duration (ms) of `launch` over `reps` back-to-back launches, from HIP
     events recorded on the library's stream around the timed region."""
@@ -365,14 +375,16 @@ def learned_solver_config(args, dwf, ctx, net):
 def run_learned(args, dwf, capi):
     V5 = NT * NX * LS
     net, tstats = train_network(args, dwf, capi)
-    theta = np.random.default_rng(1234).normal(0.0, 1.0 / np.sqrt(BETA), 2 * NT * NX)
-
-    # held-out configuration (seed 1234): --nrhs boundary sources per step
+    nrhs = args.nrhs
+    # --nrhs held-out configurations (seeds 1234 + 104729 i, i = ntrain + 1 .., which the
+    # training never saw), one boundary source each, solved as one batch
+    i0 = args.learn_configs + 1
+    thetas = np.stack([np.random.default_rng(1234 + 104729 * i).normal(0.0, 1.0 / np.sqrt(BETA), 2 * NT * NX)
+                       for i in range(i0, i0 + nrhs)])
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
-    ctx.set_gauge(theta)
+    ctx.set_gauge_batch(thetas)
     if args.no_graphs:
         ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
-    nrhs = args.nrhs
     tuned64 = ctx.autotune(nrhs, dwf.DWF_PREC_FP64, args.tune_cache)
     tuned = ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)
     log(f"autotuned stencil shapes for {nrhs} RHS: FP32 {tuned}, FP64 {tuned64}")
@@ -419,7 +431,8 @@ def run_learned(args, dwf, capi):
         "config": {
             "workload": f"learned preconditioner: trained on {tstats['training_pairs']} (source, solution) pairs "
                         f"from {tstats['training_configs']} configurations (full-volume Gaussian sources), tested "
-                        f"on a held-out configuration, {nrhs} boundary sources per step, FP64 true residual <= 1e-10",
+                        f"on {nrhs} held-out configurations (seeds 1234 + 104729 i, i = {i0} .. {i0 + nrhs - 1}), "
+                        f"one boundary source each, solved as one batch per step, FP64 true residual <= 1e-10",
             "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs,
             "network": "3 linear parallel-transport conv layers 2-8-8-2 complex channels, BF16 MFMA",
             "solver": solver,
@@ -700,6 +713,10 @@ def main():
         "mixed_solves_per_s": round(results["mixed"]["solves_per_s"], 3),
         "pcie_inclusive_solves_per_s": {k: round(v, 3) for k, v in pcie.items()},
         "iterations": {k: v["iterations"] for k, v in results.items()},
+        # HBM-roofline ceiling of the CG solvers: 8 TB/s / (iterations x algorithmic bytes per iteration)
+        "roofline_ceiling_solves_per_s": {
+            "fp64_cg": round(8e12 / (results["fp64_cg"]["iterations"] * cg_iteration_bytes(8)), 1),
+            "mixed": round(8e12 / (results["mixed"]["iterations"] * cg_iteration_bytes(4)), 1)},
         "reliable_updates": {k: v["reliable_updates"] for k, v in results.items()},
         "max_true_relres": {k: v["max_true_relres"] for k, v in results.items()},
         "dslash_gbs": {"fp32_batch%d" % args.dslash_rhs: round(gbs32, 1),
