@@ -111,8 +111,8 @@ def apply_shape(ctx, dwf, shape):
     ctx.tune(dwf.DWF_TUNE_LDS, shape["lds"])
 
 
-# Threads per block of each LDS tile mode of the stencil (DWF_TUNE_LDS 1..7).
-LDS_THREADS = {1: 256, 2: 512, 3: 64, 4: 128, 5: 384, 6: 768, 7: 1024}
+# Threads per block of each LDS tile mode of the stencil (DWF_TUNE_LDS 1..6).
+LDS_THREADS = {1: 256, 2: 512, 3: 64, 4: 128, 5: 384, 6: 768}
 
 
 def time_dslash(ctx, dwf, capi, prec, nrhs, reps):

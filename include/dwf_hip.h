@@ -111,8 +111,8 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * (non-temporal) stores of stencil outputs (the LDS-staged tiles always stream); DWF_TUNE_GRAPHS: 0/1 (default 1) replay
  * blocks of 8 CG iterations as instantiated HIP graphs instead of launching their
  * kernels one by one (same kernels, same order, same results); DWF_TUNE_LDS: 0, or
- * 1 / 2 / 3 / 4 / 5 / 6 / 7 to stage each slice of a 2D stencil tile of 256 / 512 / 64 /
- * 128 / 384 / 768 / 1024 sites and its halo in LDS (with DWF_TUNE_TILE_X between 8 and 128 dividing
+ * 1 / 2 / 3 / 4 / 5 / 6 to stage each slice of a 2D stencil tile of 256 / 512 / 64 /
+ * 128 / 384 / 768 sites and its halo in LDS (with DWF_TUNE_TILE_X between 8 and 128 dividing
  * Nx and the tile size, sites/width dividing Nt, and 2 (width + sites/width) <= sites;
  * ignored otherwise); DWF_TUNE_NET_FUSED: the learned preconditioner's kernels, 0
  * layer by layer, 1 the three layers as one fused kernel (LDS-resident hidden layers;

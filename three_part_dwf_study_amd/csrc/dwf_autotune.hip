@@ -3,7 +3,7 @@
 // For one (Nt, Nx, Ls, nrhs, precision) it times the D apply over the candidate shapes
 // -- block mapping (consecutive sites or 2D tiles of width 16/32/64 dividing Nx),
 // software-pipelined slice loop on/off, s-chunks per 4D site, streaming stores on/off,
-// and the LDS-staged tile kernel (blocks of 64..1024 threads, tiles of width 8..128 that
+// and the LDS-staged tile kernel (blocks of 64..768 threads, tiles of width 8..128 that
 // cover the lattice, full rows included) -- with HIP events on the
 // context's stream, keeps the fastest and applies it to the context.  Results can be
 // cached in a text file (one line per key), so a production run tunes once.
@@ -106,7 +106,7 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
             for (int pf : {0, 1})
                 for (int ch : chunks)
                     for (int nt : {0, 1}) cand.push_back(Shape{tx, pf, ch, nt, 0});
-        for (int mode : {1, 2, 3, 4, 5, 6, 7})
+        for (int mode : {1, 2, 3, 4, 5, 6})
             for (int tx : {8, 16, 32, 64, 96, 128})
                 if (hop_cfg(L, 1, 1, 0, 0, tx, 0, mode).lds == mode)
                     for (int ch : chunks)

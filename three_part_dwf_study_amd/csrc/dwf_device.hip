@@ -491,16 +491,10 @@ constexpr int lds_max(int nt)
         }
     return m;
 }
-// Threads per block of LDS tile mode 1..7.
+// Threads per block of LDS tile mode 1..6.
 constexpr int lds_threads(int mode)
 {
-    return mode == 2   ? 512
-           : mode == 3 ? 64
-           : mode == 4 ? 128
-           : mode == 5 ? 384
-           : mode == 6 ? 768
-           : mode == 7 ? 1024
-                       : kBlock;
+    return mode == 2 ? 512 : mode == 3 ? 64 : mode == 4 ? 128 : mode == 5 ? 384 : mode == 6 ? 768 : kBlock;
 }
 
 // s_waitcnt lgkmcnt(0) (gfx9 encoding; vmcnt and expcnt unconstrained)
@@ -942,7 +936,6 @@ void dispatch_lds(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
         case 4: dispatch_lds_nt<R, DAG, D5, 128>(epi, a, grid, s); break;
         case 5: dispatch_lds_nt<R, DAG, D5, 384>(epi, a, grid, s); break;
         case 6: dispatch_lds_nt<R, DAG, D5, 768>(epi, a, grid, s); break;
-        case 7: dispatch_lds_nt<R, DAG, D5, 1024>(epi, a, grid, s); break;
         default: dispatch_lds_nt<R, DAG, D5, 256>(epi, a, grid, s); break;
     }
 }
@@ -972,7 +965,7 @@ HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefe
     c.tiles_x = 0;
     c.bx = (L.V4 + kBlock - 1) / kBlock;
     // (the LDS kernel addresses one right-hand side's spinors with 32-bit byte offsets)
-    if (lds >= 1 && lds <= 7 && L.V5() * 32 < kBufOff) {
+    if (lds >= 1 && lds <= 6 && L.V5() * 32 < kBufOff) {
         const int nt = lds_threads(lds);
         if (lds_tile_ok(nt, tile_x) && L.Nx % tile_x == 0 && L.Nt % (nt / tile_x) == 0) {
             c.lds = lds;

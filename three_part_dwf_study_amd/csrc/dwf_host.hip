@@ -324,7 +324,7 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             ctx->use_graphs = value != 0.0 ? 1 : 0;
             return DWF_OK;
         case DWF_TUNE_LDS:
-            if (!(value >= 0.0 && value <= 7.0 && value == (double)(int)value)) break;
+            if (!(value >= 0.0 && value <= 6.0 && value == (double)(int)value)) break;
             ctx->lds = (int)value;
             return DWF_OK;
         case DWF_TUNE_NET_FUSED:

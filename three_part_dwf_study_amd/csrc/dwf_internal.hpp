@@ -53,7 +53,7 @@ struct HopCfg {
     int tx;        // 0: block = 256 consecutive 4D sites; else a 2D tile of (256/tx) rows x tx columns
     int tiles_x;   // Nx / tx
     int nt_store;  // 1: streaming (non-temporal) stores of the stencil output
-    int lds;       // 1..7: LDS-staged slice tiles of 256 / 512 / 64 / 128 / 384 / 768 / 1024 threads
+    int lds;       // 1..6: LDS-staged slice tiles of 256 / 512 / 64 / 128 / 384 / 768 threads
     __host__ __device__ int nblk() const { return bx * chunks; }
 };
 
