@@ -640,6 +640,11 @@ static_assert(4 * kFuL2Tiles * 16 >= kFuR2 && 4 * kFuL3Tiles * 16 >= kFuTy * kFu
 // with 2 * bit 3 of the site: 16 consecutive sites then cover the 64 banks once.
 __device__ __forceinline__ int hel(int site, int g) { return site * 4 + (g ^ ((site >> 2) & 2)); }
 
+struct FcgScal {
+    double rz, pq, alpha, beta;
+    double rz_prev;  // (z, r) of the previous iteration (Fletcher-Reeves beta)
+};
+
 struct NetFusedArgs {
     int Nt, Nx, Ls, V4;
     int TY, TX;            // output tile, dividing Nt and Nx
@@ -653,6 +658,13 @@ struct NetFusedArgs {
     const CgState* st;     // nullable: skip right-hand sides with flags[0] & (F_DONE | F_FROZEN)
     const float* q;        // DOTS: q = A p of the flexible CG
     double* part;          // DOTS: [nrhs][tiles][2] partials of (z, r), (z, q)
+    // DOTS, optional (null hpart): the CG iteration's check before the network (in place of
+    // k_fcg_check): (p, A p) and |r|^2 from the stencils' partials hpart, rpart ([nrhs][hnb][2]).
+    const double* hpart;
+    const double* rpart;
+    int hnb;
+    CgState* cst;          // the same states as st, updated by the block of tile 0
+    FcgScal* fs;
 };
 
 // s_waitcnt immediate (gfx9 encoding): vmcnt(0), expcnt and lgkmcnt unconstrained
@@ -711,6 +723,31 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
     const int rhs = lb / a.tiles, tile = lb - rhs * a.tiles;
     if (a.st && (a.st[rhs].flags[0] & (F_DONE | F_FROZEN))) return;  // block-uniform
+    if constexpr (DOTS) {
+        if (a.hpart && tile == 0) {
+            // the iteration's check (as k_fcg_check), by the block of tile 0 alone; the
+            // other blocks compute z regardless (a right-hand side that converges or breaks
+            // down here wastes one network application; the kernels after this one skip it)
+            if (threadIdx.x < 64) {
+                const double pq = fold64(a.hpart + (long long)rhs * a.hnb * 2, a.hnb, 0);
+                const double rr = fold64(a.rpart + (long long)rhs * a.hnb * 2, a.hnb, 0);
+                if (threadIdx.x == 0) {
+                    CgState& S = a.cst[rhs];
+                    if (!(pq > 0.0)) {
+                        S.status = 4;
+                        S.flags[0] |= F_DONE;
+                    } else {
+                        a.fs[rhs].pq = pq;
+                        a.fs[rhs].rz_prev = a.fs[rhs].rz;  // consumed by this iteration's alpha
+                        S.rr[0] = rr;
+                        S.iters += 1;
+                        S.relres = std::sqrt(rr / S.bnorm2);
+                        if (S.relres <= S.tol || rr <= S.delta * S.delta * S.rr_ref) S.flags[0] |= F_FROZEN;
+                    }
+                }
+            }
+        }
+    }
     const int TY = a.TY, TX = a.TX;
     const int W0 = TX + 6, W1 = TX + 4, W2 = TX + 2;
     const int n0 = (TY + 6) * W0, n1 = (TY + 4) * W1, n2 = (TY + 2) * W2, n3 = TY * TX;
@@ -1174,11 +1211,6 @@ __global__ void k_loss_fold(const double* part, int nb, const int* row, const do
 }
 
 // ------------------------------------------- flexible CG (FP32 + FP64 reliable updates)
-struct FcgScal {
-    double rz, pq, alpha, beta;
-    double rz_prev;  // (z, r) of the previous iteration (Fletcher-Reeves beta)
-};
-
 template <typename R> struct V4R;
 template <> struct V4R<float> { using T = float4; };
 template <> struct V4R<double> { using T = double4; };
@@ -1520,8 +1552,18 @@ bool fused_shape(const Lat& L, int& TY, int& TX)
 // Fused z = M in (k_net_fused); with part also the (z, in) and, with q, (z, q) partials,
 // part [nrhs][tiles][2].  Returns the number of tiles per lattice, or 0 if the shape does not
 // allow the fused kernel (then nothing is launched).
+// The FCG iteration's check folded into the fused network launch (see NetFusedArgs).
+struct FusedCheck {
+    const double* hpart;
+    const double* rpart;
+    int hnb;
+    CgState* st;
+    FcgScal* fs;
+};
+
 int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind, void* out, int out_kind,
-                     const int* cfg, int nrhs, const CgState* st, const float* q, double* part)
+                     const int* cfg, int nrhs, const CgState* st, const float* q, double* part,
+                     const FusedCheck* chk = nullptr)
 {
     const Lat& L = c->L;
     int TY, TX;
@@ -1546,6 +1588,14 @@ int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind
     a.st = st;
     a.q = q;
     a.part = part;
+    if (chk) {
+        if (!part) throw DwfError(DWF_ERR_ARG, "fused network: the check needs the dot-product epilogue");
+        a.hpart = chk->hpart;
+        a.rpart = chk->rpart;
+        a.hnb = chk->hnb;
+        a.cst = chk->st;
+        a.fs = chk->fs;
+    }
     const unsigned grid = (unsigned)(a.tiles * nrhs);
     hipStream_t s = c->stream;
     if (part) {
@@ -2027,7 +2077,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         a.part = hpart;
         launch_hop<float>(false, EPI_NORM, a, nrhs, s);
         // q = D^dag t with x += alpha p, r -= alpha q and |r|^2 in the stencil's epilogue
-        // (alpha = (z, r) / (p, A p)), then the breakdown and convergence checks
+        // (alpha = (z, r) / (p, A p))
         a.in = t32;
         a.out = fr ? nullptr : q32;  // Fletcher-Reeves beta needs no q
         a.x = x32;
@@ -2038,12 +2088,15 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         a.anum_stride = (int)(sizeof(FcgScal) / sizeof(double));
         a.part = rpart;
         launch_hop<float>(true, EPI_CGUPD, a, nrhs, s);
-        k_fcg_check<<<nrhs, 64, 0, s>>>(hpart, rpart, hnb, st, fs);
-        // z = M r with (z, r) and (z, q) accumulated by the fused network kernel, or
-        // layer by layer and a separate dot-product pass; then beta and p = z + beta p
-        if (launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, fr ? nullptr : q32, npart)) {
+        // the breakdown and convergence checks, then z = M r with (z, r) and (z, q): the
+        // fused network kernel does all three (checks at its start, dots in its epilogue);
+        // else a check kernel, the layer kernels and a dot-product pass.  Then beta and
+        // p = z + beta p.
+        const FusedCheck chk{hpart, rpart, hnb, st, fs};
+        if (launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, fr ? nullptr : q32, npart, &chk)) {
             k_fcg_pbeta<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs, npart, ntiles, fr);
         } else {
+            k_fcg_check<<<nrhs, 64, 0, s>>>(hpart, rpart, hnb, st, fs);
             net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
             k_fcg_dots<float><<<vgrid, 256, 0, s>>>(z32, r32, q32, sites, st, part);
             k_fcg_pbeta<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs, part, nb, fr);
