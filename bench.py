@@ -386,7 +386,8 @@ def run_learned(args, dwf, capi):
     if args.no_graphs:
         ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
     tuned64 = ctx.autotune(nrhs, dwf.DWF_PREC_FP64, args.tune_cache)
-    tuned = ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)
+    tuned = parse_shape(args.solve_shape) if args.solve_shape else ctx.autotune(nrhs, dwf.DWF_PREC_FP32,
+                                                                                args.tune_cache)
     log(f"autotuned stencil shapes for {nrhs} RHS: FP32 {tuned}, FP64 {tuned64}")
     solver = learned_solver_config(args, dwf, ctx, net)
     src = boundary_sources(nrhs, 918273)
@@ -538,6 +539,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nrhs", type=int, default=32, help="right-hand sides per step")
     ap.add_argument("--dslash-rhs", type=int, default=128)
+    ap.add_argument("--solve-shape", default=None,
+                    help="'lds,tile_x': the FP32 solvers' stencil shape instead of the autotuned one")
     ap.add_argument("--dslash-shape", default=None,
                     help="'lds,tile_x': time the FP32 D apply in this LDS tile shape instead of the autotuned one")
     ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
@@ -609,7 +612,8 @@ def main():
 
     # each solver runs with the stencil shape tuned for the precision its iterations use
     tuned = {"solves_fp64": ctx.autotune(nrhs, dwf.DWF_PREC_FP64, args.tune_cache),
-             "solves": ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache)}
+             "solves": (parse_shape(args.solve_shape) if args.solve_shape
+                        else ctx.autotune(nrhs, dwf.DWF_PREC_FP32, args.tune_cache))}
     log(f"autotuned stencil shapes for {nrhs} RHS: FP32 {tuned['solves']}, FP64 {tuned['solves_fp64']}")
     learned_solver = learned_solver_config(args, dwf, ctx, net)
     n = nrhs * LS * NT * NX * 4

@@ -91,6 +91,7 @@ enum BufId : int {
     B_FCG_X32,
     B_FCG_NPART,
     B_GAMMA_B,   // learned D D^dag solves: Gamma b
+    B_FCG_P32B,  // learned solver: the second direction buffer of the fused iteration
     B_COUNT
 };
 

@@ -21,6 +21,7 @@
 // Reductions are deterministic: fixed wavefront shuffle trees, one partial per block,
 // partials summed in a fixed order by whichever kernel consumes them.
 
+#include "dwf_context.hpp"
 #include "dwf_internal.hpp"
 
 #include <cmath>
@@ -533,6 +534,30 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
         if (pAp == 0.0 || (a.anum && !(pAp > 0.0))) return;  // block-uniform; a later kernel records the failure
         alpha = (R)((a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp);
     }
+    constexpr bool kPn = EPI == EPI_PNORM;
+    R beta = 0;
+    bool use_p = false;
+    if constexpr (kPn) {
+        // beta of the direction update p = z + beta p_old, from the network's partials; every
+        // block folds them in the same order, block 0 records (z, r) and beta
+        const double zr = sum_partials(a.zpart + (size_t)rhs * a.nz * 2, a.nz, 0, sh);
+        const double zq = sum_partials(a.zpart + (size_t)rhs * a.nz * 2, a.nz, 1, sh);
+        if (!(zr > 0.0)) {  // the preconditioner is not positive on this residual
+            if (bid == 0 && threadIdx.x == 0) {
+                a.stw[rhs].status = 5;
+                a.stw[rhs].flags[0] |= F_DONE;
+            }
+            return;  // block-uniform
+        }
+        const FcgScal& f = a.fs[rhs];
+        const double b = a.fr ? zr / f.rz_prev : -zq / f.pq;
+        if (bid == 0 && threadIdx.x == 0) {
+            a.fs[rhs].rz = zr;
+            a.fs[rhs].beta = b;
+        }
+        beta = (R)b;
+        use_p = b != 0.0;
+    }
 
     const int V4 = L.V4, Nx = L.Nx, Nt = L.Nt, Ls = L.Ls;
     const long long V5 = L.V5();
@@ -540,6 +565,21 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     constexpr int VB = (int)sizeof(V);
     const long long nbytes = V5 * VB;
     const __amdgpu_buffer_rsrc_t rin = make_rsrc(a.in + rb, nbytes);
+    const __amdgpu_buffer_rsrc_t rpo = make_rsrc(kPn ? a.pold + rb : nullptr, nbytes);
+    const __amdgpu_buffer_rsrc_t rpn = make_rsrc(kPn ? a.pnew + rb : nullptr, nbytes);
+    // EPI_PNORM: the stencil's input is p = z + beta p_old (z when beta is 0: p_old may hold
+    // anything then).  z and p_old are loaded into separate registers and combined where the
+    // value is first used, a step later, so no instruction waits for a load in flight.
+    auto comb = [&](const Sp<R>& z, const Sp<R>& q) {
+        return Sp<R>{z.ar + (use_p ? beta * q.ar : (R)0), z.ai + (use_p ? beta * q.ai : (R)0),
+                     z.br + (use_p ? beta * q.br : (R)0), z.bi + (use_p ? beta * q.bi : (R)0)};
+    };
+    auto ldin = [&](int off) {
+        if constexpr (kPn)
+            return comb(bld<R>(rin, off), bld<R>(rpo, off));
+        else
+            return bld<R>(rin, off);
+    };
     const int tx = a.cfg.tx, ty = NT / tx, W = tx + 2;
     const int tt = tile / a.cfg.tiles_x, tc = tile - tt * a.cfg.tiles_x;
     const int lt = threadIdx.x / tx, lx = threadIdx.x - lt * tx;
@@ -601,12 +641,13 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     // win[d+1] = psi(s_k + dir), win[d+2] receives psi(s_k + 2 dir); halo: hreg[d & 1]
     // holds the halo of slice s_k + dir, hreg[(d + 1) & 1] receives that of s_k + 2 dir
     Sp<R> win[4], hreg[2];
-    win[0] = bld<R>(rin, off_own(sa));
-    win[3] = bld<R>(rin, D5 ? off_own(s_prev0) : kBufOff);
-    win[1] = bld<R>(rin, (D5 || n > 1) ? off_own(wrap(sa + dir)) : kBufOff);
+    Sp<R> winp[4], hregp[2];  // EPI_PNORM: the p_old halves of loads in flight
+    win[0] = ldin(off_own(sa));
+    win[3] = ldin(D5 ? off_own(s_prev0) : kBufOff);
+    win[1] = ldin((D5 || n > 1) ? off_own(wrap(sa + dir)) : kBufOff);
     win[2] = Sp<R>{0, 0, 0, 0};
-    const Sp<R> h0 = bld<R>(rin, off_halo(sa));
-    hreg[0] = bld<R>(rin, n > 1 ? off_halo(sa + dir) : kBufOff);
+    const Sp<R> h0 = ldin(off_halo(sa));
+    hreg[0] = ldin(n > 1 ? off_halo(sa + dir) : kBufOff);
     hreg[1] = Sp<R>{0, 0, 0, 0};
     const Sp<R> cur0 = win[0], prev0 = win[3];
     put(0, own, win[0]);
@@ -646,18 +687,27 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
             const int sn = wrap(s + 2 * dir);
             if (!(whole && (sn == s_prev0 || sn == sa))) oo = off_own(sn);
         }
+        const int oh = kk + 2 < n ? off_halo(s + 2 * dir) : kBufOff;
         win[(d + 2) & 3] = bld<R>(rin, oo);
-        hreg[(d + 1) & 1] = bld<R>(rin, kk + 2 < n ? off_halo(s + 2 * dir) : kBufOff);
+        hreg[(d + 1) & 1] = bld<R>(rin, oh);
+        if constexpr (kPn) {
+            winp[(d + 2) & 3] = bld<R>(rpo, oo);
+            hregp[(d + 1) & 1] = bld<R>(rpo, oh);
+        }
         if constexpr (kPre) {
             const int en = kk + 1 < n ? off_own(s + dir) : kBufOff;
             ep[(d + 1) & 1][0] = bld<R>(rp, en);
             ep[(d + 1) & 1][1] = bld<R>(rx, en);
             ep[(d + 1) & 1][2] = bld<R>(rr, en);
         }
-        // psi(s + dir), loaded one step ago, or the kept wall slice it stands for
+        // psi(s + dir), loaded one step ago (at step 0: in the prologue, combined there), or
+        // the kept wall slice it stands for
         {
             const int sn = wrap(s + dir);
-            win[(d + 1) & 3] = pick(whole && sn == s_prev0, prev0, pick(whole && sn == sa, cur0, win[(d + 1) & 3]));
+            Sp<R> nv = win[(d + 1) & 3];
+            if constexpr (kPn)
+                if (kk > 0) nv = comb(nv, winp[(d + 1) & 3]);
+            win[(d + 1) & 3] = pick(whole && sn == s_prev0, prev0, pick(whole && sn == sa, cur0, nv));
         }
         const Sp<R>& cur = win[d];
         const Sp<R>& nxt = win[(d + 1) & 3];
@@ -667,9 +717,10 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
         const Sp<R> dn = pick(dir > 0, prv, nxt), up = pick(dir > 0, nxt, prv);
         const Sp<R> o = hop_site<R, DAG, D5>(L, k, cur, vtp, vtm, vxp, vxm, dn, up, s);
         const int eo = active ? off_own(s) : kBufOff;
-        if constexpr (EPI == EPI_STORE || EPI == EPI_NORM) {
+        if constexpr (EPI == EPI_STORE || EPI == EPI_NORM || EPI == EPI_PNORM) {
             bst<R, 2>(rout, eo, o);
-            if constexpr (EPI == EPI_NORM) acc0 += active ? norm2(o) : 0.0;
+            if constexpr (EPI != EPI_STORE) acc0 += active ? norm2(o) : 0.0;
+            if constexpr (kPn) bst<R>(rpn, eo, cur);
         } else if constexpr (EPI == EPI_CGUPD) {
             bst<R, 2>(rout, eo, o);
             Sp<R> pv, xv, rv;
@@ -707,9 +758,13 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
                 acc1 += active ? norm2(bv) : 0.0;
             }
         }
-        // stage slice s + dir for the next step
+        // stage slice s + dir for the next step (its halo loaded one step ago; at step 0 in the
+        // prologue, combined there)
         put(b ^ 1, own, nxt);
-        put(b ^ 1, hslot, hreg[d & 1]);
+        if constexpr (kPn)
+            put(b ^ 1, hslot, kk > 0 ? comb(hreg[d & 1], hregp[d & 1]) : hreg[d & 1]);
+        else
+            put(b ^ 1, hslot, hreg[d & 1]);
     };
     for (int k0 = 0; k0 < n; k0 += 4) {
         step(k0, IC<0>{});
@@ -921,6 +976,10 @@ void dispatch_lds_nt(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
         case EPI_STORE: k_hop_lds<R, DAG, D5, EPI_STORE, NT><<<grid, NT, 0, s>>>(a); break;
         case EPI_NORM: k_hop_lds<R, DAG, D5, EPI_NORM, NT><<<grid, NT, 0, s>>>(a); break;
         case EPI_CGUPD: k_hop_lds<R, DAG, D5, EPI_CGUPD, NT><<<grid, NT, 0, s>>>(a); break;
+        case EPI_PNORM:  // the learned solver's D apply (FP32, domain wall)
+            if constexpr (sizeof(R) == 4 && !DAG && D5) k_hop_lds<R, DAG, D5, EPI_PNORM, NT><<<grid, NT, 0, s>>>(a);
+            else throw DwfError(DWF_ERR_ARG, "EPI_PNORM: FP32 domain-wall D only");
+            break;
         default:
             if constexpr (sizeof(R) == 8) k_hop_lds<R, DAG, D5, EPI_RESID, NT><<<grid, NT, 0, s>>>(a);
             break;
@@ -943,6 +1002,7 @@ void dispatch_lds(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 template <typename R, bool DAG, bool D5>
 void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 {
+    if (epi == EPI_PNORM && !a.cfg.lds) throw DwfError(DWF_ERR_ARG, "EPI_PNORM needs the LDS-staged stencil");
     if (a.cfg.lds)
         dispatch_lds<R, DAG, D5>(epi, a, grid, s);
     else if (a.cfg.prefetch)

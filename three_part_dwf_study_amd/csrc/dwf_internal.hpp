@@ -70,11 +70,19 @@ __device__ __forceinline__ int xcd_logical_block(int hw, int total)
     return xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
 }
 
+// Scalars of the learned solver's preconditioned CG, per right-hand side.
+struct FcgScal {
+    double rz, pq, alpha, beta;
+    double rz_prev;  // (z, r) of the previous iteration (Fletcher-Reeves beta)
+};
+
 enum Epi : int {
     EPI_STORE = 0,  // out = Op in
     EPI_NORM = 1,   // out = Op in, partial |out|^2
     EPI_CGUPD = 2,  // alpha = rr/sum(partA); x += alpha p; r -= alpha (Op in); partial |r|^2; optionally out = Op in
-    EPI_RESID = 3   // r64 = b - Op in (+ copies to p64/r32/p32); partials |r|^2, |b|^2
+    EPI_RESID = 3,  // r64 = b - Op in (+ copies to p64/r32/p32); partials |r|^2, |b|^2
+    EPI_PNORM = 4   // FP32, learned solver: p = z + beta p_old (beta from zpart, fs), pnew = p,
+                    // out = Op p, partial |out|^2
 };
 
 template <typename R> struct HopArgs {
@@ -98,6 +106,16 @@ template <typename R> struct HopArgs {
     float* r32;          // EPI_RESID (nullable)
     float* p32;          // EPI_RESID (nullable)
     double* part;        // partial sums, 2 per block, [nrhs][nblk][2]
+    // EPI_PNORM: in = z, pold -> pnew; beta = (z, r) / rz_prev (fr) or -(z, q) / pq from the
+    // partials zpart [nrhs][nz][2] of (z, r), (z, q) and fs; a zero beta takes p = z (a cycle's
+    // first direction: rz_prev, pq set to +inf).  (z, r) <= 0 marks a breakdown in stw.
+    const R* pold;
+    R* pnew;
+    const double* zpart;
+    int nz;
+    int fr;
+    FcgScal* fs;
+    CgState* stw;
     const CgState* st;   // nullable
     int slot;
     int skip;            // skip a right-hand side whose flags[slot] & skip

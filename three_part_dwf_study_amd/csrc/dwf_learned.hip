@@ -640,11 +640,6 @@ static_assert(4 * kFuL2Tiles * 16 >= kFuR2 && 4 * kFuL3Tiles * 16 >= kFuTy * kFu
 // with 2 * bit 3 of the site: 16 consecutive sites then cover the 64 banks once.
 __device__ __forceinline__ int hel(int site, int g) { return site * 4 + (g ^ ((site >> 2) & 2)); }
 
-struct FcgScal {
-    double rz, pq, alpha, beta;
-    double rz_prev;  // (z, r) of the previous iteration (Fletcher-Reeves beta)
-};
-
 struct NetFusedArgs {
     int Nt, Nx, Ls, V4;
     int TY, TX;            // output tile, dividing Nt and Nx
@@ -1335,6 +1330,16 @@ __global__ void k_fcg_rz(const double* part, int nb, CgState* st, FcgScal* fs)
             st[rhs].flags[0] |= F_DONE;
         }
     }
+}
+
+// Start of a cycle of the fused iteration: +inf for the previous (z, r) and (p, A p), so the
+// first direction update's beta is 0 and p = z.
+__global__ void k_fcg_first(CgState* st, FcgScal* fs, int nrhs)
+{
+    const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
+    if (rhs >= nrhs || (st[rhs].flags[0] & kSkip)) return;
+    fs[rhs].rz_prev = INFINITY;
+    fs[rhs].pq = INFINITY;
 }
 
 // After the update stencil: (p, A p) from the first stencil's partials (a breakdown if
@@ -2029,6 +2034,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     double* t64 = (double*)c->buf(B_T64, nv * 8);
     float* r32 = (float*)c->buf(B_FCG_R32, nv * 4);
     float* p32 = (float*)c->buf(B_FCG_P32, nv * 4);
+    float* p32b = (float*)c->buf(B_FCG_P32B, nv * 4);
     float* q32 = (float*)c->buf(B_FCG_Q32, nv * 4);
     float* z32 = (float*)c->buf(B_FCG_Z32, nv * 4);
     float* t32 = (float*)c->buf(B_FCG_T32, nv * 4);
@@ -2070,7 +2076,44 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         k_fcg_resid64<<<vgrid, 256, 0, s>>>(b, q64, r64, sites, part);
         k_fcg_start<<<nrhs, 64, 0, s>>>(part, nb, st, rtol, c->reliable_delta, first ? 1 : 0);
     };
-    auto iteration = [&]() {
+    // The fused iteration (fused network kernel, LDS-staged stencils): the D stencil forms the
+    // direction p = z + beta p_old on the fly (EPI_PNORM, beta from the network's partials),
+    // writes it to the other direction buffer and computes t = D p with |t|^2; the D^dag
+    // stencil updates x, r; the network checks the iteration and computes z = M r with its
+    // dot products.  Three kernels; directions alternate between p32 and p32b by parity.
+    const bool fused = c->net_fused && ha.cfg.lds && fused_shape(c->L, fty, ftx);
+    auto iteration_fused = [&](int par) {
+        float* pin = par ? p32b : p32;
+        float* pout = par ? p32 : p32b;
+        HopArgs<float> a = ha;
+        a.in = z32;
+        a.pold = pin;
+        a.pnew = pout;
+        a.out = t32;
+        a.part = hpart;
+        a.zpart = npart;
+        a.nz = ntiles;
+        a.fr = fr;
+        a.fs = fs;
+        a.stw = st;
+        launch_hop<float>(false, EPI_PNORM, a, nrhs, s);
+        HopArgs<float> u = ha;
+        u.in = t32;
+        u.out = fr ? nullptr : q32;  // Fletcher-Reeves beta needs no q
+        u.x = x32;
+        u.r = r32;
+        u.p = pout;
+        u.partA = hpart;
+        u.anum = &fs[0].rz;
+        u.anum_stride = (int)(sizeof(FcgScal) / sizeof(double));
+        u.part = rpart;
+        launch_hop<float>(true, EPI_CGUPD, u, nrhs, s);
+        const FusedCheck chk{hpart, rpart, hnb, st, fs};
+        launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, fr ? nullptr : q32, npart, &chk);
+    };
+    // Otherwise (layer kernels or direct-load stencils): t = D p, the D^dag update, the check,
+    // the network, a dot-product pass and the direction update, in separate kernels.
+    auto iteration_layers = [&]() {
         HopArgs<float> a = ha;
         a.in = p32;
         a.out = t32;
@@ -2088,25 +2131,16 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         a.anum_stride = (int)(sizeof(FcgScal) / sizeof(double));
         a.part = rpart;
         launch_hop<float>(true, EPI_CGUPD, a, nrhs, s);
-        // the breakdown and convergence checks, then z = M r with (z, r) and (z, q): the
-        // fused network kernel does all three (checks at its start, dots in its epilogue);
-        // else a check kernel, the layer kernels and a dot-product pass.  Then beta and
-        // p = z + beta p.
-        const FusedCheck chk{hpart, rpart, hnb, st, fs};
-        if (launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, fr ? nullptr : q32, npart, &chk)) {
-            k_fcg_pbeta<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs, npart, ntiles, fr);
-        } else {
-            k_fcg_check<<<nrhs, 64, 0, s>>>(hpart, rpart, hnb, st, fs);
-            net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
-            k_fcg_dots<float><<<vgrid, 256, 0, s>>>(z32, r32, q32, sites, st, part);
-            k_fcg_pbeta<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs, part, nb, fr);
-        }
+        k_fcg_check<<<nrhs, 64, 0, s>>>(hpart, rpart, hnb, st, fs);
+        net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
+        k_fcg_dots<float><<<vgrid, 256, 0, s>>>(z32, r32, q32, sites, st, part);
+        k_fcg_pbeta<float><<<vgrid, 256, 0, s>>>(z32, p32, sites, st, fs, part, nb, fr);
     };
     constexpr int kBlockIters = 8;
     hipGraphExec_t graph = nullptr;
     if (c->use_graphs) {
         std::vector<unsigned long long> key{0xFC7ULL, (unsigned long long)nrhs};
-        for (const void* ptr : {(const void*)r32, (const void*)p32, (const void*)q32, (const void*)z32,
+        for (const void* ptr : {(const void*)r32, (const void*)p32, (const void*)p32b, (const void*)q32, (const void*)z32,
                                 (const void*)t32, (const void*)x32, (const void*)h1, (const void*)h2,
                                 (const void*)part, (const void*)hpart, (const void*)st, (const void*)fs,
                                 (const void*)npart, (const void*)rpart,
@@ -2118,8 +2152,14 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         const int v[] = {ha.cfg.bx, ha.cfg.chunks, ha.cfg.spt, ha.cfg.remap, ha.cfg.prefetch, ha.cfg.tx,
                          ha.cfg.tiles_x, ha.cfg.nt_store, ha.cfg.lds};
         for (int e : v) key.push_back((unsigned long long)(unsigned)e);
-        graph = c->cached_graph(key, [&] {
-            for (int i = 0; i < kBlockIters; ++i) iteration();
+        key.push_back((unsigned long long)fused);
+        graph = c->cached_graph(key, [&] {  // starts on an even iteration of the cycle
+            for (int i = 0; i < kBlockIters; ++i) {
+                if (fused)
+                    iteration_fused(i & 1);
+                else
+                    iteration_layers();
+            }
         });
     }
     auto max_iters = [&]() {
@@ -2142,16 +2182,31 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         if (all_done || max_iters() >= maxiter || cycles >= 1000) break;
         ++cycles;
         k_fcg_restart<float><<<vgrid, 256, 0, s>>>(r64, r32, x32, sites, st);
-        net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
-        k_fcg_p0<float><<<vgrid, 256, 0, s>>>(z32, r32, p32, sites, st, part);
-        k_fcg_rz<<<nrhs, 64, 0, s>>>(part, nb, st, fs);
+        if (fused) {
+            // z = M r with its (z, r) partials; the first iteration's beta is 0 (p = z)
+            launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, nullptr, npart);
+            k_fcg_first<<<(nrhs + 63) / 64, 64, 0, s>>>(st, fs, nrhs);
+        } else {
+            net_forward(c, net, r32, 1, nullptr, z32, 1, cfg, nrhs, st, h1, h2);
+            k_fcg_p0<float><<<vgrid, 256, 0, s>>>(z32, r32, p32, sites, st, part);
+            k_fcg_rz<<<nrhs, 64, 0, s>>>(part, nb, st, fs);
+        }
         c->check_launch("FCG cycle start");
         read();
+        int it = 0;  // iterations of this cycle: the fused iteration's direction buffers go by parity
+        auto iteration = [&]() {
+            if (fused)
+                iteration_fused(it & 1);
+            else
+                iteration_layers();
+            ++it;
+        };
         while (running() && max_iters() < maxiter) {
             const int n = std::min(32, maxiter - max_iters());
             int j = 0;
-            if (graph)
-                for (; j + kBlockIters <= n; j += kBlockIters) hchk(hipGraphLaunch(graph, s), "hipGraphLaunch");
+            if (graph && (it & 1) == 0)
+                for (; j + kBlockIters <= n; j += kBlockIters, it += kBlockIters)
+                    hchk(hipGraphLaunch(graph, s), "hipGraphLaunch");
             for (; j < n; ++j) iteration();
             c->check_launch("FCG iteration");
             read();
