@@ -24,6 +24,17 @@ using namespace dwfk;
 
 namespace {
 
+// v[i] uniform in [-1, 1) from a hash of i (PCG output permutation of an LCG step).
+template <typename R> __global__ void k_fill_hash(R* v, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    unsigned x = (unsigned)i * 747796405u + 2891336453u;
+    x = ((x >> ((x >> 28u) + 4u)) ^ x) * 277803737u;
+    x = (x >> 22u) ^ x;
+    v[i] = (R)((double)x * (2.0 / 4294967296.0) - 1.0);
+}
+
 struct Shape {
     int tile_x, prefetch, chunks, nt_store, lds;
 };
@@ -31,7 +42,7 @@ struct Shape {
 std::string key_of(const dwf_ctx* c, int nrhs, int prec)
 {
     std::ostringstream k;
-    k << "v3 " << (c->kind == 0 ? "dwf" : "wilson") << ' ' << c->L.Nt << ' ' << c->L.Nx << ' ' << c->L.Ls << ' ' << nrhs << ' '
+    k << "v4 " << (c->kind == 0 ? "dwf" : "wilson") << ' ' << c->L.Nt << ' ' << c->L.Nx << ' ' << c->L.Ls << ' ' << nrhs << ' '
       << (prec == DWF_PREC_FP32 ? "fp32" : "fp64");
     return k.str();
 }
@@ -116,7 +127,13 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
         const size_t n = (size_t)L.V5() * 4 * nrhs;
         void* in = c->buf(B_AUX0, n * rb);
         void* o = c->buf(B_AUX1, n * rb);
-        c->check(hipMemsetAsync(in, 0, n * rb, c->stream), "autotune fill");  // zeros time like data
+        // pseudo-random data: HBM moves zeros measurably faster than random bits, enough to
+        // change which shape wins
+        if (precision == DWF_PREC_FP32)
+            k_fill_hash<float><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>((float*)in, (long long)n);
+        else
+            k_fill_hash<double><<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>((double*)in, (long long)n);
+        c->check(hipGetLastError(), "autotune fill");
         hipEvent_t e0, e1;
         c->check(hipEventCreate(&e0), "event");
         c->check(hipEventCreate(&e1), "event");
