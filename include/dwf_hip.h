@@ -144,8 +144,10 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 typedef struct dwf_tune_result {
     int tile_x, prefetch, chunks, nt_store;
     int from_cache;
-    double microseconds;  /* one D apply of nrhs right-hand sides */
-    int lds;              /* 1 / 2: the LDS-staged tile kernel of 256 / 512 threads won */
+    double microseconds;  /* one D apply of nrhs right-hand sides (kernel time: the applies
+                             are timed as a HIP graph, without host launch overhead) */
+    int lds;              /* 1..6: the LDS-staged tile kernel of 256 / 512 / 64 / 128 / 384 /
+                             768 threads won (0: the direct-load kernel) */
     int pad;
 } dwf_tune_result;
 int dwf_autotune(dwf_ctx* ctx, int nrhs, int precision, const char* cache_path, dwf_tune_result* result);

@@ -4,8 +4,8 @@
 // -- block mapping (consecutive sites or 2D tiles of width 16/32/64 dividing Nx),
 // software-pipelined slice loop on/off, s-chunks per 4D site, streaming stores on/off,
 // and the LDS-staged tile kernel (blocks of 64..768 threads, tiles of width 8..128 that
-// cover the lattice, full rows included) -- with HIP events on the
-// context's stream, keeps the fastest and applies it to the context.  Results can be
+// cover the lattice, full rows included) -- as HIP graphs of back-to-back applies timed
+// with events on the context's stream, keeps the fastest and applies it to the context.  Results can be
 // cached in a text file (one line per key), so a production run tunes once.
 
 #include "../../include/dwf_hip.h"
@@ -42,7 +42,7 @@ struct Shape {
 std::string key_of(const dwf_ctx* c, int nrhs, int prec)
 {
     std::ostringstream k;
-    k << "v4 " << (c->kind == 0 ? "dwf" : "wilson") << ' ' << c->L.Nt << ' ' << c->L.Nx << ' ' << c->L.Ls << ' ' << nrhs << ' '
+    k << "v5 " << (c->kind == 0 ? "dwf" : "wilson") << ' ' << c->L.Nt << ' ' << c->L.Nx << ' ' << c->L.Ls << ' ' << nrhs << ' '
       << (prec == DWF_PREC_FP32 ? "fp32" : "fp64");
     return k.str();
 }
@@ -151,20 +151,29 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
             }
             cand.swap(uniq);
         }
+        // The applies are timed as one HIP graph of `reps` back-to-back launches, the way
+        // the CG solvers replay their iterations: eager launches of a small batch (a few
+        // microseconds of kernel time) measure the host's launch rate instead of the kernel.
         auto time_us = [&](const Shape& s, int reps) {
             apply_shape(c, s);
             dwf_apply_device_impl(c, DWF_OP_D, precision, in, o, nrhs);  // warm-up
-            c->check(hipEventRecord(e0, c->stream), "event");
-            for (int r = 0; r < reps; ++r) dwf_apply_device_impl(c, DWF_OP_D, precision, in, o, nrhs);
-            c->check(hipEventRecord(e1, c->stream), "event");
-            c->check(hipEventSynchronize(e1), "event sync");
+            hipGraphExec_t g = c->capture_graph([&] {
+                for (int r = 0; r < reps; ++r) dwf_apply_device_impl(c, DWF_OP_D, precision, in, o, nrhs);
+            });
             float ms = 0;
-            c->check(hipEventElapsedTime(&ms, e0, e1), "event time");
+            hipError_t e = hipGraphLaunch(g, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+            if (e == hipSuccess) e = hipGraphLaunch(g, c->stream);
+            if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            (void)hipGraphExecDestroy(g);
+            c->check(e, "autotune timing");
             return 1e3 * ms / reps;
         };
         // Pass 1 times every candidate; pass 2 re-times the four fastest three times each,
         // interleaved, and keeps each one's best time, so one noisy sample decides nothing.
-        const int reps = nrhs >= 8 ? 10 : 20;
+        const int reps = nrhs >= 8 ? 10 : 40;
         std::vector<std::pair<double, size_t>> t1;
         for (size_t i = 0; i < cand.size(); ++i) t1.emplace_back(time_us(cand[i], reps), i);
         std::sort(t1.begin(), t1.end());

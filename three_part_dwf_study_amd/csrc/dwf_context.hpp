@@ -128,6 +128,8 @@ struct dwf_ctx {
     // The instantiated graph of `body` (kernel launches on `stream` only), cached under
     // `key`, which must hold every argument the captured kernels receive.
     hipGraphExec_t cached_graph(const std::vector<unsigned long long>& key, const std::function<void()>& body);
+    // The instantiated graph of `body`, not cached: the caller destroys it.
+    hipGraphExec_t capture_graph(const std::function<void()>& body);
 
     struct Buf {
         void* p = nullptr;

@@ -40,6 +40,13 @@ hipGraphExec_t dwf_ctx::cached_graph(const std::vector<unsigned long long>& key,
     auto f = graphs.find(key);
     if (f != graphs.end()) return f->second;
     if (graphs.size() >= 64) drop_graphs();
+    hipGraphExec_t ex = capture_graph(body);
+    graphs.emplace(key, ex);
+    return ex;
+}
+
+hipGraphExec_t dwf_ctx::capture_graph(const std::function<void()>& body)
+{
     check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     hipGraph_t g = nullptr;
     try {
@@ -54,7 +61,6 @@ hipGraphExec_t dwf_ctx::cached_graph(const std::vector<unsigned long long>& key,
     const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     check(e, "hipGraphInstantiate");
-    graphs.emplace(key, ex);
     return ex;
 }
 
