@@ -145,7 +145,8 @@ typedef struct dwf_tune_result {
     int tile_x, prefetch, chunks, nt_store;
     int from_cache;
     double microseconds;  /* one D apply of nrhs right-hand sides (kernel time: the applies
-                             are timed as a HIP graph, without host launch overhead) */
+                             are timed as a HIP graph, without host launch overhead, unless
+                             DWF_TUNE_GRAPHS is 0) */
     int lds;              /* 1..6: the LDS-staged tile kernel of 256 / 512 / 64 / 128 / 384 /
                              768 threads won (0: the direct-load kernel) */
     int pad;

@@ -154,9 +154,20 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
         // The applies are timed as one HIP graph of `reps` back-to-back launches, the way
         // the CG solvers replay their iterations: eager launches of a small batch (a few
         // microseconds of kernel time) measure the host's launch rate instead of the kernel.
+        // With graphs switched off (DWF_TUNE_GRAPHS 0, e.g. under a tracing tool) the
+        // applies are launched one by one.
         auto time_us = [&](const Shape& s, int reps) {
             apply_shape(c, s);
             dwf_apply_device_impl(c, DWF_OP_D, precision, in, o, nrhs);  // warm-up
+            if (!c->use_graphs) {
+                float ms = 0;
+                c->check(hipEventRecord(e0, c->stream), "event");
+                for (int r = 0; r < reps; ++r) dwf_apply_device_impl(c, DWF_OP_D, precision, in, o, nrhs);
+                c->check(hipEventRecord(e1, c->stream), "event");
+                c->check(hipEventSynchronize(e1), "event sync");
+                c->check(hipEventElapsedTime(&ms, e0, e1), "event time");
+                return 1e3 * ms / reps;
+            }
             hipGraphExec_t g = c->capture_graph([&] {
                 for (int r = 0; r < reps; ++r) dwf_apply_device_impl(c, DWF_OP_D, precision, in, o, nrhs);
             });
