@@ -313,6 +313,45 @@ def volume_sources(rng, n, Nt, Nx, Ls):
     return rng.normal(size=shape) + 1j * rng.normal(size=shape)
 
 
+def test_learned_solve_deferred_x_update_staggered():
+    """With the LDS-staged stencils the learned solver runs its fused iteration: the D
+    stencil forms p = z + beta p_old and applies the previous iteration's x += alpha p_old
+    (one iteration late), and the end of a cycle applies each right-hand side's last
+    one.  Right-hand sides of different kinds freeze at different iterations of a cycle,
+    so some wait with an update pending while others run on.  The solutions must match
+    the layer path (separate kernels, x updated at once) to rounding -- and differ from
+    it in the last bits, which shows that the two runs took different paths -- and every
+    solve must be certified."""
+    Nt, Nx, Ls = 16, 24, 8
+    print_case(test="learned_deferred_x", Nt=Nt, Nx=Nx, Ls=Ls)
+    rng = np.random.default_rng(31)
+    th = rand_theta(rng, Nt, Nx)
+    src = np.concatenate([boundary_sources(rng, 2, Nt, Nx, Ls), volume_sources(rng, 2, Nt, Nx, Ls)])
+    net = dwf.Network(Ls, seed=5)
+    res = []
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.set_preconditioner(net)
+        ctx.tune(dwf.DWF_TUNE_LDS, 3)  # 64-thread blocks, tiles of 8 x 8 sites
+        ctx.tune(dwf.DWF_TUNE_TILE_X, 8)
+        ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA, 1e-3)
+        for fr in (1, 0):
+            ctx.tune(dwf.DWF_TUNE_FCG_BETA, fr)
+            for fused in (1, 0):
+                ctx.tune(dwf.DWF_TUNE_NET_FUSED, fused)
+                res.append(ctx.solve_dirac(src, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_LEARNED))
+    its = [st["iterations"] for st in res[0][1]]
+    assert max(its) - min(its) >= 3, its  # the batch did stagger
+    for (x, st, _) in res:
+        for r in range(len(src)):
+            assert st[r]["converged"] and st[r]["true_relres"] <= 1e-10 and st[r]["fp64_iterations"] == 0
+    for f, l in ((0, 1), (2, 3)):
+        assert not np.array_equal(res[f][0], res[l][0])
+        for r in range(len(src)):
+            assert abs(res[f][1][r]["iterations"] - res[l][1][r]["iterations"]) <= 2
+            assert rel(res[f][0][r], res[l][0][r]) <= 1e-8
+
+
 def test_training_cuts_iterations():
     """What the trained network gains: train on (source, CG solution) pairs from 4
     configurations (full-volume Gaussian sources), then solve boundary sources on 2

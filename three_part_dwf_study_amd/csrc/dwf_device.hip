@@ -531,12 +531,15 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     R alpha = 0;
     if constexpr (EPI == EPI_CGUPD) {
         const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
-        if (pAp == 0.0 || (a.anum && !(pAp > 0.0))) return;  // block-uniform; a later kernel records the failure
-        alpha = (R)((a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp);
+        const bool ok = !(pAp == 0.0 || (a.anum && !(pAp > 0.0)));
+        const double al = ok ? (a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp : 0.0;
+        if (a.aout && bid == 0 && threadIdx.x == 0) a.aout[(size_t)rhs * a.anum_stride] = al;
+        if (!ok) return;  // block-uniform; a later kernel records the failure
+        alpha = (R)al;
     }
     constexpr bool kPn = EPI == EPI_PNORM;
-    R beta = 0;
-    bool use_p = false;
+    R beta = 0, alpha_prev = 0;
+    bool use_p = false, use_x = false;
     if constexpr (kPn) {
         // beta of the direction update p = z + beta p_old, from the network's partials; every
         // block folds them in the same order, block 0 records (z, r) and beta
@@ -551,12 +554,16 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
         }
         const FcgScal& f = a.fs[rhs];
         const double b = a.fr ? zr / f.rz_prev : -zq / f.pq;
+        const double ap = f.alpha;  // the previous iteration's step (0: none pending)
         if (bid == 0 && threadIdx.x == 0) {
             a.fs[rhs].rz = zr;
             a.fs[rhs].beta = b;
+            a.fs[rhs].pbuf = a.pbuf;
         }
         beta = (R)b;
         use_p = b != 0.0;
+        alpha_prev = (R)ap;
+        use_x = a.x != nullptr && ap != 0.0;
     }
 
     const int V4 = L.V4, Nx = L.Nx, Nt = L.Nt, Ls = L.Ls;
@@ -656,9 +663,11 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     // epilogue buffers of this right-hand side (null pointers give empty buffers)
     constexpr bool kOut = EPI != EPI_RESID, kUpd = EPI == EPI_CGUPD, kRes = EPI == EPI_RESID;
     const __amdgpu_buffer_rsrc_t rout = make_rsrc(kOut && a.out ? a.out + rb : nullptr, nbytes);
-    const __amdgpu_buffer_rsrc_t rx = make_rsrc(kUpd ? a.x + rb : nullptr, nbytes);
+    // x += alpha p: EPI_CGUPD unless x is null; EPI_PNORM the previous iteration's, with p_old
+    const bool xupd = kUpd ? a.x != nullptr : (kPn && use_x);
+    const __amdgpu_buffer_rsrc_t rx = make_rsrc(xupd ? a.x + rb : nullptr, nbytes);
     const __amdgpu_buffer_rsrc_t rr = make_rsrc(kUpd ? a.r + rb : nullptr, nbytes);
-    const __amdgpu_buffer_rsrc_t rp = make_rsrc(kUpd ? a.p + rb : nullptr, nbytes);
+    const __amdgpu_buffer_rsrc_t rp = make_rsrc(xupd ? (kPn ? a.pold : a.p) + rb : nullptr, nbytes);
     const __amdgpu_buffer_rsrc_t rbv = make_rsrc(kRes ? a.b + rb : nullptr, V5 * 32);
     const __amdgpu_buffer_rsrc_t rr64 = make_rsrc(kRes ? a.r64 + rb : nullptr, V5 * 32);
     const __amdgpu_buffer_rsrc_t rp64 = make_rsrc(kRes && a.p64 ? a.p64 + rb : nullptr, V5 * 32);
@@ -666,13 +675,14 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     const __amdgpu_buffer_rsrc_t rp32 = make_rsrc(kRes && a.p32 ? a.p32 + rb : nullptr, V5 * 16);
 
     // FP32 CG update: p, x, r of a slice are loaded one step before their epilogue
-    // (ep[d & 1] serves step d, ep[(d + 1) & 1] receives the next slice's)
-    constexpr bool kPre = kUpd && sizeof(R) == 4;
+    // (ep[d & 1] serves step d, ep[(d + 1) & 1] receives the next slice's); EPI_PNORM's
+    // x update the same way with p_old, x
+    constexpr bool kPre = (kUpd && sizeof(R) == 4) || kPn;
     Sp<R> ep[2][3];
     if constexpr (kPre) {
         ep[0][0] = bld<R>(rp, off_own(sa));
         ep[0][1] = bld<R>(rx, off_own(sa));
-        ep[0][2] = bld<R>(rr, off_own(sa));
+        if constexpr (kUpd) ep[0][2] = bld<R>(rr, off_own(sa));
     }
 
     auto step = [&](const int kk, auto D) {
@@ -698,7 +708,7 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
             const int en = kk + 1 < n ? off_own(s + dir) : kBufOff;
             ep[(d + 1) & 1][0] = bld<R>(rp, en);
             ep[(d + 1) & 1][1] = bld<R>(rx, en);
-            ep[(d + 1) & 1][2] = bld<R>(rr, en);
+            if constexpr (kUpd) ep[(d + 1) & 1][2] = bld<R>(rr, en);
         }
         // psi(s + dir), loaded one step ago (at step 0: in the prologue, combined there), or
         // the kept wall slice it stands for
@@ -720,7 +730,16 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
         if constexpr (EPI == EPI_STORE || EPI == EPI_NORM || EPI == EPI_PNORM) {
             bst<R, 2>(rout, eo, o);
             if constexpr (EPI != EPI_STORE) acc0 += active ? norm2(o) : 0.0;
-            if constexpr (kPn) bst<R>(rpn, eo, cur);
+            if constexpr (kPn) {
+                bst<R>(rpn, eo, cur);
+                const Sp<R>& pv = ep[d & 1][0];
+                Sp<R> xv = ep[d & 1][1];
+                xv.ar += alpha_prev * pv.ar;
+                xv.ai += alpha_prev * pv.ai;
+                xv.br += alpha_prev * pv.br;
+                xv.bi += alpha_prev * pv.bi;
+                bst<R>(rx, eo, xv);
+            }
         } else if constexpr (EPI == EPI_CGUPD) {
             bst<R, 2>(rout, eo, o);
             Sp<R> pv, xv, rv;
@@ -1003,6 +1022,8 @@ template <typename R, bool DAG, bool D5>
 void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 {
     if (epi == EPI_PNORM && !a.cfg.lds) throw DwfError(DWF_ERR_ARG, "EPI_PNORM needs the LDS-staged stencil");
+    if (epi == EPI_CGUPD && (!a.x || a.aout) && !a.cfg.lds)
+        throw DwfError(DWF_ERR_ARG, "EPI_CGUPD without x or with aout needs the LDS-staged stencil");
     if (a.cfg.lds)
         dispatch_lds<R, DAG, D5>(epi, a, grid, s);
     else if (a.cfg.prefetch)

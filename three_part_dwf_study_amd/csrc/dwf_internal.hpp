@@ -74,15 +74,17 @@ __device__ __forceinline__ int xcd_logical_block(int hw, int total)
 struct FcgScal {
     double rz, pq, alpha, beta;
     double rz_prev;  // (z, r) of the previous iteration (Fletcher-Reeves beta)
+    double pbuf;     // direction buffer (0 / 1) the last direction update wrote
 };
 
 enum Epi : int {
     EPI_STORE = 0,  // out = Op in
     EPI_NORM = 1,   // out = Op in, partial |out|^2
-    EPI_CGUPD = 2,  // alpha = rr/sum(partA); x += alpha p; r -= alpha (Op in); partial |r|^2; optionally out = Op in
+    EPI_CGUPD = 2,  // alpha = rr/sum(partA); x += alpha p (unless x is null); r -= alpha (Op in); partial
+                    // |r|^2; optionally out = Op in and aout = alpha
     EPI_RESID = 3,  // r64 = b - Op in (+ copies to p64/r32/p32); partials |r|^2, |b|^2
     EPI_PNORM = 4   // FP32, learned solver: p = z + beta p_old (beta from zpart, fs), pnew = p,
-                    // out = Op p, partial |out|^2
+                    // out = Op p, partial |out|^2; the previous iteration's x += alpha p_old
 };
 
 template <typename R> struct HopArgs {
@@ -94,12 +96,14 @@ template <typename R> struct HopArgs {
     const int* cfg_map;  // ... or cfg_map[r] when not null
     const R* in;         // [nrhs][V5] spinors of 4 R
     R* out;              // EPI_STORE / EPI_NORM
-    R* x;                // EPI_CGUPD
+    R* x;                // EPI_CGUPD (nullable: no x update, p unused); EPI_PNORM (nullable): see below
     R* r;                // EPI_CGUPD
     const R* p;          // EPI_CGUPD
     const double* partA; // EPI_CGUPD: partials of p^dag A p from the first stencil
     const double* anum;  // EPI_CGUPD (nullable): alpha's numerator is anum[rhs * anum_stride] instead of
     int anum_stride;     //   st[rhs].rr[slot] (the flexible CG's (z, r)); out (nullable) receives Op in
+    double* aout;        // EPI_CGUPD (nullable): block 0 stores alpha (0 when p^dag A p is not positive)
+                         //   at aout[rhs * anum_stride]
     const double* b;     // EPI_RESID
     double* r64;         // EPI_RESID
     double* p64;         // EPI_RESID (nullable)
@@ -109,6 +113,9 @@ template <typename R> struct HopArgs {
     // EPI_PNORM: in = z, pold -> pnew; beta = (z, r) / rz_prev (fr) or -(z, q) / pq from the
     // partials zpart [nrhs][nz][2] of (z, r), (z, q) and fs; a zero beta takes p = z (a cycle's
     // first direction: rz_prev, pq set to +inf).  (z, r) <= 0 marks a breakdown in stw.
+    // With x set, the previous iteration's update x += alpha p_old (alpha = fs[rhs].alpha,
+    // stored by that iteration's EPI_CGUPD; 0: none) is applied here, where p_old is read
+    // anyway; block 0 records pbuf, the index of pnew's buffer, in fs[rhs].pbuf.
     const R* pold;
     R* pnew;
     const double* zpart;
@@ -116,6 +123,7 @@ template <typename R> struct HopArgs {
     int fr;
     FcgScal* fs;
     CgState* stw;
+    int pbuf;
     const CgState* st;   // nullable
     int slot;
     int skip;            // skip a right-hand side whose flags[slot] & skip

@@ -1333,13 +1333,38 @@ __global__ void k_fcg_rz(const double* part, int nb, CgState* st, FcgScal* fs)
 }
 
 // Start of a cycle of the fused iteration: +inf for the previous (z, r) and (p, A p), so the
-// first direction update's beta is 0 and p = z.
+// first direction update's beta is 0 and p = z; no x update pending (alpha 0).
 __global__ void k_fcg_first(CgState* st, FcgScal* fs, int nrhs)
 {
     const int rhs = blockIdx.x * blockDim.x + threadIdx.x;
-    if (rhs >= nrhs || (st[rhs].flags[0] & kSkip)) return;
+    if (rhs >= nrhs) return;
+    fs[rhs].alpha = 0.0;
+    if (st[rhs].flags[0] & kSkip) return;
     fs[rhs].rz_prev = INFINITY;
     fs[rhs].pq = INFINITY;
+}
+
+// End of a cycle of the fused iteration: each right-hand side's last update x += alpha p,
+// which the next iteration's D stencil would have applied (alpha 0: nothing pending).
+__global__ __launch_bounds__(256) void k_fcg_flush_x(float* x, const float* p0, const float* p1,
+                                                     const FcgScal* fs, long long sites)
+{
+    const int rhs = blockIdx.y;
+    const double al = fs[rhs].alpha;
+    if (al == 0.0) return;
+    const float a = (float)al;
+    const long long o = (long long)rhs * sites;
+    const float4* P = reinterpret_cast<const float4*>(fs[rhs].pbuf != 0.0 ? p1 : p0) + o;
+    float4* X = reinterpret_cast<float4*>(x) + o;
+    for_sites(sites, [&](long long i) {
+        const float4 pv = P[i];
+        float4 xv = X[i];
+        xv.x += a * pv.x;
+        xv.y += a * pv.y;
+        xv.z += a * pv.z;
+        xv.w += a * pv.w;
+        X[i] = xv;
+    });
 }
 
 // After the update stencil: (p, A p) from the first stencil's partials (a breakdown if
@@ -2078,9 +2103,12 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     };
     // The fused iteration (fused network kernel, LDS-staged stencils): the D stencil forms the
     // direction p = z + beta p_old on the fly (EPI_PNORM, beta from the network's partials),
-    // writes it to the other direction buffer and computes t = D p with |t|^2; the D^dag
-    // stencil updates x, r; the network checks the iteration and computes z = M r with its
-    // dot products.  Three kernels; directions alternate between p32 and p32b by parity.
+    // writes it to the other direction buffer, computes t = D p with |t|^2 and applies the
+    // previous iteration's x += alpha p_old (it reads p_old anyway); the D^dag stencil updates
+    // r and records alpha; the network checks the iteration and computes z = M r with its
+    // dot products.  Three kernels; directions alternate between p32 and p32b by parity; the
+    // last update of each right-hand side is applied at the end of the cycle
+    // (k_fcg_flush_x).
     const bool fused = c->net_fused && ha.cfg.lds && fused_shape(c->L, fty, ftx);
     auto iteration_fused = [&](int par) {
         float* pin = par ? p32b : p32;
@@ -2096,16 +2124,17 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         a.fr = fr;
         a.fs = fs;
         a.stw = st;
+        a.x = x32;
+        a.pbuf = pout == p32b ? 1 : 0;
         launch_hop<float>(false, EPI_PNORM, a, nrhs, s);
         HopArgs<float> u = ha;
         u.in = t32;
         u.out = fr ? nullptr : q32;  // Fletcher-Reeves beta needs no q
-        u.x = x32;
         u.r = r32;
-        u.p = pout;
         u.partA = hpart;
         u.anum = &fs[0].rz;
         u.anum_stride = (int)(sizeof(FcgScal) / sizeof(double));
+        u.aout = &fs[0].alpha;
         u.part = rpart;
         launch_hop<float>(true, EPI_CGUPD, u, nrhs, s);
         const FusedCheck chk{hpart, rpart, hnb, st, fs};
@@ -2211,6 +2240,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
             c->check_launch("FCG iteration");
             read();
         }
+        if (fused) k_fcg_flush_x<<<vgrid, 256, 0, s>>>(x32, p32, p32b, fs, sites);
         launch_accum_x((long long)sites * 4, x, x32, st, 0, nrhs, s);  // x += x32 (all not done)
         reliable_update(false);
         c->check_launch("FCG reliable update");
