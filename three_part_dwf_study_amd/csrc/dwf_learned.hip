@@ -896,7 +896,6 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     // s) and point 4 + g (step 1: -x in slice s, +s, -s, and none with zero transporter).
     // TG: step 0 self, +t, -t, +s (no transport at all), step 1 -s, +x, -x, none.
     int e1a[kFuL1Tiles], e1b[kFuL1Tiles], o1[kFuL1Tiles];
-    bool v1[kFuL1Tiles];
     uint2 lk1[kFuL1Tiles][2];
     // slice offsets of the lane's points in steps 0 and 1
     const int dsa = TG && g == 3 ? 1 : 0;
@@ -905,7 +904,6 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     for (int e = 0; e < kFuL1Tiles; ++e) {
         const int k = wv * kFuL1Tiles + e;
         const int sg = min(16 * k + m, n1 - 1);
-        v1[e] = 16 * k + m < n1;
         const int i = div_small(sg, inv1), jx = sg - i * W1;
         const int b0 = (i + 1) * W0 + jx + 1;
         o1[e] = hel(sg, g);
@@ -945,22 +943,21 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
                 f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wa0, B0, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Wa1, B1, acc, 0, 0, 0);
-                if (v1[e])
-                    (&ring1[S0][0][0])[o1[e]] = make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
+                // a lane past the region's end computed its clamped last site: it stores the
+                // same bits there as that site's own lane (no branch on the lane)
+                (&ring1[S0][0][0])[o1[e]] = make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
             }
     };
 
     if (wv < 4) {
         // ---- layer 2: R2 tiles wv*3 + e (origin t0-1, x0-1), input ring1 (width W1)
         int e0[kFuL2Tiles], etp[kFuL2Tiles], etm[kFuL2Tiles], exp[kFuL2Tiles], exm[kFuL2Tiles], eo[kFuL2Tiles];
-        bool valid[kFuL2Tiles];
         float2 lk[kFuL2Tiles][4];
         uint2 win[kFuL2Tiles][2];
 #pragma unroll
         for (int e = 0; e < kFuL2Tiles; ++e) {
             const int k = wv * kFuL2Tiles + e;
             const int sg = min(16 * k + m, n2 - 1);
-            valid[e] = 16 * k + m < n2;
             const int i = div_small(sg, inv2), jx = sg - i * W2;
             const int b = (i + 1) * W1 + jx + 1;
             e0[e] = hel(b, g);
@@ -995,9 +992,9 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
                         const f32x4_t acc = hop8(Kc, IC<1>{}, IC<(K + 1) & 3>{}, IC<(K + 2) & 3>{},
                                                  IC<(K + 3) & 3>{}, ring1, e0[e], etp[e], etm[e], exp[e], exm[e], lk[e], win[e],
                                                  s2 == -1);
-                        if (valid[e])
-                            (&ring2[(K + 2) & 3][0][0])[eo[e]] =
-                                make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
+                        // past the region's end: the clamped last site's value, stored there again
+                        (&ring2[(K + 2) & 3][0][0])[eo[e]] =
+                            make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
                     }
             }
             loads(sl, in_new, w_new);
