@@ -921,7 +921,9 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             lk1[e][1] = g == 3 ? make_uint2(0u, 0u) : hop_link(L, Nt, Nx, V4, 4 + g, t, x);  // -x, none, none, zero
         }
     }
-    const int nt1 = __builtin_amdgcn_readfirstlane(min(kFuL1Tiles, ((n1 + 15) >> 4) - wv * kFuL1Tiles));
+    // Every wave runs all its tile slots: a slot past the region's end computes the clamped
+    // last site again and stores the same bits there (no branch between the tiles, so the
+    // compiler interleaves their MFMA chains).
     auto layer1 = [&](auto Kc, int j) {
         constexpr int K = decltype(Kc)::value;
         const int s1 = j - 5;
@@ -931,7 +933,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         const bf16x4_t Wa0 = frag(Kc, 0, 0), Wa1 = frag(Kc, 0, 1);
 #pragma unroll
         for (int e = 0; e < kFuL1Tiles; ++e)
-            if (e < nt1) {
+            {
                 const uint2 va = ring0[sa][e1a[e]];
                 const uint2 vb = ring0[sb][e1b[e]];
                 bf16x4_t B0;
@@ -978,7 +980,6 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             }
             win[e][0] = win[e][1] = make_uint2(0u, 0u);
         }
-        const int nt = __builtin_amdgcn_readfirstlane(min(kFuL2Tiles, ((n2 + 15) >> 4) - wv * kFuL2Tiles));
         auto step = [&](auto Kc, int jv, Slices& sl, float4& in_new, uint4& w_new, const float4& in_old,
                         const uint4& w_old) {
             constexpr int K = decltype(Kc)::value;
@@ -988,7 +989,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             if (s2 >= -1 && s2 <= Ls) {
 #pragma unroll
                 for (int e = 0; e < kFuL2Tiles; ++e)
-                    if (e < nt) {
+                    {
                         const f32x4_t acc = hop8(Kc, IC<1>{}, IC<(K + 1) & 3>{}, IC<(K + 2) & 3>{},
                                                  IC<(K + 3) & 3>{}, ring1, e0[e], etp[e], etm[e], exp[e], exm[e], lk[e], win[e],
                                                  s2 == -1);
@@ -1046,7 +1047,6 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             }
             win3[e][0] = win3[e][1] = make_uint2(0u, 0u);
         }
-        const int nt3 = __builtin_amdgcn_readfirstlane(min(kFuL3Tiles, ((n3 + 15) >> 4) - wr * kFuL3Tiles));
         auto step = [&](auto Kc, int jv, Slices& sl, float4& in_new, uint4& w_new, const float4& in_old,
                         const uint4& w_old) {
             constexpr int K = decltype(Kc)::value;
@@ -1056,7 +1056,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             if (s3 >= 0 && s3 <= Ls - 1) {
 #pragma unroll
                 for (int e = 0; e < kFuL3Tiles; ++e)
-                    if (e < nt3) {
+                    {
                         const f32x4_t acc = hop8(Kc, IC<2>{}, IC<(K + 3) & 3>{}, IC<K & 3>{}, IC<(K + 1) & 3>{},
                                                  ring2, e3[e], e3p[e], e3m[e], e3xp[e], e3xm[e], lk3[e], win3[e], s3 == 0);
                         if (v3[e]) {
