@@ -529,42 +529,9 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     __shared__ V lds[2][kSlots];
 
     R alpha = 0;
-    if constexpr (EPI == EPI_CGUPD) {
-        const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
-        const bool ok = !(pAp == 0.0 || (a.anum && !(pAp > 0.0)));
-        const double al = ok ? (a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp : 0.0;
-        if (a.aout && bid == 0 && threadIdx.x == 0) a.aout[(size_t)rhs * a.anum_stride] = al;
-        if (!ok) return;  // block-uniform; a later kernel records the failure
-        alpha = (R)al;
-    }
     constexpr bool kPn = EPI == EPI_PNORM;
     R beta = 0, alpha_prev = 0;
     bool use_p = false, use_x = false;
-    if constexpr (kPn) {
-        // beta of the direction update p = z + beta p_old, from the network's partials; every
-        // block folds them in the same order, block 0 records (z, r) and beta
-        const double zr = sum_partials(a.zpart + (size_t)rhs * a.nz * 2, a.nz, 0, sh);
-        const double zq = sum_partials(a.zpart + (size_t)rhs * a.nz * 2, a.nz, 1, sh);
-        if (!(zr > 0.0)) {  // the preconditioner is not positive on this residual
-            if (bid == 0 && threadIdx.x == 0) {
-                a.stw[rhs].status = 5;
-                a.stw[rhs].flags[0] |= F_DONE;
-            }
-            return;  // block-uniform
-        }
-        const FcgScal& f = a.fs[rhs];
-        const double b = a.fr ? zr / f.rz_prev : -zq / f.pq;
-        const double ap = f.alpha;  // the previous iteration's step (0: none pending)
-        if (bid == 0 && threadIdx.x == 0) {
-            a.fs[rhs].rz = zr;
-            a.fs[rhs].beta = b;
-            a.fs[rhs].pbuf = a.pbuf;
-        }
-        beta = (R)b;
-        use_p = b != 0.0;
-        alpha_prev = (R)ap;
-        use_x = a.x != nullptr && ap != 0.0;
-    }
 
     const int V4 = L.V4, Nx = L.Nx, Nt = L.Nt, Ls = L.Ls;
     const long long V5 = L.V5();
@@ -580,12 +547,6 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     auto comb = [&](const Sp<R>& z, const Sp<R>& q) {
         return Sp<R>{z.ar + (use_p ? beta * q.ar : (R)0), z.ai + (use_p ? beta * q.ai : (R)0),
                      z.br + (use_p ? beta * q.br : (R)0), z.bi + (use_p ? beta * q.bi : (R)0)};
-    };
-    auto ldin = [&](int off) {
-        if constexpr (kPn)
-            return comb(bld<R>(rin, off), bld<R>(rpo, off));
-        else
-            return bld<R>(rin, off);
     };
     const int tx = a.cfg.tx, ty = NT / tx, W = tx + 2;
     const int tt = tile / a.cfg.tiles_x, tc = tile - tt * a.cfg.tiles_x;
@@ -649,12 +610,68 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     // holds the halo of slice s_k + dir, hreg[(d + 1) & 1] receives that of s_k + 2 dir
     Sp<R> win[4], hreg[2];
     Sp<R> winp[4], hregp[2];  // EPI_PNORM: the p_old halves of loads in flight
-    win[0] = ldin(off_own(sa));
-    win[3] = ldin(D5 ? off_own(s_prev0) : kBufOff);
-    win[1] = ldin((D5 || n > 1) ? off_own(wrap(sa + dir)) : kBufOff);
+    Sp<R> h0;
+    {
+        // The prologue's loads go out first; the block's scalars (alpha, beta: reductions
+        // over the previous kernels' partials) are folded while they are in flight.
+        const int o0 = off_own(sa), o3 = D5 ? off_own(s_prev0) : kBufOff;
+        const int o1 = (D5 || n > 1) ? off_own(wrap(sa + dir)) : kBufOff;
+        const int q0 = off_halo(sa), q1 = n > 1 ? off_halo(sa + dir) : kBufOff;
+        win[0] = bld<R>(rin, o0);
+        win[3] = bld<R>(rin, o3);
+        win[1] = bld<R>(rin, o1);
+        h0 = bld<R>(rin, q0);
+        hreg[0] = bld<R>(rin, q1);
+        Sp<R> p0, p3, p1, ph0, ph1;
+        if constexpr (kPn) {
+            p0 = bld<R>(rpo, o0);
+            p3 = bld<R>(rpo, o3);
+            p1 = bld<R>(rpo, o1);
+            ph0 = bld<R>(rpo, q0);
+            ph1 = bld<R>(rpo, q1);
+        }
+        if constexpr (EPI == EPI_CGUPD) {
+            const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
+            const bool ok = !(pAp == 0.0 || (a.anum && !(pAp > 0.0)));
+            const double al = ok ? (a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp : 0.0;
+            if (a.aout && bid == 0 && threadIdx.x == 0) a.aout[(size_t)rhs * a.anum_stride] = al;
+            if (!ok) return;  // block-uniform; a later kernel records the failure
+            alpha = (R)al;
+        }
+        if constexpr (kPn) {
+            // beta of the direction update p = z + beta p_old, from the network's partials; every
+            // block folds them in the same order, block 0 records (z, r) and beta
+            const double zr = sum_partials(a.zpart + (size_t)rhs * a.nz * 2, a.nz, 0, sh);
+            const double zq = sum_partials(a.zpart + (size_t)rhs * a.nz * 2, a.nz, 1, sh);
+            if (!(zr > 0.0)) {  // the preconditioner is not positive on this residual
+                if (bid == 0 && threadIdx.x == 0) {
+                    a.stw[rhs].status = 5;
+                    a.stw[rhs].flags[0] |= F_DONE;
+                }
+                return;  // block-uniform
+            }
+            const FcgScal& f = a.fs[rhs];
+            const double b = a.fr ? zr / f.rz_prev : -zq / f.pq;
+            const double ap = f.alpha;  // the previous iteration's step (0: none pending)
+            if (bid == 0 && threadIdx.x == 0) {
+                a.fs[rhs].rz = zr;
+                a.fs[rhs].beta = b;
+                a.fs[rhs].pbuf = a.pbuf;
+            }
+            beta = (R)b;
+            use_p = b != 0.0;
+            alpha_prev = (R)ap;
+            use_x = a.x != nullptr && ap != 0.0;
+        }
+        if constexpr (kPn) {
+            win[0] = comb(win[0], p0);
+            win[3] = comb(win[3], p3);
+            win[1] = comb(win[1], p1);
+            h0 = comb(h0, ph0);
+            hreg[0] = comb(hreg[0], ph1);
+        }
+    }
     win[2] = Sp<R>{0, 0, 0, 0};
-    const Sp<R> h0 = ldin(off_halo(sa));
-    hreg[0] = ldin(n > 1 ? off_halo(sa + dir) : kBufOff);
     hreg[1] = Sp<R>{0, 0, 0, 0};
     const Sp<R> cur0 = win[0], prev0 = win[3];
     put(0, own, win[0]);
