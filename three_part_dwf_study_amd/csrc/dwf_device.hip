@@ -630,7 +630,7 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
             ph0 = bld<R>(rpo, q0);
             ph1 = bld<R>(rpo, q1);
         }
-        if constexpr (EPI == EPI_CGUPD) {
+        if constexpr (EPI == EPI_CGUPD || EPI == EPI_RUPD) {
             const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
             const bool ok = !(pAp == 0.0 || (a.anum && !(pAp > 0.0)));
             const double al = ok ? (a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp : 0.0;
@@ -678,10 +678,12 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     put(0, hslot, h0);
 
     // epilogue buffers of this right-hand side (null pointers give empty buffers)
-    constexpr bool kOut = EPI != EPI_RESID, kUpd = EPI == EPI_CGUPD, kRes = EPI == EPI_RESID;
+    constexpr bool kOut = EPI != EPI_RESID, kRes = EPI == EPI_RESID;
+    constexpr bool kUpd = EPI == EPI_CGUPD || EPI == EPI_RUPD;  // r -= alpha (Op in)
+    constexpr bool kXp = EPI == EPI_CGUPD || kPn;                // x += alpha p
     const __amdgpu_buffer_rsrc_t rout = make_rsrc(kOut && a.out ? a.out + rb : nullptr, nbytes);
     // x += alpha p: EPI_CGUPD unless x is null; EPI_PNORM the previous iteration's, with p_old
-    const bool xupd = kUpd ? a.x != nullptr : (kPn && use_x);
+    const bool xupd = EPI == EPI_CGUPD ? a.x != nullptr : (kPn && use_x);
     const __amdgpu_buffer_rsrc_t rx = make_rsrc(xupd ? a.x + rb : nullptr, nbytes);
     const __amdgpu_buffer_rsrc_t rr = make_rsrc(kUpd ? a.r + rb : nullptr, nbytes);
     const __amdgpu_buffer_rsrc_t rp = make_rsrc(xupd ? (kPn ? a.pold : a.p) + rb : nullptr, nbytes);
@@ -697,8 +699,10 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
     constexpr bool kPre = (kUpd && sizeof(R) == 4) || kPn;
     Sp<R> ep[2][3];
     if constexpr (kPre) {
-        ep[0][0] = bld<R>(rp, off_own(sa));
-        ep[0][1] = bld<R>(rx, off_own(sa));
+        if constexpr (kXp) {
+            ep[0][0] = bld<R>(rp, off_own(sa));
+            ep[0][1] = bld<R>(rx, off_own(sa));
+        }
         if constexpr (kUpd) ep[0][2] = bld<R>(rr, off_own(sa));
     }
 
@@ -723,8 +727,10 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
         }
         if constexpr (kPre) {
             const int en = kk + 1 < n ? off_own(s + dir) : kBufOff;
-            ep[(d + 1) & 1][0] = bld<R>(rp, en);
-            ep[(d + 1) & 1][1] = bld<R>(rx, en);
+            if constexpr (kXp) {
+                ep[(d + 1) & 1][0] = bld<R>(rp, en);
+                ep[(d + 1) & 1][1] = bld<R>(rx, en);
+            }
             if constexpr (kUpd) ep[(d + 1) & 1][2] = bld<R>(rr, en);
         }
         // psi(s + dir), loaded one step ago (at step 0: in the prologue, combined there), or
@@ -757,27 +763,32 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
                 xv.bi += alpha_prev * pv.bi;
                 bst<R>(rx, eo, xv);
             }
-        } else if constexpr (EPI == EPI_CGUPD) {
+        } else if constexpr (kUpd) {
             bst<R, 2>(rout, eo, o);
-            Sp<R> pv, xv, rv;
-            if constexpr (kPre) {
-                pv = ep[d & 1][0];
-                xv = ep[d & 1][1];
+            Sp<R> rv;
+            if constexpr (kPre)
                 rv = ep[d & 1][2];
-            } else {
-                pv = bld<R>(rp, eo);
-                xv = bld<R>(rx, eo);
+            else
                 rv = bld<R>(rr, eo);
+            if constexpr (kXp) {
+                Sp<R> pv, xv;
+                if constexpr (kPre) {
+                    pv = ep[d & 1][0];
+                    xv = ep[d & 1][1];
+                } else {
+                    pv = bld<R>(rp, eo);
+                    xv = bld<R>(rx, eo);
+                }
+                xv.ar += alpha * pv.ar;
+                xv.ai += alpha * pv.ai;
+                xv.br += alpha * pv.br;
+                xv.bi += alpha * pv.bi;
+                bst<R>(rx, eo, xv);
             }
-            xv.ar += alpha * pv.ar;
-            xv.ai += alpha * pv.ai;
-            xv.br += alpha * pv.br;
-            xv.bi += alpha * pv.bi;
             rv.ar -= alpha * o.ar;
             rv.ai -= alpha * o.ai;
             rv.br -= alpha * o.br;
             rv.bi -= alpha * o.bi;
-            bst<R>(rx, eo, xv);
             bst<R>(rr, eo, rv);
             acc0 += active ? norm2(rv) : 0.0;
         } else {  // EPI_RESID (FP64 only)
@@ -1016,6 +1027,10 @@ void dispatch_lds_nt(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
             if constexpr (sizeof(R) == 4 && !DAG && D5) k_hop_lds<R, DAG, D5, EPI_PNORM, NT><<<grid, NT, 0, s>>>(a);
             else throw DwfError(DWF_ERR_ARG, "EPI_PNORM: FP32 domain-wall D only");
             break;
+        case EPI_RUPD:  // the learned solver's D^dag apply (FP32, domain wall)
+            if constexpr (sizeof(R) == 4 && DAG && D5) k_hop_lds<R, DAG, D5, EPI_RUPD, NT><<<grid, NT, 0, s>>>(a);
+            else throw DwfError(DWF_ERR_ARG, "EPI_RUPD: FP32 domain-wall D^dag only");
+            break;
         default:
             if constexpr (sizeof(R) == 8) k_hop_lds<R, DAG, D5, EPI_RESID, NT><<<grid, NT, 0, s>>>(a);
             break;
@@ -1038,7 +1053,8 @@ void dispatch_lds(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 template <typename R, bool DAG, bool D5>
 void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 {
-    if (epi == EPI_PNORM && !a.cfg.lds) throw DwfError(DWF_ERR_ARG, "EPI_PNORM needs the LDS-staged stencil");
+    if ((epi == EPI_PNORM || epi == EPI_RUPD) && !a.cfg.lds)
+        throw DwfError(DWF_ERR_ARG, "EPI_PNORM / EPI_RUPD need the LDS-staged stencil");
     if (epi == EPI_CGUPD && (!a.x || a.aout) && !a.cfg.lds)
         throw DwfError(DWF_ERR_ARG, "EPI_CGUPD without x or with aout needs the LDS-staged stencil");
     if (a.cfg.lds)

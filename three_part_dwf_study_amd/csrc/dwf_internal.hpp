@@ -83,8 +83,10 @@ enum Epi : int {
     EPI_CGUPD = 2,  // alpha = rr/sum(partA); x += alpha p (unless x is null); r -= alpha (Op in); partial
                     // |r|^2; optionally out = Op in and aout = alpha
     EPI_RESID = 3,  // r64 = b - Op in (+ copies to p64/r32/p32); partials |r|^2, |b|^2
-    EPI_PNORM = 4   // FP32, learned solver: p = z + beta p_old (beta from zpart, fs), pnew = p,
+    EPI_PNORM = 4,  // FP32, learned solver: p = z + beta p_old (beta from zpart, fs), pnew = p,
                     // out = Op p, partial |out|^2; the previous iteration's x += alpha p_old
+    EPI_RUPD = 5    // FP32, learned solver: EPI_CGUPD without x and p (r -= alpha (Op in), |r|^2,
+                    // aout = alpha)
 };
 
 template <typename R> struct HopArgs {

@@ -2102,7 +2102,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     // direction p = z + beta p_old on the fly (EPI_PNORM, beta from the network's partials),
     // writes it to the other direction buffer, computes t = D p with |t|^2 and applies the
     // previous iteration's x += alpha p_old (it reads p_old anyway); the D^dag stencil updates
-    // r and records alpha; the network checks the iteration and computes z = M r with its
+    // r and records alpha (EPI_RUPD); the network checks the iteration and computes z = M r with its
     // dot products.  Three kernels; directions alternate between p32 and p32b by parity; the
     // last update of each right-hand side is applied at the end of the cycle
     // (k_fcg_flush_x).
@@ -2133,7 +2133,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
         u.anum_stride = (int)(sizeof(FcgScal) / sizeof(double));
         u.aout = &fs[0].alpha;
         u.part = rpart;
-        launch_hop<float>(true, EPI_CGUPD, u, nrhs, s);
+        launch_hop<float>(true, EPI_RUPD, u, nrhs, s);
         const FusedCheck chk{hpart, rpart, hnb, st, fs};
         launch_net_fused(c, net, r32, 1, z32, 1, cfg, nrhs, st, fr ? nullptr : q32, npart, &chk);
     };
