@@ -124,6 +124,28 @@ __device__ __forceinline__ double sum_partials(const double* p, int n, int which
     return r;
 }
 
+// Both sums of the (pair) partials p[2 i], p[2 i + 1], i < n, in fixed order, in one pass.
+__device__ __forceinline__ void sum_partials2(const double* p, int n, double* sh, double& s0, double& s1)
+{
+    if (threadIdx.x < 64) {
+        double v0 = 0.0, v1 = 0.0;
+        for (int i = threadIdx.x; i < n; i += 64) {
+            v0 += p[2 * i];
+            v1 += p[2 * i + 1];
+        }
+        v0 = wave_sum(v0);
+        v1 = wave_sum(v1);
+        if (threadIdx.x == 0) {
+            sh[0] = v0;
+            sh[1] = v1;
+        }
+    }
+    __syncthreads();
+    s0 = sh[0];
+    s1 = sh[1];
+    __syncthreads();
+}
+
 // Block sum of two accumulators over NT threads (NT/64 wavefronts, fixed order); the
 // result is valid in thread 0.  sh holds 2 * NT / 64 doubles.
 template <int NT> __device__ __forceinline__ void block_sum2_n(double& a, double& b, double* sh)
@@ -641,8 +663,8 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
         if constexpr (kPn) {
             // beta of the direction update p = z + beta p_old, from the network's partials; every
             // block folds them in the same order, block 0 records (z, r) and beta
-            const double zr = sum_partials(a.zpart + (size_t)rhs * a.nz * 2, a.nz, 0, sh);
-            const double zq = sum_partials(a.zpart + (size_t)rhs * a.nz * 2, a.nz, 1, sh);
+            double zr, zq;
+            sum_partials2(a.zpart + (size_t)rhs * a.nz * 2, a.nz, sh, zr, zq);
             if (!(zr > 0.0)) {  // the preconditioner is not positive on this residual
                 if (bid == 0 && threadIdx.x == 0) {
                     a.stw[rhs].status = 5;
