@@ -178,6 +178,7 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
             if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            (void)hipStreamSynchronize(c->stream);  // never destroy a graph that may still run
             (void)hipGraphExecDestroy(g);
             c->check(e, "autotune timing");
             return 1e3 * ms / reps;

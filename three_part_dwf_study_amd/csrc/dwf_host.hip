@@ -31,6 +31,9 @@ int dwf_hip_fail(hipError_t e, const char* what)
 
 void dwf_ctx::drop_graphs()
 {
+    // An executable graph may still be running on the stream (launches are asynchronous):
+    // finish it before destroying it.
+    if (!graphs.empty() && stream) (void)hipStreamSynchronize(stream);
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     graphs.clear();
 }

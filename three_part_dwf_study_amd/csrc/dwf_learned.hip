@@ -2033,6 +2033,7 @@ extern "C" int dwf_ctx_set_preconditioner(dwf_ctx* c, const dwf_net* net)
         if (!c) throw DwfError(DWF_ERR_ARG, "dwf_ctx_set_preconditioner: null context");
         if (net && net->Ls != c->L.Ls)
             throw DwfError(DWF_ERR_ARG, "learned preconditioner: network Ls differs from the lattice's");
+        if (net != c->net) c->drop_graphs();  // cached iterations captured the old network's weights
         c->net = net;
         return DWF_OK;
     });
@@ -2170,7 +2171,9 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
                                 (const void*)t32, (const void*)x32, (const void*)h1, (const void*)h2,
                                 (const void*)part, (const void*)hpart, (const void*)st, (const void*)fs,
                                 (const void*)npart, (const void*)rpart,
-                                (const void*)cfg, (const void*)net, (const void*)net->wf[0], (const void*)c->l32})
+                                (const void*)cfg, (const void*)net, (const void*)net->wf[0], (const void*)net->wf[1],
+                                (const void*)net->wf[2], (const void*)net->wa[0], (const void*)net->wa[1],
+                                (const void*)net->wa[2], (const void*)net->wg0, (const void*)c->l32})
             key.push_back((unsigned long long)(uintptr_t)ptr);
         key.push_back((unsigned long long)ha.rhs_per_cfg);
         key.push_back((unsigned long long)c->net_fused);
