@@ -177,6 +177,19 @@ int dwf_apply(dwf_ctx* ctx, int op, const double* in, double* out, int nrhs);
 /* Same on device buffers in the given precision (in != out). */
 int dwf_apply_device(dwf_ctx* ctx, int op, int precision, const void* in, void* out, int nrhs);
 
+/* Allocation-free form of dwf_apply_device, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same kernels and results as
+ * dwf_apply_device; the normal operators' intermediate vector lives in a fixed device
+ * array of DWF_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of
+ * the context's workspace, so the call makes no allocation.  Limit: for DWF_OP_DDDAG and
+ * DWF_OP_DDAGD, nrhs * Ls * Nt * Nx * 4 * (8 for FP64, 4 for FP32) bytes must not exceed
+ * DWF_NOALLOC_SCRATCH_BYTES (DWF_ERR_ARG otherwise); D and D^dagger need no scratch.
+ * Every call on one GPU shares that GPU's array: calls that overlap in time on one GPU
+ * (from several threads or streams) race. */
+#define DWF_NOALLOC_SCRATCH_BYTES (64ULL << 20)
+int dwf_apply_device_noalloc(dwf_ctx* ctx, int op, int precision, const void* in, void* out, int nrhs);
+
 /* ---- solvers ----------------------------------------------------------------------- */
 
 /* Solve A x = b for nrhs right-hand sides with A = D D^dagger (DWF_OP_DDDAG) or

@@ -179,6 +179,7 @@ PROTOTYPES = {
     "dwf_set_gauge_batch": (_I, [_P, _I, _P]),
     "dwf_apply": (_I, [_P, _I, _P, _P, _I]),
     "dwf_apply_device": (_I, [_P, _I, _I, _P, _P, _I]),
+    "dwf_apply_device_noalloc": (_I, [_P, _I, _I, _P, _P, _I]),
     "dwf_solve": (_I, [_P, _I, _I, _P, _P, _D, _I, _I, _P, _P]),
     "dwf_solve_device": (_I, [_P, _I, _I, _P, _P, _D, _I, _I, _P, _P]),
     "dwf_solve_dirac": (_I, [_P, _I, _P, _P, _D, _I, _I, _P, _P]),

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,41 @@ using namespace dwfk;
 namespace {
 thread_local std::string g_last_error;
 }
+
+// ------------------------------------------------------------------ fixed scratch
+// Device memory of dwf_apply_device_noalloc (the normal operators' intermediate vector):
+// one copy per GPU, loaded with the code object and held until the process ends.  Host
+// code never uses the array's name as an address: each device's address comes from
+// hipGetSymbolAddress at context creation (with that device current) and is kept in
+// g_scratch_ptr.  Every call on one GPU shares that GPU's array, so calls that overlap
+// in time on one GPU (several host threads or streams) race on it.
+static __device__ __align__(256) unsigned char scratch[DWF_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Looks up (once per device) the device address of `scratch` on device `dev`, which must
+// be the current device.
+hipError_t scratch_setup(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[dev] = p;
+    return e;
+}
+
+void* scratch_ptr(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[dev];
+}
+}  // namespace
 
 void dwf_set_error(const std::string& msg) { g_last_error = msg; }
 
@@ -241,6 +277,7 @@ int create(dwf_ctx** out, const Lat& L, int kind)
     c->L = L;
     c->kind = kind;
     e = hipGetDevice(&c->device);
+    if (e == hipSuccess) e = scratch_setup(c->device);
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
     const size_t nl = (size_t)2 * L.V4;
     if (e == hipSuccess) e = hipMalloc(&c->theta, nl * sizeof(double));
@@ -393,6 +430,32 @@ int dwf_apply_device(dwf_ctx* c, int op, int precision, const void* in, void* ou
         if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_apply_device: no gauge field set");
         c->bind_rhs(nrhs);
         dwf_apply_device_impl(c, op, precision, in, out, nrhs);
+        return DWF_OK;
+    });
+}
+
+int dwf_apply_device_noalloc(dwf_ctx* c, int op, int precision, const void* in, void* out, int nrhs)
+{
+    return guard([&] {
+        if (!c || !in || !out || nrhs < 1 || op < DWF_OP_D || op > DWF_OP_DDAGD ||
+            (precision != DWF_PREC_FP64 && precision != DWF_PREC_FP32) || in == out)
+            throw DwfError(DWF_ERR_ARG, "dwf_apply_device_noalloc: bad argument");
+        if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_apply_device_noalloc: no gauge field set");
+        c->bind_rhs(nrhs);
+        const unsigned long long real = precision == DWF_PREC_FP64 ? 8 : 4;
+        const unsigned long long need =
+            op >= DWF_OP_DDDAG ? (unsigned long long)c->L.V5() * 4ULL * (unsigned long long)nrhs * real : 0ULL;
+        if (need > DWF_NOALLOC_SCRATCH_BYTES)
+            throw DwfError(DWF_ERR_ARG, "dwf_apply_device_noalloc: the intermediate vector needs " +
+                                            std::to_string(need) + " bytes, more than the " +
+                                            std::to_string(DWF_NOALLOC_SCRATCH_BYTES) +
+                                            "-byte scratch; pass fewer right-hand sides or use dwf_apply_device");
+        void* tmp = scratch_ptr(c->device);
+        if (!tmp) throw DwfError(DWF_ERR_HIP, "dwf_apply_device_noalloc: no scratch address for this device");
+        if (precision == DWF_PREC_FP64)
+            apply_op<double>(c, op, (const double*)in, (double*)out, op >= DWF_OP_DDDAG ? (double*)tmp : nullptr, nrhs);
+        else
+            apply_op<float>(c, op, (const float*)in, (float*)out, op >= DWF_OP_DDDAG ? (float*)tmp : nullptr, nrhs);
         return DWF_OK;
     });
 }
