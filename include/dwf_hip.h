@@ -167,6 +167,26 @@ int dwf_set_gauge_device(dwf_ctx* ctx, const double* theta_device);
  * The force, gauge and HMC entry points need a single configuration. */
 int dwf_set_gauge_batch(dwf_ctx* ctx, int ncfg, const double* theta);
 
+/* One lattice size of a multi-lattice batch: ncfg configurations of Nt x Nx. */
+typedef struct dwf_lattice_group {
+    int Nt, Nx, ncfg, pad;
+} dwf_lattice_group;
+
+/* Hold configurations of several lattice sizes at once (no counterpart in the reference,
+ * whose lattice sizes are process-global constants, domain_wall/include/parameters.hpp:7-12,
+ * and whose analyzer runs one configuration per OpenMP thread,
+ * domain_wall/analyze_configs.cpp:45-76).  groups[0..ngroups) (ngroups <= 8) list the
+ * sizes (the context's Ls, M5 and mf apply to all), theta is the concatenation over the
+ * groups of ncfg x [2][Nt][Nx] angles.  Afterwards dwf_apply*, dwf_solve* (CG64, MIXED),
+ * dwf_solve_dirac* (CG64, MIXED, EO_MIXED) and dwf_measure_batch take nrhs = k * (total
+ * configurations) right-hand sides, k per configuration, configuration-major in group
+ * order; each right-hand side's vector has its own lattice's size and layout and the
+ * vectors follow one another (the batch is the concatenation).  Every kernel launch of
+ * those entry points covers all the lattice sizes at once.  The learned preconditioner,
+ * force, gauge, HMC and autotuning entry points need a single lattice; dwf_set_gauge and
+ * dwf_set_gauge_batch return the context to its own lattice. */
+int dwf_set_gauge_groups(dwf_ctx* ctx, int ngroups, const dwf_lattice_group* groups, const double* theta);
+
 /* ---- operators --------------------------------------------------------------------- */
 
 /* out = op(in) for nrhs vectors in FP64 (host buffers).  Replaces apply_D,

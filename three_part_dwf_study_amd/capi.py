@@ -121,6 +121,10 @@ class TuneResult(ctypes.Structure):
     ]
 
 
+class LatticeGroup(ctypes.Structure):
+    _fields_ = [("Nt", ctypes.c_int), ("Nx", ctypes.c_int), ("ncfg", ctypes.c_int), ("pad", ctypes.c_int)]
+
+
 class MeasureSpec(ctypes.Structure):
     _fields_ = [
         ("method", ctypes.c_int),
@@ -177,6 +181,7 @@ PROTOTYPES = {
     "dwf_set_gauge": (_I, [_P, _P]),
     "dwf_set_gauge_device": (_I, [_P, _P]),
     "dwf_set_gauge_batch": (_I, [_P, _I, _P]),
+    "dwf_set_gauge_groups": (_I, [_P, _I, _P, _P]),
     "dwf_apply": (_I, [_P, _I, _P, _P, _I]),
     "dwf_apply_device": (_I, [_P, _I, _I, _P, _P, _I]),
     "dwf_apply_device_noalloc": (_I, [_P, _I, _I, _P, _P, _I]),
@@ -273,6 +278,8 @@ class DwfContext:
     def __init__(self, handle, geom: Geometry, kind: str, lib):
         self._h = handle
         self.geom = geom
+        self._geom0 = geom
+        self._groups = None  # [(Nt, Nx, ncfg)] while several lattice sizes are held
         self.kind = kind
         self._lib = lib
 
@@ -338,7 +345,37 @@ class DwfContext:
                "dwf_convert_f64_f32_device")
 
     # gauge --------------------------------------------------------------------------
+    def _single_lattice(self):
+        self.geom = self._geom0
+        self._groups = None
+
+    def set_gauge_groups(self, groups):
+        """Hold configurations of several lattice sizes (dwf_set_gauge_groups):
+        groups = [(Nt, Nx, thetas[ncfg][2*Nt*Nx]), ...].  Vectors of every later call are
+        the concatenation, configuration-major in group order, of each right-hand side's
+        vector on its own lattice."""
+        defs = (LatticeGroup * len(groups))()
+        parts, lat = [], []
+        for i, (Nt, Nx, th) in enumerate(groups):
+            th = np.ascontiguousarray(th, dtype=np.float64).reshape(-1, 2 * Nt * Nx)
+            defs[i] = LatticeGroup(Nt, Nx, th.shape[0], 0)
+            parts.append(th.reshape(-1))
+            lat.append((Nt, Nx, th.shape[0]))
+        allth = np.ascontiguousarray(np.concatenate(parts))
+        _check(self._lib, self._lib.dwf_set_gauge_groups(self._h, len(groups), ctypes.cast(defs, ctypes.c_void_p),
+                                                         _ptr(allth)), "dwf_set_gauge_groups")
+        big = max(lat, key=lambda g: g[0] * g[1])
+        self.geom = Geometry(big[0], big[1], self._geom0.Ls)
+        self._groups = lat if len(lat) > 1 else None
+
+    def config_lattices(self) -> list:
+        """(Nt, Nx) of every configuration held, in order."""
+        if self._groups is None:
+            return None
+        return [(Nt, Nx) for Nt, Nx, n in self._groups for _ in range(n)]
+
     def set_gauge(self, theta: np.ndarray):
+        self._single_lattice()
         th = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
         if th.size != 2 * self.geom.V4:
             raise ValueError("theta must hold 2*Nt*Nx angles")
@@ -351,6 +388,7 @@ class DwfContext:
                "dwf_ctx_set_preconditioner")
 
     def set_gauge_batch(self, thetas: np.ndarray):
+        self._single_lattice()
         th = np.ascontiguousarray(thetas, dtype=np.float64).reshape(-1)
         if th.size % (2 * self.geom.V4):
             raise ValueError("thetas must hold ncfg x 2*Nt*Nx angles")
@@ -358,11 +396,19 @@ class DwfContext:
         _check(self._lib, self._lib.dwf_set_gauge_batch(self._h, n, _ptr(th)), "dwf_set_gauge_batch")
 
     def set_gauge_device(self, theta_ptr: int):
+        self._single_lattice()
         _check(self._lib, self._lib.dwf_set_gauge_device(self._h, theta_ptr), "dwf_set_gauge_device")
 
     # operators ----------------------------------------------------------------------
     def _as_batch(self, v: np.ndarray) -> tuple[np.ndarray, int]:
         a = np.ascontiguousarray(v, dtype=np.complex128)
+        if self._groups is not None:
+            # k right-hand sides per configuration over every configuration held
+            unit = sum(n * self.geom.Ls * Nt * Nx * 2 for Nt, Nx, n in self._groups)
+            ncfg = sum(n for _, _, n in self._groups)
+            if a.size % unit:
+                raise ValueError(f"vector size {a.size} is not a multiple of {unit}")
+            return a.reshape(-1), a.size // unit * ncfg
         n = self.geom.ndof
         if a.size % n:
             raise ValueError(f"vector size {a.size} is not a multiple of {n}")
@@ -506,13 +552,17 @@ class DwfContext:
     def measure_batch(self, ncfg: int, rtol: float, maxiter: int, wall: bool = True, point: bool = True,
                       max_momentum: int = 0, noise_vectors: int = 0, seeds=None,
                       method: int = DWF_SOLVE_CG64, point_sites=None) -> list:
-        """dwf_measure for the ncfg configurations held (set_gauge_batch); one dict each.
-        point_sites: None (the reference's 4 point sources) or [ncfg][2] = (t0, x0)."""
-        Nt, Nx = self.geom.Nt, self.geom.Nx
+        """dwf_measure for the ncfg configurations held (set_gauge_batch or
+        set_gauge_groups); one dict each.  point_sites: None (the reference's 4 point
+        sources) or [ncfg][2] = (t0, x0)."""
+        lats = self.config_lattices() or [(self.geom.Nt, self.geom.Nx)] * ncfg
+        if len(lats) != ncfg:
+            raise ValueError("ncfg differs from the configurations held")
         npoint = 0 if not point else (4 if point_sites is None else 2)
         K = (4 if wall else 0) + npoint + noise_vectors
         keep, outs = [], (MeasureOut * ncfg)()
         for k in range(ncfg):
+            Nt, Nx = lats[k]
             d = dict(CPP=np.zeros(Nt), CJ5=np.zeros(Nt), momentum_CPP=np.zeros((max_momentum + 1) * Nt),
                      loop=np.zeros(Nt * Nx, np.complex128), iterations=np.zeros(max(K, 1), np.int32),
                      dirac_residual=np.zeros(4), point_CJ5=np.zeros(Nt))
@@ -527,6 +577,7 @@ class DwfContext:
                                                       None if sd is None else _ptr(sd),
                                                       ctypes.cast(outs, ctypes.c_void_p)), "dwf_measure_batch")
         for k, d in enumerate(keep):
+            Nt, Nx = lats[k]
             d["momentum_CPP"] = d["momentum_CPP"].reshape(max_momentum + 1, Nt)
             d["loop"] = d["loop"].reshape(Nt, Nx)
             d["iterations"] = d["iterations"][:K]

@@ -97,6 +97,7 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
         if (!c || nrhs < 1 || (precision != DWF_PREC_FP32 && precision != DWF_PREC_FP64))
             throw DwfError(DWF_ERR_ARG, "dwf_autotune: bad argument");
         if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_autotune: no gauge field set");
+        c->require_single_lattice("dwf_autotune");
         const std::string key = key_of(c, nrhs, precision);
         Shape best{};
         double best_us = 0;

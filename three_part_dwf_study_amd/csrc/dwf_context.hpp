@@ -92,11 +92,13 @@ enum BufId : int {
     B_FCG_NPART,
     B_GAMMA_B,   // learned D D^dag solves: Gamma b
     B_FCG_P32B,  // learned solver: the second direction buffer of the fused iteration
+    B_NOFF,      // analyzer: offsets of each configuration's noise vectors
     B_COUNT
 };
 
 struct dwf_ctx {
     dwfk::Lat L{};
+    dwfk::Lat L0{};  // the lattice the context was created with
     int kind = 0;  // 0 DWF, 1 Wilson
     int device = 0;
     hipStream_t stream = nullptr;
@@ -104,12 +106,30 @@ struct dwf_ctx {
     double2* l64 = nullptr;   // [2][V4] (cos, sin) FP64
     float2* l32 = nullptr;    // [2][V4] (cos, sin) FP32
     bool have_gauge = false;
-    int ncfg = 1;                 // configurations held (same lattice), see dwf_set_gauge_batch
+    int ncfg = 1;                 // configurations held (all lattice groups together)
     int cfg_cap = 1;              // configurations the link buffers can hold
     int rhs_per_cfg = 1 << 30;    // set per call: right-hand side r uses configuration r / rhs_per_cfg
-    void bind_rhs(int nrhs);      // checks nrhs against ncfg and sets rhs_per_cfg
+    void bind_rhs(int nrhs);      // checks nrhs against ncfg and sets rhs_per_cfg (and gr)
+    // Lattice groups (dwf_set_gauge_groups): per group (Nt, Nx, configurations); empty for
+    // the context's own lattice L.  With two or more groups, L holds the largest lattice
+    // (Ls, masses and signs are shared) and every supported entry point runs all groups in
+    // each launch.
+    struct GroupDef {
+        int Nt, Nx, ncfg;
+    };
+    std::vector<GroupDef> gdefs;
+    size_t link_cap = 0;          // complex link entries the link buffers can hold
+    dwfk::Groups gr{};            // the current call's layout (set by bind_rhs)
+    bool multi() const { return gdefs.size() > 1; }
+    void require_single_lattice(const char* what) const;
+    // The layout of nrhs right-hand sides (after bind_rhs(nrhs)): gr for several groups,
+    // else one group of L.
+    dwfk::Groups groups_for(int nrhs) const;
+    dwfk::VecLayout layout(int nrhs, int shift) const;
+    long long vec_sites(int nrhs) const { return layout(nrhs, 0).total_sites(); }  // full-lattice 5D sites
     void require_single_config(const char* what) const;
-    void reserve_configs(int n);  // grow the gauge buffers to n configurations
+    void reserve_configs(int n);  // grow the gauge buffers to n configurations of L
+    void reserve_links(size_t nl);  // grow the gauge buffers to nl complex links
     int chunks_override = 0;      // s-chunks per 4D site for the stencils (0: heuristic)
     int xcd_remap = 1;            // XCD-aware block order for the stencils
     int prefetch = 0;             // software-pipelined stencil slice loop
@@ -148,6 +168,11 @@ struct dwf_ctx {
 };
 
 void dwf_apply_device_impl(dwf_ctx* c, int op, int prec, const void* in, void* out, int nrhs);
+// The context's stencil launch shape for nrhs right-hand sides (tuned knobs; linear blocks
+// without LDS tiles when it holds several lattice groups).
+dwfk::HopCfg ctx_hop_cfg(const dwf_ctx* c, int nrhs);
+// Stencil arguments with the context's links and the call's lattice groups.
+template <typename R> dwfk::HopArgs<R> ctx_hop_args(const dwf_ctx* c, const dwfk::HopCfg& cfg, int nrhs);
 // Scalar slots in the B_SCAL buffer.
 enum ScalarSlot : int { S_GAUGE = 0, S_PLAQ, S_PF, S_KIN, S_COUNT = 8 };
 

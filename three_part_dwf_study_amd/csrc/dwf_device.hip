@@ -24,6 +24,7 @@
 #include "dwf_context.hpp"
 #include "dwf_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 
@@ -177,16 +178,23 @@ template <typename R> struct SiteK {
     R stp, stm, sxp, sxm;
 };
 
+// Links of right-hand side rhs (group g, lattice of V4 sites).
 template <typename R>
-__device__ __forceinline__ SiteK<R> site_consts(const HopArgs<R>& a, int rhs, int t, int x)
+__device__ __forceinline__ const typename Cx<R>::T* rhs_links(const HopArgs<R>& a, int g, int rhs, int V4)
 {
-    const Lat& L = a.L;
+    using C = typename Cx<R>::T;
+    const C* base = static_cast<const C*>(a.links);
+    if (a.cfg_map) return base + (long long)a.cfg_map[rhs] * a.link_stride;
+    const int cfg = (rhs - a.gr.rhs0[g]) / a.gr.rpc[g];
+    return base + a.gr.link0[g] + (long long)cfg * 2 * V4;
+}
+
+template <typename R>
+__device__ __forceinline__ SiteK<R> site_consts(const Lat& L, const typename Cx<R>::T* lk, int t, int x)
+{
     const int Nx = L.Nx, Nt = L.Nt, V4 = L.V4;
     const int tm = (t == 0) ? Nt - 1 : t - 1;
     const int xm = (x == 0) ? Nx - 1 : x - 1;
-    using C = typename Cx<R>::T;
-    const int cfg = a.cfg_map ? a.cfg_map[rhs] : rhs / a.rhs_per_cfg;
-    const C* lk = static_cast<const C*>(a.links) + (long long)cfg * a.link_stride;
     const int site = t * Nx + x;
     SiteK<R> k;
     k.ut = lk[site];
@@ -328,29 +336,35 @@ __device__ __forceinline__ void hop_epi(const HopArgs<R>& a, long long rb, long 
 template <typename R, bool DAG, bool D5, int EPI, bool PF>
 __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
 {
-    // 1D grid over (tile, s-chunk, rhs), optionally in XCD-aware order
-    const int nblk = a.cfg.nblk();
+    // 1D grid over (lattice group, rhs, s-chunk, tile), optionally in XCD-aware order
     const int total = gridDim.x;
     const int lb = a.cfg.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
-    const int rhs = lb / nblk;
-    const int bid = lb - rhs * nblk;            // block within this right-hand side
-    const int chunk = bid / a.cfg.bx;
-    const int tile = bid - chunk * a.cfg.bx;
-    const Lat& L = a.L;
+    const int g = group_of_block(a.gr, a.gb, lb);
+    const int bx = a.gb.bx[g], nblk = bx * a.cfg.chunks, pstride = a.gb.pstride;
+    const int local = lb - a.gb.blk0[g];
+    const int rl = local / nblk;                // right-hand side within the group
+    const int rhs = a.gr.rhs0[g] + rl;
+    const int bid = local - rl * nblk;          // block within this right-hand side
+    const int chunk = bid / bx;
+    const int tile = bid - chunk * bx;
+    Lat L = a.L;
+    L.Nt = a.gr.Nt[g];
+    L.Nx = a.gr.Nx[g];
+    L.V4 = L.Nt * L.Nx;
     if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
 
     __shared__ double sh[8];
 
     R alpha = 0;
     if constexpr (EPI == EPI_CGUPD) {
-        const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, 0, sh);
+        const double pAp = sum_partials(a.partA + (size_t)rhs * pstride * 2, pstride, 0, sh);
         if (pAp == 0.0 || (a.anum && !(pAp > 0.0))) return;  // block-uniform; a later kernel records the failure
         alpha = (R)((a.anum ? a.anum[(size_t)rhs * a.anum_stride] : a.st[rhs].rr[a.slot]) / pAp);
     }
 
     const int V4 = L.V4;
     const long long V5 = L.V5();
-    const long long rb = (long long)rhs * V5 * 4;  // offset in reals
+    const long long rb = (a.gr.site0[g] + (long long)rl * V5) * 4;  // offset in reals
     int t, x;
     if (a.cfg.tx) {  // 2D tile: (kBlock/tx) rows x tx columns
         const int tt = tile / a.cfg.tiles_x, tc = tile - tt * a.cfg.tiles_x;
@@ -373,7 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
         const int tm = (t == 0) ? Nt - 1 : t - 1;
         const int xp = (x + 1 == Nx) ? 0 : x + 1;
         const int xm = (x == 0) ? Nx - 1 : x - 1;
-        const SiteK<R> k = site_consts<R>(a, rhs, t, x);
+        const SiteK<R> k = site_consts<R>(L, rhs_links(a, g, rhs, V4), t, x);
         const int ntp = tp * Nx + x, ntm = tm * Nx + x, nxp = t * Nx + xp, nxm = t * Nx + xm;
 
         const R* in = a.in + rb;
@@ -432,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
     if constexpr (EPI != EPI_STORE) {
         block_sum2(acc0, acc1, sh);
         if (threadIdx.x == 0) {
-            double* pp = a.part + ((size_t)rhs * nblk + bid) * 2;
+            double* pp = a.part + ((size_t)rhs * pstride + bid) * 2;
             pp[0] = acc0;
             pp[1] = acc1;
         }
@@ -598,7 +612,8 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
             hslot = (hh - 2 * tx - ty + 1) * W + tx + 1;
         }
     }
-    const SiteK<R> k = site_consts<R>(a, rhs, t, x);
+    // (a single-lattice launch: group 0 is the lattice L)
+    const SiteK<R> k = site_consts<R>(L, rhs_links(a, 0, rhs, V4), t, x);
     double acc0 = 0.0, acc1 = 0.0;
     const int s0 = chunk * a.cfg.spt;
     const int s1 = min(Ls, s0 + a.cfg.spt);
@@ -857,7 +872,7 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
 // p = r + beta p.  Block 0 writes the next slot's scalars; the other blocks only read
 // the current slot, so nothing in one launch races.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_cg_p(const long long V5, const int nblk, R* r, R* p,
+__global__ __launch_bounds__(kBlock) void k_cg_p(const VecLayout lay, const int nblk, R* r, R* p,
                                                  const double* partA, const double* partB,
                                                  CgState* st, const int slot, const int mode)
 {
@@ -898,7 +913,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const long long V5, const int n
     }
     if (zero || (mode == MODE_CG64 && conv)) return;
     const R beta = (R)(rr_new / rr_old);
-    const long long rb = (long long)rhs * V5 * 4;
+    const int g = group_of_rhs(lay.gr, rhs);
+    const long long V5 = lay.sites(g);
+    const long long rb = lay.offset(g, rhs) * 4;
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < V5; i += (long long)gridDim.x * kBlock) {
         const Sp<R> rv = ldsp(r + rb, i);
         Sp<R> pv = ldsp(p + rb, i);
@@ -991,14 +1008,15 @@ __global__ void k_set_frozen_all(CgState* st, const int slot, const int nrhs)
 }
 
 // x64 += x32; x32 = 0 for the right-hand sides still running.
-__global__ __launch_bounds__(kBlock) void k_accum_x(const long long n4, double* x64, float* x32,
+__global__ __launch_bounds__(kBlock) void k_accum_x(const VecLayout lay, double* x64, float* x32,
                                                     const CgState* st, const int slot)
 {
     const int rhs = blockIdx.z;
     if (st[rhs].flags[slot] & F_DONE) return;
+    const int g = group_of_rhs(lay.gr, rhs);
     const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;  // site
-    if (i * 4 >= n4) return;
-    const long long rb = (long long)rhs * n4;
+    if (i >= lay.sites(g)) return;
+    const long long rb = lay.offset(g, rhs) * 4;
     const Sp<float> d = ldsp(x32 + rb, i);
     Sp<double> v = ldsp(x64 + rb, i);
     v.ar += d.ar;
@@ -1089,6 +1107,26 @@ void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 
 }  // namespace
 
+Groups single_group(const Lat& L, int nrhs, int rhs_per_cfg)
+{
+    Groups G{};
+    G.n = 1;
+    G.rhs0[0] = 0;
+    G.rhs0[1] = nrhs;
+    G.Nt[0] = L.Nt;
+    G.Nx[0] = L.Nx;
+    G.rpc[0] = rhs_per_cfg > 0 ? rhs_per_cfg : 1 << 30;
+    return G;
+}
+
+int hop_pstride(const Groups& G, const HopCfg& cfg)
+{
+    if (G.n <= 1) return cfg.nblk();
+    int m = 0;
+    for (int g = 0; g < G.n; ++g) m = std::max(m, (G.Nt[g] * G.Nx[g] + kBlock - 1) / kBlock * cfg.chunks);
+    return m;
+}
+
 HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefetch, int tile_x, int nt_store,
               int lds)
 {
@@ -1132,9 +1170,35 @@ HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefe
     return c;
 }
 
-template <typename R> void launch_hop(bool dagger, int epi, const HopArgs<R>& a, int nrhs, hipStream_t s)
+template <typename R> void launch_hop(bool dagger, int epi, const HopArgs<R>& a0, int nrhs, hipStream_t s)
 {
-    const dim3 grid((unsigned)(a.cfg.nblk() * nrhs));
+    HopArgs<R> a = a0;
+    long long nb = 0;
+    if (a.gr.n <= 1) {
+        // one lattice: the shape's own tiles (direct or LDS-staged)
+        if (a.gr.n == 0) a.gr = single_group(a.L, nrhs, a.rhs_per_cfg);
+        if (a.gr.Nt[0] != a.L.Nt || a.gr.Nx[0] != a.L.Nx || a.gr.rhs0[1] != nrhs || a.gr.site0[0] != 0 ||
+            a.gr.link0[0] != 0)
+            throw DwfError(DWF_ERR_ARG, "stencil launch: one-group layout does not match the lattice");
+        nb = (long long)a.cfg.nblk() * nrhs;
+        a.gb.blk0[0] = 0;
+        a.gb.blk0[1] = (int)nb;
+        a.gb.bx[0] = a.cfg.bx;
+        a.gb.pstride = a.cfg.nblk();
+    } else {
+        // several lattices: linear blocks of kBlock consecutive 4D sites per group
+        if (a.cfg.lds || a.cfg.tx || a.cfg_map || a.gr.n > kMaxGroups || a.gr.rhs0[a.gr.n] != nrhs)
+            throw DwfError(DWF_ERR_ARG, "stencil launch: bad lattice-group layout");
+        for (int g = 0; g < a.gr.n; ++g) {
+            a.gb.blk0[g] = (int)nb;
+            a.gb.bx[g] = (a.gr.Nt[g] * a.gr.Nx[g] + kBlock - 1) / kBlock;
+            nb += (long long)(a.gr.rhs0[g + 1] - a.gr.rhs0[g]) * a.gb.bx[g] * a.cfg.chunks;
+        }
+        a.gb.blk0[a.gr.n] = (int)nb;
+        a.gb.pstride = hop_pstride(a.gr, a.cfg);
+    }
+    if (nb <= 0 || nb > 0x7fffffffLL) throw DwfError(DWF_ERR_ARG, "stencil launch: grid size out of range");
+    const dim3 grid((unsigned)nb);
     if (a.L.d5) {
         if (dagger)
             dispatch_epi<R, true, true>(epi, a, grid, s);
@@ -1150,18 +1214,28 @@ template <typename R> void launch_hop(bool dagger, int epi, const HopArgs<R>& a,
 template void launch_hop<float>(bool, int, const HopArgs<float>&, int, hipStream_t);
 template void launch_hop<double>(bool, int, const HopArgs<double>&, int, hipStream_t);
 
+namespace {
+long long max_sites(const VecLayout& lay)
+{
+    long long m = 0;
+    for (int g = 0; g < lay.gr.n; ++g) m = std::max(m, lay.sites(g));
+    return m;
+}
+}  // namespace
+
 template <typename R>
-void launch_cg_p(long long sites, int nblk, R* r, R* p, const double* partA, const double* partB, CgState* st,
+void launch_cg_p(const VecLayout& lay, int nblk, R* r, R* p, const double* partA, const double* partB, CgState* st,
                  int slot, int mode, int nrhs, hipStream_t s)
 {
-    const long long per = (sites + 4LL * kBlock - 1) / (4LL * kBlock);  // 4 sites per thread
+    // blocks of 4 sites per thread over the largest vector (grid-stride loops cover the rest)
+    const long long per = (max_sites(lay) + 4LL * kBlock - 1) / (4LL * kBlock);
     const dim3 grid((unsigned)per, 1, nrhs);
-    k_cg_p<R><<<grid, kBlock, 0, s>>>(sites, nblk, r, p, partA, partB, st, slot, mode);
+    k_cg_p<R><<<grid, kBlock, 0, s>>>(lay, nblk, r, p, partA, partB, st, slot, mode);
 }
-template void launch_cg_p<float>(long long, int, float*, float*, const double*, const double*, CgState*, int,
+template void launch_cg_p<float>(const VecLayout&, int, float*, float*, const double*, const double*, CgState*, int,
                                  int, int, hipStream_t);
-template void launch_cg_p<double>(long long, int, double*, double*, const double*, const double*, CgState*, int,
-                                  int, int, hipStream_t);
+template void launch_cg_p<double>(const VecLayout&, int, double*, double*, const double*, const double*, CgState*,
+                                  int, int, int, hipStream_t);
 
 void launch_cg_start(int nblk, const double* partR, CgState* st, double tol, double delta, int nrhs, hipStream_t s)
 {
@@ -1183,12 +1257,11 @@ void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s)
     k_set_frozen_all<<<(nrhs + 63) / 64, 64, 0, s>>>(st, slot, nrhs);
 }
 
-void launch_accum_x(long long n4, double* x64, float* x32, const CgState* st, int slot, int nrhs,
+void launch_accum_x(const VecLayout& lay, double* x64, float* x32, const CgState* st, int slot, int nrhs,
                     hipStream_t s)
 {
-    const long long sites = n4 / 4;
-    const dim3 grid((unsigned)((sites + kBlock - 1) / kBlock), 1, nrhs);
-    k_accum_x<<<grid, kBlock, 0, s>>>(n4, x64, x32, st, slot);
+    const dim3 grid((unsigned)((max_sites(lay) + kBlock - 1) / kBlock), 1, nrhs);
+    k_accum_x<<<grid, kBlock, 0, s>>>(lay, x64, x32, st, slot);
 }
 
 template <typename R> void launch_zero(R* p, long long n, hipStream_t s)

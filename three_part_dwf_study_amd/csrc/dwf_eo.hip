@@ -21,9 +21,11 @@
 // Half-lattice layout: parity p = (t+x)&1 holds V4/2 sites, row-major (t, xh) with
 // x = 2 xh + ((t+p)&1); a vector is [rhs][s][t][xh] spinors.  Links stay lexicographic.
 
+#include "dwf_context.hpp"
 #include "dwf_internal.hpp"
 #include "dwf_eo.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace dwfk {
@@ -132,10 +134,21 @@ __device__ __forceinline__ EoSite<R> eo_site(const Lat& L, int Nxh, int tpar, in
     return g;
 }
 
-// Links of the configuration that right-hand side `rhs` belongs to.
-template <typename R> __device__ __forceinline__ const void* cfg_links(const EoArgs<R>& a, int rhs)
+// Links of the configuration that right-hand side `rhs` (group g, V4 sites) belongs to.
+template <typename R> __device__ __forceinline__ const void* cfg_links(const EoArgs<R>& a, int g, int rhs, int V4)
 {
-    return static_cast<const typename C2T<R>::T*>(a.links) + (long long)(rhs / a.rhs_per_cfg) * a.link_stride;
+    const int cfg = (rhs - a.gr.rhs0[g]) / a.gr.rpc[g];
+    return static_cast<const typename C2T<R>::T*>(a.links) + a.gr.link0[g] + (long long)cfg * 2 * V4;
+}
+
+// Lattice of group g.
+__device__ __forceinline__ Lat group_lat(const Lat& base, const Groups& G, int g)
+{
+    Lat L = base;
+    L.Nt = G.Nt[g];
+    L.Nx = G.Nx[g];
+    L.V4 = L.Nt * L.Nx;
+    return L;
 }
 
 // o += c * H psi at one slice (H: the bracket of D's hop, or D^dag's when DAG).
@@ -238,17 +251,23 @@ __global__ __launch_bounds__(kBlockA) void k_eo_a(const EoArgs<R> a)
 {
     const int total = gridDim.x;
     const int lb = a.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
-    const int rhs = lb / a.bxa;
-    const int hs = (lb - rhs * a.bxa) * kBlockA + threadIdx.x;
+    const int gi = group_of_block(a.gr, a.gb, lb);
+    const int bxa = a.gb.bx[gi];
+    const int local = lb - a.gb.blk0[gi];
+    const int rl = local / bxa;
+    const int rhs = a.gr.rhs0[gi] + rl;
+    const int hs = (local - rl * bxa) * kBlockA + threadIdx.x;
     if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;
-    if (hs >= a.V4h) return;
-    const long long V5h = (long long)a.V4h * LS;
-    const long long rb = (long long)rhs * V5h * 4;
-    const EoSite<R> g = eo_site<R>(a.L, a.Nxh, a.tpar, hs, cfg_links(a, rhs));
+    const Lat L = group_lat(a.L, a.gr, gi);
+    const int V4h = L.V4 / 2;
+    if (hs >= V4h) return;
+    const long long V5h = (long long)V4h * LS;
+    const long long rb = (a.gr.site0[gi] / 2 + (long long)rl * V5h) * 4;
+    const EoSite<R> g = eo_site<R>(L, L.Nx / 2, a.tpar, hs, cfg_links(a, gi, rhs, L.V4));
     R v0r[LS], v0i[LS], v1r[LS], v1i[LS];
 #pragma unroll
     for (int s = 0; s < LS; ++s) {
-        const long long sl = (long long)s * a.V4h;
+        const long long sl = (long long)s * V4h;
         Sp<R> o{0, 0, 0, 0};
         if (a.cb != (R)0) {
             const Sp<R> bv = ld(a.b + rb, sl + hs);
@@ -267,7 +286,7 @@ __global__ __launch_bounds__(kBlockA) void k_eo_a(const EoArgs<R> a)
         wall_solve<R, LS, !DAG>(v1r, v1i, d, mf);
     }
 #pragma unroll
-    for (int s = 0; s < LS; ++s) st(a.out + rb, (long long)s * a.V4h + hs, Sp<R>{v0r[s], v0i[s], v1r[s], v1i[s]});
+    for (int s = 0; s < LS; ++s) st(a.out + rb, (long long)s * V4h + hs, Sp<R>{v0r[s], v0i[s], v1r[s], v1i[s]});
 }
 
 // out = ca [A | A^dag | 1] psi + ch H hin on the target parity, with the CG epilogues
@@ -275,29 +294,33 @@ __global__ __launch_bounds__(kBlockA) void k_eo_a(const EoArgs<R> a)
 template <typename R, bool DAG, bool AOP, int EPI>
 __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
 {
-    const int nblk = a.bx * a.chunks;
     const int total = gridDim.x;
     const int lb = a.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
-    const int rhs = lb / nblk;
-    const int bid = lb - rhs * nblk;
-    const int chunk = bid / a.bx;
-    const int hs = (bid - chunk * a.bx) * kBlock + threadIdx.x;
+    const int gi = group_of_block(a.gr, a.gb, lb);
+    const int bx = a.gb.bx[gi], nblk = bx * a.chunks, pstride = a.gb.pstride;
+    const int local = lb - a.gb.blk0[gi];
+    const int rl = local / nblk;
+    const int rhs = a.gr.rhs0[gi] + rl;
+    const int bid = local - rl * nblk;
+    const int chunk = bid / bx;
+    const int hs = (bid - chunk * bx) * kBlock + threadIdx.x;
     if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;
     __shared__ double sh[8];
     R alpha = 0;
     if constexpr (EPI == EPI_CGUPD) {
-        const double pAp = sum_partials(a.partA + (size_t)rhs * nblk * 2, nblk, sh);
+        const double pAp = sum_partials(a.partA + (size_t)rhs * pstride * 2, pstride, sh);
         if (pAp == 0.0) return;
         alpha = (R)(a.st[rhs].rr[a.slot] / pAp);
     }
-    const int Ls = a.L.Ls, V4h = a.V4h;
+    const Lat L = group_lat(a.L, a.gr, gi);
+    const int Ls = L.Ls, V4h = L.V4 / 2;
     const long long V5h = (long long)V4h * Ls;
-    const long long rb = (long long)rhs * V5h * 4;
+    const long long rb = (a.gr.site0[gi] / 2 + (long long)rl * V5h) * 4;
     double acc0 = 0.0, acc1 = 0.0;
     const int s0 = chunk * a.spt, s1 = min(Ls, s0 + a.spt);
     if (hs < V4h && s0 < s1) {
-        const EoSite<R> g = eo_site<R>(a.L, a.Nxh, a.tpar, hs, cfg_links(a, rhs));
-        const R d = (R)a.L.diag, mf = (R)a.L.mf;
+        const EoSite<R> g = eo_site<R>(L, L.Nx / 2, a.tpar, hs, cfg_links(a, gi, rhs, L.V4));
+        const R d = (R)L.diag, mf = (R)L.mf;
         const R* in = a.in + rb;
         Sp<R> cur = ld(in, (long long)s0 * V4h + hs);
         Sp<R> dn{0, 0, 0, 0};
@@ -378,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
     if constexpr (EPI != EPI_STORE) {
         block_sum2(acc0, acc1, sh);
         if (threadIdx.x == 0) {
-            double* pp = a.part + ((size_t)rhs * nblk + bid) * 2;
+            double* pp = a.part + ((size_t)rhs * pstride + bid) * 2;
             pp[0] = acc0;
             pp[1] = acc1;
         }
@@ -386,14 +409,15 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
 }
 
 // full lexicographic [rhs][s][t][x] <-> halves [rhs][s][t][xh] (FP64)
-__global__ void k_eo_split(const int Nt, const int Nx, const int Ls, const double* full, double* ev, double* od)
+__global__ void k_eo_split(const VecLayout lay, const double* full, double* ev, double* od)
 {
-    const int V4 = Nt * Nx, V4h = V4 / 2;
-    const long long V5 = (long long)V4 * Ls;
     const int rhs = blockIdx.y;
-    const double4* F = reinterpret_cast<const double4*>(full) + rhs * V5;
-    double4* E = reinterpret_cast<double4*>(ev) + rhs * (V5 / 2);
-    double4* O = reinterpret_cast<double4*>(od) + rhs * (V5 / 2);
+    const int gi = group_of_rhs(lay.gr, rhs);
+    const int Nx = lay.gr.Nx[gi], V4 = lay.gr.Nt[gi] * Nx, V4h = V4 / 2;
+    const long long V5 = lay.sites(gi), off = lay.offset(gi, rhs);
+    const double4* F = reinterpret_cast<const double4*>(full) + off;
+    double4* E = reinterpret_cast<double4*>(ev) + off / 2;
+    double4* O = reinterpret_cast<double4*>(od) + off / 2;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < V5; i += (long long)gridDim.x * blockDim.x) {
         const int s = (int)(i / V4), site = (int)(i - (long long)s * V4);
         const int t = site / Nx, x = site - t * Nx;
@@ -403,14 +427,15 @@ __global__ void k_eo_split(const int Nt, const int Nx, const int Ls, const doubl
     }
 }
 
-__global__ void k_eo_merge(const int Nt, const int Nx, const int Ls, const double* ev, const double* od, double* full)
+__global__ void k_eo_merge(const VecLayout lay, const double* ev, const double* od, double* full)
 {
-    const int V4 = Nt * Nx, V4h = V4 / 2;
-    const long long V5 = (long long)V4 * Ls;
     const int rhs = blockIdx.y;
-    double4* F = reinterpret_cast<double4*>(full) + rhs * V5;
-    const double4* E = reinterpret_cast<const double4*>(ev) + rhs * (V5 / 2);
-    const double4* O = reinterpret_cast<const double4*>(od) + rhs * (V5 / 2);
+    const int gi = group_of_rhs(lay.gr, rhs);
+    const int Nx = lay.gr.Nx[gi], V4 = lay.gr.Nt[gi] * Nx, V4h = V4 / 2;
+    const long long V5 = lay.sites(gi), off = lay.offset(gi, rhs);
+    double4* F = reinterpret_cast<double4*>(full) + off;
+    const double4* E = reinterpret_cast<const double4*>(ev) + off / 2;
+    const double4* O = reinterpret_cast<const double4*>(od) + off / 2;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < V5; i += (long long)gridDim.x * blockDim.x) {
         const int s = (int)(i / V4), site = (int)(i - (long long)s * V4);
         const int t = site / Nx, x = site - t * Nx;
@@ -449,9 +474,40 @@ bool eo_supported(const Lat& L)
     return L.Ls == 4 || L.Ls == 8 || L.Ls == 12 || L.Ls == 16;
 }
 
-template <typename R> void launch_eo_a(bool dagger, bool ainv, const EoArgs<R>& a, int nrhs, hipStream_t s)
+namespace {
+// Normalises a's groups (n = 0: one group of a.L) and fills its block ranges, `per`
+// target sites per block, `chunks` s-chunks; returns the grid size.
+template <typename R> unsigned eo_blocks(EoArgs<R>& a, int nrhs, int per, int chunks)
 {
-    const unsigned grid = (unsigned)(a.bxa * nrhs);
+    if (a.gr.n == 0) a.gr = single_group(a.L, nrhs, a.rhs_per_cfg);
+    if (a.gr.n > kMaxGroups || a.gr.rhs0[a.gr.n] != nrhs)
+        throw DwfError(DWF_ERR_ARG, "even-odd launch: bad lattice-group layout");
+    long long nb = 0;
+    int m = 0;
+    for (int g = 0; g < a.gr.n; ++g) {
+        a.gb.blk0[g] = (int)nb;
+        a.gb.bx[g] = (a.gr.Nt[g] * a.gr.Nx[g] / 2 + per - 1) / per;
+        m = std::max(m, a.gb.bx[g] * chunks);
+        nb += (long long)(a.gr.rhs0[g + 1] - a.gr.rhs0[g]) * a.gb.bx[g] * chunks;
+    }
+    a.gb.blk0[a.gr.n] = (int)nb;
+    a.gb.pstride = m;
+    if (nb <= 0 || nb > 0x7fffffffLL) throw DwfError(DWF_ERR_ARG, "even-odd launch: grid size out of range");
+    return (unsigned)nb;
+}
+}  // namespace
+
+int eo_pstride(const Groups& G, int chunks)
+{
+    int m = 0;
+    for (int g = 0; g < G.n; ++g) m = std::max(m, (G.Nt[g] * G.Nx[g] / 2 + kBlock - 1) / kBlock * chunks);
+    return m;
+}
+
+template <typename R> void launch_eo_a(bool dagger, bool ainv, const EoArgs<R>& a0, int nrhs, hipStream_t s)
+{
+    EoArgs<R> a = a0;
+    const unsigned grid = eo_blocks(a, nrhs, kBlockA, 1);
     const int Ls = a.L.Ls;
 #define DWF_EO_A(LSV)                                      \
     if (Ls == LSV) {                                       \
@@ -470,9 +526,10 @@ template <typename R> void launch_eo_a(bool dagger, bool ainv, const EoArgs<R>& 
 template void launch_eo_a<float>(bool, bool, const EoArgs<float>&, int, hipStream_t);
 template void launch_eo_a<double>(bool, bool, const EoArgs<double>&, int, hipStream_t);
 
-template <typename R> void launch_eo_b(bool dagger, bool aop, int epi, const EoArgs<R>& a, int nrhs, hipStream_t s)
+template <typename R> void launch_eo_b(bool dagger, bool aop, int epi, const EoArgs<R>& a0, int nrhs, hipStream_t s)
 {
-    const unsigned grid = (unsigned)(a.bx * a.chunks * nrhs);
+    EoArgs<R> a = a0;
+    const unsigned grid = eo_blocks(a, nrhs, kBlock, a.chunks);
     if (dagger) {
         if (aop)
             dispatch_b<R, true, true>(epi, a, grid, s);
@@ -488,18 +545,23 @@ template <typename R> void launch_eo_b(bool dagger, bool aop, int epi, const EoA
 template void launch_eo_b<float>(bool, bool, int, const EoArgs<float>&, int, hipStream_t);
 template void launch_eo_b<double>(bool, bool, int, const EoArgs<double>&, int, hipStream_t);
 
-void launch_eo_split(const Lat& L, const double* full, double* ev, double* od, int nrhs, hipStream_t s)
+namespace {
+unsigned split_blocks(const VecLayout& lay)
 {
-    const long long V5 = L.V5();
-    const unsigned bx = (unsigned)std::min<long long>((V5 + 255) / 256, 4096);
-    k_eo_split<<<dim3(bx, nrhs), 256, 0, s>>>(L.Nt, L.Nx, L.Ls, full, ev, od);
+    long long m = 0;
+    for (int g = 0; g < lay.gr.n; ++g) m = std::max(m, lay.sites(g));
+    return (unsigned)std::min<long long>((m + 255) / 256, 4096);
+}
+}  // namespace
+
+void launch_eo_split(const VecLayout& lay, const double* full, double* ev, double* od, int nrhs, hipStream_t s)
+{
+    k_eo_split<<<dim3(split_blocks(lay), nrhs), 256, 0, s>>>(lay, full, ev, od);
 }
 
-void launch_eo_merge(const Lat& L, const double* ev, const double* od, double* full, int nrhs, hipStream_t s)
+void launch_eo_merge(const VecLayout& lay, const double* ev, const double* od, double* full, int nrhs, hipStream_t s)
 {
-    const long long V5 = L.V5();
-    const unsigned bx = (unsigned)std::min<long long>((V5 + 255) / 256, 4096);
-    k_eo_merge<<<dim3(bx, nrhs), 256, 0, s>>>(L.Nt, L.Nx, L.Ls, ev, od, full);
+    k_eo_merge<<<dim3(split_blocks(lay), nrhs), 256, 0, s>>>(lay, ev, od, full);
 }
 
 }  // namespace dwfk

@@ -158,14 +158,59 @@ void dwf_ctx::check_launch(const char* what) { check(hipGetLastError(), what); }
 
 void dwf_ctx::bind_rhs(int nrhs)
 {
+    if (nrhs < 1) throw DwfError(DWF_ERR_ARG, "the number of right-hand sides must be positive");
     if (ncfg <= 1) {
         rhs_per_cfg = 1 << 30;
+    } else {
+        if (nrhs % ncfg != 0)
+            throw DwfError(DWF_ERR_ARG, "the number of right-hand sides must be a multiple of the configurations held (" +
+                                            std::to_string(ncfg) + ")");
+        rhs_per_cfg = nrhs / ncfg;
+    }
+    if (!multi()) {
+        gr = single_group(L, nrhs, rhs_per_cfg);
         return;
     }
-    if (nrhs % ncfg != 0)
-        throw DwfError(DWF_ERR_ARG, "the number of right-hand sides must be a multiple of the configurations held (" +
-                                        std::to_string(ncfg) + ")");
-    rhs_per_cfg = nrhs / ncfg;
+    // right-hand sides configuration-major in group order, rhs_per_cfg per configuration
+    Groups G{};
+    G.n = (int)gdefs.size();
+    long long site = 0, link = 0;
+    int r = 0;
+    for (int g = 0; g < G.n; ++g) {
+        const GroupDef& d = gdefs[g];
+        G.rhs0[g] = r;
+        G.Nt[g] = d.Nt;
+        G.Nx[g] = d.Nx;
+        G.rpc[g] = rhs_per_cfg;
+        G.site0[g] = site;
+        G.link0[g] = link;
+        r += d.ncfg * rhs_per_cfg;
+        site += (long long)d.ncfg * rhs_per_cfg * L.Ls * d.Nt * d.Nx;
+        link += 2LL * d.ncfg * d.Nt * d.Nx;
+    }
+    G.rhs0[G.n] = r;
+    gr = G;
+}
+
+void dwf_ctx::require_single_lattice(const char* what) const
+{
+    if (multi()) throw DwfError(DWF_ERR_ARG, std::string(what) + " needs a context holding a single lattice size");
+}
+
+Groups dwf_ctx::groups_for(int nrhs) const
+{
+    if (!multi()) return single_group(L, nrhs, rhs_per_cfg);
+    if (gr.n < 2 || gr.rhs0[gr.n] != nrhs) throw DwfError(DWF_ERR_ARG, "lattice groups: layout not bound to this call");
+    return gr;
+}
+
+VecLayout dwf_ctx::layout(int nrhs, int shift) const
+{
+    VecLayout v{};
+    v.gr = groups_for(nrhs);
+    v.Ls = L.Ls;
+    v.shift = shift;
+    return v;
 }
 
 void dwf_ctx::require_single_config(const char* what) const
@@ -175,8 +220,14 @@ void dwf_ctx::require_single_config(const char* what) const
 
 void dwf_ctx::reserve_configs(int n)
 {
-    if (n <= cfg_cap) return;
-    const size_t nl = (size_t)2 * L.V4 * n;
+    reserve_links((size_t)2 * L.V4 * n);
+    cfg_cap = std::max(cfg_cap, n);
+}
+
+void dwf_ctx::reserve_links(size_t nl)
+{
+    if (nl <= link_cap) return;
+    drop_graphs();  // captured kernels hold the old link buffers
     (void)hipStreamSynchronize(stream);
     (void)hipFree(theta);
     (void)hipFree(l64);
@@ -184,32 +235,21 @@ void dwf_ctx::reserve_configs(int n)
     theta = nullptr;
     l64 = nullptr;
     l32 = nullptr;
-    cfg_cap = 0;
+    link_cap = 0;
     check(hipMalloc(&theta, nl * sizeof(double)), "gauge allocation");
     check(hipMalloc(&l64, nl * sizeof(double2)), "gauge allocation");
     check(hipMalloc(&l32, nl * sizeof(float2)), "gauge allocation");
-    cfg_cap = n;
+    link_cap = nl;
 }
 
 // ------------------------------------------------------------------- operators
 namespace {
 
-template <typename R> HopArgs<R> hop_base(const dwf_ctx* c, const HopCfg& cfg)
-{
-    HopArgs<R> a{};
-    a.L = c->L;
-    a.cfg = cfg;
-    a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
-    a.link_stride = 2LL * c->L.V4;
-    a.rhs_per_cfg = c->rhs_per_cfg;
-    return a;
-}
-
 // out = op(in) on device buffers; tmp is needed for the normal operators.
 template <typename R> void apply_op(dwf_ctx* c, int op, const R* in, R* out, R* tmp, int nrhs)
 {
-    const HopCfg cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store, c->lds);
-    HopArgs<R> a = hop_base<R>(c, cfg);
+    const HopCfg cfg = ctx_hop_cfg(c, nrhs);
+    HopArgs<R> a = ctx_hop_args<R>(c, cfg, nrhs);
     switch (op) {
         case DWF_OP_D:
         case DWF_OP_DDAG:
@@ -232,9 +272,29 @@ template <typename R> void apply_op(dwf_ctx* c, int op, const R* in, R* out, R* 
 
 }  // namespace
 
+HopCfg ctx_hop_cfg(const dwf_ctx* c, int nrhs)
+{
+    if (c->multi()) return hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, 0, c->nt_store, 0);
+    return hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store, c->lds);
+}
+
+template <typename R> HopArgs<R> ctx_hop_args(const dwf_ctx* c, const HopCfg& cfg, int nrhs)
+{
+    HopArgs<R> a{};
+    a.L = c->L;
+    a.cfg = cfg;
+    a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
+    a.link_stride = 2LL * c->L.V4;
+    a.rhs_per_cfg = c->rhs_per_cfg;
+    if (c->multi()) a.gr = c->groups_for(nrhs);  // else one group of L, filled in by the launcher
+    return a;
+}
+template HopArgs<float> ctx_hop_args<float>(const dwf_ctx*, const HopCfg&, int);
+template HopArgs<double> ctx_hop_args<double>(const dwf_ctx*, const HopCfg&, int);
+
 void dwf_apply_device_impl(dwf_ctx* c, int op, int prec, const void* in, void* out, int nrhs)
 {
-    const size_t n = (size_t)c->L.V5() * 4 * nrhs;
+    const size_t n = (size_t)c->vec_sites(nrhs) * 4;
     if (prec == DWF_PREC_FP64) {
         double* tmp = op >= DWF_OP_DDDAG ? (double*)c->buf(B_APPLY_TMP, n * sizeof(double)) : nullptr;
         apply_op<double>(c, op, (const double*)in, (double*)out, tmp, nrhs);
@@ -275,6 +335,7 @@ int create(dwf_ctx** out, const Lat& L, int kind)
     }
     dwf_ctx* c = new dwf_ctx();
     c->L = L;
+    c->L0 = L;
     c->kind = kind;
     e = hipGetDevice(&c->device);
     if (e == hipSuccess) e = scratch_setup(c->device);
@@ -287,6 +348,7 @@ int create(dwf_ctx** out, const Lat& L, int kind)
         delete c;
         return dwf_hip_fail(e, "context creation");
     }
+    c->link_cap = nl;
     *out = c;
     return DWF_OK;
 }
@@ -388,10 +450,23 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
     return DWF_ERR_ARG;
 }
 
+namespace {
+// Back to the context's own lattice after lattice groups.
+void single_lattice(dwf_ctx* c)
+{
+    if (c->gdefs.empty() && c->L.Nt == c->L0.Nt && c->L.Nx == c->L0.Nx) return;
+    c->drop_graphs();
+    c->gdefs.clear();
+    c->L = c->L0;
+}
+}  // namespace
+
 int dwf_set_gauge_device(dwf_ctx* c, const double* theta_device)
 {
     return guard([&] {
         if (!c || !theta_device) throw DwfError(DWF_ERR_ARG, "dwf_set_gauge: null argument");
+        single_lattice(c);
+        c->reserve_configs(1);
         const size_t nl = (size_t)2 * c->L.V4;
         if (theta_device != c->theta)
             c->check(hipMemcpyAsync(c->theta, theta_device, nl * 8, hipMemcpyDeviceToDevice, c->stream), "gauge copy");
@@ -409,10 +484,49 @@ int dwf_set_gauge_batch(dwf_ctx* c, int ncfg, const double* theta)
 {
     return guard([&] {
         if (!c || !theta || ncfg < 1) throw DwfError(DWF_ERR_ARG, "dwf_set_gauge: bad argument");
+        single_lattice(c);
         const size_t nl = (size_t)2 * c->L.V4;
         c->reserve_configs(ncfg);
         c->check(hipMemcpyAsync(c->theta, theta, nl * ncfg * 8, hipMemcpyHostToDevice, c->stream), "gauge upload");
         launch_pack_links((long long)(nl * ncfg), c->theta, c->l64, c->l32, c->stream);
+        c->check_launch("pack links");
+        c->check(hipStreamSynchronize(c->stream), "gauge upload sync");
+        c->ncfg = ncfg;
+        c->have_gauge = true;
+        return DWF_OK;
+    });
+}
+
+int dwf_set_gauge_groups(dwf_ctx* c, int ngroups, const dwf_lattice_group* groups, const double* theta)
+{
+    return guard([&] {
+        if (!c || !groups || !theta || ngroups < 1 || ngroups > kMaxGroups)
+            throw DwfError(DWF_ERR_ARG, "dwf_set_gauge_groups: bad argument (1.." + std::to_string(kMaxGroups) +
+                                            " lattice groups)");
+        if (c->kind != 0) throw DwfError(DWF_ERR_ARG, "dwf_set_gauge_groups: needs a domain-wall context");
+        size_t nl = 0;
+        int ncfg = 0, big = 0;
+        for (int g = 0; g < ngroups; ++g) {
+            const dwf_lattice_group& d = groups[g];
+            if (d.Nt < 1 || d.Nx < 1 || d.ncfg < 1 || (long long)d.Nt * d.Nx * c->L0.Ls * 2 > (1LL << 31) - 1)
+                throw DwfError(DWF_ERR_ARG, "dwf_set_gauge_groups: lattice group " + std::to_string(g) +
+                                                " needs positive extents and configurations");
+            nl += (size_t)2 * d.Nt * d.Nx * d.ncfg;
+            ncfg += d.ncfg;
+            if ((long long)d.Nt * d.Nx > (long long)groups[big].Nt * groups[big].Nx) big = g;
+        }
+        c->drop_graphs();
+        c->L = c->L0;
+        c->L.Nt = groups[big].Nt;  // the largest lattice (shared Ls, masses, signs)
+        c->L.Nx = groups[big].Nx;
+        c->L.V4 = c->L.Nt * c->L.Nx;
+        c->gdefs.clear();
+        if (ngroups > 1)
+            for (int g = 0; g < ngroups; ++g) c->gdefs.push_back({groups[g].Nt, groups[g].Nx, groups[g].ncfg});
+        c->reserve_links(nl);
+        c->cfg_cap = std::max(c->cfg_cap, ncfg);
+        c->check(hipMemcpyAsync(c->theta, theta, nl * 8, hipMemcpyHostToDevice, c->stream), "gauge upload");
+        launch_pack_links((long long)nl, c->theta, c->l64, c->l32, c->stream);
         c->check_launch("pack links");
         c->check(hipStreamSynchronize(c->stream), "gauge upload sync");
         c->ncfg = ncfg;
@@ -444,7 +558,7 @@ int dwf_apply_device_noalloc(dwf_ctx* c, int op, int precision, const void* in, 
         c->bind_rhs(nrhs);
         const unsigned long long real = precision == DWF_PREC_FP64 ? 8 : 4;
         const unsigned long long need =
-            op >= DWF_OP_DDDAG ? (unsigned long long)c->L.V5() * 4ULL * (unsigned long long)nrhs * real : 0ULL;
+            op >= DWF_OP_DDDAG ? (unsigned long long)c->vec_sites(nrhs) * 4ULL * real : 0ULL;
         if (need > DWF_NOALLOC_SCRATCH_BYTES)
             throw DwfError(DWF_ERR_ARG, "dwf_apply_device_noalloc: the intermediate vector needs " +
                                             std::to_string(need) + " bytes, more than the " +
@@ -467,7 +581,7 @@ int dwf_apply(dwf_ctx* c, int op, const double* in, double* out, int nrhs)
             throw DwfError(DWF_ERR_ARG, "dwf_apply: bad argument");
         if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_apply: no gauge field set");
         c->bind_rhs(nrhs);
-        const size_t n = (size_t)c->L.V5() * 4 * nrhs;
+        const size_t n = (size_t)c->vec_sites(nrhs) * 4;
         double* din = (double*)c->buf(B_HOST_IN, n * 8);
         double* dout = (double*)c->buf(B_HOST_OUT, n * 8);
         c->check(hipMemcpyAsync(din, in, n * 8, hipMemcpyHostToDevice, c->stream), "apply upload");

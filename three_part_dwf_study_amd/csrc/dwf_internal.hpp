@@ -42,6 +42,68 @@ struct Lat {
     __host__ __device__ long long V5() const { return (long long)V4 * Ls; }
 };
 
+// ------------------------------------------------------------------ lattice groups
+// A batch of right-hand sides on several lattices at once (the batched analyzer over
+// configurations of different Nt x Nx: every kernel launch covers all of them).  Group g
+// holds right-hand sides [rhs0[g], rhs0[g+1]) on an Nt[g] x Nx[g] x Ls lattice.  Their
+// vectors follow one another, after the vectors of all earlier groups: site0[g] 5D sites
+// of the full lattice precede the group's first vector (a half-lattice vector batch has
+// half of every count).  Right-hand side r of group g uses configuration
+// (r - rhs0[g]) / rpc[g] of that group, whose links start link0[g] complex entries into
+// the link array.  A context's single lattice is one group with site0 = link0 = 0.
+constexpr int kMaxGroups = 8;
+struct Groups {
+    int n;                       // groups (1..kMaxGroups)
+    int rhs0[kMaxGroups + 1];    // first right-hand side of each group; rhs0[n] = nrhs
+    int Nt[kMaxGroups], Nx[kMaxGroups];
+    int rpc[kMaxGroups];         // right-hand sides per configuration
+    long long site0[kMaxGroups]; // full-lattice 5D sites before the group's first vector
+    long long link0[kMaxGroups]; // complex link entries before the group's first configuration
+};
+
+// Block ranges of one launch over groups: group g's blocks are [blk0[g], blk0[g+1]), bx[g]
+// per s-chunk and right-hand side.  pstride = block partials per right-hand side in the
+// partial arrays (the largest block count; a group with fewer blocks leaves its remaining
+// partials untouched, and the solvers zero them before they are read).
+struct GroupBlocks {
+    int blk0[kMaxGroups + 1];
+    int bx[kMaxGroups];
+    int pstride;
+};
+
+// Scalar scans (n <= kMaxGroups, uniform over the block).
+__host__ __device__ __forceinline__ int group_of_block(const Groups& G, const GroupBlocks& B, int lb)
+{
+    int g = 0;
+    for (int k = 1; k < G.n; ++k) g = lb >= B.blk0[k] ? k : g;
+    return g;
+}
+__host__ __device__ __forceinline__ int group_of_rhs(const Groups& G, int rhs)
+{
+    int g = 0;
+    for (int k = 1; k < G.n; ++k) g = rhs >= G.rhs0[k] ? k : g;
+    return g;
+}
+
+// Extent and offset (in sites) of right-hand side rhs's vector in a batch over groups;
+// shift 0 for full-lattice vectors, 1 for half-lattice (even-odd) ones.
+struct VecLayout {
+    Groups gr;
+    int Ls;
+    int shift;
+    __host__ __device__ long long sites(int g) const { return ((long long)Ls * gr.Nt[g] * gr.Nx[g]) >> shift; }
+    __host__ __device__ long long offset(int g, int rhs) const
+    {
+        return (gr.site0[g] + (long long)(rhs - gr.rhs0[g]) * Ls * gr.Nt[g] * gr.Nx[g]) >> shift;
+    }
+    __host__ __device__ long long total_sites() const
+    {
+        long long t = 0;
+        for (int g = 0; g < gr.n; ++g) t += (long long)(gr.rhs0[g + 1] - gr.rhs0[g]) * sites(g);
+        return t;
+    }
+};
+
 // Launch shape of the stencil kernels: one thread per 4D site per s-chunk; each thread
 // walks spt consecutive slices reusing its four links.
 struct HopCfg {
@@ -90,8 +152,10 @@ enum Epi : int {
 };
 
 template <typename R> struct HopArgs {
-    Lat L;
-    HopCfg cfg;
+    Lat L;               // the lattice of a single-group launch (Ls, masses and signs of every group)
+    HopCfg cfg;          // launch shape; with several groups: linear blocks, bx per group in gb
+    Groups gr;           // right-hand sides by lattice (n = 0: one group from L, nrhs and rhs_per_cfg)
+    GroupBlocks gb;      // set by launch_hop
     const void* links;   // [ncfg][2][V4] complex R: (cos theta, sin theta)
     long long link_stride;  // complex entries per configuration (2 V4)
     int rhs_per_cfg;     // right-hand side r uses configuration r / rhs_per_cfg ...
@@ -135,13 +199,18 @@ template <typename R> struct HopArgs {
 void launch_pack_links(long long nlinks, const double* theta, double2* l64, float2* l32, hipStream_t s);
 template <typename R>
 void launch_hop(bool dagger, int epi, const HopArgs<R>& a, int nrhs, hipStream_t s);
+// The single-group layout of nrhs vectors of L (shift 1: half-lattice vectors).
+Groups single_group(const Lat& L, int nrhs, int rhs_per_cfg);
+// Block partials per right-hand side of the direct-load stencil over groups (linear
+// blocks, cfg.chunks s-chunks), i.e. GroupBlocks::pstride of its launches.
+int hop_pstride(const Groups& G, const HopCfg& cfg);
 template <typename R>
-void launch_cg_p(long long sites, int nblk, R* r, R* p, const double* partA, const double* partB, CgState* st,
+void launch_cg_p(const VecLayout& lay, int nblk, R* r, R* p, const double* partA, const double* partB, CgState* st,
                  int slot, int mode, int nrhs, hipStream_t s);
 void launch_cg_start(int nblk, const double* partR, CgState* st, double tol, double delta, int nrhs, hipStream_t s);
 void launch_ru_commit(int nblk, const double* partR, CgState* st, int slot, int nrhs, hipStream_t s);
 void launch_true_resid(int nblk, const double* partR, CgState* st, int nrhs, hipStream_t s);
-void launch_accum_x(long long n4, double* x64, float* x32, const CgState* st, int slot, int nrhs,
+void launch_accum_x(const VecLayout& lay, double* x64, float* x32, const CgState* st, int slot, int nrhs,
                     hipStream_t s);
 template <typename R> void launch_zero(R* p, long long n, hipStream_t s);
 void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s);

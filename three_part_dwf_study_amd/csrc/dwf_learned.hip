@@ -1546,6 +1546,7 @@ void require_net(const dwf_ctx* c, const dwf_net* net)
     if (!c || !net) throw DwfError(DWF_ERR_ARG, "learned preconditioner: null argument");
     if (c->kind != 0) throw DwfError(DWF_ERR_ARG, "learned preconditioner: needs a domain-wall context");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "learned preconditioner: no gauge field set");
+    c->require_single_lattice("the learned preconditioner");
     if (net->Ls != c->L.Ls) throw DwfError(DWF_ERR_ARG, "learned preconditioner: network Ls differs from the lattice's");
     if (c->L.V4 % 16 != 0)
         throw DwfError(DWF_ERR_ARG, "learned preconditioner: Nt*Nx must be a multiple of 16");
@@ -2241,7 +2242,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
             read();
         }
         if (fused) k_fcg_flush_x<<<vgrid, 256, 0, s>>>(x32, p32, p32b, fs, sites);
-        launch_accum_x((long long)sites * 4, x, x32, st, 0, nrhs, s);  // x += x32 (all not done)
+        launch_accum_x(c->layout(nrhs, 0), x, x32, st, 0, nrhs, s);  // x += x32 (all not done)
         reliable_update(false);
         c->check_launch("FCG reliable update");
         read();

@@ -13,6 +13,7 @@
 #include "../../include/dwf_hip.h"
 #include "dwf_context.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -41,21 +42,42 @@ __device__ __forceinline__ double bsum(double v, double* sh)
     return r;
 }
 
+// Geometry of configuration `cfg` (K right-hand sides each) in a batch over lattice
+// groups: its group, lattice, and the offset (in sites) of its first right-hand side.
+struct CfgGeo {
+    int g, Nt, Nx, V4;
+    long long V5, site;
+};
+__device__ __forceinline__ CfgGeo cfg_geo(const VecLayout& lay, int K, int cfg)
+{
+    CfgGeo c;
+    const int rhs = cfg * K;
+    c.g = group_of_rhs(lay.gr, rhs);
+    c.Nt = lay.gr.Nt[c.g];
+    c.Nx = lay.gr.Nx[c.g];
+    c.V4 = c.Nt * c.Nx;
+    c.V5 = (long long)c.V4 * lay.Ls;
+    c.site = lay.offset(c.g, rhs);
+    return c;
+}
+
 // Sources.  Wall: all x at t_src; point: the single site (t_src, x_src).  The source
 // spin ss fills P_L e_ss at s=0 and P_R e_ss at s=Ls-1 (correlator.hpp:131-138, 199-211).
 // The 4 wall / 4 reference point right-hand sides are ordered (t_src=0, ss=0), (0, 1),
 // (Nt/2, 0), (Nt/2, 1), point ones at x_src = 0.  With psite ([cfg][2] = (t0, x0)) the
 // point block is 2 right-hand sides (ss = 0, 1) at that site.  Noise: src = P_L eta at
-// s=0, + P_R eta at s=Ls-1 (:60-75).
-__global__ void k_sources(const int Nt, const int Nx, const int Ls, const int nwall, const int npoint,
-                          const int nnoise, const double* noise_all, const int* psite, double* src)
+// s=0, + P_R eta at s=Ls-1 (:60-75); configuration cfg's noise vectors start at
+// noise_all + noff[cfg] (nnoise vectors of V4 x 2 complex).
+__global__ void k_sources(const VecLayout lay, const int nwall, const int npoint, const int nnoise,
+                          const double* noise_all, const long long* noff, const int* psite, double* src)
 {
-    const int V4 = Nt * Nx;
-    const long long V5 = (long long)V4 * Ls;
     const int K = nwall + npoint + nnoise;       // right-hand sides per configuration
     const int cfg = blockIdx.y / K, rhs = blockIdx.y - cfg * K;
-    const double* noise = noise_all ? noise_all + (long long)cfg * nnoise * V4 * 4 : nullptr;
-    double4* out = reinterpret_cast<double4*>(src) + (long long)blockIdx.y * V5;
+    const CfgGeo G = cfg_geo(lay, K, cfg);
+    const int Nt = G.Nt, Nx = G.Nx, V4 = G.V4, Ls = lay.Ls;
+    const long long V5 = G.V5;
+    const double* noise = noise_all ? noise_all + noff[cfg] : nullptr;
+    double4* out = reinterpret_cast<double4*>(src) + G.site + (long long)rhs * V5;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V4; i += gridDim.x * blockDim.x) {
         const int t = i / Nx, x = i - t * Nx;
         double4 lo = make_double4(0, 0, 0, 0), hi = make_double4(0, 0, 0, 0);  // s = 0, s = Ls-1
@@ -93,18 +115,21 @@ __global__ void k_sources(const int Nt, const int Nx, const int Ls, const int nw
     }
 }
 
-// C_PP / C_J5 of the wall solutions (correlator.hpp:258-300): block per (t, source time),
-// outputs per source time k: pp[k][t], j5[k][t] summed over x and source spin.
-__global__ void k_contract_wall(const int Nt, const int Nx, const int Ls, const int K, const double* X, double* pp_all,
+// C_PP / C_J5 of the wall solutions (correlator.hpp:258-300): block per (t, source time k,
+// configuration), outputs pp[cfg][k][t], j5[cfg][k][t] (stride ntmax per source time)
+// summed over x and source spin.
+__global__ void k_contract_wall(const VecLayout lay, const int K, const int ntmax, const double* X, double* pp_all,
                                 double* j5_all)
 {
     __shared__ double sh[8];
     const int t = blockIdx.x, k = blockIdx.y, cfg = blockIdx.z;
-    const int V4 = Nt * Nx;
-    const long long V5 = (long long)V4 * Ls;
-    const double4* P = reinterpret_cast<const double4*>(X) + (long long)cfg * K * V5;
-    double* pp = pp_all + (long long)cfg * 2 * Nt;
-    double* j5 = j5_all + (long long)cfg * 2 * Nt;
+    const CfgGeo G = cfg_geo(lay, K, cfg);
+    if (t >= G.Nt) return;  // block-uniform
+    const int Nx = G.Nx, V4 = G.V4, Ls = lay.Ls;
+    const long long V5 = G.V5;
+    const double4* P = reinterpret_cast<const double4*>(X) + G.site;
+    double* pp = pp_all + (long long)cfg * 2 * ntmax;
+    double* j5 = j5_all + (long long)cfg * 2 * ntmax;
     double a = 0.0, b = 0.0;
     for (int x = threadIdx.x; x < Nx; x += blockDim.x) {
         const int i = t * Nx + x;
@@ -119,26 +144,28 @@ __global__ void k_contract_wall(const int Nt, const int Nx, const int Ls, const 
     a = bsum(a, sh);
     b = bsum(b, sh);
     if (threadIdx.x == 0) {
-        pp[k * Nt + t] = a;
-        j5[k * Nt + t] = b;
+        pp[k * ntmax + t] = a;
+        j5[k * ntmax + t] = b;
     }
 }
 
 // Point correlators (correlator.hpp:149-174): block per (t, source time k, configuration);
-// out[k][n][t] = sum_x cos(2 pi n (x - x0) / Nx) sum_ss |q|^2 and j5[k][t] the midpoint
-// correlator of k_contract_wall.  nsrc_t source times of 2 spins each follow one another
-// in the point block; x0 = psite[cfg][1] (0 without psite).
-__global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const int nmom, const int K,
-                                 const int nsrc_t, const int* psite, const double* X, double* out_all,
-                                 double* j5_all)
+// out[cfg][k][n][t] = sum_x cos(2 pi n (x - x0) / Nx) sum_ss |q|^2 and j5[cfg][k][t] the
+// midpoint correlator of k_contract_wall (stride ntmax per row).  nsrc_t source times of
+// 2 spins each follow one another in the point block, which starts at right-hand side
+// `first` of each configuration; x0 = psite[cfg][1] (0 without psite).
+__global__ void k_contract_point(const VecLayout lay, const int nmom, const int K, const int first, const int nsrc_t,
+                                 const int ntmax, const int* psite, const double* X, double* out_all, double* j5_all)
 {
     __shared__ double sh[8];
     const int t = blockIdx.x, k = blockIdx.y, cfg = blockIdx.z;
-    const int V4 = Nt * Nx;
-    const long long V5 = (long long)V4 * Ls;
-    const double4* P = reinterpret_cast<const double4*>(X) + (long long)cfg * K * V5;
-    double* out = out_all + (long long)cfg * nsrc_t * nmom * Nt;
-    double* j5 = j5_all + (long long)cfg * nsrc_t * Nt;
+    const CfgGeo G = cfg_geo(lay, K, cfg);
+    if (t >= G.Nt) return;  // block-uniform
+    const int Nx = G.Nx, V4 = G.V4, Ls = lay.Ls;
+    const long long V5 = G.V5;
+    const double4* P = reinterpret_cast<const double4*>(X) + G.site + (long long)first * V5;
+    double* out = out_all + (long long)cfg * nsrc_t * nmom * ntmax;
+    double* j5 = j5_all + (long long)cfg * nsrc_t * ntmax;
     const int x0 = psite ? psite[2 * cfg + 1] : 0;
     const double pi = 3.141592653589793238462643383279502884;
     double b = 0.0;
@@ -151,7 +178,7 @@ __global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const
         }
     }
     b = bsum(b, sh);
-    if (threadIdx.x == 0) j5[(long long)k * Nt + t] = b;
+    if (threadIdx.x == 0) j5[(long long)k * ntmax + t] = b;
     for (int n = 0; n < nmom; ++n) {
         const double p = 2.0 * pi * n / Nx;
         double v = 0.0;
@@ -166,20 +193,23 @@ __global__ void k_contract_point(const int Nt, const int Nx, const int Ls, const
             v += cos(p * (x - x0)) * local;
         }
         v = bsum(v, sh);
-        if (threadIdx.x == 0) out[((long long)k * nmom + n) * Nt + t] = v;
+        if (threadIdx.x == 0) out[((long long)k * nmom + n) * ntmax + t] = v;
     }
 }
 
-// Loop density (correlator.hpp:83-107): L(x) = (1/N) sum_noise eta(x)^dag sigma3 q(x).
-__global__ void k_contract_loop(const int Nt, const int Nx, const int Ls, const int nnoise, const int K,
-                                const double* noise_all, const double* X, double* loop_all)
+// Loop density (correlator.hpp:83-107): L(x) = (1/N) sum_noise eta(x)^dag sigma3 q(x);
+// the noise solutions start at right-hand side `first` of each configuration, the
+// output is loop[cfg][V4max] complex.
+__global__ void k_contract_loop(const VecLayout lay, const int nnoise, const int K, const int first, const int v4max,
+                                const double* noise_all, const long long* noff, const double* X, double* loop_all)
 {
-    const int V4 = Nt * Nx;
-    const long long V5 = (long long)V4 * Ls;
     const int cfg = blockIdx.y;
-    const double4* P = reinterpret_cast<const double4*>(X) + (long long)cfg * K * V5;
-    const double* noise = noise_all + (long long)cfg * nnoise * V4 * 4;
-    double* loop = loop_all + (long long)cfg * V4 * 2;
+    const CfgGeo G = cfg_geo(lay, K, cfg);
+    const int V4 = G.V4, Ls = lay.Ls;
+    const long long V5 = G.V5;
+    const double4* P = reinterpret_cast<const double4*>(X) + G.site + (long long)first * V5;
+    const double* noise = noise_all + noff[cfg];
+    double* loop = loop_all + (long long)cfg * v4max * 2;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V4; i += gridDim.x * blockDim.x) {
         double lr = 0.0, li = 0.0;
         for (int n = 0; n < nnoise; ++n) {
@@ -195,15 +225,18 @@ __global__ void k_contract_loop(const int Nt, const int Nx, const int Ls, const 
     }
 }
 
-// ||v_k||^2 per right-hand side (block per rhs, fixed order).
-__global__ void k_norms(const double* v, const long long n_per, double* out)
+// ||v_rhs||^2 per right-hand side (block per rhs, fixed order) of a batch laid out as lay.
+__global__ void k_norms(const double* v, const VecLayout lay, double* out)
 {
     __shared__ double sh[8];
-    const double* a = v + blockIdx.x * n_per;
+    const int rhs = blockIdx.x;
+    const int g = group_of_rhs(lay.gr, rhs);
+    const double* a = v + lay.offset(g, rhs) * 4;
+    const long long n_per = lay.sites(g) * 4;
     double s = 0.0;
     for (long long i = threadIdx.x; i < n_per; i += blockDim.x) s += a[i] * a[i];
     s = bsum(s, sh);
-    if (threadIdx.x == 0) out[blockIdx.x] = s;
+    if (threadIdx.x == 0) out[rhs] = s;
 }
 
 // d = a - b elementwise
@@ -237,19 +270,37 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_measure: no gauge field set");
     if (c->kind != 0) throw DwfError(DWF_ERR_ARG, "dwf_measure: needs a domain-wall context");
     if (ncfg != c->ncfg) throw DwfError(DWF_ERR_ARG, "dwf_measure: ncfg differs from the configurations held");
-    const Lat& L = c->L;
-    const int Nt = L.Nt, Nx = L.Nx, Ls = L.Ls, V4 = L.V4;
+    const int Ls = c->L.Ls;
+    // per configuration lattice (configuration-major over the lattice groups)
+    std::vector<int> cNt(ncfg), cNx(ncfg);
+    if (c->multi()) {
+        int k = 0;
+        for (const auto& d : c->gdefs)
+            for (int j = 0; j < d.ncfg; ++j, ++k) {
+                cNt[k] = d.Nt;
+                cNx[k] = d.Nx;
+            }
+    } else {
+        std::fill(cNt.begin(), cNt.end(), c->L.Nt);
+        std::fill(cNx.begin(), cNx.end(), c->L.Nx);
+    }
+    int ntmax = 0, v4max = 0, nxmin = 1 << 30;
+    for (int k = 0; k < ncfg; ++k) {
+        ntmax = std::max(ntmax, cNt[k]);
+        v4max = std::max(v4max, cNt[k] * cNx[k]);
+        nxmin = std::min(nxmin, cNx[k]);
+    }
     const int nwall = spec->wall ? 4 : 0;
     const int* hsite = spec->point ? spec->point_site : nullptr;
     const int nsrc_t = hsite ? 1 : 2;  // point source times per configuration
     const int npoint = spec->point ? 2 * nsrc_t : 0;
     const int nnoise = spec->noise_vectors;
     if (nnoise < 0) throw DwfError(DWF_ERR_ARG, "eta.noise_vectors must be positive");
-    if (spec->point && (spec->max_momentum < 0 || spec->max_momentum > Nx / 2))
+    if (spec->point && (spec->max_momentum < 0 || spec->max_momentum > nxmin / 2))
         throw DwfError(DWF_ERR_ARG, "analysis.max_momentum is outside 0..Nx/2");
     if (hsite)
         for (int k = 0; k < ncfg; ++k)
-            if (hsite[2 * k] < 0 || hsite[2 * k] >= Nt || hsite[2 * k + 1] < 0 || hsite[2 * k + 1] >= Nx)
+            if (hsite[2 * k] < 0 || hsite[2 * k] >= cNt[k] || hsite[2 * k + 1] < 0 || hsite[2 * k + 1] >= cNx[k])
                 throw DwfError(DWF_ERR_ARG, "dwf_measure: point_site outside the lattice");
     for (int k = 0; k < ncfg; ++k) {
         const dwf_measure_out& o = outs[k];
@@ -264,18 +315,29 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
     if (K == 0) return DWF_OK;
     const int nrhs = K * ncfg;
     c->bind_rhs(nrhs);
-    const long long V5 = L.V5();
-    const size_t nv = (size_t)V5 * 4;  // doubles per spinor
+    const VecLayout lay = c->layout(nrhs, 0);
+    const size_t nvt = (size_t)lay.total_sites() * 4;  // doubles of the whole batch
     hipStream_t s = c->stream;
 
-    // Z4 noise from the host stream of correlator.hpp:35-58, one engine per configuration
+    // Z4 noise from the host stream of correlator.hpp:35-58, one engine per configuration;
+    // configuration k's vectors start at noff[k] doubles
     double* dnoise = nullptr;
+    long long* dnoff = nullptr;
     if (nnoise > 0) {
-        const size_t per = (size_t)nnoise * V4 * 4;
-        std::vector<double> h(per * ncfg);
-        for (int k = 0; k < ncfg; ++k) dwf_z4_noise(seeds ? seeds[k] : spec->noise_seed, nnoise, V4 * 2, h.data() + per * k);
+        std::vector<long long> noff(ncfg);
+        long long tot = 0;
+        for (int k = 0; k < ncfg; ++k) {
+            noff[k] = tot;
+            tot += (long long)nnoise * cNt[k] * cNx[k] * 4;
+        }
+        std::vector<double> h((size_t)tot);
+        for (int k = 0; k < ncfg; ++k)
+            dwf_z4_noise(seeds ? seeds[k] : spec->noise_seed, nnoise, cNt[k] * cNx[k] * 2, h.data() + noff[k]);
         dnoise = (double*)c->buf(B_AUX3, h.size() * 8);
+        dnoff = (long long*)c->buf(B_NOFF, (size_t)ncfg * sizeof(long long));
         c->check(hipMemcpyAsync(dnoise, h.data(), h.size() * 8, hipMemcpyHostToDevice, s), "noise upload");
+        c->check(hipMemcpyAsync(dnoff, noff.data(), (size_t)ncfg * sizeof(long long), hipMemcpyHostToDevice, s),
+                 "noise upload");
         c->check(hipStreamSynchronize(s), "noise upload");
     }
     int* dsite = nullptr;
@@ -284,9 +346,9 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
         c->check(hipMemcpyAsync(dsite, hsite, (size_t)ncfg * 2 * sizeof(int), hipMemcpyHostToDevice, s), "site upload");
         c->check(hipStreamSynchronize(s), "site upload");
     }
-    double* src = (double*)c->buf(B_AUX0, nv * nrhs * 8);
-    double* X = (double*)c->buf(B_AUX2, nv * nrhs * 8);
-    k_sources<<<dim3((V4 + 255) / 256, nrhs), 256, 0, s>>>(Nt, Nx, Ls, nwall, npoint, nnoise, dnoise, dsite, src);
+    double* src = (double*)c->buf(B_AUX0, nvt * 8);
+    double* X = (double*)c->buf(B_AUX2, nvt * 8);
+    k_sources<<<dim3((v4max + 255) / 256, nrhs), 256, 0, s>>>(lay, nwall, npoint, nnoise, dnoise, dnoff, dsite, src);
     c->check_launch("sources");
     std::vector<dwf_solve_stats> stv(nrhs);
     const int rc = dwf_solve_dirac_impl(c, spec->method, src, nullptr, spec->rtol, spec->maxiter, nrhs, X, stv.data());
@@ -302,32 +364,31 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
             if (j < nwall && !st.converged) outs[k].wall_nonconverged += 1;  // "CGNR info = nonzero" (:222-223)
         }
     }
-    double* res = (double*)c->buf(B_SCAL, (size_t)std::max(64, 8 * ncfg) * 8);
     if (nwall) {
         // Dirac residual ||D x - src|| / ||src|| of the wall solves (:225-227)
-        double* dx = (double*)c->buf(B_APPLY_TMP, nv * nrhs * 8);
-        double* diff = (double*)c->buf(B_T64, nv * nrhs * 8);
+        double* dx = (double*)c->buf(B_APPLY_TMP, nvt * 8);
+        double* diff = (double*)c->buf(B_T64, nvt * 8);
         dwf_apply_device_impl(c, DWF_OP_D, DWF_PREC_FP64, X, dx, nrhs);
-        k_sub<<<1024, 256, 0, s>>>(dx, src, diff, (long long)nv * nrhs);
-        std::vector<double> hr(2 * ncfg * 4);
+        k_sub<<<1024, 256, 0, s>>>(dx, src, diff, (long long)nvt);
         double* nd = (double*)c->buf(B_AUX1, (size_t)2 * nrhs * 8);
-        k_norms<<<nrhs, 256, 0, s>>>(diff, (long long)nv, nd);
-        k_norms<<<nrhs, 256, 0, s>>>(src, (long long)nv, nd + nrhs);
-        double* pp = (double*)c->buf(B_PA, (size_t)2 * Nt * ncfg * 8);
-        double* j5 = (double*)c->buf(B_PB, (size_t)2 * Nt * ncfg * 8);
-        k_contract_wall<<<dim3(Nt, 2, ncfg), 256, 0, s>>>(Nt, Nx, Ls, K, X, pp, j5);
+        k_norms<<<nrhs, 256, 0, s>>>(diff, lay, nd);
+        k_norms<<<nrhs, 256, 0, s>>>(src, lay, nd + nrhs);
+        double* pp = (double*)c->buf(B_PA, (size_t)2 * ntmax * ncfg * 8);
+        double* j5 = (double*)c->buf(B_PB, (size_t)2 * ntmax * ncfg * 8);
+        k_contract_wall<<<dim3(ntmax, 2, ncfg), 256, 0, s>>>(lay, K, ntmax, X, pp, j5);
         c->check_launch("wall contraction");
-        std::vector<double> hpp((size_t)2 * Nt * ncfg), hj5((size_t)2 * Nt * ncfg), hn((size_t)2 * nrhs);
+        std::vector<double> hpp((size_t)2 * ntmax * ncfg), hj5((size_t)2 * ntmax * ncfg), hn((size_t)2 * nrhs);
         c->check(hipMemcpyAsync(hpp.data(), pp, hpp.size() * 8, hipMemcpyDeviceToHost, s), "readback");
         c->check(hipMemcpyAsync(hj5.data(), j5, hj5.size() * 8, hipMemcpyDeviceToHost, s), "readback");
         c->check(hipMemcpyAsync(hn.data(), nd, hn.size() * 8, hipMemcpyDeviceToHost, s), "readback");
         c->check(hipStreamSynchronize(s), "readback");
         for (int k = 0; k < ncfg; ++k) {
-            const double* a = hpp.data() + (size_t)k * 2 * Nt;
-            const double* b = hj5.data() + (size_t)k * 2 * Nt;
+            const int Nt = cNt[k];
+            const double* a = hpp.data() + (size_t)k * 2 * ntmax;
+            const double* b = hj5.data() + (size_t)k * 2 * ntmax;
             for (int dt = 0; dt < Nt; ++dt) {
-                outs[k].CPP[dt] = (a[dt] + a[Nt + (dt + Nt / 2) % Nt]) / 2.0;
-                outs[k].CJ5[dt] = (b[dt] + b[Nt + (dt + Nt / 2) % Nt]) / 2.0;
+                outs[k].CPP[dt] = (a[dt] + a[ntmax + (dt + Nt / 2) % Nt]) / 2.0;
+                outs[k].CJ5[dt] = (b[dt] + b[ntmax + (dt + Nt / 2) % Nt]) / 2.0;
             }
             if (outs[k].dirac_residual)
                 for (int j = 0; j < nwall; ++j)
@@ -336,44 +397,44 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
     }
     if (npoint) {
         const int nmom = spec->max_momentum + 1;
-        double* pm = (double*)c->buf(B_PR, (size_t)nsrc_t * nmom * Nt * ncfg * 8);
-        double* pj = (double*)c->buf(B_PA, (size_t)nsrc_t * Nt * ncfg * 8);
-        k_contract_point<<<dim3(Nt, nsrc_t, ncfg), 256, 0, s>>>(Nt, Nx, Ls, nmom, K, nsrc_t, dsite, X + nv * nwall,
-                                                                pm, pj);
+        double* pm = (double*)c->buf(B_PR, (size_t)nsrc_t * nmom * ntmax * ncfg * 8);
+        double* pj = (double*)c->buf(B_PA, (size_t)nsrc_t * ntmax * ncfg * 8);
+        k_contract_point<<<dim3(ntmax, nsrc_t, ncfg), 256, 0, s>>>(lay, nmom, K, nwall, nsrc_t, ntmax, dsite, X, pm, pj);
         c->check_launch("point contraction");
-        std::vector<double> h((size_t)nsrc_t * nmom * Nt * ncfg), hj((size_t)nsrc_t * Nt * ncfg);
+        std::vector<double> h((size_t)nsrc_t * nmom * ntmax * ncfg), hj((size_t)nsrc_t * ntmax * ncfg);
         c->check(hipMemcpyAsync(h.data(), pm, h.size() * 8, hipMemcpyDeviceToHost, s), "readback");
         c->check(hipMemcpyAsync(hj.data(), pj, hj.size() * 8, hipMemcpyDeviceToHost, s), "readback");
         c->check(hipStreamSynchronize(s), "readback");
         for (int k = 0; k < ncfg; ++k) {
-            const double* a = h.data() + (size_t)k * nsrc_t * nmom * Nt;
-            const double* b = hj.data() + (size_t)k * nsrc_t * Nt;
+            const int Nt = cNt[k];
+            const double* a = h.data() + (size_t)k * nsrc_t * nmom * ntmax;
+            const double* b = hj.data() + (size_t)k * nsrc_t * ntmax;
             // relative to the source time: folded over t0 = 0, Nt/2 (:158-174) or a single t0
             for (int dt = 0; dt < Nt; ++dt) {
                 if (!hsite) {
                     for (int n = 0; n < nmom; ++n)
                         outs[k].momentum_CPP[n * Nt + dt] =
-                            (a[n * Nt + dt] + a[(nmom + n) * Nt + (dt + Nt / 2) % Nt]) / 2.0;
-                    if (outs[k].point_CJ5) outs[k].point_CJ5[dt] = (b[dt] + b[Nt + (dt + Nt / 2) % Nt]) / 2.0;
+                            (a[n * ntmax + dt] + a[(nmom + n) * ntmax + (dt + Nt / 2) % Nt]) / 2.0;
+                    if (outs[k].point_CJ5) outs[k].point_CJ5[dt] = (b[dt] + b[ntmax + (dt + Nt / 2) % Nt]) / 2.0;
                 } else {
                     const int t = (hsite[2 * k] + dt) % Nt;
-                    for (int n = 0; n < nmom; ++n) outs[k].momentum_CPP[n * Nt + dt] = a[n * Nt + t];
+                    for (int n = 0; n < nmom; ++n) outs[k].momentum_CPP[n * Nt + dt] = a[n * ntmax + t];
                     if (outs[k].point_CJ5) outs[k].point_CJ5[dt] = b[t];
                 }
             }
         }
     }
     if (nnoise) {
-        double* lp = (double*)c->buf(B_PR, (size_t)V4 * 2 * ncfg * 8);
-        k_contract_loop<<<dim3((V4 + 255) / 256, ncfg), 256, 0, s>>>(Nt, Nx, Ls, nnoise, K, dnoise,
-                                                                    X + nv * (nwall + npoint), lp);
+        double* lp = (double*)c->buf(B_PR, (size_t)v4max * 2 * ncfg * 8);
+        k_contract_loop<<<dim3((v4max + 255) / 256, ncfg), 256, 0, s>>>(lay, nnoise, K, nwall + npoint, v4max, dnoise,
+                                                                        dnoff, X, lp);
         c->check_launch("loop contraction");
         for (int k = 0; k < ncfg; ++k)
-            c->check(hipMemcpyAsync(outs[k].loop, lp + (size_t)k * V4 * 2, (size_t)V4 * 2 * 8, hipMemcpyDeviceToHost, s),
+            c->check(hipMemcpyAsync(outs[k].loop, lp + (size_t)k * v4max * 2, (size_t)cNt[k] * cNx[k] * 2 * 8,
+                                    hipMemcpyDeviceToHost, s),
                      "readback");
         c->check(hipStreamSynchronize(s), "readback");
     }
-    (void)res;
     return DWF_OK;
 }
 

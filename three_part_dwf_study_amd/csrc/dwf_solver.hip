@@ -27,21 +27,10 @@ using namespace dwfk;
 
 namespace {
 
-template <typename R> HopArgs<R> hop_base(const dwf_ctx* c, const HopCfg& cfg)
-{
-    HopArgs<R> a{};
-    a.L = c->L;
-    a.cfg = cfg;
-    a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
-    a.link_stride = 2LL * c->L.V4;
-    a.rhs_per_cfg = c->rhs_per_cfg;
-    return a;
-}
-
 // ----------------------------------------------------------------- operators
 struct CgOperator {
-    long long sites = 0;  // spinor sites per right-hand side of x, b, r, p
-    int nblk = 0;         // block partials per right-hand side
+    VecLayout lay{};      // extent and offset of each right-hand side's x, b, r, p
+    int nblk = 0;         // block partials per right-hand side (the partial arrays' stride)
     virtual ~CgOperator() = default;
     // t = Op1 p, partials |t|^2
     virtual void first(bool f64, const void* p, void* t, double* part, const CgState* st, int slot, int skip) = 0;
@@ -64,6 +53,15 @@ void sig_ctx(const dwf_ctx* c, Sig& k)
     k.push_back(sig_ptr(c->l32));
     k.push_back((unsigned long long)c->rhs_per_cfg);
     k.push_back((unsigned long long)c->xcd_remap);
+    if (c->multi()) {  // the lattice-group layout the kernels receive
+        const Groups& G = c->gr;
+        k.push_back(0x6705ULL + (unsigned long long)G.n);
+        for (int g = 0; g < G.n; ++g)
+            k.insert(k.end(), {(unsigned long long)G.Nt[g], (unsigned long long)G.Nx[g], (unsigned long long)G.rhs0[g],
+                               (unsigned long long)G.rpc[g], (unsigned long long)G.site0[g],
+                               (unsigned long long)G.link0[g]});
+        k.push_back((unsigned long long)G.rhs0[G.n]);
+    }
 }
 
 // A = Op2 Op1 = D^dag D (first_dag = false) or D D^dag (first_dag = true).
@@ -75,13 +73,14 @@ struct FullOp : CgOperator {
     double* t64;
     FullOp(dwf_ctx* ctx, bool fd, int n, double* tmp64) : c(ctx), first_dag(fd), nrhs(n), t64(tmp64)
     {
-        cfg = hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store, c->lds);
-        sites = c->L.V5();
-        nblk = cfg.nblk();
+        cfg = ctx_hop_cfg(c, nrhs);
+        lay = c->layout(nrhs, 0);
+        nblk = hop_pstride(lay.gr, cfg);
     }
+    template <typename R> HopArgs<R> args() const { return ctx_hop_args<R>(c, cfg, nrhs); }
     template <typename R> void first_t(const R* p, R* t, double* part, const CgState* st, int slot, int skip)
     {
-        HopArgs<R> a = hop_base<R>(c, cfg);
+        HopArgs<R> a = args<R>();
         a.in = p;
         a.out = t;
         a.part = part;
@@ -94,7 +93,7 @@ struct FullOp : CgOperator {
     void second_t(const R* t, R* x, R* r, const R* p, const double* partA, double* part, const CgState* st, int slot,
                   int skip)
     {
-        HopArgs<R> a = hop_base<R>(c, cfg);
+        HopArgs<R> a = args<R>();
         a.in = t;
         a.x = x;
         a.r = r;
@@ -124,14 +123,14 @@ struct FullOp : CgOperator {
     void residual(const double* x, const double* b, double* r64, double* p64, float* r32, float* p32, double* part,
                   const CgState* st, int slot, int skip) override
     {
-        HopArgs<double> a = hop_base<double>(c, cfg);
+        HopArgs<double> a = args<double>();
         a.in = x;
         a.out = t64;
         a.st = st;
         a.slot = slot;
         a.skip = skip;
         launch_hop<double>(first_dag, EPI_STORE, a, nrhs, c->stream);
-        HopArgs<double> e = hop_base<double>(c, cfg);
+        HopArgs<double> e = args<double>();
         e.in = t64;
         e.b = b;
         e.r64 = r64;
@@ -165,27 +164,27 @@ struct SchurOp : CgOperator {
     SchurOp(dwf_ctx* ctx, int n, float* t32, double* t64, double* q) : c(ctx), nrhs(n), te32(t32), te64(t64), q64(q)
     {
         const Lat& L = c->L;
-        const int V4h = L.V4 / 2;
-        bx = (V4h + kBlock - 1) / kBlock;
+        lay = c->layout(nrhs, 1);
+        long long blocks = 0;  // blocks of one s-chunk over every right-hand side
+        for (int g = 0; g < lay.gr.n; ++g)
+            blocks += (long long)(lay.gr.rhs0[g + 1] - lay.gr.rhs0[g]) *
+                      ((lay.gr.Nt[g] * lay.gr.Nx[g] / 2 + kBlock - 1) / kBlock);
+        bx = (int)std::min<long long>(blocks / std::max(1, nrhs), 1 << 30);
         int ch = 1;
         if (c->chunks_override > 0)
             ch = c->chunks_override;
         else
-            while (ch < L.Ls && (long long)bx * ch * nrhs < 2048) ch *= 2;
+            while (ch < L.Ls && blocks * ch < 2048) ch *= 2;
         ch = std::min(ch, L.Ls);
         spt = (L.Ls + ch - 1) / ch;
         chunks = (L.Ls + spt - 1) / spt;
-        sites = (long long)V4h * L.Ls;
-        nblk = bx * chunks;
+        nblk = eo_pstride(lay.gr, chunks);
     }
     template <typename R> EoArgs<R> base(int tpar) const
     {
         EoArgs<R> a{};
         a.L = c->L;
-        a.Nxh = c->L.Nx / 2;
-        a.V4h = c->L.V4 / 2;
-        a.bx = bx;
-        a.bxa = (a.V4h + kBlockA - 1) / kBlockA;
+        if (c->multi()) a.gr = c->gr;  // else one group of L, filled in by the launchers
         a.chunks = chunks;
         a.spt = spt;
         a.remap = c->xcd_remap;
@@ -298,12 +297,14 @@ __global__ void k_mark_done(CgState* st, const CgState* pre, int nrhs)
     }
 }
 
-__global__ void k_axpy_masked(double* x, const double* d, long long n_per, const CgState* pre)
+__global__ void k_axpy_masked(double* x, const double* d, const VecLayout lay, const CgState* pre)
 {
     const int rhs = blockIdx.y;
     if (pre && (pre[rhs].flags[0] & F_DONE)) return;
-    double* xr = x + rhs * n_per;
-    const double* dr = d + rhs * n_per;
+    const int g = group_of_rhs(lay.gr, rhs);
+    const long long n_per = lay.sites(g) * 4;
+    double* xr = x + lay.offset(g, rhs) * 4;
+    const double* dr = d + lay.offset(g, rhs) * 4;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_per; i += (long long)gridDim.x * blockDim.x)
         xr[i] += dr[i];
 }
@@ -319,8 +320,7 @@ __global__ void k_sub_into(const double* a, const double* b, double* out, long l
 void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const double* x0, double rtol, int maxiter,
             int nrhs, double* x, const CgState* pre = nullptr)
 {
-    const long long n4 = op.sites * 4;
-    const size_t nv = (size_t)n4 * nrhs;
+    const size_t nv = (size_t)op.lay.total_sites() * 4;
     const size_t npart = (size_t)nrhs * op.nblk * 2;
     double* r64 = (double*)c->buf(B_R64, nv * 8);
     double* partA = (double*)c->buf(B_PA, npart * 8);
@@ -339,6 +339,12 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
         t64 = (double*)c->buf(B_TQ64, nv * 8);
     }
     hipStream_t s = c->stream;
+    if (op.lay.gr.n > 1) {
+        // a group with fewer blocks than the stride leaves partials unwritten: they must read 0
+        launch_zero<double>(partA, (long long)npart, s);
+        launch_zero<double>(partB, (long long)npart, s);
+        launch_zero<double>(partR, (long long)npart, s);
+    }
     if (x0) {
         if (x0 != x) c->check(hipMemcpyAsync(x, x0, nv * 8, hipMemcpyDeviceToDevice, s), "copy x0");
     } else {
@@ -364,18 +370,18 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
         if (mixed) {
             op.first(false, p32, t32, partA, st, slot, skip);
             op.second(false, t32, x32, r32, p32, partA, partB, st, slot, skip);
-            launch_cg_p<float>(op.sites, op.nblk, r32, p32, partA, partB, st, slot, mode, nrhs, s);
+            launch_cg_p<float>(op.lay, op.nblk, r32, p32, partA, partB, st, slot, mode, nrhs, s);
         } else {
             op.first(true, p64, t64, partA, st, slot, skip);
             op.second(true, t64, x, r64, p64, partA, partB, st, slot, skip);
-            launch_cg_p<double>(op.sites, op.nblk, r64, p64, partA, partB, st, slot, mode, nrhs, s);
+            launch_cg_p<double>(op.lay, op.nblk, r64, p64, partA, partB, st, slot, mode, nrhs, s);
         }
     };
     // Blocks of kGraphIters iterations starting on an even iteration are one graph launch.
     constexpr int kGraphIters = 8;
     hipGraphExec_t block = nullptr;
     if (c->use_graphs && maxiter >= kGraphIters) {
-        Sig key{(unsigned long long)mixed, (unsigned long long)nrhs, (unsigned long long)op.sites,
+        Sig key{(unsigned long long)mixed, (unsigned long long)nrhs, (unsigned long long)nv,
                 (unsigned long long)op.nblk, (unsigned long long)mode, sig_ptr(x),   sig_ptr(r64),  sig_ptr(p64),
                 sig_ptr(t64),            sig_ptr(r32),            sig_ptr(t32),  sig_ptr(p32), sig_ptr(x32),
                 sig_ptr(partA),          sig_ptr(partB),          sig_ptr(st)};
@@ -396,7 +402,7 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
         if (mixed) {
             // Reliable update: x64 += x32, FP64 true residual, certify or restart the cycle.
             const int slot = it & 1;
-            launch_accum_x(n4, x, x32, st, slot, nrhs, s);
+            launch_accum_x(op.lay, x, x32, st, slot, nrhs, s);
             op.residual(x, b, r64, nullptr, r32, nullptr, partR, st, slot, F_DONE);
             launch_ru_commit(op.nblk, partR, st, slot, nrhs, s);
             c->check_launch("reliable update");
@@ -466,7 +472,7 @@ int collect(dwf_ctx* c, int method, int nrhs, dwf_solve_stats* stats)
 int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0, double rtol,
                           int maxiter, int nrhs, double* x)
 {
-    const size_t nv = (size_t)c->L.V5() * 4 * nrhs;
+    const size_t nv = (size_t)c->vec_sites(nrhs) * 4;
     FullOp op(c, normal == DWF_OP_DDDAG, nrhs, (double*)c->buf(B_T64, nv * 8));
     cg_run(c, op, method == DWF_SOLVE_MIXED, b, x0, rtol, maxiter, nrhs, x);
     return DWF_OK;
@@ -548,7 +554,10 @@ int solve_normal(dwf_ctx* c, int normal, int method, const double* b, const doub
         throw DwfError(DWF_ERR_ARG, "dwf_solve: bad argument");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve: no gauge field set");
     c->bind_rhs(nrhs);
-    if (method == DWF_SOLVE_LEARNED) return solve_learned(c, normal, b, x0, rtol, maxiter, nrhs, x, stats);
+    if (method == DWF_SOLVE_LEARNED) {
+        c->require_single_lattice("DWF_SOLVE_LEARNED");
+        return solve_learned(c, normal, b, x0, rtol, maxiter, nrhs, x, stats);
+    }
     dwf_solve_device_impl(c, normal, method, b, x0, rtol, maxiter, nrhs, x);
     int rc = collect(c, method, nrhs, stats);
     if (method == DWF_SOLVE_MIXED && rc == DWF_ERR_NOT_CONVERGED) {
@@ -573,11 +582,9 @@ int solve_normal(dwf_ctx* c, int normal, int method, const double* b, const doub
 int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol, int maxiter, int nrhs, double* x,
                    dwf_solve_stats* stats)
 {
-    const Lat& L = c->L;
     hipStream_t s = c->stream;
-    const long long nfull = L.V5() * 4;           // doubles per full spinor
-    const long long nhalf = nfull / 2;
-    const size_t nvf = (size_t)nfull * nrhs, nvh = (size_t)nhalf * nrhs;
+    const VecLayout layf = c->layout(nrhs, 0);
+    const size_t nvf = (size_t)layf.total_sites() * 4, nvh = nvf / 2;  // doubles of the full / half batch
     double* B = (double*)c->buf(B_EO_B, nvf * 8);        // D^dag src (certification rhs)
     double* rsrc = (double*)c->buf(B_EO_RSRC, nvf * 8);  // current Dirac residual src - D x
     double* be = (double*)c->buf(B_EO_BE, nvh * 8);
@@ -595,10 +602,11 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
 
     FullOp full(c, false, nrhs, t64);
     double* partC = (double*)c->buf(B_PR2, (size_t)nrhs * full.nblk * 2 * 8);
+    if (layf.gr.n > 1) launch_zero<double>(partC, (long long)nrhs * full.nblk * 2, s);  // unwritten partials read 0
     SchurOp schur(c, nrhs, te32, ue, q64);
     // certification rhs B = D^dag src
     {
-        HopArgs<double> a = hop_base<double>(c, full.cfg);
+        HopArgs<double> a = full.args<double>();
         a.in = src;
         a.out = B;
         launch_hop<double>(true, EPI_STORE, a, nrhs, s);
@@ -618,7 +626,7 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
         // Dirac residual of the current iterate
         const double* r_in = src;
         if (pass > 0 || x0) {
-            HopArgs<double> a = hop_base<double>(c, full.cfg);
+            HopArgs<double> a = full.args<double>();
             a.in = x;
             a.out = t64;
             launch_hop<double>(false, EPI_STORE, a, nrhs, s);
@@ -626,7 +634,7 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
             r_in = rsrc;
         }
         // Schur right-hand side: bh = b_o + 1/2 H_oe A^-1 b_e, Bh = Dhat^dag bh
-        launch_eo_split(L, r_in, be, bo, nrhs, s);
+        launch_eo_split(layf, r_in, be, bo, nrhs, s);
         {
             EoArgs<double> a = schur.base<double>(0);
             a.b = be;
@@ -663,8 +671,8 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
             a.out = xe;
             launch_eo_a<double>(false, true, a, nrhs, s);
         }
-        launch_eo_merge(L, xe, xo, delta, nrhs, s);
-        k_axpy_masked<<<dim3(1024, nrhs), 256, 0, s>>>(x, delta, nfull, have_cert ? cert : nullptr);
+        launch_eo_merge(layf, xe, xo, delta, nrhs, s);
+        k_axpy_masked<<<dim3(1024, nrhs), 256, 0, s>>>(x, delta, layf, have_cert ? cert : nullptr);
         // certify on the reference's system (delta is free again)
         full.residual(x, B, delta, nullptr, nullptr, nullptr, partC, nullptr, 0, 0);
         launch_cg_start(full.nblk, partC, cert, rtol, c->reliable_delta, nrhs, s);
@@ -709,7 +717,10 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
         throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: bad argument");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: no gauge field set");
     c->bind_rhs(nrhs);
-    if (method == DWF_SOLVE_EO_MIXED && eo_supported(c->L)) {
+    if (method == DWF_SOLVE_LEARNED) c->require_single_lattice("DWF_SOLVE_LEARNED");
+    bool eo_ok = eo_supported(c->L);
+    for (int g = 0; eo_ok && g < c->gr.n; ++g) eo_ok = (c->gr.Nt[g] % 2 == 0) && (c->gr.Nx[g] % 2 == 0);
+    if (method == DWF_SOLVE_EO_MIXED && eo_ok) {
         const int rc = solve_dirac_eo(c, src, x0, rtol, maxiter, nrhs, x, stats);
         if (rc != DWF_ERR_NOT_CONVERGED) return rc;
         method = DWF_SOLVE_MIXED;  // fall back to the full-lattice solver from this iterate
@@ -718,10 +729,10 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
         method = DWF_SOLVE_MIXED;  // odd extents or an Ls without an even-odd kernel
     }
     // the reference's pattern: rhs = D^dag src, CG on D^dag D
-    const size_t nv = (size_t)c->L.V5() * 4 * nrhs;
+    const size_t nv = (size_t)c->vec_sites(nrhs) * 4;
     double* B = (double*)c->buf(B_EO_B, nv * 8);
     {
-        HopArgs<double> a = hop_base<double>(c, hop_cfg(c->L, nrhs, c->chunks_override, c->xcd_remap, c->prefetch, c->tile_x, c->nt_store, c->lds));
+        HopArgs<double> a = ctx_hop_args<double>(c, ctx_hop_cfg(c, nrhs), nrhs);
         a.in = src;
         a.out = B;
         launch_hop<double>(true, EPI_STORE, a, nrhs, c->stream);
@@ -770,7 +781,8 @@ int dwf_solve(dwf_ctx* c, int normal_op, int method, const double* b, const doub
 {
     return guard([&]() -> int {
         if (!c || !b || !x || nrhs < 1) throw DwfError(DWF_ERR_ARG, "dwf_solve: bad argument");
-        const size_t n = (size_t)c->L.V5() * 4 * nrhs;
+        c->bind_rhs(nrhs);
+        const size_t n = (size_t)c->vec_sites(nrhs) * 4;
         double* db = (double*)c->buf(B_HOST_IN, n * 8);
         double* dx = (double*)c->buf(B_HOST_OUT, n * 8);
         c->check(hipMemcpyAsync(db, b, n * 8, hipMemcpyHostToDevice, c->stream), "solve upload b");
@@ -796,7 +808,8 @@ int dwf_solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0,
     return guard([&]() -> int {
         if (!c || !src || !x || nrhs < 1)
             throw DwfError(DWF_ERR_ARG, "dwf_solve_dirac: bad argument");
-        const size_t n = (size_t)c->L.V5() * 4 * nrhs;
+        c->bind_rhs(nrhs);
+        const size_t n = (size_t)c->vec_sites(nrhs) * 4;
         double* db = (double*)c->buf(B_HOST_IN, n * 8);
         double* dx = (double*)c->buf(B_HOST_OUT, n * 8);
         c->check(hipMemcpyAsync(db, src, n * 8, hipMemcpyHostToDevice, c->stream), "solve upload src");
