@@ -17,15 +17,18 @@ the even-odd Schur-complement mixed solver and the full-lattice mixed solver.  T
 roofline object covers the dominant stencil, the FP32 Dirac operator, measured with HIP
 events on the library's stream over a batch of 128 right-hand sides (604 MB, larger than
 the 256 MB MALL).  The CPU baseline is the oracle's FP64 CG (the reference's algorithm and
-loop order) on one host core.
+loop order) on one host core, timed over --cpu-solves whole solves.
 
 With --workload analyzer a step is instead the batched analyzer pass: 128 seeded
 configurations, 32 each at Nx = Nt in {32, 56, 80, 96} with Ls=16, each with 2 point
 sources (one per spin) at a seeded site and 32 Z4 noise vectors -- 4,352 propagator
 solves (FP32 CG + FP64 reliable updates on the even-odd Schur complement, FP64 true
-residual <= 1e-10) and their FP64 contractions (point C_PP, C_J5, loop density).  The
-four lattice sizes run concurrently, one context and stream each; each size's 1,088
-right-hand sides are one batched solve.
+residual <= 1e-10) and their FP64 contractions (point C_PP, C_J5, loop density).  All
+configurations of all four sizes sit in one context (dwf_set_gauge_groups), so the 4,352
+right-hand sides are one batched solve and every kernel launch covers the four sizes
+(--analyzer-split: one context and stream per size instead, for comparison).  Its CPU
+baseline solves the 34 sources of one configuration per size with the oracle, spread
+over every OpenMP thread.
 
 With --workload learned: only the learned path and its comparisons (FP64 CG, even-odd
 mixed) on a held-out configuration, with the training statistics.
@@ -205,21 +208,39 @@ def parse_shape(text):
     return {"tile_x": tx, "prefetch": 0, "chunks": 1, "nt_store": 1, "lds": lds}
 
 
-def cpu_baseline(theta, b1, judge_iters, sample_iters):
+def host_info():
+    """CPU model, logical CPU count and OpenMP thread count of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(theta, bs, nsolves):
+    """The oracle's FP64 CG (the reference's algorithm and loop order, single-threaded as
+    the reference's solves are) on `nsolves` whole 96x96x16 solves to 1e-10."""
     import oracle
 
     t0 = time.perf_counter()
-    _, it, _, _, _ = oracle.dwf_cg(3, theta, b1, NT, NX, LS, M5, MF, TOL, sample_iters)
+    iters = []
+    for b in bs[:nsolves]:
+        _, it, _, cv, _ = oracle.dwf_cg(3, theta, b, NT, NX, LS, M5, MF, TOL, 100000)
+        iters.append(it)
     dt = time.perf_counter() - t0
-    per_iter = dt / max(1, it)
     return {
-        "value": 1.0 / (per_iter * judge_iters),
+        "value": len(iters) / dt,
         "unit": "solves/s",
         "cores": 1,
         "kind": "port",
-        "sample": (f"oracle FP64 CG (reference algorithm, 1 core) on one 96x96x16 source: "
-                   f"{it} iterations timed ({dt:.1f} s, {per_iter * 1e3:.1f} ms/iteration incl. the "
-                   f"initial residual), scaled by the {judge_iters:.0f} iterations the FP64 judge needs"),
+        "sample": (f"oracle FP64 CG (reference algorithm, 1 core) on {len(iters)} whole 96x96x16 boundary-source "
+                   f"solve(s) to 1e-10: {iters} iterations in {dt:.1f} s"),
+        "host": host_info(),
     }
 
 
@@ -240,50 +261,136 @@ def analyzer_inputs(per_size):
     return out
 
 
-def analyzer_cpu_baseline(inputs, sample_iters):
-    """Oracle FP64 CG (the reference's algorithm, 1 core): one point-source solve to
-    1e-10 at the smallest size gives the iteration count; `sample_iters` iterations per
-    size give the time per iteration; the whole job is scaled from those."""
+def analyzer_sources(N, site, seed, nnoise):
+    """The 2 point sources (spin 0, 1) at site = (t0, x0) and nnoise Z4 noise sources of one
+    configuration, projected as correlator.hpp:60-75, 131-138 place them; [2 + nnoise][ndof]."""
     import oracle
 
-    total, parts = 0.0, []
-    iters = None
-    for N, thetas, sites, _ in inputs:
-        src = np.zeros((LS, N, N, 2), np.complex128)
-        t0_, x0_ = int(sites[0][0]), int(sites[0][1])
-        src[LS - 1, t0_, x0_, 0] = 1.0  # P_R e_0 at s = Ls-1
-        b = oracle.dwf_apply(1, thetas[0], src.reshape(-1), N, N, LS, M5, MF)
-        if iters is None:
-            t0 = time.perf_counter()
-            _, iters, _, _, _ = oracle.dwf_cg(3, thetas[0], b, N, N, LS, M5, MF, TOL, 100000)
-            parts.append(f"N={N}: full solve {iters} iterations in {time.perf_counter() - t0:.1f} s")
-        t0 = time.perf_counter()
-        _, it, _, _, _ = oracle.dwf_cg(3, thetas[0], b, N, N, LS, M5, MF, TOL, sample_iters)
-        per_iter = (time.perf_counter() - t0) / max(1, it)
-        total += len(thetas) * 34 * iters * per_iter
-        parts.append(f"N={N}: {per_iter * 1e3:.2f} ms/iteration")
-    nsolves = sum(len(t) * 34 for _, t, _, _ in inputs)
+    out = np.zeros((2 + nnoise, LS, N, N, 2), np.complex128)
+    t0, x0 = int(site[0]), int(site[1])
+    out[0, LS - 1, t0, x0, 0] = 1.0  # P_R e_0 at s = Ls-1
+    out[1, 0, t0, x0, 1] = 1.0       # P_L e_1 at s = 0
+    eta = oracle.z4_noise(N, N, nnoise, seed)
+    out[2:, 0, :, :, 1] = eta[:, :, :, 1]
+    out[2:, LS - 1, :, :, 0] = eta[:, :, :, 0]
+    return out.reshape(2 + nnoise, -1)
+
+
+def analyzer_cpu_baseline(inputs, nrhs):
+    """The reference's analyzer on the CPU: the oracle's FP64 CGNR (the reference's
+    algorithm) on every source of one configuration per lattice size, the solves spread
+    over all OpenMP threads as domain_wall/analyze_configs.cpp:45 spreads its work.
+    Contractions (O(V4) per solve) are left out."""
+    import oracle
+
+    thetas, bs, lats = [], [], []
+    for N, th, sites, seeds in inputs:
+        src = analyzer_sources(N, sites[0], seeds[0], 32)[:nrhs]
+        for v in src:
+            thetas.append(th[0])
+            bs.append(oracle.dwf_apply(1, th[0], v, N, N, LS, M5, MF))
+            lats.append((N, N))
+    t0 = time.perf_counter()
+    _, iters, conv, threads = oracle.dwf_cg_many(thetas, bs, lats, LS, M5, MF, TOL, 100000)
+    dt = time.perf_counter() - t0
+    per = {f"N{N}": int(np.mean(iters[i * nrhs:(i + 1) * nrhs])) for i, (N, _, _, _) in enumerate(inputs)}
     return {
-        "value": nsolves / total,
+        "value": len(bs) / dt,
         "unit": "solves/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": ("oracle FP64 CG (reference algorithm, 1 core), contractions excluded; "
-                   + "; ".join(parts)
-                   + f"; every solve assumed to take the {iters} iterations of the full one"),
+        "sample": (f"oracle FP64 CGNR (reference algorithm) on {nrhs} sources (2 point + Z4) of one configuration "
+                   f"per size, {len(bs)} whole solves to 1e-10 spread over {threads} OpenMP threads in {dt:.1f} s; "
+                   f"mean iterations {per}; all converged: {bool(conv.all())}"),
+        "host": host_info(),
     }
 
 
 def run_analyzer(args, dwf):
+    per = args.analyzer_configs
+    inputs = analyzer_inputs(per)
+    method = dwf.DWF_SOLVE_EO_MIXED
+    ncfg = per * len(inputs)
+    nsolves = ncfg * 34
+    groups = [(N, N, thetas) for N, thetas, _, _ in inputs]
+    sites = np.concatenate([s for _, _, s, _ in inputs])
+    seeds = [sd for _, _, _, ss in inputs for sd in ss]
+    if args.analyzer_split:
+        step, close = analyzer_split_step(args, dwf, inputs, method)
+    else:
+        # every configuration of every size in one context: one batched solve whose
+        # kernel launches each cover all four lattice sizes
+        ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
+        if args.no_graphs:
+            ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
+
+        def step():
+            t0 = time.perf_counter()
+            ctx.set_gauge_groups(groups)
+            m = ctx.measure_batch(ncfg, TOL, 20000, wall=False, point=True, max_momentum=0, noise_vectors=32,
+                                  seeds=seeds, method=method, point_sites=sites)
+            sec = time.perf_counter() - t0
+            return [(sec, m[g * per:(g + 1) * per]) for g in range(len(inputs))]
+
+        close = ctx.close
+
+    for _ in range(args.warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step()
+    dt = time.perf_counter() - t0
+    groups_out = {}
+    for (N, _, _, _), (sec, m) in zip(inputs, out):
+        its = np.concatenate([d["iterations"] for d in m])
+        groups_out[f"N{N}"] = {"solves": per * 34, "mean_iterations": round(float(its.mean()), 1),
+                               "max_iterations": int(its.max())}
+        if args.analyzer_split:
+            groups_out[f"N{N}"]["seconds"] = round(sec, 4)
+        log(f"N={N}: {groups_out[f'N{N}']}")
+    close()
+    cpu = None if args.no_cpu else analyzer_cpu_baseline(inputs, args.cpu_rhs)
+    if cpu:
+        log(f"cpu baseline: {cpu}")
+    line = {
+        "metric": "DWF analyzer propagator solves/s (batched, 4 lattice sizes) @1e-10",
+        "value": round(nsolves * args.steps / dt, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched analyzer: {ncfg} configurations, {per} each at Nx=Nt in "
+                        f"{list(ANALYZER_SIZES)}, Ls=16, theta~N(0,1/4); per configuration 2 point sources at a "
+                        "seeded site + 32 Z4 noise vectors; point C_PP, C_J5 and loop density in FP64",
+            "solves_per_step": nsolves,
+            "solver": "even-odd Schur complement, FP32 CGNR + FP64 reliable updates, certified at 1e-10",
+            "launch": ("one context per lattice size on concurrent streams" if args.analyzer_split else
+                       "one context holding all four lattice sizes: one batched solve, every kernel launch "
+                       "covers all sizes"),
+        },
+        "groups": groups_out,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def analyzer_split_step(args, dwf, inputs, method):
+    """The same work with one context and stream per lattice size (for comparison)."""
     import threading
 
     per = args.analyzer_configs
-    inputs = analyzer_inputs(per)
     ctxs = [dwf.DwfContext.dwf(N, N, LS, M5, MF) for N, _, _, _ in inputs]
     if args.no_graphs:
         for c in ctxs:
             c.tune(dwf.DWF_TUNE_GRAPHS, 0)
-    method = dwf.DWF_SOLVE_EO_MIXED
 
     def one(g, out):
         N, thetas, sites, seeds = inputs[g]
@@ -304,49 +411,11 @@ def run_analyzer(args, dwf):
             raise SystemExit("analyzer step failed")
         return out
 
-    for _ in range(args.warmup):
-        step()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = step()
-    dt = time.perf_counter() - t0
-    nsolves = per * 34 * len(inputs)
-    groups = {}
-    for (N, _, _, _), (sec, m) in zip(inputs, out):
-        its = np.concatenate([d["iterations"] for d in m])
-        groups[f"N{N}"] = {"seconds": round(sec, 4), "solves": per * 34,
-                           "mean_iterations": round(float(its.mean()), 1), "max_iterations": int(its.max())}
-        log(f"N={N}: {groups[f'N{N}']}")
-    cpu = None if args.no_cpu else analyzer_cpu_baseline(inputs, args.cpu_iters)
-    if cpu:
-        log(f"cpu baseline: {cpu}")
-    line = {
-        "metric": "DWF analyzer propagator solves/s (batched, 4 lattice sizes) @1e-10",
-        "value": round(nsolves * args.steps / dt, 3),
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(dt / args.steps * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {
-            "workload": f"batched analyzer: {per * len(inputs)} configurations, {per} each at Nx=Nt in "
-                        f"{list(ANALYZER_SIZES)}, Ls=16, theta~N(0,1/4); per configuration 2 point sources at a "
-                        "seeded site + 32 Z4 noise vectors; point C_PP, C_J5 and loop density in FP64",
-            "solves_per_step": nsolves,
-            "solver": "even-odd Schur complement, FP32 CGNR + FP64 reliable updates, certified at 1e-10",
-        },
-        "groups": groups,
-        "roofline": None,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
-    for c in ctxs:
-        c.close()
+    def close():
+        for c in ctxs:
+            c.close()
+
+    return step, close
 
 
 def train_network(args, dwf, capi, hot=False):
@@ -601,7 +670,14 @@ def main():
                          "autotuner's D-apply shape) or 'lds,tile_x' (FP32 solvers)")
     ap.add_argument("--dslash-shape", default=None,
                     help="'lds,tile_x': time the FP32 D apply in this LDS tile shape instead of the autotuned one")
-    ap.add_argument("--cpu-iters", type=int, default=60, help="oracle CG iterations timed for the CPU baseline")
+    ap.add_argument("--cpu-iters", type=int, default=60,
+                    help="oracle CG iterations timed for the HMC workload's CPU baseline")
+    ap.add_argument("--cpu-solves", type=int, default=1,
+                    help="whole oracle solves timed for the solves workload's CPU baseline")
+    ap.add_argument("--cpu-rhs", type=int, default=34,
+                    help="analyzer CPU baseline: sources per configuration solved (34 = all)")
+    ap.add_argument("--analyzer-split", action="store_true",
+                    help="analyzer: one context per lattice size on concurrent streams instead of one launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("solves", "analyzer", "learned", "hmc"), default="solves")
     ap.add_argument("--hmc-solver", choices=("cg64", "mixed", "learned"), default="mixed")
@@ -749,8 +825,9 @@ def main():
     if not args.no_cpu:
         import oracle
 
-        b1 = oracle.dwf_apply(1, theta, src[0].reshape(-1), NT, NX, LS, M5, MF)
-        cpu = cpu_baseline(theta, b1, results["fp64_cg"]["iterations"], args.cpu_iters)
+        bs = [oracle.dwf_apply(1, theta, src[i].reshape(-1), NT, NX, LS, M5, MF)
+              for i in range(min(args.cpu_solves, nrhs))]
+        cpu = cpu_baseline(theta, bs, args.cpu_solves)
         log(f"cpu baseline: {cpu}")
 
     fast, judge, eo = results["learned"], results["fp64_cg"], results["eo_mixed"]

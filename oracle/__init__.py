@@ -24,6 +24,7 @@ _P, _I, _D, _U = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_ulongl
 _PROTOS = {
     "orc_dwf_apply": (_I, [_I, _I, _I, _I, _D, _D, _P, _P, _P]),
     "orc_dwf_cg": (_I, [_I, _I, _I, _I, _D, _D, _P, _P, _P, _D, _I, _P, _P, _P, _P]),
+    "orc_dwf_cg_many": (_I, [_I, _P, _P, _I, _D, _D, _P, _P, _D, _I, _P, _P, _P]),
     "orc_dwf_pf_action_force": (_I, [_I, _I, _I, _D, _D, _P, _P, _P, _D, _I, _P, _P, _P, _P, _P]),
     "orc_dwf_force_from_eta": (_I, [_I, _I, _I, _D, _D, _P, _P, _P]),
     "orc_gauge_action_force": (_D, [_I, _I, _D, _P, _P]),
@@ -102,6 +103,26 @@ def dwf_cg(normal, theta, b, Nt, Nx, Ls, M5, mf, rtol, maxiter, guess=None):
     rc = lib().orc_dwf_cg(normal, Nt, Nx, Ls, M5, mf, _p(th), _p(b), _p(g), rtol, maxiter, _p(x),
                           ctypes.byref(it), ctypes.byref(rr), ctypes.byref(cv))
     return x, it.value, rr.value, bool(cv.value), rc
+
+
+def dwf_cg_many(thetas, bs, lattices, Ls, M5, mf, rtol, maxiter):
+    """Independent D^dag D solves (one per right-hand side b[i] on lattices[i] = (Nt, Nx)
+    with links thetas[i]) spread over the OpenMP threads, as the reference's analyzers
+    spread configurations.  Returns (solutions, iterations, converged, threads)."""
+    n = len(bs)
+    th = [_c(t, np.float64) for t in thetas]
+    bb = [_c(b) for b in bs]
+    xs = [np.empty_like(b) for b in bb]
+    Nt = np.array([l[0] for l in lattices], np.int32)
+    Nx = np.array([l[1] for l in lattices], np.int32)
+    def ptrs(arrs):
+        return ctypes.cast((ctypes.c_void_p * n)(*[_p(a) for a in arrs]), ctypes.c_void_p)
+
+    it = np.zeros(n, np.int32)
+    cv = np.zeros(n, np.int32)
+    pt, pb, px = ptrs(th), ptrs(bb), ptrs(xs)
+    threads = lib().orc_dwf_cg_many(n, _p(Nt), _p(Nx), Ls, M5, mf, pt, pb, rtol, maxiter, px, _p(it), _p(cv))
+    return xs, it, cv.astype(bool), threads
 
 
 def dwf_pf_action_force(theta, phi, Nt, Nx, Ls, M5, mf, rtol, maxiter, guess=None):

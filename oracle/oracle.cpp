@@ -31,6 +31,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <omp.h>
 #include <random>
 #include <vector>
 
@@ -662,6 +663,30 @@ int orc_dwf_cg(int normal, int Nt, int Nx, int Ls, double M5, double mf, const d
     *relres = r.relative_residual;
     *converged = r.converged ? 1 : 0;
     return r.zero_denominator ? ORC_ZERO_DENOM : ORC_OK;
+}
+
+// --- The analyzers' parallelism (domain_wall/analyze_configs.cpp:45-76: OpenMP
+//     `parallel for schedule(dynamic)` over independent work): n independent
+//     conjugate_gradient solves on D^dag D, solve i on the Nt[i] x Nx[i] lattice with links
+//     theta[i], right-hand side b[i] and solution x[i], spread over the OpenMP threads
+//     (OMP_NUM_THREADS).  Returns the number of threads that ran.
+int orc_dwf_cg_many(int n, const int* Nt, const int* Nx, int Ls, double M5, double mf, const double* const* theta,
+                    const double* const* b, double rtol, int maxiter, double* const* x, int* iterations,
+                    int* converged)
+{
+    int threads = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+        threads = omp_get_num_threads();
+#pragma omp for schedule(dynamic)
+        for (int i = 0; i < n; ++i) {
+            double rr = 0.0;
+            orc_dwf_cg(3, Nt[i], Nx[i], Ls, M5, mf, theta[i], b[i], nullptr, rtol, maxiter, x[i], &iterations[i], &rr,
+                       &converged[i]);
+        }
+    }
+    return threads;
 }
 
 // --- pseudofermion_action_force, domain_wall/include/pseudofermion.hpp:22-166.

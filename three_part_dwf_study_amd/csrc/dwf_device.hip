@@ -969,7 +969,12 @@ __global__ void k_ru_commit(const int nblk, const double* partR, CgState* st, co
 {
     const int rhs = blockIdx.x;
     CgState& S = st[rhs];
-    if (S.flags[slot] & F_DONE) return;
+    const int f = S.flags[slot];
+    if (f & F_HOLD) {  // its FP32 cycle continues; this update skipped it
+        if (threadIdx.x == 0) S.flags[slot] = f & ~F_HOLD;
+        return;
+    }
+    if (f & F_DONE) return;
     const double* p = partR + (size_t)rhs * nblk * 2;
     double rr = 0.0;
     for (int i = threadIdx.x; i < nblk; i += 64) rr += p[2 * i];
@@ -1001,6 +1006,12 @@ __global__ void k_true_resid(const int nblk, const double* partR, CgState* st)
     }
 }
 
+__global__ void k_hold(CgState* st, const int slot, const int nrhs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrhs && !(st[i].flags[slot] & (F_DONE | F_FROZEN))) st[i].flags[slot] |= F_HOLD;
+}
+
 __global__ void k_set_frozen_all(CgState* st, const int slot, const int nrhs)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1009,10 +1020,10 @@ __global__ void k_set_frozen_all(CgState* st, const int slot, const int nrhs)
 
 // x64 += x32; x32 = 0 for the right-hand sides still running.
 __global__ __launch_bounds__(kBlock) void k_accum_x(const VecLayout lay, double* x64, float* x32,
-                                                    const CgState* st, const int slot)
+                                                    const CgState* st, const int slot, const int skip)
 {
     const int rhs = blockIdx.z;
-    if (st[rhs].flags[slot] & F_DONE) return;
+    if (st[rhs].flags[slot] & skip) return;
     const int g = group_of_rhs(lay.gr, rhs);
     const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;  // site
     if (i >= lay.sites(g)) return;
@@ -1257,11 +1268,16 @@ void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s)
     k_set_frozen_all<<<(nrhs + 63) / 64, 64, 0, s>>>(st, slot, nrhs);
 }
 
-void launch_accum_x(const VecLayout& lay, double* x64, float* x32, const CgState* st, int slot, int nrhs,
+void launch_accum_x(const VecLayout& lay, double* x64, float* x32, const CgState* st, int slot, int skip, int nrhs,
                     hipStream_t s)
 {
     const dim3 grid((unsigned)((max_sites(lay) + kBlock - 1) / kBlock), 1, nrhs);
-    k_accum_x<<<grid, kBlock, 0, s>>>(lay, x64, x32, st, slot);
+    k_accum_x<<<grid, kBlock, 0, s>>>(lay, x64, x32, st, slot, skip);
+}
+
+void launch_hold(CgState* st, int slot, int nrhs, hipStream_t s)
+{
+    k_hold<<<(nrhs + 63) / 64, 64, 0, s>>>(st, slot, nrhs);
 }
 
 template <typename R> void launch_zero(R* p, long long n, hipStream_t s)

@@ -14,7 +14,9 @@ constexpr int kBlock = 256;  // threads per block for every stencil/BLAS kernel
 // Per right-hand-side CG state.  Fields a kernel reads are never written by the same
 // launch: scalars the next iteration needs live in slot (it+1)&1 while iteration `it`
 // reads slot it&1, so blocks of one launch never race.
-enum : int { F_DONE = 1, F_FROZEN = 2 };
+// F_HOLD marks, during one reliable update, the right-hand sides whose FP32 cycle is still
+// running: that update skips them and clears the mark.
+enum : int { F_DONE = 1, F_FROZEN = 2, F_HOLD = 4 };
 enum : int { MODE_CG64 = 0, MODE_MIXED = 1 };
 
 struct CgState {
@@ -210,8 +212,10 @@ void launch_cg_p(const VecLayout& lay, int nblk, R* r, R* p, const double* partA
 void launch_cg_start(int nblk, const double* partR, CgState* st, double tol, double delta, int nrhs, hipStream_t s);
 void launch_ru_commit(int nblk, const double* partR, CgState* st, int slot, int nrhs, hipStream_t s);
 void launch_true_resid(int nblk, const double* partR, CgState* st, int nrhs, hipStream_t s);
-void launch_accum_x(const VecLayout& lay, double* x64, float* x32, const CgState* st, int slot, int nrhs,
+void launch_accum_x(const VecLayout& lay, double* x64, float* x32, const CgState* st, int slot, int skip, int nrhs,
                     hipStream_t s);
+// Marks F_HOLD on every right-hand side neither done nor frozen (all = 0), or on none.
+void launch_hold(CgState* st, int slot, int nrhs, hipStream_t s);
 template <typename R> void launch_zero(R* p, long long n, hipStream_t s);
 void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s);
 

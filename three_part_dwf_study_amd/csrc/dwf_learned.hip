@@ -2242,7 +2242,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
             read();
         }
         if (fused) k_fcg_flush_x<<<vgrid, 256, 0, s>>>(x32, p32, p32b, fs, sites);
-        launch_accum_x(c->layout(nrhs, 0), x, x32, st, 0, nrhs, s);  // x += x32 (all not done)
+        launch_accum_x(c->layout(nrhs, 0), x, x32, st, 0, F_DONE, nrhs, s);  // x += x32 (all not done)
         reliable_update(false);
         c->check_launch("FCG reliable update");
         read();

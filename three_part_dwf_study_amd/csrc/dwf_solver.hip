@@ -361,9 +361,12 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     int it = 0;
     int chunk = mixed ? 16 : 32;
     read_states(c, st, nrhs);
-    std::vector<double> last_true(nrhs);
-    std::vector<int> last_iters(nrhs, 0);
-    for (int i = 0; i < nrhs; ++i) last_true[i] = c->st_host[i].true_relres;
+    // per right-hand side: residual and iteration count at the start of its current FP32
+    // cycle, and the length of its last cycle
+    std::vector<double> c0_rel(nrhs);
+    std::vector<int> c0_it(nrhs, 0), cycle_len(nrhs, 0);
+    std::vector<char> was_frozen(nrhs, 0);
+    for (int i = 0; i < nrhs; ++i) c0_rel[i] = c->st_host[i].relres;
     const int mode = mixed ? MODE_MIXED : MODE_CG64;
     auto iteration = [&](int i) {
         const int slot = i & 1, skip = F_DONE | F_FROZEN;
@@ -400,30 +403,58 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
         for (; j < n; ++j, ++it) iteration(it);
         c->check_launch("CG iteration");
         if (mixed) {
-            // Reliable update: x64 += x32, FP64 true residual, certify or restart the cycle.
+            // Each right-hand side whose FP32 cycle ended (frozen: its residual fell by delta or
+            // met the tolerance) gets its reliable update: x64 += x32, FP64 true residual,
+            // certify or start the next cycle from it.  The others keep iterating (F_HOLD).
             const int slot = it & 1;
-            launch_accum_x(op.lay, x, x32, st, slot, nrhs, s);
-            op.residual(x, b, r64, nullptr, r32, nullptr, partR, st, slot, F_DONE);
-            launch_ru_commit(op.nblk, partR, st, slot, nrhs, s);
-            c->check_launch("reliable update");
             read_states(c, st, nrhs);
-            // Next cycle: iterations the fastest active RHS needs to shrink its residual
-            // by delta (or to reach the tolerance), from the last cycle's rate.
-            double best = 1e30;
+            int frozen = 0;
+            bool stalled = false;
+            double best = 1e30;  // iterations until the next right-hand side freezes
             for (int i = 0; i < nrhs; ++i) {
                 const CgState& S = c->st_host[i];
-                if (S.flags[slot] & F_DONE) continue;
-                const int di = S.iters - last_iters[i];
-                if (di > 0 && S.true_relres > 0.0 && last_true[i] > 0.0 && S.true_relres < last_true[i]) {
-                    const double rate = std::log(S.true_relres / last_true[i]) / di;
-                    const double need_delta = std::log(delta) / rate;
-                    const double need_tol = std::log(rtol / S.true_relres) / rate + 2.0;
-                    best = std::min(best, std::min(need_delta, need_tol));
+                const int f = S.flags[slot];
+                was_frozen[i] = 0;
+                if (f & F_DONE) continue;
+                const int di = S.iters - c0_it[i];
+                if (f & F_FROZEN) {
+                    ++frozen;
+                    was_frozen[i] = 1;
+                    cycle_len[i] = di;
+                    continue;
                 }
-                last_true[i] = S.true_relres;
-                last_iters[i] = S.iters;
+                if (di > 0 && S.relres > 0.0 && S.relres < c0_rel[i]) {
+                    const double rate = std::log(S.relres / c0_rel[i]) / di;
+                    const double target = std::max(delta * std::sqrt(S.rr_ref / S.bnorm2), S.tol);
+                    best = std::min(best, std::log(target / S.relres) / rate + 1.0);
+                } else if (di >= 64) {
+                    stalled = true;  // no progress in a whole chunk: refresh it from the FP64 residual
+                }
             }
-            if (best < 1e29) chunk = std::max(8, std::min(1024, (int)std::ceil(best)));
+            if (frozen > 0 || stalled) {
+                if (!stalled) launch_hold(st, slot, nrhs, s);
+                const int skip = F_DONE | F_HOLD;
+                launch_accum_x(op.lay, x, x32, st, slot, skip, nrhs, s);
+                op.residual(x, b, r64, nullptr, r32, nullptr, partR, st, slot, skip);
+                launch_ru_commit(op.nblk, partR, st, slot, nrhs, s);
+                c->check_launch("reliable update");
+                read_states(c, st, nrhs);
+                for (int i = 0; i < nrhs; ++i) {
+                    const CgState& S = c->st_host[i];
+                    if ((S.flags[slot] & F_DONE) || !(was_frozen[i] || stalled)) continue;
+                    // a new cycle from the FP64 residual, about as long as the last one (or
+                    // shorter if the tolerance is near)
+                    double need = cycle_len[i] > 0 ? cycle_len[i] : 16;
+                    if (cycle_len[i] > 0 && c0_rel[i] > 0.0 && S.true_relres > 0.0 && S.true_relres < c0_rel[i]) {
+                        const double rate = std::log(S.true_relres / c0_rel[i]) / cycle_len[i];
+                        need = std::min(need, std::log(S.tol / S.true_relres) / rate + 1.0);
+                    }
+                    best = std::min(best, need);
+                    c0_rel[i] = S.true_relres;
+                    c0_it[i] = S.iters;
+                }
+            }
+            chunk = best < 1e29 ? std::max(8, std::min(1024, (int)std::ceil(best))) : 16;
         } else {
             read_states(c, st, nrhs);
             chunk = 32;
