@@ -153,6 +153,22 @@ typedef struct dwf_tune_result {
 } dwf_tune_result;
 int dwf_autotune(dwf_ctx* ctx, int nrhs, int precision, const char* cache_path, dwf_tune_result* result);
 
+/* Right-hand-side blocking (no counterpart in the reference, whose analyzer solves one
+ * source at a time per OpenMP thread, domain_wall/analyze_configs.cpp:45-76): the batch
+ * size of dwf_solve_dirac* with `method` that gives the most solves per second on this
+ * context's lattice and (single) gauge field.  It times whole solves to rtol of nrhs
+ * boundary Z4-noise sources (the analyzer's source shape) for nrhs = 8, 16, 32, ... up to
+ * max_nrhs and what half the free device memory holds, stopping once a doubling gains
+ * less than 5%, and reports the best nrhs and its solves/s.  Nothing is applied to the
+ * context.  cache_path (nullable) works as for dwf_autotune. */
+typedef struct dwf_batch_tune_result {
+    int nrhs;
+    int from_cache;
+    double solves_per_s;
+} dwf_batch_tune_result;
+int dwf_autotune_batch(dwf_ctx* ctx, int method, double rtol, int max_nrhs, const char* cache_path,
+                       dwf_batch_tune_result* result);
+
 /* Upload gauge angles theta[2][Nt][Nx] (host).  The library keeps them on the device in
  * FP64 and as cos/sin link arrays in FP64 and FP32; U_mu(x) = exp(i theta_mu(x))
  * (dwf_operator.hpp:10-13). */
@@ -313,7 +329,8 @@ int dwf_hmc_trajectory(dwf_ctx* ctx, const dwf_hmc_params* params, const double*
 /* Which of the DWF analyzer's per-configuration measurements to run.  All requested
  * sources go through one batched CG on D^dagger D (rhs D^dagger src). */
 typedef struct dwf_measure_spec {
-    int method;                    /* DWF_SOLVE_CG64 (reference) or DWF_SOLVE_MIXED       */
+    int method;                    /* DWF_SOLVE_CG64 (reference), _MIXED, _EO_MIXED, or
+                                      _LEARNED (a network attached, one lattice size)     */
     int maxiter;                   /* dwf.propagator_maxiter                             */
     double rtol;                   /* dwf.propagator_rtol                                */
     int wall;                      /* 1: compute_CPP_CJ5, 4 wall-source solves          */

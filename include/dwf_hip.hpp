@@ -82,6 +82,9 @@ public:
         last_ = theta;
         have_ = true;
     }
+    // The learned preconditioner DWF_SOLVE_LEARNED uses (nullptr detaches); the network
+    // must outlive its use here.
+    void use_network(const class Network* net);
 
 private:
     using Key = std::tuple<int, int, int, int, double, double, double, double>;
@@ -111,6 +114,46 @@ private:
         return *it->second;
     }
 };
+
+// The learned preconditioner's network (no counterpart in the reference): weights are
+// dwf_net_num_weights(Ls) floats, e.g. written by train_domain_wall_network.
+class Network {
+public:
+    explicit Network(int Ls, unsigned long long seed = 0)
+    {
+        dwf_net* n = nullptr;
+        check(dwf_net_create(&n, Ls, seed));
+        net_.reset(n);
+    }
+    static size_t size(int Ls) { return (size_t)dwf_net_num_weights(Ls); }
+    void set_weights(const std::vector<float>& w) { check(dwf_net_set_weights(net_.get(), w.data())); }
+    std::vector<float> weights(int Ls) const
+    {
+        std::vector<float> w(size(Ls));
+        check(dwf_net_get_weights(net_.get(), w.data()));
+        return w;
+    }
+    dwf_net* get() const { return net_.get(); }
+
+private:
+    struct Del {
+        void operator()(dwf_net* n) const { dwf_net_destroy(n); }
+    };
+    std::unique_ptr<dwf_net, Del> net_;
+};
+
+inline void Context::use_network(const Network* net) { check(dwf_ctx_set_preconditioner(get(), net ? net->get() : nullptr)); }
+
+// Right-hand sides per batched solve with the most solves/s for this lattice and method
+// (dwf_autotune_batch, on the given configuration).
+inline int tuned_batch(Context& c, const Gauge& theta, int method, double rtol, int max_nrhs,
+                       const char* cache_path = nullptr)
+{
+    c.use_gauge(theta);
+    dwf_batch_tune_result r{};
+    check(dwf_autotune_batch(c.get(), method, rtol, max_nrhs, cache_path, &r));
+    return r.nrhs;
+}
 
 namespace detail {
 template <class V> V make(size_t n) { return V(static_cast<decltype(std::declval<V>().size())>(n)); }

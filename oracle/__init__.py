@@ -24,6 +24,7 @@ _P, _I, _D, _U = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_ulongl
 _PROTOS = {
     "orc_dwf_apply": (_I, [_I, _I, _I, _I, _D, _D, _P, _P, _P]),
     "orc_dwf_cg": (_I, [_I, _I, _I, _I, _D, _D, _P, _P, _P, _D, _I, _P, _P, _P, _P]),
+    "orc_dwf_leapfrog": (_I, [_I, _I, _I, _D, _D, _D, _D, _I, _P, _P, _P, _D, _I, _P, _P]),
     "orc_dwf_cg_many": (_I, [_I, _P, _P, _I, _D, _D, _P, _P, _D, _I, _P, _P, _P]),
     "orc_dwf_pf_action_force": (_I, [_I, _I, _I, _D, _D, _P, _P, _P, _D, _I, _P, _P, _P, _P, _P]),
     "orc_dwf_force_from_eta": (_I, [_I, _I, _I, _D, _D, _P, _P, _P]),
@@ -167,6 +168,19 @@ def dwf_hmc(theta, Nt, Nx, Ls, M5, mf, beta, epsilon, steps, ntraj, seed, pf_rto
     if rc:
         raise OracleError(rc, "orc_dwf_hmc")
     return dict(theta=th, accepted=acc, acc_prob=pacc, delta_h=dh, plaquette=plaq, cg_iterations=iters.value)
+
+
+def dwf_leapfrog(theta, momentum, phi, Nt, Nx, Ls, M5, mf, beta, epsilon, steps, pf_rtol, pf_maxiter):
+    """One leapfrog trajectory of the DWF HMC; returns (theta, momentum, H_initial, H_final)."""
+    th = _c(theta, np.float64).copy()
+    p = _c(momentum, np.float64).copy()
+    ph = _c(phi)
+    h0, h1 = ctypes.c_double(), ctypes.c_double()
+    rc = lib().orc_dwf_leapfrog(Nt, Nx, Ls, M5, mf, beta, epsilon, steps, _p(th), _p(p), _p(ph), pf_rtol, pf_maxiter,
+                                ctypes.byref(h0), ctypes.byref(h1))
+    if rc:
+        raise OracleError(rc, "orc_dwf_leapfrog")
+    return th, p, h0.value, h1.value
 
 
 def dwf_cpp_cj5(theta, Nt, Nx, Ls, M5, mf, rtol, maxiter):

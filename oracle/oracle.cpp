@@ -607,6 +607,36 @@ void wilson_force_from_eta(const Wil& w, const double* th, const vec& eta, doubl
         }
 }
 
+// The leapfrog of domain_wall/generate_configs.cpp:103-169: a half momentum step with the
+// force of the starting point (`force`, gauge + pseudofermion), then `steps` times a full
+// angle step (no wrap, :113-114) and the new force (pseudofermion solve with the previous
+// eta as its guess), with full momentum steps between them and a half step at the end.
+// th, momentum, force, eta_guess, gauge_action and pf_action hold the end point's values;
+// false if a solve failed to converge.
+bool dwf_leapfrog(const Dwf& g, double beta, double epsilon, int steps, std::vector<double>& th,
+                  std::vector<double>& momentum, std::vector<double>& force, const vec& phi, vec& eta_guess,
+                  double pf_rtol, int pf_maxiter, double& gauge_action, double& pf_action, int& iters)
+{
+    const int Ngauge = g.ngauge();
+    std::vector<double> gforce(Ngauge);
+    for (int i = 0; i < Ngauge; ++i) momentum[i] -= 0.5 * epsilon * force[i];
+    for (int step = 0; step < steps; ++step) {
+        for (int i = 0; i < Ngauge; ++i) th[i] += epsilon * momentum[i];
+        const double ga = gauge_action_force(g.Nt, g.Nx, beta, th.data(), gforce.data());
+        PfResult pf = dwf_pf_action_force(g, th.data(), phi, &eta_guess, pf_rtol, pf_maxiter);
+        if (!pf.converged) return false;
+        iters += pf.iterations;
+        eta_guess = pf.eta;
+        gauge_action = ga;
+        pf_action = pf.action;
+        for (int i = 0; i < Ngauge; ++i) force[i] = gforce[i] + pf.force[i];
+        if (step != steps - 1)
+            for (int i = 0; i < Ngauge; ++i) momentum[i] -= epsilon * force[i];
+    }
+    for (int i = 0; i < Ngauge; ++i) momentum[i] -= 0.5 * epsilon * force[i];
+    return true;
+}
+
 // wilson/include/indexing.hpp:13-18
 double wrap_angle(double a)
 {
@@ -731,6 +761,39 @@ double orc_average_plaquette(int Nt, int Nx, const double* theta)
 // --- DWF HMC, domain_wall/generate_configs.cpp:23-207.  theta (in/out) holds the
 //     start configuration and receives the final theta0.  Per trajectory outputs may be
 //     NULL.  Returns ORC_NOT_CONVERGED if a force solve fails (the reference throws).
+// --- The leapfrog of one trajectory from (theta, momentum) with pseudofermion phi
+//     (domain_wall/generate_configs.cpp:103-169, as dwf_leapfrog): a first solve without a
+//     guess gives H at the start, the integration updates theta and momentum in place, and
+//     H at the end is the kinetic term plus the actions at the last step.
+int orc_dwf_leapfrog(int Nt, int Nx, int Ls, double M5, double mf, double beta, double epsilon, int steps,
+                     double* theta, double* momentum, const double* phi, double pf_rtol, int pf_maxiter,
+                     double* h_initial, double* h_final)
+{
+    const Dwf g{Nt, Nx, Ls, M5, mf};
+    const int Ngauge = g.ngauge();
+    const vec ph = to_vec(phi, g.ndof());
+    std::vector<double> th(theta, theta + Ngauge), mom(momentum, momentum + Ngauge), gforce(Ngauge), force(Ngauge);
+    double gauge_action = gauge_action_force(Nt, Nx, beta, th.data(), gforce.data());
+    PfResult pf0 = dwf_pf_action_force(g, th.data(), ph, nullptr, pf_rtol, pf_maxiter);
+    if (!pf0.converged) return ORC_NOT_CONVERGED;
+    double kin = 0.0;
+    for (double p : mom) kin += 0.5 * p * p;
+    *h_initial = kin + gauge_action + pf0.action;
+    for (int i = 0; i < Ngauge; ++i) force[i] = gforce[i] + pf0.force[i];
+    vec eta_guess = pf0.eta;
+    double pf_action = pf0.action;
+    int iters = 0;
+    if (!dwf_leapfrog(g, beta, epsilon, steps, th, mom, force, ph, eta_guess, pf_rtol, pf_maxiter, gauge_action,
+                      pf_action, iters))
+        return ORC_NOT_CONVERGED;
+    kin = 0.0;
+    for (double p : mom) kin += 0.5 * p * p;
+    *h_final = kin + gauge_action + pf_action;
+    std::copy(th.begin(), th.end(), theta);
+    std::copy(mom.begin(), mom.end(), momentum);
+    return ORC_OK;
+}
+
 int orc_dwf_hmc(int Nt, int Nx, int Ls, double M5, double mf, double beta, double epsilon,
                 int leapfrog_steps, int trajectories, unsigned long long seed, double pf_rtol,
                 int pf_maxiter, double* theta, int* accepted, double* acc_prob, double* delta_h,
@@ -761,22 +824,10 @@ int orc_dwf_hmc(int Nt, int Nx, int Ls, double M5, double mf, double beta, doubl
         const double H0 = kin0 + g_action0 + pf0.action;
         std::vector<double> th = theta0, force(Ngauge);
         for (int i = 0; i < Ngauge; ++i) force[i] = gforce[i] + pf0.force[i];
-        for (int i = 0; i < Ngauge; ++i) momentum[i] -= 0.5 * epsilon * force[i];
         double gauge_action = g_action0, pf_action = pf0.action;
-        for (int step = 0; step < leapfrog_steps; ++step) {
-            for (int i = 0; i < Ngauge; ++i) th[i] += epsilon * momentum[i];
-            const double ga = gauge_action_force(Nt, Nx, beta, th.data(), gforce.data());
-            PfResult pf = dwf_pf_action_force(g, th.data(), phi, &eta_guess, pf_rtol, pf_maxiter);
-            if (!pf.converged) return ORC_NOT_CONVERGED;
-            total_iters += pf.iterations;
-            eta_guess = pf.eta;
-            gauge_action = ga;
-            pf_action = pf.action;
-            for (int i = 0; i < Ngauge; ++i) force[i] = gforce[i] + pf.force[i];
-            if (step != leapfrog_steps - 1)
-                for (int i = 0; i < Ngauge; ++i) momentum[i] -= epsilon * force[i];
-        }
-        for (int i = 0; i < Ngauge; ++i) momentum[i] -= 0.5 * epsilon * force[i];
+        if (!dwf_leapfrog(g, beta, epsilon, leapfrog_steps, th, momentum, force, phi, eta_guess, pf_rtol, pf_maxiter,
+                          gauge_action, pf_action, total_iters))
+            return ORC_NOT_CONVERGED;
         double kin = 0.0;
         for (double p : momentum) kin += 0.5 * p * p;
         const double H = kin + gauge_action + pf_action;

@@ -77,3 +77,22 @@ def test_eo_falls_back_on_odd_extent_and_guess():
         x2, st2, rc = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, x0=g)
         assert rc == 0 and normal_relres(th, x2, src, 8, 6, 4) <= 1e-10
         assert st2[0]["iterations"] < st[0]["iterations"]
+
+
+def test_autotune_batch(tmp_path):
+    """Right-hand-side blocking: the tuner times whole solves over doubling batch sizes and
+    reports the best; a second call with the same cache file reuses the result."""
+    Nt, Nx, Ls = 16, 16, 8
+    print_case(test="autotune_batch")
+    rng = np.random.default_rng(12)
+    cache = str(tmp_path / "tune.txt")
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(rand_theta(rng, Nt, Nx))
+        r = ctx.autotune_batch(dwf.DWF_SOLVE_EO_MIXED, 1e-10, 64, cache)
+        assert r["nrhs"] in (8, 16, 32, 64) and r["solves_per_s"] > 0 and r["from_cache"] == 0
+        r2 = ctx.autotune_batch(dwf.DWF_SOLVE_EO_MIXED, 1e-10, 64, cache)
+        assert r2["from_cache"] == 1 and r2["nrhs"] == r["nrhs"]
+        # the context is left as it was: a single-configuration solve still works
+        src = rand_spinor(rng, Ls * Nt * Nx * 2)
+        _, st, _ = ctx.solve_dirac(src, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_EO_MIXED)
+        assert st[0]["converged"]

@@ -107,14 +107,20 @@ def test_dwf_hmc_mixed_solver_matches_fp64():
     assert np.array_equal(a["accepted"], b["accepted"])
 
 
-def test_wilson_hmc_same_seed_history():
+@pytest.mark.parametrize("start", ["cold", "hot"])
+def test_wilson_hmc_same_seed_history(start):
+    """cfg1: 16x16 Wilson HMC, beta = 2, m0 = 0.1, tau = 1 in 10 steps.  The reference starts
+    cold (theta = 0, wilson/generate_configs.cpp:20); the benchmark's hot start draws theta
+    uniform on [-pi, pi) with seed 1234."""
     Nt, Nx, m0, beta, tau, steps, ntraj, seed, tol = 16, 16, 0.1, 2.0, 1.0, 10, 5, 1234, 1e-12
-    print_case(test="wilson_hmc_history")
-    o = oracle.wilson_hmc(np.zeros(2 * Nt * Nx), Nt, Nx, m0, -1.0, 1.0, beta, tau, steps, ntraj, seed, tol, 5000)
+    print_case(test="wilson_hmc_history", start=start)
+    th0 = (np.zeros(2 * Nt * Nx) if start == "cold"
+           else np.random.default_rng(1234).uniform(-np.pi, np.pi, size=2 * Nt * Nx))
+    o = oracle.wilson_hmc(th0, Nt, Nx, m0, -1.0, 1.0, beta, tau, steps, ntraj, seed, tol, 5000)
     r = capi.HostRng(seed)
     dh, acc, plaq = [], [], []
     with dwf.DwfContext.wilson(Nt, Nx, m0, -1.0, 1.0) as ctx:
-        ctx.set_gauge(np.zeros(2 * Nt * Nx))
+        ctx.set_gauge(th0)
         for _ in range(ntraj):
             mom, chi = r.wilson_heatbath(2 * Nt * Nx, Nt * Nx * 2)
             res = ctx.hmc_trajectory(chi, mom, r.uniform(), beta, tau / steps, steps, tol, 5000,

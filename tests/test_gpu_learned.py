@@ -422,3 +422,50 @@ def test_learned_dddag_solve_force_and_hmc():
                 dh.append(res["delta_h"])
             hist.append(np.array(dh))
     assert np.max(np.abs(hist[0] - hist[1])) <= 1e-8
+
+
+def test_learned_solve_full_size():
+    """cfg4 at its real size: 96x96, Ls=16.  The network is trained here on the device
+    (8 configurations, seeds 1234 + 104729 i for i = 1..8, 8 full-volume Gaussian sources
+    each, loss ||D^dag D M b - b||^2 / ||b||^2) and solves 4 boundary sources on the
+    held-out configuration (seed 1234) with the bench's settings (LDS stencil, fused
+    network, Fletcher-Reeves beta, reliable updates after a 1e-4 drop).  Every solve must
+    be certified at 1e-10 against the oracle's operator, need no FP64 fallback, and take
+    at most half of plain FP64 CG's iterations."""
+    Nt = Nx = 96
+    Ls, ntrain, per, rtol = 16, 8, 8, 1e-10
+    print_case(test="learned_full_size", Nt=Nt, Nx=Nx, Ls=Ls)
+    ths = np.stack([np.random.default_rng(1234 + 104729 * i).normal(0.0, 0.5, size=2 * Nt * Nx)
+                    for i in range(ntrain + 1)])
+    rng = np.random.default_rng(918273)
+    net = dwf.Network(Ls, seed=11)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge_batch(ths[1:])
+        src = volume_sources(rng, ntrain * per, Nt, Nx, Ls)
+        b = ctx.apply(dwf.DWF_OP_DDAG, src)  # right-hand side k uses configuration k // per
+        del src
+        hist = net.train(ctx, b, None, pair_cfg=np.repeat(np.arange(ntrain), per), steps=600, batch=8, loss=1,
+                         lr=3e-3, seed=5)
+        del b
+        assert hist[-20:].mean() < 0.1 * hist[:5].mean(), hist[[0, 100, 300, 599]]
+        ctx.set_gauge(ths[0])
+        ctx.set_preconditioner(net)
+        ctx.tune(dwf.DWF_TUNE_LDS, 2)  # 512-thread tiles of 16 x 32 sites
+        ctx.tune(dwf.DWF_TUNE_TILE_X, 16)
+        ctx.tune(dwf.DWF_TUNE_FCG_BETA, 1)
+        ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA, 1e-4)
+        s2 = boundary_sources(rng, 4, Nt, Nx, Ls)
+        xl, sl, _ = ctx.solve_dirac(s2, rtol=rtol, maxiter=20000, method=dwf.DWF_SOLVE_LEARNED)
+        _, sc, _ = ctx.solve_dirac(s2, rtol=rtol, maxiter=20000, method=dwf.DWF_SOLVE_CG64)
+    il = np.array([s["iterations"] for s in sl])
+    ic = np.array([s["iterations"] for s in sc])
+    print(f"96x96x16 held-out configuration: learned {il.tolist()} vs FP64 CG {ic.tolist()}", flush=True)
+    assert all(s["fp64_iterations"] == 0 for s in sl)  # no fallback to plain CG
+    assert np.all(il <= 0.5 * ic)
+    th = ths[0]
+    for r in range(len(s2)):
+        assert sl[r]["converged"]
+        Dx = oracle.dwf_apply(0, th, xl[r], Nt, Nx, Ls, M5, MF)
+        lhs = oracle.dwf_apply(1, th, s2[r] - Dx, Nt, Nx, Ls, M5, MF)
+        rhs = oracle.dwf_apply(1, th, s2[r], Nt, Nx, Ls, M5, MF)
+        assert np.linalg.norm(lhs) / np.linalg.norm(rhs) <= rtol * 1.01, r

@@ -221,6 +221,53 @@ def test_hmc_runs_and_records(rng):
     assert np.array_equal(r["delta_h"], r2["delta_h"])
 
 
+def test_hmc_leapfrog_reversible(rng):
+    """The integrator is reversible: integrate, flip the momenta, integrate again -- the
+    angles return and the momenta come back flipped, with H back to its start (tight
+    solves, so the chronological guesses cost nothing measurable)."""
+    Nt, Nx, Ls, M5, mf, beta, eps, steps, rtol = 8, 6, 4, 1.0, 0.1, 3.0, 0.05, 10, 1e-14
+    th0 = rng.normal(0.0, 0.5, size=2 * Nt * Nx)
+    p0 = rng.normal(size=2 * Nt * Nx)
+    chi = (rng.normal(size=Ls * Nt * Nx * 2) + 1j * rng.normal(size=Ls * Nt * Nx * 2)) / np.sqrt(2)
+    phi = oracle.dwf_apply(0, th0, chi, Nt, Nx, Ls, M5, mf)
+    th1, p1, h0, h1 = oracle.dwf_leapfrog(th0, p0, phi, Nt, Nx, Ls, M5, mf, beta, eps, steps, rtol, 20000)
+    assert np.max(np.abs(th1 - th0)) > 0.05  # it moved
+    th2, p2, h1b, h2 = oracle.dwf_leapfrog(th1, -p1, phi, Nt, Nx, Ls, M5, mf, beta, eps, steps, rtol, 20000)
+    assert abs(h1b - h1) <= 1e-10 * abs(h1)
+    assert np.max(np.abs(th2 - th0)) <= 1e-11
+    assert np.max(np.abs(p2 + p0)) <= 1e-11
+    assert abs(h2 - h0) <= 1e-10 * abs(h0)
+
+
+def test_hmc_trajectory_is_the_leapfrog():
+    """orc_dwf_hmc's Delta H is the leapfrog's H_final - H_initial from the same draws
+    (chi, then the momenta: generate_configs.cpp:56-69) and the cold start theta = 1."""
+    Nt, Nx, Ls, M5, mf, beta, eps, steps, seed, rtol = 6, 6, 4, 1.0, 0.2, 3.0, 0.05, 5, 77, 1e-12
+    th0 = np.ones(2 * Nt * Nx)
+    r = oracle.dwf_hmc(th0, Nt, Nx, Ls, M5, mf, beta, eps, steps, 1, seed, rtol, 5000)
+    chi, mom, u = oracle.dwf_heatbath_draws(Ls * Nt * Nx * 2, 2 * Nt * Nx, seed)
+    phi = oracle.dwf_apply(0, th0, chi, Nt, Nx, Ls, M5, mf)
+    th1, _, h0, h1 = oracle.dwf_leapfrog(th0, mom, phi, Nt, Nx, Ls, M5, mf, beta, eps, steps, rtol, 5000)
+    assert abs((h1 - h0) - r["delta_h"][0]) <= 1e-12 * abs(h0)
+    assert bool(r["accepted"][0]) == (u < min(1.0, np.exp(-(h1 - h0))))
+    if r["accepted"][0]:
+        assert np.array_equal(r["theta"], th1)
+
+
+def test_cg_many_equals_single_solves(rng):
+    """The OpenMP batch of independent solves (the CPU baseline of the batched analyzer)
+    gives each solve exactly what a single call gives, on mixed lattice sizes."""
+    Ls, M5, mf = 4, 1.0, 0.05
+    lats = [(6, 6), (8, 6), (6, 6), (10, 8)]
+    ths = [rng.normal(0.0, 0.5, size=2 * Nt * Nx) for Nt, Nx in lats]
+    bs = [rng.normal(size=Ls * Nt * Nx * 2) + 1j * rng.normal(size=Ls * Nt * Nx * 2) for Nt, Nx in lats]
+    xs, its, cv, threads = oracle.dwf_cg_many(ths, bs, lats, Ls, M5, mf, 1e-10, 5000)
+    assert threads >= 1 and cv.all()
+    for i, (Nt, Nx) in enumerate(lats):
+        x1, it1, _, _, _ = oracle.dwf_cg(3, ths[i], bs[i], Nt, Nx, Ls, M5, mf, 1e-10, 5000)
+        assert np.array_equal(xs[i], x1) and its[i] == it1
+
+
 # ------------------------------------------------------------------- contractions
 def _wall_or_point_source(Nt, Nx, Ls, t_src, ss, xs):
     src = np.zeros((Ls, Nt, Nx, 2), complex)

@@ -1,5 +1,6 @@
 // analyze_domain_wall -- drop-in for the reference's domain_wall/analyze_configs.cpp.
-// Configurations are measured in groups (gpu.analysis_batch, automatic when absent): one
+// Configurations are measured in groups (gpu.analysis_batch configurations, or as many as
+// the batch size dwf_autotune_batch picks for this lattice and solver): one
 // device call solves all 8 + eta.noise_vectors propagators of every configuration in the
 // group as one batch and contracts C_PP, C_J5, the momentum-projected point correlator
 // and the Z4 loop density in FP64 (dwf_measure_batch).  The ensemble contractions -- averages, m_res,
@@ -24,6 +25,7 @@ int main()
     try {
         const app::Parameters prm;
         const app::DwfParams P(prm);
+        const std::string weights = P.weights_for(P.propagator_method, "gpu.propagator_solver");  // checked before any device work
         const int Nt = P.Nt, Nx = P.Nx;
         const double pi = 3.141592653589793238462643383279502884;
         const app::NpyArray arr = app::load_npy("output/domain_wall/analysis_configs.npy");
@@ -36,12 +38,27 @@ int main()
             throw std::runtime_error("analysis.max_momentum is outside 0..Nx/2");
 
         dwfhip::Context& ctx = dwfhip::Context::dwf(Nt, Nx, P.N5, P.M5, P.mf);
+        std::unique_ptr<dwfhip::Network> net;
+        if (P.propagator_method == DWF_SOLVE_LEARNED) {
+            net.reset(new dwfhip::Network(P.N5));
+            net->set_weights(app::load_network_weights(weights,
+                                                       dwfhip::Network::size(P.N5)));
+            ctx.use_network(net.get());
+        }
         std::vector<Corr> CPP_all(ncfg), CJ5_all(ncfg);
         std::vector<std::vector<Complex>> loops_all(ncfg);
         std::vector<std::vector<Corr>> mom_all(ncfg);
-        // about 512 right-hand sides per batched solve unless gpu.analysis_batch says otherwise
+        // configurations per batched solve: gpu.analysis_batch, else the right-hand-side
+        // batch with the most solves/s on this lattice (autotuned on the first configuration)
         const int per_cfg = 8 + P.eta_noise_vectors;
-        const int group = P.analysis_batch > 0 ? P.analysis_batch : std::max(1, 512 / per_cfg);
+        int group = P.analysis_batch;
+        if (group <= 0 && ncfg > 0) {
+            const dwfhip::Gauge first(arr.data.begin(), arr.data.begin() + ngauge);
+            const int rhs = dwfhip::tuned_batch(ctx, first, P.propagator_method, P.propagator_rtol, per_cfg * ncfg,
+                                                P.tune_cache.empty() ? nullptr : P.tune_cache.c_str());
+            group = std::max(1, rhs / per_cfg);
+            std::cout << "right-hand sides per batched solve: " << rhs << " (" << group << " configurations)\n";
+        }
         for (int c0 = 0; c0 < ncfg; c0 += group) {
             const int n = std::min(group, ncfg - c0);
             std::vector<unsigned long long> seeds(n);
@@ -161,6 +178,7 @@ int main()
                   << "  output/domain_wall/correlator.csv\n"
                   << "  output/domain_wall/channels_by_config.csv\n"
                   << "  output/domain_wall/momentum_correlators.csv\n";
+        ctx.use_network(nullptr);
         return 0;
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << '\n';

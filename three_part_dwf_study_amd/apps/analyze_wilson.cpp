@@ -1,13 +1,15 @@
-// analyze_wilson -- drop-in for the propagator part of the reference's
-// wilson/analyze_configs.cpp: for every configuration of the WILCFG1 ensemble, the
-// point-source pseudoscalar correlator C(t) = sum_x,a,b |G(t,x;a,b)|^2 from two CGNR
-// solves (wilson/include/correlator.hpp:10-40) on the GPU, written to
-// output/wilson/correlators_by_config.csv in the reference's layout.  The jackknife
-// effective-mass and plateau statistics (wilson/include/statistics.hpp) are host
-// post-processing of that file and are not part of this program.
+// analyze_wilson -- drop-in for the reference's wilson/analyze_configs.cpp: for every
+// configuration of the WILCFG1 ensemble, the point-source pseudoscalar correlator
+// C(t) = sum_x,a,b |G(t,x;a,b)|^2 from two CGNR solves (wilson/include/correlator.hpp:10-40)
+// on the GPU, written to output/wilson/correlators_by_config.csv in the reference's layout;
+// then on the host the jackknife cosh effective mass, the flat-plateau search and the
+// printed "Wilson pseudoscalar mass = m +/- dm" (analyze_configs.cpp:36-79,
+// wilson/include/statistics.hpp), with the plotted series written to
+// output/wilson/effective_mass.csv in place of the reference's SVG plots.
 
 #include "dwf_hip.hpp"
 #include "dwf_params.hpp"
+#include "wilson_stats.hpp"
 
 #include <complex>
 #include <iomanip>
@@ -41,6 +43,10 @@ int main()
                     for (int sink = 0; sink < 2; ++sink)
                         for (int ss = 0; ss < 2; ++ss) all[i][t] += std::norm(prop[ss][(size_t)((t * Nx + x) * 2 + sink)]);
         }
+        const app::WilsonMass m = app::wilson_mass(
+            all, app::PlateauSettings{P.plateau_window_size, P.plateau_flatness_tolerance, P.earliest_plateau_time},
+            &std::cout);
+        app::write_effective_mass_csv("output/wilson/effective_mass.csv", m);
         std::ofstream corr("output/wilson/correlators_by_config.csv");
         corr << "config";
         for (int t = 0; t < Nt; ++t) corr << ",t" << t;
@@ -50,7 +56,7 @@ int main()
             for (double v : all[i]) corr << "," << v;
             corr << "\n";
         }
-        std::cout << "Wrote:\n  output/wilson/correlators_by_config.csv\n";
+        std::cout << "Wrote:\n  output/wilson/effective_mass.csv\n  output/wilson/correlators_by_config.csv\n";
         return 0;
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << "\n";

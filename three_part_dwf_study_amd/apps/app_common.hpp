@@ -203,6 +203,21 @@ inline std::vector<std::vector<double>> load_wilcfg(const std::string& path, int
 }
 
 // Solver choice for the GPU (an additive key; the reference's keys are unchanged).
+// Learned-preconditioner weights on disk: an NPY <f8 vector of n values (the library's
+// float weights, widened).
+inline std::vector<float> load_network_weights(const std::string& path, size_t n)
+{
+    const NpyArray a = load_npy(path);
+    if (a.shape.size() != 1 || a.shape[0] != n)
+        throw std::runtime_error(path + " must hold " + std::to_string(n) + " network weights");
+    return std::vector<float>(a.data.begin(), a.data.end());
+}
+
+inline void save_network_weights(const std::string& path, const std::vector<float>& w)
+{
+    save_npy(path, {w.size()}, std::vector<double>(w.begin(), w.end()));
+}
+
 inline int solver_method(const Parameters& p, const std::string& key, int dflt)
 {
     const std::string v = p.text_or(key, "");
@@ -210,7 +225,8 @@ inline int solver_method(const Parameters& p, const std::string& key, int dflt)
     if (v == "cg64") return 0;
     if (v == "mixed") return 1;
     if (v == "eo_mixed") return 2;
-    throw std::runtime_error("Invalid value for " + key + ": " + v + " (cg64, mixed or eo_mixed)");
+    if (v == "learned") return 3;
+    throw std::runtime_error("Invalid value for " + key + ": " + v + " (cg64, mixed, eo_mixed or learned)");
 }
 
 }  // namespace app

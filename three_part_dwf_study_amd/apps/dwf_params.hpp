@@ -20,7 +20,9 @@ struct DwfParams {
     int max_momentum;
     unsigned long long random_seed;
     int force_method, propagator_method;  // gpu.* (additive keys)
-    int analysis_batch;                   // configurations per batched solve (0 = automatic)
+    int analysis_batch;                   // configurations per batched solve (0 = autotuned)
+    std::string network_weights;          // learned preconditioner weights (NPY), for "learned" solvers
+    std::string tune_cache;               // autotuner result file (empty: none)
 
     explicit DwfParams(const Parameters& p)
     {
@@ -49,6 +51,15 @@ struct DwfParams {
         propagator_method = solver_method(p, "gpu.propagator_solver", 0);
         analysis_batch = std::stoi(p.text_or("gpu.analysis_batch", "0"));
         if (analysis_batch < 0) throw std::runtime_error("gpu.analysis_batch must be >= 0");
+        network_weights = p.text_or("gpu.network_weights", "");
+        tune_cache = p.text_or("gpu.tune_cache", "");
+    }
+    // The weights file a "learned" solver needs.
+    const std::string& weights_for(int method, const char* key) const
+    {
+        if (method == 3 && network_weights.empty())
+            throw std::runtime_error(std::string(key) + " = learned needs gpu.network_weights");
+        return network_weights;
     }
 };
 

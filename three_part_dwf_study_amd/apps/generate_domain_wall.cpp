@@ -4,7 +4,8 @@
 // output/domain_wall/configs.npy (every trajectory's theta0), analysis_configs.npy
 // (after the thermalization cut, thinned) and run_info_history.csv.  Each trajectory
 // runs on the device in FP64 (dwf_hmc_trajectory); the force solves use
-// gpu.force_solver (cg64 by default, the reference's algorithm).
+// gpu.force_solver (cg64 by default, the reference's algorithm; "learned" with the
+// network weights of gpu.network_weights, e.g. from train_domain_wall_network).
 
 #include "dwf_hip.hpp"
 #include "dwf_params.hpp"
@@ -19,9 +20,17 @@ int main()
     try {
         const app::Parameters prm;
         const app::DwfParams P(prm);
+        const std::string weights = P.weights_for(P.force_method, "gpu.force_solver");  // checked before any device work
         std::filesystem::create_directories("output/domain_wall");
         dwfhip::HostRng rng(P.random_seed);
         dwfhip::Context& ctx = dwfhip::Context::dwf(P.Nt, P.Nx, P.N5, P.M5, P.mf);
+        std::unique_ptr<dwfhip::Network> net;
+        if (P.force_method == DWF_SOLVE_LEARNED) {
+            net.reset(new dwfhip::Network(P.N5));
+            net->set_weights(app::load_network_weights(weights,
+                                                       dwfhip::Network::size(P.N5)));
+            ctx.use_network(net.get());
+        }
         const size_t ngauge = (size_t)2 * P.Nt * P.Nx;
         std::vector<double> theta0(ngauge, 1.0);  // cold start, generate_configs.cpp:27-28
 
@@ -60,6 +69,7 @@ int main()
                   << P.trajectories << ", 2, " << P.Nt << ", " << P.Nx << ")"
                   << "\nanalysis configs = (" << nan << ", 2, " << P.Nt << ", " << P.Nx << ")\n"
                   << "saved configs.npy\nsaved analysis_configs.npy\nsaved run_info_history.csv\n";
+        ctx.use_network(nullptr);
         return 0;
     } catch (const std::exception& e) {
         std::cerr << "error: " << e.what() << '\n';

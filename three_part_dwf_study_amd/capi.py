@@ -121,6 +121,10 @@ class TuneResult(ctypes.Structure):
     ]
 
 
+class BatchTuneResult(ctypes.Structure):
+    _fields_ = [("nrhs", ctypes.c_int), ("from_cache", ctypes.c_int), ("solves_per_s", ctypes.c_double)]
+
+
 class LatticeGroup(ctypes.Structure):
     _fields_ = [("Nt", ctypes.c_int), ("Nx", ctypes.c_int), ("ncfg", ctypes.c_int), ("pad", ctypes.c_int)]
 
@@ -178,6 +182,7 @@ PROTOTYPES = {
     "dwf_ctx_stream": (_P, [_P]),
     "dwf_ctx_tune": (_I, [_P, _I, _D]),
     "dwf_autotune": (_I, [_P, _I, _I, ctypes.c_char_p, _P]),
+    "dwf_autotune_batch": (_I, [_P, _I, _D, _I, ctypes.c_char_p, _P]),
     "dwf_set_gauge": (_I, [_P, _P]),
     "dwf_set_gauge_device": (_I, [_P, _P]),
     "dwf_set_gauge_batch": (_I, [_P, _I, _P]),
@@ -332,6 +337,14 @@ class DwfContext:
         _check(self._lib, self._lib.dwf_autotune(self._h, nrhs, precision,
                                                  cache_path.encode() if cache_path else None, ctypes.byref(r)),
                "dwf_autotune")
+        return {name: getattr(r, name) for name, _ in r._fields_}
+
+    def autotune_batch(self, method: int, rtol: float, max_nrhs: int, cache_path: str | None = None) -> dict:
+        """dwf_autotune_batch: the solve batch size with the most solves/s."""
+        r = BatchTuneResult()
+        _check(self._lib, self._lib.dwf_autotune_batch(self._h, method, rtol, max_nrhs,
+                                                       cache_path.encode() if cache_path else None, ctypes.byref(r)),
+               "dwf_autotune_batch")
         return {name: getattr(r, name) for name, _ in r._fields_}
 
     def synchronize(self):

@@ -213,3 +213,130 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
         return DWF_OK;
     });
 }
+
+// ------------------------------------------------------------ right-hand-side blocking
+namespace {
+
+// Boundary sources of the analyzer's shape: hash-based Z4 values {1, i, -1, -i} in spin 1
+// of slice 0 and spin 0 of slice Ls-1 (where correlator.hpp:60-75 places its noise),
+// zeros elsewhere; nrhs vectors of V5 sites (FP64).
+__global__ void k_fill_boundary(double* v, long long V4, int Ls, long long nrhs)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;  // over nrhs * Ls * V4 sites
+    if (i >= nrhs * Ls * V4) return;
+    const long long r = i / (Ls * V4), rem = i - r * Ls * V4;
+    const int s = (int)(rem / V4);
+    unsigned x = (unsigned)i * 747796405u + 2891336453u;
+    x = ((x >> ((x >> 28u) + 4u)) ^ x) * 277803737u;
+    x = ((x >> 22u) ^ x) >> 30;  // 0..3
+    const double re = x == 0 ? 1.0 : x == 2 ? -1.0 : 0.0, im = x == 1 ? 1.0 : x == 3 ? -1.0 : 0.0;
+    double4 out = make_double4(0, 0, 0, 0);
+    if (s == 0) out.z = re, out.w = im;          // P_L eta at s = 0: spin 1
+    if (s == Ls - 1) out.x = re, out.y = im;     // P_R eta at s = Ls-1: spin 0
+    reinterpret_cast<double4*>(v)[i] = out;
+}
+
+std::string batch_key(const dwf_ctx* c, int method, double rtol)
+{
+    std::ostringstream k;
+    k << "b1 " << (c->kind == 0 ? "dwf" : "wilson") << ' ' << c->L.Nt << ' ' << c->L.Nx << ' ' << c->L.Ls << ' ' << method
+      << ' ' << rtol;
+    return k.str();
+}
+
+}  // namespace
+
+extern "C" int dwf_autotune_batch(dwf_ctx* c, int method, double rtol, int max_nrhs, const char* cache_path,
+                                  dwf_batch_tune_result* out)
+{
+    return guard([&]() -> int {
+        if (!c || max_nrhs < 1 || !(rtol > 0.0) ||
+            (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_EO_MIXED &&
+             method != DWF_SOLVE_LEARNED))
+            throw DwfError(DWF_ERR_ARG, "dwf_autotune_batch: bad argument");
+        if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_autotune_batch: no gauge field set");
+        c->require_single_lattice("dwf_autotune_batch");
+        c->require_single_config("dwf_autotune_batch");
+        const std::string key = batch_key(c, method, rtol);
+        if (cache_path) {
+            std::ifstream f(cache_path);
+            std::string line;
+            int best = 0;
+            double sps = 0;
+            while (std::getline(f, line)) {
+                if (line.compare(0, key.size(), key) != 0 || line.size() <= key.size() || line[key.size()] != ' ') continue;
+                std::istringstream rest(line.substr(key.size()));
+                int n = 0;
+                double v = 0;
+                if (rest >> n >> v && n >= 1 && n <= max_nrhs) {
+                    best = n;
+                    sps = v;
+                }
+            }
+            if (best > 0) {
+                if (out) *out = dwf_batch_tune_result{best, 1, sps};
+                return DWF_OK;
+            }
+        }
+        // device memory bounds the batch: about 16 FP64 full-lattice vectors per right-hand
+        // side cover the even-odd solver's workspace with room to spare
+        const long long V4 = c->L.V4, V5 = c->L.V5();
+        size_t free_b = 0, total_b = 0;
+        c->check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const long long per_rhs = 16LL * V5 * 32;
+        const int cap = (int)std::max<long long>(1, std::min<long long>(max_nrhs, (long long)(0.5 * free_b) / per_rhs));
+        std::vector<int> cand;
+        for (int n = 8; n < cap; n *= 2) cand.push_back(n);
+        cand.push_back(cap);
+        hipEvent_t e0, e1;
+        c->check(hipEventCreate(&e0), "event");
+        c->check(hipEventCreate(&e1), "event");
+        int best = cand[0];
+        double best_sps = 0.0;
+        for (int n : cand) {
+            double* src = nullptr;
+            double* x = nullptr;
+            const size_t bytes = (size_t)n * V5 * 32;
+            c->check(hipMalloc(&src, bytes), "autotune buffers");
+            if (hipMalloc(&x, bytes) != hipSuccess) {
+                (void)hipFree(src);
+                break;
+            }
+            const long long sites = (long long)n * V5;
+            k_fill_boundary<<<(unsigned)((sites + 255) / 256), 256, 0, c->stream>>>(src, V4, c->L.Ls, n);
+            std::vector<dwf_solve_stats> st(n);
+            float ms = 0;
+            int rc = DWF_OK;
+            for (int rep = 0; rep < 2 && rc == DWF_OK; ++rep) {  // the first run grows the workspace
+                c->check(hipEventRecord(e0, c->stream), "event");
+                rc = dwf_solve_dirac_impl(c, method, src, nullptr, rtol, 100000, n, x, st.data());
+                c->check(hipEventRecord(e1, c->stream), "event");
+                c->check(hipEventSynchronize(e1), "event sync");
+                c->check(hipEventElapsedTime(&ms, e0, e1), "event time");
+            }
+            (void)hipFree(src);
+            (void)hipFree(x);
+            if (rc != DWF_OK) {
+                (void)hipEventDestroy(e0);
+                (void)hipEventDestroy(e1);
+                throw DwfError(rc, std::string("dwf_autotune_batch: timing solve failed: ") + dwf_last_error());
+            }
+            const double sps = n / (1e-3 * ms);
+            const bool gain = sps > 1.05 * best_sps;
+            if (sps > best_sps) {
+                best_sps = sps;
+                best = n;
+            }
+            if (!gain) break;  // a doubling gained less than 5%
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        c->bind_rhs(1);
+        if (cache_path) {
+            std::ofstream f(cache_path, std::ios::app);
+            if (f) f << key << ' ' << best << ' ' << best_sps << '\n';
+        }
+        if (out) *out = dwf_batch_tune_result{best, 0, best_sps};
+        return DWF_OK;
+    });
+}

@@ -308,7 +308,7 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
             throw DwfError(DWF_ERR_ARG, "dwf_measure: missing output buffer");
     }
     if (spec->maxiter < 0 || (spec->method != DWF_SOLVE_CG64 && spec->method != DWF_SOLVE_MIXED &&
-                              spec->method != DWF_SOLVE_EO_MIXED))
+                              spec->method != DWF_SOLVE_EO_MIXED && spec->method != DWF_SOLVE_LEARNED))
         throw DwfError(DWF_ERR_ARG, "dwf_measure: bad argument");
     const int K = nwall + npoint + nnoise;  // right-hand sides per configuration
     for (int k = 0; k < ncfg; ++k) outs[k].wall_nonconverged = 0;
