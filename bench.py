@@ -576,7 +576,10 @@ def run_learned(args, dwf, capi):
 
 
 def run_hmc(args, dwf, capi):
-    theta0 = np.random.default_rng(1234).uniform(-np.pi, np.pi, 2 * NT * NX)
+    # hot start theta ~ U[-pi, pi) (seed 1234), or the reference's cold start theta = 1
+    # (domain_wall/generate_configs.cpp:27-28)
+    theta0 = (np.random.default_rng(1234).uniform(-np.pi, np.pi, 2 * NT * NX) if args.hmc_start == "hot"
+              else np.ones(2 * NT * NX))
     ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
     ctx.set_gauge(theta0)
     if args.no_graphs:
@@ -592,8 +595,9 @@ def run_hmc(args, dwf, capi):
     eps, nsteps = 0.05, 20
     net, tstats = None, None
     if method == dwf.DWF_SOLVE_LEARNED:
-        # trained on hot-start configurations (held out: the trajectory's own seed)
-        net, tstats = train_network(args, dwf, capi, hot=True)
+        # trained on configurations like the chain's (hot: uniform angles; cold: theta ~
+        # N(0, 1/beta), what the cold start relaxes to), the trajectory's own seed held out
+        net, tstats = train_network(args, dwf, capi, hot=args.hmc_start == "hot")
         learned_solver_config(args, dwf, ctx, net)
         ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA, args.reliable_delta)
 
@@ -640,9 +644,12 @@ def run_hmc(args, dwf, capi):
         "data": "synthetic",
         "config": {
             "workload": "Nf=2 Shamir DWF pseudofermion HMC, 96x96 Ls=16 beta=4 mf=0.05 M5=1, tau=1, 20 leapfrog "
-                        "steps, hot start seed 1234, force solves to 1e-10 with chronological guesses",
+                        + ("steps, hot start seed 1234" if args.hmc_start == "hot" else "steps, cold start theta=1")
+                        + ", force solves to 1e-10 with chronological guesses",
             "force_solver": args.hmc_solver,
+            "start": args.hmc_start,
         },
+        "acceptance_rate": float(np.mean([h["accepted"] for h in hist])),
         "cg_iterations_per_trajectory": cg_per_traj,
         "stencil_shape": {k: shape[k] for k in ("tile_x", "prefetch", "chunks", "nt_store", "lds")},
         "training": tstats,
@@ -681,6 +688,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("solves", "analyzer", "learned", "hmc"), default="solves")
     ap.add_argument("--hmc-solver", choices=("cg64", "mixed", "learned"), default="mixed")
+    ap.add_argument("--hmc-start", choices=("hot", "cold"), default="hot",
+                    help="HMC start: theta ~ U[-pi, pi) (seed 1234) or the reference's theta = 1")
     ap.add_argument("--learn-configs", type=int, default=64)
     ap.add_argument("--learn-pairs", type=int, default=64, help="training pairs per configuration")
     ap.add_argument("--learn-steps", type=int, default=1000)

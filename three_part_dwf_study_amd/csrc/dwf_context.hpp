@@ -93,6 +93,7 @@ enum BufId : int {
     B_GAMMA_B,   // learned D D^dag solves: Gamma b
     B_FCG_P32B,  // learned solver: the second direction buffer of the fused iteration
     B_NOFF,      // analyzer: offsets of each configuration's noise vectors
+    B_HOLD,      // mixed CG: right-hand sides a reliable update skips
     B_COUNT
 };
 
@@ -121,6 +122,14 @@ struct dwf_ctx {
     size_t link_cap = 0;          // complex link entries the link buffers can hold
     dwfk::Groups gr{};            // the current call's layout (set by bind_rhs)
     bool multi() const { return gdefs.size() > 1; }
+    // Device copy of gr for multi-lattice launches (uploaded when the layout changes; the
+    // kernels of one call all read the call's layout, and cached graphs are keyed by it).
+    const dwfk::Groups* groups_device();
+    dwfk::Groups* gr_dev = nullptr;
+    dwfk::Groups gr_dev_copy{};
+    bool gr_dev_valid = false;
+    // gr for a multi-lattice launcher (its grid), null for one lattice
+    const dwfk::Groups* groups_host() const { return multi() ? &gr : nullptr; }
     void require_single_lattice(const char* what) const;
     // The layout of nrhs right-hand sides (after bind_rhs(nrhs)): gr for several groups,
     // else one group of L.

@@ -178,15 +178,12 @@ template <typename R> struct SiteK {
     R stp, stm, sxp, sxm;
 };
 
-// Links of right-hand side rhs (group g, lattice of V4 sites).
-template <typename R>
-__device__ __forceinline__ const typename Cx<R>::T* rhs_links(const HopArgs<R>& a, int g, int rhs, int V4)
+// Links of right-hand side rhs of a single-lattice launch.
+template <typename R> __device__ __forceinline__ const typename Cx<R>::T* rhs_links(const HopArgs<R>& a, int rhs)
 {
     using C = typename Cx<R>::T;
-    const C* base = static_cast<const C*>(a.links);
-    if (a.cfg_map) return base + (long long)a.cfg_map[rhs] * a.link_stride;
-    const int cfg = (rhs - a.gr.rhs0[g]) / a.gr.rpc[g];
-    return base + a.gr.link0[g] + (long long)cfg * 2 * V4;
+    const int cfg = a.cfg_map ? a.cfg_map[rhs] : rhs / a.rhs_per_cfg;
+    return static_cast<const C*>(a.links) + (long long)cfg * a.link_stride;
 }
 
 template <typename R>
@@ -333,24 +330,45 @@ __device__ __forceinline__ void hop_epi(const HopArgs<R>& a, long long rb, long 
 
 }
 
-template <typename R, bool DAG, bool D5, int EPI, bool PF>
+// MG: a launch over several lattices (a.groups set); the single-lattice instantiations
+// carry none of the group code.
+template <typename R, bool DAG, bool D5, int EPI, bool PF, bool MG = false>
 __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
 {
-    // 1D grid over (lattice group, rhs, s-chunk, tile), optionally in XCD-aware order
+    // 1D grid over (rhs, s-chunk, tile) -- over (lattice group, rhs, s-chunk, block) with
+    // several lattices -- optionally in XCD-aware order
     const int total = gridDim.x;
     const int lb = a.cfg.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
-    const int g = group_of_block(a.gr, a.gb, lb);
-    const int bx = a.gb.bx[g], nblk = bx * a.cfg.chunks, pstride = a.gb.pstride;
-    const int local = lb - a.gb.blk0[g];
-    const int rl = local / nblk;                // right-hand side within the group
-    const int rhs = a.gr.rhs0[g] + rl;
-    const int bid = local - rl * nblk;          // block within this right-hand side
+    Lat L = a.L;
+    int rhs, bid, bx, pstride;
+    long long rb;                       // this right-hand side's spinors (offset in reals)
+    const typename Cx<R>::T* lk;        // and links
+    if constexpr (!MG) {
+        const int nblk = a.cfg.nblk();
+        rhs = lb / nblk;
+        bid = lb - rhs * nblk;
+        bx = a.cfg.bx;
+        pstride = nblk;
+        rb = (long long)rhs * L.V5() * 4;
+        lk = rhs_links(a, rhs);
+    } else {
+        const int chunks = a.cfg.chunks;
+        int blk0, nblk;
+        const GroupPos p = block_group(a.groups, lb, [chunks](int v4) { return (v4 + kBlock - 1) / kBlock * chunks; },
+                                       blk0, nblk);
+        const int local = lb - blk0, rl = local / nblk;  // right-hand side within the group
+        rhs = p.rhs0 + rl;
+        bid = local - rl * nblk;
+        bx = nblk / chunks;
+        pstride = a.pstride;
+        L.Nt = p.Nt;
+        L.Nx = p.Nx;
+        L.V4 = L.Nt * L.Nx;
+        rb = (p.site0 + (long long)rl * L.V5()) * 4;
+        lk = static_cast<const typename Cx<R>::T*>(a.links) + p.link0 + (long long)(rl / p.rpc) * 2 * L.V4;
+    }
     const int chunk = bid / bx;
     const int tile = bid - chunk * bx;
-    Lat L = a.L;
-    L.Nt = a.gr.Nt[g];
-    L.Nx = a.gr.Nx[g];
-    L.V4 = L.Nt * L.Nx;
     if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
 
     __shared__ double sh[8];
@@ -363,8 +381,6 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
     }
 
     const int V4 = L.V4;
-    const long long V5 = L.V5();
-    const long long rb = (a.gr.site0[g] + (long long)rl * V5) * 4;  // offset in reals
     int t, x;
     if (a.cfg.tx) {  // 2D tile: (kBlock/tx) rows x tx columns
         const int tt = tile / a.cfg.tiles_x, tc = tile - tt * a.cfg.tiles_x;
@@ -387,7 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
         const int tm = (t == 0) ? Nt - 1 : t - 1;
         const int xp = (x + 1 == Nx) ? 0 : x + 1;
         const int xm = (x == 0) ? Nx - 1 : x - 1;
-        const SiteK<R> k = site_consts<R>(L, rhs_links(a, g, rhs, V4), t, x);
+        const SiteK<R> k = site_consts<R>(L, lk, t, x);
         const int ntp = tp * Nx + x, ntm = tm * Nx + x, nxp = t * Nx + xp, nxm = t * Nx + xm;
 
         const R* in = a.in + rb;
@@ -612,8 +628,7 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
             hslot = (hh - 2 * tx - ty + 1) * W + tx + 1;
         }
     }
-    // (a single-lattice launch: group 0 is the lattice L)
-    const SiteK<R> k = site_consts<R>(L, rhs_links(a, 0, rhs, V4), t, x);
+    const SiteK<R> k = site_consts<R>(L, rhs_links(a, rhs), t, x);
     double acc0 = 0.0, acc1 = 0.0;
     const int s0 = chunk * a.cfg.spt;
     const int s1 = min(Ls, s0 + a.cfg.spt);
@@ -872,9 +887,9 @@ __global__ __launch_bounds__(NT) void k_hop_lds(const HopArgs<R> a)
 // p = r + beta p.  Block 0 writes the next slot's scalars; the other blocks only read
 // the current slot, so nothing in one launch races.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_cg_p(const VecLayout lay, const int nblk, R* r, R* p,
-                                                 const double* partA, const double* partB,
-                                                 CgState* st, const int slot, const int mode)
+__global__ __launch_bounds__(kBlock) void k_cg_p(const long long sites, const int nblk, R* r, R* p,
+                                                 const double* partA, const double* partB, CgState* st, const int slot,
+                                                 const int mode, const Groups* groups, const int Ls, const int shift)
 {
     const int rhs = blockIdx.z;
     CgState& S = st[rhs];
@@ -913,9 +928,13 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const VecLayout lay, const int 
     }
     if (zero || (mode == MODE_CG64 && conv)) return;
     const R beta = (R)(rr_new / rr_old);
-    const int g = group_of_rhs(lay.gr, rhs);
-    const long long V5 = lay.sites(g);
-    const long long rb = lay.offset(g, rhs) * 4;
+    long long V5 = sites, rb = (long long)rhs * sites * 4;
+    if (groups) {  // several lattices: this right-hand side's extent and offset
+        const GroupPos q = group_for_rhs(*groups, rhs);
+        const long long v5 = (long long)Ls * q.Nt * q.Nx;
+        V5 = v5 >> shift;
+        rb = ((q.site0 + (long long)(rhs - q.rhs0) * v5) >> shift) * 4;
+    }
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < V5; i += (long long)gridDim.x * kBlock) {
         const Sp<R> rv = ldsp(r + rb, i);
         Sp<R> pv = ldsp(p + rb, i);
@@ -1006,10 +1025,11 @@ __global__ void k_true_resid(const int nblk, const double* partR, CgState* st)
     }
 }
 
-__global__ void k_hold(CgState* st, const int slot, const int nrhs)
+// F_HOLD on the right-hand sides the host marked (hold[i] != 0) that are not done.
+__global__ void k_hold(CgState* st, const int* hold, const int slot, const int nrhs)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nrhs && !(st[i].flags[slot] & (F_DONE | F_FROZEN))) st[i].flags[slot] |= F_HOLD;
+    if (i < nrhs && hold[i] && !(st[i].flags[slot] & F_DONE)) st[i].flags[slot] |= F_HOLD;
 }
 
 __global__ void k_set_frozen_all(CgState* st, const int slot, const int nrhs)
@@ -1019,15 +1039,21 @@ __global__ void k_set_frozen_all(CgState* st, const int slot, const int nrhs)
 }
 
 // x64 += x32; x32 = 0 for the right-hand sides still running.
-__global__ __launch_bounds__(kBlock) void k_accum_x(const VecLayout lay, double* x64, float* x32,
-                                                    const CgState* st, const int slot, const int skip)
+__global__ __launch_bounds__(kBlock) void k_accum_x(const long long sites, double* x64, float* x32,
+                                                    const CgState* st, const int slot, const int skip,
+                                                    const Groups* groups, const int Ls, const int shift)
 {
     const int rhs = blockIdx.z;
     if (st[rhs].flags[slot] & skip) return;
-    const int g = group_of_rhs(lay.gr, rhs);
+    long long n = sites, rb = (long long)rhs * sites * 4;
+    if (groups) {
+        const GroupPos q = group_for_rhs(*groups, rhs);
+        const long long v5 = (long long)Ls * q.Nt * q.Nx;
+        n = v5 >> shift;
+        rb = ((q.site0 + (long long)(rhs - q.rhs0) * v5) >> shift) * 4;
+    }
     const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;  // site
-    if (i >= lay.sites(g)) return;
-    const long long rb = lay.offset(g, rhs) * 4;
+    if (i >= n) return;
     const Sp<float> d = ldsp(x32 + rb, i);
     Sp<double> v = ldsp(x64 + rb, i);
     v.ar += d.ar;
@@ -1054,15 +1080,15 @@ __global__ void k_pack_links(const int n, const double* theta, double2* l64, flo
     l32[i] = make_float2((float)c, (float)s);
 }
 
-template <typename R, bool DAG, bool D5, bool PF>
+template <typename R, bool DAG, bool D5, bool PF, bool MG = false>
 void dispatch_epi_pf(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
 {
     switch (epi) {
-        case EPI_STORE: k_hop<R, DAG, D5, EPI_STORE, PF><<<grid, kBlock, 0, s>>>(a); break;
-        case EPI_NORM: k_hop<R, DAG, D5, EPI_NORM, PF><<<grid, kBlock, 0, s>>>(a); break;
-        case EPI_CGUPD: k_hop<R, DAG, D5, EPI_CGUPD, PF><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_STORE: k_hop<R, DAG, D5, EPI_STORE, PF, MG><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_NORM: k_hop<R, DAG, D5, EPI_NORM, PF, MG><<<grid, kBlock, 0, s>>>(a); break;
+        case EPI_CGUPD: k_hop<R, DAG, D5, EPI_CGUPD, PF, MG><<<grid, kBlock, 0, s>>>(a); break;
         default:
-            if constexpr (sizeof(R) == 8) k_hop<R, DAG, D5, EPI_RESID, PF><<<grid, kBlock, 0, s>>>(a);
+            if constexpr (sizeof(R) == 8) k_hop<R, DAG, D5, EPI_RESID, PF, MG><<<grid, kBlock, 0, s>>>(a);
             break;
     }
 }
@@ -1108,7 +1134,9 @@ void dispatch_epi(int epi, const HopArgs<R>& a, dim3 grid, hipStream_t s)
         throw DwfError(DWF_ERR_ARG, "EPI_PNORM / EPI_RUPD need the LDS-staged stencil");
     if (epi == EPI_CGUPD && (!a.x || a.aout) && !a.cfg.lds)
         throw DwfError(DWF_ERR_ARG, "EPI_CGUPD without x or with aout needs the LDS-staged stencil");
-    if (a.cfg.lds)
+    if (a.groups)
+        dispatch_epi_pf<R, DAG, D5, false, true>(epi, a, grid, s);  // lattice groups: linear blocks, no prefetch
+    else if (a.cfg.lds)
         dispatch_lds<R, DAG, D5>(epi, a, grid, s);
     else if (a.cfg.prefetch)
         dispatch_epi_pf<R, DAG, D5, true>(epi, a, grid, s);
@@ -1181,32 +1209,22 @@ HopCfg hop_cfg(const Lat& L, int nrhs, int chunks_override, int remap, int prefe
     return c;
 }
 
-template <typename R> void launch_hop(bool dagger, int epi, const HopArgs<R>& a0, int nrhs, hipStream_t s)
+template <typename R>
+void launch_hop(bool dagger, int epi, const HopArgs<R>& a0, int nrhs, hipStream_t s, const Groups* hg)
 {
     HopArgs<R> a = a0;
     long long nb = 0;
-    if (a.gr.n <= 1) {
+    if (!a.groups) {
         // one lattice: the shape's own tiles (direct or LDS-staged)
-        if (a.gr.n == 0) a.gr = single_group(a.L, nrhs, a.rhs_per_cfg);
-        if (a.gr.Nt[0] != a.L.Nt || a.gr.Nx[0] != a.L.Nx || a.gr.rhs0[1] != nrhs || a.gr.site0[0] != 0 ||
-            a.gr.link0[0] != 0)
-            throw DwfError(DWF_ERR_ARG, "stencil launch: one-group layout does not match the lattice");
         nb = (long long)a.cfg.nblk() * nrhs;
-        a.gb.blk0[0] = 0;
-        a.gb.blk0[1] = (int)nb;
-        a.gb.bx[0] = a.cfg.bx;
-        a.gb.pstride = a.cfg.nblk();
     } else {
         // several lattices: linear blocks of kBlock consecutive 4D sites per group
-        if (a.cfg.lds || a.cfg.tx || a.cfg_map || a.gr.n > kMaxGroups || a.gr.rhs0[a.gr.n] != nrhs)
+        if (!hg || a.cfg.lds || a.cfg.tx || a.cfg_map || hg->n < 1 || hg->n > kMaxGroups || hg->rhs0[hg->n] != nrhs)
             throw DwfError(DWF_ERR_ARG, "stencil launch: bad lattice-group layout");
-        for (int g = 0; g < a.gr.n; ++g) {
-            a.gb.blk0[g] = (int)nb;
-            a.gb.bx[g] = (a.gr.Nt[g] * a.gr.Nx[g] + kBlock - 1) / kBlock;
-            nb += (long long)(a.gr.rhs0[g + 1] - a.gr.rhs0[g]) * a.gb.bx[g] * a.cfg.chunks;
-        }
-        a.gb.blk0[a.gr.n] = (int)nb;
-        a.gb.pstride = hop_pstride(a.gr, a.cfg);
+        for (int g = 0; g < hg->n; ++g)
+            nb += (long long)(hg->rhs0[g + 1] - hg->rhs0[g]) * ((hg->Nt[g] * hg->Nx[g] + kBlock - 1) / kBlock) *
+                  a.cfg.chunks;
+        a.pstride = hop_pstride(*hg, a.cfg);
     }
     if (nb <= 0 || nb > 0x7fffffffLL) throw DwfError(DWF_ERR_ARG, "stencil launch: grid size out of range");
     const dim3 grid((unsigned)nb);
@@ -1222,8 +1240,8 @@ template <typename R> void launch_hop(bool dagger, int epi, const HopArgs<R>& a0
             dispatch_epi<R, false, false>(epi, a, grid, s);
     }
 }
-template void launch_hop<float>(bool, int, const HopArgs<float>&, int, hipStream_t);
-template void launch_hop<double>(bool, int, const HopArgs<double>&, int, hipStream_t);
+template void launch_hop<float>(bool, int, const HopArgs<float>&, int, hipStream_t, const Groups*);
+template void launch_hop<double>(bool, int, const HopArgs<double>&, int, hipStream_t, const Groups*);
 
 namespace {
 long long max_sites(const VecLayout& lay)
@@ -1241,7 +1259,10 @@ void launch_cg_p(const VecLayout& lay, int nblk, R* r, R* p, const double* partA
     // blocks of 4 sites per thread over the largest vector (grid-stride loops cover the rest)
     const long long per = (max_sites(lay) + 4LL * kBlock - 1) / (4LL * kBlock);
     const dim3 grid((unsigned)per, 1, nrhs);
-    k_cg_p<R><<<grid, kBlock, 0, s>>>(lay, nblk, r, p, partA, partB, st, slot, mode);
+    const bool one = lay.gr.n == 1;
+    if (!one && !lay.dev) throw DwfError(DWF_ERR_ARG, "CG update: lattice groups without their device table");
+    k_cg_p<R><<<grid, kBlock, 0, s>>>(one ? lay.sites(0) : 0, nblk, r, p, partA, partB, st, slot, mode,
+                                      one ? nullptr : lay.dev, lay.Ls, lay.shift);
 }
 template void launch_cg_p<float>(const VecLayout&, int, float*, float*, const double*, const double*, CgState*, int,
                                  int, int, hipStream_t);
@@ -1272,12 +1293,15 @@ void launch_accum_x(const VecLayout& lay, double* x64, float* x32, const CgState
                     hipStream_t s)
 {
     const dim3 grid((unsigned)((max_sites(lay) + kBlock - 1) / kBlock), 1, nrhs);
-    k_accum_x<<<grid, kBlock, 0, s>>>(lay, x64, x32, st, slot, skip);
+    const bool one = lay.gr.n == 1;
+    if (!one && !lay.dev) throw DwfError(DWF_ERR_ARG, "x update: lattice groups without their device table");
+    k_accum_x<<<grid, kBlock, 0, s>>>(one ? lay.sites(0) : 0, x64, x32, st, slot, skip, one ? nullptr : lay.dev,
+                                      lay.Ls, lay.shift);
 }
 
-void launch_hold(CgState* st, int slot, int nrhs, hipStream_t s)
+void launch_hold(CgState* st, const int* hold, int slot, int nrhs, hipStream_t s)
 {
-    k_hold<<<(nrhs + 63) / 64, 64, 0, s>>>(st, slot, nrhs);
+    k_hold<<<(nrhs + 63) / 64, 64, 0, s>>>(st, hold, slot, nrhs);
 }
 
 template <typename R> void launch_zero(R* p, long long n, hipStream_t s)

@@ -134,21 +134,46 @@ __device__ __forceinline__ EoSite<R> eo_site(const Lat& L, int Nxh, int tpar, in
     return g;
 }
 
-// Links of the configuration that right-hand side `rhs` (group g, V4 sites) belongs to.
-template <typename R> __device__ __forceinline__ const void* cfg_links(const EoArgs<R>& a, int g, int rhs, int V4)
-{
-    const int cfg = (rhs - a.gr.rhs0[g]) / a.gr.rpc[g];
-    return static_cast<const typename C2T<R>::T*>(a.links) + a.gr.link0[g] + (long long)cfg * 2 * V4;
-}
+// Where one block of an even-odd launch works: its right-hand side, block within it, the
+// lattice, half-lattice sizes, spinor offset (reals) and links.
+template <typename R> struct EoPos {
+    int rhs, bid;
+    Lat L;
+    int Nxh, V4h;
+    long long rb;
+    const void* links;
+};
 
-// Lattice of group g.
-__device__ __forceinline__ Lat group_lat(const Lat& base, const Groups& G, int g)
+// per: target sites per block; chunks: s-chunks (1 for k_eo_a); LS: the s-extent.
+template <typename R>
+__device__ __forceinline__ EoPos<R> eo_pos(const EoArgs<R>& a, int lb, int per, int chunks, int nblk1)
 {
-    Lat L = base;
-    L.Nt = G.Nt[g];
-    L.Nx = G.Nx[g];
-    L.V4 = L.Nt * L.Nx;
-    return L;
+    using C = typename C2T<R>::T;
+    EoPos<R> q;
+    q.L = a.L;
+    if (!a.groups) {
+        q.rhs = lb / nblk1;
+        q.bid = lb - q.rhs * nblk1;
+        q.Nxh = a.Nxh;
+        q.V4h = a.V4h;
+        q.rb = (long long)q.rhs * a.V4h * a.L.Ls * 4;
+        q.links = static_cast<const C*>(a.links) + (long long)(q.rhs / a.rhs_per_cfg) * a.link_stride;
+    } else {
+        int blk0, nblk;
+        const GroupPos p = block_group(a.groups, lb, [per, chunks](int v4) { return (v4 / 2 + per - 1) / per * chunks; },
+                                       blk0, nblk);
+        const int local = lb - blk0, rl = local / nblk;
+        q.rhs = p.rhs0 + rl;
+        q.bid = local - rl * nblk;
+        q.L.Nt = p.Nt;
+        q.L.Nx = p.Nx;
+        q.L.V4 = p.Nt * p.Nx;
+        q.Nxh = p.Nx / 2;
+        q.V4h = q.L.V4 / 2;
+        q.rb = (p.site0 / 2 + (long long)rl * q.V4h * a.L.Ls) * 4;
+        q.links = static_cast<const C*>(a.links) + p.link0 + (long long)(rl / p.rpc) * 2 * q.L.V4;
+    }
+    return q;
 }
 
 // o += c * H psi at one slice (H: the bracket of D's hop, or D^dag's when DAG).
@@ -251,19 +276,15 @@ __global__ __launch_bounds__(kBlockA) void k_eo_a(const EoArgs<R> a)
 {
     const int total = gridDim.x;
     const int lb = a.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
-    const int gi = group_of_block(a.gr, a.gb, lb);
-    const int bxa = a.gb.bx[gi];
-    const int local = lb - a.gb.blk0[gi];
-    const int rl = local / bxa;
-    const int rhs = a.gr.rhs0[gi] + rl;
-    const int hs = (local - rl * bxa) * kBlockA + threadIdx.x;
+    const EoPos<R> q = eo_pos(a, lb, kBlockA, 1, a.bxa);
+    const int rhs = q.rhs;
+    const int hs = q.bid * kBlockA + threadIdx.x;
     if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;
-    const Lat L = group_lat(a.L, a.gr, gi);
-    const int V4h = L.V4 / 2;
+    const Lat& L = q.L;
+    const int V4h = q.V4h;
     if (hs >= V4h) return;
-    const long long V5h = (long long)V4h * LS;
-    const long long rb = (a.gr.site0[gi] / 2 + (long long)rl * V5h) * 4;
-    const EoSite<R> g = eo_site<R>(L, L.Nx / 2, a.tpar, hs, cfg_links(a, gi, rhs, L.V4));
+    const long long rb = q.rb;
+    const EoSite<R> g = eo_site<R>(L, q.Nxh, a.tpar, hs, q.links);
     R v0r[LS], v0i[LS], v1r[LS], v1i[LS];
 #pragma unroll
     for (int s = 0; s < LS; ++s) {
@@ -296,12 +317,10 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
 {
     const int total = gridDim.x;
     const int lb = a.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
-    const int gi = group_of_block(a.gr, a.gb, lb);
-    const int bx = a.gb.bx[gi], nblk = bx * a.chunks, pstride = a.gb.pstride;
-    const int local = lb - a.gb.blk0[gi];
-    const int rl = local / nblk;
-    const int rhs = a.gr.rhs0[gi] + rl;
-    const int bid = local - rl * nblk;
+    const EoPos<R> q = eo_pos(a, lb, kBlock, a.chunks, a.bx * a.chunks);
+    const int rhs = q.rhs, bid = q.bid;
+    const int bx = a.groups ? (q.V4h + kBlock - 1) / kBlock : a.bx;
+    const int pstride = a.groups ? a.pstride : a.bx * a.chunks;
     const int chunk = bid / bx;
     const int hs = (bid - chunk * bx) * kBlock + threadIdx.x;
     if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;
@@ -312,14 +331,13 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
         if (pAp == 0.0) return;
         alpha = (R)(a.st[rhs].rr[a.slot] / pAp);
     }
-    const Lat L = group_lat(a.L, a.gr, gi);
-    const int Ls = L.Ls, V4h = L.V4 / 2;
-    const long long V5h = (long long)V4h * Ls;
-    const long long rb = (a.gr.site0[gi] / 2 + (long long)rl * V5h) * 4;
+    const Lat& L = q.L;
+    const int Ls = L.Ls, V4h = q.V4h;
+    const long long rb = q.rb;
     double acc0 = 0.0, acc1 = 0.0;
     const int s0 = chunk * a.spt, s1 = min(Ls, s0 + a.spt);
     if (hs < V4h && s0 < s1) {
-        const EoSite<R> g = eo_site<R>(L, L.Nx / 2, a.tpar, hs, cfg_links(a, gi, rhs, L.V4));
+        const EoSite<R> g = eo_site<R>(L, q.Nxh, a.tpar, hs, q.links);
         const R d = (R)L.diag, mf = (R)L.mf;
         const R* in = a.in + rb;
         Sp<R> cur = ld(in, (long long)s0 * V4h + hs);
@@ -412,9 +430,10 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
 __global__ void k_eo_split(const VecLayout lay, const double* full, double* ev, double* od)
 {
     const int rhs = blockIdx.y;
-    const int gi = group_of_rhs(lay.gr, rhs);
-    const int Nx = lay.gr.Nx[gi], V4 = lay.gr.Nt[gi] * Nx, V4h = V4 / 2;
-    const long long V5 = lay.sites(gi), off = lay.offset(gi, rhs);
+    const GroupPos gp = group_for_rhs(lay.gr, rhs);
+    const int Nx = gp.Nx, V4 = gp.Nt * Nx, V4h = V4 / 2;
+    long long V5, off;
+    lay.locate(rhs, V5, off);
     const double4* F = reinterpret_cast<const double4*>(full) + off;
     double4* E = reinterpret_cast<double4*>(ev) + off / 2;
     double4* O = reinterpret_cast<double4*>(od) + off / 2;
@@ -430,9 +449,10 @@ __global__ void k_eo_split(const VecLayout lay, const double* full, double* ev, 
 __global__ void k_eo_merge(const VecLayout lay, const double* ev, const double* od, double* full)
 {
     const int rhs = blockIdx.y;
-    const int gi = group_of_rhs(lay.gr, rhs);
-    const int Nx = lay.gr.Nx[gi], V4 = lay.gr.Nt[gi] * Nx, V4h = V4 / 2;
-    const long long V5 = lay.sites(gi), off = lay.offset(gi, rhs);
+    const GroupPos gp = group_for_rhs(lay.gr, rhs);
+    const int Nx = gp.Nx, V4 = gp.Nt * Nx, V4h = V4 / 2;
+    long long V5, off;
+    lay.locate(rhs, V5, off);
     double4* F = reinterpret_cast<double4*>(full) + off;
     const double4* E = reinterpret_cast<const double4*>(ev) + off / 2;
     const double4* O = reinterpret_cast<const double4*>(od) + off / 2;
@@ -475,23 +495,19 @@ bool eo_supported(const Lat& L)
 }
 
 namespace {
-// Normalises a's groups (n = 0: one group of a.L) and fills its block ranges, `per`
-// target sites per block, `chunks` s-chunks; returns the grid size.
-template <typename R> unsigned eo_blocks(EoArgs<R>& a, int nrhs, int per, int chunks)
+// Grid size of an even-odd launch: per target sites per block and `chunks` s-chunks per
+// right-hand side, over one lattice or the groups of hg.
+template <typename R> unsigned eo_grid(const EoArgs<R>& a, int nrhs, int per, int chunks, const Groups* hg)
 {
-    if (a.gr.n == 0) a.gr = single_group(a.L, nrhs, a.rhs_per_cfg);
-    if (a.gr.n > kMaxGroups || a.gr.rhs0[a.gr.n] != nrhs)
-        throw DwfError(DWF_ERR_ARG, "even-odd launch: bad lattice-group layout");
     long long nb = 0;
-    int m = 0;
-    for (int g = 0; g < a.gr.n; ++g) {
-        a.gb.blk0[g] = (int)nb;
-        a.gb.bx[g] = (a.gr.Nt[g] * a.gr.Nx[g] / 2 + per - 1) / per;
-        m = std::max(m, a.gb.bx[g] * chunks);
-        nb += (long long)(a.gr.rhs0[g + 1] - a.gr.rhs0[g]) * a.gb.bx[g] * chunks;
+    if (!a.groups) {
+        nb = (long long)((a.V4h + per - 1) / per) * chunks * nrhs;
+    } else {
+        if (!hg || hg->n < 1 || hg->n > kMaxGroups || hg->rhs0[hg->n] != nrhs)
+            throw DwfError(DWF_ERR_ARG, "even-odd launch: bad lattice-group layout");
+        for (int g = 0; g < hg->n; ++g)
+            nb += (long long)(hg->rhs0[g + 1] - hg->rhs0[g]) * ((hg->Nt[g] * hg->Nx[g] / 2 + per - 1) / per) * chunks;
     }
-    a.gb.blk0[a.gr.n] = (int)nb;
-    a.gb.pstride = m;
     if (nb <= 0 || nb > 0x7fffffffLL) throw DwfError(DWF_ERR_ARG, "even-odd launch: grid size out of range");
     return (unsigned)nb;
 }
@@ -504,10 +520,10 @@ int eo_pstride(const Groups& G, int chunks)
     return m;
 }
 
-template <typename R> void launch_eo_a(bool dagger, bool ainv, const EoArgs<R>& a0, int nrhs, hipStream_t s)
+template <typename R>
+void launch_eo_a(bool dagger, bool ainv, const EoArgs<R>& a, int nrhs, hipStream_t s, const Groups* hg)
 {
-    EoArgs<R> a = a0;
-    const unsigned grid = eo_blocks(a, nrhs, kBlockA, 1);
+    const unsigned grid = eo_grid(a, nrhs, kBlockA, 1, hg);
     const int Ls = a.L.Ls;
 #define DWF_EO_A(LSV)                                      \
     if (Ls == LSV) {                                       \
@@ -523,13 +539,15 @@ template <typename R> void launch_eo_a(bool dagger, bool ainv, const EoArgs<R>& 
     DWF_EO_A(16)
 #undef DWF_EO_A
 }
-template void launch_eo_a<float>(bool, bool, const EoArgs<float>&, int, hipStream_t);
-template void launch_eo_a<double>(bool, bool, const EoArgs<double>&, int, hipStream_t);
+template void launch_eo_a<float>(bool, bool, const EoArgs<float>&, int, hipStream_t, const Groups*);
+template void launch_eo_a<double>(bool, bool, const EoArgs<double>&, int, hipStream_t, const Groups*);
 
-template <typename R> void launch_eo_b(bool dagger, bool aop, int epi, const EoArgs<R>& a0, int nrhs, hipStream_t s)
+template <typename R>
+void launch_eo_b(bool dagger, bool aop, int epi, const EoArgs<R>& a0, int nrhs, hipStream_t s, const Groups* hg)
 {
     EoArgs<R> a = a0;
-    const unsigned grid = eo_blocks(a, nrhs, kBlock, a.chunks);
+    if (a.groups) a.pstride = eo_pstride(*hg, a.chunks);
+    const unsigned grid = eo_grid(a, nrhs, kBlock, a.chunks, hg);
     if (dagger) {
         if (aop)
             dispatch_b<R, true, true>(epi, a, grid, s);
@@ -542,8 +560,8 @@ template <typename R> void launch_eo_b(bool dagger, bool aop, int epi, const EoA
             dispatch_b<R, false, false>(epi, a, grid, s);
     }
 }
-template void launch_eo_b<float>(bool, bool, int, const EoArgs<float>&, int, hipStream_t);
-template void launch_eo_b<double>(bool, bool, int, const EoArgs<double>&, int, hipStream_t);
+template void launch_eo_b<float>(bool, bool, int, const EoArgs<float>&, int, hipStream_t, const Groups*);
+template void launch_eo_b<double>(bool, bool, int, const EoArgs<double>&, int, hipStream_t, const Groups*);
 
 namespace {
 unsigned split_blocks(const VecLayout& lay)

@@ -8,10 +8,10 @@ namespace dwfk {
 constexpr int kBlockA = 64;  // k_eo_a owns a whole s-column per thread: small blocks, more of them
 
 template <typename R> struct EoArgs {
-    Lat L;               // Ls, masses and signs (Nt, Nx per group in gr)
-    Groups gr;           // right-hand sides by lattice (n = 0: one group from L, nrhs and rhs_per_cfg)
-    GroupBlocks gb;      // set by the launchers: k_eo_b blocks of kBlock target sites per s-chunk,
-                         // k_eo_a blocks of kBlockA target sites (whole s-columns)
+    Lat L;
+    int Nxh, V4h;        // Nx/2, V4/2
+    int bx;              // k_eo_b: blocks of kBlock threads over the V4h target sites
+    int bxa;             // k_eo_a: blocks of kBlockA threads over the V4h target sites
     int chunks, spt;     // s-chunks (k_eo_b only)
     int remap;           // XCD-aware block order
     int tpar;            // target parity (0 even, 1 odd)
@@ -35,13 +35,22 @@ template <typename R> struct EoArgs {
     const CgState* st;
     int slot;
     int skip;
+    // Several lattices (null for one): the context's device copy of the call's lattice
+    // groups (Nt, Nx, Nxh, V4h, bx and bxa then follow from each group) and the partial
+    // arrays' stride (eo_pstride).
+    const Groups* groups;
+    int pstride;
 };
 
 bool eo_supported(const Lat& L);
 // Block partials per right-hand side of k_eo_b over groups with `chunks` s-chunks.
 int eo_pstride(const Groups& G, int chunks);
-template <typename R> void launch_eo_a(bool dagger, bool ainv, const EoArgs<R>& a, int nrhs, hipStream_t s);
-template <typename R> void launch_eo_b(bool dagger, bool aop, int epi, const EoArgs<R>& a, int nrhs, hipStream_t s);
+// hg: the host copy of a.groups' table for a multi-lattice launch (it sizes the grid), else null.
+template <typename R>
+void launch_eo_a(bool dagger, bool ainv, const EoArgs<R>& a, int nrhs, hipStream_t s, const Groups* hg = nullptr);
+template <typename R>
+void launch_eo_b(bool dagger, bool aop, int epi, const EoArgs<R>& a, int nrhs, hipStream_t s,
+                 const Groups* hg = nullptr);
 // full lexicographic vectors <-> even / odd halves, for the batch layout of lay (full-lattice
 // offsets: shift 0)
 void launch_eo_split(const VecLayout& lay, const double* full, double* ev, double* od, int nrhs, hipStream_t s);

@@ -112,6 +112,7 @@ dwf_ctx::~dwf_ctx()
     if (theta) (void)hipFree(theta);
     if (l64) (void)hipFree(l64);
     if (l32) (void)hipFree(l32);
+    if (gr_dev) (void)hipFree(gr_dev);
     if (st_host) (void)hipHostFree(st_host);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -190,6 +191,7 @@ void dwf_ctx::bind_rhs(int nrhs)
     }
     G.rhs0[G.n] = r;
     gr = G;
+    groups_device();
 }
 
 void dwf_ctx::require_single_lattice(const char* what) const
@@ -210,7 +212,21 @@ VecLayout dwf_ctx::layout(int nrhs, int shift) const
     v.gr = groups_for(nrhs);
     v.Ls = L.Ls;
     v.shift = shift;
+    v.dev = multi() ? gr_dev : nullptr;
     return v;
+}
+
+const Groups* dwf_ctx::groups_device()
+{
+    if (!multi()) return nullptr;
+    if (!gr_dev) check(hipMalloc(&gr_dev, sizeof(Groups)), "group table allocation");
+    if (!gr_dev_valid || std::memcmp(&gr_dev_copy, &gr, sizeof(Groups)) != 0) {
+        check(hipStreamSynchronize(stream), "group table update");  // no kernel still reads the old one
+        check(hipMemcpy(gr_dev, &gr, sizeof(Groups), hipMemcpyHostToDevice), "group table update");
+        gr_dev_copy = gr;
+        gr_dev_valid = true;
+    }
+    return gr_dev;
 }
 
 void dwf_ctx::require_single_config(const char* what) const
@@ -255,16 +271,16 @@ template <typename R> void apply_op(dwf_ctx* c, int op, const R* in, R* out, R* 
         case DWF_OP_DDAG:
             a.in = in;
             a.out = out;
-            launch_hop<R>(op == DWF_OP_DDAG, EPI_STORE, a, nrhs, c->stream);
+            launch_hop<R>(op == DWF_OP_DDAG, EPI_STORE, a, nrhs, c->stream, c->groups_host());
             break;
         default: {
             const bool first_dag = (op == DWF_OP_DDDAG);
             a.in = in;
             a.out = tmp;
-            launch_hop<R>(first_dag, EPI_STORE, a, nrhs, c->stream);
+            launch_hop<R>(first_dag, EPI_STORE, a, nrhs, c->stream, c->groups_host());
             a.in = tmp;
             a.out = out;
-            launch_hop<R>(!first_dag, EPI_STORE, a, nrhs, c->stream);
+            launch_hop<R>(!first_dag, EPI_STORE, a, nrhs, c->stream, c->groups_host());
         }
     }
     c->check_launch("stencil launch");
@@ -286,7 +302,10 @@ template <typename R> HopArgs<R> ctx_hop_args(const dwf_ctx* c, const HopCfg& cf
     a.links = sizeof(R) == 8 ? (const void*)c->l64 : (const void*)c->l32;
     a.link_stride = 2LL * c->L.V4;
     a.rhs_per_cfg = c->rhs_per_cfg;
-    if (c->multi()) a.gr = c->groups_for(nrhs);  // else one group of L, filled in by the launcher
+    if (c->multi()) {
+        (void)c->groups_for(nrhs);  // checks that the call's layout is bound
+        a.groups = c->gr_dev;
+    }
     return a;
 }
 template HopArgs<float> ctx_hop_args<float>(const dwf_ctx*, const HopCfg&, int);

@@ -63,28 +63,51 @@ struct Groups {
     long long link0[kMaxGroups]; // complex link entries before the group's first configuration
 };
 
-// Block ranges of one launch over groups: group g's blocks are [blk0[g], blk0[g+1]), bx[g]
-// per s-chunk and right-hand side.  pstride = block partials per right-hand side in the
-// partial arrays (the largest block count; a group with fewer blocks leaves its remaining
-// partials untouched, and the solvers zero them before they are read).
-struct GroupBlocks {
-    int blk0[kMaxGroups + 1];
-    int bx[kMaxGroups];
-    int pstride;
-};
-
-// Scalar scans (n <= kMaxGroups, uniform over the block).
-__host__ __device__ __forceinline__ int group_of_block(const Groups& G, const GroupBlocks& B, int lb)
-{
-    int g = 0;
-    for (int k = 1; k < G.n; ++k) g = lb >= B.blk0[k] ? k : g;
-    return g;
-}
+// Scalar scan (n <= kMaxGroups, uniform over the block).
 __host__ __device__ __forceinline__ int group_of_rhs(const Groups& G, int rhs)
 {
     int g = 0;
     for (int k = 1; k < G.n; ++k) g = rhs >= G.rhs0[k] ? k : g;
     return g;
+}
+
+// One group's fields as scalars.
+struct GroupPos {
+    int g, Nt, Nx, rhs0, rpc;
+    long long site0, link0;
+};
+__host__ __device__ __forceinline__ GroupPos group_fields(const Groups& G, int g)
+{
+    return GroupPos{g, G.Nt[g], G.Nx[g], G.rhs0[g], G.rpc[g], G.site0[g], G.link0[g]};
+}
+__host__ __device__ __forceinline__ GroupPos group_for_rhs(const Groups& G, int rhs)
+{
+    if (G.n == 1) return group_fields(G, 0);
+    return group_fields(G, group_of_rhs(G, rhs));
+}
+
+// The group of block lb of a launch over several lattices whose group g gives each of its
+// right-hand sides bpr(Nt[g] * Nx[g]) consecutive blocks, the groups one after another.
+// Returns the group's fields, the group's first block and its blocks per right-hand side.
+// The table is the context's device copy (one kernel argument, read through the scalar
+// cache), so single-lattice launches keep their small argument structs.
+template <typename F>
+__device__ __forceinline__ GroupPos block_group(const Groups* G, int lb, F bpr, int& blk0, int& per_rhs)
+{
+    const int n = G->n;
+    int start = 0, g = n - 1, nb = 0;
+    for (int k = 0; k < n; ++k) {
+        nb = bpr(G->Nt[k] * G->Nx[k]);
+        const int cnt = (G->rhs0[k + 1] - G->rhs0[k]) * nb;
+        if (lb < start + cnt) {
+            g = k;
+            break;
+        }
+        start += cnt;
+    }
+    blk0 = start;
+    per_rhs = nb;
+    return group_fields(*G, g);
 }
 
 // Extent and offset (in sites) of right-hand side rhs's vector in a batch over groups;
@@ -93,10 +116,19 @@ struct VecLayout {
     Groups gr;
     int Ls;
     int shift;
+    const Groups* dev;  // the context's device copy of gr (several groups), for the hot kernels
     __host__ __device__ long long sites(int g) const { return ((long long)Ls * gr.Nt[g] * gr.Nx[g]) >> shift; }
     __host__ __device__ long long offset(int g, int rhs) const
     {
         return (gr.site0[g] + (long long)(rhs - gr.rhs0[g]) * Ls * gr.Nt[g] * gr.Nx[g]) >> shift;
+    }
+    // (extent, offset) of right-hand side rhs, with the one-group fast path
+    __host__ __device__ void locate(int rhs, long long& n, long long& off) const
+    {
+        const GroupPos p = group_for_rhs(gr, rhs);
+        const long long v5 = (long long)Ls * p.Nt * p.Nx;
+        n = v5 >> shift;
+        off = (p.site0 + (long long)(rhs - p.rhs0) * v5) >> shift;
     }
     __host__ __device__ long long total_sites() const
     {
@@ -156,8 +188,6 @@ enum Epi : int {
 template <typename R> struct HopArgs {
     Lat L;               // the lattice of a single-group launch (Ls, masses and signs of every group)
     HopCfg cfg;          // launch shape; with several groups: linear blocks, bx per group in gb
-    Groups gr;           // right-hand sides by lattice (n = 0: one group from L, nrhs and rhs_per_cfg)
-    GroupBlocks gb;      // set by launch_hop
     const void* links;   // [ncfg][2][V4] complex R: (cos theta, sin theta)
     long long link_stride;  // complex entries per configuration (2 V4)
     int rhs_per_cfg;     // right-hand side r uses configuration r / rhs_per_cfg ...
@@ -195,16 +225,23 @@ template <typename R> struct HopArgs {
     const CgState* st;   // nullable
     int slot;
     int skip;            // skip a right-hand side whose flags[slot] & skip
+    // Several lattices (null for one): the context's device copy of the call's lattice
+    // groups, and the partial arrays' stride (the largest group's blocks per right-hand
+    // side; a group with fewer leaves the rest of its partials at the zero the solvers put
+    // there).  A multi-lattice launch uses linear blocks (cfg.tx = 0, no LDS tiles).
+    const Groups* groups;
+    int pstride;
 };
 
 // Host launchers (dwf_device.hip).
 void launch_pack_links(long long nlinks, const double* theta, double2* l64, float2* l32, hipStream_t s);
+// hg: the host copy of a.groups' table for a multi-lattice launch (it sizes the grid), else null.
 template <typename R>
-void launch_hop(bool dagger, int epi, const HopArgs<R>& a, int nrhs, hipStream_t s);
-// The single-group layout of nrhs vectors of L (shift 1: half-lattice vectors).
+void launch_hop(bool dagger, int epi, const HopArgs<R>& a, int nrhs, hipStream_t s, const Groups* hg = nullptr);
+// The single-group layout of nrhs vectors of L.
 Groups single_group(const Lat& L, int nrhs, int rhs_per_cfg);
 // Block partials per right-hand side of the direct-load stencil over groups (linear
-// blocks, cfg.chunks s-chunks), i.e. GroupBlocks::pstride of its launches.
+// blocks, cfg.chunks s-chunks): HopArgs::pstride of its multi-lattice launches.
 int hop_pstride(const Groups& G, const HopCfg& cfg);
 template <typename R>
 void launch_cg_p(const VecLayout& lay, int nblk, R* r, R* p, const double* partA, const double* partB, CgState* st,
@@ -214,8 +251,8 @@ void launch_ru_commit(int nblk, const double* partR, CgState* st, int slot, int 
 void launch_true_resid(int nblk, const double* partR, CgState* st, int nrhs, hipStream_t s);
 void launch_accum_x(const VecLayout& lay, double* x64, float* x32, const CgState* st, int slot, int skip, int nrhs,
                     hipStream_t s);
-// Marks F_HOLD on every right-hand side neither done nor frozen (all = 0), or on none.
-void launch_hold(CgState* st, int slot, int nrhs, hipStream_t s);
+// Marks F_HOLD on the right-hand sides with hold[i] != 0 (device array) that are not done.
+void launch_hold(CgState* st, const int* hold, int slot, int nrhs, hipStream_t s);
 template <typename R> void launch_zero(R* p, long long n, hipStream_t s);
 void launch_set_frozen_all(CgState* st, int slot, int nrhs, hipStream_t s);
 

@@ -230,9 +230,10 @@ __global__ void k_norms(const double* v, const VecLayout lay, double* out)
 {
     __shared__ double sh[8];
     const int rhs = blockIdx.x;
-    const int g = group_of_rhs(lay.gr, rhs);
-    const double* a = v + lay.offset(g, rhs) * 4;
-    const long long n_per = lay.sites(g) * 4;
+    long long n, off;
+    lay.locate(rhs, n, off);
+    const double* a = v + off * 4;
+    const long long n_per = n * 4;
     double s = 0.0;
     for (long long i = threadIdx.x; i < n_per; i += blockDim.x) s += a[i] * a[i];
     s = bsum(s, sh);
@@ -270,7 +271,6 @@ int measure_impl(dwf_ctx* c, const dwf_measure_spec* spec, int ncfg, const unsig
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_measure: no gauge field set");
     if (c->kind != 0) throw DwfError(DWF_ERR_ARG, "dwf_measure: needs a domain-wall context");
     if (ncfg != c->ncfg) throw DwfError(DWF_ERR_ARG, "dwf_measure: ncfg differs from the configurations held");
-    const int Ls = c->L.Ls;
     // per configuration lattice (configuration-major over the lattice groups)
     std::vector<int> cNt(ncfg), cNx(ncfg);
     if (c->multi()) {

@@ -56,6 +56,7 @@ void sig_ctx(const dwf_ctx* c, Sig& k)
     if (c->multi()) {  // the lattice-group layout the kernels receive
         const Groups& G = c->gr;
         k.push_back(0x6705ULL + (unsigned long long)G.n);
+        k.push_back(sig_ptr(c->gr_dev));
         for (int g = 0; g < G.n; ++g)
             k.insert(k.end(), {(unsigned long long)G.Nt[g], (unsigned long long)G.Nx[g], (unsigned long long)G.rhs0[g],
                                (unsigned long long)G.rpc[g], (unsigned long long)G.site0[g],
@@ -87,7 +88,7 @@ struct FullOp : CgOperator {
         a.st = st;
         a.slot = slot;
         a.skip = skip;
-        launch_hop<R>(first_dag, EPI_NORM, a, nrhs, c->stream);
+        launch_hop<R>(first_dag, EPI_NORM, a, nrhs, c->stream, c->groups_host());
     }
     template <typename R>
     void second_t(const R* t, R* x, R* r, const R* p, const double* partA, double* part, const CgState* st, int slot,
@@ -103,7 +104,7 @@ struct FullOp : CgOperator {
         a.st = st;
         a.slot = slot;
         a.skip = skip;
-        launch_hop<R>(!first_dag, EPI_CGUPD, a, nrhs, c->stream);
+        launch_hop<R>(!first_dag, EPI_CGUPD, a, nrhs, c->stream, c->groups_host());
     }
     void first(bool f64, const void* p, void* t, double* part, const CgState* st, int slot, int skip) override
     {
@@ -129,7 +130,7 @@ struct FullOp : CgOperator {
         a.st = st;
         a.slot = slot;
         a.skip = skip;
-        launch_hop<double>(first_dag, EPI_STORE, a, nrhs, c->stream);
+        launch_hop<double>(first_dag, EPI_STORE, a, nrhs, c->stream, c->groups_host());
         HopArgs<double> e = args<double>();
         e.in = t64;
         e.b = b;
@@ -141,7 +142,7 @@ struct FullOp : CgOperator {
         e.st = st;
         e.slot = slot;
         e.skip = skip;
-        launch_hop<double>(!first_dag, EPI_RESID, e, nrhs, c->stream);
+        launch_hop<double>(!first_dag, EPI_RESID, e, nrhs, c->stream, c->groups_host());
     }
     void signature(Sig& k) const override
     {
@@ -184,7 +185,11 @@ struct SchurOp : CgOperator {
     {
         EoArgs<R> a{};
         a.L = c->L;
-        if (c->multi()) a.gr = c->gr;  // else one group of L, filled in by the launchers
+        a.Nxh = c->L.Nx / 2;
+        a.V4h = c->L.V4 / 2;
+        a.bx = (a.V4h + kBlock - 1) / kBlock;
+        a.bxa = (a.V4h + kBlockA - 1) / kBlockA;
+        a.groups = c->multi() ? c->gr_dev : nullptr;  // then each group's sizes replace these
         a.chunks = chunks;
         a.spt = spt;
         a.remap = c->xcd_remap;
@@ -204,7 +209,7 @@ struct SchurOp : CgOperator {
         a.st = st;
         a.slot = slot;
         a.skip = skip;
-        launch_eo_a<R>(dag, true, a, nrhs, c->stream);  // te = A^-1 H_eo in   (A^-dag H^dag when dag)
+        launch_eo_a<R>(dag, true, a, nrhs, c->stream, c->groups_host());  // te = A^-1 H_eo in   (A^-dag H^dag when dag)
         EoArgs<R> b = base<R>(1);
         b.in = in;
         b.ca = (R)1;
@@ -221,12 +226,12 @@ struct SchurOp : CgOperator {
             EoArgs<double> b = apply_args<double>(false, (const double*)p, te64, st, slot, skip);
             b.out = (double*)t;
             b.part = part;
-            launch_eo_b<double>(false, true, EPI_NORM, b, nrhs, c->stream);
+            launch_eo_b<double>(false, true, EPI_NORM, b, nrhs, c->stream, c->groups_host());
         } else {
             EoArgs<float> b = apply_args<float>(false, (const float*)p, te32, st, slot, skip);
             b.out = (float*)t;
             b.part = part;
-            launch_eo_b<float>(false, true, EPI_NORM, b, nrhs, c->stream);
+            launch_eo_b<float>(false, true, EPI_NORM, b, nrhs, c->stream, c->groups_host());
         }
     }
     void second(bool f64, const void* t, void* x, void* r, const void* p, const double* partA, double* part,
@@ -239,7 +244,7 @@ struct SchurOp : CgOperator {
             b.p = (const double*)p;
             b.partA = partA;
             b.part = part;
-            launch_eo_b<double>(true, true, EPI_CGUPD, b, nrhs, c->stream);
+            launch_eo_b<double>(true, true, EPI_CGUPD, b, nrhs, c->stream, c->groups_host());
         } else {
             EoArgs<float> b = apply_args<float>(true, (const float*)t, te32, st, slot, skip);
             b.x = (float*)x;
@@ -247,7 +252,7 @@ struct SchurOp : CgOperator {
             b.p = (const float*)p;
             b.partA = partA;
             b.part = part;
-            launch_eo_b<float>(true, true, EPI_CGUPD, b, nrhs, c->stream);
+            launch_eo_b<float>(true, true, EPI_CGUPD, b, nrhs, c->stream, c->groups_host());
         }
     }
     void residual(const double* x, const double* b64, double* r64, double* p64, float* r32, float* p32, double* part,
@@ -255,7 +260,7 @@ struct SchurOp : CgOperator {
     {
         EoArgs<double> b = apply_args<double>(false, x, te64, st, slot, skip);
         b.out = q64;
-        launch_eo_b<double>(false, true, EPI_STORE, b, nrhs, c->stream);
+        launch_eo_b<double>(false, true, EPI_STORE, b, nrhs, c->stream, c->groups_host());
         EoArgs<double> e = apply_args<double>(true, q64, te64, st, slot, skip);
         e.b = b64;
         e.r64 = r64;
@@ -263,7 +268,7 @@ struct SchurOp : CgOperator {
         e.r32 = r32;
         e.p32 = p32;
         e.part = part;
-        launch_eo_b<double>(true, true, EPI_RESID, e, nrhs, c->stream);
+        launch_eo_b<double>(true, true, EPI_RESID, e, nrhs, c->stream, c->groups_host());
     }
     void signature(Sig& k) const override
     {
@@ -301,10 +306,11 @@ __global__ void k_axpy_masked(double* x, const double* d, const VecLayout lay, c
 {
     const int rhs = blockIdx.y;
     if (pre && (pre[rhs].flags[0] & F_DONE)) return;
-    const int g = group_of_rhs(lay.gr, rhs);
-    const long long n_per = lay.sites(g) * 4;
-    double* xr = x + lay.offset(g, rhs) * 4;
-    const double* dr = d + lay.offset(g, rhs) * 4;
+    long long n, off;
+    lay.locate(rhs, n, off);
+    const long long n_per = n * 4;
+    double* xr = x + off * 4;
+    const double* dr = d + off * 4;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_per; i += (long long)gridDim.x * blockDim.x)
         xr[i] += dr[i];
 }
@@ -365,8 +371,11 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     // cycle, and the length of its last cycle
     std::vector<double> c0_rel(nrhs);
     std::vector<int> c0_it(nrhs, 0), cycle_len(nrhs, 0);
-    std::vector<char> was_frozen(nrhs, 0);
+    std::vector<char> upd(nrhs, 0);
+    std::vector<int> hold_h(mixed ? nrhs : 0);
+    int* hold_d = mixed ? (int*)c->buf(B_HOLD, sizeof(int) * nrhs) : nullptr;
     for (int i = 0; i < nrhs; ++i) c0_rel[i] = c->st_host[i].relres;
+    std::vector<double> prev_rel(c0_rel);  // residual at the start of the previous cycle
     const int mode = mixed ? MODE_MIXED : MODE_CG64;
     auto iteration = [&](int i) {
         const int slot = i & 1, skip = F_DONE | F_FROZEN;
@@ -402,37 +411,81 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
                 c->check(hipGraphLaunch(block, s), "hipGraphLaunch");
         for (; j < n; ++j, ++it) iteration(it);
         c->check_launch("CG iteration");
-        if (mixed) {
-            // Each right-hand side whose FP32 cycle ended (frozen: its residual fell by delta or
-            // met the tolerance) gets its reliable update: x64 += x32, FP64 true residual,
-            // certify or start the next cycle from it.  The others keep iterating (F_HOLD).
+        if (mixed && op.lay.gr.n == 1) {
+            // One lattice: the right-hand sides converge at similar rates, so every one not
+            // done gets its reliable update (x64 += x32, FP64 true residual, certify or start
+            // the next cycle from it) at each chunk end, which keeps their cycles in step;
+            // the next chunk is the fewest iterations any of them needs to shrink its true
+            // residual by delta (or reach the tolerance) at its last cycle's rate.
+            const int slot = it & 1;
+            launch_accum_x(op.lay, x, x32, st, slot, F_DONE, nrhs, s);
+            op.residual(x, b, r64, nullptr, r32, nullptr, partR, st, slot, F_DONE);
+            launch_ru_commit(op.nblk, partR, st, slot, nrhs, s);
+            c->check_launch("reliable update");
+            read_states(c, st, nrhs);
+            double best = 1e30;
+            for (int i = 0; i < nrhs; ++i) {
+                const CgState& S = c->st_host[i];
+                if (S.flags[slot] & F_DONE) continue;
+                const int di = S.iters - c0_it[i];
+                if (di > 0 && S.true_relres > 0.0 && c0_rel[i] > 0.0 && S.true_relres < c0_rel[i]) {
+                    const double rate = std::log(S.true_relres / c0_rel[i]) / di;
+                    const double need_delta = std::log(delta) / rate;
+                    const double need_tol = std::log(rtol / S.true_relres) / rate + 2.0;
+                    best = std::min(best, std::min(need_delta, need_tol));
+                }
+                c0_rel[i] = S.true_relres;
+                c0_it[i] = S.iters;
+            }
+            if (best < 1e29) chunk = std::max(8, std::min(1024, (int)std::ceil(best)));
+        } else if (mixed) {
+            // Several lattices converge at different rates: per right-hand side reliable
+            // updates (x64 += x32, FP64 true residual, certify or start the next
+            // cycle from it) for the right-hand sides whose FP32 cycle ended (frozen: the
+            // residual fell by delta or met the tolerance), and for those past half of their
+            // expected cycle: they would freeze soon and, updated together, stay in step, so
+            // one chunk end serves them all.  The others keep iterating (F_HOLD).
             const int slot = it & 1;
             read_states(c, st, nrhs);
-            int frozen = 0;
+            int frozen = 0, updated = 0, running = 0;
             bool stalled = false;
-            double best = 1e30;  // iterations until the next right-hand side freezes
+            std::vector<double> rem(nrhs, -1.0);  // iterations until each running one freezes (-1: unknown)
             for (int i = 0; i < nrhs; ++i) {
                 const CgState& S = c->st_host[i];
                 const int f = S.flags[slot];
-                was_frozen[i] = 0;
+                upd[i] = 0;
                 if (f & F_DONE) continue;
                 const int di = S.iters - c0_it[i];
                 if (f & F_FROZEN) {
                     ++frozen;
-                    was_frozen[i] = 1;
+                    upd[i] = 1;
                     cycle_len[i] = di;
                     continue;
                 }
+                ++running;
                 if (di > 0 && S.relres > 0.0 && S.relres < c0_rel[i]) {
                     const double rate = std::log(S.relres / c0_rel[i]) / di;
                     const double target = std::max(delta * std::sqrt(S.rr_ref / S.bnorm2), S.tol);
-                    best = std::min(best, std::log(target / S.relres) / rate + 1.0);
+                    rem[i] = std::max(0.0, std::log(target / S.relres) / rate);
                 } else if (di >= 64) {
                     stalled = true;  // no progress in a whole chunk: refresh it from the FP64 residual
                 }
             }
             if (frozen > 0 || stalled) {
-                if (!stalled) launch_hold(st, slot, nrhs, s);
+                for (int i = 0; i < nrhs; ++i) {
+                    const CgState& S = c->st_host[i];
+                    if ((S.flags[slot] & (F_DONE | F_FROZEN)) || upd[i]) continue;
+                    const int di = S.iters - c0_it[i];
+                    // past half of its cycle (remaining at most what it has done), or stalled
+                    if (stalled || (rem[i] >= 0.0 && rem[i] <= di)) {
+                        upd[i] = 1;
+                        cycle_len[i] = (int)std::ceil(di + rem[i]);
+                    }
+                }
+                for (int i = 0; i < nrhs; ++i) hold_h[i] = upd[i] ? 0 : 1;
+                c->check(hipMemcpyAsync(hold_d, hold_h.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, s),
+                         "hold upload");
+                launch_hold(st, hold_d, slot, nrhs, s);
                 const int skip = F_DONE | F_HOLD;
                 launch_accum_x(op.lay, x, x32, st, slot, skip, nrhs, s);
                 op.residual(x, b, r64, nullptr, r32, nullptr, partR, st, slot, skip);
@@ -441,19 +494,34 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
                 read_states(c, st, nrhs);
                 for (int i = 0; i < nrhs; ++i) {
                     const CgState& S = c->st_host[i];
-                    if ((S.flags[slot] & F_DONE) || !(was_frozen[i] || stalled)) continue;
-                    // a new cycle from the FP64 residual, about as long as the last one (or
-                    // shorter if the tolerance is near)
-                    double need = cycle_len[i] > 0 ? cycle_len[i] : 16;
-                    if (cycle_len[i] > 0 && c0_rel[i] > 0.0 && S.true_relres > 0.0 && S.true_relres < c0_rel[i]) {
-                        const double rate = std::log(S.true_relres / c0_rel[i]) / cycle_len[i];
-                        need = std::min(need, std::log(S.tol / S.true_relres) / rate + 1.0);
-                    }
-                    best = std::min(best, need);
+                    if (!upd[i]) continue;
+                    ++updated;
                     c0_rel[i] = S.true_relres;
                     c0_it[i] = S.iters;
                 }
             }
+            // next chunk end: the first expected freeze -- a held one's remaining
+            // iterations, or a restarted one's cycle (as long as its last one, or shorter
+            // if the tolerance is near)
+            double best = 1e30;
+            for (int i = 0; i < nrhs; ++i) {
+                const CgState& S = c->st_host[i];
+                if (S.flags[slot] & F_DONE) continue;
+                if (upd[i]) {
+                    double need = cycle_len[i] > 0 ? cycle_len[i] : 16;
+                    if (cycle_len[i] > 0 && prev_rel[i] > 0.0 && S.true_relres > 0.0 && S.true_relres < prev_rel[i]) {
+                        const double rate = std::log(S.true_relres / prev_rel[i]) / cycle_len[i];
+                        need = std::min(need, std::log(S.tol / S.true_relres) / rate + 1.0);
+                    }
+                    best = std::min(best, need);
+                } else if (rem[i] >= 0.0) {
+                    best = std::min(best, rem[i] + 1.0);
+                }
+            }
+            for (int i = 0; i < nrhs; ++i)
+                if (upd[i]) prev_rel[i] = c->st_host[i].true_relres;
+            (void)updated;
+            (void)running;
             chunk = best < 1e29 ? std::max(8, std::min(1024, (int)std::ceil(best))) : 16;
         } else {
             read_states(c, st, nrhs);
@@ -640,7 +708,7 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
         HopArgs<double> a = full.args<double>();
         a.in = src;
         a.out = B;
-        launch_hop<double>(true, EPI_STORE, a, nrhs, s);
+        launch_hop<double>(true, EPI_STORE, a, nrhs, s, c->groups_host());
     }
     if (x0) {
         if (x0 != x) c->check(hipMemcpyAsync(x, x0, nvf * 8, hipMemcpyDeviceToDevice, s), "copy x0");
@@ -660,7 +728,7 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
             HopArgs<double> a = full.args<double>();
             a.in = x;
             a.out = t64;
-            launch_hop<double>(false, EPI_STORE, a, nrhs, s);
+            launch_hop<double>(false, EPI_STORE, a, nrhs, s, c->groups_host());
             k_sub_into<<<2048, 256, 0, s>>>(src, t64, rsrc, (long long)nvf);
             r_in = rsrc;
         }
@@ -671,17 +739,17 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
             a.b = be;
             a.cb = 1.0;
             a.out = ue;
-            launch_eo_a<double>(false, true, a, nrhs, s);
+            launch_eo_a<double>(false, true, a, nrhs, s, c->groups_host());
             EoArgs<double> b = schur.base<double>(1);
             b.in = bo;
             b.ca = 1.0;
             b.hin = ue;
             b.ch = 0.5;
             b.out = bh;
-            launch_eo_b<double>(false, false, EPI_STORE, b, nrhs, s);
+            launch_eo_b<double>(false, false, EPI_STORE, b, nrhs, s, c->groups_host());
             EoArgs<double> d = schur.apply_args<double>(true, bh, ue, nullptr, 0, 0);
             d.out = Bh;
-            launch_eo_b<double>(true, true, EPI_STORE, d, nrhs, s);
+            launch_eo_b<double>(true, true, EPI_STORE, d, nrhs, s, c->groups_host());
         }
         c->check_launch("Schur right-hand side");
         const double rtol_pass = std::min(0.5, 0.25 * rtol / std::max(worst, 1e-300));
@@ -700,7 +768,7 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
             a.hin = xo;
             a.ch = 0.5;
             a.out = xe;
-            launch_eo_a<double>(false, true, a, nrhs, s);
+            launch_eo_a<double>(false, true, a, nrhs, s, c->groups_host());
         }
         launch_eo_merge(layf, xe, xo, delta, nrhs, s);
         k_axpy_masked<<<dim3(1024, nrhs), 256, 0, s>>>(x, delta, layf, have_cert ? cert : nullptr);
@@ -766,7 +834,7 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
         HopArgs<double> a = ctx_hop_args<double>(c, ctx_hop_cfg(c, nrhs), nrhs);
         a.in = src;
         a.out = B;
-        launch_hop<double>(true, EPI_STORE, a, nrhs, c->stream);
+        launch_hop<double>(true, EPI_STORE, a, nrhs, c->stream, c->groups_host());
     }
     if (method == DWF_SOLVE_LEARNED) return solve_learned(c, DWF_OP_DDAGD, B, x0, rtol, maxiter, nrhs, x, stats);
     return solve_normal(c, DWF_OP_DDAGD, method, B, x0, rtol, maxiter, nrhs, x, stats);
