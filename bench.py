@@ -681,8 +681,9 @@ def main():
                     help="oracle CG iterations timed for the HMC workload's CPU baseline")
     ap.add_argument("--cpu-solves", type=int, default=1,
                     help="whole oracle solves timed for the solves workload's CPU baseline")
-    ap.add_argument("--cpu-rhs", type=int, default=34,
-                    help="analyzer CPU baseline: sources per configuration solved (34 = all)")
+    ap.add_argument("--cpu-rhs", type=int, default=8,
+                    help="analyzer CPU baseline: sources per configuration solved (34 = all; 8 x 4 sizes "
+                         "= two solves per thread of 16, about 30 s)")
     ap.add_argument("--analyzer-split", action="store_true",
                     help="analyzer: one context per lattice size on concurrent streams instead of one launch")
     ap.add_argument("--no-cpu", action="store_true")
