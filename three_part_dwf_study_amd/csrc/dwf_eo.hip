@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 namespace dwfk {
 namespace {
@@ -176,15 +177,17 @@ __device__ __forceinline__ EoPos<R> eo_pos(const EoArgs<R>& a, int lb, int per, 
     return q;
 }
 
-// o += c * H psi at one slice (H: the bracket of D's hop, or D^dag's when DAG).
-template <typename R, bool DAG>
-__device__ __forceinline__ void add_hop(Sp<R>& o, R c, const EoSite<R>& g, const R* in, long long sl)
+// o += c * H psi at one slice (H: the bracket of D's hop, or D^dag's when DAG), from the
+// values of psi at the site's +t, -t, +x, -x neighbours.
+// SIGNS false: the boundary signs are already folded into the links (fold_signs).
+template <typename R, bool DAG, bool SIGNS = true>
+__device__ __forceinline__ void hop_sum(Sp<R>& o, R c, const EoSite<R>& g, const Sp<R>& vtp, const Sp<R>& vtm,
+                                        const Sp<R>& vxp, const Sp<R>& vxm)
 {
     constexpr R sg = DAG ? (R)-1 : (R)1;
-    const Sp<R> vtp = ld(in, sl + g.ntp), vtm = ld(in, sl + g.ntm);
-    const Sp<R> vxp = ld(in, sl + g.nxp), vxm = ld(in, sl + g.nxm);
+    auto sgn = [](R s, R v) { return SIGNS ? s * v : v; };
     {
-        const R ar = g.stp * (vtp.ar - sg * vtp.br), ai = g.stp * (vtp.ai - sg * vtp.bi);
+        const R ar = sgn(g.stp, vtp.ar - sg * vtp.br), ai = sgn(g.stp, vtp.ai - sg * vtp.bi);
         const R hr = g.ut.x * ar - g.ut.y * ai, hi = g.ut.x * ai + g.ut.y * ar;
         o.ar += c * hr;
         o.ai += c * hi;
@@ -192,7 +195,7 @@ __device__ __forceinline__ void add_hop(Sp<R>& o, R c, const EoSite<R>& g, const
         o.bi -= c * sg * hi;
     }
     {
-        const R br = g.stm * (vtm.ar + sg * vtm.br), bi = g.stm * (vtm.ai + sg * vtm.bi);
+        const R br = sgn(g.stm, vtm.ar + sg * vtm.br), bi = sgn(g.stm, vtm.ai + sg * vtm.bi);
         const R hr = g.utm.x * br + g.utm.y * bi, hi = g.utm.x * bi - g.utm.y * br;
         o.ar += c * hr;
         o.ai += c * hi;
@@ -200,7 +203,7 @@ __device__ __forceinline__ void add_hop(Sp<R>& o, R c, const EoSite<R>& g, const
         o.bi += c * sg * hi;
     }
     {
-        const R cr = g.sxp * (vxp.ar - sg * vxp.bi), ci = g.sxp * (vxp.ai + sg * vxp.br);
+        const R cr = sgn(g.sxp, vxp.ar - sg * vxp.bi), ci = sgn(g.sxp, vxp.ai + sg * vxp.br);
         const R hr = g.ux.x * cr - g.ux.y * ci, hi = g.ux.x * ci + g.ux.y * cr;
         o.ar += c * hr;
         o.ai += c * hi;
@@ -208,13 +211,32 @@ __device__ __forceinline__ void add_hop(Sp<R>& o, R c, const EoSite<R>& g, const
         o.bi -= c * sg * hr;
     }
     {
-        const R dr = g.sxm * (vxm.ar + sg * vxm.bi), di = g.sxm * (vxm.ai - sg * vxm.br);
+        const R dr = sgn(g.sxm, vxm.ar + sg * vxm.bi), di = sgn(g.sxm, vxm.ai - sg * vxm.br);
         const R hr = g.uxm.x * dr + g.uxm.y * di, hi = g.uxm.x * di - g.uxm.y * dr;
         o.ar += c * hr;
         o.ai += c * hi;
         o.br -= c * sg * hi;
         o.bi += c * sg * hr;
     }
+}
+
+// Multiplies the links by the boundary signs (+-1, so exactly), for hop_sum<..., false>.
+template <typename R> __device__ __forceinline__ void fold_signs(EoSite<R>& g)
+{
+    g.ut.x *= g.stp;
+    g.ut.y *= g.stp;
+    g.utm.x *= g.stm;
+    g.utm.y *= g.stm;
+    g.ux.x *= g.sxp;
+    g.ux.y *= g.sxp;
+    g.uxm.x *= g.sxm;
+    g.uxm.y *= g.sxm;
+}
+
+template <typename R, bool DAG>
+__device__ __forceinline__ void add_hop(Sp<R>& o, R c, const EoSite<R>& g, const R* in, long long sl)
+{
+    hop_sum<R, DAG>(o, c, g, ld(in, sl + g.ntp), ld(in, sl + g.ntm), ld(in, sl + g.nxp), ld(in, sl + g.nxm));
 }
 
 // In-register solve along s.  LOWER: d y_s - y_{s-1} = v_s (s>0), d y_0 + mf y_{L-1} = v_0.
@@ -269,6 +291,42 @@ __device__ __forceinline__ void wall_solve(R (&vr)[LS], R (&vi)[LS], R d, R mf)
     }
 }
 
+// wall_solve on one spin component kept in LDS, (re, im) of slice s at col[s * cs],
+// col[s * cs + 1]: the same operations in the same order, streamed through LDS so only
+// the recurrence's carries live in registers.
+template <typename R, int LS, bool UPPER>
+__device__ __forceinline__ void wall_solve_col(R* col, int cs, R d, R mf)
+{
+    const R inv = (R)1 / d;
+    R dpow = (R)1;  // d^{-(L-1)}
+#pragma unroll
+    for (int k = 1; k < LS; ++k) dpow *= inv;
+    const R den = (R)1 / (d + mf * dpow);
+    // lower: slices 1 .. L-1 upwards, wall slice 0; upper: slices L-2 .. 0 downwards, wall L-1
+    auto at = [&](int k) { return UPPER ? LS - 1 - k : k; };
+    R br = 0, bi = 0;
+#pragma unroll 4
+    for (int k = 1; k < LS; ++k) {
+        R* c = col + at(k) * cs;
+        br = (c[0] + br) * inv;
+        bi = (c[1] + bi) * inv;
+        c[0] = br;
+        c[1] = bi;
+    }
+    R* cw = col + at(0) * cs;
+    const R ywr = (cw[0] - mf * br) * den, ywi = (cw[1] - mf * bi) * den;
+    cw[0] = ywr;
+    cw[1] = ywi;
+    R a = (R)1;
+#pragma unroll 4
+    for (int k = 1; k < LS; ++k) {
+        a *= inv;
+        R* c = col + at(k) * cs;
+        c[0] += a * ywr;
+        c[1] += a * ywi;
+    }
+}
+
 // out = [A^-1 | A^-dag | 1] ( cb b + ch H psi_other ) on the target parity, whole
 // s-column per thread.  b and the hop are optional (cb == 0 / ch == 0 skip them).
 template <typename R, bool DAG, int LS, bool AINV>
@@ -308,6 +366,49 @@ __global__ __launch_bounds__(kBlockA) void k_eo_a(const EoArgs<R> a)
     }
 #pragma unroll
     for (int s = 0; s < LS; ++s) st(a.out + rb, (long long)s * V4h + hs, Sp<R>{v0r[s], v0i[s], v1r[s], v1i[s]});
+}
+
+// The per-site epilogue of the even-odd operator kernels: o is the operator's value at
+// index idx (within right-hand side offset rb) of the target parity.
+template <typename R, int EPI>
+__device__ __forceinline__ void eo_epi(const EoArgs<R>& a, long long rb, long long idx, const Sp<R>& o, R alpha,
+                                       double& acc0, double& acc1)
+{
+    if constexpr (EPI == EPI_STORE) {
+        st(a.out + rb, idx, o);
+    } else if constexpr (EPI == EPI_NORM) {
+        st(a.out + rb, idx, o);
+        acc0 += n2(o);
+    } else if constexpr (EPI == EPI_CGUPD) {
+        const Sp<R> pv = ld(a.p + rb, idx);
+        Sp<R> xv = ld(a.x + rb, idx);
+        Sp<R> rv = ld(a.r + rb, idx);
+        xv.ar += alpha * pv.ar;
+        xv.ai += alpha * pv.ai;
+        xv.br += alpha * pv.br;
+        xv.bi += alpha * pv.bi;
+        rv.ar -= alpha * o.ar;
+        rv.ai -= alpha * o.ai;
+        rv.br -= alpha * o.br;
+        rv.bi -= alpha * o.bi;
+        st(a.x + rb, idx, xv);
+        st(a.r + rb, idx, rv);
+        acc0 += n2(rv);
+    } else {
+        if constexpr (sizeof(R) == 8) {
+            const Sp<double> bv = ld(a.b + rb, idx);
+            const Sp<double> rv{bv.ar - o.ar, bv.ai - o.ai, bv.br - o.br, bv.bi - o.bi};
+            st(a.r64 + rb, idx, rv);
+            if (a.p64) st(a.p64 + rb, idx, rv);
+            if (a.r32 || a.p32) {
+                const Sp<float> fv{(float)rv.ar, (float)rv.ai, (float)rv.br, (float)rv.bi};
+                if (a.r32) st(a.r32 + rb, idx, fv);
+                if (a.p32) st(a.p32 + rb, idx, fv);
+            }
+            acc0 += n2(rv);
+            acc1 += n2(bv);
+        }
+    }
 }
 
 // out = ca [A | A^dag | 1] psi + ch H hin on the target parity, with the CG epilogues
@@ -372,42 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
             o.br *= a.ca;
             o.bi *= a.ca;
             add_hop<R, DAG>(o, a.ch, g, a.hin + rb, sl);
-            const long long idx = sl + hs;
-            if constexpr (EPI == EPI_STORE) {
-                st(a.out + rb, idx, o);
-            } else if constexpr (EPI == EPI_NORM) {
-                st(a.out + rb, idx, o);
-                acc0 += n2(o);
-            } else if constexpr (EPI == EPI_CGUPD) {
-                const Sp<R> pv = ld(a.p + rb, idx);
-                Sp<R> xv = ld(a.x + rb, idx);
-                Sp<R> rv = ld(a.r + rb, idx);
-                xv.ar += alpha * pv.ar;
-                xv.ai += alpha * pv.ai;
-                xv.br += alpha * pv.br;
-                xv.bi += alpha * pv.bi;
-                rv.ar -= alpha * o.ar;
-                rv.ai -= alpha * o.ai;
-                rv.br -= alpha * o.br;
-                rv.bi -= alpha * o.bi;
-                st(a.x + rb, idx, xv);
-                st(a.r + rb, idx, rv);
-                acc0 += n2(rv);
-            } else {
-                if constexpr (sizeof(R) == 8) {
-                    const Sp<double> bv = ld(a.b + rb, idx);
-                    const Sp<double> rv{bv.ar - o.ar, bv.ai - o.ai, bv.br - o.br, bv.bi - o.bi};
-                    st(a.r64 + rb, idx, rv);
-                    if (a.p64) st(a.p64 + rb, idx, rv);
-                    if (a.r32 || a.p32) {
-                        const Sp<float> fv{(float)rv.ar, (float)rv.ai, (float)rv.br, (float)rv.bi};
-                        if (a.r32) st(a.r32 + rb, idx, fv);
-                        if (a.p32) st(a.p32 + rb, idx, fv);
-                    }
-                    acc0 += n2(rv);
-                    acc1 += n2(bv);
-                }
-            }
+            eo_epi<R, EPI>(a, rb, sl + hs, o, alpha, acc0, acc1);
             if constexpr (AOP) {
                 dn = cur;
                 cur = up;
@@ -422,6 +488,216 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
             double* pp = a.part + ((size_t)rhs * pstride + bid) * 2;
             pp[0] = acc0;
             pp[1] = acc1;
+        }
+    }
+}
+
+// The Schur operator in one launch: out_o = [A | A^dag] psi_o + ch H w with
+// w = [A^-1 | A^-dag] H psi_o on the other parity, w never leaving the chip (the
+// two-kernel form writes w to device memory and reads it back: three vector passes more).
+//
+// A block owns a tile of ty x tx 4D sites of one right-hand side (ty, tx even, so a
+// site's parity is that of its coordinates in the tile; at most 64 target sites).
+//   1a  psi on the region of target-parity sites around the tile ([-2, ty+1] x
+//       [-2, tx+1]: every neighbour of the box below), all Ls slices, into LDS; two
+//       threads per region site, each issuing half of its slices' loads at once.
+//   1b  the hop H psi at every other-parity site of the box [-1, ty] x [-1, tx] (all of
+//       them neighbour a target site but two corners), two threads per site (half of
+//       the slices each; the hop is local to a slice), from LDS; each phase-2 thread
+//       also takes its target's psi window from LDS.  Then w overwrites psi in LDS.
+//   1c  A^-1 (or A^-dag) along s, one thread per box site and spin component (the two
+//       O(Ls) recurrences are independent), in place in LDS.
+//   2   wavefront c walks s-chunk c (Ls / 4 slices) of target site `lane`: the A term
+//       from its window in registers, the hop of w from LDS, k_eo_b's epilogue.
+// The boundary signs are folded into the links (exact).  The arithmetic otherwise
+// follows k_eo_a and k_eo_b; box or region sites that wrap around a small lattice are
+// computed twice, into two LDS entries.
+template <typename R, bool DAG, int LS, int EPI>
+__global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(const EoArgs<R> a, const EoTile tl)
+{
+    extern __shared__ __align__(16) unsigned char eo_smem[];
+    Sp<R>* w = reinterpret_cast<Sp<R>*>(eo_smem);  // [LS][nbox]
+    __shared__ double sh[8];
+    using C = typename C2T<R>::T;
+    const int total = gridDim.x;
+    const int lb = a.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
+    const int TY = tl.ty, TX = tl.tx;
+    Lat L = a.L;
+    int rhs, tile, pstride;
+    long long rb;
+    const C* links;
+    if (!a.groups) {
+        const int tiles = (L.Nt / TY) * (L.Nx / TX);
+        rhs = lb / tiles;
+        tile = lb - rhs * tiles;
+        pstride = tiles;
+        rb = (long long)rhs * (L.V4 / 2) * LS * 4;
+        links = static_cast<const C*>(a.links) + (long long)(rhs / a.rhs_per_cfg) * a.link_stride;
+    } else {
+        int blk0, per;
+        const GroupPos p = block_group(a.groups, lb, [TY, TX](int v4) { return v4 / (TY * TX); }, blk0, per);
+        const int local = lb - blk0, rl = local / per;
+        rhs = p.rhs0 + rl;
+        tile = local - rl * per;
+        pstride = a.pstride;
+        L.Nt = p.Nt;
+        L.Nx = p.Nx;
+        L.V4 = p.Nt * p.Nx;
+        rb = (p.site0 / 2 + (long long)rl * (L.V4 / 2) * LS) * 4;
+        links = static_cast<const C*>(a.links) + p.link0 + (long long)(rl / p.rpc) * 2 * L.V4;
+    }
+    if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
+    R alpha = 0;
+    if constexpr (EPI == EPI_CGUPD) {
+        const double pAp = sum_partials(a.partA + (size_t)rhs * pstride * 2, pstride, sh);
+        if (pAp == 0.0) return;
+        alpha = (R)(a.st[rhs].rr[a.slot] / pAp);
+    }
+    const int Nxh = L.Nx / 2, V4h = L.V4 / 2;
+    const int tiles_x = L.Nx / TX;
+    const int tyi = tile / tiles_x;
+    const int t0 = tyi * TY, x0 = (tile - tyi * tiles_x) * TX;
+    const int WB = (TX + 2) / 2, nbox = (TY + 2) * WB;
+    const int opar = 1 - a.tpar;
+    const R* in = a.in + rb;
+    const R d = (R)L.diag, mf = (R)L.mf;
+
+    // ---- phase 1a: psi on the region [-2, ty + 1] x [-2, tx + 1] of target-parity sites
+    // (every neighbour of the box), all slices, into LDS: two threads per region site,
+    // each loading half of its slices (independent loads, all in flight at once)
+    constexpr int LH = LS / 2;
+    const int WR = (TX + 4) / 2, nreg = (TY + 4) * WR;
+    Sp<R>* P = w;  // [LS][nreg], later overwritten by w [LS][nbox]
+    {
+        const int r = threadIdx.x >> 1, h = threadIdx.x & 1;
+        if (r < nreg) {
+            const int ir = r / WR, jr = 2 * (r - ir * WR) + ((ir + a.tpar) & 1);
+            int t = t0 + ir - 2, x = x0 + jr - 2;
+            t = t < 0 ? t + L.Nt : (t >= L.Nt ? t - L.Nt : t);
+            x = x < 0 ? x + L.Nx : (x >= L.Nx ? x - L.Nx : x);
+            const R* src = in + ((long long)h * LH * V4h + t * Nxh + (x >> 1)) * 4;
+            Sp<R> v[LH];
+#pragma unroll
+            for (int k = 0; k < LH; ++k) v[k] = ld(src, (long long)k * V4h);
+#pragma unroll
+            for (int k = 0; k < LH; ++k) P[(h * LH + k) * nreg + r] = v[k];
+        }
+    }
+    __syncthreads();
+    // ---- phase 1b: the hop H psi at the box's other-parity sites, two threads per site
+    // (half of the slices each; the hop is local to a slice)
+    const int eb = threadIdx.x >> 1, hb = threadIdx.x & 1;
+    Sp<R> hv[LH];
+    if (eb < nbox) {
+        const int ib = eb / WB, xb = 2 * (eb - ib * WB) + ((opar + ib) & 1);
+        int t = t0 + ib - 1, x = x0 + xb - 1;
+        t = t < 0 ? t + L.Nt : (t >= L.Nt ? t - L.Nt : t);
+        x = x < 0 ? x + L.Nx : (x >= L.Nx ? x - L.Nx : x);
+        EoSite<R> g = eo_site<R>(L, Nxh, opar, t * Nxh + (x >> 1), links);
+        fold_signs(g);
+        // the box site's neighbours in the region: +t, -t, +x, -x
+        const int rtp = (ib + 2) * WR + ((xb + 1) >> 1), rtm = ib * WR + ((xb + 1) >> 1);
+        const int rxp = (ib + 1) * WR + ((xb + 2) >> 1), rxm = (ib + 1) * WR + (xb >> 1);
+#pragma unroll
+        for (int k = 0; k < LH; ++k) {
+            const Sp<R>* Ps = P + (hb * LH + k) * nreg;
+            Sp<R> o{0, 0, 0, 0};
+            hop_sum<R, DAG, false>(o, (R)1, g, Ps[rtp], Ps[rtm], Ps[rxp], Ps[rxm]);
+            hv[k] = o;
+        }
+    }
+    // phase 2's thread: wavefront c walks s-chunk c (SPT slices) of target site k = lane;
+    // its window psi(s0 - 1 .. s0 + SPT) comes from the region in LDS
+    constexpr int SPT = LS / 4;
+    const int TXh = TX / 2, ntarg = TY * TXh;
+    const int k2 = threadIdx.x & 63, s0 = (threadIdx.x >> 6) * SPT;
+    const int i2 = k2 / TXh, j2 = 2 * (k2 - i2 * TXh) + ((i2 + a.tpar) & 1);
+    const bool act2 = k2 < ntarg;
+    Sp<R> win[SPT + 2];
+    if (act2) {
+        const int rt = (i2 + 2) * WR + ((j2 + 2) >> 1);
+#pragma unroll
+        for (int q = 0; q < SPT + 2; ++q) {
+            int sq = s0 - 1 + q;
+            sq = sq < 0 ? sq + LS : (sq >= LS ? sq - LS : sq);
+            win[q] = P[sq * nreg + rt];
+        }
+    }
+    __syncthreads();  // every read of P is done
+    if (eb < nbox) {
+#pragma unroll
+        for (int k = 0; k < LH; ++k) w[(hb * LH + k) * nbox + eb] = hv[k];
+    }
+    __syncthreads();
+    // ---- phase 1c: w = A^-1 (or A^-dag) of the hop, one thread per site and spin component
+    // (the two recurrences are independent); A: spin 0 lower, spin 1 upper, A^dag swapped
+    {
+        constexpr int kHalf = kBlockS / 2;
+        const int sp = threadIdx.x >= kHalf ? 1 : 0;  // wave-uniform
+        const int e = threadIdx.x - sp * kHalf;
+        if (e < nbox) {
+            R* col = reinterpret_cast<R*>(w) + e * 4 + 2 * sp;
+            if (sp == (DAG ? 1 : 0))
+                wall_solve_col<R, LS, false>(col, nbox * 4, d, mf);
+            else
+                wall_solve_col<R, LS, true>(col, nbox * 4, d, mf);
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: the operator on the tile's target sites (ca = 1)
+    double acc0 = 0.0, acc1 = 0.0;
+    if (act2) {
+        const int hs = (t0 + i2) * Nxh + ((x0 + j2) >> 1);
+        EoSite<R> g = eo_site<R>(L, Nxh, a.tpar, hs, links);
+        fold_signs(g);
+        const int etp = (i2 + 2) * WB + ((j2 + 1) >> 1), etm = i2 * WB + ((j2 + 1) >> 1);
+        const int exp = (i2 + 1) * WB + ((j2 + 2) >> 1), exm = (i2 + 1) * WB + (j2 >> 1);
+#pragma unroll
+        for (int q = 0; q < SPT; ++q) {
+            const int s = s0 + q;
+            const Sp<R>& cur = win[q + 1];
+            const Sp<R>& up = win[q + 2];
+            const Sp<R>& dn = win[q];
+            // A psi: d psi + D5 (D: spin1 <- s+1, spin0 <- s-1; D^dag swapped)
+            const R cup = (s + 1 == LS) ? mf : (R)-1;
+            const R cdn = (s == 0) ? mf : (R)-1;
+            Sp<R> o{d * cur.ar, d * cur.ai, d * cur.br, d * cur.bi};
+            if constexpr (!DAG) {
+                o.br += cup * up.br;
+                o.bi += cup * up.bi;
+                o.ar += cdn * dn.ar;
+                o.ai += cdn * dn.ai;
+            } else {
+                o.br += cdn * dn.br;
+                o.bi += cdn * dn.bi;
+                o.ar += cup * up.ar;
+                o.ai += cup * up.ai;
+            }
+            const Sp<R>* ws = w + s * nbox;
+            hop_sum<R, DAG, false>(o, a.ch, g, ws[etp], ws[etm], ws[exp], ws[exm]);
+            eo_epi<R, EPI>(a, rb, (long long)s * V4h + hs, o, alpha, acc0, acc1);
+        }
+    }
+    if constexpr (EPI != EPI_STORE) {
+        acc0 = wave_sum(acc0);
+        acc1 = wave_sum(acc1);
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        __syncthreads();  // sh may still hold sum_partials' value being read
+        if (lane == 0) {
+            sh[wv] = acc0;
+            sh[4 + wv] = acc1;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s0 = 0.0, s1 = 0.0;
+            for (int v = 0; v < kBlockS / 64; ++v) {
+                s0 += sh[v];
+                s1 += sh[4 + v];
+            }
+            double* pp = a.part + ((size_t)rhs * pstride + tile) * 2;
+            pp[0] = s0;
+            pp[1] = s1;
         }
     }
 }
@@ -562,6 +838,95 @@ void launch_eo_b(bool dagger, bool aop, int epi, const EoArgs<R>& a0, int nrhs, 
 }
 template void launch_eo_b<float>(bool, bool, int, const EoArgs<float>&, int, hipStream_t, const Groups*);
 template void launch_eo_b<double>(bool, bool, int, const EoArgs<double>&, int, hipStream_t, const Groups*);
+
+bool eo_fused_tile(const Groups& G, int Ls, EoTile& t)
+{
+    int ty = 0, tx = 0;
+    for (int c = 8; c >= 2 && !ty; c -= 2) {
+        bool ok = true;
+        for (int g = 0; g < G.n; ++g) ok = ok && G.Nt[g] % c == 0;
+        if (ok) ty = c;
+    }
+    for (int c = 16; c >= 2 && !tx; c -= 2) {
+        bool ok = true;
+        for (int g = 0; g < G.n; ++g) ok = ok && G.Nx[g] % c == 0;
+        if (ok) tx = c;
+    }
+    if (!ty || !tx || Ls < 1) return false;
+    if (Ls % 4 || ty * tx / 2 > 64) return false;
+    t.ty = ty;
+    t.tx = tx;
+    t.nch = 4;  // one wavefront per s-chunk, one lane per target site
+    t.spt = Ls / 4;
+    return true;
+}
+
+int eo_fused_pstride(const Groups& G, const EoTile& t)
+{
+    int m = 0;
+    for (int g = 0; g < G.n; ++g) m = std::max(m, G.Nt[g] * G.Nx[g] / (t.ty * t.tx));
+    return m;
+}
+
+namespace {
+template <typename R, bool DAG, int LS>
+void dispatch_schur(int epi, const EoArgs<R>& a, const EoTile& t, unsigned grid, size_t lds, hipStream_t s)
+{
+    switch (epi) {
+        case EPI_STORE: k_eo_schur<R, DAG, LS, EPI_STORE><<<grid, kBlockS, lds, s>>>(a, t); break;
+        case EPI_NORM: k_eo_schur<R, DAG, LS, EPI_NORM><<<grid, kBlockS, lds, s>>>(a, t); break;
+        case EPI_CGUPD: k_eo_schur<R, DAG, LS, EPI_CGUPD><<<grid, kBlockS, lds, s>>>(a, t); break;
+        default:
+            if constexpr (sizeof(R) == 8) k_eo_schur<R, DAG, LS, EPI_RESID><<<grid, kBlockS, lds, s>>>(a, t);
+            break;
+    }
+}
+}  // namespace
+
+template <typename R>
+void launch_eo_schur(bool dagger, int epi, const EoArgs<R>& a0, const EoTile& t, int nrhs, hipStream_t s,
+                     const Groups* hg)
+{
+    EoArgs<R> a = a0;
+    long long nb = 0;
+    if (!a.groups) {
+        if (a.L.Nt % t.ty || a.L.Nx % t.tx) throw DwfError(DWF_ERR_ARG, "fused even-odd launch: tile does not fit");
+        nb = (long long)(a.L.V4 / (t.ty * t.tx)) * nrhs;
+    } else {
+        if (!hg || hg->n < 1 || hg->n > kMaxGroups || hg->rhs0[hg->n] != nrhs)
+            throw DwfError(DWF_ERR_ARG, "fused even-odd launch: bad lattice-group layout");
+        for (int g = 0; g < hg->n; ++g) {
+            if (hg->Nt[g] % t.ty || hg->Nx[g] % t.tx) throw DwfError(DWF_ERR_ARG, "fused even-odd launch: tile does not fit");
+            nb += (long long)(hg->rhs0[g + 1] - hg->rhs0[g]) * (hg->Nt[g] * hg->Nx[g] / (t.ty * t.tx));
+        }
+        a.pstride = eo_fused_pstride(*hg, t);
+    }
+    if (nb <= 0 || nb > 0x7fffffffLL) throw DwfError(DWF_ERR_ARG, "fused even-odd launch: grid size out of range");
+    if (a.ca != (R)1) throw DwfError(DWF_ERR_ARG, "fused even-odd launch: the A term's coefficient must be 1");
+    if (t.ty < 2 || t.ty > 8 || (t.ty & 1) || t.tx < 2 || t.tx > 16 || (t.tx & 1) || t.nch < 1 || t.spt < 1 ||
+        t.nch != 4 || t.spt * 4 != a.L.Ls || t.ty * t.tx / 2 > 64 || (t.ty + 4) * ((t.tx + 4) / 2) > kBlockS / 2)
+        throw DwfError(DWF_ERR_ARG, "fused even-odd launch: bad tile");
+    // psi's region (phase 1), later w's box (phase 2)
+    const size_t lds = (size_t)(t.ty + 4) * ((t.tx + 4) / 2) * a.L.Ls * sizeof(Sp<R>);
+    const unsigned grid = (unsigned)nb;
+    const int Ls = a.L.Ls;
+#define DWF_EO_S(LSV)                                                  \
+    if (Ls == LSV) {                                                   \
+        if (dagger)                                                    \
+            dispatch_schur<R, true, LSV>(epi, a, t, grid, lds, s);     \
+        else                                                           \
+            dispatch_schur<R, false, LSV>(epi, a, t, grid, lds, s);    \
+        return;                                                        \
+    }
+    DWF_EO_S(4)
+    DWF_EO_S(8)
+    DWF_EO_S(12)
+    DWF_EO_S(16)
+#undef DWF_EO_S
+    throw DwfError(DWF_ERR_ARG, "fused even-odd launch: Ls must be 4, 8, 12 or 16");
+}
+template void launch_eo_schur<float>(bool, int, const EoArgs<float>&, const EoTile&, int, hipStream_t, const Groups*);
+template void launch_eo_schur<double>(bool, int, const EoArgs<double>&, const EoTile&, int, hipStream_t, const Groups*);
 
 namespace {
 unsigned split_blocks(const VecLayout& lay)

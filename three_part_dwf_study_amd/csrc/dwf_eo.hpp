@@ -42,7 +42,26 @@ template <typename R> struct EoArgs {
     int pstride;
 };
 
+// Tile of the fused Schur-operator kernel (k_eo_schur): ty x tx 4D sites (both even,
+// ty <= 8, tx <= 16, dividing every lattice's Nt and Nx) per block of kBlockS threads,
+// its target-parity sites (at most 64, one per lane) walked in nch = 4 s-chunks (one per
+// wavefront) of spt = Ls / 4 slices.
+struct EoTile {
+    int ty, tx, nch, spt;
+};
+constexpr int kBlockS = 256;
+
 bool eo_supported(const Lat& L);
+// The largest tile for the lattices of G (false: none fits).
+bool eo_fused_tile(const Groups& G, int Ls, EoTile& t);
+// Block partials per right-hand side of k_eo_schur over the groups of G.
+int eo_fused_pstride(const Groups& G, const EoTile& t);
+// out_o = a.ca [A | A^dag] a.in + a.ch H w, w = [A^-1 | A^-dag] H a.in (the Schur operator
+// Dhat or Dhat^dag for ca = 1, ch = -1/4), with the epilogue epi of launch_eo_b; one
+// partial pair per tile.
+template <typename R>
+void launch_eo_schur(bool dagger, int epi, const EoArgs<R>& a, const EoTile& t, int nrhs, hipStream_t s,
+                     const Groups* hg = nullptr);
 // Block partials per right-hand side of k_eo_b over groups with `chunks` s-chunks.
 int eo_pstride(const Groups& G, int chunks);
 // hg: the host copy of a.groups' table for a multi-lattice launch (it sizes the grid), else null.

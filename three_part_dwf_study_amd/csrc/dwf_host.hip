@@ -462,6 +462,10 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             if (!(value == 0.0 || value == 1.0)) break;
             ctx->fcg_beta = (int)value;
             return DWF_OK;
+        case DWF_TUNE_EO_FUSED:
+            if (!(value == 0.0 || value == 1.0)) break;
+            ctx->eo_fused = (int)value;
+            return DWF_OK;
         default:
             break;
     }

@@ -162,6 +162,8 @@ struct SchurOp : CgOperator {
     float* te32;
     double* te64;
     double* q64;
+    bool fused = false;  // one k_eo_schur launch per application (A^-1 H psi kept on the chip)
+    EoTile tile{};
     SchurOp(dwf_ctx* ctx, int n, float* t32, double* t64, double* q) : c(ctx), nrhs(n), te32(t32), te64(t64), q64(q)
     {
         const Lat& L = c->L;
@@ -180,6 +182,22 @@ struct SchurOp : CgOperator {
         spt = (L.Ls + ch - 1) / ch;
         chunks = (L.Ls + spt - 1) / spt;
         nblk = eo_pstride(lay.gr, chunks);
+        if (c->eo_fused && eo_fused_tile(lay.gr, L.Ls, tile)) {
+            fused = true;
+            nblk = eo_fused_pstride(lay.gr, tile);
+        }
+    }
+    // the fused kernel's arguments for out = [Dhat | Dhat^dag] in (epilogue fields set by the caller)
+    template <typename R> EoArgs<R> fused_args(const R* in, const CgState* st, int slot, int skip) const
+    {
+        EoArgs<R> b = base<R>(1);
+        b.in = in;
+        b.ca = (R)1;
+        b.ch = (R)-0.25;
+        b.st = st;
+        b.slot = slot;
+        b.skip = skip;
+        return b;
     }
     template <typename R> EoArgs<R> base(int tpar) const
     {
@@ -222,6 +240,20 @@ struct SchurOp : CgOperator {
     }
     void first(bool f64, const void* p, void* t, double* part, const CgState* st, int slot, int skip) override
     {
+        if (fused) {
+            if (f64) {
+                EoArgs<double> b = fused_args<double>((const double*)p, st, slot, skip);
+                b.out = (double*)t;
+                b.part = part;
+                launch_eo_schur<double>(false, EPI_NORM, b, tile, nrhs, c->stream, c->groups_host());
+            } else {
+                EoArgs<float> b = fused_args<float>((const float*)p, st, slot, skip);
+                b.out = (float*)t;
+                b.part = part;
+                launch_eo_schur<float>(false, EPI_NORM, b, tile, nrhs, c->stream, c->groups_host());
+            }
+            return;
+        }
         if (f64) {
             EoArgs<double> b = apply_args<double>(false, (const double*)p, te64, st, slot, skip);
             b.out = (double*)t;
@@ -237,6 +269,26 @@ struct SchurOp : CgOperator {
     void second(bool f64, const void* t, void* x, void* r, const void* p, const double* partA, double* part,
                 const CgState* st, int slot, int skip) override
     {
+        if (fused) {
+            if (f64) {
+                EoArgs<double> b = fused_args<double>((const double*)t, st, slot, skip);
+                b.x = (double*)x;
+                b.r = (double*)r;
+                b.p = (const double*)p;
+                b.partA = partA;
+                b.part = part;
+                launch_eo_schur<double>(true, EPI_CGUPD, b, tile, nrhs, c->stream, c->groups_host());
+            } else {
+                EoArgs<float> b = fused_args<float>((const float*)t, st, slot, skip);
+                b.x = (float*)x;
+                b.r = (float*)r;
+                b.p = (const float*)p;
+                b.partA = partA;
+                b.part = part;
+                launch_eo_schur<float>(true, EPI_CGUPD, b, tile, nrhs, c->stream, c->groups_host());
+            }
+            return;
+        }
         if (f64) {
             EoArgs<double> b = apply_args<double>(true, (const double*)t, te64, st, slot, skip);
             b.x = (double*)x;
@@ -258,6 +310,20 @@ struct SchurOp : CgOperator {
     void residual(const double* x, const double* b64, double* r64, double* p64, float* r32, float* p32, double* part,
                   const CgState* st, int slot, int skip) override
     {
+        if (fused) {
+            EoArgs<double> b = fused_args<double>(x, st, slot, skip);
+            b.out = q64;
+            launch_eo_schur<double>(false, EPI_STORE, b, tile, nrhs, c->stream, c->groups_host());
+            EoArgs<double> e = fused_args<double>(q64, st, slot, skip);
+            e.b = b64;
+            e.r64 = r64;
+            e.p64 = p64;
+            e.r32 = r32;
+            e.p32 = p32;
+            e.part = part;
+            launch_eo_schur<double>(true, EPI_RESID, e, tile, nrhs, c->stream, c->groups_host());
+            return;
+        }
         EoArgs<double> b = apply_args<double>(false, x, te64, st, slot, skip);
         b.out = q64;
         launch_eo_b<double>(false, true, EPI_STORE, b, nrhs, c->stream, c->groups_host());
@@ -274,7 +340,9 @@ struct SchurOp : CgOperator {
     {
         sig_ctx(c, k);
         k.insert(k.end(), {2ULL, (unsigned long long)nrhs, (unsigned long long)bx, (unsigned long long)chunks,
-                           (unsigned long long)spt, sig_ptr(te32), sig_ptr(te64), sig_ptr(q64)});
+                           (unsigned long long)spt, sig_ptr(te32), sig_ptr(te64), sig_ptr(q64),
+                           (unsigned long long)fused, (unsigned long long)tile.ty, (unsigned long long)tile.tx,
+                           (unsigned long long)tile.nch, (unsigned long long)tile.spt});
     }
 };
 
