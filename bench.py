@@ -92,6 +92,18 @@ def cg_iteration_bytes(prec_bytes):
     return 2 * (2 * Bv + BL) + 9 * Bv
 
 
+def eo_iteration_bytes():
+    """Algorithmic bytes of one FP32 iteration of the even-odd Schur CG on Dhat^dag Dhat
+    for one right-hand side, vectors on the odd half lattice (B_h = B_v / 2): the Dhat
+    launch reads r and the old direction and writes the new direction and t = Dhat p
+    (4 B_h), the Dhat^dag launch reads t, p, x, r and writes x, r (6 B_h); A^-1 H p stays
+    on the chip; each launch reads the links (B_L)."""
+    V4 = NT * NX
+    Bh = V4 * LS * 4 * 4 // 2
+    BL = V4 * 4 * 4
+    return 10 * Bh + 2 * BL
+
+
 def time_kernel(ctx, capi, launch, reps):
     """Average duration (ms) of `launch` over `reps` back-to-back launches, from HIP
     events recorded on the library's stream around the timed region."""
@@ -872,7 +884,8 @@ def main():
         # HBM-roofline ceiling of the CG solvers: 8 TB/s / (iterations x algorithmic bytes per iteration)
         "roofline_ceiling_solves_per_s": {
             "fp64_cg": round(8e12 / (results["fp64_cg"]["iterations"] * cg_iteration_bytes(8)), 1),
-            "mixed": round(8e12 / (results["mixed"]["iterations"] * cg_iteration_bytes(4)), 1)},
+            "mixed": round(8e12 / (results["mixed"]["iterations"] * cg_iteration_bytes(4)), 1),
+            "eo_mixed": round(8e12 / (results["eo_mixed"]["iterations"] * eo_iteration_bytes()), 1)},
         "reliable_updates": {k: v["reliable_updates"] for k, v in results.items()},
         "max_true_relres": {k: v["max_true_relres"] for k, v in results.items()},
         "dslash_gbs": {"fp32_batch%d" % args.dslash_rhs: round(gbs32, 1),

@@ -122,9 +122,13 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * update of DWF_SOLVE_LEARNED, 0 (default) flexible CG's beta = -(z, A p) / (p, A p)
  * (Notay's FCG(1), for any preconditioner), 1 Fletcher-Reeves beta = (z, r) / (z, r)_prev
  * (exact for a symmetric preconditioner; one vector pass less per iteration);
- * DWF_TUNE_EO_FUSED: the Schur operator of DWF_SOLVE_EO_MIXED, 1 (default) one kernel per
- * application (A^-1 H psi kept in LDS), 0 two kernels with A^-1 H psi through device
- * memory (same arithmetic). */
+ * DWF_TUNE_EO_FUSED: the Schur operator of DWF_SOLVE_EO_MIXED, 0 two kernels per
+ * application with A^-1 H psi through device memory, 1 one kernel (A^-1 H psi kept in
+ * LDS; same arithmetic), 2 (default) one kernel with the FP32 CG's direction update
+ * p = r + beta p inside the next operator launch (one vector pass and one launch less per
+ * iteration); DWF_TUNE_NET_TILE: the fused network's output tile,
+ * ty * 256 + tx sites (ty <= 8, tx <= 16, dividing Nt and Nx), 0 (default) the largest
+ * that divides. */
 enum dwf_tune_param {
     DWF_TUNE_SCHUNKS = 0,
     DWF_TUNE_XCD_REMAP = 1,
@@ -136,7 +140,8 @@ enum dwf_tune_param {
     DWF_TUNE_LDS = 7,
     DWF_TUNE_NET_FUSED = 8,
     DWF_TUNE_FCG_BETA = 9,
-    DWF_TUNE_EO_FUSED = 10
+    DWF_TUNE_EO_FUSED = 10,
+    DWF_TUNE_NET_TILE = 11
 };
 int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 

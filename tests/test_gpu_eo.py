@@ -48,7 +48,11 @@ def test_eo_solve_certified(Nt, Nx, Ls, nrhs):
 def test_eo_fused_matches_two_kernel(Nt, Nx, Ls, nrhs):
     """The one-kernel Schur operator (DWF_TUNE_EO_FUSED 1: A^-1 H psi kept in LDS, tiles of
     8 x 16, 6 x 10 ...) against the two-kernel form (0, A^-1 H psi through device memory):
-    the same arithmetic, so the same iterations and, both certified, the same solutions."""
+    the same arithmetic, so the same iterations and, both certified, the same solutions.
+    The pipelined form (2, the default: p = r + beta p inside the next operator launch,
+    directions double-buffered per right-hand side) continues a cycle after a reliable
+    update from the new residual, so its iterations may differ by a few: certified, and
+    its solution equal to the others within the tolerance's reach (1e-6)."""
     print_case(test="eo_fused", Nt=Nt, Nx=Nx, Ls=Ls, nrhs=nrhs)
     rng = np.random.default_rng(Nt * 7 + Nx)
     th = rand_theta(rng, Nt, Nx)
@@ -56,17 +60,21 @@ def test_eo_fused_matches_two_kernel(Nt, Nx, Ls, nrhs):
     out = []
     with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
         ctx.set_gauge(th)
-        for fused in (0, 1):
+        for fused in (0, 1, 2):
             ctx.tune(dwf.DWF_TUNE_EO_FUSED, fused)
             out.append(ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, method=dwf.DWF_SOLVE_EO_MIXED))
-    (x0, st0, rc0), (x1, st1, rc1) = out
-    assert rc0 == 0 and rc1 == 0
+    (x0, st0, rc0), (x1, st1, rc1), (x2, st2, rc2) = out
+    assert rc0 == 0 and rc1 == 0 and rc2 == 0
     for i in range(nrhs):
-        assert st1[i]["converged"] and st1[i]["passes"] >= 1, st1[i]
+        for st in (st1, st2):
+            assert st[i]["converged"] and st[i]["passes"] >= 1, st[i]
         assert abs(st1[i]["iterations"] - st0[i]["iterations"]) <= 2, (st0[i], st1[i])
+        assert abs(st2[i]["iterations"] - st0[i]["iterations"]) <= 0.05 * st0[i]["iterations"] + 4, (st0[i], st2[i])
         if Nt * Nx <= 144:
             assert normal_relres(th, x1[i], src[i], Nt, Nx, Ls) <= 1e-10
+            assert normal_relres(th, x2[i], src[i], Nt, Nx, Ls) <= 1e-10
         assert np.linalg.norm(x1[i] - x0[i]) / np.linalg.norm(x0[i]) < 1e-8
+        assert np.linalg.norm(x2[i] - x0[i]) / np.linalg.norm(x0[i]) < 1e-6
 
 
 def test_eo_matches_oracle_cg():

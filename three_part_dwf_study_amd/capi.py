@@ -50,6 +50,7 @@ DWF_TUNE_LDS = 7
 DWF_TUNE_NET_FUSED = 8
 DWF_TUNE_FCG_BETA = 9
 DWF_TUNE_EO_FUSED = 10
+DWF_TUNE_NET_TILE = 11
 
 
 class DwfError(RuntimeError):

@@ -29,6 +29,7 @@ enum BufId : int {
     B_R32,
     B_T32,
     B_P32,
+    B_P32B,      // pipelined even-odd CG: the second direction buffer
     B_X32,
     B_PA,
     B_PB,
@@ -148,6 +149,8 @@ struct dwf_ctx {
     int net_fused = 2;            // learned preconditioner: 0 layer by layer, 1 fused, 2 fused in a gauge per block
     int fcg_beta = 0;             // learned solver: 0 flexible CG (Notay's FCG(1)), 1 Fletcher-Reeves
     int eo_fused = 1;             // Schur operator: 1 one kernel per application, 0 two
+    int eo_pipe = 1;              // fused Schur CG: 1 direction update inside the operator launch
+    int net_tile = 0;             // fused network's output tile ty * 256 + tx (0: the largest that divides)
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs
     const dwf_net* net = nullptr; // learned preconditioner of DWF_SOLVE_LEARNED (not owned)

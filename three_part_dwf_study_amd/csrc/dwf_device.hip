@@ -354,7 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_hop(const HopArgs<R> a)
     } else {
         const int chunks = a.cfg.chunks;
         int blk0, nblk;
-        const GroupPos p = block_group(a.groups, lb, [chunks](int v4) { return (v4 + kBlock - 1) / kBlock * chunks; },
+        const GroupPos p = block_group(a.groups, lb, [chunks](int nt, int nx) { return (nt * nx + kBlock - 1) / kBlock * chunks; },
                                        blk0, nblk);
         const int local = lb - blk0, rl = local / nblk;  // right-hand side within the group
         rhs = p.rhs0 + rl;
@@ -983,6 +983,65 @@ __global__ void k_cg_start(const int nblk, const double* partR, CgState* st, con
     }
 }
 
+// The bookkeeping of k_cg_p without its vector pass, for the pipelined even-odd CG (whose
+// next operator launch forms p = r + beta p itself): convergence, freezing and the states
+// of slot ^ 1, beta for the next direction, and the direction buffer flip.  One 64-thread
+// block per right-hand side.
+__global__ void k_cg_flags(const int nblk, const double* partA, const double* partB, CgState* st, const int slot,
+                           const int mode)
+{
+    const int rhs = blockIdx.x;
+    CgState& S = st[rhs];
+    const int f = S.flags[slot];
+    const int ns = slot ^ 1;
+    if (f & (F_DONE | F_FROZEN)) {
+        if (threadIdx.x == 0) {
+            S.flags[ns] = f;
+            S.rr[ns] = S.rr[slot];
+        }
+        return;
+    }
+    const double* pa = partA + (size_t)rhs * nblk * 2;
+    const double* pb = partB + (size_t)rhs * nblk * 2;
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += 64) {
+        a += pa[2 * i];
+        b += pb[2 * i];
+    }
+    const double pAp = wave_sum(a);
+    const double rr_new = wave_sum(b);
+    if (threadIdx.x != 0) return;
+    const double rr_old = S.rr[slot];
+    int nf = f & ~F_FRESH;
+    if (pAp == 0.0) {
+        S.status = 4;
+        nf |= F_DONE;
+        S.rr[ns] = rr_old;
+    } else {
+        const double relres = std::sqrt(rr_new) / std::sqrt(S.bnorm2);
+        const bool conv = relres <= S.tol;
+        S.rr[ns] = rr_new;
+        S.relres = relres;
+        S.iters += 1;
+        S.beta = rr_new / rr_old;
+        S.pbuf ^= 1;  // this iteration's direction (written to the other buffer) is current
+        if (mode == MODE_CG64 && conv) nf |= F_DONE;
+        if (mode == MODE_MIXED && (conv || rr_new <= S.delta * S.delta * S.rr_ref)) nf |= F_FROZEN;
+    }
+    S.flags[ns] = nf;
+}
+
+// Start of a pipelined even-odd CG: every right-hand side's first direction is its residual.
+__global__ void k_set_fresh(CgState* st, const int nrhs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrhs) {
+        st[i].flags[0] |= F_FRESH;
+        st[i].beta = 0.0;
+        st[i].pbuf = 0;
+    }
+}
+
 // After a reliable update: FP64 true residual -> certify, or restart the FP32 cycle.
 __global__ void k_ru_commit(const int nblk, const double* partR, CgState* st, const int slot)
 {
@@ -1272,6 +1331,17 @@ template void launch_cg_p<double>(const VecLayout&, int, double*, double*, const
 void launch_cg_start(int nblk, const double* partR, CgState* st, double tol, double delta, int nrhs, hipStream_t s)
 {
     k_cg_start<<<nrhs, 64, 0, s>>>(nblk, partR, st, tol, delta);
+}
+
+void launch_cg_flags(int nblk, const double* partA, const double* partB, CgState* st, int slot, int mode, int nrhs,
+                     hipStream_t s)
+{
+    k_cg_flags<<<nrhs, 64, 0, s>>>(nblk, partA, partB, st, slot, mode);
+}
+
+void launch_set_fresh(CgState* st, int nrhs, hipStream_t s)
+{
+    k_set_fresh<<<(nrhs + 63) / 64, 64, 0, s>>>(st, nrhs);
 }
 
 void launch_ru_commit(int nblk, const double* partR, CgState* st, int slot, int nrhs, hipStream_t s)

@@ -161,7 +161,7 @@ __device__ __forceinline__ EoPos<R> eo_pos(const EoArgs<R>& a, int lb, int per, 
         q.links = static_cast<const C*>(a.links) + (long long)(q.rhs / a.rhs_per_cfg) * a.link_stride;
     } else {
         int blk0, nblk;
-        const GroupPos p = block_group(a.groups, lb, [per, chunks](int v4) { return (v4 / 2 + per - 1) / per * chunks; },
+        const GroupPos p = block_group(a.groups, lb, [per, chunks](int nt, int nx) { return (nt * nx / 2 + per - 1) / per * chunks; },
                                        blk0, nblk);
         const int local = lb - blk0, rl = local / nblk;
         q.rhs = p.rhs0 + rl;
@@ -371,8 +371,8 @@ __global__ __launch_bounds__(kBlockA) void k_eo_a(const EoArgs<R> a)
 // The per-site epilogue of the even-odd operator kernels: o is the operator's value at
 // index idx (within right-hand side offset rb) of the target parity.
 template <typename R, int EPI>
-__device__ __forceinline__ void eo_epi(const EoArgs<R>& a, long long rb, long long idx, const Sp<R>& o, R alpha,
-                                       double& acc0, double& acc1)
+__device__ __forceinline__ void eo_epi(const EoArgs<R>& a, const R* pdir, long long rb, long long idx, const Sp<R>& o,
+                                       R alpha, double& acc0, double& acc1)
 {
     if constexpr (EPI == EPI_STORE) {
         st(a.out + rb, idx, o);
@@ -380,18 +380,18 @@ __device__ __forceinline__ void eo_epi(const EoArgs<R>& a, long long rb, long lo
         st(a.out + rb, idx, o);
         acc0 += n2(o);
     } else if constexpr (EPI == EPI_CGUPD) {
-        const Sp<R> pv = ld(a.p + rb, idx);
+        const Sp<R> pv = ld(pdir + rb, idx);
         Sp<R> xv = ld(a.x + rb, idx);
-        Sp<R> rv = ld(a.r + rb, idx);
         xv.ar += alpha * pv.ar;
         xv.ai += alpha * pv.ai;
         xv.br += alpha * pv.br;
         xv.bi += alpha * pv.bi;
+        st(a.x + rb, idx, xv);
+        Sp<R> rv = ld(a.r + rb, idx);
         rv.ar -= alpha * o.ar;
         rv.ai -= alpha * o.ai;
         rv.br -= alpha * o.br;
         rv.bi -= alpha * o.bi;
-        st(a.x + rb, idx, xv);
         st(a.r + rb, idx, rv);
         acc0 += n2(rv);
     } else {
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
             o.br *= a.ca;
             o.bi *= a.ca;
             add_hop<R, DAG>(o, a.ch, g, a.hin + rb, sl);
-            eo_epi<R, EPI>(a, rb, sl + hs, o, alpha, acc0, acc1);
+            eo_epi<R, EPI>(a, a.p, rb, sl + hs, o, alpha, acc0, acc1);
             if constexpr (AOP) {
                 dn = cur;
                 cur = up;
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
 // The boundary signs are folded into the links (exact).  The arithmetic otherwise
 // follows k_eo_a and k_eo_b; box or region sites that wrap around a small lattice are
 // computed twice, into two LDS entries.
-template <typename R, bool DAG, int LS, int EPI>
+template <typename R, bool DAG, int LS, int EPI, bool PIPE = false>
 __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(const EoArgs<R> a, const EoTile tl)
 {
     extern __shared__ __align__(16) unsigned char eo_smem[];
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
     using C = typename C2T<R>::T;
     const int total = gridDim.x;
     const int lb = a.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
-    const int TY = tl.ty, TX = tl.tx;
+    int TY = tl.ty, TX = tl.tx;  // one lattice; several: each group's own tile (eo_tile_dim)
     Lat L = a.L;
     int rhs, tile, pstride;
     long long rb;
@@ -535,7 +535,11 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
         links = static_cast<const C*>(a.links) + (long long)(rhs / a.rhs_per_cfg) * a.link_stride;
     } else {
         int blk0, per;
-        const GroupPos p = block_group(a.groups, lb, [TY, TX](int v4) { return v4 / (TY * TX); }, blk0, per);
+        const GroupPos p = block_group(
+            a.groups, lb, [](int nt, int nx) { return (nt / eo_tile_dim(nt, 8)) * (nx / eo_tile_dim(nx, 16)); }, blk0,
+            per);
+        TY = eo_tile_dim(p.Nt, 8);
+        TX = eo_tile_dim(p.Nx, 16);
         const int local = lb - blk0, rl = local / per;
         rhs = p.rhs0 + rl;
         tile = local - rl * per;
@@ -559,7 +563,7 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
     const int t0 = tyi * TY, x0 = (tile - tyi * tiles_x) * TX;
     const int WB = (TX + 2) / 2, nbox = (TY + 2) * WB;
     const int opar = 1 - a.tpar;
-    const R* in = a.in + rb;
+    const R* in = PIPE ? nullptr : a.in + rb;
     const R d = (R)L.diag, mf = (R)L.mf;
 
     // ---- phase 1a: psi on the region [-2, ty + 1] x [-2, tx + 1] of target-parity sites
@@ -575,10 +579,41 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
             int t = t0 + ir - 2, x = x0 + jr - 2;
             t = t < 0 ? t + L.Nt : (t >= L.Nt ? t - L.Nt : t);
             x = x < 0 ? x + L.Nx : (x >= L.Nx ? x - L.Nx : x);
-            const R* src = in + ((long long)h * LH * V4h + t * Nxh + (x >> 1)) * 4;
+            const long long i0 = (long long)h * LH * V4h + t * Nxh + (x >> 1);
             Sp<R> v[LH];
+            if constexpr (PIPE) {
+                // p = r + beta p_old (k_cg_p's update), on the fly; the tile's own sites are
+                // stored to the new direction buffer (a wrapped duplicate outside the tile is not)
+                const CgState& S = a.st[rhs];
+                const bool fresh = (S.flags[a.slot] & F_FRESH) != 0;
+                const R beta = (R)S.beta;
+                const R* rsrc = a.r + rb + i0 * 4;
+                const bool own = ir >= 2 && ir < TY + 2 && jr >= 2 && jr < TX + 2;
 #pragma unroll
-            for (int k = 0; k < LH; ++k) v[k] = ld(src, (long long)k * V4h);
+                for (int k = 0; k < LH; ++k) v[k] = ld(rsrc, (long long)k * V4h);
+                if (!fresh) {
+                    const R* psrc = (S.pbuf ? a.pd1 : a.pd0) + rb + i0 * 4;
+                    Sp<R> po[LH];
+#pragma unroll
+                    for (int k = 0; k < LH; ++k) po[k] = ld(psrc, (long long)k * V4h);
+#pragma unroll
+                    for (int k = 0; k < LH; ++k) {
+                        v[k].ar = v[k].ar + beta * po[k].ar;
+                        v[k].ai = v[k].ai + beta * po[k].ai;
+                        v[k].br = v[k].br + beta * po[k].br;
+                        v[k].bi = v[k].bi + beta * po[k].bi;
+                    }
+                }
+                if (own) {
+                    R* pdst = (S.pbuf ? a.pd0 : a.pd1) + rb + i0 * 4;
+#pragma unroll
+                    for (int k = 0; k < LH; ++k) st(pdst, (long long)k * V4h, v[k]);
+                }
+            } else {
+                const R* src = in + i0 * 4;
+#pragma unroll
+                for (int k = 0; k < LH; ++k) v[k] = ld(src, (long long)k * V4h);
+            }
 #pragma unroll
             for (int k = 0; k < LH; ++k) P[(h * LH + k) * nreg + r] = v[k];
         }
@@ -647,6 +682,11 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
 
     // ---- phase 2: the operator on the tile's target sites (ca = 1)
     double acc0 = 0.0, acc1 = 0.0;
+    // the direction of x += alpha p: in the pipelined CG, this iteration's (pd[pbuf ^ 1];
+    // the state update after this launch flips pbuf)
+    const R* pdir = a.p;
+    if constexpr (EPI == EPI_CGUPD)
+        if (a.pd0) pdir = a.st[rhs].pbuf ? a.pd0 : a.pd1;
     if (act2) {
         const int hs = (t0 + i2) * Nxh + ((x0 + j2) >> 1);
         EoSite<R> g = eo_site<R>(L, Nxh, a.tpar, hs, links);
@@ -676,7 +716,7 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
             }
             const Sp<R>* ws = w + s * nbox;
             hop_sum<R, DAG, false>(o, a.ch, g, ws[etp], ws[etm], ws[exp], ws[exm]);
-            eo_epi<R, EPI>(a, rb, (long long)s * V4h + hs, o, alpha, acc0, acc1);
+            eo_epi<R, EPI>(a, pdir, rb, (long long)s * V4h + hs, o, alpha, acc0, acc1);
         }
     }
     if constexpr (EPI != EPI_STORE) {
@@ -841,30 +881,27 @@ template void launch_eo_b<double>(bool, bool, int, const EoArgs<double>&, int, h
 
 bool eo_fused_tile(const Groups& G, int Ls, EoTile& t)
 {
-    int ty = 0, tx = 0;
-    for (int c = 8; c >= 2 && !ty; c -= 2) {
-        bool ok = true;
-        for (int g = 0; g < G.n; ++g) ok = ok && G.Nt[g] % c == 0;
-        if (ok) ty = c;
+    if (Ls < 4 || Ls % 4) return false;
+    for (int g = 0; g < G.n; ++g) {  // every lattice needs a tile of its own (its group's, in the kernel)
+        const int ty = eo_tile_dim(G.Nt[g], 8), tx = eo_tile_dim(G.Nx[g], 16);
+        if (!ty || !tx || ty * tx / 2 > 64) return false;
     }
-    for (int c = 16; c >= 2 && !tx; c -= 2) {
-        bool ok = true;
-        for (int g = 0; g < G.n; ++g) ok = ok && G.Nx[g] % c == 0;
-        if (ok) tx = c;
-    }
-    if (!ty || !tx || Ls < 1) return false;
-    if (Ls % 4 || ty * tx / 2 > 64) return false;
-    t.ty = ty;
-    t.tx = tx;
+    t.ty = eo_tile_dim(G.Nt[0], 8);  // the tile of a one-lattice launch
+    t.tx = eo_tile_dim(G.Nx[0], 16);
     t.nch = 4;  // one wavefront per s-chunk, one lane per target site
     t.spt = Ls / 4;
     return true;
 }
 
+namespace {
+int eo_group_tiles(const Groups& G, int g) { return (G.Nt[g] / eo_tile_dim(G.Nt[g], 8)) * (G.Nx[g] / eo_tile_dim(G.Nx[g], 16)); }
+}  // namespace
+
 int eo_fused_pstride(const Groups& G, const EoTile& t)
 {
+    if (G.n == 1) return G.Nt[0] * G.Nx[0] / (t.ty * t.tx);
     int m = 0;
-    for (int g = 0; g < G.n; ++g) m = std::max(m, G.Nt[g] * G.Nx[g] / (t.ty * t.tx));
+    for (int g = 0; g < G.n; ++g) m = std::max(m, eo_group_tiles(G, g));
     return m;
 }
 
@@ -895,10 +932,9 @@ void launch_eo_schur(bool dagger, int epi, const EoArgs<R>& a0, const EoTile& t,
     } else {
         if (!hg || hg->n < 1 || hg->n > kMaxGroups || hg->rhs0[hg->n] != nrhs)
             throw DwfError(DWF_ERR_ARG, "fused even-odd launch: bad lattice-group layout");
-        for (int g = 0; g < hg->n; ++g) {
-            if (hg->Nt[g] % t.ty || hg->Nx[g] % t.tx) throw DwfError(DWF_ERR_ARG, "fused even-odd launch: tile does not fit");
-            nb += (long long)(hg->rhs0[g + 1] - hg->rhs0[g]) * (hg->Nt[g] * hg->Nx[g] / (t.ty * t.tx));
-        }
+        EoTile chk;
+        if (!eo_fused_tile(*hg, a.L.Ls, chk)) throw DwfError(DWF_ERR_ARG, "fused even-odd launch: no tile fits a lattice");
+        for (int g = 0; g < hg->n; ++g) nb += (long long)(hg->rhs0[g + 1] - hg->rhs0[g]) * eo_group_tiles(*hg, g);
         a.pstride = eo_fused_pstride(*hg, t);
     }
     if (nb <= 0 || nb > 0x7fffffffLL) throw DwfError(DWF_ERR_ARG, "fused even-odd launch: grid size out of range");
@@ -906,8 +942,8 @@ void launch_eo_schur(bool dagger, int epi, const EoArgs<R>& a0, const EoTile& t,
     if (t.ty < 2 || t.ty > 8 || (t.ty & 1) || t.tx < 2 || t.tx > 16 || (t.tx & 1) || t.nch < 1 || t.spt < 1 ||
         t.nch != 4 || t.spt * 4 != a.L.Ls || t.ty * t.tx / 2 > 64 || (t.ty + 4) * ((t.tx + 4) / 2) > kBlockS / 2)
         throw DwfError(DWF_ERR_ARG, "fused even-odd launch: bad tile");
-    // psi's region (phase 1), later w's box (phase 2)
-    const size_t lds = (size_t)(t.ty + 4) * ((t.tx + 4) / 2) * a.L.Ls * sizeof(Sp<R>);
+    // psi's region (phase 1), later w's box (phase 2); several lattices: the largest tile's
+    const size_t lds = (size_t)(a.groups ? 12 * 10 : (t.ty + 4) * ((t.tx + 4) / 2)) * a.L.Ls * sizeof(Sp<R>);
     const unsigned grid = (unsigned)nb;
     const int Ls = a.L.Ls;
 #define DWF_EO_S(LSV)                                                  \
@@ -926,6 +962,35 @@ void launch_eo_schur(bool dagger, int epi, const EoArgs<R>& a0, const EoTile& t,
     throw DwfError(DWF_ERR_ARG, "fused even-odd launch: Ls must be 4, 8, 12 or 16");
 }
 template void launch_eo_schur<float>(bool, int, const EoArgs<float>&, const EoTile&, int, hipStream_t, const Groups*);
+
+void launch_eo_schur_pipe(const EoArgs<float>& a0, const EoTile& t, int nrhs, hipStream_t s, const Groups* hg)
+{
+    EoArgs<float> a = a0;
+    if (!a.st || !a.r || !a.pd0 || !a.pd1 || !a.out || !a.part)
+        throw DwfError(DWF_ERR_ARG, "pipelined even-odd launch: missing vectors or states");
+    long long nb = 0;
+    if (!a.groups) {
+        nb = (long long)(a.L.V4 / (t.ty * t.tx)) * nrhs;
+    } else {
+        if (!hg || hg->n < 1 || hg->n > kMaxGroups || hg->rhs0[hg->n] != nrhs)
+            throw DwfError(DWF_ERR_ARG, "pipelined even-odd launch: bad lattice-group layout");
+        EoTile chk;
+        if (!eo_fused_tile(*hg, a.L.Ls, chk)) throw DwfError(DWF_ERR_ARG, "pipelined even-odd launch: no tile fits a lattice");
+        for (int g = 0; g < hg->n; ++g) nb += (long long)(hg->rhs0[g + 1] - hg->rhs0[g]) * eo_group_tiles(*hg, g);
+        a.pstride = eo_fused_pstride(*hg, t);
+    }
+    if (nb <= 0 || nb > 0x7fffffffLL || t.nch != 4 || t.spt * 4 != a.L.Ls || a.ca != 1.f)
+        throw DwfError(DWF_ERR_ARG, "pipelined even-odd launch: bad shape");
+    const size_t lds = (size_t)(a.groups ? 12 * 10 : (t.ty + 4) * ((t.tx + 4) / 2)) * a.L.Ls * sizeof(Sp<float>);
+    const unsigned grid = (unsigned)nb;
+    switch (a.L.Ls) {
+        case 4: k_eo_schur<float, false, 4, EPI_NORM, true><<<grid, kBlockS, lds, s>>>(a, t); break;
+        case 8: k_eo_schur<float, false, 8, EPI_NORM, true><<<grid, kBlockS, lds, s>>>(a, t); break;
+        case 12: k_eo_schur<float, false, 12, EPI_NORM, true><<<grid, kBlockS, lds, s>>>(a, t); break;
+        case 16: k_eo_schur<float, false, 16, EPI_NORM, true><<<grid, kBlockS, lds, s>>>(a, t); break;
+        default: throw DwfError(DWF_ERR_ARG, "pipelined even-odd launch: Ls must be 4, 8, 12 or 16");
+    }
+}
 template void launch_eo_schur<double>(bool, int, const EoArgs<double>&, const EoTile&, int, hipStream_t, const Groups*);
 
 namespace {

@@ -40,6 +40,10 @@ template <typename R> struct EoArgs {
     // arrays' stride (eo_pstride).
     const Groups* groups;
     int pstride;
+    // pipelined even-odd CG (k_eo_schur only; null otherwise): the two direction buffers,
+    // CgState::pbuf naming the current one
+    R* pd0;
+    R* pd1;
 };
 
 // Tile of the fused Schur-operator kernel (k_eo_schur): ty x tx 4D sites (both even,
@@ -62,6 +66,11 @@ int eo_fused_pstride(const Groups& G, const EoTile& t);
 template <typename R>
 void launch_eo_schur(bool dagger, int epi, const EoArgs<R>& a, const EoTile& t, int nrhs, hipStream_t s,
                      const Groups* hg = nullptr);
+// The pipelined even-odd CG's operator launch (FP32): the direction p = r + beta p_old
+// (p = r on an F_FRESH iteration; p_old = a.pd[pbuf], p written to a.pd[pbuf ^ 1]) formed
+// on the fly from a.r, then out = Dhat p with |out|^2 partials.  With a.pd0 set, the
+// EPI_CGUPD launch (Dhat^dag) takes that new direction for x += alpha p.
+void launch_eo_schur_pipe(const EoArgs<float>& a, const EoTile& t, int nrhs, hipStream_t s, const Groups* hg = nullptr);
 // Block partials per right-hand side of k_eo_b over groups with `chunks` s-chunks.
 int eo_pstride(const Groups& G, int chunks);
 // hg: the host copy of a.groups' table for a multi-lattice launch (it sizes the grid), else null.

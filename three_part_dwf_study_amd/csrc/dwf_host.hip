@@ -463,9 +463,18 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             ctx->fcg_beta = (int)value;
             return DWF_OK;
         case DWF_TUNE_EO_FUSED:
-            if (!(value == 0.0 || value == 1.0)) break;
-            ctx->eo_fused = (int)value;
+            if (!(value == 0.0 || value == 1.0 || value == 2.0)) break;
+            ctx->eo_fused = value != 0.0 ? 1 : 0;
+            ctx->eo_pipe = value == 2.0 ? 1 : 0;
             return DWF_OK;
+        case DWF_TUNE_NET_TILE: {
+            if (!(value >= 0.0 && value == (double)(int)value)) break;
+            const int v = (int)value, ty = v >> 8, tx = v & 255;
+            if (v != 0 && (ty < 1 || ty > 8 || tx < 1 || tx > 16 || ctx->L.Nt % ty || ctx->L.Nx % tx)) break;
+            if (v != ctx->net_tile) ctx->drop_graphs();
+            ctx->net_tile = v;
+            return DWF_OK;
+        }
         default:
             break;
     }

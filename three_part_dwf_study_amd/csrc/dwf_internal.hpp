@@ -16,7 +16,8 @@ constexpr int kBlock = 256;  // threads per block for every stencil/BLAS kernel
 // reads slot it&1, so blocks of one launch never race.
 // F_HOLD marks, during one reliable update, the right-hand sides whose FP32 cycle is still
 // running: that update skips them and clears the mark.
-enum : int { F_DONE = 1, F_FROZEN = 2, F_HOLD = 4 };
+// F_FRESH: the first iteration of a pipelined even-odd CG (its direction is the residual).
+enum : int { F_DONE = 1, F_FROZEN = 2, F_HOLD = 4, F_FRESH = 8 };
 enum : int { MODE_CG64 = 0, MODE_MIXED = 1 };
 
 struct CgState {
@@ -32,6 +33,9 @@ struct CgState {
     int status;         // 0, or 4 for a zero denominator
     int reliable;       // reliable updates performed
     int fp64_iters;     // FP64 iterations (fallback)
+    // pipelined even-odd CG (the direction update inside the next operator launch):
+    double beta;        // r.r / r.r_prev of the last iteration, for the next direction
+    int pbuf;           // which of the two direction buffers holds the current direction
 };
 
 // Lattice and operator constants.  Ls = 1 and d5 = 0 give the Wilson operator.
@@ -87,7 +91,7 @@ __host__ __device__ __forceinline__ GroupPos group_for_rhs(const Groups& G, int 
 }
 
 // The group of block lb of a launch over several lattices whose group g gives each of its
-// right-hand sides bpr(Nt[g] * Nx[g]) consecutive blocks, the groups one after another.
+// right-hand sides bpr(Nt[g], Nx[g]) consecutive blocks, the groups one after another.
 // Returns the group's fields, the group's first block and its blocks per right-hand side.
 // The table is the context's device copy (one kernel argument, read through the scalar
 // cache), so single-lattice launches keep their small argument structs.
@@ -97,7 +101,7 @@ __device__ __forceinline__ GroupPos block_group(const Groups* G, int lb, F bpr, 
     const int n = G->n;
     int start = 0, g = n - 1, nb = 0;
     for (int k = 0; k < n; ++k) {
-        nb = bpr(G->Nt[k] * G->Nx[k]);
+        nb = bpr(G->Nt[k], G->Nx[k]);
         const int cnt = (G->rhs0[k + 1] - G->rhs0[k]) * nb;
         if (lb < start + cnt) {
             g = k;
@@ -108,6 +112,15 @@ __device__ __forceinline__ GroupPos block_group(const Groups* G, int lb, F bpr, 
     blk0 = start;
     per_rhs = nb;
     return group_fields(*G, g);
+}
+
+// Side of the fused even-odd kernel's tile along a lattice extent n: the largest even
+// divisor of n up to maxd (0: none).
+__host__ __device__ __forceinline__ int eo_tile_dim(int n, int maxd)
+{
+    for (int c = maxd; c >= 2; c -= 2)
+        if (n % c == 0) return c;
+    return 0;
 }
 
 // Extent and offset (in sites) of right-hand side rhs's vector in a batch over groups;
@@ -248,6 +261,11 @@ void launch_cg_p(const VecLayout& lay, int nblk, R* r, R* p, const double* partA
                  int slot, int mode, int nrhs, hipStream_t s);
 void launch_cg_start(int nblk, const double* partR, CgState* st, double tol, double delta, int nrhs, hipStream_t s);
 void launch_ru_commit(int nblk, const double* partR, CgState* st, int slot, int nrhs, hipStream_t s);
+// k_cg_p's state update alone (the pipelined even-odd CG forms the direction itself), and
+// the pipelined CG's start (F_FRESH, direction buffer 0).
+void launch_cg_flags(int nblk, const double* partA, const double* partB, CgState* st, int slot, int mode, int nrhs,
+                     hipStream_t s);
+void launch_set_fresh(CgState* st, int nrhs, hipStream_t s);
 void launch_true_resid(int nblk, const double* partR, CgState* st, int nrhs, hipStream_t s);
 void launch_accum_x(const VecLayout& lay, double* x64, float* x32, const CgState* st, int slot, int skip, int nrhs,
                     hipStream_t s);

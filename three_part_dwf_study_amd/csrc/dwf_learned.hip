@@ -1565,9 +1565,15 @@ void repack(const dwf_net* net, hipStream_t s)
 }
 
 // Output tile of the fused network kernel: the largest divisors of Nt (<= 8) and Nx (<= 16).
-bool fused_shape(const Lat& L, int& TY, int& TX)
+bool fused_shape(const dwf_ctx* c, int& TY, int& TX)
 {
+    const Lat& L = c->L;
     if (L.Nt < 3 || L.Nx < 3) return false;
+    if (c->net_tile > 0) {  // set by hand (DWF_TUNE_NET_TILE), validated there
+        TY = c->net_tile >> 8;
+        TX = c->net_tile & 255;
+        return true;
+    }
     TY = 1;
     TX = 1;
     for (int d = 1; d <= kFuTy; ++d)
@@ -1595,7 +1601,7 @@ int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind
 {
     const Lat& L = c->L;
     int TY, TX;
-    if (!c->net_fused || !fused_shape(L, TY, TX)) return 0;
+    if (!c->net_fused || !fused_shape(c, TY, TX)) return 0;
     NetFusedArgs a{};
     a.Nt = L.Nt;
     a.Nx = L.Nx;
@@ -2080,7 +2086,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     const dim3 vgrid(nb, nrhs);
     const int fr = c->fcg_beta;
     int fty = 0, ftx = 0;
-    const int ntiles = fused_shape(c->L, fty, ftx) ? (c->L.Nt / fty) * (c->L.Nx / ftx) : 1;
+    const int ntiles = fused_shape(c, fty, ftx) ? (c->L.Nt / fty) * (c->L.Nx / ftx) : 1;
     double* npart = (double*)c->buf(B_FCG_NPART, (size_t)nrhs * ntiles * 2 * 8);
 
     if (!have_guess) launch_zero<double>(x, (long long)nv, s);
@@ -2108,7 +2114,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     // dot products.  Three kernels; directions alternate between p32 and p32b by parity; the
     // last update of each right-hand side is applied at the end of the cycle
     // (k_fcg_flush_x).
-    const bool fused = c->net_fused && ha.cfg.lds && fused_shape(c->L, fty, ftx);
+    const bool fused = c->net_fused && ha.cfg.lds && fused_shape(c, fty, ftx);
     auto iteration_fused = [&](int par) {
         float* pin = par ? p32b : p32;
         float* pout = par ? p32 : p32b;

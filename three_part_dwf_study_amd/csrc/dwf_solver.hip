@@ -42,6 +42,21 @@ struct CgOperator {
                           double* part, const CgState* st, int slot, int skip) = 0;
     // Every value first()/second() pass to their kernels beyond their own arguments.
     virtual void signature(std::vector<unsigned long long>& k) const = 0;
+    // Pipelined FP32 iteration (optional): first_pipe forms p = r + beta p_old from the
+    // states and the two direction buffers (CgState::pbuf) inside its operator launch and
+    // computes t = Op1 p; second_pipe is second() with that direction; the iteration ends
+    // with the state update alone (launch_cg_flags).
+    virtual bool pipelined() const { return false; }
+    virtual void first_pipe(float* p0, float* p1, const float* r, float* t, double* part, CgState* st, int slot,
+                            int skip)
+    {
+        (void)p0, (void)p1, (void)r, (void)t, (void)part, (void)st, (void)slot, (void)skip;
+    }
+    virtual void second_pipe(const float* t, float* x, float* r, float* p0, float* p1, const double* partA,
+                             double* part, CgState* st, int slot, int skip)
+    {
+        (void)t, (void)x, (void)r, (void)p0, (void)p1, (void)partA, (void)part, (void)st, (void)slot, (void)skip;
+    }
 };
 
 using Sig = std::vector<unsigned long long>;
@@ -307,6 +322,30 @@ struct SchurOp : CgOperator {
             launch_eo_b<float>(true, true, EPI_CGUPD, b, nrhs, c->stream, c->groups_host());
         }
     }
+    bool pipelined() const override { return fused && c->eo_pipe; }
+    void first_pipe(float* p0, float* p1, const float* r, float* t, double* part, CgState* st, int slot,
+                    int skip) override
+    {
+        EoArgs<float> b = fused_args<float>(nullptr, st, slot, skip);
+        b.r = const_cast<float*>(r);
+        b.pd0 = p0;
+        b.pd1 = p1;
+        b.out = t;
+        b.part = part;
+        launch_eo_schur_pipe(b, tile, nrhs, c->stream, c->groups_host());
+    }
+    void second_pipe(const float* t, float* x, float* r, float* p0, float* p1, const double* partA, double* part,
+                     CgState* st, int slot, int skip) override
+    {
+        EoArgs<float> b = fused_args<float>(t, st, slot, skip);
+        b.x = x;
+        b.r = r;
+        b.pd0 = p0;
+        b.pd1 = p1;
+        b.partA = partA;
+        b.part = part;
+        launch_eo_schur<float>(true, EPI_CGUPD, b, tile, nrhs, c->stream, c->groups_host());
+    }
     void residual(const double* x, const double* b64, double* r64, double* p64, float* r32, float* p32, double* part,
                   const CgState* st, int slot, int skip) override
     {
@@ -342,7 +381,8 @@ struct SchurOp : CgOperator {
         k.insert(k.end(), {2ULL, (unsigned long long)nrhs, (unsigned long long)bx, (unsigned long long)chunks,
                            (unsigned long long)spt, sig_ptr(te32), sig_ptr(te64), sig_ptr(q64),
                            (unsigned long long)fused, (unsigned long long)tile.ty, (unsigned long long)tile.tx,
-                           (unsigned long long)tile.nch, (unsigned long long)tile.spt});
+                           (unsigned long long)tile.nch, (unsigned long long)tile.spt,
+                           (unsigned long long)pipelined()});
     }
 };
 
@@ -403,11 +443,14 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     CgState* st = (CgState*)c->buf(B_STATE, sizeof(CgState) * nrhs);
     double *p64 = nullptr, *t64 = nullptr;
     float *r32 = nullptr, *t32 = nullptr, *p32 = nullptr, *x32 = nullptr;
+    const bool pipe = mixed && op.pipelined();
+    float* p32b = nullptr;
     if (mixed) {
         r32 = (float*)c->buf(B_R32, nv * 4);
         t32 = (float*)c->buf(B_T32, nv * 4);
         p32 = (float*)c->buf(B_P32, nv * 4);
         x32 = (float*)c->buf(B_X32, nv * 4);
+        if (pipe) p32b = (float*)c->buf(B_P32B, nv * 4);
     } else {
         p64 = (double*)c->buf(B_P64, nv * 8);
         t64 = (double*)c->buf(B_TQ64, nv * 8);
@@ -429,6 +472,7 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     const double delta = c->reliable_delta;
     launch_cg_start(op.nblk, partR, st, rtol, delta, nrhs, s);
     if (pre) k_mark_done<<<(nrhs + 63) / 64, 64, 0, s>>>(st, pre, nrhs);
+    if (pipe) launch_set_fresh(st, nrhs, s);
     if (mixed) launch_zero<float>(x32, (long long)nv, s);
     c->check_launch("solver start");
 
@@ -447,7 +491,11 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     const int mode = mixed ? MODE_MIXED : MODE_CG64;
     auto iteration = [&](int i) {
         const int slot = i & 1, skip = F_DONE | F_FROZEN;
-        if (mixed) {
+        if (pipe) {
+            op.first_pipe(p32, p32b, r32, t32, partA, st, slot, skip);
+            op.second_pipe(t32, x32, r32, p32, p32b, partA, partB, st, slot, skip);
+            launch_cg_flags(op.nblk, partA, partB, st, slot, mode, nrhs, s);
+        } else if (mixed) {
             op.first(false, p32, t32, partA, st, slot, skip);
             op.second(false, t32, x32, r32, p32, partA, partB, st, slot, skip);
             launch_cg_p<float>(op.lay, op.nblk, r32, p32, partA, partB, st, slot, mode, nrhs, s);
@@ -464,7 +512,7 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
         Sig key{(unsigned long long)mixed, (unsigned long long)nrhs, (unsigned long long)nv,
                 (unsigned long long)op.nblk, (unsigned long long)mode, sig_ptr(x),   sig_ptr(r64),  sig_ptr(p64),
                 sig_ptr(t64),            sig_ptr(r32),            sig_ptr(t32),  sig_ptr(p32), sig_ptr(x32),
-                sig_ptr(partA),          sig_ptr(partB),          sig_ptr(st)};
+                sig_ptr(partA),          sig_ptr(partB),          sig_ptr(st),   sig_ptr(p32b)};
         op.signature(key);
         block = c->cached_graph(key, [&] {
             for (int i = 0; i < kGraphIters; ++i) iteration(i);
@@ -555,7 +603,7 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
                          "hold upload");
                 launch_hold(st, hold_d, slot, nrhs, s);
                 const int skip = F_DONE | F_HOLD;
-                launch_accum_x(op.lay, x, x32, st, slot, skip, nrhs, s);
+                    launch_accum_x(op.lay, x, x32, st, slot, skip, nrhs, s);
                 op.residual(x, b, r64, nullptr, r32, nullptr, partR, st, slot, skip);
                 launch_ru_commit(op.nblk, partR, st, slot, nrhs, s);
                 c->check_launch("reliable update");
