@@ -31,7 +31,11 @@ baseline solves the 34 sources of one configuration per size with the oracle, sp
 over every OpenMP thread.
 
 With --workload learned: only the learned path and its comparisons (FP64 CG, even-odd
-mixed) on a held-out configuration, with the training statistics.
+mixed, full-lattice mixed) on held-out configurations, with the training statistics and
+the FP32 D apply's bandwidth.  --lattice reference-default runs it on the reference's own
+default lattice (parameters.txt:25-30: 32x50, N5 = 12, M5 = 1, mf = 0.2, beta = 3), where
+no LDS tile of the D stencil fits (a tile's width divides Nx and the block size), so the
+stencils take the direct-load kernel and the learned iteration its unfused form.
 
 With --workload hmc a step is one Nf=2 domain-wall HMC trajectory at 96x96, Ls=16,
 beta=4, mf=0.05, M5=1, tau=1 with 20 leapfrog steps (epsilon=0.05), from a hot start
@@ -40,6 +44,7 @@ with chronological initial guesses; the gauge update, forces and Delta H stay in
 the device.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload solves|analyzer|learned|hmc]
+                      [--lattice bench|reference-default]
 """
 
 import argparse
@@ -525,7 +530,7 @@ def run_learned(args, dwf, capi):
     d_x = capi.DeviceArray(nrhs * V5 * 32)
     results, shapes = {}, {}
     for name, method in (("learned", dwf.DWF_SOLVE_LEARNED), ("fp64_cg", dwf.DWF_SOLVE_CG64),
-                         ("eo_mixed", dwf.DWF_SOLVE_EO_MIXED)):
+                         ("eo_mixed", dwf.DWF_SOLVE_EO_MIXED), ("mixed", dwf.DWF_SOLVE_MIXED)):
         ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA,
                  args.reliable_delta if method == dwf.DWF_SOLVE_LEARNED else LIB_RELIABLE_DELTA)
         shapes[name] = solve_shape(
@@ -547,6 +552,11 @@ def run_learned(args, dwf, capi):
                              fp64_iterations=float(np.mean([q["fp64_iterations"] for q in st])),
                              max_true_relres=tr)
         log(f"{name}: {results[name]}")
+    # the FP32 D apply at 128 right-hand sides in the autotuner's shape (on a lattice no LDS
+    # tile covers, the direct-load kernel: "lds" 0)
+    d128 = ctx.autotune(args.dslash_rhs, dwf.DWF_PREC_FP32, args.tune_cache)
+    ms_d, gbs_d = time_dslash(ctx, dwf, capi, dwf.DWF_PREC_FP32, args.dslash_rhs, 20)
+    log(f"dslash fp32 x{args.dslash_rhs}: {ms_d:.4f} ms {gbs_d:.0f} GB/s, shape {d128}")
     fast, judge = results["learned"], results["fp64_cg"]
     line = {
         "metric": METRIC,
@@ -566,14 +576,19 @@ def run_learned(args, dwf, capi):
                         f"from {tstats['training_configs']} configurations (full-volume Gaussian sources), tested "
                         f"on {nrhs} held-out configurations (seeds 1234 + 104729 i, i = {i0} .. {i0 + nrhs - 1}), "
                         f"one boundary source each, solved as one batch per step, FP64 true residual <= 1e-10",
-            "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs,
+            "Nt": NT, "Nx": NX, "Ls": LS, "mf": MF, "beta": BETA, "rhs_per_step": nrhs,
+            "lattice": args.lattice,
             "network": "3 linear parallel-transport conv layers 2-8-8-2 complex channels, BF16 MFMA",
             "solver": solver,
         },
         "training": tstats,
         "fp64_cg_solves_per_s": round(judge["solves_per_s"], 3),
         "eo_mixed_solves_per_s": round(results["eo_mixed"]["solves_per_s"], 3),
+        "mixed_solves_per_s": round(results["mixed"]["solves_per_s"], 3),
         "speedup_vs_fp64_cg": round(fast["solves_per_s"] / judge["solves_per_s"], 3),
+        "dslash_fp32_gbs": {"batch%d" % args.dslash_rhs: round(gbs_d, 1),
+                            "frac_of_8TBs": round(gbs_d / HBM_PEAK_GBS, 4),
+                            "shape": {k: d128[k] for k in ("tile_x", "prefetch", "chunks", "nt_store", "lds")}},
         "iterations": {k: v["iterations"] for k, v in results.items()},
         "iteration_ratio_vs_fp64_cg": round(fast["iterations"] / judge["iterations"], 3),
         "max_true_relres": {k: v["max_true_relres"] for k, v in results.items()},
@@ -717,6 +732,9 @@ def main():
     ap.add_argument("--fcg-beta", type=int, default=1, choices=(0, 1),
                     help="learned solver direction update: 0 flexible CG (FCG(1)), 1 Fletcher-Reeves")
     ap.add_argument("--analyzer-configs", type=int, default=32, help="configurations per lattice size")
+    ap.add_argument("--lattice", choices=("bench", "reference-default"), default="bench",
+                    help="learned workload's lattice: 96x96 Ls=16, mf 0.05, beta 4, or the reference's own default "
+                         "(parameters.txt:25-30: 32x50, N5 = 12, mf 0.2, beta 3), which no LDS stencil tile covers")
     ap.add_argument("--tune-cache", default=None, help="autotuner result file (reused when present)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch CG iterations one by one instead of replaying HIP graphs (for tracing tools)")
@@ -734,6 +752,11 @@ def main():
     if args.workload == "analyzer":
         run_analyzer(args, dwf)
         return
+    if args.lattice == "reference-default":
+        if args.workload != "learned":
+            raise SystemExit("--lattice reference-default applies to --workload learned")
+        global NT, NX, LS, MF, BETA
+        NT, NX, LS, MF, BETA = 32, 50, 12, 0.2, 3.0
     if args.workload == "learned":
         run_learned(args, dwf, capi)
         return

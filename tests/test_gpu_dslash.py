@@ -184,3 +184,27 @@ def test_autotune_and_cache(tmp_path):
         assert all(r1[k] == r2[k] for k in ("tile_x", "prefetch", "chunks", "nt_store", "lds"))
         got = ctx.apply(0, psi)
         assert rel(got, oracle.dwf_apply(0, th, psi, Nt, Nx, Ls, M5, MF)) <= 1e-12
+
+
+def test_autotune_binds_its_batch_to_the_configurations():
+    """The autotuner times applies of its own batch size: with several configurations held
+    it maps its right-hand sides onto them (nrhs / ncfg each), whatever batch the context's
+    previous call used (a 4-RHS solve on 4 configurations before a 16-RHS autotune: 4 per
+    configuration, not 1), and rejects a batch that is not a multiple of them.  Afterwards
+    a 16-RHS apply matches the oracle configuration by configuration (right-hand side r
+    on configuration r // 4)."""
+    Nt, Nx, Ls, ncfg = 8, 10, 4, 4
+    print_case(test="autotune_batch_map")
+    rng = np.random.default_rng(81)
+    ths = np.stack([rand_theta(rng, Nt, Nx) for _ in range(ncfg)])
+    src = rand_spinor(rng, ncfg * Ls * Nt * Nx * 2).reshape(ncfg, -1)
+    psi = rand_spinor(rng, 4 * ncfg * Ls * Nt * Nx * 2).reshape(4 * ncfg, -1)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge_batch(ths)
+        ctx.solve_dirac(src, rtol=1e-8, maxiter=5000, method=dwf.DWF_SOLVE_MIXED)  # one RHS per configuration
+        ctx.autotune(4 * ncfg, dwf.DWF_PREC_FP32)
+        with pytest.raises(capi.DwfError):
+            ctx.autotune(4 * ncfg + 1, dwf.DWF_PREC_FP32)
+        got = ctx.apply(0, psi)
+    for r in range(4 * ncfg):
+        assert rel(got[r], oracle.dwf_apply(0, ths[r // 4], psi[r], Nt, Nx, Ls, M5, MF)) <= 1e-12

@@ -98,6 +98,9 @@ extern "C" int dwf_autotune(dwf_ctx* c, int nrhs, int precision, const char* cac
             throw DwfError(DWF_ERR_ARG, "dwf_autotune: bad argument");
         if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_autotune: no gauge field set");
         c->require_single_lattice("dwf_autotune");
+        // the timed applies' right-hand side -> configuration map (nrhs / ncfg per
+        // configuration), not the one the context's previous call left behind
+        c->bind_rhs(nrhs);
         const std::string key = key_of(c, nrhs, precision);
         Shape best{};
         double best_us = 0;
