@@ -43,7 +43,10 @@ theta ~ U[-pi, pi) (seed 1234), the host random stream seeded 1234, force solves
 with chronological initial guesses; the gauge update, forces and Delta H stay in FP64 on
 the device.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload solves|analyzer|learned|hmc]
+With --workload eo: the even-odd solver alone on the default batch (for kernel traces and
+counter runs of its kernels), with its fraction of the Schur CG's HBM ceiling.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload solves|analyzer|learned|hmc|eo]
                       [--lattice bench|reference-default]
 """
 
@@ -602,6 +605,48 @@ def run_learned(args, dwf, capi):
     net.close()
 
 
+def run_eo(args, dwf, capi):
+    """EO_MIXED alone on the bench's batch (32 boundary sources, 96x96, Ls=16, seed-1234
+    configuration): solves/s, iterations and the fraction of the Schur CG's HBM ceiling
+    (eo_iteration_bytes), for kernel traces and counter runs of the even-odd kernels."""
+    theta = np.random.default_rng(1234).normal(0.0, 1.0 / np.sqrt(BETA), size=2 * NT * NX)
+    nrhs = args.nrhs
+    ctx = dwf.DwfContext.dwf(NT, NX, LS, M5, MF)
+    ctx.set_gauge(theta)
+    if args.no_graphs:
+        ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
+    src = boundary_sources(nrhs, 918273)
+    d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
+    d_x = capi.DeviceArray(src.size * 16)
+    for _ in range(args.warmup):
+        ctx.solve_dirac_device(dwf.DWF_SOLVE_EO_MIXED, d_src.ptr, None, TOL, 20000, nrhs, d_x.ptr)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st, _ = ctx.solve_dirac_device(dwf.DWF_SOLVE_EO_MIXED, d_src.ptr, None, TOL, 20000, nrhs, d_x.ptr)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    tr = max(q["true_relres"] for q in st)
+    if tr > TOL or not all(q["converged"] for q in st):
+        raise SystemExit(f"eo_mixed solve failed certification: max true residual {tr:.3e}")
+    its = float(np.mean([q["iterations"] for q in st]))
+    sps = args.steps * nrhs / dt
+    ceiling = 8e12 / (its * eo_iteration_bytes())
+    line = {
+        "metric": "DWF EO_MIXED solves/s 96^2,Ls=16 @1e-10", "value": round(sps, 3), "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": "even-odd Schur complement solves (fused Schur operator, pipelined FP32 CG, FP64 "
+                               "reliable updates), 32 boundary sources, 96x96 Ls=16, certified at 1e-10",
+                   "Nt": NT, "Nx": NX, "Ls": LS, "rhs_per_step": nrhs},
+        "iterations": its, "max_true_relres": tr,
+        "hbm_ceiling_solves_per_s": round(ceiling, 1), "fraction_of_ceiling": round(sps / ceiling, 4),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def run_hmc(args, dwf, capi):
     # hot start theta ~ U[-pi, pi) (seed 1234), or the reference's cold start theta = 1
     # (domain_wall/generate_configs.cpp:27-28)
@@ -714,7 +759,7 @@ def main():
     ap.add_argument("--analyzer-split", action="store_true",
                     help="analyzer: one context per lattice size on concurrent streams instead of one launch")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("solves", "analyzer", "learned", "hmc"), default="solves")
+    ap.add_argument("--workload", choices=("solves", "analyzer", "learned", "hmc", "eo"), default="solves")
     ap.add_argument("--hmc-solver", choices=("cg64", "mixed", "learned"), default="mixed")
     ap.add_argument("--hmc-start", choices=("hot", "cold"), default="hot",
                     help="HMC start: theta ~ U[-pi, pi) (seed 1234) or the reference's theta = 1")
@@ -762,6 +807,9 @@ def main():
         return
     if args.workload == "hmc":
         run_hmc(args, dwf, capi)
+        return
+    if args.workload == "eo":
+        run_eo(args, dwf, capi)
         return
     theta = np.random.default_rng(1234).normal(0.0, 1.0 / np.sqrt(BETA), size=2 * NT * NX)
     if args.kernels_only:
