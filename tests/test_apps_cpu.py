@@ -146,6 +146,15 @@ def test_wilson_statistics_errors(tmp_path):
     assert rc == 1 and "No contiguous effective-mass window passed the flatness test" in err
 
 
+@pytest.mark.parametrize("prog", ["generate_domain_wall", "generate_wilson"])
+def test_force_solver_rejects_eo_mixed(tmp_path, prog):
+    """The HMC force solves run on D D^dag, which the even-odd solver does not take: the
+    programs stop before any device work with a message naming the key."""
+    write_parameters(tmp_path, gpu__force_solver="eo_mixed")
+    rc, out, err = run(prog, tmp_path)
+    assert rc == 1 and "Invalid value for gpu.force_solver: eo_mixed" in err, err
+
+
 def test_learned_solver_needs_weights(tmp_path):
     write_parameters(tmp_path, gpu__force_solver="learned")
     rc, out, err = run("generate_domain_wall", tmp_path)

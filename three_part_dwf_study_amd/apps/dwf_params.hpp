@@ -47,7 +47,7 @@ struct DwfParams {
         eta_random_seed = p.unsigned_integer("eta.random_seed");
         max_momentum = p.integer("analysis.max_momentum");
         random_seed = p.unsigned_integer("dwf.random_seed");
-        force_method = solver_method(p, "gpu.force_solver", 0);
+        force_method = force_solver_method(p, "gpu.force_solver");
         propagator_method = solver_method(p, "gpu.propagator_solver", 0);
         analysis_batch = std::stoi(p.text_or("gpu.analysis_batch", "0"));
         if (analysis_batch < 0) throw std::runtime_error("gpu.analysis_batch must be >= 0");
@@ -103,7 +103,7 @@ struct WilsonParams {
         earliest_plateau_time = p.integer("analysis.earliest_plateau_time");
         analysis_threads = p.integer("analysis.threads");
         random_seed = p.unsigned_integer("wilson.random_seed");
-        force_method = solver_method(p, "gpu.force_solver", 0);
+        force_method = force_solver_method(p, "gpu.force_solver");
         propagator_method = solver_method(p, "gpu.propagator_solver", 0);
     }
 };
