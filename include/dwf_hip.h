@@ -124,9 +124,11 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * (exact for a symmetric preconditioner; one vector pass less per iteration);
  * DWF_TUNE_EO_FUSED: the Schur operator of DWF_SOLVE_EO_MIXED, 0 two kernels per
  * application with A^-1 H psi through device memory, 1 one kernel (A^-1 H psi kept in
- * LDS; same arithmetic), 2 (default) one kernel with the FP32 CG's direction update
+ * LDS; same arithmetic), 2 one kernel with the FP32 CG's direction update
  * p = r + beta p inside the next operator launch (one vector pass and one launch less per
- * iteration); DWF_TUNE_NET_TILE: the fused network's output tile,
+ * iteration), 3 (default) as 2 with the CG's per-iteration state update also done inside
+ * the next operator launch in the replayed blocks of 8 iterations (7 launches less per
+ * block); DWF_TUNE_NET_TILE: the fused network's output tile,
  * ty * 256 + tx sites (ty <= 8, tx <= 16, dividing Nt and Nx), 0 (default) the largest
  * that divides. */
 enum dwf_tune_param {

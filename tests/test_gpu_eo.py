@@ -77,6 +77,46 @@ def test_eo_fused_matches_two_kernel(Nt, Nx, Ls, nrhs):
         assert np.linalg.norm(x2[i] - x0[i]) / np.linalg.norm(x0[i]) < 1e-6
 
 
+@pytest.mark.parametrize("graphs", [1, 0])
+@pytest.mark.parametrize("Nt,Nx,Ls,nrhs,groups", [(12, 10, 8, 3, False), (96, 96, 16, 4, False), (8, 8, 4, 0, True)])
+def test_eo_state_fold_matches_state_kernel(Nt, Nx, Ls, nrhs, groups, graphs):
+    """DWF_TUNE_EO_FUSED 3 (the default): in the replayed blocks of 8 iterations the
+    operator launch that forms the direction also does the previous iteration's state
+    update (convergence, freezing, beta, the direction buffer), which 2 launches as a kernel
+    of its own.  The same sums in the same order, so the same iterations and bit-identical
+    solutions, with graphs (the fold) and without (no fold: the state kernel every
+    iteration), on one lattice and on lattice groups of several sizes."""
+    print_case(test="eo_state_fold", Nt=Nt, Nx=Nx, Ls=Ls, nrhs=nrhs, groups=groups, graphs=graphs)
+    out = []
+    for var in (2, 3):
+        if groups:
+            # two configurations of 8 x 8 and one of 12 x 12, two right-hand sides each
+            r2 = np.random.default_rng(Nt * 5 + Nx + nrhs)
+            grp = [(8, 8, np.stack([rand_theta(r2, 8, 8) for _ in range(2)])),
+                   (12, 12, rand_theta(r2, 12, 12)[None, :])]
+            src = np.concatenate([rand_spinor(r2, Ls * a * b * 2) for a, b, th in grp for _ in range(2 * len(th))])
+            with dwf.DwfContext.dwf(8, 8, Ls, M5, MF) as ctx:
+                ctx.set_gauge_groups(grp)
+                ctx.tune(dwf.DWF_TUNE_GRAPHS, graphs)
+                ctx.tune(dwf.DWF_TUNE_EO_FUSED, var)
+                out.append(ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, method=dwf.DWF_SOLVE_EO_MIXED))
+        else:
+            r2 = np.random.default_rng(Nt * 5 + Nx + nrhs)
+            th = rand_theta(r2, Nt, Nx)
+            src = rand_spinor(r2, nrhs * Ls * Nt * Nx * 2).reshape(nrhs, -1)
+            with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+                ctx.set_gauge(th)
+                ctx.tune(dwf.DWF_TUNE_GRAPHS, graphs)
+                ctx.tune(dwf.DWF_TUNE_EO_FUSED, var)
+                out.append(ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, method=dwf.DWF_SOLVE_EO_MIXED))
+    (xa, sta, rca), (xb, stb, rcb) = out
+    assert rca == 0 and rcb == 0
+    assert np.array_equal(np.asarray(xa), np.asarray(xb))
+    for a, b in zip(sta, stb):
+        assert a["converged"] and b["converged"] and b["true_relres"] <= 1e-10
+        assert a["iterations"] == b["iterations"] and a["true_relres"] == b["true_relres"], (a, b)
+
+
 def test_eo_matches_oracle_cg():
     Nt, Nx, Ls = 8, 8, 8
     print_case(test="eo_vs_oracle")

@@ -32,6 +32,7 @@ enum BufId : int {
     B_P32B,      // pipelined even-odd CG: the second direction buffer
     B_X32,
     B_PA,
+    B_PA2,       // pipelined even-odd CG: the operator's partials of odd iterations
     B_PB,
     B_PR,
     B_STATE,
@@ -149,7 +150,8 @@ struct dwf_ctx {
     int net_fused = 2;            // learned preconditioner: 0 layer by layer, 1 fused, 2 fused in a gauge per block
     int fcg_beta = 0;             // learned solver: 0 flexible CG (Notay's FCG(1)), 1 Fletcher-Reeves
     int eo_fused = 1;             // Schur operator: 1 one kernel per application, 0 two
-    int eo_pipe = 1;              // fused Schur CG: 1 direction update inside the operator launch
+    int eo_pipe = 2;              // fused Schur CG: 1 direction update inside the operator launch,
+                                  // 2 also the state update inside the next one (graph blocks)
     int net_tile = 0;             // fused network's output tile ty * 256 + tx (0: the largest that divides)
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs

@@ -986,7 +986,8 @@ __global__ void k_cg_start(const int nblk, const double* partR, CgState* st, con
 // The bookkeeping of k_cg_p without its vector pass, for the pipelined even-odd CG (whose
 // next operator launch forms p = r + beta p itself): convergence, freezing and the states
 // of slot ^ 1, beta for the next direction, and the direction buffer flip.  One 64-thread
-// block per right-hand side.
+// block per right-hand side.  The operator launch of the next iteration can do the same
+// itself (k_eo_schur's fold: the same arithmetic, in every block).
 __global__ void k_cg_flags(const int nblk, const double* partA, const double* partB, CgState* st, const int slot,
                            const int mode)
 {
@@ -998,6 +999,7 @@ __global__ void k_cg_flags(const int nblk, const double* partA, const double* pa
         if (threadIdx.x == 0) {
             S.flags[ns] = f;
             S.rr[ns] = S.rr[slot];
+            S.pbuf[ns] = S.pbuf[slot];
         }
         return;
     }
@@ -1017,6 +1019,7 @@ __global__ void k_cg_flags(const int nblk, const double* partA, const double* pa
         S.status = 4;
         nf |= F_DONE;
         S.rr[ns] = rr_old;
+        S.pbuf[ns] = S.pbuf[slot];
     } else {
         const double relres = std::sqrt(rr_new) / std::sqrt(S.bnorm2);
         const bool conv = relres <= S.tol;
@@ -1024,7 +1027,7 @@ __global__ void k_cg_flags(const int nblk, const double* partA, const double* pa
         S.relres = relres;
         S.iters += 1;
         S.beta = rr_new / rr_old;
-        S.pbuf ^= 1;  // this iteration's direction (written to the other buffer) is current
+        S.pbuf[ns] = S.pbuf[slot] ^ 1;  // this iteration's direction (written to the other buffer) is current
         if (mode == MODE_CG64 && conv) nf |= F_DONE;
         if (mode == MODE_MIXED && (conv || rr_new <= S.delta * S.delta * S.rr_ref)) nf |= F_FROZEN;
     }
@@ -1038,7 +1041,7 @@ __global__ void k_set_fresh(CgState* st, const int nrhs)
     if (i < nrhs) {
         st[i].flags[0] |= F_FRESH;
         st[i].beta = 0.0;
-        st[i].pbuf = 0;
+        st[i].pbuf[0] = st[i].pbuf[1] = 0;
     }
 }
 

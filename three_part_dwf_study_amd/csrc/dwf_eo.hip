@@ -550,7 +550,49 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
         rb = (p.site0 / 2 + (long long)rl * (L.V4 / 2) * LS) * 4;
         links = static_cast<const C*>(a.links) + p.link0 + (long long)(rl / p.rpc) * 2 * L.V4;
     }
-    if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
+    // PIPE: the flags, beta and direction buffer this iteration starts from -- from the
+    // state, or (fold) computed here from the previous iteration's partials exactly as
+    // k_cg_flags does, by every wave alike; block 0 of the right-hand side records them.
+    // The fold runs after this block's loads of r and p_old are issued, in wave sums with
+    // no barrier, so the loads stay in flight meanwhile.
+    int pflags = 0, pb = 0;
+    R pbeta = 0;
+    bool fold_pending = false;
+    const bool lead = tile == 0 && threadIdx.x == 0;
+    // fold: the previous slot's state, read here (before psi's loads) for the same reason
+    int f_prev = 0, it_prev = 0, pb_prev = 0;
+    double rr_prev = 0.0, bn2 = 1.0, tol = 0.0, del2rr = 0.0;
+    if constexpr (PIPE) {
+        const CgState& S = a.st[rhs];
+        if (!a.fold) {
+            pflags = S.flags[a.slot];
+            pbeta = (R)S.beta;
+            pb = S.pbuf[a.slot];
+            if (pflags & a.skip) return;  // block-uniform
+        } else {
+            const int ps = a.slot ^ 1, f = S.flags[ps];
+            if (f & (F_DONE | F_FROZEN)) {  // block-uniform: carried to this slot
+                if (lead) {
+                    CgState& W = a.stw[rhs];
+                    W.rr[a.slot] = S.rr[ps];
+                    W.pbuf[a.slot] = S.pbuf[ps];
+                    W.flags[a.slot] = f;
+                }
+                return;
+            }
+            f_prev = f;
+            pb_prev = S.pbuf[ps];
+            it_prev = S.iters;
+            rr_prev = S.rr[ps];
+            bn2 = S.bnorm2;
+            tol = S.tol;
+            del2rr = S.delta * S.delta * S.rr_ref;
+            pb = pb_prev ^ 1;  // the direction the previous iteration formed is current
+            fold_pending = true;
+        }
+    } else {
+        if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
+    }
     R alpha = 0;
     if constexpr (EPI == EPI_CGUPD) {
         const double pAp = sum_partials(a.partA + (size_t)rhs * pstride * 2, pstride, sh);
@@ -574,28 +616,95 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
     Sp<R>* P = w;  // [LS][nreg], later overwritten by w [LS][nbox]
     {
         const int r = threadIdx.x >> 1, h = threadIdx.x & 1;
-        if (r < nreg) {
-            const int ir = r / WR, jr = 2 * (r - ir * WR) + ((ir + a.tpar) & 1);
+        const bool act = r < nreg;
+        long long i0 = 0;
+        bool own = false;
+        Sp<R> v[LH], po[LH];
+        // PIPE: p = r + beta p_old (k_cg_p's update), on the fly; the tile's own sites are
+        // stored to the new direction buffer (a wrapped duplicate outside the tile is not).
+        // A folding launch never starts a solve, so it always has a p_old.
+        const bool with_old = PIPE && (fold_pending || !(pflags & F_FRESH));
+        // Loads are issued by every thread (a thread past the region's end loads its last site
+        // again and stores nothing) and p_old always (in bounds; unused on a fresh start): no
+        // branch around them, so the waits below count them exactly (at a join the compiler
+        // would wait for every load in flight)
+        // the fold's partials, loaded before psi so that waiting for them (vmcnt counts loads
+        // in issue order) leaves psi's loads in flight: entries lane and lane + 64, clamped
+        // (a remainder loop covers longer partial arrays)
+        double fa0 = 0.0, fa1 = 0.0, fb0 = 0.0, fb1 = 0.0;
+        if constexpr (PIPE) {
+            if (fold_pending) {
+                const int lane = threadIdx.x & 63;
+                const double* pa = a.partAp + (size_t)rhs * pstride * 2;
+                const double* pbp = a.partB + (size_t)rhs * pstride * 2;
+                const int i0c = min(lane, pstride - 1), i1c = min(lane + 64, pstride - 1);
+                fa0 = pa[2 * i0c];
+                fb0 = pbp[2 * i0c];
+                fa1 = pa[2 * i1c];
+                fb1 = pbp[2 * i1c];  // masked where summed (a use here would wait for them at once)
+            }
+        }
+        {
+            const int rc = act ? r : nreg - 1;
+            const int ir = rc / WR, jr = 2 * (rc - ir * WR) + ((ir + a.tpar) & 1);
             int t = t0 + ir - 2, x = x0 + jr - 2;
             t = t < 0 ? t + L.Nt : (t >= L.Nt ? t - L.Nt : t);
             x = x < 0 ? x + L.Nx : (x >= L.Nx ? x - L.Nx : x);
-            const long long i0 = (long long)h * LH * V4h + t * Nxh + (x >> 1);
-            Sp<R> v[LH];
+            i0 = (long long)h * LH * V4h + t * Nxh + (x >> 1);
+            own = act && ir >= 2 && ir < TY + 2 && jr >= 2 && jr < TX + 2;
+            const R* src = PIPE ? a.r + rb + i0 * 4 : in + i0 * 4;
+#pragma unroll
+            for (int k = 0; k < LH; ++k) v[k] = ld(src, (long long)k * V4h);
             if constexpr (PIPE) {
-                // p = r + beta p_old (k_cg_p's update), on the fly; the tile's own sites are
-                // stored to the new direction buffer (a wrapped duplicate outside the tile is not)
-                const CgState& S = a.st[rhs];
-                const bool fresh = (S.flags[a.slot] & F_FRESH) != 0;
-                const R beta = (R)S.beta;
-                const R* rsrc = a.r + rb + i0 * 4;
-                const bool own = ir >= 2 && ir < TY + 2 && jr >= 2 && jr < TX + 2;
+                const R* psrc = (pb ? a.pd1 : a.pd0) + rb + i0 * 4;
 #pragma unroll
-                for (int k = 0; k < LH; ++k) v[k] = ld(rsrc, (long long)k * V4h);
-                if (!fresh) {
-                    const R* psrc = (S.pbuf ? a.pd1 : a.pd0) + rb + i0 * 4;
-                    Sp<R> po[LH];
-#pragma unroll
-                    for (int k = 0; k < LH; ++k) po[k] = ld(psrc, (long long)k * V4h);
+                for (int k = 0; k < LH; ++k) po[k] = ld(psrc, (long long)k * V4h);
+            }
+        }
+        if constexpr (PIPE) {
+            if (fold_pending) {
+                const int lane = threadIdx.x & 63;
+                const bool m0 = lane < pstride, m1 = lane + 64 < pstride;
+                double sa = (m0 ? fa0 : 0.0) + (m1 ? fa1 : 0.0), sb = (m0 ? fb0 : 0.0) + (m1 ? fb1 : 0.0);
+                if (pstride > 128) {
+                    const double* pa = a.partAp + (size_t)rhs * pstride * 2;
+                    const double* pbp = a.partB + (size_t)rhs * pstride * 2;
+                    for (int i = lane + 128; i < pstride; i += 64) {
+                        sa += pa[2 * i];
+                        sb += pbp[2 * i];
+                    }
+                }
+                const double pAp = wave_sum(sa), rr_new = wave_sum(sb);  // the same in every wave
+                int nf = f_prev & ~F_FRESH;
+                CgState& W = a.stw[rhs];
+                if (pAp == 0.0) {
+                    nf |= F_DONE;
+                    if (lead) {
+                        W.status = 4;
+                        W.rr[a.slot] = rr_prev;
+                        W.pbuf[a.slot] = pb_prev;
+                    }
+                } else {
+                    const double relres = std::sqrt(rr_new) / std::sqrt(bn2);
+                    // the mixed solver's freeze (the pipelined CG runs only there)
+                    if (relres <= tol || rr_new <= del2rr) nf |= F_FROZEN;
+                    const double beta = rr_new / rr_prev;
+                    pbeta = (R)beta;
+                    if (lead) {
+                        W.rr[a.slot] = rr_new;
+                        W.relres = relres;
+                        W.iters = it_prev + 1;
+                        W.beta = beta;
+                        W.pbuf[a.slot] = pb;
+                    }
+                }
+                pflags = nf;
+                if (lead) W.flags[a.slot] = nf;
+                if (pflags & a.skip) return;  // block-uniform (every wave computed the same flags)
+            }
+            if (act) {
+                if (with_old) {
+                    const R beta = pbeta;
 #pragma unroll
                     for (int k = 0; k < LH; ++k) {
                         v[k].ar = v[k].ar + beta * po[k].ar;
@@ -605,15 +714,13 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
                     }
                 }
                 if (own) {
-                    R* pdst = (S.pbuf ? a.pd0 : a.pd1) + rb + i0 * 4;
+                    R* pdst = (pb ? a.pd0 : a.pd1) + rb + i0 * 4;
 #pragma unroll
                     for (int k = 0; k < LH; ++k) st(pdst, (long long)k * V4h, v[k]);
                 }
-            } else {
-                const R* src = in + i0 * 4;
-#pragma unroll
-                for (int k = 0; k < LH; ++k) v[k] = ld(src, (long long)k * V4h);
             }
+        }
+        if (act) {
 #pragma unroll
             for (int k = 0; k < LH; ++k) P[(h * LH + k) * nreg + r] = v[k];
         }
@@ -683,10 +790,10 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
     // ---- phase 2: the operator on the tile's target sites (ca = 1)
     double acc0 = 0.0, acc1 = 0.0;
     // the direction of x += alpha p: in the pipelined CG, this iteration's (pd[pbuf ^ 1];
-    // the state update after this launch flips pbuf)
+    // the next state update records pbuf ^ 1 for the next slot)
     const R* pdir = a.p;
     if constexpr (EPI == EPI_CGUPD)
-        if (a.pd0) pdir = a.st[rhs].pbuf ? a.pd0 : a.pd1;
+        if (a.pd0) pdir = a.st[rhs].pbuf[a.slot] ? a.pd0 : a.pd1;
     if (act2) {
         const int hs = (t0 + i2) * Nxh + ((x0 + j2) >> 1);
         EoSite<R> g = eo_site<R>(L, Nxh, a.tpar, hs, links);
@@ -968,6 +1075,8 @@ void launch_eo_schur_pipe(const EoArgs<float>& a0, const EoTile& t, int nrhs, hi
     EoArgs<float> a = a0;
     if (!a.st || !a.r || !a.pd0 || !a.pd1 || !a.out || !a.part)
         throw DwfError(DWF_ERR_ARG, "pipelined even-odd launch: missing vectors or states");
+    if (a.fold && (!a.partAp || !a.partB || !a.stw || a.partAp == a.part))
+        throw DwfError(DWF_ERR_ARG, "pipelined even-odd launch: the state fold needs the previous partials");
     long long nb = 0;
     if (!a.groups) {
         nb = (long long)(a.L.V4 / (t.ty * t.tx)) * nrhs;

@@ -44,6 +44,14 @@ template <typename R> struct EoArgs {
     // CgState::pbuf naming the current one
     R* pd0;
     R* pd1;
+    // pipelined even-odd CG, the direction-forming launch: with fold set, it also does the
+    // previous iteration's state update (k_cg_flags) from that iteration's partials
+    // (partAp: its operator's |t|^2, partB: its r.r) into slot `slot` of stw (block 0 of each
+    // right-hand side writes; every block computes the same values)
+    int fold;
+    const double* partAp;
+    const double* partB;
+    CgState* stw;
 };
 
 // Tile of the fused Schur-operator kernel (k_eo_schur): ty x tx 4D sites (both even,

@@ -463,9 +463,9 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             ctx->fcg_beta = (int)value;
             return DWF_OK;
         case DWF_TUNE_EO_FUSED:
-            if (!(value == 0.0 || value == 1.0 || value == 2.0)) break;
+            if (!(value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0)) break;
             ctx->eo_fused = value != 0.0 ? 1 : 0;
-            ctx->eo_pipe = value == 2.0 ? 1 : 0;
+            ctx->eo_pipe = value >= 2.0 ? (int)value - 1 : 0;
             return DWF_OK;
         case DWF_TUNE_NET_TILE: {
             if (!(value >= 0.0 && value == (double)(int)value)) break;

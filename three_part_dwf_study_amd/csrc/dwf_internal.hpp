@@ -35,7 +35,8 @@ struct CgState {
     int fp64_iters;     // FP64 iterations (fallback)
     // pipelined even-odd CG (the direction update inside the next operator launch):
     double beta;        // r.r / r.r_prev of the last iteration, for the next direction
-    int pbuf;           // which of the two direction buffers holds the current direction
+    int pbuf[2];        // by slot: which of the two direction buffers holds the direction an
+                        // iteration of that slot starts from
 };
 
 // Lattice and operator constants.  Ls = 1 and d5 = 0 give the Wilson operator.

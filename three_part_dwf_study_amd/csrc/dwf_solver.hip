@@ -45,12 +45,14 @@ struct CgOperator {
     // Pipelined FP32 iteration (optional): first_pipe forms p = r + beta p_old from the
     // states and the two direction buffers (CgState::pbuf) inside its operator launch and
     // computes t = Op1 p; second_pipe is second() with that direction; the iteration ends
-    // with the state update alone (launch_cg_flags).
+    // with the state update alone (launch_cg_flags), or leaves it to the next first_pipe
+    // (fold: that launch does it from the previous iteration's partials partAp, partB).
     virtual bool pipelined() const { return false; }
     virtual void first_pipe(float* p0, float* p1, const float* r, float* t, double* part, CgState* st, int slot,
-                            int skip)
+                            int skip, int fold, const double* partAp, const double* partB)
     {
         (void)p0, (void)p1, (void)r, (void)t, (void)part, (void)st, (void)slot, (void)skip;
+        (void)fold, (void)partAp, (void)partB;
     }
     virtual void second_pipe(const float* t, float* x, float* r, float* p0, float* p1, const double* partA,
                              double* part, CgState* st, int slot, int skip)
@@ -324,7 +326,7 @@ struct SchurOp : CgOperator {
     }
     bool pipelined() const override { return fused && c->eo_pipe; }
     void first_pipe(float* p0, float* p1, const float* r, float* t, double* part, CgState* st, int slot,
-                    int skip) override
+                    int skip, int fold, const double* partAp, const double* partB) override
     {
         EoArgs<float> b = fused_args<float>(nullptr, st, slot, skip);
         b.r = const_cast<float*>(r);
@@ -332,6 +334,10 @@ struct SchurOp : CgOperator {
         b.pd1 = p1;
         b.out = t;
         b.part = part;
+        b.fold = fold;
+        b.partAp = partAp;
+        b.partB = partB;
+        b.stw = st;
         launch_eo_schur_pipe(b, tile, nrhs, c->stream, c->groups_host());
     }
     void second_pipe(const float* t, float* x, float* r, float* p0, float* p1, const double* partA, double* part,
@@ -445,6 +451,9 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     float *r32 = nullptr, *t32 = nullptr, *p32 = nullptr, *x32 = nullptr;
     const bool pipe = mixed && op.pipelined();
     float* p32b = nullptr;
+    // pipelined: the operator's partials by iteration parity (a launch that folds the previous
+    // iteration's state update reads that iteration's while it writes its own)
+    double* partA2 = pipe ? (double*)c->buf(B_PA2, npart * 8) : nullptr;
     if (mixed) {
         r32 = (float*)c->buf(B_R32, nv * 4);
         t32 = (float*)c->buf(B_T32, nv * 4);
@@ -459,6 +468,7 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     if (op.lay.gr.n > 1) {
         // a group with fewer blocks than the stride leaves partials unwritten: they must read 0
         launch_zero<double>(partA, (long long)npart, s);
+        if (partA2) launch_zero<double>(partA2, (long long)npart, s);
         launch_zero<double>(partB, (long long)npart, s);
         launch_zero<double>(partR, (long long)npart, s);
     }
@@ -489,12 +499,15 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     for (int i = 0; i < nrhs; ++i) c0_rel[i] = c->st_host[i].relres;
     std::vector<double> prev_rel(c0_rel);  // residual at the start of the previous cycle
     const int mode = mixed ? MODE_MIXED : MODE_CG64;
-    auto iteration = [&](int i) {
+    // fold: the pipelined operator launch also does the previous iteration's state update;
+    // flags: this iteration's state update is launched (else the next iteration folds it)
+    auto iteration = [&](int i, bool fold, bool flags) {
         const int slot = i & 1, skip = F_DONE | F_FROZEN;
         if (pipe) {
-            op.first_pipe(p32, p32b, r32, t32, partA, st, slot, skip);
-            op.second_pipe(t32, x32, r32, p32, p32b, partA, partB, st, slot, skip);
-            launch_cg_flags(op.nblk, partA, partB, st, slot, mode, nrhs, s);
+            double* pa = slot ? partA2 : partA;
+            op.first_pipe(p32, p32b, r32, t32, pa, st, slot, skip, fold ? 1 : 0, slot ? partA : partA2, partB);
+            op.second_pipe(t32, x32, r32, p32, p32b, pa, partB, st, slot, skip);
+            if (flags) launch_cg_flags(op.nblk, pa, partB, st, slot, mode, nrhs, s);
         } else if (mixed) {
             op.first(false, p32, t32, partA, st, slot, skip);
             op.second(false, t32, x32, r32, p32, partA, partB, st, slot, skip);
@@ -512,10 +525,15 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
         Sig key{(unsigned long long)mixed, (unsigned long long)nrhs, (unsigned long long)nv,
                 (unsigned long long)op.nblk, (unsigned long long)mode, sig_ptr(x),   sig_ptr(r64),  sig_ptr(p64),
                 sig_ptr(t64),            sig_ptr(r32),            sig_ptr(t32),  sig_ptr(p32), sig_ptr(x32),
-                sig_ptr(partA),          sig_ptr(partB),          sig_ptr(st),   sig_ptr(p32b)};
+                sig_ptr(partA),          sig_ptr(partB),          sig_ptr(st),   sig_ptr(p32b),
+                sig_ptr(partA2)};
         op.signature(key);
+        // pipelined: iterations 1..7 of the block fold their predecessor's state update, the
+        // block's last one launches its own (so a block always ends with complete states)
+        const bool fold = pipe && c->eo_pipe == 2;
+        key.push_back((unsigned long long)fold);
         block = c->cached_graph(key, [&] {
-            for (int i = 0; i < kGraphIters; ++i) iteration(i);
+            for (int i = 0; i < kGraphIters; ++i) iteration(i, fold && i > 0, !fold || i == kGraphIters - 1);
         });
     }
     while (!all_done(c->st_host, it & 1, nrhs)) {
@@ -525,7 +543,7 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
         if (block && (it & 1) == 0)
             for (; j + kGraphIters <= n; j += kGraphIters, it += kGraphIters)
                 c->check(hipGraphLaunch(block, s), "hipGraphLaunch");
-        for (; j < n; ++j, ++it) iteration(it);
+        for (; j < n; ++j, ++it) iteration(it, false, true);
         c->check_launch("CG iteration");
         if (mixed && op.lay.gr.n == 1) {
             // One lattice: the right-hand sides converge at similar rates, so every one not
