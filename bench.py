@@ -187,6 +187,31 @@ def solve_shape(args, ctx, dwf, solve_once, tuned, log):
     return shape
 
 
+def pick_streams(args, ctx, dwf, solve_once, log):
+    """Concurrent sub-batch streams of a batched solve (DWF_TUNE_STREAMS), chosen on whole
+    solves like the stencil shape: 1, 2 and 4, each after a warm-up solve, timed twice;
+    --streams fixes it."""
+    if args.streams != "auto":
+        ctx.tune(dwf.DWF_TUNE_STREAMS, int(args.streams))
+        return int(args.streams)
+    best = None
+    for k in (1, 2, 4):
+        ctx.tune(dwf.DWF_TUNE_STREAMS, k)
+        solve_once()
+        ctx.synchronize()
+        t = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            solve_once()
+            ctx.synchronize()
+            t.append(time.perf_counter() - t0)
+        log(f"streams {k}: {min(t) * 1e3:.2f} ms per solve")
+        if best is None or min(t) < best[1]:
+            best = (k, min(t))
+    ctx.tune(dwf.DWF_TUNE_STREAMS, best[0])
+    return best[0]
+
+
 def time_dslash(ctx, dwf, capi, prec, nrhs, reps):
     n = nrhs * LS * NT * NX * 4
     dt = np.float32 if prec == dwf.DWF_PREC_FP32 else np.float64
@@ -539,6 +564,8 @@ def run_learned(args, dwf, capi):
         shapes[name] = solve_shape(
             args, ctx, dwf, lambda: ctx.solve_dirac_device(method, d_tsrc.ptr, None, TOL, 20000, nrhs, d_x.ptr),
             tuned64 if method == dwf.DWF_SOLVE_CG64 else tuned, log)
+        shapes[name]["streams"] = pick_streams(
+            args, ctx, dwf, lambda: ctx.solve_dirac_device(method, d_tsrc.ptr, None, TOL, 20000, nrhs, d_x.ptr), log)
         for _ in range(args.warmup):
             ctx.solve_dirac_device(method, d_src.ptr, None, TOL, 20000, nrhs, d_x.ptr)
         ctx.synchronize()
@@ -747,6 +774,9 @@ def main():
     ap.add_argument("--solve-shape", default="auto",
                     help="stencil shape of the solvers: 'auto' (picked on whole solves), 'tuned' (the "
                          "autotuner's D-apply shape) or 'lds,tile_x' (FP32 solvers)")
+    ap.add_argument("--streams", default="auto", choices=("auto", "1", "2", "3", "4"),
+                    help="concurrent sub-batch streams of the batched solves (DWF_TUNE_STREAMS): 'auto' "
+                         "picks 1, 2 or 4 per solver on whole solves")
     ap.add_argument("--dslash-shape", default=None,
                     help="'lds,tile_x': time the FP32 D apply in this LDS tile shape instead of the autotuned one")
     ap.add_argument("--cpu-iters", type=int, default=60,
@@ -859,6 +889,8 @@ def main():
         tuned["solve_" + name] = solve_shape(
             args, ctx, dwf, lambda: ctx.solve_dirac_device(method, d_tsrc.ptr, None, TOL, 20000, nrhs, d_x.ptr),
             tuned["solves_fp64" if method == dwf.DWF_SOLVE_CG64 else "solves"], log)
+        tuned["solve_" + name]["streams"] = pick_streams(
+            args, ctx, dwf, lambda: ctx.solve_dirac_device(method, d_tsrc.ptr, None, TOL, 20000, nrhs, d_x.ptr), log)
         steps = args.steps
         for _ in range(args.warmup):
             solve(method)
@@ -887,6 +919,7 @@ def main():
         ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA,
                  args.reliable_delta if method == dwf.DWF_SOLVE_LEARNED else LIB_RELIABLE_DELTA)
         apply_shape(ctx, dwf, tuned["solve_" + name])
+        ctx.tune(dwf.DWF_TUNE_STREAMS, tuned["solve_" + name]["streams"])
         ctx.solve_dirac(host_src, rtol=TOL, maxiter=20000, method=method)
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -977,7 +1010,8 @@ def main():
             "bytes_per_launch": dslash_bytes(4, args.dslash_rhs),
         },
         "cpu_baseline": cpu,
-        "autotuned_shapes": {k: {kk: v[kk] for kk in ("tile_x", "prefetch", "chunks", "nt_store", "lds")}
+        "autotuned_shapes": {k: {kk: v[kk] for kk in ("tile_x", "prefetch", "chunks", "nt_store", "lds", "streams")
+                                 if kk in v}
                              for k, v in tuned.items()},
     }
     print(json.dumps(line), flush=True)

@@ -130,7 +130,11 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * the next operator launch in the replayed blocks of 8 iterations (7 launches less per
  * block); DWF_TUNE_NET_TILE: the fused network's output tile,
  * ty * 256 + tx sites (ty <= 8, tx <= 16, dividing Nt and Nx), 0 (default) the largest
- * that divides. */
+ * that divides; DWF_TUNE_STREAMS: 1 to 8 (default 2) concurrent sub-batches of a batched
+ * one-lattice dwf_solve_dirac* of at least twice as many right-hand sides (contiguous
+ * ranges, whole configurations when several are held), each on a stream, workspace and
+ * host thread of its own: the same per-right-hand-side solves and certification, the
+ * batch schedule (chunk lengths, reliable updates) per sub-batch; 1 solves the batch as one. */
 enum dwf_tune_param {
     DWF_TUNE_SCHUNKS = 0,
     DWF_TUNE_XCD_REMAP = 1,
@@ -143,7 +147,8 @@ enum dwf_tune_param {
     DWF_TUNE_NET_FUSED = 8,
     DWF_TUNE_FCG_BETA = 9,
     DWF_TUNE_EO_FUSED = 10,
-    DWF_TUNE_NET_TILE = 11
+    DWF_TUNE_NET_TILE = 11,
+    DWF_TUNE_STREAMS = 12
 };
 int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 

@@ -51,6 +51,7 @@ DWF_TUNE_NET_FUSED = 8
 DWF_TUNE_FCG_BETA = 9
 DWF_TUNE_EO_FUSED = 10
 DWF_TUNE_NET_TILE = 11
+DWF_TUNE_STREAMS = 12
 
 
 class DwfError(RuntimeError):

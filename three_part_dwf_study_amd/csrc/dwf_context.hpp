@@ -6,6 +6,7 @@
 
 #include <functional>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -156,6 +157,18 @@ struct dwf_ctx {
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs
     const dwf_net* net = nullptr; // learned preconditioner of DWF_SOLVE_LEARNED (not owned)
+    // Concurrent streams of a batched Dirac solve (DWF_TUNE_STREAMS): a one-lattice solve of
+    // at least 2 * streams right-hand sides runs as `streams` sub-batches, each on a sub-context
+    // of its own (stream, workspace, graphs; this context's gauge buffers borrowed), from as
+    // many host threads, so one sub-batch's kernels fill the others' launch tails and host
+    // round trips.
+    int streams = 2;
+    std::vector<std::unique_ptr<dwf_ctx>> kids;
+    bool borrowed = false;        // a sub-context: theta, l64, l32 belong to its parent
+    hipEvent_t ev = nullptr;      // orders a sub-context's stream after its parent's, and back
+    // Sub-context i, created on first use, with this context's lattice, settings and
+    // configurations cfg0 .. cfg0 + ncfg - 1 (borrowed link buffers).
+    dwf_ctx* kid(int i, int cfg0, int kid_ncfg);
 
     // Instantiated CG iteration blocks, keyed by every argument their kernels capture.
     std::map<std::vector<unsigned long long>, hipGraphExec_t> graphs;

@@ -72,6 +72,7 @@ void dwf_ctx::drop_graphs()
     if (!graphs.empty() && stream) (void)hipStreamSynchronize(stream);
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
     graphs.clear();
+    for (auto& k : kids) k->drop_graphs();  // their graphs hold this context's settings too
 }
 
 hipGraphExec_t dwf_ctx::cached_graph(const std::vector<unsigned long long>& key, const std::function<void()>& body)
@@ -107,14 +108,66 @@ dwf_ctx::~dwf_ctx()
 {
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graphs();
+    kids.clear();  // sub-contexts first: they borrow this context's gauge buffers
     for (auto& b : bufs)
         if (b.p) (void)hipFree(b.p);
-    if (theta) (void)hipFree(theta);
-    if (l64) (void)hipFree(l64);
-    if (l32) (void)hipFree(l32);
+    if (!borrowed) {
+        if (theta) (void)hipFree(theta);
+        if (l64) (void)hipFree(l64);
+        if (l32) (void)hipFree(l32);
+    }
+    if (ev) (void)hipEventDestroy(ev);
     if (gr_dev) (void)hipFree(gr_dev);
     if (st_host) (void)hipHostFree(st_host);
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+dwf_ctx* dwf_ctx::kid(int i, int cfg0, int kid_ncfg)
+{
+    if (multi()) throw DwfError(DWF_ERR_ARG, "sub-contexts hold one lattice");
+    while ((int)kids.size() <= i) {
+        std::unique_ptr<dwf_ctx> k(new dwf_ctx());
+        k->borrowed = true;
+        k->device = device;
+        check(hipStreamCreateWithFlags(&k->stream, hipStreamNonBlocking), "sub-context stream");
+        check(hipEventCreateWithFlags(&k->ev, hipEventDisableTiming), "sub-context event");
+        kids.push_back(std::move(k));
+    }
+    if (!ev) check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "context event");
+    dwf_ctx* k = kids[i].get();
+    // the gauge field of configurations cfg0 ..: the parent's buffers at an offset
+    const long long nl = 2LL * L.V4;
+    double* th = theta + cfg0 * nl;
+    double2* a64 = l64 + cfg0 * nl;
+    float2* a32 = l32 + cfg0 * nl;
+    if (k->l32 != a32 || k->l64 != a64 || k->theta != th) k->drop_graphs();  // graphs hold link pointers
+    k->theta = th;
+    k->l64 = a64;
+    k->l32 = a32;
+    k->L = L;
+    k->L0 = L0;
+    k->kind = kind;
+    k->have_gauge = have_gauge;
+    k->ncfg = kid_ncfg;
+    k->cfg_cap = kid_ncfg;
+    k->link_cap = (size_t)kid_ncfg * nl;
+    k->gdefs.clear();
+    k->chunks_override = chunks_override;
+    k->xcd_remap = xcd_remap;
+    k->prefetch = prefetch;
+    k->tile_x = tile_x;
+    k->nt_store = nt_store;
+    k->lds = lds;
+    k->net_fused = net_fused;
+    k->fcg_beta = fcg_beta;
+    k->eo_fused = eo_fused;
+    k->eo_pipe = eo_pipe;
+    k->net_tile = net_tile;
+    k->reliable_delta = reliable_delta;
+    k->use_graphs = use_graphs;
+    k->net = net;
+    k->streams = 1;
+    return k;
 }
 
 void* dwf_ctx::buf(int id, size_t bytes)
@@ -466,6 +519,10 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             if (!(value == 0.0 || value == 1.0 || value == 2.0 || value == 3.0)) break;
             ctx->eo_fused = value != 0.0 ? 1 : 0;
             ctx->eo_pipe = value >= 2.0 ? (int)value - 1 : 0;
+            return DWF_OK;
+        case DWF_TUNE_STREAMS:
+            if (!(value >= 1.0 && value <= 8.0 && value == (double)(int)value)) break;
+            ctx->streams = (int)value;
             return DWF_OK;
         case DWF_TUNE_NET_TILE: {
             if (!(value >= 0.0 && value == (double)(int)value)) break;

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace dwfk;
@@ -941,8 +942,8 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
     return rc;
 }
 
-int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter, int nrhs,
-                double* x, dwf_solve_stats* stats)
+int solve_dirac_one(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter, int nrhs,
+                    double* x, dwf_solve_stats* stats)
 {
     if (!c || !src || !x || nrhs < 1 || maxiter < 0 || !(rtol >= 0.0) ||
         (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_EO_MIXED &&
@@ -972,6 +973,63 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
     }
     if (method == DWF_SOLVE_LEARNED) return solve_learned(c, DWF_OP_DDAGD, B, x0, rtol, maxiter, nrhs, x, stats);
     return solve_normal(c, DWF_OP_DDAGD, method, B, x0, rtol, maxiter, nrhs, x, stats);
+}
+
+// A batched solve of one lattice as c->streams concurrent sub-batches (contiguous ranges of
+// right-hand sides, whole configurations when several are held), each on a sub-context of
+// its own from a host thread of its own, writing the caller's buffers in place.  Every
+// right-hand side is solved and certified exactly as in one batch; only the batch-level
+// schedule (chunk lengths, reliable-update points) is per sub-batch.
+int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter, int nrhs,
+                double* x, dwf_solve_stats* stats)
+{
+    const int k = c ? c->streams : 1;
+    if (k < 2 || !src || !x || c->borrowed || c->multi() || !c->have_gauge || nrhs < 2 * k)
+        return solve_dirac_one(c, method, src, x0, rtol, maxiter, nrhs, x, stats);
+    c->bind_rhs(nrhs);
+    const int unit = c->ncfg > 1 ? c->rhs_per_cfg : 1;  // right-hand sides per split unit
+    const int units = nrhs / unit;
+    if (units < k) return solve_dirac_one(c, method, src, x0, rtol, maxiter, nrhs, x, stats);
+    const size_t per = (size_t)c->L.V5() * 4;  // doubles per right-hand side
+    if (!c->ev) c->check(hipEventCreateWithFlags(&c->ev, hipEventDisableTiming), "split solve: event");
+    c->check(hipEventRecord(c->ev, c->stream), "split solve: input event");
+    std::vector<dwf_ctx*> kc(k);
+    std::vector<int> r0(k + 1);
+    for (int i = 0; i <= k; ++i) r0[i] = (int)((long long)units * i / k) * unit;
+    for (int i = 0; i < k; ++i) {
+        const int cfg0 = c->ncfg > 1 ? r0[i] / unit : 0, nc = c->ncfg > 1 ? (r0[i + 1] - r0[i]) / unit : 1;
+        kc[i] = c->kid(i, cfg0, nc);
+        c->check(hipStreamWaitEvent(kc[i]->stream, c->ev, 0), "split solve: wait for the inputs");
+    }
+    std::vector<int> rc(k, DWF_OK), code(k, DWF_OK);
+    std::vector<std::string> err(k);
+    std::vector<std::thread> th;
+    for (int i = 0; i < k; ++i)
+        th.emplace_back([&, i] {
+            try {
+                dwf_ctx::check(hipSetDevice(c->device), "split solve: hipSetDevice");
+                const size_t o = (size_t)r0[i] * per;
+                rc[i] = solve_dirac_one(kc[i], method, src + o, x0 ? x0 + o : nullptr, rtol, maxiter,
+                                        r0[i + 1] - r0[i], x + o, stats ? stats + r0[i] : nullptr);
+            } catch (const DwfError& e) {
+                code[i] = e.code;
+                err[i] = e.what();
+            } catch (const std::exception& e) {
+                code[i] = DWF_ERR_ARG;
+                err[i] = e.what();
+            }
+        });
+    for (auto& t : th) t.join();
+    // the caller's stream continues after every sub-batch's work
+    for (int i = 0; i < k; ++i) {
+        c->check(hipEventRecord(kc[i]->ev, kc[i]->stream), "split solve: output event");
+        c->check(hipStreamWaitEvent(c->stream, kc[i]->ev, 0), "split solve: wait for the outputs");
+    }
+    for (int i = 0; i < k; ++i)
+        if (code[i] != DWF_OK) throw DwfError(code[i], err[i]);
+    for (int i = 0; i < k; ++i)
+        if (rc[i] != DWF_OK) return rc[i];
+    return DWF_OK;
 }
 
 int guard(const std::function<int()>& f)
