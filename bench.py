@@ -189,13 +189,13 @@ def solve_shape(args, ctx, dwf, solve_once, tuned, log):
 
 def pick_streams(args, ctx, dwf, solve_once, log):
     """Concurrent sub-batch streams of a batched solve (DWF_TUNE_STREAMS), chosen on whole
-    solves like the stencil shape: 1, 2 and 4, each after a warm-up solve, timed twice;
+    solves like the stencil shape: 1, 2, 4 and 8, each after a warm-up solve, timed twice;
     --streams fixes it."""
     if args.streams != "auto":
         ctx.tune(dwf.DWF_TUNE_STREAMS, int(args.streams))
         return int(args.streams)
     best = None
-    for k in (1, 2, 4):
+    for k in (1, 2, 4, 8):
         ctx.tune(dwf.DWF_TUNE_STREAMS, k)
         solve_once()
         ctx.synchronize()
@@ -774,9 +774,9 @@ def main():
     ap.add_argument("--solve-shape", default="auto",
                     help="stencil shape of the solvers: 'auto' (picked on whole solves), 'tuned' (the "
                          "autotuner's D-apply shape) or 'lds,tile_x' (FP32 solvers)")
-    ap.add_argument("--streams", default="auto", choices=("auto", "1", "2", "3", "4"),
+    ap.add_argument("--streams", default="auto", choices=("auto", "1", "2", "3", "4", "6", "8"),
                     help="concurrent sub-batch streams of the batched solves (DWF_TUNE_STREAMS): 'auto' "
-                         "picks 1, 2 or 4 per solver on whole solves")
+                         "picks 1, 2, 4 or 8 per solver on whole solves")
     ap.add_argument("--dslash-shape", default=None,
                     help="'lds,tile_x': time the FP32 D apply in this LDS tile shape instead of the autotuned one")
     ap.add_argument("--cpu-iters", type=int, default=60,
