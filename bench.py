@@ -642,6 +642,8 @@ def run_eo(args, dwf, capi):
     ctx.set_gauge(theta)
     if args.no_graphs:
         ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
+    if args.streams != "auto":  # else the library's default
+        ctx.tune(dwf.DWF_TUNE_STREAMS, int(args.streams))
     src = boundary_sources(nrhs, 918273)
     d_src = capi.DeviceArray.from_host(src.reshape(-1).view(np.float64))
     d_x = capi.DeviceArray(src.size * 16)
