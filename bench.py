@@ -41,7 +41,8 @@ With --workload hmc a step is one Nf=2 domain-wall HMC trajectory at 96x96, Ls=1
 beta=4, mf=0.05, M5=1, tau=1 with 20 leapfrog steps (epsilon=0.05), from a hot start
 theta ~ U[-pi, pi) (seed 1234), the host random stream seeded 1234, force solves to 1e-10
 with chronological initial guesses; the gauge update, forces and Delta H stay in FP64 on
-the device.
+the device.  --hmc-solver picks the force solve: cg64, mixed (default), eo_mixed (two
+even-odd Dirac solves per solve, certified on D D^dag) or learned.
 
 With --workload eo: the even-odd solver alone on the default batch (for kernel traces and
 counter runs of its kernels), with its fraction of the Schur CG's HBM ceiling.
@@ -687,7 +688,7 @@ def run_hmc(args, dwf, capi):
         ctx.tune(dwf.DWF_TUNE_GRAPHS, 0)
     rng = capi.HostRng(1234)
     ndof, ngauge = LS * NT * NX * 2, 2 * NT * NX
-    method = {"cg64": dwf.DWF_SOLVE_CG64, "mixed": dwf.DWF_SOLVE_MIXED,
+    method = {"cg64": dwf.DWF_SOLVE_CG64, "mixed": dwf.DWF_SOLVE_MIXED, "eo_mixed": dwf.DWF_SOLVE_EO_MIXED,
               "learned": dwf.DWF_SOLVE_LEARNED}[args.hmc_solver]
     # force solves have one right-hand side: stencil shape tuned for that (graph-timed)
     shape = ctx.autotune(1, dwf.DWF_PREC_FP64 if method == dwf.DWF_SOLVE_CG64 else dwf.DWF_PREC_FP32,
@@ -721,14 +722,26 @@ def run_hmc(args, dwf, capi):
     if not args.no_cpu:
         import oracle
 
+        # the reference's FP64 CG iterations per trajectory: the timed ones for cg64, else
+        # the chain's first trajectory re-run (untimed) with FP64 CG force solves
+        ref_iters = cg_per_traj
+        if method != dwf.DWF_SOLVE_CG64:
+            with dwf.DwfContext.dwf(NT, NX, LS, M5, MF) as c64:
+                c64.set_gauge(theta0)
+                chi, mom = capi.HostRng(1234).dwf_heatbath(ndof, ngauge)
+                ref_iters = float(c64.hmc_trajectory(chi, mom, 0.5, BETA, eps, nsteps, TOL, 20000,
+                                                     method=dwf.DWF_SOLVE_CG64)["cg_iterations"])
+            log(f"FP64 CG iterations of the first trajectory: {ref_iters:.0f}")
         src = boundary_sources(1, 918273)[0].reshape(-1)
         t1 = time.perf_counter()
         _, it, _, _, _ = oracle.dwf_cg(2, theta0, src, NT, NX, LS, M5, MF, TOL, args.cpu_iters)
         per_iter = (time.perf_counter() - t1) / max(1, it)
-        cpu = {"value": 1.0 / (per_iter * cg_per_traj), "unit": "trajectories/s", "cores": 1, "kind": "port",
+        cpu = {"value": 1.0 / (per_iter * ref_iters), "unit": "trajectories/s", "cores": 1, "kind": "port",
                "sample": (f"oracle FP64 CG on D D^dag (reference algorithm, 1 core): {it} iterations at 96x96x16 "
-                          f"timed ({per_iter * 1e3:.1f} ms/iteration), scaled by the {cg_per_traj:.0f} CG "
-                          "iterations one device trajectory needs; forces and gauge update excluded")}
+                          f"timed ({per_iter * 1e3:.1f} ms/iteration), scaled by the {ref_iters:.0f} FP64 CG "
+                          "iterations of one trajectory" + ("" if method == dwf.DWF_SOLVE_CG64 else
+                                                            " (the chain's first, re-run with FP64 CG)")
+                          + "; forces and gauge update excluded")}
         log(f"cpu baseline: {cpu}")
     line = {
         "metric": "DWF HMC trajectories/s 96²,Ls=16 (tau=1, 20 leapfrog steps)",
@@ -741,7 +754,8 @@ def run_hmc(args, dwf, capi):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"cg64": "f64", "mixed": "f32+f64", "learned": "bf16+f32+f64"}[args.hmc_solver],
+        "dtype": {"cg64": "f64", "mixed": "f32+f64", "eo_mixed": "f32+f64",
+                  "learned": "bf16+f32+f64"}[args.hmc_solver],
         "data": "synthetic",
         "config": {
             "workload": "Nf=2 Shamir DWF pseudofermion HMC, 96x96 Ls=16 beta=4 mf=0.05 M5=1, tau=1, 20 leapfrog "
@@ -792,7 +806,7 @@ def main():
                     help="analyzer: one context per lattice size on concurrent streams instead of one launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("solves", "analyzer", "learned", "hmc", "eo"), default="solves")
-    ap.add_argument("--hmc-solver", choices=("cg64", "mixed", "learned"), default="mixed")
+    ap.add_argument("--hmc-solver", choices=("cg64", "mixed", "eo_mixed", "learned"), default="mixed")
     ap.add_argument("--hmc-start", choices=("hot", "cold"), default="hot",
                     help="HMC start: theta ~ U[-pi, pi) (seed 1234) or the reference's theta = 1")
     ap.add_argument("--learn-configs", type=int, default=64)

@@ -246,7 +246,13 @@ int dwf_apply_device_noalloc(dwf_ctx* ctx, int op, int precision, const void* in
 
 /* Solve A x = b for nrhs right-hand sides with A = D D^dagger (DWF_OP_DDDAG) or
  * D^dagger D (DWF_OP_DDAGD).  x0 (nullable) is the initial guess for every right-hand
- * side's own slot.  stats (nullable) receives nrhs records.  Replaces
+ * side's own slot.  stats (nullable) receives nrhs records.  DWF_SOLVE_EO_MIXED solves
+ * (D D^dagger)^-1 = Gamma D^-1 Gamma D^-1 (D^dagger = Gamma D Gamma, Gamma = sigma3 R)
+ * with two even-odd Dirac solves per correction pass and certifies every pass on the FP64
+ * residual of A itself (up to four passes, then the mixed CG continues and decides; stats
+ * sum the Dirac solves' iterations, passes = correction passes); on a Wilson context or a
+ * lattice without an even-odd kernel it is the mixed CG.  DWF_SOLVE_EO_MIXED and
+ * DWF_SOLVE_LEARNED need a context holding one lattice size.  Replaces
  * conjugate_gradient (domain_wall/include/cg_solver.hpp:17-64); the Wilson solves use
  * the same entry point (wilson/include/cg_solver.hpp:19-78).  Returns
  * DWF_ERR_NOT_CONVERGED if any right-hand side missed the tolerance (x still holds
@@ -274,7 +280,7 @@ int dwf_solve_dirac_device(dwf_ctx* ctx, int method, const double* src_device,
 
 /* ---- pseudofermion force, gauge action, HMC ---------------------------------------- */
 
-/* eta = (D D^dagger)^-1 phi by CG (method, initial guess eta_guess nullable),
+/* eta = (D D^dagger)^-1 phi by the method's solve of dwf_solve (initial guess eta_guess nullable),
  * action = Re phi^dagger eta, force[2][Nt][Nx] = dS/dtheta with xi = D^dagger eta.
  * Replaces pseudofermion_action_force (domain_wall/include/pseudofermion.hpp:22-166)
  * and, on a Wilson context, pseudofermion_action_force + solve_DDdag
@@ -312,7 +318,7 @@ typedef struct dwf_hmc_params {
     double epsilon;   /* leapfrog step                                     */
     int steps;        /* leapfrog steps                                    */
     int integrator;   /* enum dwf_integrator                               */
-    int method;       /* force solver: DWF_SOLVE_CG64, _MIXED or _LEARNED  */
+    int method;       /* force solver: DWF_SOLVE_CG64, _MIXED, _EO_MIXED or _LEARNED */
     int maxiter;      /* force CG iteration limit                          */
     double rtol;      /* force CG tolerance                                */
 } dwf_hmc_params;

@@ -229,15 +229,8 @@ inline int solver_method(const Parameters& p, const std::string& key, int dflt)
     throw std::runtime_error("Invalid value for " + key + ": " + v + " (cg64, mixed, eo_mixed or learned)");
 }
 
-// The HMC force solves run on the normal operator D D^dag (the reference's pseudofermion
-// solve), which the even-odd solver does not take: cg64, mixed or learned.
-inline int force_solver_method(const Parameters& p, const std::string& key)
-{
-    const int m = solver_method(p, key, 0);
-    if (m == 2)
-        throw std::runtime_error("Invalid value for " + key + ": eo_mixed (the HMC force solves take cg64, mixed or "
-                                 "learned)");
-    return m;
-}
+// The HMC force solves: cg64 (the reference's CG), mixed, eo_mixed (two even-odd Dirac
+// solves per correction pass, certified on D D^dag) or learned.
+inline int force_solver_method(const Parameters& p, const std::string& key) { return solver_method(p, key, 0); }
 
 }  // namespace app

@@ -4,8 +4,9 @@
 // output/domain_wall/configs.npy (every trajectory's theta0), analysis_configs.npy
 // (after the thermalization cut, thinned) and run_info_history.csv.  Each trajectory
 // runs on the device in FP64 (dwf_hmc_trajectory); the force solves use
-// gpu.force_solver (cg64 by default, the reference's algorithm; "learned" with the
-// network weights of gpu.network_weights, e.g. from train_domain_wall_network).
+// gpu.force_solver (cg64 by default, the reference's algorithm; "mixed"; "eo_mixed", two
+// even-odd Dirac solves per solve; "learned" with the network weights of
+// gpu.network_weights, e.g. from train_domain_wall_network).
 
 #include "dwf_hip.hpp"
 #include "dwf_params.hpp"

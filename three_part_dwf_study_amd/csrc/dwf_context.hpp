@@ -97,6 +97,10 @@ enum BufId : int {
     B_FCG_P32B,  // learned solver: the second direction buffer of the fused iteration
     B_NOFF,      // analyzer: offsets of each configuration's noise vectors
     B_HOLD,      // mixed CG: right-hand sides a reliable update skips
+    B_NEO_R,     // even-odd normal solves: residual of the normal system, then the second solution
+    B_NEO_Y,     // even-odd normal solves: the first Dirac solution
+    B_NEO_PART,  // even-odd normal solves: certification partials
+    B_NEO_STATE, // even-odd normal solves: certification states
     B_COUNT
 };
 
@@ -212,5 +216,8 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
 // the plain FP64 CG fallback; D D^dag = Gamma D^dag D Gamma (Gamma = sigma3 R).
 int dwf_solve_learned_impl(dwf_ctx* c, int normal, const double* b, const double* x0, double rtol, int maxiter,
                            int nrhs, double* x, dwf_solve_stats* stats);
+// dwf_solve_device without the error guard: every method, with its certified fallbacks.
+int dwf_solve_normal_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0, double rtol,
+                          int maxiter, int nrhs, double* x, dwf_solve_stats* stats);
 int dwf_solve_device_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0,
                           double rtol, int maxiter, int nrhs, double* x);

@@ -73,9 +73,10 @@ PfOut pf_dev(Dev& d, int method, const double* phi, const double* guess, double 
 {
     dwf_ctx* c = d.c;
     PfOut o;
-    if (method == DWF_SOLVE_LEARNED) {
-        // learned preconditioner with the certified FP64 CG fallback (certified or failed)
-        const int rc = dwf_solve_learned_impl(c, DWF_OP_DDDAG, phi, guess, rtol, maxiter, 1, eta, &o.stats);
+    if (method == DWF_SOLVE_LEARNED || method == DWF_SOLVE_EO_MIXED) {
+        // learned preconditioner, or two even-odd Dirac solves per correction pass, each
+        // with its certified CG fallback (certified on D D^dag or failed)
+        const int rc = dwf_solve_normal_impl(c, DWF_OP_DDDAG, method, phi, guess, rtol, maxiter, 1, eta, &o.stats);
         o.iterations = o.stats.iterations + o.stats.fp64_iterations;
         if (rc != DWF_OK || !o.stats.converged) {
             o.rc = DWF_ERR_NOT_CONVERGED;  // pseudofermion.hpp:39-41
@@ -154,7 +155,8 @@ int dwf_pf_action_force(dwf_ctx* c, int method, const double* phi, const double*
     return guard([&]() -> int {
         require(c);
         if (!phi || !action || !force || maxiter < 0 ||
-            (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_LEARNED))
+            (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_EO_MIXED &&
+             method != DWF_SOLVE_LEARNED))
             throw DwfError(DWF_ERR_ARG, "dwf_pf_action_force: bad argument");
         Dev d(c);
         double* dphi = d.vec(B_PHI);
@@ -239,7 +241,8 @@ int dwf_hmc_trajectory(dwf_ctx* c, const dwf_hmc_params* P, const double* chi, c
         require(c);
         if (!P || !chi || !momentum || !res || P->steps < 0 || P->maxiter < 0 ||
             (P->integrator != DWF_HMC_DOMAIN_WALL && P->integrator != DWF_HMC_WILSON) ||
-            (P->method != DWF_SOLVE_CG64 && P->method != DWF_SOLVE_MIXED && P->method != DWF_SOLVE_LEARNED))
+            (P->method != DWF_SOLVE_CG64 && P->method != DWF_SOLVE_MIXED && P->method != DWF_SOLVE_EO_MIXED &&
+             P->method != DWF_SOLVE_LEARNED))
             throw DwfError(DWF_ERR_ARG, "dwf_hmc_trajectory: bad argument");
         Dev d(c);
         const int ng = d.ng;

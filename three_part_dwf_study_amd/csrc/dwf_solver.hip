@@ -779,18 +779,27 @@ int solve_learned(dwf_ctx* c, int normal, const double* b, const double* x0, dou
     return rc;
 }
 
+int solve_normal_eo(dwf_ctx* c, int normal, const double* b, const double* x0, double rtol, int maxiter, int nrhs,
+                    double* x, dwf_solve_stats* stats);
+
 int solve_normal(dwf_ctx* c, int normal, int method, const double* b, const double* x0, double rtol, int maxiter,
                  int nrhs, double* x, dwf_solve_stats* stats)
 {
     if (!c || !b || !x || nrhs < 1 || maxiter < 0 || !(rtol >= 0.0) ||
         (normal != DWF_OP_DDDAG && normal != DWF_OP_DDAGD) ||
-        (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_LEARNED))
+        (method != DWF_SOLVE_CG64 && method != DWF_SOLVE_MIXED && method != DWF_SOLVE_EO_MIXED &&
+         method != DWF_SOLVE_LEARNED))
         throw DwfError(DWF_ERR_ARG, "dwf_solve: bad argument");
     if (!c->have_gauge) throw DwfError(DWF_ERR_ARG, "dwf_solve: no gauge field set");
     c->bind_rhs(nrhs);
     if (method == DWF_SOLVE_LEARNED) {
         c->require_single_lattice("DWF_SOLVE_LEARNED");
         return solve_learned(c, normal, b, x0, rtol, maxiter, nrhs, x, stats);
+    }
+    if (method == DWF_SOLVE_EO_MIXED) {
+        c->require_single_lattice("DWF_SOLVE_EO_MIXED on a normal operator");
+        if (eo_supported(c->L)) return solve_normal_eo(c, normal, b, x0, rtol, maxiter, nrhs, x, stats);
+        method = DWF_SOLVE_MIXED;  // Wilson, odd extents or an Ls without an even-odd kernel
     }
     dwf_solve_device_impl(c, normal, method, b, x0, rtol, maxiter, nrhs, x);
     int rc = collect(c, method, nrhs, stats);
@@ -1032,6 +1041,100 @@ int solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, dou
     return DWF_OK;
 }
 
+// A x = b with A = D D^dag or D^dag D through even-odd Dirac solves.  D^dag = Gamma D Gamma
+// (Gamma = sigma3 R) gives (D D^dag)^-1 = Gamma D^-1 Gamma D^-1, and D^dag D = Gamma D D^dag
+// Gamma turns the second system into the first.  Each pass certifies the iterate on the FP64
+// residual of A itself, the system of the reference's CG (domain_wall/include/cg_solver.hpp:
+// 17-64), and while a right-hand side misses rtol adds Gamma D^-1 Gamma D^-1 r, the two
+// Dirac solves run to a tolerance scaled to what that pass has left: a Dirac solve's
+// certified residual bounds the normal one only up to the condition number of D, so one pass
+// need not reach rtol.  After four passes the mixed CG (with its FP64 CG fallback) continues
+// from the iterate and decides.  stats: iterations and reliable updates summed over every
+// Dirac solve, passes = the correction passes, fp64_iterations = the fallback's.
+int solve_normal_eo(dwf_ctx* c, int normal, const double* b, const double* x0, double rtol, int maxiter, int nrhs,
+                    double* x, dwf_solve_stats* stats)
+{
+    hipStream_t s = c->stream;
+    const size_t nv = (size_t)c->L.V5() * 4 * nrhs;
+    if (x0) {
+        if (x0 != x) c->check(hipMemcpyAsync(x, x0, nv * 8, hipMemcpyDeviceToDevice, s), "copy x0");
+    } else {
+        launch_zero<double>(x, (long long)nv, s);
+    }
+    if (normal == DWF_OP_DDAGD) {  // D^dag D x = b  <=>  D D^dag (Gamma x) = Gamma b
+        double* gb = (double*)c->buf(B_GAMMA_B, nv * 8);
+        c->check(hipMemcpyAsync(gb, b, nv * 8, hipMemcpyDeviceToDevice, s), "copy b");
+        gamma_r(c, gb, nrhs);
+        gamma_r(c, x, nrhs);
+        b = gb;
+    }
+    std::vector<dwf_solve_stats> acc(nrhs), one(nrhs);
+    auto add = [&]() {
+        for (int i = 0; i < nrhs; ++i) {
+            acc[i].iterations += one[i].iterations;
+            acc[i].reliable_updates += one[i].reliable_updates;
+            acc[i].fp64_iterations += one[i].fp64_iterations;
+        }
+    };
+    const int passes = 4;
+    int used = 0;
+    bool all = false;
+    for (int pass = 0;; ++pass) {
+        // workspace fetched per pass: the Dirac solves below size buffers of their own
+        double* r = (double*)c->buf(B_NEO_R, nv * 8);
+        double* y = (double*)c->buf(B_NEO_Y, nv * 8);
+        FullOp op(c, true, nrhs, (double*)c->buf(B_T64, nv * 8));
+        double* part = (double*)c->buf(B_NEO_PART, (size_t)nrhs * op.nblk * 2 * 8);
+        CgState* cert = (CgState*)c->buf(B_NEO_STATE, sizeof(CgState) * nrhs);
+        op.residual(x, b, r, nullptr, nullptr, nullptr, part, nullptr, 0, 0);  // r = b - D D^dag x
+        launch_cg_start(op.nblk, part, cert, rtol, c->reliable_delta, nrhs, s);
+        c->check_launch("even-odd normal solve: certification");
+        read_states(c, cert, nrhs);
+        double worst = 0.0;
+        all = true;
+        for (int i = 0; i < nrhs; ++i) {
+            acc[i].relres = acc[i].true_relres = c->st_host[i].relres;
+            if (!(c->st_host[i].flags[0] & F_DONE)) {
+                all = false;
+                worst = std::max(worst, c->st_host[i].relres);
+            }
+        }
+        if (all || pass == passes) break;
+        // the Dirac solves' outcome is judged by the next certification, not by their codes
+        const double tol_in = std::min(0.5, 0.25 * rtol / worst);
+        solve_dirac(c, DWF_SOLVE_EO_MIXED, r, nullptr, tol_in, maxiter, nrhs, y, one.data());  // y = D^-1 r
+        add();
+        gamma_r(c, y, nrhs);
+        solve_dirac(c, DWF_SOLVE_EO_MIXED, y, nullptr, tol_in, maxiter, nrhs, r, one.data());  // r = D^-1 Gamma y
+        add();
+        gamma_r(c, r, nrhs);
+        k_axpy_masked<<<dim3(1024, nrhs), 256, 0, s>>>(x, r, c->layout(nrhs, 0), cert);
+        c->check_launch("even-odd normal solve: correction");
+        used = pass + 1;
+    }
+    int rc = DWF_OK;
+    if (all) {
+        for (int i = 0; i < nrhs; ++i) acc[i].converged = 1;
+    } else {
+        std::vector<dwf_solve_stats> fb(nrhs);
+        rc = solve_normal(c, DWF_OP_DDDAG, DWF_SOLVE_MIXED, b, x, rtol, maxiter, nrhs, x, fb.data());
+        for (int i = 0; i < nrhs; ++i) {
+            acc[i].iterations += fb[i].iterations;
+            acc[i].reliable_updates += fb[i].reliable_updates;
+            acc[i].fp64_iterations += fb[i].fp64_iterations;
+            acc[i].converged = fb[i].converged;
+            acc[i].relres = fb[i].relres;
+            acc[i].true_relres = fb[i].true_relres;
+        }
+        if (rc == DWF_ERR_NOT_CONVERGED) rc = DWF_ERR_CERTIFY;
+    }
+    if (normal == DWF_OP_DDAGD) gamma_r(c, x, nrhs);
+    c->check_launch("even-odd normal solve");
+    for (int i = 0; i < nrhs; ++i) acc[i].passes = used;
+    if (stats) std::copy(acc.begin(), acc.end(), stats);
+    return rc;
+}
+
 int guard(const std::function<int()>& f)
 {
     try {
@@ -1051,6 +1154,12 @@ int dwf_solve_learned_impl(dwf_ctx* c, int normal, const double* b, const double
                            int nrhs, double* x, dwf_solve_stats* stats)
 {
     return solve_learned(c, normal, b, x0, rtol, maxiter, nrhs, x, stats);
+}
+
+int dwf_solve_normal_impl(dwf_ctx* c, int normal, int method, const double* b, const double* x0, double rtol,
+                          int maxiter, int nrhs, double* x, dwf_solve_stats* stats)
+{
+    return solve_normal(c, normal, method, b, x0, rtol, maxiter, nrhs, x, stats);
 }
 
 int dwf_solve_dirac_impl(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter,
