@@ -512,7 +512,10 @@ __global__ __launch_bounds__(kBlock) void k_eo_b(const EoArgs<R> a)
 // The boundary signs are folded into the links (exact).  The arithmetic otherwise
 // follows k_eo_a and k_eo_b; box or region sites that wrap around a small lattice are
 // computed twice, into two LDS entries.
-template <typename R, bool DAG, int LS, int EPI, bool PIPE = false>
+//
+// TYC, TXC > 0: the tile is TYC x TXC at compile time (one lattice only), so the region and
+// box strides, the index divisions and the LDS offsets fold into constants; 0: from tl.
+template <typename R, bool DAG, int LS, int EPI, bool PIPE = false, int TYC = 0, int TXC = 0>
 __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(const EoArgs<R> a, const EoTile tl)
 {
     extern __shared__ __align__(16) unsigned char eo_smem[];
@@ -521,7 +524,7 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
     using C = typename C2T<R>::T;
     const int total = gridDim.x;
     const int lb = a.remap ? xcd_logical_block(blockIdx.x, total) : (int)blockIdx.x;
-    int TY = tl.ty, TX = tl.tx;  // one lattice; several: each group's own tile (eo_tile_dim)
+    int TY = TYC > 0 ? TYC : tl.ty, TX = TXC > 0 ? TXC : tl.tx;  // one lattice; several: each group's own tile
     Lat L = a.L;
     int rhs, tile, pstride;
     long long rb;
@@ -538,8 +541,10 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
         const GroupPos p = block_group(
             a.groups, lb, [](int nt, int nx) { return (nt / eo_tile_dim(nt, 8)) * (nx / eo_tile_dim(nx, 16)); }, blk0,
             per);
-        TY = eo_tile_dim(p.Nt, 8);
-        TX = eo_tile_dim(p.Nx, 16);
+        if constexpr (TYC == 0) {  // (a compile-time tile is launched for one lattice only)
+            TY = eo_tile_dim(p.Nt, 8);
+            TX = eo_tile_dim(p.Nx, 16);
+        }
         const int local = lb - blk0, rl = local / per;
         rhs = p.rhs0 + rl;
         tile = local - rl * per;
@@ -1016,6 +1021,16 @@ namespace {
 template <typename R, bool DAG, int LS>
 void dispatch_schur(int epi, const EoArgs<R>& a, const EoTile& t, unsigned grid, size_t lds, hipStream_t s)
 {
+    if constexpr (sizeof(R) == 4) {
+        if (!a.groups && t.ty == 8 && t.tx == 16) {  // the FP32 CG's kernels on the 8 x 16 tile
+            switch (epi) {
+                case EPI_STORE: k_eo_schur<R, DAG, LS, EPI_STORE, false, 8, 16><<<grid, kBlockS, lds, s>>>(a, t); return;
+                case EPI_NORM: k_eo_schur<R, DAG, LS, EPI_NORM, false, 8, 16><<<grid, kBlockS, lds, s>>>(a, t); return;
+                case EPI_CGUPD: k_eo_schur<R, DAG, LS, EPI_CGUPD, false, 8, 16><<<grid, kBlockS, lds, s>>>(a, t); return;
+                default: break;
+            }
+        }
+    }
     switch (epi) {
         case EPI_STORE: k_eo_schur<R, DAG, LS, EPI_STORE><<<grid, kBlockS, lds, s>>>(a, t); break;
         case EPI_NORM: k_eo_schur<R, DAG, LS, EPI_NORM><<<grid, kBlockS, lds, s>>>(a, t); break;
@@ -1092,6 +1107,15 @@ void launch_eo_schur_pipe(const EoArgs<float>& a0, const EoTile& t, int nrhs, hi
         throw DwfError(DWF_ERR_ARG, "pipelined even-odd launch: bad shape");
     const size_t lds = (size_t)(a.groups ? 12 * 10 : (t.ty + 4) * ((t.tx + 4) / 2)) * a.L.Ls * sizeof(Sp<float>);
     const unsigned grid = (unsigned)nb;
+    if (!a.groups && t.ty == 8 && t.tx == 16) {  // compile-time tile
+        switch (a.L.Ls) {
+            case 4: k_eo_schur<float, false, 4, EPI_NORM, true, 8, 16><<<grid, kBlockS, lds, s>>>(a, t); return;
+            case 8: k_eo_schur<float, false, 8, EPI_NORM, true, 8, 16><<<grid, kBlockS, lds, s>>>(a, t); return;
+            case 12: k_eo_schur<float, false, 12, EPI_NORM, true, 8, 16><<<grid, kBlockS, lds, s>>>(a, t); return;
+            case 16: k_eo_schur<float, false, 16, EPI_NORM, true, 8, 16><<<grid, kBlockS, lds, s>>>(a, t); return;
+            default: break;
+        }
+    }
     switch (a.L.Ls) {
         case 4: k_eo_schur<float, false, 4, EPI_NORM, true><<<grid, kBlockS, lds, s>>>(a, t); break;
         case 8: k_eo_schur<float, false, 8, EPI_NORM, true><<<grid, kBlockS, lds, s>>>(a, t); break;
