@@ -103,25 +103,27 @@ def test_dwf_hmc_same_seed_history(Nt, Nx, Ls):
 HMC96 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "dwf_hmc_96x96x16.npz")
 
 
-@pytest.mark.parametrize("method", ["cg64", "mixed", "learned"])
+@pytest.mark.parametrize("method", ["cg64", "mixed", "eo_mixed", "learned"])
 def test_dwf_hmc_96_matches_oracle_fixture(method):
     """cfg5 at its real size against the oracle: two trajectories at 96x96, Ls=16, beta=4,
     eps=0.05, 20 leapfrog steps from the reference's cold start theta = 1, seed 1234,
     force solves to 1e-12 (tests/golden/make_dwf_hmc96.py wrote the oracle's history).
-    The GPU path draws the same momenta and pseudofermions; CG64, MIXED and learned force
-    solves (a network trained briefly here; every learned solve is certified on its FP64
-    true residual) must reproduce the oracle's Delta H, accept/reject decisions,
-    plaquettes and final angles.  Delta H: within 1e-8 for CG64 (the oracle's algorithm,
-    differing in summation order only); MIXED and learned stop on their FP64 true residual
-    at the same 1e-12 but at a different iterate, so their pseudofermion actions (about
-    3e5 here) may differ from the oracle's by ~rtol |S_f| kappa: within 1e-6 (measured:
-    CG64 3e-9, MIXED 3e-8, learned 3e-7)."""
+    The GPU path draws the same momenta and pseudofermions; CG64, MIXED, EO_MIXED (two
+    even-odd Dirac solves per force solve) and learned force solves (a network trained
+    briefly here; every MIXED, EO_MIXED and learned solve is certified on its FP64 true
+    residual) must reproduce the oracle's Delta H, accept/reject decisions, plaquettes
+    and final angles.  Delta H: within 1e-8 for CG64 (the oracle's algorithm, differing in
+    summation order only); the others stop on their FP64 true residual at the same 1e-12
+    but at a different iterate, so their pseudofermion actions (about 3e5 here) may differ
+    from the oracle's by ~rtol |S_f| kappa: within 1e-6 (measured: CG64 3e-9, MIXED 3e-8,
+    learned 3e-7)."""
     f = np.load(HMC96)
     Nt, Nx, Ls, m5, mf, beta, eps, steps, ntraj, seed, rtol = f["params"]
     Nt, Nx, Ls, steps, ntraj, seed = int(Nt), int(Nx), int(Ls), int(steps), int(ntraj), int(seed)
     assert (m5, mf) == (M5, MF)
     print_case(test="dwf_hmc96", method=method)
-    kind = {"cg64": dwf.DWF_SOLVE_CG64, "mixed": dwf.DWF_SOLVE_MIXED, "learned": dwf.DWF_SOLVE_LEARNED}[method]
+    kind = {"cg64": dwf.DWF_SOLVE_CG64, "mixed": dwf.DWF_SOLVE_MIXED, "eo_mixed": dwf.DWF_SOLVE_EO_MIXED,
+            "learned": dwf.DWF_SOLVE_LEARNED}[method]
     net = None
     if method == "learned":
         net = dwf.Network(Ls, seed=7)
