@@ -522,7 +522,7 @@ def train_network(args, dwf, capi, hot=False):
     return net, stats
 
 
-LIB_RELIABLE_DELTA = 0.01  # the library's default residual drop between reliable updates
+LIB_RELIABLE_DELTA = 0.0  # the library's defaults: each solver's own residual drop between reliable updates
 
 
 def learned_solver_config(args, dwf, ctx, net):

@@ -105,7 +105,8 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * autotuner or by hand).  DWF_TUNE_SCHUNKS: s-chunks per 4D site of the stencils
  * (0 = heuristic, 1..Ls); DWF_TUNE_XCD_REMAP: 0/1 XCD-aware block order;
  * DWF_TUNE_RELIABLE_DELTA: residual drop (0 < delta < 1) that triggers a reliable update
- * of the mixed solver; DWF_TUNE_PREFETCH: 0/1 software-pipelined stencil slice loop;
+ * of the mixed solvers (unset, or set to 0: 1e-2 for DWF_SOLVE_MIXED and the learned
+ * solver, 3e-4 for the even-odd Schur CG of DWF_SOLVE_EO_MIXED); DWF_TUNE_PREFETCH: 0/1 software-pipelined stencil slice loop;
  * DWF_TUNE_TILE_X: 0 for blocks of consecutive 4D sites, or the width (dividing Nx and
  * 256) of 2D stencil tiles of 256/width rows; DWF_TUNE_NT_STORE: 0/1 streaming
  * (non-temporal) stores of stencil outputs (the LDS-staged tiles always stream); DWF_TUNE_GRAPHS: 0/1 (default 1) replay

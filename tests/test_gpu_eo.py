@@ -117,6 +117,36 @@ def test_eo_state_fold_matches_state_kernel(Nt, Nx, Ls, nrhs, groups, graphs):
         assert a["iterations"] == b["iterations"] and a["true_relres"] == b["true_relres"], (a, b)
 
 
+def test_eo_reliable_delta_default():
+    """The even-odd CG's reliable updates follow its own residual drop (3e-4) unless one
+    is set; setting 0 restores it.  Every solve is certified either way."""
+    Nt, Nx, Ls, nrhs = 32, 32, 16, 4
+    print_case(test="eo_delta")
+    rng = np.random.default_rng(515)
+    th = rand_theta(rng, Nt, Nx)
+    src = rand_spinor(rng, nrhs * Ls * Nt * Nx * 2).reshape(nrhs, -1)
+    res = {}
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_STREAMS, 1)
+        for key, delta in (("default", None), ("1e-2", 1e-2), ("3e-4", 3e-4), ("reset", 0.0)):
+            if delta is not None:
+                ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA, delta)
+            res[key] = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, method=dwf.DWF_SOLVE_EO_MIXED)
+        with pytest.raises(dwf.DwfError):
+            ctx.tune(dwf.DWF_TUNE_RELIABLE_DELTA, 1.0)
+    for key, (x, st, rc) in res.items():
+        assert rc == 0, key
+        for r in range(nrhs):
+            assert st[r]["converged"] and st[r]["true_relres"] <= 1e-10, (key, r)
+            assert normal_relres(th, x[r], src[r], Nt, Nx, Ls) <= 1.01e-10, (key, r)
+    rel = lambda k: sum(q["reliable_updates"] for q in res[k][1])
+    assert rel("default") < rel("1e-2"), (rel("default"), rel("1e-2"))
+    # the default is 3e-4, and 0 restores it: the same solves bit for bit
+    assert np.array_equal(res["default"][0], res["3e-4"][0])
+    assert np.array_equal(res["default"][0], res["reset"][0])
+
+
 def test_eo_matches_oracle_cg():
     Nt, Nx, Ls = 8, 8, 8
     print_case(test="eo_vs_oracle")

@@ -159,6 +159,7 @@ struct dwf_ctx {
                                   // 2 also the state update inside the next one (graph blocks)
     int net_tile = 0;             // fused network's output tile ty * 256 + tx (0: the largest that divides)
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
+    bool delta_set = false;       // reliable_delta set by hand (else each operator's default)
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs
     const dwf_net* net = nullptr; // learned preconditioner of DWF_SOLVE_LEARNED (not owned)
     // Concurrent streams of a batched Dirac solve (DWF_TUNE_STREAMS): a one-lattice solve of

@@ -164,6 +164,7 @@ dwf_ctx* dwf_ctx::kid(int i, int cfg0, int kid_ncfg)
     k->eo_pipe = eo_pipe;
     k->net_tile = net_tile;
     k->reliable_delta = reliable_delta;
+    k->delta_set = delta_set;
     k->use_graphs = use_graphs;
     k->net = net;
     k->streams = 1;
@@ -497,8 +498,14 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             ctx->nt_store = value != 0.0 ? 1 : 0;
             return DWF_OK;
         case DWF_TUNE_RELIABLE_DELTA:
+            if (value == 0.0) {  // back to each operator's default
+                ctx->reliable_delta = 0.01;
+                ctx->delta_set = false;
+                return DWF_OK;
+            }
             if (!(value > 0.0 && value < 1.0)) break;
             ctx->reliable_delta = value;
+            ctx->delta_set = true;
             return DWF_OK;
         case DWF_TUNE_GRAPHS:
             ctx->use_graphs = value != 0.0 ? 1 : 0;

@@ -49,6 +49,9 @@ struct CgOperator {
     // with the state update alone (launch_cg_flags), or leaves it to the next first_pipe
     // (fold: that launch does it from the previous iteration's partials partAp, partB).
     virtual bool pipelined() const { return false; }
+    // The mixed solver's reliable-update drop for this operator when none is set by hand
+    // (0: the context's default).
+    virtual double default_delta() const { return 0.0; }
     virtual void first_pipe(float* p0, float* p1, const float* r, float* t, double* part, CgState* st, int slot,
                             int skip, int fold, const double* partAp, const double* partB)
     {
@@ -326,6 +329,9 @@ struct SchurOp : CgOperator {
         }
     }
     bool pipelined() const override { return fused && c->eo_pipe; }
+    // 3e-4: at 96^2 x 16, 32 right-hand sides, 5 reliable updates per solve instead of 8 at
+    // 1e-2 with the same FP32 iterations (+2% solves/s)
+    double default_delta() const override { return 3e-4; }
     void first_pipe(float* p0, float* p1, const float* r, float* t, double* part, CgState* st, int slot,
                     int skip, int fold, const double* partAp, const double* partB) override
     {
@@ -480,7 +486,7 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     }
     // r = b - A x0 (the reference applies A even to a zero guess, cg_solver.hpp:24-27)
     op.residual(x, b, r64, mixed ? nullptr : p64, mixed ? r32 : nullptr, mixed ? p32 : nullptr, partR, nullptr, 0, 0);
-    const double delta = c->reliable_delta;
+    const double delta = (!c->delta_set && op.default_delta() > 0.0) ? op.default_delta() : c->reliable_delta;
     launch_cg_start(op.nblk, partR, st, rtol, delta, nrhs, s);
     if (pre) k_mark_done<<<(nrhs + 63) / 64, 64, 0, s>>>(st, pre, nrhs);
     if (pipe) launch_set_fresh(st, nrhs, s);
@@ -865,7 +871,26 @@ int solve_dirac_eo(dwf_ctx* c, const double* src, const double* x0, double rtol,
     double worst = 1.0;
     const int passes = 4;
     int used = 0;
-    for (int pass = 0; pass < passes; ++pass) {
+    bool guess_done = false;
+    if (x0) {
+        // certify the guess first: the first pass then reduces only what the guess leaves
+        // (its tolerance scales with the worst residual), and a right-hand side the guess
+        // already solves takes no pass
+        full.residual(x, B, delta, nullptr, nullptr, nullptr, partC, nullptr, 0, 0);
+        launch_cg_start(full.nblk, partC, cert, rtol, c->reliable_delta, nrhs, s);
+        c->check_launch("certification of the guess");
+        have_cert = true;
+        read_states(c, cert, nrhs);
+        worst = 0.0;
+        guess_done = true;
+        for (int i = 0; i < nrhs; ++i) {
+            if (!(c->st_host[i].flags[0] & F_DONE)) {
+                guess_done = false;
+                worst = std::max(worst, c->st_host[i].relres);
+            }
+        }
+    }
+    for (int pass = 0; pass < passes && !guess_done; ++pass) {
         // Dirac residual of the current iterate
         const double* r_in = src;
         if (pass > 0 || x0) {
