@@ -599,10 +599,16 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
         if (a.st != nullptr && (a.st[rhs].flags[a.slot] & a.skip)) return;  // block-uniform
     }
     R alpha = 0;
+    // CGUPD: the partials of p^dag A p are loaded first and folded (in every wave, no
+    // barrier) after psi's loads are issued, so their latency overlaps; entries lane and
+    // lane + 64, clamped (a remainder loop covers longer partial arrays)
+    double cpa0 = 0.0, cpa1 = 0.0, crr = 0.0;
     if constexpr (EPI == EPI_CGUPD) {
-        const double pAp = sum_partials(a.partA + (size_t)rhs * pstride * 2, pstride, sh);
-        if (pAp == 0.0) return;
-        alpha = (R)(a.st[rhs].rr[a.slot] / pAp);
+        const int lane = threadIdx.x & 63;
+        const double* pa = a.partA + (size_t)rhs * pstride * 2;
+        cpa0 = pa[2 * min(lane, pstride - 1)];
+        cpa1 = pa[2 * min(lane + 64, pstride - 1)];
+        crr = a.st[rhs].rr[a.slot];
     }
     const int Nxh = L.Nx / 2, V4h = L.V4 / 2;
     const int tiles_x = L.Nx / TX;
@@ -665,6 +671,18 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
 #pragma unroll
                 for (int k = 0; k < LH; ++k) po[k] = ld(psrc, (long long)k * V4h);
             }
+        }
+        if constexpr (EPI == EPI_CGUPD) {
+            // alpha = rr / p^dag A p: the sum of sum_partials in the same order, per wave
+            const int lane = threadIdx.x & 63;
+            double v = (lane < pstride ? cpa0 : 0.0) + (lane + 64 < pstride ? cpa1 : 0.0);
+            if (pstride > 128) {
+                const double* pa = a.partA + (size_t)rhs * pstride * 2;
+                for (int i = lane + 128; i < pstride; i += 64) v += pa[2 * i];
+            }
+            const double pAp = wave_sum(v);
+            if (pAp == 0.0) return;  // block-uniform (every wave has the same sum)
+            alpha = (R)(crr / pAp);
         }
         if constexpr (PIPE) {
             if (fold_pending) {
