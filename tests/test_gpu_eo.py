@@ -180,12 +180,14 @@ def test_eo_falls_back_on_odd_extent_and_guess():
         x2, st2, rc = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, x0=g)
         assert rc == 0 and normal_relres(th, x2, src, 8, 6, 4) <= 1e-10
         assert st2[0]["iterations"] < st[0]["iterations"]
-        # a guess that already solves the system is certified and returned: no pass
-        x3, st3, rc = ctx.solve_dirac(np.stack([src, src]), rtol=1e-10, maxiter=20000,
-                                      x0=np.stack([x, g]))
+        # a guess that already solves the system is certified first: no pass for a batch of
+        # such guesses, no iteration for such a right-hand side beside one that needs them
+        x3, st3, rc = ctx.solve_dirac(src, rtol=1e-10, maxiter=20000, x0=x)
         assert rc == 0 and st3[0]["converged"] and st3[0]["passes"] == 0 and st3[0]["iterations"] == 0
-        assert np.array_equal(x3[0], x)
-        assert st3[1]["converged"] and st3[1]["passes"] >= 1 and normal_relres(th, x3[1], src, 8, 6, 4) <= 1e-10
+        assert np.array_equal(x3, x)
+        x4, st4, rc = ctx.solve_dirac(np.stack([src, src]), rtol=1e-10, maxiter=20000, x0=np.stack([x, g]))
+        assert rc == 0 and st4[0]["converged"] and st4[0]["iterations"] == 0 and np.array_equal(x4[0], x)
+        assert st4[1]["converged"] and st4[1]["passes"] >= 1 and normal_relres(th, x4[1], src, 8, 6, 4) <= 1e-10
 
 
 def test_autotune_batch(tmp_path):

@@ -527,7 +527,10 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
     };
     // Blocks of kGraphIters iterations starting on an even iteration are one graph launch.
     constexpr int kGraphIters = 8;
-    hipGraphExec_t block = nullptr;
+    // Blocks starting on an odd iteration have a graph of their own (the slots alternate
+    // the other way), so every full block of a chunk is replayed whatever the parity the
+    // previous chunk ended on.
+    hipGraphExec_t block[2] = {nullptr, nullptr};
     if (c->use_graphs && maxiter >= kGraphIters) {
         Sig key{(unsigned long long)mixed, (unsigned long long)nrhs, (unsigned long long)nv,
                 (unsigned long long)op.nblk, (unsigned long long)mode, sig_ptr(x),   sig_ptr(r64),  sig_ptr(p64),
@@ -539,17 +542,24 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
         // block's last one launches its own (so a block always ends with complete states)
         const bool fold = pipe && c->eo_pipe == 2;
         key.push_back((unsigned long long)fold);
-        block = c->cached_graph(key, [&] {
-            for (int i = 0; i < kGraphIters; ++i) iteration(i, fold && i > 0, !fold || i == kGraphIters - 1);
-        });
+        // room for both in the cache, so capturing the second cannot evict the first
+        if (c->graphs.size() + 2 > 64) c->drop_graphs();
+        for (int par = 0; par < 2; ++par) {
+            Sig kp = key;
+            kp.push_back((unsigned long long)par);
+            block[par] = c->cached_graph(kp, [&] {
+                for (int i = 0; i < kGraphIters; ++i)
+                    iteration(par + i, fold && i > 0, !fold || i == kGraphIters - 1);
+            });
+        }
     }
     while (!all_done(c->st_host, it & 1, nrhs)) {
         const int n = std::min(chunk, maxiter - it);
         if (n <= 0) break;
         int j = 0;
-        if (block && (it & 1) == 0)
+        if (block[it & 1])
             for (; j + kGraphIters <= n; j += kGraphIters, it += kGraphIters)
-                c->check(hipGraphLaunch(block, s), "hipGraphLaunch");
+                c->check(hipGraphLaunch(block[it & 1], s), "hipGraphLaunch");
         for (; j < n; ++j, ++it) iteration(it, false, true);
         c->check_launch("CG iteration");
         if (mixed && op.lay.gr.n == 1) {
