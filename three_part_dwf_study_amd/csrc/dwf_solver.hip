@@ -589,6 +589,11 @@ void cg_run(dwf_ctx* c, CgOperator& op, bool mixed, const double* b, const doubl
                 c0_it[i] = S.iters;
             }
             if (best < 1e29) chunk = std::max(8, std::min(1024, (int)std::ceil(best)));
+            // whole blocks of kGraphIters (replayed as graphs when they are on; the same
+            // schedule without them, so both give the same iterates): a right-hand side that
+            // freezes before the chunk ends skips the rest, where a remainder would run as
+            // eager launches
+            chunk = (chunk + kGraphIters - 1) / kGraphIters * kGraphIters;
         } else if (mixed) {
             // Several lattices converge at different rates: per right-hand side reliable
             // updates (x64 += x32, FP64 true residual, certify or start the next
