@@ -823,6 +823,17 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
         fold_signs(g);
         const int etp = (i2 + 2) * WB + ((j2 + 1) >> 1), etm = i2 * WB + ((j2 + 1) >> 1);
         const int exp = (i2 + 1) * WB + ((j2 + 2) >> 1), exm = (i2 + 1) * WB + (j2 >> 1);
+        // CGUPD: p, x, r of slice q + 1 are loaded before slice q's stores (software
+        // pipelined by hand: loads after a store to the same arrays would each wait for a
+        // full round trip, as the stores may alias them); eo_epi's arithmetic
+        constexpr bool PIPE2 = EPI == EPI_CGUPD;
+        Sp<R> ep_p[2], ep_x[2], ep_r[2];
+        if constexpr (PIPE2) {
+            const long long i0 = (long long)s0 * V4h + hs;
+            ep_p[0] = ld(pdir + rb, i0);
+            ep_x[0] = ld(a.x + rb, i0);
+            ep_r[0] = ld(a.r + rb, i0);
+        }
 #pragma unroll
         for (int q = 0; q < SPT; ++q) {
             const int s = s0 + q;
@@ -846,7 +857,31 @@ __global__ __launch_bounds__(kBlockS, sizeof(R) == 4 ? 5 : 2) void k_eo_schur(co
             }
             const Sp<R>* ws = w + s * nbox;
             hop_sum<R, DAG, false>(o, a.ch, g, ws[etp], ws[etm], ws[exp], ws[exm]);
-            eo_epi<R, EPI>(a, pdir, rb, (long long)s * V4h + hs, o, alpha, acc0, acc1);
+            if constexpr (PIPE2) {
+                if (q + 1 < SPT) {
+                    const long long in = (long long)(s + 1) * V4h + hs;
+                    ep_p[(q + 1) & 1] = ld(pdir + rb, in);
+                    ep_x[(q + 1) & 1] = ld(a.x + rb, in);
+                    ep_r[(q + 1) & 1] = ld(a.r + rb, in);
+                }
+                const long long idx = (long long)s * V4h + hs;
+                const Sp<R>& pv = ep_p[q & 1];
+                Sp<R> xv = ep_x[q & 1];
+                xv.ar += alpha * pv.ar;
+                xv.ai += alpha * pv.ai;
+                xv.br += alpha * pv.br;
+                xv.bi += alpha * pv.bi;
+                st(a.x + rb, idx, xv);
+                Sp<R> rv = ep_r[q & 1];
+                rv.ar -= alpha * o.ar;
+                rv.ai -= alpha * o.ai;
+                rv.br -= alpha * o.br;
+                rv.bi -= alpha * o.bi;
+                st(a.r + rb, idx, rv);
+                acc0 += n2(rv);
+            } else {
+                eo_epi<R, EPI>(a, pdir, rb, (long long)s * V4h + hs, o, alpha, acc0, acc1);
+            }
         }
     }
     if constexpr (EPI != EPI_STORE) {
