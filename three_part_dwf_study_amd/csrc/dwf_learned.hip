@@ -853,24 +853,24 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         return pack_pairs(v.x, v.y);
     };
 
-    // One 16-site tile of a C_in = 8 layer: rin the input ring, SM / S0 / SP the slots of
-    // slices s-1, s, s+1, e0 the lane's element (hel) in the input region and etp / etm /
-    // exp / exm those of its +t / -t / +x / -x neighbours, u the transporters at the
-    // output site; win[0], win[1] the values at s-1 and s (loaded here when `first`),
-    // advanced to s, s+1.
-    auto hop8 = [&](auto Kc, auto Lc, auto SMc, auto S0c, auto SPc, const auto& rin, int e0, int etp, int etm,
-                    int exp, int exm, const float2 (&u)[4], uint2 (&win)[2], bool first) {
-        constexpr int SM = decltype(SMc)::value, S0 = decltype(S0c)::value, SP = decltype(SPc)::value;
-        const uint2* r0 = &rin[SM][0][0];
+    // One 16-site tile of a C_in = 8 layer: rin the input ring, S0 the slot of slice s,
+    // etp / etm / exp / exm the lane's +t / -t / +x / -x neighbours (hel) in the input
+    // region, u the transporters at the output site; win[0], win[1] the lane's own values
+    // at s-1 and s and up its value at s+1.
+    //
+    // The window is advanced in every step of the walk, also where the layer is outside
+    // its range (advance8): at step j the slot of slice s+1 holds the value the layer
+    // below wrote in step j-1, so the two steps before a layer's first slice fill its
+    // window with the same values a load at that slice would read, and no step branches
+    // on a first slice (a branch there left the window in different registers on its two
+    // sides, paid for with register moves in every step).  Steps before the layer below
+    // has written read slots not yet written; those values leave the window before use.
+    auto hop8 = [&](auto Kc, auto Lc, auto S0c, const auto& rin, int etp, int etm, int exp, int exm,
+                    const float2 (&u)[4], const uint2 (&win)[2], uint2 up) {
+        constexpr int S0 = decltype(S0c)::value;
         const uint2* rs = &rin[S0][0][0];
-        const uint2* rp = &rin[SP][0][0];
-        if (first) {
-            win[0] = r0[e0];
-            win[1] = rs[e0];
-        }
         bf16x4_t h[7], W[7];
         (void)u;
-        const uint2 up = rp[e0];
         const uint2 vtp = rs[etp], vtm = rs[etm], vxp = rs[exp], vxm = rs[exm];
         h[0] = pack_pairs(win[1].x, win[1].y);
         h[1] = pack_pairs(vtp.x, vtp.y);
@@ -881,14 +881,21 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         h[6] = pack_pairs(win[0].x, win[0].y);
 #pragma unroll
         for (int q = 0; q < 7; ++q) W[q] = frag(Kc, decltype(Lc)::value, q);
-        win[0] = win[1];
-        win[1] = up;
         if constexpr (TG) {
             const float2 ux[2] = {u[2], u[3]};
             return tile8_tg(W, h, ux);
         } else {
             return tile8(W, h, u);
         }
+    };
+
+    // the values at slice s+1 (slot SP of the ring) of every tile, read in every step
+    auto ahead8 = [&](auto SPc, const auto& rin, int e0) {
+        return (&rin[decltype(SPc)::value][0][0])[e0];
+    };
+    auto advance8 = [&](uint2 (&win)[2], uint2 up) {
+        win[0] = win[1];
+        win[1] = up;
     };
 
     // ---- layer 1 (every wave): R1 tiles wv*2 + e (origin t0-2, x0-2), input ring0 (width W0).
@@ -986,18 +993,22 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             const int j = __builtin_amdgcn_readfirstlane(jv);  // keep the walk's arithmetic scalar
             const int s2 = j - 7;
             layer1(Kc, j);
+            uint2 up[kFuL2Tiles];
+#pragma unroll
+            for (int e = 0; e < kFuL2Tiles; ++e) up[e] = ahead8(IC<(K + 3) & 3>{}, ring1, e0[e]);
             if (s2 >= -1 && s2 <= Ls) {
 #pragma unroll
                 for (int e = 0; e < kFuL2Tiles; ++e)
                     {
-                        const f32x4_t acc = hop8(Kc, IC<1>{}, IC<(K + 1) & 3>{}, IC<(K + 2) & 3>{},
-                                                 IC<(K + 3) & 3>{}, ring1, e0[e], etp[e], etm[e], exp[e], exm[e], lk[e], win[e],
-                                                 s2 == -1);
+                        const f32x4_t acc = hop8(Kc, IC<1>{}, IC<(K + 2) & 3>{}, ring1, etp[e], etm[e], exp[e], exm[e],
+                                                 lk[e], win[e], up[e]);
                         // past the region's end: the clamped last site's value, stored there again
                         (&ring2[(K + 2) & 3][0][0])[eo[e]] =
                             make_uint2(pk_bf16(acc[0], acc[1]), pk_bf16(acc[2], acc[3]));
                     }
             }
+#pragma unroll
+            for (int e = 0; e < kFuL2Tiles; ++e) advance8(win[e], up[e]);
             loads(sl, in_new, w_new);
             stores(Kc, j, in_old, w_old);
         };
@@ -1053,12 +1064,15 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             const int j = __builtin_amdgcn_readfirstlane(jv);  // keep the walk's arithmetic scalar
             const int s3 = j - 9;
             layer1(Kc, j);
+            uint2 up3[kFuL3Tiles];
+#pragma unroll
+            for (int e = 0; e < kFuL3Tiles; ++e) up3[e] = ahead8(IC<(K + 1) & 3>{}, ring2, e3[e]);
             if (s3 >= 0 && s3 <= Ls - 1) {
 #pragma unroll
                 for (int e = 0; e < kFuL3Tiles; ++e)
                     {
-                        const f32x4_t acc = hop8(Kc, IC<2>{}, IC<(K + 3) & 3>{}, IC<K & 3>{}, IC<(K + 1) & 3>{},
-                                                 ring2, e3[e], e3p[e], e3m[e], e3xp[e], e3xm[e], lk3[e], win3[e], s3 == 0);
+                        const f32x4_t acc = hop8(Kc, IC<2>{}, IC<K & 3>{}, ring2, e3p[e], e3m[e], e3xp[e], e3xm[e],
+                                                 lk3[e], win3[e], up3[e]);
                         if (v3[e]) {
                             if constexpr (TG) {
                                 const float2 c0 = cmulf(om3[e], make_float2(acc[0], acc[1]));
@@ -1070,6 +1084,8 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
                         }
                     }
             }
+#pragma unroll
+            for (int e = 0; e < kFuL3Tiles; ++e) advance8(win3[e], up3[e]);
             loads(sl, in_new, w_new);
             stores(Kc, j, in_old, w_old);
         };
