@@ -262,7 +262,9 @@ int dwf_solve(dwf_ctx* ctx, int normal_op, int method, const double* b, const do
               double rtol, int maxiter, int nrhs, double* x, dwf_solve_stats* stats);
 
 /* Device-buffer form (FP64 b, x0 nullable, x).  Synchronises the context stream before
- * returning because convergence is decided on the host. */
+ * returning because convergence is decided on the host.  x may not overlap b, and x0 is
+ * either x itself or disjoint from it (DWF_ERR_ARG otherwise); the same holds for src
+ * in dwf_solve_dirac_device. */
 int dwf_solve_device(dwf_ctx* ctx, int normal_op, int method, const double* b_device,
                      const double* x0_device, double rtol, int maxiter, int nrhs,
                      double* x_device, dwf_solve_stats* stats);

@@ -122,3 +122,38 @@ def test_graph_replay_matches_eager(method):
             g1, gs1, _ = ctx.solve_dirac(src[:1], rtol=1e-10, maxiter=20000, method=method)
             assert np.array_equal(g1, eager1), rep
             assert gs1[0]["iterations"] == st1[0]["iterations"]
+
+
+def test_device_solve_rejects_overlapping_operands():
+    """The device solve entry points refuse an x that shares memory with the right-hand
+    side, or a guess that partly overlaps x, before any device work; a guess passed as x
+    itself stays allowed.  Every pointer tried lies inside one buffer three vectors long,
+    so no candidate reaches outside it."""
+    Nt, Nx, Ls = 8, 6, 4
+    print_case(test="device_solve_overlap")
+    rng = np.random.default_rng(8)
+    th = rand_theta(rng, Nt, Nx)
+    n = Ls * Nt * Nx * 2  # complex values of one vector
+    b = rand_spinor(rng, n)
+    host = np.zeros(3 * n, complex)
+    host[n:2 * n] = b
+    vb = n * 16  # bytes of one vector
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        big = capi.DeviceArray.from_host(host.view(np.float64))
+        pb, px = big.ptr + vb, big.ptr  # b in the middle third, x in the first
+        calls = (
+            lambda x, x0: ctx.solve_dirac_device(dwf.DWF_SOLVE_MIXED, pb, x0, 1e-10, 5000, 1, x),
+            lambda x, x0: ctx.solve_device(dwf.DWF_OP_DDDAG, dwf.DWF_SOLVE_CG64, pb, x0, 1e-10, 5000, 1, x),
+        )
+        for call in calls:
+            for x, x0 in ((pb, None), (pb + 64, None), (px + 64, None), (px, px + 64)):
+                with pytest.raises(capi.DwfError) as e:
+                    call(x, x0)
+                assert e.value.code == capi.DWF_ERR_ARG and "overlap" in str(e.value)
+            st, rc = call(px, None)
+            assert rc == 0 and st[0]["converged"]
+            st2, rc = call(px, px)  # the previous solution as the guess, in place
+            assert rc == 0 and st2[0]["converged"] and st2[0]["iterations"] < st[0]["iterations"] // 4
+        assert np.array_equal(big.download(np.complex128)[n:2 * n], b)  # b never written
+        big.free()

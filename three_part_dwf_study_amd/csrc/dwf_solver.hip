@@ -1208,12 +1208,33 @@ int dwf_solve_dirac_impl(dwf_ctx* c, int method, const double* src, const double
     return solve_dirac(c, method, src, x0, rtol, maxiter, nrhs, x, stats);
 }
 
+namespace {
+// The solvers write x while they still read the right-hand side (every residual pass)
+// and read a guess before x is first written, so x may not overlap the right-hand side,
+// and a guess is either x itself or disjoint from it.
+void check_device_operands(dwf_ctx* c, int nrhs, const double* in, const double* x0, const double* x,
+                           const char* who)
+{
+    if (!c || !in || !x || nrhs < 1) return;  // the solver reports these itself
+    c->bind_rhs(nrhs);
+    const size_t n = (size_t)c->vec_sites(nrhs) * 4;
+    auto overlap = [n](const double* a, const double* b) { return a < b + n && b < a + n; };
+    if (overlap(in, x))
+        throw DwfError(DWF_ERR_ARG, std::string(who) + ": x must not overlap the right-hand side");
+    if (x0 && x0 != x && overlap(x0, x))
+        throw DwfError(DWF_ERR_ARG, std::string(who) + ": x0 must be x itself or not overlap it");
+}
+}  // namespace
+
 extern "C" {
 
 int dwf_solve_device(dwf_ctx* c, int normal_op, int method, const double* b, const double* x0, double rtol,
                      int maxiter, int nrhs, double* x, dwf_solve_stats* stats)
 {
-    return guard([&]() -> int { return solve_normal(c, normal_op, method, b, x0, rtol, maxiter, nrhs, x, stats); });
+    return guard([&]() -> int {
+        check_device_operands(c, nrhs, b, x0, x, "dwf_solve_device");
+        return solve_normal(c, normal_op, method, b, x0, rtol, maxiter, nrhs, x, stats);
+    });
 }
 
 int dwf_solve(dwf_ctx* c, int normal_op, int method, const double* b, const double* x0, double rtol, int maxiter,
@@ -1239,7 +1260,10 @@ int dwf_solve(dwf_ctx* c, int normal_op, int method, const double* b, const doub
 int dwf_solve_dirac_device(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter,
                            int nrhs, double* x, dwf_solve_stats* stats)
 {
-    return guard([&]() -> int { return solve_dirac(c, method, src, x0, rtol, maxiter, nrhs, x, stats); });
+    return guard([&]() -> int {
+        check_device_operands(c, nrhs, src, x0, x, "dwf_solve_dirac_device");
+        return solve_dirac(c, method, src, x0, rtol, maxiter, nrhs, x, stats);
+    });
 }
 
 int dwf_solve_dirac(dwf_ctx* c, int method, const double* src, const double* x0, double rtol, int maxiter, int nrhs,
