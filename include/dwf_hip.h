@@ -251,7 +251,9 @@ int dwf_apply_device_noalloc(dwf_ctx* ctx, int op, int precision, const void* in
  * (D D^dagger)^-1 = Gamma D^-1 Gamma D^-1 (D^dagger = Gamma D Gamma, Gamma = sigma3 R)
  * with two even-odd Dirac solves per correction pass and certifies every pass on the FP64
  * residual of A itself (up to four passes, then the mixed CG continues and decides; stats
- * sum the Dirac solves' iterations, passes = correction passes); on a Wilson context or a
+ * sum the Dirac solves' iterations, passes = correction passes; maxiter bounds each Dirac
+ * solve and the continuing CG on its own, as it bounds the mixed CG's FP32 cycle and its
+ * FP64 fallback on their own); on a Wilson context or a
  * lattice without an even-odd kernel it is the mixed CG.  DWF_SOLVE_EO_MIXED and
  * DWF_SOLVE_LEARNED need a context holding one lattice size.  Replaces
  * conjugate_gradient (domain_wall/include/cg_solver.hpp:17-64); the Wilson solves use
