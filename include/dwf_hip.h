@@ -135,7 +135,11 @@ void* dwf_ctx_stream(dwf_ctx* ctx);
  * one-lattice dwf_solve_dirac* of at least twice as many right-hand sides (contiguous
  * ranges, whole configurations when several are held), each on a stream, workspace and
  * host thread of its own: the same per-right-hand-side solves and certification, the
- * batch schedule (chunk lengths, reliable updates) per sub-batch; 1 solves the batch as one. */
+ * batch schedule (chunk lengths, reliable updates) per sub-batch; 1 solves the batch as one;
+ * DWF_TUNE_NET_SPLIT: the fused network's parts of the fifth dimension per output tile,
+ * one block each (n dividing Ls; the parts recompute their neighbours' edge slices, the
+ * same values bit for bit), 0 (default) the most parts up to 4 of at least 4 slices each
+ * that keep a launch within 256 blocks, one per CU (one right-hand side at 96 x 96: 2). */
 enum dwf_tune_param {
     DWF_TUNE_SCHUNKS = 0,
     DWF_TUNE_XCD_REMAP = 1,
@@ -149,7 +153,8 @@ enum dwf_tune_param {
     DWF_TUNE_FCG_BETA = 9,
     DWF_TUNE_EO_FUSED = 10,
     DWF_TUNE_NET_TILE = 11,
-    DWF_TUNE_STREAMS = 12
+    DWF_TUNE_STREAMS = 12,
+    DWF_TUNE_NET_SPLIT = 13
 };
 int dwf_ctx_tune(dwf_ctx* ctx, int param, double value);
 

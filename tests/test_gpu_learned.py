@@ -469,3 +469,61 @@ def test_learned_solve_full_size():
         lhs = oracle.dwf_apply(1, th, s2[r] - Dx, Nt, Nx, Ls, M5, MF)
         rhs = oracle.dwf_apply(1, th, s2[r], Nt, Nx, Ls, M5, MF)
         assert np.linalg.norm(lhs) / np.linalg.norm(rhs) <= rtol * 1.01, r
+
+
+@pytest.mark.parametrize("variant", [1, 2])
+def test_fused_network_split_fifth_dimension(variant):
+    """Splitting each output tile's walk over the fifth dimension into parts of one block
+    each (DWF_TUNE_NET_SPLIT) recomputes the parts' edge slices from the same data and
+    weights, so every split gives the unsplit kernel's output bit for bit, in both kernel
+    variants, down to parts of one slice; an Ls that a split does not divide is refused."""
+    Nt, Nx, Ls = 16, 12, 8
+    print_case(test="fused_network_split", variant=variant)
+    rng = np.random.default_rng(44 + variant)
+    th = rand_theta(rng, Nt, Nx)
+    w = weights32(rng, Ls)
+    v = rand_spinor(rng, 2 * Ls * Nt * Nx * 2).reshape(2, -1)
+    net = dwf.Network(Ls, seed=1)
+    net.set_weights(w)
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.tune(dwf.DWF_TUNE_NET_FUSED, variant)
+        ctx.tune(dwf.DWF_TUNE_NET_SPLIT, 1)
+        z1 = net.apply(ctx, v)
+        for n in (2, 4, 8, 0):
+            ctx.tune(dwf.DWF_TUNE_NET_SPLIT, n)
+            assert np.array_equal(net.apply(ctx, v), z1), n
+        for bad in (3, 16, -1):
+            with pytest.raises(dwf.DwfError):
+                ctx.tune(dwf.DWF_TUNE_NET_SPLIT, bad)
+    zq = oracle.net_apply(th, w.astype(np.float64), v, Nt, Nx, Ls, bf16=True)
+    if variant == 1:
+        for r in range(2):
+            assert rel(z1[r], zq[r]) <= 1e-3, rel(z1[r], zq[r])
+
+
+def test_learned_solve_split_network():
+    """One right-hand side, the regime the split is for: the learned solve with the
+    network's walk split in 4 parts (8 tiles x 4 = 32 blocks, the default there) and
+    unsplit both certify at the FP64 true residual with the same iteration count up to the
+    last bits of the partial sums' order."""
+    Nt, Nx, Ls = 32, 32, 16
+    print_case(test="learned_solve_split")
+    rng = np.random.default_rng(5)
+    th = rand_theta(rng, Nt, Nx)
+    src = boundary_sources(rng, 1, Nt, Nx, Ls)
+    net = dwf.Network(Ls, seed=3)
+    its = {}
+    with dwf.DwfContext.dwf(Nt, Nx, Ls, M5, MF) as ctx:
+        ctx.set_gauge(th)
+        ctx.set_preconditioner(net)
+        for n in (1, 4, 0):
+            ctx.tune(dwf.DWF_TUNE_NET_SPLIT, n)
+            x, st, _ = ctx.solve_dirac(src, rtol=1e-10, maxiter=5000, method=dwf.DWF_SOLVE_LEARNED)
+            assert st[0]["converged"] and st[0]["true_relres"] <= 1e-10 and st[0]["fp64_iterations"] == 0
+            its[n] = st[0]["iterations"]
+            Dx = oracle.dwf_apply(0, th, x[0], Nt, Nx, Ls, M5, MF)
+            lhs = oracle.dwf_apply(1, th, src[0] - Dx, Nt, Nx, Ls, M5, MF)
+            rhs = oracle.dwf_apply(1, th, src[0], Nt, Nx, Ls, M5, MF)
+            assert np.linalg.norm(lhs) / np.linalg.norm(rhs) <= 1.01e-10
+    assert its[4] == its[0] and abs(its[4] - its[1]) <= 3, its

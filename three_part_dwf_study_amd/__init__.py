@@ -26,6 +26,7 @@ from .capi import (  # noqa: F401
     DWF_TUNE_EO_FUSED,
     DWF_TUNE_NET_TILE,
     DWF_TUNE_STREAMS,
+    DWF_TUNE_NET_SPLIT,
     DWF_TUNE_PREFETCH,
     DWF_TUNE_RELIABLE_DELTA,
     DWF_TUNE_SCHUNKS,

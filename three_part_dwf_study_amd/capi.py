@@ -52,6 +52,7 @@ DWF_TUNE_FCG_BETA = 9
 DWF_TUNE_EO_FUSED = 10
 DWF_TUNE_NET_TILE = 11
 DWF_TUNE_STREAMS = 12
+DWF_TUNE_NET_SPLIT = 13
 
 
 class DwfError(RuntimeError):

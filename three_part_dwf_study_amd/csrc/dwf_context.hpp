@@ -158,6 +158,7 @@ struct dwf_ctx {
     int eo_pipe = 2;              // fused Schur CG: 1 direction update inside the operator launch,
                                   // 2 also the state update inside the next one (graph blocks)
     int net_tile = 0;             // fused network's output tile ty * 256 + tx (0: the largest that divides)
+    int net_split = 0;            // fused network's parts of the fifth dimension per tile (0: by block count)
     double reliable_delta = 0.01; // mixed CG: reliable update after a 100x residual drop
     bool delta_set = false;       // reliable_delta set by hand (else each operator's default)
     int use_graphs = 1;           // replay blocks of CG iterations as HIP graphs

@@ -163,6 +163,7 @@ dwf_ctx* dwf_ctx::kid(int i, int cfg0, int kid_ncfg)
     k->eo_fused = eo_fused;
     k->eo_pipe = eo_pipe;
     k->net_tile = net_tile;
+    k->net_split = net_split;
     k->reliable_delta = reliable_delta;
     k->delta_set = delta_set;
     k->use_graphs = use_graphs;
@@ -537,6 +538,14 @@ int dwf_ctx_tune(dwf_ctx* ctx, int param, double value)
             if (v != 0 && (ty < 1 || ty > 8 || tx < 1 || tx > 16 || ctx->L.Nt % ty || ctx->L.Nx % tx)) break;
             if (v != ctx->net_tile) ctx->drop_graphs();
             ctx->net_tile = v;
+            return DWF_OK;
+        }
+        case DWF_TUNE_NET_SPLIT: {
+            if (!(value >= 0.0 && value == (double)(int)value)) break;
+            const int v = (int)value;
+            if (v != 0 && (v > ctx->L.Ls || ctx->L.Ls % v)) break;
+            if (v != ctx->net_split) ctx->drop_graphs();
+            ctx->net_split = v;
             return DWF_OK;
         }
         default:

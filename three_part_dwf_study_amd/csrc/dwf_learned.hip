@@ -644,6 +644,7 @@ struct NetFusedArgs {
     int Nt, Nx, Ls, V4;
     int TY, TX;            // output tile, dividing Nt and Nx
     int tiles_x, tiles;    // tiles along x, tiles per lattice
+    int nsplit;            // parts of the fifth dimension per tile (one block each), dividing Ls
     const void* in;        // [nrhs][Ls][V4] spinors (TIN)
     void* out;             // [nrhs][Ls][V4] spinors (TOUT)
     const float2* links;   // [cfg][2][V4] (cos, sin)
@@ -716,10 +717,11 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     __shared__ double red[2 * kFuWaves];
 
     const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-    const int rhs = lb / a.tiles, tile = lb - rhs * a.tiles;
+    const int nsp = a.nsplit, rt = lb / nsp, part = lb - rt * nsp;
+    const int rhs = rt / a.tiles, tile = rt - rhs * a.tiles;
     if (a.st && (a.st[rhs].flags[0] & (F_DONE | F_FROZEN))) return;  // block-uniform
     if constexpr (DOTS) {
-        if (a.hpart && tile == 0) {
+        if (a.hpart && tile == 0 && part == 0) {
             // the iteration's check (as k_fcg_check), by the block of tile 0 alone; the
             // other blocks compute z regardless (a right-hand side that converges or breaks
             // down here wastes one network application; the kernels after this one skip it)
@@ -794,12 +796,17 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     const TIN* inr = static_cast<const TIN*>(a.in) + (long long)rhs * Ls * V4 * 4;
     TOUT* outr = static_cast<TOUT*>(a.out) + (long long)rhs * Ls * V4 * 4;
     auto smod = [Ls](int s) { return ((s % Ls) + Ls) % Ls; };
+    // The block's output slices s_lo .. s_lo + Lw - 1: the walk below runs in slices
+    // relative to s_lo (its ranges end at Lw), global loads and stores at s_lo + s (mod
+    // Ls).  Parts of one tile recompute their neighbours' edge slices (the walk's extended
+    // range), which buys blocks when a launch has too few to fill the GPU.
+    const int Lw = Ls / nsp, s_lo = part * Lw;
     // this thread's 16 bytes of a step's weight record: layer 1 (threads 0..63), layer 2
     // (64..287), layer 3 (288..511); the layers run 5, 7 and 9 steps behind the step index
     const int wl = tid < 64 ? 0 : (tid < 288 ? 1 : 2);
     const int wbase = (tid < 64 ? tid : (tid < 288 ? tid - 64 : tid - 288)) * 8;  // shorts
 
-    // The walk runs steps j = -3 .. Ls + 8 four at a time; the position K = (j + 3) mod 4
+    // The walk runs steps j = -3 .. Lw + 8 four at a time; the position K = (j + 3) mod 4
     // is a compile-time constant in each copy of the step, and so is every ring slot
     // (slot(s) = s mod 4): slot(j - lag + d) = (K + 13 - lag + d) mod 4.  Nothing runs
     // before step 3, whose layer 1 needs input slices -3 .. -1 (loaded in steps -1 .. 1).
@@ -811,15 +818,18 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     // the path.  Step j loads input slice j-2 and the weights of step j+2, and stores
     // input slice j-3 (its ring slot held slice j-7, last read by layer 1 in step j-1) and
     // the weights of step j+1 (their buffer was last read in step j-1).
-    const int jend = -3 + 4 * ((Ls + 15) / 4);
-    // The slices a step loads, smod(j - 2) (input; past slice Ls + 2 the loaded value is
-    // not used) and smod(j - 3), smod(j - 5), smod(j - 7) (weights of layers 1-3), are
+    const int jend = -3 + 4 * ((Lw + 15) / 4);
+    // The slices a step loads, smod(s_lo + j - 2) (input; past relative slice Lw + 2 the
+    // loaded value is not used) and smod(s_lo + j - 3), smod(s_lo + j - 5), smod(s_lo + j - 7)
+    // (weights of layers 1-3), are
     // carried from step to step as scalar counters: a modulo per step costs ~25 scalar
     // instructions.
     struct Slices {
         int c2, c3, c5, c7;
     };
-    auto slices_at = [&](int j) { return Slices{smod(j - 2), smod(j - 3), smod(j - 5), smod(j - 7)}; };
+    auto slices_at = [&](int j) {
+        return Slices{smod(s_lo + j - 2), smod(s_lo + j - 3), smod(s_lo + j - 5), smod(s_lo + j - 7)};
+    };
     auto loads = [&](Slices& c, float4& in_new, uint4& w_new) {
         in_new = ld_spinor(inr, (long long)c.c2 * V4 + max(gsite, 0));
         // the three layers' slices are wave-uniform; the thread picks its layer's record
@@ -835,7 +845,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     };
     auto stores = [&](auto Kc, int j, const float4& in_old, const uint4& w_old) {
         constexpr int K = decltype(Kc)::value;
-        if (j - 3 >= -3 && j - 3 <= Ls + 2 && gsite >= 0) {
+        if (j - 3 >= -3 && j - 3 <= Lw + 2 && gsite >= 0) {
             float4 v = in_old;
             if constexpr (TG) {
                 const float2 c0 = cmulf(om_in, make_float2(v.x, v.y)), c1 = cmulf(om_in, make_float2(v.z, v.w));
@@ -934,7 +944,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
     auto layer1 = [&](auto Kc, int j) {
         constexpr int K = decltype(Kc)::value;
         const int s1 = j - 5;
-        if (s1 < -2 || s1 > Ls + 1) return;
+        if (s1 < -2 || s1 > Lw + 1) return;
         constexpr int S0 = K & 3;
         const int sa = (K + dsa) & 3, sb = (K + dsb) & 3;  // slots of the lane's step-0 / step-1 points
         const bf16x4_t Wa0 = frag(Kc, 0, 0), Wa1 = frag(Kc, 0, 1);
@@ -996,7 +1006,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             uint2 up[kFuL2Tiles];
 #pragma unroll
             for (int e = 0; e < kFuL2Tiles; ++e) up[e] = ahead8(IC<(K + 3) & 3>{}, ring1, e0[e]);
-            if (s2 >= -1 && s2 <= Ls) {
+            if (s2 >= -1 && s2 <= Lw) {
 #pragma unroll
                 for (int e = 0; e < kFuL2Tiles; ++e)
                     {
@@ -1067,7 +1077,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
             uint2 up3[kFuL3Tiles];
 #pragma unroll
             for (int e = 0; e < kFuL3Tiles; ++e) up3[e] = ahead8(IC<(K + 1) & 3>{}, ring2, e3[e]);
-            if (s3 >= 0 && s3 <= Ls - 1) {
+            if (s3 >= 0 && s3 <= Lw - 1) {
 #pragma unroll
                 for (int e = 0; e < kFuL3Tiles; ++e)
                     {
@@ -1077,9 +1087,9 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
                             if constexpr (TG) {
                                 const float2 c0 = cmulf(om3[e], make_float2(acc[0], acc[1]));
                                 const float2 c1 = cmulf(om3[e], make_float2(acc[2], acc[3]));
-                                st_spinor(outr, (long long)s3 * V4 + o3[e], c0.x, c0.y, c1.x, c1.y);
+                                st_spinor(outr, (long long)(s_lo + s3) * V4 + o3[e], c0.x, c0.y, c1.x, c1.y);
                             } else {
-                                st_spinor(outr, (long long)s3 * V4 + o3[e], acc[0], acc[1], acc[2], acc[3]);
+                                st_spinor(outr, (long long)(s_lo + s3) * V4 + o3[e], acc[0], acc[1], acc[2], acc[3]);
                             }
                         }
                     }
@@ -1108,8 +1118,8 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
         const float4* Rv = reinterpret_cast<const float4*>(inr);
         const float4* Q = a.q ? reinterpret_cast<const float4*>(a.q) + (long long)rhs * Ls * V4 : nullptr;
         double dzr = 0.0, dzq = 0.0;
-        for (int e = tid; e < n3 * Ls; e += kFuThreads) {
-            const int s = e / n3, k = e - s * n3;
+        for (int e = tid; e < n3 * Lw; e += kFuThreads) {
+            const int sr = e / n3, k = e - sr * n3, s = s_lo + sr;
             const int i = div_small(k, inv3), jx = k - i * TX;
             const long long site = (long long)s * V4 + (t0 + i) * Nx + x0 + jx;
             const float4 z = Z[site], r = Rv[site];
@@ -1132,7 +1142,7 @@ __global__ __launch_bounds__(kFuThreads) __attribute__((amdgpu_waves_per_eu(4)))
                 zr += red[2 * w];
                 zq += red[2 * w + 1];
             }
-            double* pp = a.part + ((long long)rhs * a.tiles + tile) * 2;
+            double* pp = a.part + (((long long)rhs * a.tiles + tile) * nsp + part) * 2;
             pp[0] = zr;
             pp[1] = zq;
         }
@@ -1599,9 +1609,27 @@ bool fused_shape(const dwf_ctx* c, int& TY, int& TX)
     return true;
 }
 
+// Parts of the fifth dimension per output tile of the fused network (one block each): a
+// launch of few blocks (few right-hand sides) is bound by the latency of its walk, one
+// barrier-separated step per slice, so splitting the walk buys blocks and shortens each
+// block's walk at the price of the parts' recomputed edge slices.  DWF_TUNE_NET_SPLIT sets
+// it; 0 picks the most parts (up to 4, each at least 4 slices) that keep the launch within
+// kNetSplitBlocks blocks.
+constexpr int kNetSplitBlocks = 256;
+int net_split(const dwf_ctx* c, int tiles, int nrhs)
+{
+    const int Ls = c->L.Ls;
+    if (c->net_split > 0) return c->net_split;
+    int n = 1;
+    for (int d = 2; d <= 4; d *= 2)
+        if (Ls % d == 0 && Ls / d >= 4 && (long long)tiles * nrhs * d <= kNetSplitBlocks) n = d;
+    return n;
+}
+
 // Fused z = M in (k_net_fused); with part also the (z, in) and, with q, (z, q) partials,
-// part [nrhs][tiles][2].  Returns the number of tiles per lattice, or 0 if the shape does not
-// allow the fused kernel (then nothing is launched).
+// part [nrhs][tiles * net_split][2].  Returns the number of partials per right-hand side
+// (tiles per lattice times parts), or 0 if the shape does not allow the fused kernel (then
+// nothing is launched).
 // The FCG iteration's check folded into the fused network launch (see NetFusedArgs).
 struct FusedCheck {
     const double* hpart;
@@ -1627,6 +1655,7 @@ int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind
     a.TX = TX;
     a.tiles_x = L.Nx / TX;
     a.tiles = (L.Nt / TY) * a.tiles_x;
+    a.nsplit = net_split(c, a.tiles, nrhs);
     a.in = in;
     a.out = out;
     a.links = c->l32;
@@ -1646,7 +1675,7 @@ int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind
         a.cst = chk->st;
         a.fs = chk->fs;
     }
-    const unsigned grid = (unsigned)(a.tiles * nrhs);
+    const unsigned grid = (unsigned)(a.tiles * nrhs * a.nsplit);
     hipStream_t s = c->stream;
     if (part) {
         if (in_kind != 1 || out_kind != 1) throw DwfError(DWF_ERR_ARG, "fused network: dot products need FP32 spinors");
@@ -1667,7 +1696,7 @@ int launch_net_fused(dwf_ctx* c, const dwf_net* net, const void* in, int in_kind
     } else {
         throw DwfError(DWF_ERR_ARG, "fused network: unsupported spinor types");
     }
-    return a.tiles;
+    return a.tiles * a.nsplit;
 }
 
 // z = M in for nrhs right-hand sides, configuration per cfg[rhs]: the fused kernel when the
@@ -2102,7 +2131,8 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
     const dim3 vgrid(nb, nrhs);
     const int fr = c->fcg_beta;
     int fty = 0, ftx = 0;
-    const int ntiles = fused_shape(c, fty, ftx) ? (c->L.Nt / fty) * (c->L.Nx / ftx) : 1;
+    const int ftiles = fused_shape(c, fty, ftx) ? (c->L.Nt / fty) * (c->L.Nx / ftx) : 1;
+    const int ntiles = ftiles * net_split(c, ftiles, nrhs);  // the fused network's partials
     double* npart = (double*)c->buf(B_FCG_NPART, (size_t)nrhs * ntiles * 2 * 8);
 
     if (!have_guess) launch_zero<double>(x, (long long)nv, s);
@@ -2200,6 +2230,7 @@ int dwf_fcg_solve(dwf_ctx* c, const double* b, double* x, bool have_guess, doubl
             key.push_back((unsigned long long)(uintptr_t)ptr);
         key.push_back((unsigned long long)ha.rhs_per_cfg);
         key.push_back((unsigned long long)c->net_fused);
+        key.push_back((unsigned long long)ntiles);
         key.push_back((unsigned long long)fr);
         const int v[] = {ha.cfg.bx, ha.cfg.chunks, ha.cfg.spt, ha.cfg.remap, ha.cfg.prefetch, ha.cfg.tx,
                          ha.cfg.tiles_x, ha.cfg.nt_store, ha.cfg.lds};
